@@ -1,0 +1,191 @@
+/* mpart_amd.h -- C ABI of the MI355X monotone-component library (libmpart_amd.so).
+ *
+ * Drop-in boundary for MParT's monotone-component path.  Every entry point
+ * takes device pointers (HBM) and a HIP stream, returns 0 on success or a
+ * negative status, and never throws.  The message of the last failure on the
+ * calling thread is returned by mpart_last_error().
+ *
+ * Status codes:
+ *   MPART_OK               0
+ *   MPART_INVALID_ARGUMENT -1  (reference: std::invalid_argument)
+ *   MPART_RUNTIME_ERROR    -2  (reference: std::runtime_error)
+ *   MPART_HIP_ERROR        -3  (a HIP runtime call failed)
+ *   MPART_NOT_IMPLEMENTED  -4
+ *
+ * Matrix arguments are strided views {ptr, rows, cols, row_stride,
+ * col_stride} in units of doubles: element (i, j) is ptr[i*row_stride +
+ * j*col_stride].  Point matrices are inputDim x N with a column per point,
+ * exactly like the reference's StridedMatrix views built from
+ * (ptr, (rows, cols), (rowStride, colStride)) tuples
+ * (MParT/Utilities/ArrayConversions.h:102-130); the fields are 64-bit here.
+ * Calls are asynchronous on `stream` (a hipStream_t; NULL is the null stream).
+ */
+#ifndef MPART_AMD_H
+#define MPART_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPART_OK 0
+#define MPART_INVALID_ARGUMENT (-1)
+#define MPART_RUNTIME_ERROR (-2)
+#define MPART_HIP_ERROR (-3)
+#define MPART_NOT_IMPLEMENTED (-4)
+
+/* Enumerations of MParT/MapOptions.h:11-29. */
+#define MPART_BASIS_PROBABILIST_HERMITE 0
+#define MPART_BASIS_PHYSICIST_HERMITE 1
+#define MPART_BASIS_HERMITE_FUNCTIONS 2
+#define MPART_POSFUNC_EXP 0
+#define MPART_POSFUNC_SOFTPLUS 1
+#define MPART_QUAD_CLENSHAW_CURTIS 0
+#define MPART_QUAD_ADAPTIVE_SIMPSON 1
+#define MPART_QUAD_ADAPTIVE_CLENSHAW_CURTIS 2
+
+/* Replaces mpart::MapOptions (MParT/MapOptions.h:58-123); defaults from
+ * mpart_default_options() are the reference defaults. */
+typedef struct mpart_options {
+    int32_t basisType;
+    int32_t posFuncType;
+    int32_t quadType;
+    int32_t contDeriv;
+    int32_t basisNorm;
+    uint32_t quadMaxSub;
+    uint32_t quadMinSub;
+    uint32_t quadPts;
+    double quadAbsTol;
+    double quadRelTol;
+    double nugget;
+} mpart_options;
+
+typedef struct mpart_map_s* mpart_map;
+
+typedef struct mpart_cview {
+    const double* ptr;
+    int64_t rows, cols, row_stride, col_stride;
+} mpart_cview;
+
+typedef struct mpart_view {
+    double* ptr;
+    int64_t rows, cols, row_stride, col_stride;
+} mpart_view;
+
+typedef void* mpart_stream; /* hipStream_t */
+
+const char* mpart_last_error(void);
+
+/* Replaces mpart::Initialize (MParT/Initialization.h:78-116): idempotent. */
+int mpart_init(void);
+
+void mpart_default_options(mpart_options* opts);
+
+/* FixedMultiIndexSet(dim, maxOrder) (src/MultiIndices/FixedMultiIndexSet.cpp:208-233,
+ * 487-549): the total-order set in compressed form, host arrays sized by a
+ * first call with NULL pointers.  Needs no GPU. */
+int mpart_total_order_set(uint32_t dim, uint32_t order, uint32_t* nterms, uint32_t* nnz, uint32_t* nzStarts,
+                          uint32_t* nzDims, uint32_t* nzOrders);
+
+/* MapFactory::CreateComponent(FixedMultiIndexSet, MapOptions)
+ * (MParT/MapFactory.h:49-60, src/MapFactory.cpp:22-27).  The multi-index set
+ * is given in the compressed form of FixedMultiIndexSet
+ * (MParT/MultiIndices/FixedMultiIndexSet.h: nzStarts[nterms+1], nzDims, nzOrders).
+ * Coefficients start at zero (src/MapFactoryImpl6.cpp:24-25). */
+int mpart_create_component(const mpart_options* opts, uint32_t dim, uint32_t nterms, const uint32_t* nzStarts,
+                           const uint32_t* nzDims, const uint32_t* nzOrders, mpart_map* out);
+
+/* CreateComponent with FixedMultiIndexSet(dim, order) (total order,
+ * src/MultiIndices/FixedMultiIndexSet.cpp:208-233). */
+int mpart_create_total_order_component(const mpart_options* opts, uint32_t dim, uint32_t order, mpart_map* out);
+
+/* MapFactory::CreateTriangular(inputDim, outputDim, totalOrder, options)
+ * (MParT/MapFactory.h:62-87, src/MapFactory.cpp:105-125). */
+int mpart_create_triangular(uint32_t inputDim, uint32_t outputDim, uint32_t totalOrder, const mpart_options* opts,
+                            mpart_map* out);
+
+/* TriangularMap(components) (src/TriangularMap.cpp:10-61).  The components
+ * must share one MapOptions; their coefficients are copied into the new map. */
+int mpart_create_triangular_from(const mpart_map* comps, uint32_t ncomps, mpart_map* out);
+
+int mpart_destroy(mpart_map map);
+
+/* inputDim / outputDim / numCoeffs of ConditionalMapBase. */
+int mpart_map_dims(mpart_map map, uint32_t* inputDim, uint32_t* outputDim, uint32_t* numCoeffs);
+
+/* Number of components and the multi-index set of component i (host arrays
+ * sized by a first call with NULL pointers). */
+int mpart_map_num_components(mpart_map map, uint32_t* ncomps);
+int mpart_component_multiset(mpart_map map, uint32_t i, uint32_t* dim, uint32_t* nterms, uint32_t* nnz,
+                             uint32_t* nzStarts, uint32_t* nzDims, uint32_t* nzOrders);
+
+/* ParameterizedFunctionBase::SetCoeffs (deep copy, src/ParameterizedFunctionBase.cpp:184-214)
+ * and WrapCoeffs (alias, 219-228).  coeffs is a device vector with `stride`. */
+int mpart_set_coeffs(mpart_map map, const double* coeffs, int64_t len, int64_t stride, mpart_stream stream);
+int mpart_wrap_coeffs(mpart_map map, double* coeffs, int64_t len, int64_t stride);
+/* Copies the current coefficients into a device vector. */
+int mpart_get_coeffs(mpart_map map, double* out, mpart_stream stream);
+
+/* ConditionalMapBase::EvaluateImpl (MParT/ParameterizedFunctionBase.h:101-110):
+ * out (outputDim x N) = T(pts). */
+int mpart_evaluate(mpart_map map, mpart_cview pts, mpart_view out, mpart_stream stream);
+
+/* ConditionalMapBase::LogDeterminantImpl (MParT/ConditionalMapBase.h:79-90):
+ * out[n*stride] = log det dT/dx at pts(:, n). */
+int mpart_log_determinant(mpart_map map, mpart_cview pts, double* out, int64_t stride, mpart_stream stream);
+
+/* ParameterizedFunctionBase::GradientImpl (MParT/ParameterizedFunctionBase.h:140-160):
+ * out (inputDim x N) = sum_i sens(i, n) d T_i / dx. */
+int mpart_gradient(mpart_map map, mpart_cview pts, mpart_cview sens, mpart_view out, mpart_stream stream);
+
+/* ParameterizedFunctionBase::CoeffGradImpl (MParT/ParameterizedFunctionBase.h:162-182):
+ * out (numCoeffs x N) = sum_i sens(i, n) d T_i / dw. */
+int mpart_coeff_grad(mpart_map map, mpart_cview pts, mpart_cview sens, mpart_view out, mpart_stream stream);
+
+/* ConditionalMapBase::LogDeterminantCoeffGradImpl (MParT/ConditionalMapBase.h:130-140). */
+int mpart_log_determinant_coeff_grad(mpart_map map, mpart_cview pts, mpart_view out, mpart_stream stream);
+
+/* ConditionalMapBase::LogDeterminantInputGradImpl (MParT/ConditionalMapBase.h:148-159). */
+int mpart_log_determinant_input_grad(mpart_map map, mpart_cview pts, mpart_view out, mpart_stream stream);
+
+/* ConditionalMapBase::InverseImpl (MParT/ConditionalMapBase.h:99-110) with the
+ * Bracket method and tolerances xtol, ytol (MParT/MonotoneComponent.h:331-443).
+ * Component: x1 is dim x (1 or N) with the initial guess in its last row, r is
+ * 1 x N.  Triangular map: x1 is (inputDim-outputDim) x N, r is outputDim x N. */
+int mpart_inverse(mpart_map map, mpart_cview x1, mpart_cview r, mpart_view out, double xtol, double ytol,
+                  mpart_stream stream);
+
+/* Fused forward + adjoint of the Gaussian reverse-KL objective
+ * (FastGaussianReverseKLObjective::ObjectivePlusCoeffGradImpl, src/MapObjective.cpp:93-234,
+ * generalized to every output row):  *loss = mean_n(0.5 |T(x_n)|^2 - logdet(x_n)),
+ * grad = mean_n(sum_i T_i dT_i/dw - dlogdet/dw).  One launch over every
+ * component; no per-point Jacobian is stored.  loss and grad are device pointers. */
+int mpart_kl_value_and_grad(mpart_map map, mpart_cview pts, double* loss, double* grad, mpart_stream stream);
+
+/* The same objective computed the way the reference structures it: per
+ * component Evaluate, LogDeterminant, CoeffGrad(sens = T) and
+ * LogDeterminantCoeffGrad launches with the numCoeffs x N Jacobians stored,
+ * then a row reduction (src/MapObjective.cpp:107-215). */
+int mpart_kl_value_and_grad_plain(mpart_map map, mpart_cview pts, double* loss, double* grad, mpart_stream stream);
+
+/* Fused vector-Jacobian product used by the torch backward
+ * (bindings/python/package/torch_helpers.py:91-107):
+ * grad = sum_n [ CoeffGrad(sens)(:, n) + logdetSens[n] LogDeterminantCoeffGrad(:, n) ].
+ * sens.ptr or logdetSens may be NULL. */
+int mpart_coeff_vjp(mpart_map map, mpart_cview pts, mpart_cview sens, const double* logdetSens, int64_t lstride,
+                    double* grad, mpart_stream stream);
+
+/* Evaluate that also returns the number of integrand evaluations per point and
+ * component (ncomps x N, int32) -- exposes the adaptive trees for parity checks. */
+int mpart_evaluate_counted(mpart_map map, mpart_cview pts, mpart_view out, int32_t* nevals, mpart_stream stream);
+
+/* Selects per-component launches ("plain", 1) or one launch over all
+ * components (0, default) for Evaluate and LogDeterminant. */
+int mpart_set_plain_launches(mpart_map map, int plain);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MPART_AMD_H */

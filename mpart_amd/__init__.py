@@ -1,0 +1,472 @@
+"""MI355X monotone-component maps (MParT's monotone-component path on HIP).
+
+The Python surface mirrors the reference's ``pympart`` module
+(bindings/python/src/Wrapper.cpp:7-49): ``MapOptions``, ``FixedMultiIndexSet``,
+``CreateComponent``, ``CreateTriangular`` and maps with ``SetCoeffs``,
+``WrapCoeffs``, ``CoeffMap``, ``Evaluate``, ``LogDeterminant``, ``Gradient``,
+``CoeffGrad``, ``LogDeterminantCoeffGrad``, ``LogDeterminantInputGrad`` and
+``Inverse``, plus the fused ``KLValueAndGrad``.  Every call goes through the C
+ABI of ``libmpart_amd.so`` (include/mpart_amd.h); torch provides device memory
+and the stream.  The library must be built (``python -m mpart_amd.build``);
+importing fails loudly otherwise.
+
+Point arrays are inputDim x N (a column per point), as in the reference.
+numpy inputs are copied to the GPU and results copied back; torch CUDA tensors
+stay on the device.
+"""
+import ctypes
+import enum
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libmpart_amd.so")
+
+if not os.path.exists(_LIB_PATH):
+    raise ImportError(f"{_LIB_PATH} is missing: build it with `python mpart_amd/build.py`")
+
+_lib = ctypes.CDLL(_LIB_PATH)
+
+MPART_OK, MPART_INVALID_ARGUMENT, MPART_RUNTIME_ERROR, MPART_HIP_ERROR, MPART_NOT_IMPLEMENTED = 0, -1, -2, -3, -4
+
+
+class _COptions(ctypes.Structure):
+    _fields_ = [
+        ("basisType", ctypes.c_int32),
+        ("posFuncType", ctypes.c_int32),
+        ("quadType", ctypes.c_int32),
+        ("contDeriv", ctypes.c_int32),
+        ("basisNorm", ctypes.c_int32),
+        ("quadMaxSub", ctypes.c_uint32),
+        ("quadMinSub", ctypes.c_uint32),
+        ("quadPts", ctypes.c_uint32),
+        ("quadAbsTol", ctypes.c_double),
+        ("quadRelTol", ctypes.c_double),
+        ("nugget", ctypes.c_double),
+    ]
+
+
+class _CView(ctypes.Structure):
+    _fields_ = [("ptr", ctypes.c_void_p), ("rows", ctypes.c_int64), ("cols", ctypes.c_int64),
+                ("row_stride", ctypes.c_int64), ("col_stride", ctypes.c_int64)]
+
+
+_vp, _u32, _i64, _dbl, _int = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int64, ctypes.c_double, ctypes.c_int
+_U32P = ctypes.POINTER(ctypes.c_uint32)
+_OPTP = ctypes.POINTER(_COptions)
+_MAPP = ctypes.POINTER(ctypes.c_void_p)
+
+_SIGS = {
+    "mpart_init": [],
+    "mpart_total_order_set": [_u32, _u32, _U32P, _U32P, _U32P, _U32P, _U32P],
+    "mpart_create_component": [_OPTP, _u32, _u32, _U32P, _U32P, _U32P, _MAPP],
+    "mpart_create_total_order_component": [_OPTP, _u32, _u32, _MAPP],
+    "mpart_create_triangular": [_u32, _u32, _u32, _OPTP, _MAPP],
+    "mpart_create_triangular_from": [_MAPP, _u32, _MAPP],
+    "mpart_destroy": [_vp],
+    "mpart_map_dims": [_vp, _U32P, _U32P, _U32P],
+    "mpart_map_num_components": [_vp, _U32P],
+    "mpart_component_multiset": [_vp, _u32, _U32P, _U32P, _U32P, _U32P, _U32P, _U32P],
+    "mpart_set_coeffs": [_vp, _vp, _i64, _i64, _vp],
+    "mpart_wrap_coeffs": [_vp, _vp, _i64, _i64],
+    "mpart_get_coeffs": [_vp, _vp, _vp],
+    "mpart_evaluate": [_vp, _CView, _CView, _vp],
+    "mpart_evaluate_counted": [_vp, _CView, _CView, _vp, _vp],
+    "mpart_log_determinant": [_vp, _CView, _vp, _i64, _vp],
+    "mpart_gradient": [_vp, _CView, _CView, _CView, _vp],
+    "mpart_coeff_grad": [_vp, _CView, _CView, _CView, _vp],
+    "mpart_log_determinant_coeff_grad": [_vp, _CView, _CView, _vp],
+    "mpart_log_determinant_input_grad": [_vp, _CView, _CView, _vp],
+    "mpart_inverse": [_vp, _CView, _CView, _CView, _dbl, _dbl, _vp],
+    "mpart_kl_value_and_grad": [_vp, _CView, _vp, _vp, _vp],
+    "mpart_kl_value_and_grad_plain": [_vp, _CView, _vp, _vp, _vp],
+    "mpart_coeff_vjp": [_vp, _CView, _CView, _vp, _i64, _vp, _vp],
+    "mpart_set_plain_launches": [_vp, _int],
+}
+for _name, _args in _SIGS.items():
+    _f = getattr(_lib, _name)
+    _f.argtypes = _args
+    _f.restype = ctypes.c_int
+_lib.mpart_last_error.restype = ctypes.c_char_p
+_lib.mpart_last_error.argtypes = []
+_lib.mpart_default_options.argtypes = [_OPTP]
+_lib.mpart_default_options.restype = None
+
+LIB_PATH = _LIB_PATH
+C_EXPORTS = tuple(_SIGS) + ("mpart_last_error", "mpart_default_options")
+
+
+def _check(rc):
+    if rc == MPART_OK:
+        return
+    msg = _lib.mpart_last_error().decode()
+    if rc == MPART_INVALID_ARGUMENT:
+        raise ValueError(msg)
+    if rc == MPART_NOT_IMPLEMENTED:
+        raise NotImplementedError(msg)
+    raise RuntimeError(msg)
+
+
+def Initialize():
+    """Idempotent (reference: mpart::Initialize, MParT/Initialization.h:78-116)."""
+    _check(_lib.mpart_init())
+
+
+# ---------------------------------------------------------------------------
+class BasisTypes(enum.IntEnum):
+    ProbabilistHermite = 0
+    PhysicistHermite = 1
+    HermiteFunctions = 2
+
+
+class PosFuncTypes(enum.IntEnum):
+    Exp = 0
+    SoftPlus = 1
+
+
+class QuadTypes(enum.IntEnum):
+    ClenshawCurtis = 0
+    AdaptiveSimpson = 1
+    AdaptiveClenshawCurtis = 2
+
+
+class MapOptions:
+    """Options of MParT/MapOptions.h:58-123 with the same defaults."""
+
+    _FIELDS = ("basisType", "posFuncType", "quadType", "quadAbsTol", "quadRelTol", "quadMaxSub", "quadMinSub",
+               "quadPts", "contDeriv", "basisNorm", "nugget")
+
+    def __init__(self, **kw):
+        c = _COptions()
+        _lib.mpart_default_options(ctypes.byref(c))
+        self.basisType = BasisTypes(c.basisType)
+        self.posFuncType = PosFuncTypes(c.posFuncType)
+        self.quadType = QuadTypes(c.quadType)
+        self.quadAbsTol = c.quadAbsTol
+        self.quadRelTol = c.quadRelTol
+        self.quadMaxSub = c.quadMaxSub
+        self.quadMinSub = c.quadMinSub
+        self.quadPts = c.quadPts
+        self.contDeriv = bool(c.contDeriv)
+        self.basisNorm = bool(c.basisNorm)
+        self.nugget = c.nugget
+        for k, v in kw.items():
+            if k not in self._FIELDS:
+                raise ValueError(f"unknown MapOptions field {k}")
+            setattr(self, k, v)
+
+    def _c(self):
+        c = _COptions()
+        c.basisType = int(self.basisType)
+        c.posFuncType = int(self.posFuncType)
+        c.quadType = int(self.quadType)
+        c.contDeriv = int(bool(self.contDeriv))
+        c.basisNorm = int(bool(self.basisNorm))
+        c.quadMaxSub = int(self.quadMaxSub)
+        c.quadMinSub = int(self.quadMinSub)
+        c.quadPts = int(self.quadPts)
+        c.quadAbsTol = float(self.quadAbsTol)
+        c.quadRelTol = float(self.quadRelTol)
+        c.nugget = float(self.nugget)
+        return c
+
+    def __repr__(self):
+        return "MapOptions(" + ", ".join(f"{k}={getattr(self, k)!r}" for k in self._FIELDS) + ")"
+
+
+class FixedMultiIndexSet:
+    """Compressed multi-index set (MParT/MultiIndices/FixedMultiIndexSet.h).
+
+    ``FixedMultiIndexSet(dim, maxOrder)`` enumerates the total-order set in the
+    reference order; ``FixedMultiIndexSet.from_csr(dim, nzStarts, nzDims, nzOrders)``
+    wraps a compressed set."""
+
+    def __init__(self, dim=None, maxOrder=None):
+        self.dim = dim
+        if dim is None:
+            return
+        nt, nz = _u32(), _u32()
+        _check(_lib.mpart_total_order_set(dim, maxOrder, ctypes.byref(nt), ctypes.byref(nz), None, None, None))
+        self.nzStarts = np.zeros(nt.value + 1, np.uint32)
+        self.nzDims = np.zeros(nz.value, np.uint32)
+        self.nzOrders = np.zeros(nz.value, np.uint32)
+        _check(_lib.mpart_total_order_set(dim, maxOrder, None, None, self.nzStarts.ctypes.data_as(_U32P),
+                                          self.nzDims.ctypes.data_as(_U32P), self.nzOrders.ctypes.data_as(_U32P)))
+
+    @classmethod
+    def from_csr(cls, dim, nzStarts, nzDims, nzOrders):
+        s = cls()
+        s.dim = int(dim)
+        s.nzStarts = np.ascontiguousarray(nzStarts, np.uint32)
+        s.nzDims = np.ascontiguousarray(nzDims, np.uint32)
+        s.nzOrders = np.ascontiguousarray(nzOrders, np.uint32)
+        return s
+
+    def Size(self):
+        return len(self.nzStarts) - 1
+
+    def Length(self):
+        return self.dim
+
+    def MaxDegrees(self):
+        out = np.zeros(self.dim, np.uint32)
+        for d, o in zip(self.nzDims, self.nzOrders):
+            out[d] = max(out[d], o)
+        return out
+
+    def IndexToMulti(self, i):
+        m = np.zeros(self.dim, np.uint32)
+        for k in range(self.nzStarts[i], self.nzStarts[i + 1]):
+            m[self.nzDims[k]] = self.nzOrders[k]
+        return m
+
+
+def _read_multiset(h, i):
+    dim, nt, nnz = _u32(), _u32(), _u32()
+    _check(_lib.mpart_component_multiset(h, i, ctypes.byref(dim), ctypes.byref(nt), ctypes.byref(nnz), None, None,
+                                         None))
+    starts = np.zeros(nt.value + 1, np.uint32)
+    dims = np.zeros(nnz.value, np.uint32)
+    orders = np.zeros(nnz.value, np.uint32)
+    _check(_lib.mpart_component_multiset(h, i, None, None, None, starts.ctypes.data_as(_U32P),
+                                         dims.ctypes.data_as(_U32P), orders.ctypes.data_as(_U32P)))
+    return dim.value, starts, dims, orders
+
+
+# ---------------------------------------------------------------------------
+def _torch():
+    import torch
+    return torch
+
+
+def _stream():
+    torch = _torch()
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _as_dev(a):
+    """(device tensor, was_numpy).  numpy arrays are copied to cuda:0."""
+    torch = _torch()
+    if isinstance(a, torch.Tensor):
+        if a.dtype != torch.float64:
+            raise ValueError(f"Currently only tensors with float64 datatype are supported, got {a.dtype}")
+        if not a.is_cuda:
+            return a.cuda(), True
+        return a, False
+    arr = np.asarray(a, dtype=np.float64)
+    return torch.from_numpy(np.ascontiguousarray(arr)).cuda(), True
+
+
+def _view(t, rows=None):
+    """Strided view of a 1-D or 2-D float64 cuda tensor."""
+    if t is None:
+        return _CView(None, 0, 0, 0, 0)
+    if t.dim() == 1:
+        return _CView(t.data_ptr(), 1, t.shape[0], 0, t.stride(0))
+    return _CView(t.data_ptr(), t.shape[0], t.shape[1], t.stride(0), t.stride(1))
+
+
+class ConditionalMap:
+    """A monotone component or a triangular map held by libmpart_amd.so
+    (the role of mpart::ConditionalMapBase<DeviceSpace>)."""
+
+    def __init__(self, handle):
+        self._h = ctypes.c_void_p(handle)
+        i, o, c = _u32(), _u32(), _u32()
+        _check(_lib.mpart_map_dims(self._h, ctypes.byref(i), ctypes.byref(o), ctypes.byref(c)))
+        self.inputDim, self.outputDim, self.numCoeffs = i.value, o.value, c.value
+        self._wrapped = None
+
+    def __del__(self):
+        try:
+            _lib.mpart_destroy(self._h)
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def NumComponents(self):
+        n = _u32()
+        _check(_lib.mpart_map_num_components(self._h, ctypes.byref(n)))
+        return n.value
+
+    def GetMultiIndexSet(self, i=0):
+        dim, s, d, o = _read_multiset(self._h, i)
+        return FixedMultiIndexSet.from_csr(dim, s, d, o)
+
+    # coefficients ---------------------------------------------------------
+    def SetCoeffs(self, coeffs):
+        """Deep copy (src/ParameterizedFunctionBase.cpp:184-214)."""
+        t, _ = _as_dev(coeffs)
+        if t.dim() != 1:
+            raise ValueError("coefficients must be a vector")
+        _check(_lib.mpart_set_coeffs(self._h, t.data_ptr(), t.shape[0], t.stride(0), _stream()))
+        _torch().cuda.current_stream().synchronize()
+        self._wrapped = None
+
+    def WrapCoeffs(self, coeffs):
+        """Alias a float64 cuda vector without copying (219-228)."""
+        torch = _torch()
+        if not (isinstance(coeffs, torch.Tensor) and coeffs.is_cuda and coeffs.dtype == torch.float64):
+            raise ValueError("WrapCoeffs needs a float64 cuda tensor")
+        _check(_lib.mpart_wrap_coeffs(self._h, coeffs.data_ptr(), coeffs.shape[0], coeffs.stride(0)))
+        self._wrapped = coeffs
+
+    def CoeffMap(self):
+        torch = _torch()
+        out = torch.empty(self.numCoeffs, dtype=torch.float64, device="cuda")
+        _check(_lib.mpart_get_coeffs(self._h, out.data_ptr(), _stream()))
+        return out.cpu().numpy()
+
+    def SetPlainLaunches(self, plain=True):
+        _check(_lib.mpart_set_plain_launches(self._h, int(bool(plain))))
+
+    # Impl methods (device tensors, preallocated outputs) --------------------
+    def EvaluateImpl(self, pts, out):
+        _check(_lib.mpart_evaluate(self._h, _view(pts), _view(out), _stream()))
+
+    def LogDeterminantImpl(self, pts, out):
+        _check(_lib.mpart_log_determinant(self._h, _view(pts), out.data_ptr(), out.stride(0), _stream()))
+
+    def GradientImpl(self, pts, sens, out):
+        _check(_lib.mpart_gradient(self._h, _view(pts), _view(sens), _view(out), _stream()))
+
+    def CoeffGradImpl(self, pts, sens, out):
+        _check(_lib.mpart_coeff_grad(self._h, _view(pts), _view(sens), _view(out), _stream()))
+
+    def LogDeterminantCoeffGradImpl(self, pts, out):
+        _check(_lib.mpart_log_determinant_coeff_grad(self._h, _view(pts), _view(out), _stream()))
+
+    def LogDeterminantInputGradImpl(self, pts, out):
+        _check(_lib.mpart_log_determinant_input_grad(self._h, _view(pts), _view(out), _stream()))
+
+    def InverseImpl(self, x1, r, out, xtol=1e-6, ytol=1e-6):
+        _check(_lib.mpart_inverse(self._h, _view(x1), _view(r), _view(out), xtol, ytol, _stream()))
+
+    # allocating methods -----------------------------------------------------
+    def _run(self, fn, shape, *ins):
+        torch = _torch()
+        devs = [_as_dev(a) for a in ins]
+        host = any(h for _, h in devs)
+        out = torch.zeros(shape, dtype=torch.float64, device="cuda")
+        fn(*[t for t, _ in devs], out)
+        if host:
+            torch.cuda.current_stream().synchronize()
+            return out.cpu().numpy()
+        return out
+
+    def Evaluate(self, pts):
+        return self._run(self.EvaluateImpl, (self.outputDim, _ncols(pts)), pts)
+
+    def EvaluateCounted(self, pts):
+        """Evaluate plus the number of integrand evaluations per (component, point)."""
+        torch = _torch()
+        p, host = _as_dev(pts)
+        out = torch.zeros((self.outputDim, p.shape[1]), dtype=torch.float64, device="cuda")
+        ne = torch.zeros((self.NumComponents(), p.shape[1]), dtype=torch.int32, device="cuda")
+        _check(_lib.mpart_evaluate_counted(self._h, _view(p), _view(out), ne.data_ptr(), _stream()))
+        if host:
+            torch.cuda.current_stream().synchronize()
+            return out.cpu().numpy(), ne.cpu().numpy()
+        return out, ne
+
+    def LogDeterminant(self, pts):
+        return self._run(self.LogDeterminantImpl, (_ncols(pts),), pts)
+
+    def Gradient(self, pts, sens):
+        return self._run(self.GradientImpl, (self.inputDim, _ncols(pts)), pts, sens)
+
+    def CoeffGrad(self, pts, sens):
+        return self._run(self.CoeffGradImpl, (self.numCoeffs, _ncols(pts)), pts, sens)
+
+    def LogDeterminantCoeffGrad(self, pts):
+        return self._run(self.LogDeterminantCoeffGradImpl, (self.numCoeffs, _ncols(pts)), pts)
+
+    def LogDeterminantInputGrad(self, pts):
+        return self._run(self.LogDeterminantInputGradImpl, (self.inputDim, _ncols(pts)), pts)
+
+    def Inverse(self, x1, r, xtol=1e-6, ytol=1e-6):
+        torch = _torch()
+        rr, host_r = _as_dev(r)
+        if rr.dim() == 1:
+            rr = rr[None, :]
+        if x1 is None or (hasattr(x1, "shape") and x1.shape[0] == 0):
+            xx, host_x = torch.zeros((0, rr.shape[1]), dtype=torch.float64, device="cuda"), False
+        else:
+            xx, host_x = _as_dev(x1)
+        out = torch.zeros((self.outputDim, rr.shape[1]), dtype=torch.float64, device="cuda")
+        self.InverseImpl(xx, rr, out, xtol, ytol)
+        if host_r or host_x:
+            torch.cuda.current_stream().synchronize()
+            return out.cpu().numpy()
+        return out
+
+    def KLValueAndGrad(self, pts, plain=False):
+        """Fused forward + adjoint of mean(0.5 |T|^2 - logdet) and its coefficient gradient."""
+        torch = _torch()
+        p, host = _as_dev(pts)
+        loss = torch.zeros(1, dtype=torch.float64, device="cuda")
+        grad = torch.zeros(self.numCoeffs, dtype=torch.float64, device="cuda")
+        fn = _lib.mpart_kl_value_and_grad_plain if plain else _lib.mpart_kl_value_and_grad
+        _check(fn(self._h, _view(p), loss.data_ptr(), grad.data_ptr(), _stream()))
+        if host:
+            torch.cuda.current_stream().synchronize()
+            return float(loss.item()), grad.cpu().numpy()
+        return loss, grad
+
+    def CoeffVJP(self, pts, sens=None, logdet_sens=None, out=None):
+        """sum_n CoeffGrad(sens)(:, n) + logdet_sens[n] LogDeterminantCoeffGrad(:, n), fused."""
+        torch = _torch()
+        if out is None:
+            out = torch.zeros(self.numCoeffs, dtype=torch.float64, device="cuda")
+        ls_ptr = logdet_sens.data_ptr() if logdet_sens is not None else None
+        ls_stride = logdet_sens.stride(0) if logdet_sens is not None else 0
+        _check(_lib.mpart_coeff_vjp(self._h, _view(pts), _view(sens), ls_ptr, ls_stride, out.data_ptr(), _stream()))
+        return out
+
+
+def _ncols(a):
+    return a.shape[1]
+
+
+# ---------------------------------------------------------------------------
+def CreateComponent(mset, opts=None):
+    """MapFactory::CreateComponent (src/MapFactory.cpp:22-27)."""
+    opts = opts or MapOptions()
+    h = ctypes.c_void_p()
+    s = np.ascontiguousarray(mset.nzStarts, np.uint32)
+    d = np.ascontiguousarray(mset.nzDims, np.uint32)
+    o = np.ascontiguousarray(mset.nzOrders, np.uint32)
+    _check(_lib.mpart_create_component(ctypes.byref(opts._c()), mset.dim, len(s) - 1, s.ctypes.data_as(_U32P),
+                                       d.ctypes.data_as(_U32P), o.ctypes.data_as(_U32P), ctypes.byref(h)))
+    return ConditionalMap(h.value)
+
+
+def CreateTotalOrderComponent(dim, order, opts=None):
+    opts = opts or MapOptions()
+    h = ctypes.c_void_p()
+    _check(_lib.mpart_create_total_order_component(ctypes.byref(opts._c()), dim, order, ctypes.byref(h)))
+    return ConditionalMap(h.value)
+
+
+def CreateTriangular(inputDim, outputDim, totalOrder, opts=None):
+    """MapFactory::CreateTriangular (src/MapFactory.cpp:105-125)."""
+    opts = opts or MapOptions()
+    h = ctypes.c_void_p()
+    _check(_lib.mpart_create_triangular(inputDim, outputDim, totalOrder, ctypes.byref(opts._c()), ctypes.byref(h)))
+    return ConditionalMap(h.value)
+
+
+def TriangularMap(components):
+    """TriangularMap(components) with the components' coefficients copied in."""
+    arr = (ctypes.c_void_p * len(components))(*[c.handle.value for c in components])
+    h = ctypes.c_void_p()
+    _check(_lib.mpart_create_triangular_from(arr, len(components), ctypes.byref(h)))
+    return ConditionalMap(h.value)
+
+
+__all__ = ["MapOptions", "BasisTypes", "PosFuncTypes", "QuadTypes", "FixedMultiIndexSet", "ConditionalMap",
+           "CreateComponent", "CreateTotalOrderComponent", "CreateTriangular", "TriangularMap", "Initialize"]

@@ -1,0 +1,19 @@
+#include "launchers.hpp"
+#include "mono_kernels.hpp"
+
+namespace mpd {
+
+hipError_t launch_input_grad(int var, const Launch& L, const MapDev& M, const double* w, long ws, const double* pts,
+                             long prs, long pcs, long N, const double* sens, long srs, long scs, double* out, long ors,
+                             long ocs, int c, int mode, double* gtab) {
+    PtsArg P{pts, prs, pcs, N};
+    OutArg O{out, ors, ocs};
+#define CALL(B, Q, R)                                                                                              \
+    hipLaunchKernelGGL((k_input_grad<B, Q, R>), L.grid, L.block, L.smem, L.stream, M, w, ws, P, sens, srs, scs, O, \
+                       c, mode, gtab)
+    MPD_DISPATCH(var, CALL)
+#undef CALL
+    return hipGetLastError();
+}
+
+}  // namespace mpd

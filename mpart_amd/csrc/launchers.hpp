@@ -1,0 +1,66 @@
+// launchers.hpp -- host-side launchers for the kernel variants (one
+// translation unit per kernel family so they compile in parallel).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "mono_device.hpp"
+
+namespace mpd {
+
+struct PtsArg;
+struct OutArg;
+
+struct Launch {
+    dim3 grid, block;
+    size_t smem;
+    hipStream_t stream;
+};
+
+// Variant id: bk * 100 + pk * 10 + qk.
+hipError_t launch_eval(int var, const Launch& L, const MapDev& M, const double* w, long ws, const double* pts, long prs,
+                       long pcs, long N, double* out, long ors, long ocs, double* ld, long lds, int ldAcc, int* nevals,
+                       int cb, int ce, double* gtab);
+hipError_t launch_coeff_grad(int var, const Launch& L, const MapDev& M, const double* w, long ws, const double* pts,
+                             long prs, long pcs, long N, const double* sens, long srs, long scs, double* out, long ors,
+                             long ocs, int c, double* gtab);
+hipError_t launch_logdet_coeff_grad(int var, const Launch& L, const MapDev& M, const double* w, long ws,
+                                    const double* pts, long prs, long pcs, long N, double* out, long ors, long ocs,
+                                    int c, double* gtab);
+hipError_t launch_input_grad(int var, const Launch& L, const MapDev& M, const double* w, long ws, const double* pts,
+                             long prs, long pcs, long N, const double* sens, long srs, long scs, double* out, long ors,
+                             long ocs, int c, int mode, double* gtab);
+hipError_t launch_inverse(int var, const Launch& L, const MapDev& M, const double* w, long ws, const double* xs,
+                          long xrows, long xcols, long xrs, long xcs, const double* ys, long yss, long N, double* out,
+                          long os, int c, double xtol, double ytol, double* gtab);
+hipError_t launch_fused(int var, const Launch& L, const MapDev& M, const double* w, long ws, const double* pts,
+                        long prs, long pcs, long N, const double* sens, long srs, long scs, const double* lsens,
+                        long lss, int klMode, double* partialGrad, double* partialLoss, double* gtab,
+                        double* gradGlobal);
+int occupancy_fused(int var, int block, size_t smem);
+
+hipError_t launch_reduce_rows(const double* in, long nrows, long K, double scale, double* out, hipStream_t s);
+hipError_t launch_kl_rows(const double* G, const double* LG, long N, long K, double* grad, hipStream_t s);
+hipError_t launch_kl_loss(const double* T, long M, const double* ld, long N, double* partial, hipStream_t s);
+hipError_t launch_fill(double* p, long rows, long cols, long rs, long cs, double v, hipStream_t s);
+hipError_t launch_copy(const double* src, long rows, long cols, long srs, long scs, double* dst, long drs, long dcs,
+                       hipStream_t s);
+
+}  // namespace mpd
+
+// Expands CALL(BK, PK, QK) for the variant id `var`; returns
+// hipErrorInvalidValue for an unknown id.
+#define MPD_DISPATCH(var, CALL)                          \
+    switch (var) {                                       \
+        case 0: CALL(0, 0, 0); break;                    \
+        case 1: CALL(0, 0, 1); break;                    \
+        case 2: CALL(0, 0, 2); break;                    \
+        case 10: CALL(0, 1, 0); break;                   \
+        case 11: CALL(0, 1, 1); break;                   \
+        case 12: CALL(0, 1, 2); break;                   \
+        case 100: CALL(1, 0, 0); break;                  \
+        case 101: CALL(1, 0, 1); break;                  \
+        case 102: CALL(1, 0, 2); break;                  \
+        case 110: CALL(1, 1, 0); break;                  \
+        case 111: CALL(1, 1, 1); break;                  \
+        case 112: CALL(1, 1, 2); break;                  \
+        default: return hipErrorInvalidValue;            \
+    }

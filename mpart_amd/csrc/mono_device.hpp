@@ -1,0 +1,735 @@
+// mono_device.hpp -- device building blocks of the monotone-component path.
+//
+// Lane-per-point design.  For one point x and one component of input dimension
+// k+1 the expansion f(x_0..x_k) = sum_j w_j prod_e phi_{d_e,o_e}(x_{d_e}) * phi_q(x_k)
+// is regrouped by the order q of the last coordinate:
+//     f(x_<k, s) = sum_q C_q phi~_q(s),   C_q = sum_{j: q_j = q} w_j P_j(x_<k),
+// where P_j is the product of the term's off-diagonal factors and phi~_0 = 1
+// (a term without a last-coordinate factor).  Every integrand evaluation of
+//     T(x) = f(x_<k, 0) + int_0^1 x_k g(d_k f(x_<k, t x_k)) dt
+// then costs O(p) instead of O(#terms), and the coefficient adjoint factorizes:
+//     dT/dw_j = P_j (phi~_q(0) + S_q),   S_q = int_0^1 x_k g'(d_k f) phi~'_q(t x_k) dt.
+// The S_q are accumulated at accepted leaves of the same adaptive tree the
+// forward integral uses (the reference's error estimate looks only at the
+// first integrand component, MParT/Quadrature.h:448-451, so its vector
+// integrals share the scalar tree).
+//
+// The basis, bijector, quadrature traversal and acceptance rule follow
+// MParT/OrthogonalPolynomial.h, MParT/HermiteFunction.h, MParT/PositiveBijectors.h
+// and MParT/Quadrature.h:553-676 (adaptive Simpson), 819-968 (adaptive
+// Clenshaw-Curtis).
+#pragma once
+
+#ifndef __HIPCC_RTC__
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#endif
+
+namespace mpd {
+
+constexpr int MAXP = 12;        // largest last-coordinate degree held in registers
+constexpr int MAXND = 8;        // largest number of off-diagonal factors per term
+constexpr int MAXSUB = 32;      // largest adaptive depth (quadMaxSub)
+constexpr int MAXNF = 17;       // largest nested Clenshaw-Curtis fine rule
+constexpr int MAXORD = 16;      // largest off-diagonal degree in the basis table
+constexpr uint32_t DUMMY = 0xFFFFFFFFu;
+
+enum BasisKind { BK_PH = 0, BK_HF = 1 };
+enum PosKind { PK_EXP = 0, PK_SP = 1 };
+enum QuadKind { QK_AS = 0, QK_ACC5 = 1, QK_ACC = 2 };
+enum Flavor { FL_VAL = 0, FL_COEF = 1, FL_INPUT = 2 };
+
+// Nested Clenshaw-Curtis pair and adaptive parameters.
+struct QuadDev {
+    double absTol, relTol;
+    int maxSub, minSub;
+    int nc, nf;  // coarse / fine rule sizes (2^L+1, 2^(L+1)+1)
+    double cw[MAXNF], cp[MAXNF], fw[MAXNF], fp[MAXNF];
+};
+
+// One monotone component of a map.
+struct CompDev {
+    int dim;       // input dimension of the component (k+1)
+    int nterms;
+    int coeffOff;  // offset of its coefficients in the map's coefficient vector
+    int outRow;    // output row of the component in the map
+    int p;         // largest order of the last coordinate
+    int groupOff;  // index of its q = 0 group in the group table
+    int pad0, pad1;
+};
+
+// Terms of one component with a given last-coordinate order q.  The term
+// records [start, start+countPad) are padded with dummies to a multiple of 16.
+struct GroupDev {
+    int start, count, countPad, pad;
+};
+
+// Everything a kernel needs to know about a map (passed by value).
+struct MapDev {
+    const CompDev* comps;
+    const GroupDev* groups;
+    const uint32_t* terms;   // records of trec uint32: [coeff index, slot_0 .. slot_{nd-1}]
+    const int* slotDim;      // basis-table slot -> coordinate (slot 0 holds 1.0)
+    const int* dimSlot;      // coordinate -> first slot (order 1)
+    const int* dimDeg;       // coordinate -> largest off-diagonal order in the table
+    int trec, nd;            // record length and number of factor slots (trec = nd + 1)
+    int nslots;              // table slots including slot 0
+    int D;                   // input dimension of the map
+    int ncomp, numCoeffs;
+    int normalize;           // ProbabilistHermite basisNorm
+    int pmax;                // largest p over components
+    double nugget;
+    double norm[MAXORD + 1];   // ProbabilistHermite normalization sqrt(sqrt(2 pi) k!)
+    double rho[MAXP + 1];      // phi~'_q = rho_q * B'_q (folded normalization)
+    double phi0[MAXP + 1];     // phi~_q(0), with phi~_0 = 1
+    double hfA[MAXORD + 1];    // Hermite-function recurrence: sqrt(0.5 (i-1))
+    double hfB[MAXORD + 1];    //                              sqrt(0.5 i)
+    double hfS[MAXORD + 1];    // (2^i i!)^(-1/2)
+    double hfPi;               // pi^(-1/4)
+    double hfSqrt2;            // sqrt(2)
+    QuadDev quad;
+};
+
+// ---------------------------------------------------------------------------
+// Positive bijector g, g', g'' (MParT/PositiveBijectors.h:12-57).
+template <int PK>
+struct PosF {
+    double g, gp;
+};
+
+template <int PK>
+__device__ __forceinline__ PosF<PK> posfunc(double z) {
+    PosF<PK> r;
+    if constexpr (PK == PK_EXP) {
+        r.g = exp(z);
+        r.gp = r.g;
+    } else {
+        // log(1 + exp(-|z|)) + max(z, 0); derivative shares e = exp(-|z|).
+        double e = exp(-fabs(z));
+        r.g = log(1.0 + e) + fmax(z, 0.0);
+        r.gp = (z < 0) ? e / (e + 1.0) : 1.0 / (1.0 + e);
+    }
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// Diagonal (last-coordinate) basis with folded normalization.
+//
+// ProbabilistHermite: phi'_q = He'_q / n_q = q He_{q-1} / n_q, so with
+// Cd_q = C_q / n_q:   d_k f(y) = sum_{r=0}^{p-1} (r+1) Cd_{r+1} He_r(y).
+// Hermite functions (q >= 2, k = q-2): phi'_q = pi^-1/4 s_k e^{-y^2/2} (2k H_{k-1} - y H_k)
+// with physicist H; phi'_1 = 1 and phi'_0 = 0.
+struct DiagCoef {
+    double cd[MAXP + 1];  // C_q * rho_q
+    int p;
+    double x;   // x_k
+    double nug; // nugget added to g inside the integrand
+};
+
+// d_k f at y (and optionally the second derivative).
+template <int BK, bool SECOND>
+__device__ __forceinline__ double diag_eval(const DiagCoef& dc, double y, double* z2out) {
+    const int p = dc.p;
+    if constexpr (BK == BK_PH) {
+        // He_0 = 1, He_1 = y, He_r = y He_{r-1} - (r-1) He_{r-2}
+        double hm2 = 0.0, hm1 = 1.0;  // He_{r-2}, He_{r-1}
+        double z = 0.0, z2 = 0.0;
+#pragma unroll
+        for (int r = 0; r < MAXP; ++r) {
+            if (r >= p) break;
+            double hr = (r == 0) ? 1.0 : ((r == 1) ? y : y * hm1 - (r - 1.0) * hm2);
+            z += (r + 1.0) * dc.cd[r + 1] * hr;
+            if constexpr (SECOND) {
+                if (r + 2 <= p) z2 += (r + 2.0) * (r + 1.0) * dc.cd[r + 2] * hr;
+            }
+            if (r >= 1) {
+                hm2 = hm1;
+                hm1 = hr;
+            } else {
+                hm1 = hr;
+            }
+        }
+        if constexpr (SECOND) *z2out = z2;
+        return z;
+    } else {
+        double z = (p >= 1) ? dc.cd[1] : 0.0;
+        if (p < 2) {
+            if constexpr (SECOND) *z2out = 0.0;
+            return z;
+        }
+        double ex = exp(-0.5 * y * y);
+        // H_0 = 1, H_1 = 2y, H_k = 2y H_{k-1} - 2(k-1) H_{k-2}
+        double hkm1 = 0.0, hkm2 = 0.0, s = 0.0, s2 = 0.0;
+#pragma unroll
+        for (int k = 0; k < MAXP - 1; ++k) {
+            if (k + 2 > p) break;
+            double hk = (k == 0) ? 1.0 : ((k == 1) ? 2.0 * y : 2.0 * y * hkm1 - 2.0 * (k - 1.0) * hkm2);
+            double dk = (k == 0) ? 0.0 : 2.0 * k * hkm1;  // H'_k = 2k H_{k-1}
+            s += dc.cd[k + 2] * (dk - y * hk);
+            if constexpr (SECOND) s2 += dc.cd[k + 2] * (y * y - (2.0 * k + 1.0)) * hk;
+            hkm2 = hkm1;
+            hkm1 = hk;
+        }
+        if constexpr (SECOND) *z2out = ex * s2;
+        return z + ex * s;
+    }
+}
+
+// B'_q(y) for q = 0..p (raw derivative basis; phi~'_q = rho_q B'_q).
+template <int BK>
+__device__ __forceinline__ void diag_basis(int p, double y, double* bq) {
+    bq[0] = 0.0;
+    if constexpr (BK == BK_PH) {
+        double hm2 = 0.0, hm1 = 1.0;
+#pragma unroll
+        for (int r = 0; r < MAXP; ++r) {
+            if (r >= p) break;
+            double hr = (r == 0) ? 1.0 : ((r == 1) ? y : y * hm1 - (r - 1.0) * hm2);
+            bq[r + 1] = (r + 1.0) * hr;
+            if (r >= 1) {
+                hm2 = hm1;
+                hm1 = hr;
+            } else {
+                hm1 = hr;
+            }
+        }
+    } else {
+        if (p >= 1) bq[1] = 1.0;
+        if (p < 2) return;
+        double ex = exp(-0.5 * y * y);
+        double hkm1 = 0.0, hkm2 = 0.0;
+#pragma unroll
+        for (int k = 0; k < MAXP - 1; ++k) {
+            if (k + 2 > p) break;
+            double hk = (k == 0) ? 1.0 : ((k == 1) ? 2.0 * y : 2.0 * y * hkm1 - 2.0 * (k - 1.0) * hkm2);
+            double dk = (k == 0) ? 0.0 : 2.0 * k * hkm1;
+            bq[k + 2] = ex * (dk - y * hk);
+            hkm2 = hkm1;
+            hkm1 = hk;
+        }
+    }
+}
+
+// B''_q(y) (raw second-derivative basis, same folding as B'_q).
+template <int BK>
+__device__ __forceinline__ void diag_basis2(int p, double y, double* bq) {
+    bq[0] = 0.0;
+    if (p >= 1) bq[1] = 0.0;
+    if constexpr (BK == BK_PH) {
+        // He''_q = q (q-1) He_{q-2}
+        double hm2 = 0.0, hm1 = 1.0;
+#pragma unroll
+        for (int r = 0; r < MAXP - 1; ++r) {
+            if (r + 2 > p) break;
+            double hr = (r == 0) ? 1.0 : ((r == 1) ? y : y * hm1 - (r - 1.0) * hm2);
+            bq[r + 2] = (r + 2.0) * (r + 1.0) * hr;
+            if (r >= 1) {
+                hm2 = hm1;
+                hm1 = hr;
+            } else {
+                hm1 = hr;
+            }
+        }
+    } else {
+        if (p < 2) return;
+        double ex = exp(-0.5 * y * y);
+        double hkm1 = 0.0, hkm2 = 0.0;
+#pragma unroll
+        for (int k = 0; k < MAXP - 1; ++k) {
+            if (k + 2 > p) break;
+            double hk = (k == 0) ? 1.0 : ((k == 1) ? 2.0 * y : 2.0 * y * hkm1 - 2.0 * (k - 1.0) * hkm2);
+            bq[k + 2] = ex * (y * y - (2.0 * k + 1.0)) * hk;
+            hkm2 = hkm1;
+            hkm1 = hk;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Off-diagonal basis table.  Slot 0 holds 1.0; coordinate d occupies slots
+// dimSlot[d] .. dimSlot[d]+dimDeg[d]-1 for orders 1..dimDeg[d].  The table of a
+// lane lives at tab[slot * stride].
+//
+// Values: EvaluateAll (OrthogonalPolynomial.h:25-42, HermiteFunction.h:22-43).
+template <int BK>
+__device__ void fill_values(const MapDev& M, int ndims, const double* pt, long prs, double* tab, int stride) {
+    tab[0] = 1.0;
+    for (int d = 0; d < ndims; ++d) {
+        const int deg = M.dimDeg[d];
+        if (deg == 0) continue;
+        const double x = pt[d * prs];
+        double* out = tab + (long)M.dimSlot[d] * stride - stride;  // out[o*stride], o >= 1
+        if constexpr (BK == BK_PH) {
+            double vm2 = 1.0, vm1 = x;
+            for (int o = 1; o <= deg; ++o) {
+                double v = (o == 1) ? x : (1.0 * x + 0.0) * vm1 - (o - 1.0) * vm2;
+                if (o >= 2) {
+                    vm2 = vm1;
+                    vm1 = v;
+                }
+                out[o * stride] = M.normalize ? v / M.norm[o] : v;
+            }
+        } else {
+            // phi_1 = x, phi_2 = pi^-1/4 exp(-x^2/2), phi_3 = sqrt(2) x phi_2,
+            // phi_{i+2} = (x phi_{i+1} - sqrt((i-1)/2) phi_i) / sqrt(i/2)
+            double pm1 = 0.0, pm2 = 0.0;
+            for (int o = 1; o <= deg; ++o) {
+                double v;
+                if (o == 1) v = x;
+                else if (o == 2) v = M.hfPi * exp(-0.5 * x * x);
+                else if (o == 3) v = M.hfSqrt2 * x * pm1;
+                else {
+                    int i = o - 2;
+                    v = (x * pm1 - M.hfA[i] * pm2) / M.hfB[i];
+                }
+                pm2 = pm1;
+                pm1 = v;
+                out[o * stride] = v;
+            }
+        }
+    }
+}
+
+// Values and first derivatives as EvaluateDerivatives computes them
+// (OrthogonalPolynomial.h:87-118, HermiteFunction.h:45-80); used by the
+// input-gradient paths (FillCache1 with DerivativeFlags::Input).
+template <int BK>
+__device__ void fill_values_derivs(const MapDev& M, int ndims, const double* pt, long prs, double* tab, double* dtab,
+                                   int stride) {
+    tab[0] = 1.0;
+    dtab[0] = 0.0;
+    for (int d = 0; d < ndims; ++d) {
+        const int deg = M.dimDeg[d];
+        if (deg == 0) continue;
+        const double x = pt[d * prs];
+        double* out = tab + (long)M.dimSlot[d] * stride - stride;
+        double* dout = dtab + (long)M.dimSlot[d] * stride - stride;
+        if constexpr (BK == BK_PH) {
+            double vm2 = 1.0, vm1 = x, dm2 = 0.0, dm1 = 1.0;
+            for (int o = 1; o <= deg; ++o) {
+                double v, dv;
+                if (o == 1) {
+                    v = x;
+                    dv = 1.0;
+                } else {
+                    v = (1.0 * x + 0.0) * vm1 - (o - 1.0) * vm2;
+                    dv = 1.0 * vm1 + (1.0 * x + 0.0) * dm1 - (o - 1.0) * dm2;
+                    vm2 = vm1;
+                    vm1 = v;
+                    dm2 = dm1;
+                    dm1 = dv;
+                }
+                if (M.normalize) {
+                    double n = M.norm[o];
+                    v /= n;
+                    dv /= n;
+                }
+                out[o * stride] = v;
+                dout[o * stride] = dv;
+            }
+        } else {
+            // phi_1 = x (derivative 1); phi_{k+2}: physicist H_k and H'_k scaled
+            // by pi^-1/4 e^{-x^2/2} (2^k k!)^-1/2, derivative (H'_k - x H_k) * scale.
+            const double base = M.hfPi * exp(-0.5 * x * x);
+            double hm1 = 0.0, hm2 = 0.0, dm1 = 0.0, dm2 = 0.0;
+            for (int o = 1; o <= deg; ++o) {
+                if (o == 1) {
+                    out[o * stride] = x;
+                    dout[o * stride] = 1.0;
+                    continue;
+                }
+                int k = o - 2;
+                double h, dh;
+                if (k == 0) {
+                    h = 1.0;
+                    dh = 0.0;
+                } else if (k == 1) {
+                    h = 2.0 * x;
+                    dh = 2.0;
+                } else {
+                    h = (2.0 * x + 0.0) * hm1 - 2.0 * (k - 1.0) * hm2;
+                    dh = 2.0 * hm1 + (2.0 * x + 0.0) * dm1 - 2.0 * (k - 1.0) * dm2;
+                }
+                hm2 = hm1;
+                hm1 = h;
+                dm2 = dm1;
+                dm1 = dh;
+                double scale = (k == 0) ? base : base * M.hfS[k];
+                double dd = dh - x * h;
+                out[o * stride] = h * scale;
+                dout[o * stride] = dd * scale;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Group passes over the term records of one component.
+
+// C_q = sum_{j in group q} w_j P_j for q = 0..p.
+template <int ND>
+__device__ __forceinline__ void coeff_groups_nd(const MapDev& M, const CompDev& c, const double* w, long ws,
+                                                const double* tab, int stride, double* C) {
+#pragma unroll
+    for (int q = 0; q <= MAXP; ++q) {
+        if (q > c.p) break;
+        const GroupDev g = M.groups[c.groupOff + q];
+        double acc = 0.0;
+        for (int j = g.start; j < g.start + g.count; ++j) {
+            const uint32_t* rec = M.terms + (long)j * (ND + 1);
+            double P = tab[(long)rec[1] * stride];
+#pragma unroll
+            for (int e = 2; e <= ND; ++e) P *= tab[(long)rec[e] * stride];
+            acc += w[(long)rec[0] * ws] * P;
+        }
+        C[q] = acc;
+    }
+}
+
+__device__ __forceinline__ void coeff_groups(const MapDev& M, const CompDev& c, const double* w, long ws,
+                                             const double* tab, int stride, double* C) {
+    switch (M.nd) {
+        case 1: coeff_groups_nd<1>(M, c, w, ws, tab, stride, C); break;
+        case 2: coeff_groups_nd<2>(M, c, w, ws, tab, stride, C); break;
+        case 3: coeff_groups_nd<3>(M, c, w, ws, tab, stride, C); break;
+        case 4: coeff_groups_nd<4>(M, c, w, ws, tab, stride, C); break;
+        case 5: coeff_groups_nd<5>(M, c, w, ws, tab, stride, C); break;
+        case 6: coeff_groups_nd<6>(M, c, w, ws, tab, stride, C); break;
+        case 7: coeff_groups_nd<7>(M, c, w, ws, tab, stride, C); break;
+        default: coeff_groups_nd<8>(M, c, w, ws, tab, stride, C); break;
+    }
+}
+
+// Visit every (real) term: op(coeff index, P_j, q).
+template <int ND, class Op>
+__device__ __forceinline__ void visit_terms_nd(const MapDev& M, const CompDev& c, const double* tab, int stride,
+                                               Op& op) {
+#pragma unroll
+    for (int q = 0; q <= MAXP; ++q) {
+        if (q > c.p) break;
+        const GroupDev g = M.groups[c.groupOff + q];
+        for (int j = g.start; j < g.start + g.count; ++j) {
+            const uint32_t* rec = M.terms + (long)j * (ND + 1);
+            double P = tab[(long)rec[1] * stride];
+#pragma unroll
+            for (int e = 2; e <= ND; ++e) P *= tab[(long)rec[e] * stride];
+            op(rec[0], P, q);
+        }
+    }
+}
+
+template <class Op>
+__device__ __forceinline__ void visit_terms(const MapDev& M, const CompDev& c, const double* tab, int stride,
+                                            Op& op) {
+    switch (M.nd) {
+        case 1: visit_terms_nd<1>(M, c, tab, stride, op); break;
+        case 2: visit_terms_nd<2>(M, c, tab, stride, op); break;
+        case 3: visit_terms_nd<3>(M, c, tab, stride, op); break;
+        case 4: visit_terms_nd<4>(M, c, tab, stride, op); break;
+        case 5: visit_terms_nd<5>(M, c, tab, stride, op); break;
+        case 6: visit_terms_nd<6>(M, c, tab, stride, op); break;
+        case 7: visit_terms_nd<7>(M, c, tab, stride, op); break;
+        default: visit_terms_nd<8>(M, c, tab, stride, op); break;
+    }
+}
+
+// Diagonal coefficients of component c from its group sums.
+__device__ __forceinline__ void make_diag(const MapDev& M, const CompDev& c, const double* C, double xk, double nug,
+                                          DiagCoef& dc) {
+    dc.p = c.p;
+    dc.x = xk;
+    dc.nug = nug;
+#pragma unroll
+    for (int q = 0; q <= MAXP; ++q) dc.cd[q] = (q <= c.p) ? C[q] * M.rho[q] : 0.0;
+}
+
+// f(x_<k, 0) = sum_q C_q phi~_q(0).
+__device__ __forceinline__ double f_at_zero(const MapDev& M, const CompDev& c, const double* C) {
+    double f = 0.0;
+#pragma unroll
+    for (int q = 0; q <= MAXP; ++q) {
+        if (q > c.p) break;
+        f += C[q] * M.phi0[q];
+    }
+    return f;
+}
+
+// ---------------------------------------------------------------------------
+// Adaptive quadrature of F(t) = x (g(d_k f(x_<k, t x)) + nugget) on [0, 1]
+// with optional adjoint accumulation at accepted leaves.
+//
+// FL_COEF accumulates S'_q = int h(t) B'_q(t x) dt with h = x g'(d_k f).
+// FL_INPUT additionally accumulates U = int (g + nugget) + x t g'(d_k f) d^2_k f dt,
+// the x_k-derivative of the integral (MonotoneIntegrand.h:193-202).
+template <int FL>
+struct NodeV;
+template <>
+struct NodeV<FL_VAL> {
+    double F;
+};
+template <>
+struct NodeV<FL_COEF> {
+    double F, h;
+};
+template <>
+struct NodeV<FL_INPUT> {
+    double F, h, g1;
+};
+
+template <int BK, int PK, int FL>
+__device__ __forceinline__ NodeV<FL> eval_node(const DiagCoef& dc, double t) {
+    double z = diag_eval<BK, false>(dc, t * dc.x, nullptr);
+    PosF<PK> gv = posfunc<PK>(z);
+    NodeV<FL> v;
+    v.F = dc.x * (gv.g + dc.nug);
+    if constexpr (FL >= FL_COEF) v.h = dc.x * gv.gp;
+    if constexpr (FL == FL_INPUT) v.g1 = gv.g + dc.nug;
+    return v;
+}
+
+struct AdjAcc {
+    double S[MAXP + 1];
+    double U;
+};
+
+// Adds w * v(t) at one node to the adjoint accumulators.
+template <int BK, int FL>
+__device__ __forceinline__ void adj_node(const DiagCoef& dc, double t, double w, const NodeV<FL>& v, AdjAcc& A) {
+    if constexpr (FL >= FL_COEF) {
+        double bq[MAXP + 1];
+        const double y = t * dc.x;
+        diag_basis<BK>(dc.p, y, bq);
+        const double wh = w * v.h;
+#pragma unroll
+        for (int q = 1; q <= MAXP; ++q) {
+            if (q > dc.p) break;
+            A.S[q] += wh * bq[q];
+        }
+        if constexpr (FL == FL_INPUT) {
+            double z2;
+            diag_eval<BK, true>(dc, y, &z2);
+            A.U += w * (v.g1 + t * v.h * z2);
+        }
+    }
+}
+
+template <int FL>
+struct AsRec {
+    double a, b;
+    int level;
+    NodeV<FL> fa, fm, fb;
+};
+
+// AdaptiveSimpson::Integrate (Quadrature.h:553-676): depth-first, left first,
+// accepted intervals add intFine in left-to-right order.
+template <int BK, int PK, int FL>
+__device__ double integrate_as(const QuadDev& Q, const DiagCoef& dc, AdjAcc& A, int* nevals) {
+    AsRec<FL> stk[MAXSUB];
+    int sp = 0;
+    NodeV<FL> fa = eval_node<BK, PK, FL>(dc, 0.0);
+    NodeV<FL> fb = eval_node<BK, PK, FL>(dc, 1.0);
+    NodeV<FL> fm = eval_node<BK, PK, FL>(dc, 0.5);
+    int ne = 3;
+    double a = 0.0, b = 1.0, res = 0.0;
+    int level = 0;
+    while (true) {
+        const double m = 0.5 * (a + b);
+        const double lm = 0.5 * (a + m);
+        const double rm = 0.5 * (m + b);
+        NodeV<FL> flm = eval_node<BK, PK, FL>(dc, lm);
+        NodeV<FL> frm = eval_node<BK, PK, FL>(dc, rm);
+        ne += 2;
+        const double coarse = ((b - a) / 6.0) * (fa.F + 4.0 * fm.F + fb.F);
+        double fine = ((m - a) / 6.0) * (fa.F + 4.0 * flm.F + fm.F);
+        fine += ((b - m) / 6.0) * (fm.F + 4.0 * frm.F + fb.F);
+        const double err = fabs(fine - coarse);
+        const double tol = fmax(Q.relTol * fabs(coarse), Q.absTol);
+        if (((err < tol) || (level == Q.maxSub - 1)) && (level >= Q.minSub)) {
+            res += fine;
+            if constexpr (FL >= FL_COEF) {
+                const double wl = (m - a) / 6.0, wr = (b - m) / 6.0;
+                adj_node<BK, FL>(dc, a, wl, fa, A);
+                adj_node<BK, FL>(dc, lm, 4.0 * wl, flm, A);
+                adj_node<BK, FL>(dc, m, wl + wr, fm, A);
+                adj_node<BK, FL>(dc, rm, 4.0 * wr, frm, A);
+                adj_node<BK, FL>(dc, b, wr, fb, A);
+            }
+            if (sp == 0) break;
+            --sp;
+            a = stk[sp].a;
+            b = stk[sp].b;
+            level = stk[sp].level;
+            fa = stk[sp].fa;
+            fm = stk[sp].fm;
+            fb = stk[sp].fb;
+        } else {
+            stk[sp].a = m;
+            stk[sp].b = b;
+            stk[sp].level = level + 1;
+            stk[sp].fa = fm;
+            stk[sp].fm = frm;
+            stk[sp].fb = fb;
+            ++sp;
+            b = m;
+            fb = fm;
+            fm = flm;
+            ++level;
+        }
+    }
+    if (nevals) *nevals += ne;
+    return res;
+}
+
+template <int FL>
+struct AccRec {
+    double a, b;
+    int level;
+    NodeV<FL> fa, fb;
+};
+
+// AdaptiveClenshawCurtis::Integrate (Quadrature.h:819-968).  NF is the fine
+// rule size when known at compile time (5 for quadPts = 5), or MAXNF with the
+// runtime size in Q.nf.
+template <int BK, int PK, int FL, int NF>
+__device__ double integrate_acc(const QuadDev& Q, const DiagCoef& dc, AdjAcc& A, int* nevals) {
+    AccRec<FL> stk[MAXSUB];
+    int sp = 0;
+    const int nf = (NF == MAXNF) ? Q.nf : NF;
+    const int nc = (nf - 1) / 2 + 1;
+    const int cRight = nc - 1, cMid = cRight / 2;
+    const int fRight = nf - 1, fMid = fRight / 2;
+    NodeV<FL> fa = eval_node<BK, PK, FL>(dc, 0.0);
+    NodeV<FL> fb = eval_node<BK, PK, FL>(dc, 1.0);
+    int ne = 2;
+    double a = 0.0, b = 1.0, res = 0.0;
+    int level = 0;
+    NodeV<FL> inner[NF];
+    while (true) {
+        const double mid = 0.5 * (a + b);
+        const double scale = 0.5 * (b - a);
+        NodeV<FL> fm = eval_node<BK, PK, FL>(dc, mid);
+        ne += 1;
+        double coarse = scale * Q.cw[0] * fa.F;
+        coarse += scale * Q.cw[cMid] * fm.F;
+        coarse += scale * Q.cw[cRight] * fb.F;
+        double fine = scale * Q.fw[0] * fa.F;
+        fine += scale * Q.fw[fMid] * fm.F;
+        fine += scale * Q.fw[fRight] * fb.F;
+#pragma unroll
+        for (int i = 1; i < NF; ++i) {
+            if (i >= cMid) break;
+            NodeV<FL> v = eval_node<BK, PK, FL>(dc, mid + scale * Q.cp[i]);
+            ne += 1;
+            coarse += scale * Q.cw[i] * v.F;
+            fine += scale * Q.fw[2 * i] * v.F;
+            inner[2 * i] = v;
+        }
+#pragma unroll
+        for (int i = 1; i < NF; ++i) {
+            if (i + cMid + 1 >= cRight + 1) break;
+            const int ii = cMid + i;
+            if (ii >= cRight) break;
+            NodeV<FL> v = eval_node<BK, PK, FL>(dc, mid + scale * Q.cp[ii]);
+            ne += 1;
+            coarse += scale * Q.cw[ii] * v.F;
+            fine += scale * Q.fw[2 * ii] * v.F;
+            inner[2 * ii] = v;
+        }
+#pragma unroll
+        for (int i = 1; i < NF; i += 2) {
+            if (i >= nf) break;
+            NodeV<FL> v = eval_node<BK, PK, FL>(dc, mid + scale * Q.fp[i]);
+            ne += 1;
+            fine += scale * Q.fw[i] * v.F;
+            inner[i] = v;
+        }
+        const double err = fabs(fine - coarse);
+        const double tol = fmax(Q.relTol * fabs(coarse), Q.absTol);
+        if (((err < tol) || (level == Q.maxSub - 1)) && (level >= Q.minSub)) {
+            res += fine;
+            if constexpr (FL >= FL_COEF) {
+                adj_node<BK, FL>(dc, a, scale * Q.fw[0], fa, A);
+                adj_node<BK, FL>(dc, mid, scale * Q.fw[fMid], fm, A);
+                adj_node<BK, FL>(dc, b, scale * Q.fw[fRight], fb, A);
+#pragma unroll
+                for (int i = 1; i < NF - 1; ++i) {
+                    if (i >= fRight) break;
+                    if (i == fMid) continue;
+                    adj_node<BK, FL>(dc, mid + scale * Q.fp[i], scale * Q.fw[i], inner[i], A);
+                }
+            }
+            if (sp == 0) break;
+            --sp;
+            a = stk[sp].a;
+            b = stk[sp].b;
+            level = stk[sp].level;
+            fa = stk[sp].fa;
+            fb = stk[sp].fb;
+        } else {
+            stk[sp].a = mid;
+            stk[sp].b = b;
+            stk[sp].level = level + 1;
+            stk[sp].fa = fm;
+            stk[sp].fb = fb;
+            ++sp;
+            b = mid;
+            fb = fm;
+            ++level;
+        }
+    }
+    if (nevals) *nevals += ne;
+    return res;
+}
+
+template <int BK, int PK, int QK, int FL>
+__device__ __forceinline__ double integrate(const QuadDev& Q, const DiagCoef& dc, AdjAcc& A, int* nevals) {
+    if constexpr (QK == QK_AS) return integrate_as<BK, PK, FL>(Q, dc, A, nevals);
+    else if constexpr (QK == QK_ACC5) return integrate_acc<BK, PK, FL, 5>(Q, dc, A, nevals);
+    else return integrate_acc<BK, PK, FL, MAXNF>(Q, dc, A, nevals);
+}
+
+__device__ __forceinline__ void zero_adj(AdjAcc& A) {
+#pragma unroll
+    for (int q = 0; q <= MAXP; ++q) A.S[q] = 0.0;
+    A.U = 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// Wavefront reduction of 16 per-lane values into per-row partial sums.
+// After the four exchange stages lane l holds, in v[0], the sum over its row
+// of 16 lanes of the values for term (l & 15).  Partners are l^8 (row_ror:8),
+// l^7 (row_half_mirror), l^3 (quad_perm 3,2,1,0) and l^1 (quad_perm 1,0,3,2):
+// each flips the deciding bit and leaves the higher bits unchanged.
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
+    int lo = __double2loint(v);
+    int hi = __double2hiint(v);
+    lo = __builtin_amdgcn_mov_dpp(lo, CTRL, 0xF, 0xF, true);
+    hi = __builtin_amdgcn_mov_dpp(hi, CTRL, 0xF, 0xF, true);
+    return __hiloint2double(hi, lo);
+}
+
+template <int H, int CTRL>
+__device__ __forceinline__ void bfly_stage(double* v, bool upper) {
+#pragma unroll
+    for (int i = 0; i < H; ++i) {
+        const double keep = upper ? v[H + i] : v[i];
+        const double send = upper ? v[i] : v[H + i];
+        v[i] = keep + dpp_mov<CTRL>(send);
+    }
+}
+
+__device__ __forceinline__ double row_reduce16(double* v, int lane) {
+    bfly_stage<8, 0x128>(v, (lane & 8) != 0);
+    bfly_stage<4, 0x141>(v, (lane & 4) != 0);
+    bfly_stage<2, 0x1B>(v, (lane & 2) != 0);
+    bfly_stage<1, 0xB1>(v, (lane & 1) != 0);
+    return v[0];
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+}  // namespace mpd

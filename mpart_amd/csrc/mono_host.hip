@@ -1,0 +1,943 @@
+// mono_host.hip -- C ABI (include/mpart_amd.h): map construction, device plans
+// and kernel launches.  Host code only; the kernels live in k_*.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/mpart_amd.h"
+#include "launchers.hpp"
+
+namespace {
+
+using mpd::CompDev;
+using mpd::GroupDev;
+using mpd::MapDev;
+
+thread_local std::string g_err;
+
+struct ArgError : std::invalid_argument {
+    using std::invalid_argument::invalid_argument;
+};
+struct RtError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct NotImpl : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return MPART_OK;
+    } catch (ArgError& e) {
+        g_err = e.what();
+        return MPART_INVALID_ARGUMENT;
+    } catch (std::invalid_argument& e) {
+        g_err = e.what();
+        return MPART_INVALID_ARGUMENT;
+    } catch (HipError& e) {
+        g_err = e.what();
+        return MPART_HIP_ERROR;
+    } catch (NotImpl& e) {
+        g_err = e.what();
+        return MPART_NOT_IMPLEMENTED;
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return MPART_RUNTIME_ERROR;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Compressed multi-index set (same layout as FixedMultiIndexSet).
+struct MultiSet {
+    unsigned dim = 0;
+    std::vector<uint32_t> starts, dims, orders, maxDeg;
+    unsigned size() const { return unsigned(starts.size()) - 1; }
+    void max_degrees() {
+        maxDeg.assign(dim, 0u);
+        for (size_t i = 0; i < dims.size(); ++i) maxDeg[dims[i]] = std::max(maxDeg[dims[i]], orders[i]);
+    }
+};
+
+// Total-order set in the enumeration order of FixedMultiIndexSet's total-order
+// constructor (src/MultiIndices/FixedMultiIndexSet.cpp:507-549): multi-indices
+// in lexicographic order of (alpha_0, ..., alpha_{dim-1}).
+MultiSet total_order(unsigned dim, unsigned order) {
+    MultiSet s;
+    s.dim = dim;
+    std::vector<unsigned> a(dim, 0u);
+    // Odometer over the last coordinate first; each state is a multi-index with
+    // |a| <= order visited in lexicographic order.
+    while (true) {
+        s.starts.push_back(uint32_t(s.dims.size()));
+        for (unsigned i = 0; i < dim; ++i)
+            if (a[i] > 0) {
+                s.dims.push_back(i);
+                s.orders.push_back(a[i]);
+            }
+        // advance
+        unsigned sum = 0;
+        for (unsigned v : a) sum += v;
+        int i = int(dim) - 1;
+        while (i >= 0) {
+            if (sum < order) {
+                a[i]++;
+                break;
+            }
+            sum -= a[i];
+            a[i] = 0;
+            --i;
+            if (i < 0) break;
+            // carry into a[i]; recompute sum condition on next loop
+        }
+        if (i < 0) break;
+    }
+    s.starts.push_back(uint32_t(s.dims.size()));
+    s.max_degrees();
+    return s;
+}
+
+MultiSet csr_set(unsigned dim, unsigned nterms, const uint32_t* starts, const uint32_t* dims, const uint32_t* orders) {
+    if (dim == 0) throw ArgError("CreateComponent: dimension must be positive");
+    MultiSet s;
+    s.dim = dim;
+    s.starts.assign(starts, starts + nterms + 1);
+    const unsigned nnz = starts[nterms];
+    s.dims.assign(dims, dims + nnz);
+    s.orders.assign(orders, orders + nnz);
+    for (unsigned t = 0; t < nterms; ++t) {
+        if (s.starts[t] > s.starts[t + 1]) throw ArgError("CreateComponent: nzStarts must be non-decreasing");
+        std::vector<std::pair<uint32_t, uint32_t>> e;
+        for (unsigned i = s.starts[t]; i < s.starts[t + 1]; ++i) {
+            if (s.dims[i] >= dim) throw ArgError("CreateComponent: nzDims entry out of range");
+            if (s.orders[i] == 0) throw ArgError("CreateComponent: nzOrders entries must be positive");
+            e.push_back({s.dims[i], s.orders[i]});
+        }
+        std::stable_sort(e.begin(), e.end(), [](auto& a, auto& b) { return a.first < b.first; });
+        for (size_t k = 0; k < e.size(); ++k) {
+            s.dims[s.starts[t] + k] = e[k].first;
+            s.orders[s.starts[t] + k] = e[k].second;
+        }
+    }
+    s.max_degrees();
+    return s;
+}
+
+// ---------------------------------------------------------------------------
+// Device buffer.
+struct DBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    ~DBuf() {
+        if (p) (void)hipFree(p);
+    }
+    void ensure(size_t b) {
+        if (b <= bytes) return;
+        if (p) hip_check(hipFree(p), "hipFree");
+        p = nullptr;
+        bytes = 0;
+        hip_check(hipMalloc(&p, b), "hipMalloc");
+        bytes = b;
+    }
+    template <class T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+};
+
+template <class T>
+void upload(DBuf& b, const std::vector<T>& v) {
+    b.ensure(std::max<size_t>(v.size() * sizeof(T), 8));
+    if (!v.empty()) hip_check(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy");
+}
+
+unsigned factorial_u32(unsigned d) {
+    unsigned out = 1;
+    for (unsigned i = 2; i <= d; ++i) out *= i;
+    return out;
+}
+
+int g_numCU = 0;
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+struct mpart_map_s {
+    mpart_options opts{};
+    bool triangular = false;
+    unsigned inDim = 0, outDim = 0, numCoeffs = 0;
+    std::vector<MultiSet> comps;  // per component
+    std::vector<unsigned> coeffOff;
+    int variant = 0;
+    bool plain = false;
+
+    // device plan
+    MapDev dev{};
+    DBuf dComps, dGroups, dTerms, dSlotDim, dDimSlot, dDimDeg;
+
+    // coefficients
+    DBuf ownCoeffs;
+    double* coeffs = nullptr;
+    long coeffStride = 1;
+
+    // work buffers
+    DBuf partialGrad, partialLoss, gtab, wT, wLd, wG, wLG, wFull;
+};
+
+namespace {
+
+void build_plan(mpart_map_s& m) {
+    const mpart_options& o = m.opts;
+    if (o.basisType == MPART_BASIS_PHYSICIST_HERMITE)
+        throw NotImpl("basisType PhysicistHermite is not provided by this library");
+    if (o.quadType == MPART_QUAD_CLENSHAW_CURTIS)
+        throw NotImpl("quadType ClenshawCurtis (non-adaptive) is not provided by this library");
+    if (o.posFuncType != MPART_POSFUNC_EXP && o.posFuncType != MPART_POSFUNC_SOFTPLUS)
+        throw ArgError("unknown posFuncType");
+    if (o.quadMaxSub < 1 || o.quadMaxSub > (unsigned)mpd::MAXSUB)
+        throw ArgError("quadMaxSub must be in [1, " + std::to_string(mpd::MAXSUB) + "]");
+    if (o.quadMinSub >= o.quadMaxSub)
+        throw ArgError("quadMinSub must be smaller than quadMaxSub (the subdivision depth is bounded by quadMaxSub)");
+    if (o.nugget < 0) throw ArgError("nugget must be non-negative");
+
+    const int bk = (o.basisType == MPART_BASIS_HERMITE_FUNCTIONS) ? mpd::BK_HF : mpd::BK_PH;
+    const int pk = (o.posFuncType == MPART_POSFUNC_EXP) ? mpd::PK_EXP : mpd::PK_SP;
+    MapDev& M = m.dev;
+    std::memset(&M, 0, sizeof(M));
+
+    // Quadrature.
+    M.quad.absTol = o.quadAbsTol;
+    M.quad.relTol = o.quadRelTol;
+    M.quad.maxSub = (int)o.quadMaxSub;
+    M.quad.minSub = (int)o.quadMinSub;
+    int qk = mpd::QK_AS;
+    if (o.quadType == MPART_QUAD_ADAPTIVE_CLENSHAW_CURTIS) {
+        if (o.quadPts < 3) throw ArgError("AdaptiveClenshawCurtis needs quadPts >= 4");
+        // level = floor(log2(quadPts - 2)) (src/MapFactoryImpl4.cpp:18)
+        const unsigned level = (unsigned)std::log2(o.quadPts - 2);
+        if (level < 1) throw ArgError("AdaptiveClenshawCurtis needs a nested level >= 1 (quadPts >= 4)");
+        const unsigned nc = (unsigned)std::pow(2, level) + 1, nf = (unsigned)std::pow(2, level + 1) + 1;
+        if (nf > (unsigned)mpd::MAXNF) throw NotImpl("AdaptiveClenshawCurtis with quadPts > 17 is not provided");
+        // Clenshaw-Curtis rule (MParT/Quadrature.h:212-258).
+        auto rule = [](unsigned n, double* w, double* p) {
+            for (unsigned i = 0; i < n; ++i) p[i] = std::cos((n - (i + 1)) * M_PI / (n - 1));
+            p[0] = -1.0;
+            if (n % 2 == 1) p[(n - 1) / 2] = 0.0;
+            p[n - 1] = 1.0;
+            for (unsigned i = 0; i < n; ++i) {
+                w[i] = 1.0;
+                double theta = i * M_PI / (n - 1);
+                for (unsigned j = 0; j < (n - 1) / 2; ++j) {
+                    double b = (2 * (j + 1) == (n - 1)) ? 1.0 : 2.0;
+                    w[i] = w[i] - b * std::cos(2.0 * (j + 1) * theta) / (4 * (j + 1) * (j + 1) - 1);
+                }
+            }
+            w[0] = w[0] / (n - 1.0);
+            for (unsigned i = 1; i < (n - 1); ++i) w[i] = 2.0 * w[i] / (n - 1.0);
+            w[n - 1] = w[n - 1] / (n - 1.0);
+        };
+        rule(nc, M.quad.cw, M.quad.cp);
+        rule(nf, M.quad.fw, M.quad.fp);
+        M.quad.nc = (int)nc;
+        M.quad.nf = (int)nf;
+        qk = (nf == 5) ? mpd::QK_ACC5 : mpd::QK_ACC;
+    }
+    m.variant = bk * 100 + pk * 10 + qk;
+
+    // Basis constants.
+    M.normalize = (bk == mpd::BK_PH && o.basisNorm) ? 1 : 0;
+    for (int k = 0; k <= mpd::MAXORD; ++k) M.norm[k] = std::sqrt(std::sqrt(2.0 * M_PI) * factorial_u32(k));
+    M.hfPi = std::pow(M_PI, -0.25);
+    M.hfSqrt2 = std::sqrt(2.0);
+    {
+        double fact = 1;
+        for (int i = 0; i <= mpd::MAXORD; ++i) {
+            M.hfA[i] = (i >= 1) ? std::sqrt(0.5 * (i - 1)) : 0.0;
+            M.hfB[i] = std::sqrt(0.5 * i);
+            if (i >= 1) fact *= i;
+            M.hfS[i] = (i == 0) ? 1.0 : std::pow(std::pow(2, i) * fact, -0.5);
+        }
+    }
+    // phi~_q(0) and the derivative folding rho_q.
+    M.phi0[0] = 1.0;
+    M.rho[0] = 0.0;
+    if (bk == mpd::BK_PH) {
+        double hm2 = 1.0, hm1 = 0.0;  // He_0(0), He_1(0)
+        for (int q = 1; q <= mpd::MAXP; ++q) {
+            double h = (q == 1) ? 0.0 : (1.0 * 0.0 + 0.0) * hm1 - (q - 1.0) * hm2;
+            if (q >= 2) {
+                hm2 = hm1;
+                hm1 = h;
+            }
+            M.phi0[q] = M.normalize ? h / M.norm[q] : h;
+            M.rho[q] = M.normalize ? 1.0 / M.norm[q] : 1.0;
+        }
+    } else {
+        // EvaluateAll at 0: phi_1 = 0, phi_2 = pi^-1/4, phi_3 = 0, ...
+        double v[mpd::MAXP + 1];
+        v[0] = 1.0;
+        v[1] = 0.0;
+        if (mpd::MAXP >= 2) v[2] = std::pow(M_PI, -0.25) * std::exp(-0.5 * 0.0 * 0.0);
+        if (mpd::MAXP >= 3) v[3] = std::sqrt(2.0) * 0.0 * v[2];
+        for (int i = 2; i + 2 <= mpd::MAXP; ++i)
+            v[i + 2] = (0.0 * v[i + 1] - std::sqrt(0.5 * (i - 1)) * v[i]) / std::sqrt(0.5 * i);
+        for (int q = 1; q <= mpd::MAXP; ++q) {
+            M.phi0[q] = v[q];
+            M.rho[q] = (q == 1) ? 1.0 : M.hfPi * M.hfS[q - 2];
+        }
+    }
+    M.nugget = o.nugget;
+
+    // Off-diagonal table slots: max off-diagonal degree of every coordinate.
+    const unsigned D = m.inDim;
+    std::vector<int> dimDeg(D, 0), dimSlot(D, 0), slotDim(1, -1);
+    for (auto& s : m.comps)
+        for (unsigned d = 0; d + 1 < s.dim; ++d) dimDeg[d] = std::max<int>(dimDeg[d], (int)s.maxDeg[d]);
+    int nslots = 1;
+    for (unsigned d = 0; d < D; ++d) {
+        if (dimDeg[d] > mpd::MAXORD)
+            throw NotImpl("off-diagonal degree above " + std::to_string(mpd::MAXORD) + " is not provided");
+        dimSlot[d] = nslots;
+        for (int k = 0; k < dimDeg[d]; ++k) slotDim.push_back((int)d);
+        nslots += dimDeg[d];
+    }
+
+    // Term records grouped by the order of the last coordinate.
+    int nd = 1;
+    for (auto& s : m.comps)
+        for (unsigned t = 0; t < s.size(); ++t) {
+            int off = 0;
+            for (unsigned i = s.starts[t]; i < s.starts[t + 1]; ++i)
+                if (s.dims[i] != s.dim - 1) off++;
+            nd = std::max(nd, off);
+        }
+    if (nd > mpd::MAXND) throw NotImpl("terms with more than 8 off-diagonal factors are not provided");
+    const int trec = nd + 1;
+    std::vector<CompDev> comps;
+    std::vector<GroupDev> groups;
+    std::vector<uint32_t> terms;
+    int recCount = 0;
+    unsigned row = 0;
+    for (size_t c = 0; c < m.comps.size(); ++c) {
+        const MultiSet& s = m.comps[c];
+        CompDev cd{};
+        cd.dim = (int)s.dim;
+        cd.nterms = (int)s.size();
+        cd.coeffOff = (int)m.coeffOff[c];
+        cd.outRow = (int)row;
+        row += 1;
+        cd.p = (int)s.maxDeg[s.dim - 1];
+        if (cd.p > mpd::MAXP) throw NotImpl("last-coordinate degree above " + std::to_string(mpd::MAXP) + " is not provided");
+        cd.groupOff = (int)groups.size();
+        std::vector<std::vector<unsigned>> byq(cd.p + 1);
+        for (unsigned t = 0; t < s.size(); ++t) {
+            unsigned q = 0;
+            if (s.starts[t + 1] > s.starts[t] && s.dims[s.starts[t + 1] - 1] == s.dim - 1) q = s.orders[s.starts[t + 1] - 1];
+            byq[q].push_back(t);
+        }
+        for (int q = 0; q <= cd.p; ++q) {
+            GroupDev g{};
+            g.start = recCount;
+            g.count = (int)byq[q].size();
+            g.countPad = (g.count + 15) / 16 * 16;
+            for (int r = 0; r < g.countPad; ++r) {
+                std::vector<uint32_t> rec(trec, 0u);
+                if (r < g.count) {
+                    unsigned t = byq[q][r];
+                    rec[0] = t;
+                    int e = 1;
+                    for (unsigned i = s.starts[t]; i < s.starts[t + 1]; ++i) {
+                        if (s.dims[i] == s.dim - 1) continue;
+                        rec[e++] = uint32_t(dimSlot[s.dims[i]] + (int)s.orders[i] - 1);
+                    }
+                } else {
+                    rec[0] = mpd::DUMMY;
+                }
+                terms.insert(terms.end(), rec.begin(), rec.end());
+            }
+            recCount += g.countPad;
+            groups.push_back(g);
+        }
+        comps.push_back(cd);
+    }
+    upload(m.dComps, comps);
+    upload(m.dGroups, groups);
+    upload(m.dTerms, terms);
+    upload(m.dSlotDim, slotDim);
+    upload(m.dDimSlot, dimSlot);
+    upload(m.dDimDeg, dimDeg);
+    M.comps = m.dComps.as<CompDev>();
+    M.groups = m.dGroups.as<GroupDev>();
+    M.terms = m.dTerms.as<uint32_t>();
+    M.slotDim = m.dSlotDim.as<int>();
+    M.dimSlot = m.dDimSlot.as<int>();
+    M.dimDeg = m.dDimDeg.as<int>();
+    M.trec = trec;
+    M.nd = nd;
+    M.nslots = nslots;
+    M.D = (int)D;
+    M.ncomp = (int)comps.size();
+    M.numCoeffs = (int)m.numCoeffs;
+    int pmax = 0;
+    for (auto& cd : comps) pmax = std::max(pmax, cd.p);
+    M.pmax = pmax;
+}
+
+void init_coeffs_zero(mpart_map_s& m) {
+    m.ownCoeffs.ensure(std::max<size_t>(8, m.numCoeffs * sizeof(double)));
+    hip_check(hipMemset(m.ownCoeffs.p, 0, std::max<size_t>(8, m.numCoeffs * sizeof(double))), "hipMemset");
+    m.coeffs = m.ownCoeffs.as<double>();
+    m.coeffStride = 1;
+}
+
+mpart_map_s* make_map(const mpart_options& o, std::vector<MultiSet> comps, bool triangular) {
+    auto m = std::make_unique<mpart_map_s>();
+    m->opts = o;
+    m->triangular = triangular;
+    m->comps = std::move(comps);
+    m->inDim = m->comps.back().dim;
+    m->outDim = (unsigned)m->comps.size();
+    unsigned off = 0;
+    for (size_t i = 0; i < m->comps.size(); ++i) {
+        if (i > 0 && m->comps[i].dim != m->comps[i - 1].dim + 1)
+            throw ArgError("TriangularMap: the input dimension of component " + std::to_string(i) +
+                           " must be one more than that of component " + std::to_string(i - 1));
+        m->coeffOff.push_back(off);
+        off += m->comps[i].size();
+    }
+    m->numCoeffs = off;
+    build_plan(*m);
+    init_coeffs_zero(*m);
+    return m.release();
+}
+
+void check_map(mpart_map m) {
+    if (!m) throw ArgError("null map handle");
+}
+void check_coeffs(mpart_map m) {
+    if (!m->coeffs) throw RtError("The coefficients have not been set for this map");
+}
+hipStream_t S(mpart_stream s) { return static_cast<hipStream_t>(s); }
+
+constexpr size_t LDS_LIMIT = 64 * 1024;
+
+// Block size and table placement of a lane-per-point launch.
+struct Shape {
+    mpd::Launch L;
+    double* gtab;
+};
+
+Shape shape_for(mpart_map_s& m, long N, hipStream_t s, int tables = 1) {
+    Shape sh{};
+    const size_t perLane = (size_t)m.dev.nslots * tables * sizeof(double);
+    int block = 256;
+    while (block > 64 && perLane * block > LDS_LIMIT) block /= 2;
+    sh.L.block = dim3(block);
+    const long grid = std::max<long>(1, (N + block - 1) / block);
+    sh.L.grid = dim3((unsigned)grid);
+    sh.L.stream = s;
+    if (perLane * block <= LDS_LIMIT) {
+        sh.L.smem = perLane * block;
+        sh.gtab = nullptr;
+    } else {
+        sh.L.smem = 0;
+        m.gtab.ensure(perLane * (size_t)grid * block);
+        sh.gtab = m.gtab.as<double>();
+    }
+    return sh;
+}
+
+void check_view(const char* what, int64_t rows, int64_t cols, int64_t erows, int64_t ecols) {
+    if (rows != erows || cols != ecols)
+        throw ArgError(std::string(what) + ": expected a " + std::to_string(erows) + " x " + std::to_string(ecols) +
+                       " matrix, got " + std::to_string(rows) + " x " + std::to_string(cols));
+}
+
+void do_evaluate(mpart_map_s& m, const mpart_cview& p, const mpart_view& o, int32_t* nevals, hipStream_t s) {
+    check_coeffs(&m);
+    check_view("Evaluate: pts", p.rows, p.cols, m.inDim, p.cols);
+    check_view("Evaluate: output", o.rows, o.cols, m.outDim, p.cols);
+    if (p.cols == 0) return;
+    Shape sh = shape_for(m, p.cols, s);
+    if (m.plain) {
+        for (int c = 0; c < m.dev.ncomp; ++c)
+            hip_check(mpd::launch_eval(m.variant, sh.L, m.dev, m.coeffs, m.coeffStride, p.ptr, p.row_stride,
+                                       p.col_stride, p.cols, o.ptr, o.row_stride, o.col_stride, nullptr, 0, 0,
+                                       nevals, c, c + 1, sh.gtab),
+                      "launch evaluate");
+    } else {
+        hip_check(mpd::launch_eval(m.variant, sh.L, m.dev, m.coeffs, m.coeffStride, p.ptr, p.row_stride, p.col_stride,
+                                   p.cols, o.ptr, o.row_stride, o.col_stride, nullptr, 0, 0, nevals, 0,
+                                   m.dev.ncomp, sh.gtab),
+                  "launch evaluate");
+    }
+}
+
+void do_logdet(mpart_map_s& m, const mpart_cview& p, double* out, long os, hipStream_t s) {
+    check_coeffs(&m);
+    check_view("LogDeterminant: pts", p.rows, p.cols, m.inDim, p.cols);
+    if (!m.opts.contDeriv) throw NotImpl("LogDeterminant with contDeriv = false is not provided yet");
+    if (p.cols == 0) return;
+    Shape sh = shape_for(m, p.cols, s);
+    if (m.plain) {
+        for (int c = 0; c < m.dev.ncomp; ++c)
+            hip_check(mpd::launch_eval(m.variant, sh.L, m.dev, m.coeffs, m.coeffStride, p.ptr, p.row_stride,
+                                       p.col_stride, p.cols, nullptr, 0, 0, out, os, c > 0 ? 1 : 0, nullptr, c, c + 1,
+                                       sh.gtab),
+                      "launch logdet");
+    } else {
+        hip_check(mpd::launch_eval(m.variant, sh.L, m.dev, m.coeffs, m.coeffStride, p.ptr, p.row_stride, p.col_stride,
+                                   p.cols, nullptr, 0, 0, out, os, 0, nullptr, 0, m.dev.ncomp, sh.gtab),
+                  "launch logdet");
+    }
+}
+
+void do_coeff_grad(mpart_map_s& m, const mpart_cview& p, const mpart_cview& sens, const mpart_view& o,
+                   hipStream_t s) {
+    check_coeffs(&m);
+    check_view("CoeffGrad: pts", p.rows, p.cols, m.inDim, p.cols);
+    check_view("CoeffGrad: sens", sens.rows, sens.cols, m.outDim, p.cols);
+    check_view("CoeffGrad: output", o.rows, o.cols, m.numCoeffs, p.cols);
+    if (p.cols == 0) return;
+    Shape sh = shape_for(m, p.cols, s);
+    for (int c = 0; c < m.dev.ncomp; ++c)
+        hip_check(mpd::launch_coeff_grad(m.variant, sh.L, m.dev, m.coeffs, m.coeffStride, p.ptr, p.row_stride,
+                                         p.col_stride, p.cols, sens.ptr, sens.row_stride, sens.col_stride, o.ptr,
+                                         o.row_stride, o.col_stride, c, sh.gtab),
+                  "launch coeff_grad");
+}
+
+void do_logdet_coeff_grad(mpart_map_s& m, const mpart_cview& p, const mpart_view& o, hipStream_t s) {
+    check_coeffs(&m);
+    check_view("LogDeterminantCoeffGrad: pts", p.rows, p.cols, m.inDim, p.cols);
+    check_view("LogDeterminantCoeffGrad: output", o.rows, o.cols, m.numCoeffs, p.cols);
+    if (!m.opts.contDeriv) throw NotImpl("LogDeterminantCoeffGrad with contDeriv = false is not provided yet");
+    if (p.cols == 0) return;
+    Shape sh = shape_for(m, p.cols, s);
+    for (int c = 0; c < m.dev.ncomp; ++c)
+        hip_check(mpd::launch_logdet_coeff_grad(m.variant, sh.L, m.dev, m.coeffs, m.coeffStride, p.ptr, p.row_stride,
+                                                p.col_stride, p.cols, o.ptr, o.row_stride, o.col_stride, c, sh.gtab),
+                  "launch logdet_coeff_grad");
+}
+
+void do_input_grad(mpart_map_s& m, const mpart_cview& p, const mpart_cview* sens, const mpart_view& o, int mode,
+                   hipStream_t s) {
+    check_coeffs(&m);
+    const char* nm = mode == 0 ? "Gradient" : "LogDeterminantInputGrad";
+    check_view((std::string(nm) + ": pts").c_str(), p.rows, p.cols, m.inDim, p.cols);
+    if (sens) check_view((std::string(nm) + ": sens").c_str(), sens->rows, sens->cols, m.outDim, p.cols);
+    check_view((std::string(nm) + ": output").c_str(), o.rows, o.cols, m.inDim, p.cols);
+    if (mode == 1 && !m.opts.contDeriv)
+        throw ArgError("Discrete derivative version is not implemented yet (To Do)");
+    if (p.cols == 0) return;
+    hip_check(mpd::launch_fill(o.ptr, o.rows, o.cols, o.row_stride, o.col_stride, 0.0, s), "launch fill");
+    Shape sh = shape_for(m, p.cols, s, 2);
+    for (int c = 0; c < m.dev.ncomp; ++c)
+        hip_check(mpd::launch_input_grad(m.variant, sh.L, m.dev, m.coeffs, m.coeffStride, p.ptr, p.row_stride,
+                                         p.col_stride, p.cols, sens ? sens->ptr : nullptr, sens ? sens->row_stride : 0,
+                                         sens ? sens->col_stride : 0, o.ptr, o.row_stride, o.col_stride, c, mode,
+                                         sh.gtab),
+                  "launch input_grad");
+}
+
+void do_inverse(mpart_map_s& m, const mpart_cview& x1, const mpart_cview& r, const mpart_view& o, double xtol,
+                double ytol, hipStream_t s) {
+    check_coeffs(&m);
+    if (xtol < 0) throw ArgError("Invalid tolerance \"xtol\" given to MonotoneComponent::Inverse.  Value must be non-negative");
+    if (ytol < 0) throw ArgError("Invalid tolerance \"ytol\" given to MonotoneComponent::Inverse.  Value must be non-negative");
+    if (ytol <= 2.220446049250313e-16 && xtol <= 2.220446049250313e-16)
+        throw ArgError("Invalid tolerances given to MonotoneComponent::Inverse.  Either \"xtol\" or \"ytol\" must be nonzero");
+    const long N = r.cols;
+    check_view("Inverse: r", r.rows, r.cols, m.outDim, N);
+    check_view("Inverse: output", o.rows, o.cols, m.outDim, N);
+    if (N == 0) return;
+    if (!m.triangular) {
+        if (x1.rows != m.inDim)
+            throw ArgError("Inverse: x1 must have " + std::to_string(m.inDim) + " rows");
+        if (x1.cols != 1 && x1.cols != N)
+            throw ArgError("Invalid argument sizes given to MonotoneComponent::Inverse. The number of x points is " +
+                           std::to_string(x1.cols) + ", but the number of y points is " + std::to_string(N) +
+                           ".  If the number of xs is not 1 then it must match the number of ys.");
+        Shape sh = shape_for(m, N, s);
+        hip_check(mpd::launch_inverse(m.variant, sh.L, m.dev, m.coeffs, m.coeffStride, x1.ptr, x1.rows, x1.cols,
+                                      x1.row_stride, x1.col_stride, r.ptr, r.col_stride, N, o.ptr, o.col_stride, 0,
+                                      xtol, ytol, sh.gtab),
+                  "launch inverse");
+        return;
+    }
+    // TriangularMap::InverseImpl (src/TriangularMap.cpp:147-184): components
+    // run in order on a zero-initialized full point buffer.
+    const long ip = m.inDim, op = m.outDim, extra = ip - op;
+    check_view("Inverse: x1", x1.rows, x1.rows ? x1.cols : N, extra, N);
+    m.wFull.ensure((size_t)ip * N * sizeof(double));
+    double* full = m.wFull.as<double>();
+    hip_check(mpd::launch_fill(full, ip, N, N, 1, 0.0, s), "launch fill");
+    if (extra > 0)
+        hip_check(mpd::launch_copy(x1.ptr, extra, N, x1.row_stride, x1.col_stride, full, N, 1, s), "launch copy");
+    Shape sh = shape_for(m, N, s);
+    for (int c = 0; c < m.dev.ncomp; ++c) {
+        const long dimc = m.comps[c].dim;
+        hip_check(mpd::launch_inverse(m.variant, sh.L, m.dev, m.coeffs, m.coeffStride, full, dimc, N, N, 1,
+                                      r.ptr + (long)c * r.row_stride, r.col_stride, N, full + (extra + c) * N, 1, c,
+                                      xtol, ytol, sh.gtab),
+                  "launch inverse");
+    }
+    hip_check(mpd::launch_copy(full + extra * N, op, N, N, 1, o.ptr, o.row_stride, o.col_stride, s), "launch copy");
+}
+
+// Launch shape of the fused kernel: persistent grid, LDS accumulators when
+// they fit, otherwise global atomics.
+void do_fused(mpart_map_s& m, const mpart_cview& p, const double* sens, long srs, long scs, const double* lsens,
+              long lss, int klMode, double* loss, double* grad, double scale, hipStream_t s) {
+    check_coeffs(&m);
+    const long N = p.cols;
+    const long K = m.numCoeffs;
+    int block = 256;
+    size_t tabLane = (size_t)m.dev.nslots * sizeof(double);
+    auto smem_for = [&](int b, bool ldsAcc) { return (ldsAcc ? (size_t)(b / 64) * K * sizeof(double) : 0) + tabLane * b; };
+    bool ldsAcc = true;
+    while (block > 64 && smem_for(block, true) > LDS_LIMIT) block /= 2;
+    if (smem_for(block, true) > LDS_LIMIT) ldsAcc = false;
+    block = 256;
+    while (block > 64 && smem_for(block, ldsAcc) > LDS_LIMIT) block /= 2;
+    const bool tabLds = smem_for(block, ldsAcc) <= LDS_LIMIT;
+    const size_t smem = tabLds ? smem_for(block, ldsAcc) : (ldsAcc ? (size_t)(block / 64) * K * sizeof(double) : 0);
+    const long ntiles = std::max<long>(1, (N + block - 1) / block);
+    if (g_numCU == 0) {
+        int dev = 0;
+        hip_check(hipGetDevice(&dev), "hipGetDevice");
+        hip_check(hipDeviceGetAttribute(&g_numCU, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
+    }
+    const int occ = mpd::occupancy_fused(m.variant, block, smem);
+    const long grid = std::min<long>(ntiles, (long)g_numCU * occ);
+    mpd::Launch L{dim3((unsigned)grid), dim3(block), smem, s};
+    double* gtab = nullptr;
+    if (!tabLds) {
+        m.gtab.ensure(tabLane * (size_t)grid * block);
+        gtab = m.gtab.as<double>();
+    }
+    m.partialGrad.ensure(std::max<size_t>(8, (size_t)grid * K * sizeof(double)));
+    m.partialLoss.ensure(std::max<size_t>(8, (size_t)grid * sizeof(double)));
+    double* gradGlobal = nullptr;
+    if (!ldsAcc) {
+        gradGlobal = m.partialGrad.as<double>();
+        hip_check(mpd::launch_fill(gradGlobal, 1, K, K, 1, 0.0, s), "launch fill");
+    }
+    if (N > 0)
+        hip_check(mpd::launch_fused(m.variant, L, m.dev, m.coeffs, m.coeffStride, p.ptr, p.row_stride, p.col_stride, N,
+                                    sens, srs, scs, lsens, lss, klMode, m.partialGrad.as<double>(),
+                                    m.partialLoss.as<double>(), gtab, gradGlobal),
+                  "launch fused");
+    const long rows = (N > 0) ? (ldsAcc ? grid : 1) : 0;
+    hip_check(mpd::launch_reduce_rows(m.partialGrad.as<double>(), rows, K, scale, grad, s), "launch reduce");
+    if (loss)
+        hip_check(mpd::launch_reduce_rows(m.partialLoss.as<double>(), N > 0 ? grid : 0, 1, scale, loss, s),
+                  "launch reduce");
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* mpart_last_error(void) { return g_err.c_str(); }
+
+int mpart_init(void) {
+    return guarded([&] {
+        static std::once_flag once;
+        std::call_once(once, [] {
+            int n = 0;
+            if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return;
+        });
+    });
+}
+
+void mpart_default_options(mpart_options* o) {
+    o->basisType = MPART_BASIS_PROBABILIST_HERMITE;
+    o->posFuncType = MPART_POSFUNC_SOFTPLUS;
+    o->quadType = MPART_QUAD_ADAPTIVE_SIMPSON;
+    o->contDeriv = 1;
+    o->basisNorm = 1;
+    o->quadMaxSub = 30;
+    o->quadMinSub = 0;
+    o->quadPts = 5;
+    o->quadAbsTol = 1e-6;
+    o->quadRelTol = 1e-6;
+    o->nugget = 0.0;
+}
+
+int mpart_total_order_set(uint32_t dim, uint32_t order, uint32_t* nterms, uint32_t* nnz, uint32_t* nzStarts,
+                          uint32_t* nzDims, uint32_t* nzOrders) {
+    return guarded([&] {
+        if (dim == 0) throw ArgError("FixedMultiIndexSet: dimension must be positive");
+        MultiSet s = total_order(dim, order);
+        if (nterms) *nterms = s.size();
+        if (nnz) *nnz = (uint32_t)s.dims.size();
+        if (nzStarts) std::copy(s.starts.begin(), s.starts.end(), nzStarts);
+        if (nzDims) std::copy(s.dims.begin(), s.dims.end(), nzDims);
+        if (nzOrders) std::copy(s.orders.begin(), s.orders.end(), nzOrders);
+    });
+}
+
+int mpart_create_component(const mpart_options* opts, uint32_t dim, uint32_t nterms, const uint32_t* nzStarts,
+                           const uint32_t* nzDims, const uint32_t* nzOrders, mpart_map* out) {
+    return guarded([&] {
+        if (!opts || !out || !nzStarts) throw ArgError("null argument");
+        if (nterms == 0) throw ArgError("CreateComponent: empty multi-index set");
+        std::vector<MultiSet> c{csr_set(dim, nterms, nzStarts, nzDims, nzOrders)};
+        *out = make_map(*opts, std::move(c), false);
+    });
+}
+
+int mpart_create_total_order_component(const mpart_options* opts, uint32_t dim, uint32_t order, mpart_map* out) {
+    return guarded([&] {
+        if (!opts || !out) throw ArgError("null argument");
+        if (dim == 0) throw ArgError("CreateComponent: dimension must be positive");
+        std::vector<MultiSet> c{total_order(dim, order)};
+        *out = make_map(*opts, std::move(c), false);
+    });
+}
+
+int mpart_create_triangular(uint32_t inputDim, uint32_t outputDim, uint32_t totalOrder, const mpart_options* opts,
+                            mpart_map* out) {
+    return guarded([&] {
+        if (!opts || !out) throw ArgError("null argument");
+        if (outputDim == 0 || outputDim > inputDim)
+            throw ArgError("CreateTriangular: need 0 < outputDim <= inputDim");
+        const unsigned extra = inputDim - outputDim;
+        std::vector<MultiSet> c;
+        for (unsigned i = 0; i < outputDim; ++i) c.push_back(total_order(i + extra + 1, totalOrder));
+        *out = make_map(*opts, std::move(c), true);
+    });
+}
+
+int mpart_create_triangular_from(const mpart_map* comps, uint32_t n, mpart_map* out) {
+    return guarded([&] {
+        if (!comps || !out || n == 0) throw ArgError("null argument");
+        std::vector<MultiSet> c;
+        for (uint32_t i = 0; i < n; ++i) {
+            check_map(comps[i]);
+            if (comps[i]->triangular) throw NotImpl("nested triangular maps are not provided");
+            if (std::memcmp(&comps[i]->opts, &comps[0]->opts, sizeof(mpart_options)) != 0)
+                throw NotImpl("components with different MapOptions are not provided");
+            c.push_back(comps[i]->comps[0]);
+        }
+        mpart_map m = make_map(comps[0]->opts, std::move(c), true);
+        // moveCoeffs semantics (src/TriangularMap.cpp:40-60): copy each component's coefficients.
+        for (uint32_t i = 0; i < n; ++i)
+            hip_check(mpd::launch_copy(comps[i]->coeffs, 1, comps[i]->numCoeffs, 0, comps[i]->coeffStride,
+                                       m->coeffs + m->coeffOff[i], 0, 1, nullptr),
+                      "launch copy");
+        hip_check(hipStreamSynchronize(nullptr), "hipStreamSynchronize");
+        *out = m;
+    });
+}
+
+int mpart_destroy(mpart_map map) {
+    return guarded([&] { delete map; });
+}
+
+int mpart_map_dims(mpart_map m, uint32_t* i, uint32_t* o, uint32_t* c) {
+    return guarded([&] {
+        check_map(m);
+        if (i) *i = m->inDim;
+        if (o) *o = m->outDim;
+        if (c) *c = m->numCoeffs;
+    });
+}
+
+int mpart_map_num_components(mpart_map m, uint32_t* n) {
+    return guarded([&] {
+        check_map(m);
+        *n = (uint32_t)m->comps.size();
+    });
+}
+
+int mpart_component_multiset(mpart_map m, uint32_t i, uint32_t* dim, uint32_t* nterms, uint32_t* nnz,
+                             uint32_t* nzStarts, uint32_t* nzDims, uint32_t* nzOrders) {
+    return guarded([&] {
+        check_map(m);
+        if (i >= m->comps.size()) throw ArgError("component index out of range");
+        const MultiSet& s = m->comps[i];
+        if (dim) *dim = s.dim;
+        if (nterms) *nterms = s.size();
+        if (nnz) *nnz = (uint32_t)s.dims.size();
+        if (nzStarts) std::copy(s.starts.begin(), s.starts.end(), nzStarts);
+        if (nzDims) std::copy(s.dims.begin(), s.dims.end(), nzDims);
+        if (nzOrders) std::copy(s.orders.begin(), s.orders.end(), nzOrders);
+    });
+}
+
+int mpart_set_coeffs(mpart_map m, const double* c, int64_t len, int64_t stride, mpart_stream s) {
+    return guarded([&] {
+        check_map(m);
+        if (len != m->numCoeffs)
+            throw ArgError("SetCoeffs: expected " + std::to_string(m->numCoeffs) + " coefficients, got " +
+                           std::to_string(len));
+        m->ownCoeffs.ensure(std::max<size_t>(8, m->numCoeffs * sizeof(double)));
+        hip_check(mpd::launch_copy(c, 1, len, 0, stride, m->ownCoeffs.as<double>(), 0, 1, S(s)), "launch copy");
+        m->coeffs = m->ownCoeffs.as<double>();
+        m->coeffStride = 1;
+    });
+}
+
+int mpart_wrap_coeffs(mpart_map m, double* c, int64_t len, int64_t stride) {
+    return guarded([&] {
+        check_map(m);
+        if (len != m->numCoeffs)
+            throw ArgError("WrapCoeffs: expected " + std::to_string(m->numCoeffs) + " coefficients, got " +
+                           std::to_string(len));
+        m->coeffs = c;
+        m->coeffStride = stride;
+    });
+}
+
+int mpart_get_coeffs(mpart_map m, double* out, mpart_stream s) {
+    return guarded([&] {
+        check_map(m);
+        check_coeffs(m);
+        hip_check(mpd::launch_copy(m->coeffs, 1, m->numCoeffs, 0, m->coeffStride, out, 0, 1, S(s)), "launch copy");
+    });
+}
+
+int mpart_evaluate(mpart_map m, mpart_cview pts, mpart_view out, mpart_stream s) {
+    return guarded([&] {
+        check_map(m);
+        do_evaluate(*m, pts, out, nullptr, S(s));
+    });
+}
+
+int mpart_evaluate_counted(mpart_map m, mpart_cview pts, mpart_view out, int32_t* nevals, mpart_stream s) {
+    return guarded([&] {
+        check_map(m);
+        do_evaluate(*m, pts, out, nevals, S(s));
+    });
+}
+
+int mpart_log_determinant(mpart_map m, mpart_cview pts, double* out, int64_t stride, mpart_stream s) {
+    return guarded([&] {
+        check_map(m);
+        do_logdet(*m, pts, out, stride, S(s));
+    });
+}
+
+int mpart_gradient(mpart_map m, mpart_cview pts, mpart_cview sens, mpart_view out, mpart_stream s) {
+    return guarded([&] {
+        check_map(m);
+        do_input_grad(*m, pts, &sens, out, 0, S(s));
+    });
+}
+
+int mpart_coeff_grad(mpart_map m, mpart_cview pts, mpart_cview sens, mpart_view out, mpart_stream s) {
+    return guarded([&] {
+        check_map(m);
+        do_coeff_grad(*m, pts, sens, out, S(s));
+    });
+}
+
+int mpart_log_determinant_coeff_grad(mpart_map m, mpart_cview pts, mpart_view out, mpart_stream s) {
+    return guarded([&] {
+        check_map(m);
+        do_logdet_coeff_grad(*m, pts, out, S(s));
+    });
+}
+
+int mpart_log_determinant_input_grad(mpart_map m, mpart_cview pts, mpart_view out, mpart_stream s) {
+    return guarded([&] {
+        check_map(m);
+        do_input_grad(*m, pts, nullptr, out, 1, S(s));
+    });
+}
+
+int mpart_inverse(mpart_map m, mpart_cview x1, mpart_cview r, mpart_view out, double xtol, double ytol,
+                  mpart_stream s) {
+    return guarded([&] {
+        check_map(m);
+        do_inverse(*m, x1, r, out, xtol, ytol, S(s));
+    });
+}
+
+int mpart_kl_value_and_grad(mpart_map m, mpart_cview pts, double* loss, double* grad, mpart_stream s) {
+    return guarded([&] {
+        check_map(m);
+        check_view("KL: pts", pts.rows, pts.cols, m->inDim, pts.cols);
+        if (pts.cols == 0) throw ArgError("KL: no points");
+        if (!m->opts.contDeriv) throw NotImpl("KL with contDeriv = false is not provided yet");
+        do_fused(*m, pts, nullptr, 0, 0, nullptr, 0, 1, loss, grad, 1.0 / (double)pts.cols, S(s));
+    });
+}
+
+int mpart_coeff_vjp(mpart_map m, mpart_cview pts, mpart_cview sens, const double* lsens, int64_t lss, double* grad,
+                    mpart_stream s) {
+    return guarded([&] {
+        check_map(m);
+        check_view("vjp: pts", pts.rows, pts.cols, m->inDim, pts.cols);
+        if (sens.ptr) check_view("vjp: sens", sens.rows, sens.cols, m->outDim, pts.cols);
+        if (!m->opts.contDeriv) throw NotImpl("contDeriv = false is not provided yet");
+        do_fused(*m, pts, sens.ptr, sens.row_stride, sens.col_stride, lsens, lss, 0, nullptr, grad, 1.0, S(s));
+    });
+}
+
+int mpart_kl_value_and_grad_plain(mpart_map m, mpart_cview pts, double* loss, double* grad, mpart_stream s) {
+    return guarded([&] {
+        check_map(m);
+        check_coeffs(m);
+        const long N = pts.cols, K = m->numCoeffs, Mo = m->outDim;
+        check_view("KL: pts", pts.rows, pts.cols, m->inDim, N);
+        if (N == 0) throw ArgError("KL: no points");
+        if (!m->opts.contDeriv) throw NotImpl("KL with contDeriv = false is not provided yet");
+        hipStream_t st = S(s);
+        m->wT.ensure((size_t)Mo * N * sizeof(double));
+        m->wLd.ensure((size_t)N * sizeof(double));
+        m->wG.ensure((size_t)K * N * sizeof(double));
+        m->wLG.ensure((size_t)K * N * sizeof(double));
+        const bool plainSave = m->plain;
+        m->plain = true;
+        mpart_view T{m->wT.as<double>(), Mo, N, N, 1};
+        mpart_cview Tc{m->wT.as<double>(), Mo, N, N, 1};
+        mpart_view G{m->wG.as<double>(), K, N, N, 1};
+        mpart_view LG{m->wLG.as<double>(), K, N, N, 1};
+        try {
+            do_evaluate(*m, pts, T, nullptr, st);
+            do_logdet(*m, pts, m->wLd.as<double>(), 1, st);
+            do_coeff_grad(*m, pts, Tc, G, st);
+            do_logdet_coeff_grad(*m, pts, LG, st);
+        } catch (...) {
+            m->plain = plainSave;
+            throw;
+        }
+        m->plain = plainSave;
+        hip_check(mpd::launch_kl_rows(m->wG.as<double>(), m->wLG.as<double>(), N, K, grad, st), "launch kl_rows");
+        const long nb = (N + 255) / 256;
+        m->partialLoss.ensure(std::max<size_t>(8, (size_t)nb * sizeof(double)));
+        hip_check(mpd::launch_kl_loss(m->wT.as<double>(), Mo, m->wLd.as<double>(), N, m->partialLoss.as<double>(), st),
+                  "launch kl_loss");
+        hip_check(mpd::launch_reduce_rows(m->partialLoss.as<double>(), nb, 1, 1.0 / (double)N, loss, st),
+                  "launch reduce");
+    });
+}
+
+int mpart_set_plain_launches(mpart_map m, int plain) {
+    return guarded([&] {
+        check_map(m);
+        m->plain = plain != 0;
+    });
+}
+
+}  // extern "C"

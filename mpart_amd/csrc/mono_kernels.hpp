@@ -1,0 +1,594 @@
+// mono_kernels.hpp -- HIP kernels of the monotone-component path (lane = point).
+//
+// Template parameters: BK basis (ProbabilistHermite / Hermite functions), PK
+// positive bijector (Exp / SoftPlus), QK quadrature (adaptive Simpson, adaptive
+// Clenshaw-Curtis with a 5-point fine rule, or with a runtime fine rule <= 17).
+//
+// The per-lane off-diagonal basis table lives in LDS (tab[slot * blockDim +
+// tid]) unless the launcher passes a global table (large maps), in which case
+// it is gtab[slot * stride + global thread index].
+#pragma once
+#include "mono_device.hpp"
+
+namespace mpd {
+
+struct PtsArg {
+    const double* p;
+    long rs, cs, N;
+};
+struct OutArg {
+    double* p;
+    long rs, cs;
+};
+
+__device__ __forceinline__ double* lane_table(double* smem_tab, double* gtab, long& stride) {
+    if (gtab) {
+        stride = (long)gridDim.x * blockDim.x;
+        return gtab + (long)blockIdx.x * blockDim.x + threadIdx.x;
+    }
+    stride = blockDim.x;
+    return smem_tab + threadIdx.x;
+}
+
+// Largest coordinate whose table entries the components [cb, ce) read.
+__device__ __forceinline__ int table_dims(const MapDev& M, int cb, int ce) {
+    int nd = 0;
+    for (int c = cb; c < ce; ++c) nd = max(nd, M.comps[c].dim - 1);
+    return nd;
+}
+
+// ---------------------------------------------------------------------------
+// Evaluate (MonotoneComponent::EvaluateImpl, MonotoneComponent.h:255-299, with
+// the nugget dropped as at line 287) and/or LogDeterminant
+// (ContinuousDerivative + log, 105-125, 488-531) for components [cb, ce).
+// logdet is summed over the components (TriangularMap.cpp:100-125); with
+// ldAccumulate the launch adds to the existing values.
+template <int BK, int PK, int QK>
+__global__ void __launch_bounds__(256) k_eval(MapDev M, const double* w, long ws, PtsArg P, OutArg out, double* ld,
+                                              long lds, int ldAccumulate, int* nevals, int cb, int ce, double* gtab) {
+    extern __shared__ double smem[];
+    const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= P.N) return;
+    long stride;
+    double* tab = lane_table(smem, gtab, stride);
+    const double* pt = P.p + n * P.cs;
+    fill_values<BK>(M, table_dims(M, cb, ce), pt, P.rs, tab, (int)stride);
+    double ldsum = 0.0;
+    for (int c = cb; c < ce; ++c) {
+        const CompDev cd = M.comps[c];
+        double C[MAXP + 1];
+        coeff_groups(M, cd, w + (long)cd.coeffOff * ws, ws, tab, (int)stride, C);
+        const double xk = pt[(long)(cd.dim - 1) * P.rs];
+        DiagCoef dc;
+        if (out.p) {
+            make_diag(M, cd, C, xk, 0.0, dc);
+            AdjAcc A;
+            int ne = 0;
+            double I = integrate<BK, PK, QK, FL_VAL>(M.quad, dc, A, nevals ? &ne : nullptr);
+            out.p[(long)cd.outRow * out.rs + n * out.cs] = I + f_at_zero(M, cd, C);
+            if (nevals) nevals[(long)c * P.N + n] = ne;
+        }
+        if (ld) {
+            make_diag(M, cd, C, xk, 0.0, dc);
+            double z = diag_eval<BK, false>(dc, xk, nullptr);
+            double g = posfunc<PK>(z).g;
+            ldsum += (g <= 0) ? -INFINITY : log(g);
+        }
+    }
+    if (ld) {
+        if (ldAccumulate) ld[n * lds] += ldsum;
+        else ld[n * lds] = ldsum;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// CoeffGrad of one component (CoeffJacobian + sensitivity scaling,
+// MonotoneComponent.h:166-182, 685-741): rows coeffOff.. of out receive
+//   sens_n * P_j (phi~_q(0) + rho_q S'_q).
+template <int BK, int PK, int QK>
+__global__ void __launch_bounds__(256) k_coeff_grad(MapDev M, const double* w, long ws, PtsArg P, const double* sens,
+                                                    long srs, long scs, OutArg out, int c, double* gtab) {
+    extern __shared__ double smem[];
+    const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= P.N) return;
+    long stride;
+    double* tab = lane_table(smem, gtab, stride);
+    const double* pt = P.p + n * P.cs;
+    const CompDev cd = M.comps[c];
+    fill_values<BK>(M, cd.dim - 1, pt, P.rs, tab, (int)stride);
+    double C[MAXP + 1];
+    coeff_groups(M, cd, w + (long)cd.coeffOff * ws, ws, tab, (int)stride, C);
+    DiagCoef dc;
+    make_diag(M, cd, C, pt[(long)(cd.dim - 1) * P.rs], M.nugget, dc);
+    AdjAcc A;
+    zero_adj(A);
+    integrate<BK, PK, QK, FL_COEF>(M.quad, dc, A, nullptr);
+    const double s = sens[(long)cd.outRow * srs + n * scs];
+    double G[MAXP + 1];
+#pragma unroll
+    for (int q = 0; q <= MAXP; ++q) G[q] = (q <= cd.p) ? (M.phi0[q] + M.rho[q] * A.S[q]) : 0.0;
+    double* o = out.p + (long)cd.coeffOff * out.rs + n * out.cs;
+    struct Op {
+        double* o;
+        long rs;
+        const double* G;
+        double s;
+        __device__ void operator()(uint32_t idx, double Pj, int q) { o[(long)idx * rs] = (Pj * G[q]) * s; }
+    } op{o, out.rs, G, s};
+    visit_terms(M, cd, tab, (int)stride, op);
+}
+
+// LogDeterminantCoeffGrad of one component with contDeriv (ContinuousMixedJacobian
+// + ContinuousDerivative, MonotoneComponent.h:185-206, 825-874):
+//   P_j phi~'_q(x_k) g'(d_k f) * (1 / g(d_k f)).
+template <int BK, int PK, int QK>
+__global__ void __launch_bounds__(256) k_logdet_coeff_grad(MapDev M, const double* w, long ws, PtsArg P, OutArg out,
+                                                           int c, double* gtab) {
+    extern __shared__ double smem[];
+    const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= P.N) return;
+    long stride;
+    double* tab = lane_table(smem, gtab, stride);
+    const double* pt = P.p + n * P.cs;
+    const CompDev cd = M.comps[c];
+    fill_values<BK>(M, cd.dim - 1, pt, P.rs, tab, (int)stride);
+    double C[MAXP + 1];
+    coeff_groups(M, cd, w + (long)cd.coeffOff * ws, ws, tab, (int)stride, C);
+    DiagCoef dc;
+    const double xk = pt[(long)(cd.dim - 1) * P.rs];
+    make_diag(M, cd, C, xk, 0.0, dc);
+    const double z = diag_eval<BK, false>(dc, xk, nullptr);
+    const PosF<PK> gv = posfunc<PK>(z);
+    const double inv = 1.0 / gv.g;
+    double bq[MAXP + 1];
+    diag_basis<BK>(cd.p, xk, bq);
+    double G[MAXP + 1];
+#pragma unroll
+    for (int q = 0; q <= MAXP; ++q) G[q] = (q <= cd.p) ? M.rho[q] * bq[q] : 0.0;
+    double* o = out.p + (long)cd.coeffOff * out.rs + n * out.cs;
+    struct Op {
+        double* o;
+        long rs;
+        const double* G;
+        double gp, inv;
+        __device__ void operator()(uint32_t idx, double Pj, int q) { o[(long)idx * rs] = ((Pj * G[q]) * gp) * inv; }
+    } op{o, out.rs, G, gv.gp, inv};
+    visit_terms(M, cd, tab, (int)stride, op);
+}
+
+// ---------------------------------------------------------------------------
+// Input gradients of one component, accumulated into out (inDim x N):
+//  mode 0: Gradient      sens_n * dT/dx  (InputJacobian, MonotoneComponent.h:148-164, 758-818)
+//  mode 1: LogDeterminantInputGrad       (ContinuousMixedInputJacobian, 208-232, 877-926)
+// dT/dx_i = sum_q D^(i)_q (phi~_q(0) + rho_q S'_q) for i < k, dT/dx_k = U;
+// with D^(i)_q = sum_{j in group q} w_j dP_j/dx_i.
+template <int ND>
+__device__ __forceinline__ void input_groups_nd(const MapDev& M, const CompDev& c, const double* w, long ws,
+                                                const double* tab, const double* dtab, int stride, int wrt,
+                                                double* Dq) {
+#pragma unroll
+    for (int q = 0; q <= MAXP; ++q) {
+        if (q > c.p) break;
+        const GroupDev g = M.groups[c.groupOff + q];
+        double acc = 0.0;
+        for (int j = g.start; j < g.start + g.count; ++j) {
+            const uint32_t* rec = M.terms + (long)j * (ND + 1);
+            bool has = false;
+            double Pj = 1.0;
+#pragma unroll
+            for (int e = 1; e <= ND; ++e) {
+                const uint32_t s = rec[e];
+                if (s != 0 && M.slotDim[s] == wrt) {
+                    has = true;
+                    Pj *= dtab[(long)s * stride];
+                } else {
+                    Pj *= tab[(long)s * stride];
+                }
+            }
+            if (has) acc += w[(long)rec[0] * ws] * Pj;
+        }
+        Dq[q] = acc;
+    }
+}
+
+__device__ __forceinline__ void input_groups(const MapDev& M, const CompDev& c, const double* w, long ws,
+                                             const double* tab, const double* dtab, int stride, int wrt, double* Dq) {
+    switch (M.nd) {
+        case 1: input_groups_nd<1>(M, c, w, ws, tab, dtab, stride, wrt, Dq); break;
+        case 2: input_groups_nd<2>(M, c, w, ws, tab, dtab, stride, wrt, Dq); break;
+        case 3: input_groups_nd<3>(M, c, w, ws, tab, dtab, stride, wrt, Dq); break;
+        case 4: input_groups_nd<4>(M, c, w, ws, tab, dtab, stride, wrt, Dq); break;
+        case 5: input_groups_nd<5>(M, c, w, ws, tab, dtab, stride, wrt, Dq); break;
+        case 6: input_groups_nd<6>(M, c, w, ws, tab, dtab, stride, wrt, Dq); break;
+        case 7: input_groups_nd<7>(M, c, w, ws, tab, dtab, stride, wrt, Dq); break;
+        default: input_groups_nd<8>(M, c, w, ws, tab, dtab, stride, wrt, Dq); break;
+    }
+}
+
+template <int BK, int PK, int QK>
+__global__ void __launch_bounds__(256) k_input_grad(MapDev M, const double* w, long ws, PtsArg P, const double* sens,
+                                                    long srs, long scs, OutArg out, int c, int mode, double* gtab) {
+    extern __shared__ double smem[];
+    const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= P.N) return;
+    long stride;
+    double* tab;
+    if (gtab) {
+        stride = (long)gridDim.x * blockDim.x;
+        tab = gtab + n;
+    } else {
+        stride = blockDim.x;
+        tab = smem + threadIdx.x;
+    }
+    double* dtab = tab + (long)M.nslots * stride;
+    const double* pt = P.p + n * P.cs;
+    const CompDev cd = M.comps[c];
+    const int k = cd.dim - 1;
+    fill_values_derivs<BK>(M, k, pt, P.rs, tab, dtab, (int)stride);
+    const double* wc = w + (long)cd.coeffOff * ws;
+    double C[MAXP + 1];
+    coeff_groups(M, cd, wc, ws, tab, (int)stride, C);
+    const double xk = pt[(long)k * P.rs];
+    DiagCoef dc;
+    double* o = out.p + n * out.cs;
+    if (mode == 0) {
+        make_diag(M, cd, C, xk, M.nugget, dc);
+        AdjAcc A;
+        zero_adj(A);
+        integrate<BK, PK, QK, FL_INPUT>(M.quad, dc, A, nullptr);
+        const double s = sens[(long)cd.outRow * srs + n * scs];
+        double G[MAXP + 1];
+#pragma unroll
+        for (int q = 0; q <= MAXP; ++q) G[q] = (q <= cd.p) ? (M.phi0[q] + M.rho[q] * A.S[q]) : 0.0;
+        for (int i = 0; i < k; ++i) {
+            double Dq[MAXP + 1];
+            input_groups(M, cd, wc, ws, tab, dtab, (int)stride, i, Dq);
+            double v = 0.0;
+#pragma unroll
+            for (int q = 0; q <= MAXP; ++q) {
+                if (q > cd.p) break;
+                v += Dq[q] * G[q];
+            }
+            o[(long)i * out.rs] += v * s;
+        }
+        o[(long)k * out.rs] += A.U * s;
+    } else {
+        make_diag(M, cd, C, xk, 0.0, dc);
+        double z2;
+        const double z = diag_eval<BK, true>(dc, xk, &z2);
+        const PosF<PK> gv = posfunc<PK>(z);
+        const double inv = 1.0 / gv.g;
+        double bq[MAXP + 1];
+        diag_basis<BK>(cd.p, xk, bq);
+        for (int i = 0; i < k; ++i) {
+            double Dq[MAXP + 1];
+            input_groups(M, cd, wc, ws, tab, dtab, (int)stride, i, Dq);
+            double v = 0.0;
+#pragma unroll
+            for (int q = 0; q <= MAXP; ++q) {
+                if (q > cd.p) break;
+                v += Dq[q] * (M.rho[q] * bq[q]);
+            }
+            o[(long)i * out.rs] += (v * gv.gp) * inv;
+        }
+        o[(long)k * out.rs] += (z2 * gv.gp) * inv;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Inverse of one component (MonotoneComponent::InverseImpl, MonotoneComponent.h:331-443;
+// RootFinding.h:20-178).  xs has xrows rows and 1 or N columns; its last row
+// holds the initial guess.  out may alias that row.
+template <int BK, int PK, int QK>
+__device__ double eval_T(const MapDev& M, const DiagCoef& dc0, double f0, double x) {
+    DiagCoef dc = dc0;
+    dc.x = x;
+    AdjAcc A;
+    return integrate<BK, PK, QK, FL_VAL>(M.quad, dc, A, nullptr) + f0;
+}
+
+template <int BK, int PK, int QK>
+__global__ void __launch_bounds__(256) k_inverse(MapDev M, const double* w, long ws, const double* xs, long xrows,
+                                                 long xcols, long xrs, long xcs, const double* ys, long yss, long N,
+                                                 double* out, long os, int c, double xtol, double ytol, double* gtab) {
+    extern __shared__ double smem[];
+    const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    const long xi = (xcols == 1) ? 0 : n;
+    const double* pt = xs + xi * xcs;
+    for (long i = 0; i < xrows; ++i) {
+        if (isnan(pt[i * xrs])) {
+            out[n * os] = NAN;
+            return;
+        }
+    }
+    const double x0 = pt[(xrows - 1) * xrs];
+    const double yd = ys[n * yss];
+    long stride;
+    double* tab = lane_table(smem, gtab, stride);
+    const CompDev cd = M.comps[c];
+    fill_values<BK>(M, cd.dim - 1, pt, xrs, tab, (int)stride);
+    double C[MAXP + 1];
+    coeff_groups(M, cd, w + (long)cd.coeffOff * ws, ws, tab, (int)stride, C);
+    const double f0 = f_at_zero(M, cd, C);
+    DiagCoef dc;
+    make_diag(M, cd, C, x0, M.nugget, dc);
+
+    // FindBracket
+    double xlb = x0, xub = x0;
+    double ylb = eval_T<BK, PK, QK>(M, dc, f0, xlb);
+    double yub = ylb;
+    int info = 0;
+    {
+        const unsigned maxIts = 128;
+        double stepSize = 1.0;
+        ylb = eval_T<BK, PK, QK>(M, dc, f0, xlb);
+        yub = eval_T<BK, PK, QK>(M, dc, f0, xub);
+        if (ylb > yd) {
+            double t = ylb; ylb = yub; yub = t;
+            t = xlb; xlb = xub; xub = t;
+            unsigned i;
+            for (i = 0; i < maxIts; ++i) {
+                xlb = xub - stepSize;
+                ylb = eval_T<BK, PK, QK>(M, dc, f0, xlb);
+                if ((fabs((yub - ylb) / (xub - xlb)) < 1e-12) && ((xub - xlb) > 10)) { info = -1; break; }
+                if (ylb > yd) { yub = ylb; xub = xlb; stepSize *= 2.0; } else break;
+            }
+            if (info == 0 && i >= maxIts) info = -2;
+        } else {
+            unsigned i;
+            for (i = 0; i < maxIts; ++i) {
+                xub = xlb + stepSize;
+                yub = eval_T<BK, PK, QK>(M, dc, f0, xub);
+                if ((fabs((yub - ylb) / (xub - xlb)) < 1e-12) && ((xub - xlb) > 10)) { info = -1; break; }
+                if (yub < yd) { ylb = yub; xlb = xub; stepSize *= 2.0; } else break;
+            }
+            if (info == 0 && i >= maxIts) info = -2;
+        }
+    }
+    if ((info < 0) || ((ylb > yd) || (yub < yd))) {
+        out[n * os] = NAN;
+        return;
+    }
+    // ITP (k1 = 0.1, k2 = 2, n0 = 1)
+    const double k1 = 0.1, k2 = 2.0, n0 = 1.0;
+    const double nhalf = ceil(log2(0.5 * (xub - xlb) / xtol));
+    const unsigned maxIts = 10000;
+    for (unsigned it = 0; it < maxIts; ++it) {
+        const double xb = 0.5 * (xub + xlb);
+        double xf = (xub * ylb - xlb * yub) / (ylb - yub);
+        const double sigma = ((xb - xf) > 0) ? 1.0 : -1.0;
+        const double delta = fmin(k1 * pow((xub - xlb), k2), fabs(xb - xf));
+        xf += delta * sigma;
+        const double rho = fmin(xtol * pow(2.0, nhalf + n0 - (double)it) - 0.5 * (xub - xlb), fabs(xf - xb));
+        const double xc = xb - sigma * rho;
+        const double yc = eval_T<BK, PK, QK>(M, dc, f0, xc);
+        if (fabs(yc - yd) < ytol) {
+            out[n * os] = xc;
+            return;
+        } else if (yc > yd) {
+            xub = xc;
+            yub = yc;
+        } else {
+            xlb = xc;
+            ylb = yc;
+        }
+        if (((xub - xlb) < xtol) || ((yub - ylb) < ytol)) break;
+    }
+    out[n * os] = 0.5 * (xub + xlb);
+}
+
+// ---------------------------------------------------------------------------
+// Fused forward + adjoint over every component of a map (one launch).
+//
+// For each point n and component c it computes T_c, log g(d_c f), the
+// adjoint integrals S'_q on the forward tree, and
+//   G_q = s_c (phi~_q(0) + rho_q S'_q) + l_n rho_q B'_q(x_c) g'/g,
+// then reduces sum_n P_j(x_n) G_q(x_n) over the wavefront (row butterfly +
+// two cross-row shuffles) into per-wave LDS accumulators.  klMode uses
+// s_c = T_c and l_n = -1 (gradient of mean(0.5 |T|^2 - logdet) up to 1/N) and
+// also sums 0.5 |T|^2 - logdet.  Blocks loop over point tiles (persistent
+// grid) and write one partial gradient row per block.
+//
+// LDS layout: [acc: (blockDim/64) * numCoeffs] [table: nslots * blockDim].
+template <int BK, int PK, int QK>
+__global__ void __launch_bounds__(256) k_fused(MapDev M, const double* w, long ws, PtsArg P, const double* sens,
+                                               long srs, long scs, const double* lsens, long lss, int klMode,
+                                               double* partialGrad, double* partialLoss, double* gtab,
+                                               double* gradGlobal) {
+    extern __shared__ double smem[];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int nwaves = blockDim.x >> 6;
+    const int K = M.numCoeffs;
+    const bool ldsAcc = (gradGlobal == nullptr);
+    double* acc = smem;
+    double* tabBase = smem + (ldsAcc ? (long)nwaves * K : 0);
+    if (ldsAcc) {
+        for (int i = tid; i < nwaves * K; i += blockDim.x) acc[i] = 0.0;
+    }
+    __syncthreads();
+    double* myAcc = acc + (long)wave * K;
+    long stride;
+    double* tab;
+    if (gtab) {
+        stride = (long)gridDim.x * blockDim.x;
+        tab = gtab + (long)blockIdx.x * blockDim.x + tid;
+    } else {
+        stride = blockDim.x;
+        tab = tabBase + tid;
+    }
+    const int ndims = M.D - 1;
+    double lossLane = 0.0;
+    const long ntiles = (P.N + blockDim.x - 1) / blockDim.x;
+    for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const long n0 = tile * blockDim.x + tid;
+        const bool active = n0 < P.N;
+        const long n = active ? n0 : P.N - 1;
+        const double* pt = P.p + n * P.cs;
+        fill_values<BK>(M, ndims, pt, P.rs, tab, (int)stride);
+        for (int c = 0; c < M.ncomp; ++c) {
+            const CompDev cd = M.comps[c];
+            const double* wc = w + (long)cd.coeffOff * ws;
+            double C[MAXP + 1];
+            coeff_groups(M, cd, wc, ws, tab, (int)stride, C);
+            const double xk = pt[(long)(cd.dim - 1) * P.rs];
+            const double f0 = f_at_zero(M, cd, C);
+            DiagCoef dc;
+            make_diag(M, cd, C, xk, M.nugget, dc);
+            AdjAcc A;
+            zero_adj(A);
+            double G[MAXP + 1];
+#pragma unroll
+            for (int q = 0; q <= MAXP; ++q) G[q] = 0.0;
+            if (active) {
+                double T = integrate<BK, PK, QK, FL_COEF>(M.quad, dc, A, nullptr) + f0;
+                if (M.nugget != 0.0) {
+                    // Evaluate drops the nugget (MonotoneComponent.h:287): its tree may differ.
+                    DiagCoef dv = dc;
+                    dv.nug = 0.0;
+                    AdjAcc Av;
+                    T = integrate<BK, PK, QK, FL_VAL>(M.quad, dv, Av, nullptr) + f0;
+                }
+                const double z = diag_eval<BK, false>(dc, xk, nullptr);
+                const PosF<PK> gv = posfunc<PK>(z);
+                const double ldv = (gv.g <= 0) ? -INFINITY : log(gv.g);
+                const double r = gv.gp * (1.0 / gv.g);
+                double s, l;
+                if (klMode) {
+                    s = T;
+                    l = -1.0;
+                    lossLane += 0.5 * T * T - ldv;
+                } else {
+                    s = sens ? sens[(long)cd.outRow * srs + n * scs] : 0.0;
+                    l = lsens ? lsens[n * lss] : 0.0;
+                }
+                double bq[MAXP + 1];
+                diag_basis<BK>(cd.p, xk, bq);
+                G[0] = s;
+#pragma unroll
+                for (int q = 1; q <= MAXP; ++q) {
+                    if (q > cd.p) break;
+                    G[q] = s * (M.phi0[q] + M.rho[q] * A.S[q]) + (l * r) * (M.rho[q] * bq[q]);
+                }
+            }
+            // Gradient pass: chunks of 16 term records per group.
+#pragma unroll
+            for (int q = 0; q <= MAXP; ++q) {
+                if (q > cd.p) break;
+                const GroupDev g = M.groups[cd.groupOff + q];
+                const double Gq = G[q];
+                for (int j0 = g.start; j0 < g.start + g.countPad; j0 += 16) {
+                    double v[16];
+#pragma unroll
+                    for (int u = 0; u < 16; ++u) {
+                        const uint32_t* rec = M.terms + (long)(j0 + u) * M.trec;
+                        double Pj = tab[(long)rec[1] * stride];
+                        for (int e = 2; e <= M.nd; ++e) Pj *= tab[(long)rec[e] * stride];
+                        v[u] = active ? Pj * Gq : 0.0;
+                    }
+                    double rsum = row_reduce16(v, lane);
+                    rsum += __shfl_xor(rsum, 16, 64);
+                    rsum += __shfl_xor(rsum, 32, 64);
+                    if (lane < 16) {
+                        const uint32_t idx = M.terms[(long)(j0 + lane) * M.trec];
+                        if (idx != DUMMY) {
+                            if (ldsAcc) myAcc[cd.coeffOff + idx] += rsum;
+                            else atomicAdd(gradGlobal + cd.coeffOff + idx, rsum);
+                        }
+                    }
+                }
+            }
+        }
+    }
+    // Loss: wave sum, then block sum.
+    double lw = wave_sum(lossLane);
+    __shared__ double lossWave[4];
+    if (lane == 0) lossWave[wave] = lw;
+    __syncthreads();
+    if (tid == 0) {
+        double l = 0.0;
+        for (int i = 0; i < nwaves; ++i) l += lossWave[i];
+        partialLoss[blockIdx.x] = l;
+    }
+    if (ldsAcc) {
+        for (int k = tid; k < K; k += blockDim.x) {
+            double s = 0.0;
+            for (int i = 0; i < nwaves; ++i) s += acc[(long)i * K + k];
+            partialGrad[(long)blockIdx.x * K + k] = s;
+        }
+    }
+}
+
+#ifdef MPD_UTILITY_KERNELS
+// Column sums of a (nrows x K) row-major block of partials: out[k] = scale * sum_r in[r*K + k].
+__global__ void __launch_bounds__(256) k_reduce_rows(const double* in, long nrows, long K, double scale, double* out) {
+    __shared__ double red[8][33];
+    const int kk = threadIdx.x & 31, rr = threadIdx.x >> 5;
+    const long k = (long)blockIdx.x * 32 + kk;
+    double s = 0.0;
+    if (k < K)
+        for (long r = rr; r < nrows; r += 8) s += in[r * K + k];
+    red[rr][kk] = s;
+    __syncthreads();
+    if (rr == 0 && k < K) {
+        double t = 0.0;
+        for (int i = 0; i < 8; ++i) t += red[i][kk];
+        out[k] = scale * t;
+    }
+}
+
+// Plain KL reduction (FastGaussianReverseKLObjective, MapObjective.cpp:146-215):
+// grad[k] = (1/N) sum_n (G[k][n] - LG[k][n]), one block per coefficient row.
+__global__ void __launch_bounds__(256) k_kl_rows(const double* G, const double* LG, long N, long K, double* grad) {
+    __shared__ double red[256];
+    const long k = blockIdx.x;
+    double s = 0.0;
+    for (long n = threadIdx.x; n < N; n += blockDim.x) s += G[k * N + n] - LG[k * N + n];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) grad[k] = red[0] / (double)N;
+}
+
+// Per-block partial sums of sum_i 0.5 T_i^2 - ld (T is M x N row-major).
+__global__ void __launch_bounds__(256) k_kl_loss(const double* T, long M, const double* ld, long N, double* partial) {
+    __shared__ double red[256];
+    const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double v = 0.0;
+    if (n < N) {
+        for (long i = 0; i < M; ++i) v += 0.5 * T[i * N + n] * T[i * N + n];
+        v -= ld[n];
+    }
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// Fill a strided matrix with a value (used instead of memset so that captured
+// graphs hold no memset nodes).
+__global__ void k_fill(double* p, long rows, long cols, long rs, long cs, double v) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < rows * cols) p[(i / cols) * rs + (i % cols) * cs] = v;
+}
+
+// Strided copy (SetCoeffs deep copy, Inverse buffers).
+__global__ void k_copy(const double* src, long rows, long cols, long srs, long scs, double* dst, long drs, long dcs) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < rows * cols) {
+        const long r = i / cols, c = i % cols;
+        dst[r * drs + c * dcs] = src[r * srs + c * scs];
+    }
+}
+
+#endif  // MPD_UTILITY_KERNELS
+
+}  // namespace mpd
