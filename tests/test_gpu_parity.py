@@ -240,16 +240,28 @@ def test_c1_full_size():
     close(g.LogDeterminant(x), r.LogDeterminant(x), what="C1 LogDeterminant")
 
 
-def test_c3_sample_kl():
-    # Config 3 map on a sample of its inputs (banana data), random coefficients.
-    _print_case("c3_sample")
-    x = banana_points(3000)
+@pytest.mark.parametrize("variant", ["banana_zero_coeffs", "normal_data"])
+def test_c3_sample_kl(variant):
+    # Config 3 map on a sample: the literal banana inputs at the factory's zero
+    # coefficients, and N(0, I_8) inputs with 0.1 N(0,1) coefficients (the
+    # banana recursion explodes, so nonzero coefficients drive SoftPlus to
+    # underflow and the log-determinant to -inf, as in the reference).
+    _print_case("c3_sample", variant=variant)
+    if variant == "banana_zero_coeffs":
+        x = banana_points(3000)
+        c = np.zeros(494)
+    else:
+        s = Stream(3)
+        x = s.points(8, 3000)
+        c = 0.1 * s.normal(494)
     g = mt.CreateTriangular(8, 8, 3, mt.MapOptions())
     r = orc.CreateTriangular(8, 8, 3, orc.options())
-    c = 0.01 * Stream(1).normal(g.numCoeffs)
     g.SetCoeffs(c)
     r.SetCoeffs(c)
     lr, gr = r.KL(x)
     lf, gf = g.KLValueAndGrad(x)
+    lp, gp = g.KLValueAndGrad(x, plain=True)
     close(lf, lr, what="C3 loss")
     close(gf, gr, what="C3 grad")
+    close(lp, lr, what="C3 loss plain")
+    close(gp, gr, what="C3 grad plain")
