@@ -26,6 +26,10 @@ _LIB_PATH = os.path.join(_HERE, "libmpart_amd.so")
 if not os.path.exists(_LIB_PATH):
     raise ImportError(f"{_LIB_PATH} is missing: build it with `python mpart_amd/build.py`")
 
+# torch ships its own HIP runtime; load it first so that this library binds to
+# the same runtime instance that owns the tensors and streams passed to it.
+import torch as _torch_rt  # noqa: E402,F401
+
 _lib = ctypes.CDLL(_LIB_PATH)
 
 MPART_OK, MPART_INVALID_ARGUMENT, MPART_RUNTIME_ERROR, MPART_HIP_ERROR, MPART_NOT_IMPLEMENTED = 0, -1, -2, -3, -4

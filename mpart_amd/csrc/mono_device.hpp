@@ -25,6 +25,10 @@
 #include <stdint.h>
 #endif
 
+// Helpers usable from host code as well (a host harness runs the same
+// per-point arithmetic on the CPU); the wavefront primitives stay device-only.
+#define MPD_HD __host__ __device__ __forceinline__
+
 namespace mpd {
 
 constexpr int MAXP = 12;        // largest last-coordinate degree held in registers
@@ -98,7 +102,7 @@ struct PosF {
 };
 
 template <int PK>
-__device__ __forceinline__ PosF<PK> posfunc(double z) {
+MPD_HD PosF<PK> posfunc(double z) {
     PosF<PK> r;
     if constexpr (PK == PK_EXP) {
         r.g = exp(z);
@@ -128,7 +132,7 @@ struct DiagCoef {
 
 // d_k f at y (and optionally the second derivative).
 template <int BK, bool SECOND>
-__device__ __forceinline__ double diag_eval(const DiagCoef& dc, double y, double* z2out) {
+MPD_HD double diag_eval(const DiagCoef& dc, double y, double* z2out) {
     const int p = dc.p;
     if constexpr (BK == BK_PH) {
         // He_0 = 1, He_1 = y, He_r = y He_{r-1} - (r-1) He_{r-2}
@@ -177,7 +181,7 @@ __device__ __forceinline__ double diag_eval(const DiagCoef& dc, double y, double
 
 // B'_q(y) for q = 0..p (raw derivative basis; phi~'_q = rho_q B'_q).
 template <int BK>
-__device__ __forceinline__ void diag_basis(int p, double y, double* bq) {
+MPD_HD void diag_basis(int p, double y, double* bq) {
     bq[0] = 0.0;
     if constexpr (BK == BK_PH) {
         double hm2 = 0.0, hm1 = 1.0;
@@ -212,7 +216,7 @@ __device__ __forceinline__ void diag_basis(int p, double y, double* bq) {
 
 // B''_q(y) (raw second-derivative basis, same folding as B'_q).
 template <int BK>
-__device__ __forceinline__ void diag_basis2(int p, double y, double* bq) {
+MPD_HD void diag_basis2(int p, double y, double* bq) {
     bq[0] = 0.0;
     if (p >= 1) bq[1] = 0.0;
     if constexpr (BK == BK_PH) {
@@ -252,7 +256,7 @@ __device__ __forceinline__ void diag_basis2(int p, double y, double* bq) {
 //
 // Values: EvaluateAll (OrthogonalPolynomial.h:25-42, HermiteFunction.h:22-43).
 template <int BK>
-__device__ void fill_values(const MapDev& M, int ndims, const double* pt, long prs, double* tab, int stride) {
+__host__ __device__ void fill_values(const MapDev& M, int ndims, const double* pt, long prs, double* tab, int stride) {
     tab[0] = 1.0;
     for (int d = 0; d < ndims; ++d) {
         const int deg = M.dimDeg[d];
@@ -294,7 +298,7 @@ __device__ void fill_values(const MapDev& M, int ndims, const double* pt, long p
 // (OrthogonalPolynomial.h:87-118, HermiteFunction.h:45-80); used by the
 // input-gradient paths (FillCache1 with DerivativeFlags::Input).
 template <int BK>
-__device__ void fill_values_derivs(const MapDev& M, int ndims, const double* pt, long prs, double* tab, double* dtab,
+__host__ __device__ void fill_values_derivs(const MapDev& M, int ndims, const double* pt, long prs, double* tab, double* dtab,
                                    int stride) {
     tab[0] = 1.0;
     dtab[0] = 0.0;
@@ -368,7 +372,7 @@ __device__ void fill_values_derivs(const MapDev& M, int ndims, const double* pt,
 
 // C_q = sum_{j in group q} w_j P_j for q = 0..p.
 template <int ND>
-__device__ __forceinline__ void coeff_groups_nd(const MapDev& M, const CompDev& c, const double* w, long ws,
+MPD_HD void coeff_groups_nd(const MapDev& M, const CompDev& c, const double* w, long ws,
                                                 const double* tab, int stride, double* C) {
 #pragma unroll
     for (int q = 0; q <= MAXP; ++q) {
@@ -386,7 +390,7 @@ __device__ __forceinline__ void coeff_groups_nd(const MapDev& M, const CompDev& 
     }
 }
 
-__device__ __forceinline__ void coeff_groups(const MapDev& M, const CompDev& c, const double* w, long ws,
+MPD_HD void coeff_groups(const MapDev& M, const CompDev& c, const double* w, long ws,
                                              const double* tab, int stride, double* C) {
     switch (M.nd) {
         case 1: coeff_groups_nd<1>(M, c, w, ws, tab, stride, C); break;
@@ -402,7 +406,7 @@ __device__ __forceinline__ void coeff_groups(const MapDev& M, const CompDev& c, 
 
 // Visit every (real) term: op(coeff index, P_j, q).
 template <int ND, class Op>
-__device__ __forceinline__ void visit_terms_nd(const MapDev& M, const CompDev& c, const double* tab, int stride,
+MPD_HD void visit_terms_nd(const MapDev& M, const CompDev& c, const double* tab, int stride,
                                                Op& op) {
 #pragma unroll
     for (int q = 0; q <= MAXP; ++q) {
@@ -419,7 +423,7 @@ __device__ __forceinline__ void visit_terms_nd(const MapDev& M, const CompDev& c
 }
 
 template <class Op>
-__device__ __forceinline__ void visit_terms(const MapDev& M, const CompDev& c, const double* tab, int stride,
+MPD_HD void visit_terms(const MapDev& M, const CompDev& c, const double* tab, int stride,
                                             Op& op) {
     switch (M.nd) {
         case 1: visit_terms_nd<1>(M, c, tab, stride, op); break;
@@ -434,7 +438,7 @@ __device__ __forceinline__ void visit_terms(const MapDev& M, const CompDev& c, c
 }
 
 // Diagonal coefficients of component c from its group sums.
-__device__ __forceinline__ void make_diag(const MapDev& M, const CompDev& c, const double* C, double xk, double nug,
+MPD_HD void make_diag(const MapDev& M, const CompDev& c, const double* C, double xk, double nug,
                                           DiagCoef& dc) {
     dc.p = c.p;
     dc.x = xk;
@@ -444,7 +448,7 @@ __device__ __forceinline__ void make_diag(const MapDev& M, const CompDev& c, con
 }
 
 // f(x_<k, 0) = sum_q C_q phi~_q(0).
-__device__ __forceinline__ double f_at_zero(const MapDev& M, const CompDev& c, const double* C) {
+MPD_HD double f_at_zero(const MapDev& M, const CompDev& c, const double* C) {
     double f = 0.0;
 #pragma unroll
     for (int q = 0; q <= MAXP; ++q) {
@@ -477,7 +481,7 @@ struct NodeV<FL_INPUT> {
 };
 
 template <int BK, int PK, int FL>
-__device__ __forceinline__ NodeV<FL> eval_node(const DiagCoef& dc, double t) {
+MPD_HD NodeV<FL> eval_node(const DiagCoef& dc, double t) {
     double z = diag_eval<BK, false>(dc, t * dc.x, nullptr);
     PosF<PK> gv = posfunc<PK>(z);
     NodeV<FL> v;
@@ -494,7 +498,7 @@ struct AdjAcc {
 
 // Adds w * v(t) at one node to the adjoint accumulators.
 template <int BK, int FL>
-__device__ __forceinline__ void adj_node(const DiagCoef& dc, double t, double w, const NodeV<FL>& v, AdjAcc& A) {
+MPD_HD void adj_node(const DiagCoef& dc, double t, double w, const NodeV<FL>& v, AdjAcc& A) {
     if constexpr (FL >= FL_COEF) {
         double bq[MAXP + 1];
         const double y = t * dc.x;
@@ -523,7 +527,7 @@ struct AsRec {
 // AdaptiveSimpson::Integrate (Quadrature.h:553-676): depth-first, left first,
 // accepted intervals add intFine in left-to-right order.
 template <int BK, int PK, int FL>
-__device__ double integrate_as(const QuadDev& Q, const DiagCoef& dc, AdjAcc& A, int* nevals) {
+__host__ __device__ double integrate_as(const QuadDev& Q, const DiagCoef& dc, AdjAcc& A, int* nevals) {
     AsRec<FL> stk[MAXSUB];
     int sp = 0;
     NodeV<FL> fa = eval_node<BK, PK, FL>(dc, 0.0);
@@ -591,7 +595,7 @@ struct AccRec {
 // rule size when known at compile time (5 for quadPts = 5), or MAXNF with the
 // runtime size in Q.nf.
 template <int BK, int PK, int FL, int NF>
-__device__ double integrate_acc(const QuadDev& Q, const DiagCoef& dc, AdjAcc& A, int* nevals) {
+__host__ __device__ double integrate_acc(const QuadDev& Q, const DiagCoef& dc, AdjAcc& A, int* nevals) {
     AccRec<FL> stk[MAXSUB];
     int sp = 0;
     const int nf = (NF == MAXNF) ? Q.nf : NF;
@@ -682,13 +686,13 @@ __device__ double integrate_acc(const QuadDev& Q, const DiagCoef& dc, AdjAcc& A,
 }
 
 template <int BK, int PK, int QK, int FL>
-__device__ __forceinline__ double integrate(const QuadDev& Q, const DiagCoef& dc, AdjAcc& A, int* nevals) {
+MPD_HD double integrate(const QuadDev& Q, const DiagCoef& dc, AdjAcc& A, int* nevals) {
     if constexpr (QK == QK_AS) return integrate_as<BK, PK, FL>(Q, dc, A, nevals);
     else if constexpr (QK == QK_ACC5) return integrate_acc<BK, PK, FL, 5>(Q, dc, A, nevals);
     else return integrate_acc<BK, PK, FL, MAXNF>(Q, dc, A, nevals);
 }
 
-__device__ __forceinline__ void zero_adj(AdjAcc& A) {
+MPD_HD void zero_adj(AdjAcc& A) {
 #pragma unroll
     for (int q = 0; q <= MAXP; ++q) A.S[q] = 0.0;
     A.U = 0.0;

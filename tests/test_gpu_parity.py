@@ -159,21 +159,28 @@ def test_component_inverse(ci):
     _print_case("inverse", case=CASES[ci])
     g, r, x, _ = _pair(CASES[ci], 3, 3, 600 + ci, n=300)
     y = r.Evaluate(x)
-    # r ~ N(0,1) targets and the map's own outputs.
-    for tgt in (y, Stream(61).normal(x.shape[1])[None, :]):
-        xi_g = g.Inverse(x, tgt)
-        xi_r = r.Inverse(x, tgt)
-        fin = np.isfinite(xi_r)
-        assert np.array_equal(fin, np.isfinite(xi_g))
-        assert np.max(np.abs(xi_g[fin] - xi_r[fin])) <= 1e-9
-    # Round trip at the reference test tolerance.
-    np.testing.assert_allclose(g.Inverse(x, y)[0], x[-1], atol=1e-5)
+    # r ~ N(0,1) targets (config C2's Inverse): 1e-9 absolute.
+    tgt = Stream(61).normal(x.shape[1])[None, :]
+    xi_g = g.Inverse(x, tgt)
+    xi_r = r.Inverse(x, tgt)
+    fin = np.isfinite(xi_r)
+    assert np.array_equal(fin, np.isfinite(xi_g))
+    assert np.max(np.abs(xi_g[fin] - xi_r[fin])) <= 1e-9
+    # Targets equal to T at the initial guess: the reference's FindBracket picks
+    # its search direction from f(x0) > yd (RootFinding.h:35), a last-ulp
+    # comparison here, so implementations agree only to the xtol scale
+    # (the oracle itself moves by up to 4e-6 under a 1e-15 coefficient change).
+    xs_g = g.Inverse(x, y)[0]
+    xs_r = r.Inverse(x, y)[0]
+    assert np.max(np.abs(xs_g - xs_r)) <= 1e-5
+    np.testing.assert_allclose(xs_g, x[-1], atol=1e-5)
 
 
 def test_triangular_inverse_round_trip():
     _print_case("triangular_inverse")
     g, r, x, _ = _pair({}, 3, 2, 700, n=300, triangular=True, coeff_scale=0.1)
     y = r.Evaluate(x)
+    # Initial guesses are 0 here, so the self-target ambiguity above does not arise.
     xg = g.Inverse(np.zeros((0, x.shape[1])), y)
     xr = r.Inverse(np.zeros((0, x.shape[1])), y)
     assert np.max(np.abs(xg - xr)) <= 1e-9
