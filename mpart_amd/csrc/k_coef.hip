@@ -8,8 +8,8 @@ hipError_t launch_coeff_grad(int var, const Launch& L, const MapDev& M, const do
                              long ocs, int c, double* gtab) {
     PtsArg P{pts, prs, pcs, N};
     OutArg O{out, ors, ocs};
-#define CALL(B, Q, R) \
-    hipLaunchKernelGGL((k_coeff_grad<B, Q, R>), L.grid, L.block, L.smem, L.stream, M, w, ws, P, sens, srs, scs, O, c, gtab)
+#define CALL(B, Q, R, PM) \
+    hipLaunchKernelGGL((k_coeff_grad<B, Q, R, PM>), L.grid, L.block, L.smem, L.stream, M, w, ws, P, sens, srs, scs, O, c, gtab)
     MPD_DISPATCH(var, CALL)
 #undef CALL
     return hipGetLastError();
@@ -20,8 +20,8 @@ hipError_t launch_logdet_coeff_grad(int var, const Launch& L, const MapDev& M, c
                                     int c, double* gtab) {
     PtsArg P{pts, prs, pcs, N};
     OutArg O{out, ors, ocs};
-#define CALL(B, Q, R) \
-    hipLaunchKernelGGL((k_logdet_coeff_grad<B, Q, R>), L.grid, L.block, L.smem, L.stream, M, w, ws, P, O, c, gtab)
+#define CALL(B, Q, R, PM) \
+    hipLaunchKernelGGL((k_logdet_coeff_grad<B, Q, R, PM>), L.grid, L.block, L.smem, L.stream, M, w, ws, P, O, c, gtab)
     MPD_DISPATCH(var, CALL)
 #undef CALL
     return hipGetLastError();

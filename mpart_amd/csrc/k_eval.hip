@@ -9,8 +9,8 @@ hipError_t launch_eval(int var, const Launch& L, const MapDev& M, const double* 
                        int cb, int ce, double* gtab) {
     PtsArg P{pts, prs, pcs, N};
     OutArg O{out, ors, ocs};
-#define CALL(B, Q, R) \
-    hipLaunchKernelGGL((k_eval<B, Q, R>), L.grid, L.block, L.smem, L.stream, M, w, ws, P, O, ld, lds, ldAcc, nevals, cb, ce, gtab)
+#define CALL(B, Q, R, PM) \
+    hipLaunchKernelGGL((k_eval<B, Q, R, PM>), L.grid, L.block, L.smem, L.stream, M, w, ws, P, O, ld, lds, ldAcc, nevals, cb, ce, gtab)
     MPD_DISPATCH(var, CALL)
 #undef CALL
     return hipGetLastError();

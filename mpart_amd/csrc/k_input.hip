@@ -8,8 +8,8 @@ hipError_t launch_input_grad(int var, const Launch& L, const MapDev& M, const do
                              long ocs, int c, int mode, double* gtab) {
     PtsArg P{pts, prs, pcs, N};
     OutArg O{out, ors, ocs};
-#define CALL(B, Q, R)                                                                                              \
-    hipLaunchKernelGGL((k_input_grad<B, Q, R>), L.grid, L.block, L.smem, L.stream, M, w, ws, P, sens, srs, scs, O, \
+#define CALL(B, Q, R, PM)                                                                                              \
+    hipLaunchKernelGGL((k_input_grad<B, Q, R, PM>), L.grid, L.block, L.smem, L.stream, M, w, ws, P, sens, srs, scs, O, \
                        c, mode, gtab)
     MPD_DISPATCH(var, CALL)
 #undef CALL

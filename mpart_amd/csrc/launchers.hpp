@@ -46,21 +46,26 @@ hipError_t launch_copy(const double* src, long rows, long cols, long srs, long s
 
 }  // namespace mpd
 
-// Expands CALL(BK, PK, QK) for the variant id `var`; returns
-// hipErrorInvalidValue for an unknown id.
-#define MPD_DISPATCH(var, CALL)                          \
-    switch (var) {                                       \
-        case 0: CALL(0, 0, 0); break;                    \
-        case 1: CALL(0, 0, 1); break;                    \
-        case 2: CALL(0, 0, 2); break;                    \
-        case 10: CALL(0, 1, 0); break;                   \
-        case 11: CALL(0, 1, 1); break;                   \
-        case 12: CALL(0, 1, 2); break;                   \
-        case 100: CALL(1, 0, 0); break;                  \
-        case 101: CALL(1, 0, 1); break;                  \
-        case 102: CALL(1, 0, 2); break;                  \
-        case 110: CALL(1, 1, 0); break;                  \
-        case 111: CALL(1, 1, 1); break;                  \
-        case 112: CALL(1, 1, 2); break;                  \
-        default: return hipErrorInvalidValue;            \
+// Expands CALL(BK, PK, QK, PM) for the variant id
+//   var = pmClass * 1000 + bk * 100 + pk * 10 + qk,  PM = 4 (pmClass 0) or 12 (pmClass 1);
+// returns hipErrorInvalidValue for an unknown id.
+#define MPD_CASES_PM(PMC, PMV, CALL)                   \
+    case PMC * 1000 + 0: CALL(0, 0, 0, PMV); break;     \
+    case PMC * 1000 + 1: CALL(0, 0, 1, PMV); break;     \
+    case PMC * 1000 + 2: CALL(0, 0, 2, PMV); break;     \
+    case PMC * 1000 + 10: CALL(0, 1, 0, PMV); break;    \
+    case PMC * 1000 + 11: CALL(0, 1, 1, PMV); break;    \
+    case PMC * 1000 + 12: CALL(0, 1, 2, PMV); break;    \
+    case PMC * 1000 + 100: CALL(1, 0, 0, PMV); break;   \
+    case PMC * 1000 + 101: CALL(1, 0, 1, PMV); break;   \
+    case PMC * 1000 + 102: CALL(1, 0, 2, PMV); break;   \
+    case PMC * 1000 + 110: CALL(1, 1, 0, PMV); break;   \
+    case PMC * 1000 + 111: CALL(1, 1, 1, PMV); break;   \
+    case PMC * 1000 + 112: CALL(1, 1, 2, PMV); break;
+
+#define MPD_DISPATCH(var, CALL)              \
+    switch (var) {                           \
+        MPD_CASES_PM(0, 4, CALL)             \
+        MPD_CASES_PM(1, 12, CALL)            \
+        default: return hipErrorInvalidValue; \
     }

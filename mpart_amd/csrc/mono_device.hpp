@@ -123,23 +123,24 @@ MPD_HD PosF<PK> posfunc(double z) {
 // Cd_q = C_q / n_q:   d_k f(y) = sum_{r=0}^{p-1} (r+1) Cd_{r+1} He_r(y).
 // Hermite functions (q >= 2, k = q-2): phi'_q = pi^-1/4 s_k e^{-y^2/2} (2k H_{k-1} - y H_k)
 // with physicist H; phi'_1 = 1 and phi'_0 = 0.
+template <int PM>
 struct DiagCoef {
-    double cd[MAXP + 1];  // C_q * rho_q
+    double cd[PM + 1];  // C_q * rho_q
     int p;
     double x;   // x_k
     double nug; // nugget added to g inside the integrand
 };
 
 // d_k f at y (and optionally the second derivative).
-template <int BK, bool SECOND>
-MPD_HD double diag_eval(const DiagCoef& dc, double y, double* z2out) {
+template <int BK, bool SECOND, int PM>
+MPD_HD double diag_eval(const DiagCoef<PM>& dc, double y, double* z2out) {
     const int p = dc.p;
     if constexpr (BK == BK_PH) {
         // He_0 = 1, He_1 = y, He_r = y He_{r-1} - (r-1) He_{r-2}
         double hm2 = 0.0, hm1 = 1.0;  // He_{r-2}, He_{r-1}
         double z = 0.0, z2 = 0.0;
 #pragma unroll
-        for (int r = 0; r < MAXP; ++r) {
+        for (int r = 0; r < PM; ++r) {
             if (r >= p) break;
             double hr = (r == 0) ? 1.0 : ((r == 1) ? y : y * hm1 - (r - 1.0) * hm2);
             z += (r + 1.0) * dc.cd[r + 1] * hr;
@@ -165,7 +166,7 @@ MPD_HD double diag_eval(const DiagCoef& dc, double y, double* z2out) {
         // H_0 = 1, H_1 = 2y, H_k = 2y H_{k-1} - 2(k-1) H_{k-2}
         double hkm1 = 0.0, hkm2 = 0.0, s = 0.0, s2 = 0.0;
 #pragma unroll
-        for (int k = 0; k < MAXP - 1; ++k) {
+        for (int k = 0; k < PM - 1; ++k) {
             if (k + 2 > p) break;
             double hk = (k == 0) ? 1.0 : ((k == 1) ? 2.0 * y : 2.0 * y * hkm1 - 2.0 * (k - 1.0) * hkm2);
             double dk = (k == 0) ? 0.0 : 2.0 * k * hkm1;  // H'_k = 2k H_{k-1}
@@ -180,13 +181,13 @@ MPD_HD double diag_eval(const DiagCoef& dc, double y, double* z2out) {
 }
 
 // B'_q(y) for q = 0..p (raw derivative basis; phi~'_q = rho_q B'_q).
-template <int BK>
+template <int BK, int PM>
 MPD_HD void diag_basis(int p, double y, double* bq) {
     bq[0] = 0.0;
     if constexpr (BK == BK_PH) {
         double hm2 = 0.0, hm1 = 1.0;
 #pragma unroll
-        for (int r = 0; r < MAXP; ++r) {
+        for (int r = 0; r < PM; ++r) {
             if (r >= p) break;
             double hr = (r == 0) ? 1.0 : ((r == 1) ? y : y * hm1 - (r - 1.0) * hm2);
             bq[r + 1] = (r + 1.0) * hr;
@@ -203,7 +204,7 @@ MPD_HD void diag_basis(int p, double y, double* bq) {
         double ex = exp(-0.5 * y * y);
         double hkm1 = 0.0, hkm2 = 0.0;
 #pragma unroll
-        for (int k = 0; k < MAXP - 1; ++k) {
+        for (int k = 0; k < PM - 1; ++k) {
             if (k + 2 > p) break;
             double hk = (k == 0) ? 1.0 : ((k == 1) ? 2.0 * y : 2.0 * y * hkm1 - 2.0 * (k - 1.0) * hkm2);
             double dk = (k == 0) ? 0.0 : 2.0 * k * hkm1;
@@ -215,7 +216,7 @@ MPD_HD void diag_basis(int p, double y, double* bq) {
 }
 
 // B''_q(y) (raw second-derivative basis, same folding as B'_q).
-template <int BK>
+template <int BK, int PM>
 MPD_HD void diag_basis2(int p, double y, double* bq) {
     bq[0] = 0.0;
     if (p >= 1) bq[1] = 0.0;
@@ -223,7 +224,7 @@ MPD_HD void diag_basis2(int p, double y, double* bq) {
         // He''_q = q (q-1) He_{q-2}
         double hm2 = 0.0, hm1 = 1.0;
 #pragma unroll
-        for (int r = 0; r < MAXP - 1; ++r) {
+        for (int r = 0; r < PM - 1; ++r) {
             if (r + 2 > p) break;
             double hr = (r == 0) ? 1.0 : ((r == 1) ? y : y * hm1 - (r - 1.0) * hm2);
             bq[r + 2] = (r + 2.0) * (r + 1.0) * hr;
@@ -239,7 +240,7 @@ MPD_HD void diag_basis2(int p, double y, double* bq) {
         double ex = exp(-0.5 * y * y);
         double hkm1 = 0.0, hkm2 = 0.0;
 #pragma unroll
-        for (int k = 0; k < MAXP - 1; ++k) {
+        for (int k = 0; k < PM - 1; ++k) {
             if (k + 2 > p) break;
             double hk = (k == 0) ? 1.0 : ((k == 1) ? 2.0 * y : 2.0 * y * hkm1 - 2.0 * (k - 1.0) * hkm2);
             bq[k + 2] = ex * (y * y - (2.0 * k + 1.0)) * hk;
@@ -371,11 +372,11 @@ __host__ __device__ void fill_values_derivs(const MapDev& M, int ndims, const do
 // Group passes over the term records of one component.
 
 // C_q = sum_{j in group q} w_j P_j for q = 0..p.
-template <int ND>
+template <int ND, int PM>
 MPD_HD void coeff_groups_nd(const MapDev& M, const CompDev& c, const double* w, long ws,
                                                 const double* tab, int stride, double* C) {
 #pragma unroll
-    for (int q = 0; q <= MAXP; ++q) {
+    for (int q = 0; q <= PM; ++q) {
         if (q > c.p) break;
         const GroupDev g = M.groups[c.groupOff + q];
         double acc = 0.0;
@@ -390,26 +391,27 @@ MPD_HD void coeff_groups_nd(const MapDev& M, const CompDev& c, const double* w, 
     }
 }
 
+template <int PM>
 MPD_HD void coeff_groups(const MapDev& M, const CompDev& c, const double* w, long ws,
                                              const double* tab, int stride, double* C) {
     switch (M.nd) {
-        case 1: coeff_groups_nd<1>(M, c, w, ws, tab, stride, C); break;
-        case 2: coeff_groups_nd<2>(M, c, w, ws, tab, stride, C); break;
-        case 3: coeff_groups_nd<3>(M, c, w, ws, tab, stride, C); break;
-        case 4: coeff_groups_nd<4>(M, c, w, ws, tab, stride, C); break;
-        case 5: coeff_groups_nd<5>(M, c, w, ws, tab, stride, C); break;
-        case 6: coeff_groups_nd<6>(M, c, w, ws, tab, stride, C); break;
-        case 7: coeff_groups_nd<7>(M, c, w, ws, tab, stride, C); break;
-        default: coeff_groups_nd<8>(M, c, w, ws, tab, stride, C); break;
+        case 1: coeff_groups_nd<1, PM>(M, c, w, ws, tab, stride, C); break;
+        case 2: coeff_groups_nd<2, PM>(M, c, w, ws, tab, stride, C); break;
+        case 3: coeff_groups_nd<3, PM>(M, c, w, ws, tab, stride, C); break;
+        case 4: coeff_groups_nd<4, PM>(M, c, w, ws, tab, stride, C); break;
+        case 5: coeff_groups_nd<5, PM>(M, c, w, ws, tab, stride, C); break;
+        case 6: coeff_groups_nd<6, PM>(M, c, w, ws, tab, stride, C); break;
+        case 7: coeff_groups_nd<7, PM>(M, c, w, ws, tab, stride, C); break;
+        default: coeff_groups_nd<8, PM>(M, c, w, ws, tab, stride, C); break;
     }
 }
 
 // Visit every (real) term: op(coeff index, P_j, q).
-template <int ND, class Op>
+template <int ND, int PM, class Op>
 MPD_HD void visit_terms_nd(const MapDev& M, const CompDev& c, const double* tab, int stride,
                                                Op& op) {
 #pragma unroll
-    for (int q = 0; q <= MAXP; ++q) {
+    for (int q = 0; q <= PM; ++q) {
         if (q > c.p) break;
         const GroupDev g = M.groups[c.groupOff + q];
         for (int j = g.start; j < g.start + g.count; ++j) {
@@ -422,36 +424,38 @@ MPD_HD void visit_terms_nd(const MapDev& M, const CompDev& c, const double* tab,
     }
 }
 
-template <class Op>
+template <int PM, class Op>
 MPD_HD void visit_terms(const MapDev& M, const CompDev& c, const double* tab, int stride,
                                             Op& op) {
     switch (M.nd) {
-        case 1: visit_terms_nd<1>(M, c, tab, stride, op); break;
-        case 2: visit_terms_nd<2>(M, c, tab, stride, op); break;
-        case 3: visit_terms_nd<3>(M, c, tab, stride, op); break;
-        case 4: visit_terms_nd<4>(M, c, tab, stride, op); break;
-        case 5: visit_terms_nd<5>(M, c, tab, stride, op); break;
-        case 6: visit_terms_nd<6>(M, c, tab, stride, op); break;
-        case 7: visit_terms_nd<7>(M, c, tab, stride, op); break;
-        default: visit_terms_nd<8>(M, c, tab, stride, op); break;
+        case 1: visit_terms_nd<1, PM>(M, c, tab, stride, op); break;
+        case 2: visit_terms_nd<2, PM>(M, c, tab, stride, op); break;
+        case 3: visit_terms_nd<3, PM>(M, c, tab, stride, op); break;
+        case 4: visit_terms_nd<4, PM>(M, c, tab, stride, op); break;
+        case 5: visit_terms_nd<5, PM>(M, c, tab, stride, op); break;
+        case 6: visit_terms_nd<6, PM>(M, c, tab, stride, op); break;
+        case 7: visit_terms_nd<7, PM>(M, c, tab, stride, op); break;
+        default: visit_terms_nd<8, PM>(M, c, tab, stride, op); break;
     }
 }
 
 // Diagonal coefficients of component c from its group sums.
+template <int PM>
 MPD_HD void make_diag(const MapDev& M, const CompDev& c, const double* C, double xk, double nug,
-                                          DiagCoef& dc) {
+                                          DiagCoef<PM>& dc) {
     dc.p = c.p;
     dc.x = xk;
     dc.nug = nug;
 #pragma unroll
-    for (int q = 0; q <= MAXP; ++q) dc.cd[q] = (q <= c.p) ? C[q] * M.rho[q] : 0.0;
+    for (int q = 0; q <= PM; ++q) dc.cd[q] = (q <= c.p) ? C[q] * M.rho[q] : 0.0;
 }
 
 // f(x_<k, 0) = sum_q C_q phi~_q(0).
+template <int PM>
 MPD_HD double f_at_zero(const MapDev& M, const CompDev& c, const double* C) {
     double f = 0.0;
 #pragma unroll
-    for (int q = 0; q <= MAXP; ++q) {
+    for (int q = 0; q <= PM; ++q) {
         if (q > c.p) break;
         f += C[q] * M.phi0[q];
     }
@@ -480,9 +484,9 @@ struct NodeV<FL_INPUT> {
     double F, h, g1;
 };
 
-template <int BK, int PK, int FL>
-MPD_HD NodeV<FL> eval_node(const DiagCoef& dc, double t) {
-    double z = diag_eval<BK, false>(dc, t * dc.x, nullptr);
+template <int BK, int PK, int FL, int PM>
+MPD_HD NodeV<FL> eval_node(const DiagCoef<PM>& dc, double t) {
+    double z = diag_eval<BK, false, PM>(dc, t * dc.x, nullptr);
     PosF<PK> gv = posfunc<PK>(z);
     NodeV<FL> v;
     v.F = dc.x * (gv.g + dc.nug);
@@ -491,27 +495,28 @@ MPD_HD NodeV<FL> eval_node(const DiagCoef& dc, double t) {
     return v;
 }
 
+template <int PM>
 struct AdjAcc {
-    double S[MAXP + 1];
+    double S[PM + 1];
     double U;
 };
 
 // Adds w * v(t) at one node to the adjoint accumulators.
-template <int BK, int FL>
-MPD_HD void adj_node(const DiagCoef& dc, double t, double w, const NodeV<FL>& v, AdjAcc& A) {
+template <int BK, int FL, int PM>
+MPD_HD void adj_node(const DiagCoef<PM>& dc, double t, double w, const NodeV<FL>& v, AdjAcc<PM>& A) {
     if constexpr (FL >= FL_COEF) {
-        double bq[MAXP + 1];
+        double bq[PM + 1];
         const double y = t * dc.x;
-        diag_basis<BK>(dc.p, y, bq);
+        diag_basis<BK, PM>(dc.p, y, bq);
         const double wh = w * v.h;
 #pragma unroll
-        for (int q = 1; q <= MAXP; ++q) {
+        for (int q = 1; q <= PM; ++q) {
             if (q > dc.p) break;
             A.S[q] += wh * bq[q];
         }
         if constexpr (FL == FL_INPUT) {
             double z2;
-            diag_eval<BK, true>(dc, y, &z2);
+            diag_eval<BK, true, PM>(dc, y, &z2);
             A.U += w * (v.g1 + t * v.h * z2);
         }
     }
@@ -526,13 +531,13 @@ struct AsRec {
 
 // AdaptiveSimpson::Integrate (Quadrature.h:553-676): depth-first, left first,
 // accepted intervals add intFine in left-to-right order.
-template <int BK, int PK, int FL>
-__host__ __device__ double integrate_as(const QuadDev& Q, const DiagCoef& dc, AdjAcc& A, int* nevals) {
+template <int BK, int PK, int FL, int PM>
+__host__ __device__ double integrate_as(const QuadDev& Q, const DiagCoef<PM>& dc, AdjAcc<PM>& A, int* nevals) {
     AsRec<FL> stk[MAXSUB];
     int sp = 0;
-    NodeV<FL> fa = eval_node<BK, PK, FL>(dc, 0.0);
-    NodeV<FL> fb = eval_node<BK, PK, FL>(dc, 1.0);
-    NodeV<FL> fm = eval_node<BK, PK, FL>(dc, 0.5);
+    NodeV<FL> fa = eval_node<BK, PK, FL, PM>(dc, 0.0);
+    NodeV<FL> fb = eval_node<BK, PK, FL, PM>(dc, 1.0);
+    NodeV<FL> fm = eval_node<BK, PK, FL, PM>(dc, 0.5);
     int ne = 3;
     double a = 0.0, b = 1.0, res = 0.0;
     int level = 0;
@@ -540,8 +545,8 @@ __host__ __device__ double integrate_as(const QuadDev& Q, const DiagCoef& dc, Ad
         const double m = 0.5 * (a + b);
         const double lm = 0.5 * (a + m);
         const double rm = 0.5 * (m + b);
-        NodeV<FL> flm = eval_node<BK, PK, FL>(dc, lm);
-        NodeV<FL> frm = eval_node<BK, PK, FL>(dc, rm);
+        NodeV<FL> flm = eval_node<BK, PK, FL, PM>(dc, lm);
+        NodeV<FL> frm = eval_node<BK, PK, FL, PM>(dc, rm);
         ne += 2;
         const double coarse = ((b - a) / 6.0) * (fa.F + 4.0 * fm.F + fb.F);
         double fine = ((m - a) / 6.0) * (fa.F + 4.0 * flm.F + fm.F);
@@ -552,11 +557,11 @@ __host__ __device__ double integrate_as(const QuadDev& Q, const DiagCoef& dc, Ad
             res += fine;
             if constexpr (FL >= FL_COEF) {
                 const double wl = (m - a) / 6.0, wr = (b - m) / 6.0;
-                adj_node<BK, FL>(dc, a, wl, fa, A);
-                adj_node<BK, FL>(dc, lm, 4.0 * wl, flm, A);
-                adj_node<BK, FL>(dc, m, wl + wr, fm, A);
-                adj_node<BK, FL>(dc, rm, 4.0 * wr, frm, A);
-                adj_node<BK, FL>(dc, b, wr, fb, A);
+                adj_node<BK, FL, PM>(dc, a, wl, fa, A);
+                adj_node<BK, FL, PM>(dc, lm, 4.0 * wl, flm, A);
+                adj_node<BK, FL, PM>(dc, m, wl + wr, fm, A);
+                adj_node<BK, FL, PM>(dc, rm, 4.0 * wr, frm, A);
+                adj_node<BK, FL, PM>(dc, b, wr, fb, A);
             }
             if (sp == 0) break;
             --sp;
@@ -594,16 +599,16 @@ struct AccRec {
 // AdaptiveClenshawCurtis::Integrate (Quadrature.h:819-968).  NF is the fine
 // rule size when known at compile time (5 for quadPts = 5), or MAXNF with the
 // runtime size in Q.nf.
-template <int BK, int PK, int FL, int NF>
-__host__ __device__ double integrate_acc(const QuadDev& Q, const DiagCoef& dc, AdjAcc& A, int* nevals) {
+template <int BK, int PK, int FL, int NF, int PM>
+__host__ __device__ double integrate_acc(const QuadDev& Q, const DiagCoef<PM>& dc, AdjAcc<PM>& A, int* nevals) {
     AccRec<FL> stk[MAXSUB];
     int sp = 0;
     const int nf = (NF == MAXNF) ? Q.nf : NF;
     const int nc = (nf - 1) / 2 + 1;
     const int cRight = nc - 1, cMid = cRight / 2;
     const int fRight = nf - 1, fMid = fRight / 2;
-    NodeV<FL> fa = eval_node<BK, PK, FL>(dc, 0.0);
-    NodeV<FL> fb = eval_node<BK, PK, FL>(dc, 1.0);
+    NodeV<FL> fa = eval_node<BK, PK, FL, PM>(dc, 0.0);
+    NodeV<FL> fb = eval_node<BK, PK, FL, PM>(dc, 1.0);
     int ne = 2;
     double a = 0.0, b = 1.0, res = 0.0;
     int level = 0;
@@ -611,7 +616,7 @@ __host__ __device__ double integrate_acc(const QuadDev& Q, const DiagCoef& dc, A
     while (true) {
         const double mid = 0.5 * (a + b);
         const double scale = 0.5 * (b - a);
-        NodeV<FL> fm = eval_node<BK, PK, FL>(dc, mid);
+        NodeV<FL> fm = eval_node<BK, PK, FL, PM>(dc, mid);
         ne += 1;
         double coarse = scale * Q.cw[0] * fa.F;
         coarse += scale * Q.cw[cMid] * fm.F;
@@ -622,7 +627,7 @@ __host__ __device__ double integrate_acc(const QuadDev& Q, const DiagCoef& dc, A
 #pragma unroll
         for (int i = 1; i < NF; ++i) {
             if (i >= cMid) break;
-            NodeV<FL> v = eval_node<BK, PK, FL>(dc, mid + scale * Q.cp[i]);
+            NodeV<FL> v = eval_node<BK, PK, FL, PM>(dc, mid + scale * Q.cp[i]);
             ne += 1;
             coarse += scale * Q.cw[i] * v.F;
             fine += scale * Q.fw[2 * i] * v.F;
@@ -633,7 +638,7 @@ __host__ __device__ double integrate_acc(const QuadDev& Q, const DiagCoef& dc, A
             if (i + cMid + 1 >= cRight + 1) break;
             const int ii = cMid + i;
             if (ii >= cRight) break;
-            NodeV<FL> v = eval_node<BK, PK, FL>(dc, mid + scale * Q.cp[ii]);
+            NodeV<FL> v = eval_node<BK, PK, FL, PM>(dc, mid + scale * Q.cp[ii]);
             ne += 1;
             coarse += scale * Q.cw[ii] * v.F;
             fine += scale * Q.fw[2 * ii] * v.F;
@@ -642,7 +647,7 @@ __host__ __device__ double integrate_acc(const QuadDev& Q, const DiagCoef& dc, A
 #pragma unroll
         for (int i = 1; i < NF; i += 2) {
             if (i >= nf) break;
-            NodeV<FL> v = eval_node<BK, PK, FL>(dc, mid + scale * Q.fp[i]);
+            NodeV<FL> v = eval_node<BK, PK, FL, PM>(dc, mid + scale * Q.fp[i]);
             ne += 1;
             fine += scale * Q.fw[i] * v.F;
             inner[i] = v;
@@ -652,14 +657,14 @@ __host__ __device__ double integrate_acc(const QuadDev& Q, const DiagCoef& dc, A
         if (((err < tol) || (level == Q.maxSub - 1)) && (level >= Q.minSub)) {
             res += fine;
             if constexpr (FL >= FL_COEF) {
-                adj_node<BK, FL>(dc, a, scale * Q.fw[0], fa, A);
-                adj_node<BK, FL>(dc, mid, scale * Q.fw[fMid], fm, A);
-                adj_node<BK, FL>(dc, b, scale * Q.fw[fRight], fb, A);
+                adj_node<BK, FL, PM>(dc, a, scale * Q.fw[0], fa, A);
+                adj_node<BK, FL, PM>(dc, mid, scale * Q.fw[fMid], fm, A);
+                adj_node<BK, FL, PM>(dc, b, scale * Q.fw[fRight], fb, A);
 #pragma unroll
                 for (int i = 1; i < NF - 1; ++i) {
                     if (i >= fRight) break;
                     if (i == fMid) continue;
-                    adj_node<BK, FL>(dc, mid + scale * Q.fp[i], scale * Q.fw[i], inner[i], A);
+                    adj_node<BK, FL, PM>(dc, mid + scale * Q.fp[i], scale * Q.fw[i], inner[i], A);
                 }
             }
             if (sp == 0) break;
@@ -685,19 +690,66 @@ __host__ __device__ double integrate_acc(const QuadDev& Q, const DiagCoef& dc, A
     return res;
 }
 
-template <int BK, int PK, int QK, int FL>
-MPD_HD double integrate(const QuadDev& Q, const DiagCoef& dc, AdjAcc& A, int* nevals) {
-    if constexpr (QK == QK_AS) return integrate_as<BK, PK, FL>(Q, dc, A, nevals);
-    else if constexpr (QK == QK_ACC5) return integrate_acc<BK, PK, FL, 5>(Q, dc, A, nevals);
-    else return integrate_acc<BK, PK, FL, MAXNF>(Q, dc, A, nevals);
+template <int BK, int PK, int QK, int FL, int PM>
+MPD_HD double integrate(const QuadDev& Q, const DiagCoef<PM>& dc, AdjAcc<PM>& A, int* nevals) {
+    if constexpr (QK == QK_AS) return integrate_as<BK, PK, FL, PM>(Q, dc, A, nevals);
+    else if constexpr (QK == QK_ACC5) return integrate_acc<BK, PK, FL, 5, PM>(Q, dc, A, nevals);
+    else return integrate_acc<BK, PK, FL, MAXNF, PM>(Q, dc, A, nevals);
 }
 
-MPD_HD void zero_adj(AdjAcc& A) {
+template <int PM>
+MPD_HD void zero_adj(AdjAcc<PM>& A) {
 #pragma unroll
-    for (int q = 0; q <= MAXP; ++q) A.S[q] = 0.0;
+    for (int q = 0; q <= PM; ++q) A.S[q] = 0.0;
     A.U = 0.0;
 }
 
+
+// ---------------------------------------------------------------------------
+// Forward + adjoint of one component at one point for the fused pass: T_c,
+// log g(d_c f), the adjoint integrals and
+//   G_q = s (phi~_q(0) + rho_q S'_q) + l rho_q B'_q(x_c) g'/g,
+// with s = T_c and l = -1 in KL mode (then loss += 0.5 T_c^2 - logdet), or the
+// given sensitivities otherwise.  G[0..p] is written; G is zero for an
+// inactive lane.
+template <int BK, int PK, int QK, int PM>
+MPD_HD void comp_core(const MapDev& M, const CompDev& cd, const double* C, double xk, bool active, int klMode,
+                      double s_in, double l_in, double* G, double& loss) {
+#pragma unroll
+    for (int q = 0; q <= PM; ++q) G[q] = 0.0;
+    if (!active) return;
+    const double f0 = f_at_zero<PM>(M, cd, C);
+    DiagCoef<PM> dc;
+    make_diag<PM>(M, cd, C, xk, M.nugget, dc);
+    AdjAcc<PM> A;
+    zero_adj<PM>(A);
+    double T = integrate<BK, PK, QK, FL_COEF, PM>(M.quad, dc, A, nullptr) + f0;
+    if (M.nugget != 0.0) {
+        // Evaluate drops the nugget (MonotoneComponent.h:287): its tree may differ.
+        DiagCoef<PM> dv = dc;
+        dv.nug = 0.0;
+        AdjAcc<PM> Av;
+        T = integrate<BK, PK, QK, FL_VAL, PM>(M.quad, dv, Av, nullptr) + f0;
+    }
+    const double z = diag_eval<BK, false, PM>(dc, xk, nullptr);
+    const PosF<PK> gv = posfunc<PK>(z);
+    const double ldv = (gv.g <= 0) ? -__builtin_inf() : log(gv.g);
+    const double r = gv.gp * (1.0 / gv.g);
+    double s = s_in, l = l_in;
+    if (klMode) {
+        s = T;
+        l = -1.0;
+        loss += 0.5 * T * T - ldv;
+    }
+    double bq[PM + 1];
+    diag_basis<BK, PM>(cd.p, xk, bq);
+    G[0] = s;
+#pragma unroll
+    for (int q = 1; q <= PM; ++q) {
+        if (q > cd.p) break;
+        G[q] = s * (M.phi0[q] + M.rho[q] * A.S[q]) + (l * r) * (M.rho[q] * bq[q]);
+    }
+}
 // ---------------------------------------------------------------------------
 // Wavefront reduction of 16 per-lane values into per-row partial sums.
 // After the four exchange stages lane l holds, in v[0], the sum over its row

@@ -397,6 +397,8 @@ void build_plan(mpart_map_s& m) {
     int pmax = 0;
     for (auto& cd : comps) pmax = std::max(pmax, cd.p);
     M.pmax = pmax;
+    // Register arrays of the kernels are sized by a compile-time degree bound.
+    if (pmax > 4) m.variant += 1000;
 }
 
 void init_coeffs_zero(mpart_map_s& m) {

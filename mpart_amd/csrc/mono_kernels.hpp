@@ -43,7 +43,7 @@ __device__ __forceinline__ int table_dims(const MapDev& M, int cb, int ce) {
 // (ContinuousDerivative + log, 105-125, 488-531) for components [cb, ce).
 // logdet is summed over the components (TriangularMap.cpp:100-125); with
 // ldAccumulate the launch adds to the existing values.
-template <int BK, int PK, int QK>
+template <int BK, int PK, int QK, int PM>
 __global__ void __launch_bounds__(256) k_eval(MapDev M, const double* w, long ws, PtsArg P, OutArg out, double* ld,
                                               long lds, int ldAccumulate, int* nevals, int cb, int ce, double* gtab) {
     extern __shared__ double smem[];
@@ -56,23 +56,23 @@ __global__ void __launch_bounds__(256) k_eval(MapDev M, const double* w, long ws
     double ldsum = 0.0;
     for (int c = cb; c < ce; ++c) {
         const CompDev cd = M.comps[c];
-        double C[MAXP + 1];
-        coeff_groups(M, cd, w + (long)cd.coeffOff * ws, ws, tab, (int)stride, C);
+        double C[PM + 1];
+        coeff_groups<PM>(M, cd, w + (long)cd.coeffOff * ws, ws, tab, (int)stride, C);
         const double xk = pt[(long)(cd.dim - 1) * P.rs];
-        DiagCoef dc;
+        DiagCoef<PM> dc;
         if (out.p) {
-            make_diag(M, cd, C, xk, 0.0, dc);
-            AdjAcc A;
+            make_diag<PM>(M, cd, C, xk, 0.0, dc);
+            AdjAcc<PM> A;
             int ne = 0;
-            double I = integrate<BK, PK, QK, FL_VAL>(M.quad, dc, A, nevals ? &ne : nullptr);
-            out.p[(long)cd.outRow * out.rs + n * out.cs] = I + f_at_zero(M, cd, C);
+            double I = integrate<BK, PK, QK, FL_VAL, PM>(M.quad, dc, A, nevals ? &ne : nullptr);
+            out.p[(long)cd.outRow * out.rs + n * out.cs] = I + f_at_zero<PM>(M, cd, C);
             if (nevals) nevals[(long)c * P.N + n] = ne;
         }
         if (ld) {
-            make_diag(M, cd, C, xk, 0.0, dc);
-            double z = diag_eval<BK, false>(dc, xk, nullptr);
+            make_diag<PM>(M, cd, C, xk, 0.0, dc);
+            double z = diag_eval<BK, false, PM>(dc, xk, nullptr);
             double g = posfunc<PK>(z).g;
-            ldsum += (g <= 0) ? -INFINITY : log(g);
+            ldsum += (g <= 0) ? -__builtin_inf() : log(g);
         }
     }
     if (ld) {
@@ -85,7 +85,7 @@ __global__ void __launch_bounds__(256) k_eval(MapDev M, const double* w, long ws
 // CoeffGrad of one component (CoeffJacobian + sensitivity scaling,
 // MonotoneComponent.h:166-182, 685-741): rows coeffOff.. of out receive
 //   sens_n * P_j (phi~_q(0) + rho_q S'_q).
-template <int BK, int PK, int QK>
+template <int BK, int PK, int QK, int PM>
 __global__ void __launch_bounds__(256) k_coeff_grad(MapDev M, const double* w, long ws, PtsArg P, const double* sens,
                                                     long srs, long scs, OutArg out, int c, double* gtab) {
     extern __shared__ double smem[];
@@ -96,17 +96,17 @@ __global__ void __launch_bounds__(256) k_coeff_grad(MapDev M, const double* w, l
     const double* pt = P.p + n * P.cs;
     const CompDev cd = M.comps[c];
     fill_values<BK>(M, cd.dim - 1, pt, P.rs, tab, (int)stride);
-    double C[MAXP + 1];
-    coeff_groups(M, cd, w + (long)cd.coeffOff * ws, ws, tab, (int)stride, C);
-    DiagCoef dc;
-    make_diag(M, cd, C, pt[(long)(cd.dim - 1) * P.rs], M.nugget, dc);
-    AdjAcc A;
-    zero_adj(A);
-    integrate<BK, PK, QK, FL_COEF>(M.quad, dc, A, nullptr);
+    double C[PM + 1];
+    coeff_groups<PM>(M, cd, w + (long)cd.coeffOff * ws, ws, tab, (int)stride, C);
+    DiagCoef<PM> dc;
+    make_diag<PM>(M, cd, C, pt[(long)(cd.dim - 1) * P.rs], M.nugget, dc);
+    AdjAcc<PM> A;
+    zero_adj<PM>(A);
+    integrate<BK, PK, QK, FL_COEF, PM>(M.quad, dc, A, nullptr);
     const double s = sens[(long)cd.outRow * srs + n * scs];
-    double G[MAXP + 1];
+    double G[PM + 1];
 #pragma unroll
-    for (int q = 0; q <= MAXP; ++q) G[q] = (q <= cd.p) ? (M.phi0[q] + M.rho[q] * A.S[q]) : 0.0;
+    for (int q = 0; q <= PM; ++q) G[q] = (q <= cd.p) ? (M.phi0[q] + M.rho[q] * A.S[q]) : 0.0;
     double* o = out.p + (long)cd.coeffOff * out.rs + n * out.cs;
     struct Op {
         double* o;
@@ -115,13 +115,13 @@ __global__ void __launch_bounds__(256) k_coeff_grad(MapDev M, const double* w, l
         double s;
         __device__ void operator()(uint32_t idx, double Pj, int q) { o[(long)idx * rs] = (Pj * G[q]) * s; }
     } op{o, out.rs, G, s};
-    visit_terms(M, cd, tab, (int)stride, op);
+    visit_terms<PM>(M, cd, tab, (int)stride, op);
 }
 
 // LogDeterminantCoeffGrad of one component with contDeriv (ContinuousMixedJacobian
 // + ContinuousDerivative, MonotoneComponent.h:185-206, 825-874):
 //   P_j phi~'_q(x_k) g'(d_k f) * (1 / g(d_k f)).
-template <int BK, int PK, int QK>
+template <int BK, int PK, int QK, int PM>
 __global__ void __launch_bounds__(256) k_logdet_coeff_grad(MapDev M, const double* w, long ws, PtsArg P, OutArg out,
                                                            int c, double* gtab) {
     extern __shared__ double smem[];
@@ -132,19 +132,19 @@ __global__ void __launch_bounds__(256) k_logdet_coeff_grad(MapDev M, const doubl
     const double* pt = P.p + n * P.cs;
     const CompDev cd = M.comps[c];
     fill_values<BK>(M, cd.dim - 1, pt, P.rs, tab, (int)stride);
-    double C[MAXP + 1];
-    coeff_groups(M, cd, w + (long)cd.coeffOff * ws, ws, tab, (int)stride, C);
-    DiagCoef dc;
+    double C[PM + 1];
+    coeff_groups<PM>(M, cd, w + (long)cd.coeffOff * ws, ws, tab, (int)stride, C);
+    DiagCoef<PM> dc;
     const double xk = pt[(long)(cd.dim - 1) * P.rs];
-    make_diag(M, cd, C, xk, 0.0, dc);
-    const double z = diag_eval<BK, false>(dc, xk, nullptr);
+    make_diag<PM>(M, cd, C, xk, 0.0, dc);
+    const double z = diag_eval<BK, false, PM>(dc, xk, nullptr);
     const PosF<PK> gv = posfunc<PK>(z);
     const double inv = 1.0 / gv.g;
-    double bq[MAXP + 1];
-    diag_basis<BK>(cd.p, xk, bq);
-    double G[MAXP + 1];
+    double bq[PM + 1];
+    diag_basis<BK, PM>(cd.p, xk, bq);
+    double G[PM + 1];
 #pragma unroll
-    for (int q = 0; q <= MAXP; ++q) G[q] = (q <= cd.p) ? M.rho[q] * bq[q] : 0.0;
+    for (int q = 0; q <= PM; ++q) G[q] = (q <= cd.p) ? M.rho[q] * bq[q] : 0.0;
     double* o = out.p + (long)cd.coeffOff * out.rs + n * out.cs;
     struct Op {
         double* o;
@@ -153,7 +153,7 @@ __global__ void __launch_bounds__(256) k_logdet_coeff_grad(MapDev M, const doubl
         double gp, inv;
         __device__ void operator()(uint32_t idx, double Pj, int q) { o[(long)idx * rs] = ((Pj * G[q]) * gp) * inv; }
     } op{o, out.rs, G, gv.gp, inv};
-    visit_terms(M, cd, tab, (int)stride, op);
+    visit_terms<PM>(M, cd, tab, (int)stride, op);
 }
 
 // ---------------------------------------------------------------------------
@@ -162,12 +162,12 @@ __global__ void __launch_bounds__(256) k_logdet_coeff_grad(MapDev M, const doubl
 //  mode 1: LogDeterminantInputGrad       (ContinuousMixedInputJacobian, 208-232, 877-926)
 // dT/dx_i = sum_q D^(i)_q (phi~_q(0) + rho_q S'_q) for i < k, dT/dx_k = U;
 // with D^(i)_q = sum_{j in group q} w_j dP_j/dx_i.
-template <int ND>
+template <int ND, int PM>
 __device__ __forceinline__ void input_groups_nd(const MapDev& M, const CompDev& c, const double* w, long ws,
                                                 const double* tab, const double* dtab, int stride, int wrt,
                                                 double* Dq) {
 #pragma unroll
-    for (int q = 0; q <= MAXP; ++q) {
+    for (int q = 0; q <= PM; ++q) {
         if (q > c.p) break;
         const GroupDev g = M.groups[c.groupOff + q];
         double acc = 0.0;
@@ -191,21 +191,22 @@ __device__ __forceinline__ void input_groups_nd(const MapDev& M, const CompDev& 
     }
 }
 
+template <int PM>
 __device__ __forceinline__ void input_groups(const MapDev& M, const CompDev& c, const double* w, long ws,
                                              const double* tab, const double* dtab, int stride, int wrt, double* Dq) {
     switch (M.nd) {
-        case 1: input_groups_nd<1>(M, c, w, ws, tab, dtab, stride, wrt, Dq); break;
-        case 2: input_groups_nd<2>(M, c, w, ws, tab, dtab, stride, wrt, Dq); break;
-        case 3: input_groups_nd<3>(M, c, w, ws, tab, dtab, stride, wrt, Dq); break;
-        case 4: input_groups_nd<4>(M, c, w, ws, tab, dtab, stride, wrt, Dq); break;
-        case 5: input_groups_nd<5>(M, c, w, ws, tab, dtab, stride, wrt, Dq); break;
-        case 6: input_groups_nd<6>(M, c, w, ws, tab, dtab, stride, wrt, Dq); break;
-        case 7: input_groups_nd<7>(M, c, w, ws, tab, dtab, stride, wrt, Dq); break;
-        default: input_groups_nd<8>(M, c, w, ws, tab, dtab, stride, wrt, Dq); break;
+        case 1: input_groups_nd<1, PM>(M, c, w, ws, tab, dtab, stride, wrt, Dq); break;
+        case 2: input_groups_nd<2, PM>(M, c, w, ws, tab, dtab, stride, wrt, Dq); break;
+        case 3: input_groups_nd<3, PM>(M, c, w, ws, tab, dtab, stride, wrt, Dq); break;
+        case 4: input_groups_nd<4, PM>(M, c, w, ws, tab, dtab, stride, wrt, Dq); break;
+        case 5: input_groups_nd<5, PM>(M, c, w, ws, tab, dtab, stride, wrt, Dq); break;
+        case 6: input_groups_nd<6, PM>(M, c, w, ws, tab, dtab, stride, wrt, Dq); break;
+        case 7: input_groups_nd<7, PM>(M, c, w, ws, tab, dtab, stride, wrt, Dq); break;
+        default: input_groups_nd<8, PM>(M, c, w, ws, tab, dtab, stride, wrt, Dq); break;
     }
 }
 
-template <int BK, int PK, int QK>
+template <int BK, int PK, int QK, int PM>
 __global__ void __launch_bounds__(256) k_input_grad(MapDev M, const double* w, long ws, PtsArg P, const double* sens,
                                                     long srs, long scs, OutArg out, int c, int mode, double* gtab) {
     extern __shared__ double smem[];
@@ -226,26 +227,26 @@ __global__ void __launch_bounds__(256) k_input_grad(MapDev M, const double* w, l
     const int k = cd.dim - 1;
     fill_values_derivs<BK>(M, k, pt, P.rs, tab, dtab, (int)stride);
     const double* wc = w + (long)cd.coeffOff * ws;
-    double C[MAXP + 1];
-    coeff_groups(M, cd, wc, ws, tab, (int)stride, C);
+    double C[PM + 1];
+    coeff_groups<PM>(M, cd, wc, ws, tab, (int)stride, C);
     const double xk = pt[(long)k * P.rs];
-    DiagCoef dc;
+    DiagCoef<PM> dc;
     double* o = out.p + n * out.cs;
     if (mode == 0) {
-        make_diag(M, cd, C, xk, M.nugget, dc);
-        AdjAcc A;
-        zero_adj(A);
-        integrate<BK, PK, QK, FL_INPUT>(M.quad, dc, A, nullptr);
+        make_diag<PM>(M, cd, C, xk, M.nugget, dc);
+        AdjAcc<PM> A;
+        zero_adj<PM>(A);
+        integrate<BK, PK, QK, FL_INPUT, PM>(M.quad, dc, A, nullptr);
         const double s = sens[(long)cd.outRow * srs + n * scs];
-        double G[MAXP + 1];
+        double G[PM + 1];
 #pragma unroll
-        for (int q = 0; q <= MAXP; ++q) G[q] = (q <= cd.p) ? (M.phi0[q] + M.rho[q] * A.S[q]) : 0.0;
+        for (int q = 0; q <= PM; ++q) G[q] = (q <= cd.p) ? (M.phi0[q] + M.rho[q] * A.S[q]) : 0.0;
         for (int i = 0; i < k; ++i) {
-            double Dq[MAXP + 1];
-            input_groups(M, cd, wc, ws, tab, dtab, (int)stride, i, Dq);
+            double Dq[PM + 1];
+            input_groups<PM>(M, cd, wc, ws, tab, dtab, (int)stride, i, Dq);
             double v = 0.0;
 #pragma unroll
-            for (int q = 0; q <= MAXP; ++q) {
+            for (int q = 0; q <= PM; ++q) {
                 if (q > cd.p) break;
                 v += Dq[q] * G[q];
             }
@@ -253,19 +254,19 @@ __global__ void __launch_bounds__(256) k_input_grad(MapDev M, const double* w, l
         }
         o[(long)k * out.rs] += A.U * s;
     } else {
-        make_diag(M, cd, C, xk, 0.0, dc);
+        make_diag<PM>(M, cd, C, xk, 0.0, dc);
         double z2;
-        const double z = diag_eval<BK, true>(dc, xk, &z2);
+        const double z = diag_eval<BK, true, PM>(dc, xk, &z2);
         const PosF<PK> gv = posfunc<PK>(z);
         const double inv = 1.0 / gv.g;
-        double bq[MAXP + 1];
-        diag_basis<BK>(cd.p, xk, bq);
+        double bq[PM + 1];
+        diag_basis<BK, PM>(cd.p, xk, bq);
         for (int i = 0; i < k; ++i) {
-            double Dq[MAXP + 1];
-            input_groups(M, cd, wc, ws, tab, dtab, (int)stride, i, Dq);
+            double Dq[PM + 1];
+            input_groups<PM>(M, cd, wc, ws, tab, dtab, (int)stride, i, Dq);
             double v = 0.0;
 #pragma unroll
-            for (int q = 0; q <= MAXP; ++q) {
+            for (int q = 0; q <= PM; ++q) {
                 if (q > cd.p) break;
                 v += Dq[q] * (M.rho[q] * bq[q]);
             }
@@ -279,15 +280,15 @@ __global__ void __launch_bounds__(256) k_input_grad(MapDev M, const double* w, l
 // Inverse of one component (MonotoneComponent::InverseImpl, MonotoneComponent.h:331-443;
 // RootFinding.h:20-178).  xs has xrows rows and 1 or N columns; its last row
 // holds the initial guess.  out may alias that row.
-template <int BK, int PK, int QK>
-__device__ double eval_T(const MapDev& M, const DiagCoef& dc0, double f0, double x) {
-    DiagCoef dc = dc0;
+template <int BK, int PK, int QK, int PM>
+__device__ double eval_T(const MapDev& M, const DiagCoef<PM>& dc0, double f0, double x) {
+    DiagCoef<PM> dc = dc0;
     dc.x = x;
-    AdjAcc A;
-    return integrate<BK, PK, QK, FL_VAL>(M.quad, dc, A, nullptr) + f0;
+    AdjAcc<PM> A;
+    return integrate<BK, PK, QK, FL_VAL, PM>(M.quad, dc, A, nullptr) + f0;
 }
 
-template <int BK, int PK, int QK>
+template <int BK, int PK, int QK, int PM>
 __global__ void __launch_bounds__(256) k_inverse(MapDev M, const double* w, long ws, const double* xs, long xrows,
                                                  long xcols, long xrs, long xcs, const double* ys, long yss, long N,
                                                  double* out, long os, int c, double xtol, double ytol, double* gtab) {
@@ -308,29 +309,29 @@ __global__ void __launch_bounds__(256) k_inverse(MapDev M, const double* w, long
     double* tab = lane_table(smem, gtab, stride);
     const CompDev cd = M.comps[c];
     fill_values<BK>(M, cd.dim - 1, pt, xrs, tab, (int)stride);
-    double C[MAXP + 1];
-    coeff_groups(M, cd, w + (long)cd.coeffOff * ws, ws, tab, (int)stride, C);
-    const double f0 = f_at_zero(M, cd, C);
-    DiagCoef dc;
-    make_diag(M, cd, C, x0, M.nugget, dc);
+    double C[PM + 1];
+    coeff_groups<PM>(M, cd, w + (long)cd.coeffOff * ws, ws, tab, (int)stride, C);
+    const double f0 = f_at_zero<PM>(M, cd, C);
+    DiagCoef<PM> dc;
+    make_diag<PM>(M, cd, C, x0, M.nugget, dc);
 
     // FindBracket
     double xlb = x0, xub = x0;
-    double ylb = eval_T<BK, PK, QK>(M, dc, f0, xlb);
+    double ylb = eval_T<BK, PK, QK, PM>(M, dc, f0, xlb);
     double yub = ylb;
     int info = 0;
     {
         const unsigned maxIts = 128;
         double stepSize = 1.0;
-        ylb = eval_T<BK, PK, QK>(M, dc, f0, xlb);
-        yub = eval_T<BK, PK, QK>(M, dc, f0, xub);
+        ylb = eval_T<BK, PK, QK, PM>(M, dc, f0, xlb);
+        yub = eval_T<BK, PK, QK, PM>(M, dc, f0, xub);
         if (ylb > yd) {
             double t = ylb; ylb = yub; yub = t;
             t = xlb; xlb = xub; xub = t;
             unsigned i;
             for (i = 0; i < maxIts; ++i) {
                 xlb = xub - stepSize;
-                ylb = eval_T<BK, PK, QK>(M, dc, f0, xlb);
+                ylb = eval_T<BK, PK, QK, PM>(M, dc, f0, xlb);
                 if ((fabs((yub - ylb) / (xub - xlb)) < 1e-12) && ((xub - xlb) > 10)) { info = -1; break; }
                 if (ylb > yd) { yub = ylb; xub = xlb; stepSize *= 2.0; } else break;
             }
@@ -339,7 +340,7 @@ __global__ void __launch_bounds__(256) k_inverse(MapDev M, const double* w, long
             unsigned i;
             for (i = 0; i < maxIts; ++i) {
                 xub = xlb + stepSize;
-                yub = eval_T<BK, PK, QK>(M, dc, f0, xub);
+                yub = eval_T<BK, PK, QK, PM>(M, dc, f0, xub);
                 if ((fabs((yub - ylb) / (xub - xlb)) < 1e-12) && ((xub - xlb) > 10)) { info = -1; break; }
                 if (yub < yd) { ylb = yub; xlb = xub; stepSize *= 2.0; } else break;
             }
@@ -362,7 +363,7 @@ __global__ void __launch_bounds__(256) k_inverse(MapDev M, const double* w, long
         xf += delta * sigma;
         const double rho = fmin(xtol * pow(2.0, nhalf + n0 - (double)it) - 0.5 * (xub - xlb), fabs(xf - xb));
         const double xc = xb - sigma * rho;
-        const double yc = eval_T<BK, PK, QK>(M, dc, f0, xc);
+        const double yc = eval_T<BK, PK, QK, PM>(M, dc, f0, xc);
         if (fabs(yc - yd) < ytol) {
             out[n * os] = xc;
             return;
@@ -391,7 +392,7 @@ __global__ void __launch_bounds__(256) k_inverse(MapDev M, const double* w, long
 // grid) and write one partial gradient row per block.
 //
 // LDS layout: [acc: (blockDim/64) * numCoeffs] [table: nslots * blockDim].
-template <int BK, int PK, int QK>
+template <int BK, int PK, int QK, int PM>
 __global__ void __launch_bounds__(256) k_fused(MapDev M, const double* w, long ws, PtsArg P, const double* sens,
                                                long srs, long scs, const double* lsens, long lss, int klMode,
                                                double* partialGrad, double* partialLoss, double* gtab,
@@ -431,51 +432,16 @@ __global__ void __launch_bounds__(256) k_fused(MapDev M, const double* w, long w
         for (int c = 0; c < M.ncomp; ++c) {
             const CompDev cd = M.comps[c];
             const double* wc = w + (long)cd.coeffOff * ws;
-            double C[MAXP + 1];
-            coeff_groups(M, cd, wc, ws, tab, (int)stride, C);
+            double C[PM + 1];
+            coeff_groups<PM>(M, cd, wc, ws, tab, (int)stride, C);
             const double xk = pt[(long)(cd.dim - 1) * P.rs];
-            const double f0 = f_at_zero(M, cd, C);
-            DiagCoef dc;
-            make_diag(M, cd, C, xk, M.nugget, dc);
-            AdjAcc A;
-            zero_adj(A);
-            double G[MAXP + 1];
-#pragma unroll
-            for (int q = 0; q <= MAXP; ++q) G[q] = 0.0;
-            if (active) {
-                double T = integrate<BK, PK, QK, FL_COEF>(M.quad, dc, A, nullptr) + f0;
-                if (M.nugget != 0.0) {
-                    // Evaluate drops the nugget (MonotoneComponent.h:287): its tree may differ.
-                    DiagCoef dv = dc;
-                    dv.nug = 0.0;
-                    AdjAcc Av;
-                    T = integrate<BK, PK, QK, FL_VAL>(M.quad, dv, Av, nullptr) + f0;
-                }
-                const double z = diag_eval<BK, false>(dc, xk, nullptr);
-                const PosF<PK> gv = posfunc<PK>(z);
-                const double ldv = (gv.g <= 0) ? -INFINITY : log(gv.g);
-                const double r = gv.gp * (1.0 / gv.g);
-                double s, l;
-                if (klMode) {
-                    s = T;
-                    l = -1.0;
-                    lossLane += 0.5 * T * T - ldv;
-                } else {
-                    s = sens ? sens[(long)cd.outRow * srs + n * scs] : 0.0;
-                    l = lsens ? lsens[n * lss] : 0.0;
-                }
-                double bq[MAXP + 1];
-                diag_basis<BK>(cd.p, xk, bq);
-                G[0] = s;
-#pragma unroll
-                for (int q = 1; q <= MAXP; ++q) {
-                    if (q > cd.p) break;
-                    G[q] = s * (M.phi0[q] + M.rho[q] * A.S[q]) + (l * r) * (M.rho[q] * bq[q]);
-                }
-            }
+            const double sv = (!klMode && sens) ? sens[(long)cd.outRow * srs + n * scs] : 0.0;
+            const double lv = (!klMode && lsens) ? lsens[n * lss] : 0.0;
+            double G[PM + 1];
+            comp_core<BK, PK, QK, PM>(M, cd, C, xk, active, klMode, sv, lv, G, lossLane);
             // Gradient pass: chunks of 16 term records per group.
 #pragma unroll
-            for (int q = 0; q <= MAXP; ++q) {
+            for (int q = 0; q <= PM; ++q) {
                 if (q > cd.p) break;
                 const GroupDev g = M.groups[cd.groupOff + q];
                 const double Gq = G[q];
