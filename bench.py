@@ -82,6 +82,17 @@ def measure(tri, x, steps, warmup, plain_steps, do_plain):
         res["fused_vs_plain_max_rel_err"] = float(max(abs(float(lp.item()) - fl) / abs(fl),
                                                       np.max(np.abs(gp - fg)) / np.max(np.abs(gp))))
         log(f"plain {pms:.3f} ms/step")
+        # The same fused pass through the precompiled generic kernel.
+        tri.SetCodegen(False)
+        tri.KLValueAndGrad(x)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(plain_steps):
+            tri.KLValueAndGrad(x)
+        torch.cuda.synchronize()
+        res["generic_fused_ms_per_step"] = (time.perf_counter() - t0) * 1e3 / plain_steps
+        tri.SetCodegen(True)
+        log(f"generic fused {res['generic_fused_ms_per_step']:.3f} ms/step")
     msets = [tri.GetMultiIndexSet(c) for c in range(tri.NumComponents())]
     _, counts = tri.EvaluateCounted(x)
     counts = counts.cpu().numpy()
@@ -130,6 +141,8 @@ def main():
 
     N = args.n
     tri = mt.CreateTriangular(D, D, ORDER, mt.MapOptions())
+    spec, compile_s, gen_err = tri.PrepareCodegen()
+    log(f"specialized kernels: {spec} ({compile_s:.2f} s) {gen_err}")
 
     # Headline: literal C3 (banana data, zero coefficients).
     x_np = banana_points(N, D, seed=0)
@@ -176,6 +189,7 @@ def main():
                                "ProbabilistHermite/SoftPlus/AdaptiveSimpson, N=500000",
                    "N": N, "dim": D, "order": ORDER, "num_coeffs": tri.numCoeffs},
         "fused": head,
+        "specialized_kernels": {"active": spec, "compile_s": compile_s, "error": gen_err},
         "active_quadrature_workload": second,
         "roofline": {"bound": "fp64-valu", "achieved": head["achieved_tflops"], "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": head["achieved_tflops"] / PEAK_FP64_TFLOPS, "traffic": None,

@@ -184,6 +184,26 @@ int mpart_evaluate_counted(mpart_map map, mpart_cview pts, mpart_view out, int32
  * components (0, default) for Evaluate and LogDeterminant. */
 int mpart_set_plain_launches(mpart_map map, int plain);
 
+/* Runtime specialization.  By default the first Evaluate / LogDeterminant /
+ * CoeffGrad / LogDeterminantCoeffGrad / KL / VJP call on a map compiles, with
+ * hipRTC, kernels specialized to the map's multi-index sets (every term a
+ * straight-line expression over register-resident basis values) and uses them
+ * from then on; maps with more than 4096 coefficients, or a failed
+ * compilation, use the precompiled generic kernels.  Equal structures share
+ * one compiled module per process.  No reference counterpart: the reference
+ * evaluates the multi-index set through its Kokkos kernels
+ * (MParT/MultivariateExpansionWorker.h:140-420).
+ *
+ * mpart_set_codegen(map, 0) selects the generic kernels for this map (1
+ * restores the default).  mpart_codegen_prepare compiles now (instead of on
+ * the first call) and reports *specialized = 1 when the specialized kernels
+ * are in use, 0 otherwise, and the compile time of the module in seconds (0
+ * when it came from the cache).  mpart_codegen_error returns the reason of a
+ * failed compilation ("" otherwise). */
+int mpart_set_codegen(mpart_map map, int enable);
+int mpart_codegen_prepare(mpart_map map, int32_t* specialized, double* compileSeconds);
+const char* mpart_codegen_error(mpart_map map);
+
 #ifdef __cplusplus
 }
 #endif

@@ -87,6 +87,8 @@ _SIGS = {
     "mpart_kl_value_and_grad_plain": [_vp, _CView, _vp, _vp, _vp],
     "mpart_coeff_vjp": [_vp, _CView, _CView, _vp, _i64, _vp, _vp],
     "mpart_set_plain_launches": [_vp, _int],
+    "mpart_set_codegen": [_vp, _int],
+    "mpart_codegen_prepare": [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)],
 }
 for _name, _args in _SIGS.items():
     _f = getattr(_lib, _name)
@@ -96,9 +98,11 @@ _lib.mpart_last_error.restype = ctypes.c_char_p
 _lib.mpart_last_error.argtypes = []
 _lib.mpart_default_options.argtypes = [_OPTP]
 _lib.mpart_default_options.restype = None
+_lib.mpart_codegen_error.argtypes = [_vp]
+_lib.mpart_codegen_error.restype = ctypes.c_char_p
 
 LIB_PATH = _LIB_PATH
-C_EXPORTS = tuple(_SIGS) + ("mpart_last_error", "mpart_default_options")
+C_EXPORTS = tuple(_SIGS) + ("mpart_last_error", "mpart_default_options", "mpart_codegen_error")
 
 
 def _check(rc):
@@ -327,6 +331,17 @@ class ConditionalMap:
 
     def SetPlainLaunches(self, plain=True):
         _check(_lib.mpart_set_plain_launches(self._h, int(bool(plain))))
+
+    def SetCodegen(self, enable=True):
+        """Specialized (hipRTC) kernels when enabled (default), generic kernels otherwise."""
+        _check(_lib.mpart_set_codegen(self._h, int(bool(enable))))
+
+    def PrepareCodegen(self):
+        """Compiles the specialized kernels now; returns (specialized, compile_seconds, error)."""
+        spec = ctypes.c_int32(0)
+        secs = ctypes.c_double(0.0)
+        _check(_lib.mpart_codegen_prepare(self._h, ctypes.byref(spec), ctypes.byref(secs)))
+        return bool(spec.value), float(secs.value), _lib.mpart_codegen_error(self._h).decode()
 
     # Impl methods (device tensors, preallocated outputs) --------------------
     def EvaluateImpl(self, pts, out):

@@ -24,7 +24,24 @@ CFLAGS = [
     "-Wno-unused-result",
 ]
 
-SOURCES = ["mono_host.hip", "k_eval.hip", "k_coef.hip", "k_input.hip", "k_inverse.hip", "k_fused.hip"]
+SOURCES = ["mono_host.hip", "codegen.hip", "k_eval.hip", "k_coef.hip", "k_input.hip", "k_inverse.hip",
+           "k_fused.hip"]
+ROCM_LIB = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib")
+
+
+def embed_device_header():
+    """Writes csrc/_mono_device_src.inc: mono_device.hpp as a C++ raw string
+    literal, compiled into the library for the runtime (hipRTC) specialization."""
+    with open(os.path.join(CSRC, "mono_device.hpp")) as f:
+        text = f.read()
+    if ")MPDSRC\"" in text:
+        raise RuntimeError("mono_device.hpp contains the raw-string delimiter")
+    out = os.path.join(CSRC, "_mono_device_src.inc")
+    body = 'R"MPDSRC(' + text + ')MPDSRC"\n'
+    old = open(out).read() if os.path.exists(out) else None
+    if old != body:
+        with open(out, "w") as f:
+            f.write(body)
 
 
 def _compile(src):
@@ -38,11 +55,12 @@ def _compile(src):
 
 def build(verbose=True, jobs=None):
     os.makedirs(BUILD, exist_ok=True)
+    embed_device_header()
     jobs = jobs or min(len(SOURCES), max(1, (os.cpu_count() or 4)))
     with cf.ThreadPoolExecutor(jobs) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", tmp, *objs]
+    cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", tmp, *objs, f"-L{ROCM_LIB}", "-lhiprtc"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-6000:]}")

@@ -1,0 +1,344 @@
+// codegen.hip -- source generation and hipRTC compilation (host code).
+#include "codegen.hpp"
+
+#include <hip/hiprtc.h>
+
+#include <chrono>
+#include <memory>
+#include <mutex>
+#include <sstream>
+#include <unordered_map>
+
+namespace mpg {
+
+// Embedded copy of mono_device.hpp (written by the build into _mono_device_src.inc).
+static const char* kDeviceHeader =
+#include "_mono_device_src.inc"
+    ;
+
+namespace {
+
+// Product of the off-diagonal basis factors of term t ("1.0" if none).
+std::string term_product(const GenComp& c, unsigned t, unsigned* q) {
+    std::string expr;
+    *q = 0;
+    for (unsigned i = c.starts[t]; i < c.starts[t + 1]; ++i) {
+        if (c.dims[i] == c.dim - 1) {
+            *q = c.orders[i];
+            continue;
+        }
+        std::string f = "b" + std::to_string(c.dims[i]) + "[" + std::to_string(c.orders[i]) + "]";
+        expr = expr.empty() ? f : "(" + expr + " * " + f + ")";
+    }
+    return expr.empty() ? std::string("1.0") : expr;
+}
+
+void emit_basis(std::ostringstream& o, const GenSpec& s, unsigned ndims) {
+    for (unsigned d = 0; d < ndims && d < s.dimDeg.size(); ++d) {
+        const int deg = s.dimDeg[d];
+        if (deg == 0) continue;
+        o << "    double b" << d << "[" << deg + 1 << "];\n";
+        o << "    basis_vals<GBK, " << deg << ">(M, active ? pt[" << d << " * prs] : 0.0, b" << d << ");\n";
+        o << "    if (!active) { ";
+        for (int k = 0; k <= deg; ++k) o << "b" << d << "[" << k << "] = 0.0; ";
+        o << "}\n";
+    }
+}
+
+// Empty asm with the basis values as in/out operands.  It costs no
+// instruction but makes the values opaque at this point, so the products of
+// the term expressions are formed where they are used: without it LICM hoists
+// every (loop-invariant) product out of the component loop and GVN keeps the
+// forward products alive across the quadrature for the gradient pass, which
+// multiplies the live registers.
+void emit_pin(std::ostringstream& o, const GenSpec& s, unsigned ndims, const char* indent) {
+    for (unsigned d = 0; d < ndims && d < s.dimDeg.size(); ++d)
+        for (int k = 1; k <= s.dimDeg[d]; ++k)
+            o << indent << "asm volatile(\"\" : \"+v\"(b" << d << "[" << k << "]));\n";
+}
+
+void emit_coeff_sums(std::ostringstream& o, const GenComp& c) {
+    for (unsigned q = 0; q <= c.p; ++q) o << "        C[" << q << "] = 0.0;\n";
+    const unsigned nt = unsigned(c.starts.size()) - 1;
+    for (unsigned t = 0; t < nt; ++t) {
+        unsigned q;
+        std::string P = term_product(c, t, &q);
+        o << "        C[" << q << "] += w[" << c.coeffOff + t << "] * " << P << ";\n";
+    }
+}
+
+}  // namespace
+
+std::string generate_source(const GenSpec& s) {
+    std::ostringstream o;
+    o << "#include \"mono_device.hpp\"\nusing namespace mpd;\n";
+    o << "#define GBK " << s.bk << "\n#define GPK " << s.pk << "\n#define GQK " << s.qk << "\n#define GPM " << s.pm
+      << "\n";
+    const unsigned ncomp = unsigned(s.comps.size());
+
+    // ---------------- fused forward + adjoint (KL / VJP) ----------------
+    if (s.fusedWavesPerEU > 0)
+        o << "#define GFUSED_ATTR __attribute__((amdgpu_waves_per_eu(" << s.fusedWavesPerEU << ")))\n";
+    else
+        o << "#define GFUSED_ATTR\n";
+    o << R"(
+extern "C" __global__ void __launch_bounds__(256) GFUSED_ATTR mpart_gen_fused(MapDev M, const double* __restrict__ w,
+        const double* __restrict__ pts, long prs, long pcs, long N, const double* sens, long srs, long scs,
+        const double* lsens, long lss, int klMode, double* partialGrad, double* partialLoss) {
+    extern __shared__ double acc[];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int nwaves = blockDim.x >> 6;
+)";
+    o << "    const int K = " << s.numCoeffs << ";\n";
+    o << R"(    for (int i = tid; i < nwaves * K; i += blockDim.x) acc[i] = 0.0;
+    __syncthreads();
+    double* myAcc = acc + (long)wave * K;
+    double lossLane = 0.0;
+    const long ntiles = (N + blockDim.x - 1) / blockDim.x;
+    for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const long n0 = tile * blockDim.x + tid;
+        const bool active = n0 < N;
+        const long n = active ? n0 : N - 1;
+        const double* pt = pts + n * pcs;
+)";
+    emit_basis(o, s, s.D - 1);
+    o << "#pragma unroll 1\n        for (int c = 0; c < " << ncomp << "; ++c) {\n";
+    emit_pin(o, s, s.D - 1, "        ");
+    o << "        const CompDev cd = M.comps[c];\n        double C[GPM + 1];\n        switch (c) {\n";
+    for (unsigned c = 0; c < ncomp; ++c) {
+        o << "        case " << c << ": {\n";
+        emit_coeff_sums(o, s.comps[c]);
+        o << "        break; }\n";
+    }
+    o << R"(        }
+        const double xk = pt[(long)(cd.dim - 1) * prs];
+        const double sv = (!klMode && sens) ? sens[(long)cd.outRow * srs + n * scs] : 0.0;
+        const double lv = (!klMode && lsens) ? lsens[n * lss] : 0.0;
+        double G[GPM + 1];
+        comp_core<GBK, GPK, GQK, GPM>(M, cd, C, xk, active, klMode, sv, lv, G, lossLane);
+)";
+    emit_pin(o, s, s.D - 1, "        ");
+    o << "        switch (c) {\n";
+    for (unsigned c = 0; c < ncomp; ++c) {
+        const GenComp& gc = s.comps[c];
+        const unsigned nt = unsigned(gc.starts.size()) - 1;
+        o << "        case " << c << ": {\n";
+        for (unsigned t0 = 0; t0 < nt; t0 += 16) {
+            const unsigned nv = std::min(16u, nt - t0);
+            o << "            { double v[16];\n";
+            for (unsigned u = 0; u < 16; ++u) {
+                if (u < nv) {
+                    unsigned q;
+                    std::string P = term_product(gc, t0 + u, &q);
+                    o << "              v[" << u << "] = G[" << q << "] * " << P << ";\n";
+                } else {
+                    o << "              v[" << u << "] = 0.0;\n";
+                }
+            }
+            o << "              reduce16_add(v, lane, myAcc + " << gc.coeffOff + t0 << ", " << nv << "); }\n";
+        }
+        o << "        break; }\n";
+    }
+    o << R"(        }
+        }
+    }
+    const double lw = wave_sum(lossLane);
+    __shared__ double lossWave[4];
+    if (lane == 0) lossWave[wave] = lw;
+    __syncthreads();
+    if (tid == 0) {
+        double l = 0.0;
+        for (int i = 0; i < nwaves; ++i) l += lossWave[i];
+        partialLoss[blockIdx.x] = l;
+    }
+    for (int k = tid; k < K; k += blockDim.x) {
+        double t = 0.0;
+        for (int i = 0; i < nwaves; ++i) t += acc[(long)i * K + k];
+        partialGrad[(long)blockIdx.x * K + k] = t;
+    }
+}
+)";
+
+    // ---------------- Evaluate / LogDeterminant ----------------
+    o << R"(
+extern "C" __global__ void __launch_bounds__(256) mpart_gen_eval(MapDev M, const double* __restrict__ w,
+        const double* __restrict__ pts, long prs, long pcs, long N, double* out, long ors, long ocs, double* ld,
+        long lds, int ldAcc, int* nevals, int cb, int ce) {
+    const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    const bool active = true;
+    const double* pt = pts + n * pcs;
+)";
+    emit_basis(o, s, s.D - 1);
+    o << "    double ldsum = 0.0;\n#pragma unroll 1\n    for (int c = cb; c < ce; ++c) {\n";
+    emit_pin(o, s, s.D - 1, "        ");
+    o << "        const CompDev cd = M.comps[c];\n        double C[GPM + 1];\n        switch (c) {\n";
+    for (unsigned c = 0; c < ncomp; ++c) {
+        o << "        case " << c << ": {\n";
+        emit_coeff_sums(o, s.comps[c]);
+        o << "        break; }\n";
+    }
+    o << R"(        }
+        const double xk = pt[(long)(cd.dim - 1) * prs];
+        DiagCoef<GPM> dc;
+        make_diag<GPM>(M, cd, C, xk, 0.0, dc);
+        if (out) {
+            AdjAcc<GPM> A;
+            int ne = 0;
+            const double I = integrate<GBK, GPK, GQK, FL_VAL, GPM>(M.quad, dc, A, nevals ? &ne : nullptr);
+            out[(long)cd.outRow * ors + n * ocs] = I + f_at_zero<GPM>(M, cd, C);
+            if (nevals) nevals[(long)c * N + n] = ne;
+        }
+        if (ld) {
+            const double z = diag_eval<GBK, false, GPM>(dc, xk, nullptr);
+            const double g = posfunc<GPK>(z).g;
+            ldsum += (g <= 0) ? -__builtin_inf() : log(g);
+        }
+    }
+    if (ld) {
+        if (ldAcc) ld[n * lds] += ldsum;
+        else ld[n * lds] = ldsum;
+    }
+}
+)";
+
+    // ---------------- CoeffGrad / LogDeterminantCoeffGrad (one component) ----------------
+    for (int kind = 0; kind < 2; ++kind) {
+        if (kind == 0)
+            o << R"(
+extern "C" __global__ void __launch_bounds__(256) mpart_gen_coef(MapDev M, const double* __restrict__ w,
+        const double* __restrict__ pts, long prs, long pcs, long N, const double* sens, long srs, long scs,
+        double* out, long ors, long ocs, int c) {
+)";
+        else
+            o << R"(
+extern "C" __global__ void __launch_bounds__(256) mpart_gen_ldcoef(MapDev M, const double* __restrict__ w,
+        const double* __restrict__ pts, long prs, long pcs, long N, double* out, long ors, long ocs, int c) {
+)";
+        o << R"(    const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    const bool active = true;
+    const double* pt = pts + n * pcs;
+    const CompDev cd = M.comps[c];
+)";
+        // Only the coordinates of component c are needed; the basis of every
+        // coordinate below the largest component is computed (cheap).
+        emit_basis(o, s, s.D - 1);
+        o << "    double C[GPM + 1];\n    switch (c) {\n";
+        for (unsigned c = 0; c < ncomp; ++c) {
+            o << "    case " << c << ": {\n";
+            emit_coeff_sums(o, s.comps[c]);
+            o << "    break; }\n";
+        }
+        o << "    }\n    const double xk = pt[(long)(cd.dim - 1) * prs];\n    DiagCoef<GPM> dc;\n    double G[GPM + 1];\n";
+        if (kind == 0) {
+            o << R"(    make_diag<GPM>(M, cd, C, xk, M.nugget, dc);
+    AdjAcc<GPM> A;
+    zero_adj<GPM>(A);
+    integrate<GBK, GPK, GQK, FL_COEF, GPM>(M.quad, dc, A, nullptr);
+    const double s = sens[(long)cd.outRow * srs + n * scs];
+#pragma unroll
+    for (int q = 0; q <= GPM; ++q) G[q] = (q <= cd.p) ? (M.phi0[q] + M.rho[q] * A.S[q]) : 0.0;
+)";
+        } else {
+            o << R"(    make_diag<GPM>(M, cd, C, xk, 0.0, dc);
+    const double z = diag_eval<GBK, false, GPM>(dc, xk, nullptr);
+    const PosF<GPK> gv = posfunc<GPK>(z);
+    const double inv = 1.0 / gv.g;
+    double bq[GPM + 1];
+    diag_basis<GBK, GPM>(cd.p, xk, bq);
+#pragma unroll
+    for (int q = 0; q <= GPM; ++q) G[q] = (q <= cd.p) ? M.rho[q] * bq[q] : 0.0;
+)";
+        }
+        emit_pin(o, s, s.D - 1, "    ");
+        o << "    double* o = out + n * ocs;\n    switch (c) {\n";
+        for (unsigned c = 0; c < ncomp; ++c) {
+            const GenComp& gc = s.comps[c];
+            const unsigned nt = unsigned(gc.starts.size()) - 1;
+            o << "    case " << c << ": {\n";
+            for (unsigned t = 0; t < nt; ++t) {
+                unsigned q;
+                std::string P = term_product(gc, t, &q);
+                if (kind == 0)
+                    o << "        o[" << gc.coeffOff + t << "L * ors] = (" << P << " * G[" << q << "]) * s;\n";
+                else
+                    o << "        o[" << gc.coeffOff + t << "L * ors] = ((" << P << " * G[" << q
+                      << "]) * gv.gp) * inv;\n";
+            }
+            o << "    break; }\n";
+        }
+        o << "    }\n}\n";
+    }
+    return o.str();
+}
+
+namespace {
+std::mutex g_mu;
+std::unordered_map<std::string, std::unique_ptr<GenModule>> g_cache;
+}  // namespace
+
+bool compile_source(const std::string& src, const std::string& arch, std::vector<char>& code, std::string& err) {
+    hiprtcProgram prog;
+    const char* hdrs[] = {kDeviceHeader};
+    const char* names[] = {"mono_device.hpp"};
+    if (hiprtcCreateProgram(&prog, src.c_str(), "mpart_gen.hip", 1, hdrs, names) != HIPRTC_SUCCESS) {
+        err = "hiprtcCreateProgram failed";
+        return false;
+    }
+    const std::string archOpt = "--offload-arch=" + arch;
+    const char* opts[] = {archOpt.c_str(), "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
+                          "-packed-fp32-ops"};
+    if (hiprtcCompileProgram(prog, 7, opts) != HIPRTC_SUCCESS) {
+        size_t ls = 0;
+        hiprtcGetProgramLogSize(prog, &ls);
+        std::string log(ls + 1, '\0');
+        hiprtcGetProgramLog(prog, &log[0]);
+        err = "hipRTC compilation failed: " + log.substr(0, 4000);
+        hiprtcDestroyProgram(&prog);
+        return false;
+    }
+    size_t cs = 0;
+    hiprtcGetCodeSize(prog, &cs);
+    code.resize(cs);
+    hiprtcGetCode(prog, code.data());
+    hiprtcDestroyProgram(&prog);
+    return true;
+}
+
+GenModule* get_module(const GenSpec& spec, std::string& err) {
+    const std::string src = generate_source(spec);
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_cache.find(src);
+    if (it != g_cache.end()) return it->second.get();
+    auto t0 = std::chrono::steady_clock::now();
+    int dev = 0;
+    std::string arch = "gfx950";
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess) {
+        const std::string a = prop.gcnArchName;
+        arch = a.substr(0, a.find(':'));
+    }
+    std::vector<char> code;
+    if (!compile_source(src, arch, code, err)) return nullptr;
+    auto m = std::make_unique<GenModule>();
+    if (hipModuleLoadData(&m->mod, code.data()) != hipSuccess) {
+        err = "hipModuleLoadData failed";
+        return nullptr;
+    }
+    if (hipModuleGetFunction(&m->fused, m->mod, "mpart_gen_fused") != hipSuccess ||
+        hipModuleGetFunction(&m->eval, m->mod, "mpart_gen_eval") != hipSuccess ||
+        hipModuleGetFunction(&m->coef, m->mod, "mpart_gen_coef") != hipSuccess ||
+        hipModuleGetFunction(&m->ldcoef, m->mod, "mpart_gen_ldcoef") != hipSuccess) {
+        err = "hipModuleGetFunction failed";
+        return nullptr;
+    }
+    m->compileSeconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    GenModule* out = m.get();
+    g_cache.emplace(src, std::move(m));
+    return out;
+}
+
+}  // namespace mpg

@@ -23,6 +23,8 @@
 #ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#else
+typedef __hip_internal::uint32_t uint32_t;
 #endif
 
 // Helpers usable from host code as well (a host harness runs the same
@@ -291,6 +293,40 @@ __host__ __device__ void fill_values(const MapDev& M, int ndims, const double* p
                 pm1 = v;
                 out[o * stride] = v;
             }
+        }
+    }
+}
+
+
+// Off-diagonal basis values phi_1..phi_DEG at x into b[1..DEG] (b[0] = 1),
+// the same arithmetic as fill_values; used by generated kernels that keep the
+// values in registers.
+template <int BK, int DEG>
+MPD_HD void basis_vals(const MapDev& M, double x, double* b) {
+    b[0] = 1.0;
+    if constexpr (BK == BK_PH) {
+        double vm2 = 1.0, vm1 = x;
+#pragma unroll
+        for (int o = 1; o <= DEG; ++o) {
+            double v = (o == 1) ? x : (1.0 * x + 0.0) * vm1 - (o - 1.0) * vm2;
+            if (o >= 2) {
+                vm2 = vm1;
+                vm1 = v;
+            }
+            b[o] = M.normalize ? v / M.norm[o] : v;
+        }
+    } else {
+        double pm1 = 0.0, pm2 = 0.0;
+#pragma unroll
+        for (int o = 1; o <= DEG; ++o) {
+            double v;
+            if (o == 1) v = x;
+            else if (o == 2) v = M.hfPi * exp(-0.5 * x * x);
+            else if (o == 3) v = M.hfSqrt2 * x * pm1;
+            else v = (x * pm1 - M.hfA[o - 2] * pm2) / M.hfB[o - 2];
+            pm2 = pm1;
+            pm1 = v;
+            b[o] = v;
         }
     }
 }
@@ -781,6 +817,15 @@ __device__ __forceinline__ double row_reduce16(double* v, int lane) {
     bfly_stage<2, 0x1B>(v, (lane & 2) != 0);
     bfly_stage<1, 0xB1>(v, (lane & 1) != 0);
     return v[0];
+}
+
+// Reduces 16 per-lane values over the wavefront and adds the 16 sums to
+// dst[0..nvalid) (lane i < nvalid adds the sum for value i).
+__device__ __forceinline__ void reduce16_add(double* v, int lane, double* dst, int nvalid) {
+    double r = row_reduce16(v, lane);
+    r += __shfl_xor(r, 16, 64);
+    r += __shfl_xor(r, 32, 64);
+    if (lane < nvalid) dst[lane] += r;
 }
 
 __device__ __forceinline__ double wave_sum(double v) {

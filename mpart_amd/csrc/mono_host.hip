@@ -3,7 +3,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstdint>
 #include <cstring>
 #include <memory>
@@ -13,6 +15,7 @@
 #include <vector>
 
 #include "../../include/mpart_amd.h"
+#include "codegen.hpp"
 #include "launchers.hpp"
 
 namespace {
@@ -197,7 +200,16 @@ struct mpart_map_s {
     long coeffStride = 1;
 
     // work buffers
-    DBuf partialGrad, partialLoss, gtab, wT, wLd, wG, wLG, wFull;
+    DBuf partialGrad, partialLoss, gtab, wT, wLd, wG, wLG, wFull, wCoeffs;
+
+    // runtime-specialized kernels (codegen.hip)
+    bool genEnabled = true;
+    bool genTried = false;
+    mpg::GenModule* gen = nullptr;
+    double genSeconds = 0.0;
+    std::string genErr;
+    int genOcc = 0;
+    size_t genOccSmem = 0;
 };
 
 namespace {
@@ -471,11 +483,93 @@ void check_view(const char* what, int64_t rows, int64_t cols, int64_t erows, int
                        " matrix, got " + std::to_string(rows) + " x " + std::to_string(cols));
 }
 
+// ---------------------------------------------------------------------------
+// Runtime-specialized kernels.
+constexpr unsigned GEN_MAX_COEFFS = 4096;
+
+bool codegen_env_enabled() {
+    static const bool on = [] {
+        const char* e = std::getenv("MPART_CODEGEN");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
+mpg::GenModule* gen_for(mpart_map_s& m) {
+    if (!m.genEnabled || !codegen_env_enabled()) return nullptr;
+    if (m.genTried) return m.gen;
+    m.genTried = true;
+    if (m.numCoeffs > GEN_MAX_COEFFS) {
+        m.genErr = "more than " + std::to_string(GEN_MAX_COEFFS) + " coefficients: generic kernels";
+        return nullptr;
+    }
+    mpg::GenSpec spec;
+    spec.bk = (m.variant / 100) % 10;
+    spec.pk = (m.variant / 10) % 10;
+    spec.qk = m.variant % 10;
+    spec.pm = std::max(1, m.dev.pmax);
+    spec.D = m.inDim;
+    spec.numCoeffs = m.numCoeffs;
+    spec.dimDeg.assign(m.inDim, 0);
+    for (auto& c : m.comps)
+        for (unsigned d = 0; d + 1 < c.dim; ++d) spec.dimDeg[d] = std::max<int>(spec.dimDeg[d], (int)c.maxDeg[d]);
+    for (size_t c = 0; c < m.comps.size(); ++c) {
+        const MultiSet& s = m.comps[c];
+        mpg::GenComp g;
+        g.dim = s.dim;
+        g.coeffOff = m.coeffOff[c];
+        g.p = s.maxDeg[s.dim - 1];
+        g.starts = s.starts;
+        g.dims = s.dims;
+        g.orders = s.orders;
+        spec.comps.push_back(std::move(g));
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    m.gen = mpg::get_module(spec, m.genErr);
+    m.genSeconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return m.gen;
+}
+
+// Coefficients as a contiguous vector (the specialized kernels index w[i]).
+const double* gen_coeffs(mpart_map_s& m, hipStream_t s) {
+    if (m.coeffStride == 1) return m.coeffs;
+    m.wCoeffs.ensure(std::max<size_t>(8, m.numCoeffs * sizeof(double)));
+    hip_check(mpd::launch_copy(m.coeffs, 1, m.numCoeffs, 0, m.coeffStride, m.wCoeffs.as<double>(), 0, 1, s),
+              "launch copy");
+    return m.wCoeffs.as<double>();
+}
+
+void gen_launch(hipFunction_t f, long grid, int block, size_t smem, hipStream_t s, void** args, const char* what) {
+    hip_check(hipModuleLaunchKernel(f, (unsigned)grid, 1, 1, (unsigned)block, 1, 1, (unsigned)smem, s, args, nullptr),
+              what);
+}
+
+// Evaluate / LogDeterminant through the specialized module for components [cb, ce).
+void gen_eval(mpart_map_s& m, mpg::GenModule* g, const double* w, const mpart_cview& p, double* out, long ors,
+              long ocs, double* ld, long lds, int ldAcc, int32_t* nevals, int cb, int ce, hipStream_t s) {
+    MapDev M = m.dev;
+    const double* pts = p.ptr;
+    long prs = p.row_stride, pcs = p.col_stride, N = p.cols;
+    int* ne = nevals;
+    void* args[] = {&M, &w, &pts, &prs, &pcs, &N, &out, &ors, &ocs, &ld, &lds, &ldAcc, &ne, &cb, &ce};
+    gen_launch(g->eval, (N + 255) / 256, 256, 0, s, args, "launch specialized evaluate");
+}
+
 void do_evaluate(mpart_map_s& m, const mpart_cview& p, const mpart_view& o, int32_t* nevals, hipStream_t s) {
     check_coeffs(&m);
     check_view("Evaluate: pts", p.rows, p.cols, m.inDim, p.cols);
     check_view("Evaluate: output", o.rows, o.cols, m.outDim, p.cols);
     if (p.cols == 0) return;
+    if (mpg::GenModule* g = gen_for(m)) {
+        const double* w = gen_coeffs(m, s);
+        if (m.plain) {
+            for (int c = 0; c < m.dev.ncomp; ++c)
+                gen_eval(m, g, w, p, o.ptr, o.row_stride, o.col_stride, nullptr, 0, 0, nevals, c, c + 1, s);
+        } else {
+            gen_eval(m, g, w, p, o.ptr, o.row_stride, o.col_stride, nullptr, 0, 0, nevals, 0, m.dev.ncomp, s);
+        }
+        return;
+    }
     Shape sh = shape_for(m, p.cols, s);
     if (m.plain) {
         for (int c = 0; c < m.dev.ncomp; ++c)
@@ -496,6 +590,16 @@ void do_logdet(mpart_map_s& m, const mpart_cview& p, double* out, long os, hipSt
     check_view("LogDeterminant: pts", p.rows, p.cols, m.inDim, p.cols);
     if (!m.opts.contDeriv) throw NotImpl("LogDeterminant with contDeriv = false is not provided yet");
     if (p.cols == 0) return;
+    if (mpg::GenModule* g = gen_for(m)) {
+        const double* w = gen_coeffs(m, s);
+        if (m.plain) {
+            for (int c = 0; c < m.dev.ncomp; ++c)
+                gen_eval(m, g, w, p, nullptr, 0, 0, out, os, c > 0 ? 1 : 0, nullptr, c, c + 1, s);
+        } else {
+            gen_eval(m, g, w, p, nullptr, 0, 0, out, os, 0, nullptr, 0, m.dev.ncomp, s);
+        }
+        return;
+    }
     Shape sh = shape_for(m, p.cols, s);
     if (m.plain) {
         for (int c = 0; c < m.dev.ncomp; ++c)
@@ -517,6 +621,21 @@ void do_coeff_grad(mpart_map_s& m, const mpart_cview& p, const mpart_cview& sens
     check_view("CoeffGrad: sens", sens.rows, sens.cols, m.outDim, p.cols);
     check_view("CoeffGrad: output", o.rows, o.cols, m.numCoeffs, p.cols);
     if (p.cols == 0) return;
+    if (mpg::GenModule* g = gen_for(m)) {
+        const double* w = gen_coeffs(m, s);
+        MapDev M = m.dev;
+        const double* pts = p.ptr;
+        long prs = p.row_stride, pcs = p.col_stride, N = p.cols;
+        const double* sp = sens.ptr;
+        long srs = sens.row_stride, scs = sens.col_stride;
+        double* op = o.ptr;
+        long ors = o.row_stride, ocs = o.col_stride;
+        for (int c = 0; c < m.dev.ncomp; ++c) {
+            void* args[] = {&M, &w, &pts, &prs, &pcs, &N, &sp, &srs, &scs, &op, &ors, &ocs, &c};
+            gen_launch(g->coef, (N + 255) / 256, 256, 0, s, args, "launch specialized coeff_grad");
+        }
+        return;
+    }
     Shape sh = shape_for(m, p.cols, s);
     for (int c = 0; c < m.dev.ncomp; ++c)
         hip_check(mpd::launch_coeff_grad(m.variant, sh.L, m.dev, m.coeffs, m.coeffStride, p.ptr, p.row_stride,
@@ -531,6 +650,19 @@ void do_logdet_coeff_grad(mpart_map_s& m, const mpart_cview& p, const mpart_view
     check_view("LogDeterminantCoeffGrad: output", o.rows, o.cols, m.numCoeffs, p.cols);
     if (!m.opts.contDeriv) throw NotImpl("LogDeterminantCoeffGrad with contDeriv = false is not provided yet");
     if (p.cols == 0) return;
+    if (mpg::GenModule* g = gen_for(m)) {
+        const double* w = gen_coeffs(m, s);
+        MapDev M = m.dev;
+        const double* pts = p.ptr;
+        long prs = p.row_stride, pcs = p.col_stride, N = p.cols;
+        double* op = o.ptr;
+        long ors = o.row_stride, ocs = o.col_stride;
+        for (int c = 0; c < m.dev.ncomp; ++c) {
+            void* args[] = {&M, &w, &pts, &prs, &pcs, &N, &op, &ors, &ocs, &c};
+            gen_launch(g->ldcoef, (N + 255) / 256, 256, 0, s, args, "launch specialized logdet_coeff_grad");
+        }
+        return;
+    }
     Shape sh = shape_for(m, p.cols, s);
     for (int c = 0; c < m.dev.ncomp; ++c)
         hip_check(mpd::launch_logdet_coeff_grad(m.variant, sh.L, m.dev, m.coeffs, m.coeffStride, p.ptr, p.row_stride,
@@ -610,6 +742,48 @@ void do_fused(mpart_map_s& m, const mpart_cview& p, const double* sens, long srs
     check_coeffs(&m);
     const long N = p.cols;
     const long K = m.numCoeffs;
+    if (g_numCU == 0) {
+        int dev = 0;
+        hip_check(hipGetDevice(&dev), "hipGetDevice");
+        hip_check(hipDeviceGetAttribute(&g_numCU, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
+    }
+    if (mpg::GenModule* g = gen_for(m)) {
+        // Per-wave LDS accumulators of K doubles; K <= 4096 keeps them within 64 KiB at 128 lanes.
+        int gblock = 256;
+        while (gblock > 64 && (size_t)(gblock / 64) * K * sizeof(double) > LDS_LIMIT) gblock /= 2;
+        const size_t gsmem = (size_t)(gblock / 64) * K * sizeof(double);
+        if (m.genOcc == 0 || m.genOccSmem != gsmem) {
+            int nb = 0;
+            hip_check(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nb, g->fused, gblock, gsmem),
+                      "hipModuleOccupancyMaxActiveBlocksPerMultiprocessor");
+            m.genOcc = std::max(1, nb);
+            m.genOccSmem = gsmem;
+        }
+        const long ntiles = std::max<long>(1, (N + gblock - 1) / gblock);
+        const long grid = std::min<long>(ntiles, (long)g_numCU * m.genOcc);
+        m.partialGrad.ensure(std::max<size_t>(8, (size_t)grid * K * sizeof(double)));
+        m.partialLoss.ensure(std::max<size_t>(8, (size_t)grid * sizeof(double)));
+        if (N > 0) {
+            const double* w = gen_coeffs(m, s);
+            MapDev M = m.dev;
+            const double* pts = p.ptr;
+            long prs = p.row_stride, pcs = p.col_stride, NN = N;
+            const double* sp = sens;
+            long srs_ = srs, scs_ = scs;
+            const double* lp = lsens;
+            long lss_ = lss;
+            int kl = klMode;
+            double* pg = m.partialGrad.as<double>();
+            double* pl = m.partialLoss.as<double>();
+            void* args[] = {&M, &w, &pts, &prs, &pcs, &NN, &sp, &srs_, &scs_, &lp, &lss_, &kl, &pg, &pl};
+            gen_launch(g->fused, grid, gblock, gsmem, s, args, "launch specialized fused");
+        }
+        const long rows = (N > 0) ? grid : 0;
+        hip_check(mpd::launch_reduce_rows(m.partialGrad.as<double>(), rows, K, scale, grad, s), "launch reduce");
+        if (loss)
+            hip_check(mpd::launch_reduce_rows(m.partialLoss.as<double>(), rows, 1, scale, loss, s), "launch reduce");
+        return;
+    }
     int block = 256;
     size_t tabLane = (size_t)m.dev.nslots * sizeof(double);
     auto smem_for = [&](int b, bool ldsAcc) { return (ldsAcc ? (size_t)(b / 64) * K * sizeof(double) : 0) + tabLane * b; };
@@ -621,11 +795,6 @@ void do_fused(mpart_map_s& m, const mpart_cview& p, const double* sens, long srs
     const bool tabLds = smem_for(block, ldsAcc) <= LDS_LIMIT;
     const size_t smem = tabLds ? smem_for(block, ldsAcc) : (ldsAcc ? (size_t)(block / 64) * K * sizeof(double) : 0);
     const long ntiles = std::max<long>(1, (N + block - 1) / block);
-    if (g_numCU == 0) {
-        int dev = 0;
-        hip_check(hipGetDevice(&dev), "hipGetDevice");
-        hip_check(hipDeviceGetAttribute(&g_numCU, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
-    }
     const int occ = mpd::occupancy_fused(m.variant, block, smem);
     const long grid = std::min<long>(ntiles, (long)g_numCU * occ);
     mpd::Launch L{dim3((unsigned)grid), dim3(block), smem, s};
@@ -941,5 +1110,24 @@ int mpart_set_plain_launches(mpart_map m, int plain) {
         m->plain = plain != 0;
     });
 }
+
+int mpart_set_codegen(mpart_map m, int enable) {
+    return guarded([&] {
+        check_map(m);
+        m->genEnabled = enable != 0;
+    });
+}
+
+int mpart_codegen_prepare(mpart_map m, int32_t* specialized, double* compileSeconds) {
+    return guarded([&] {
+        check_map(m);
+        const bool fresh = !m->genTried;
+        mpg::GenModule* g = gen_for(*m);
+        if (specialized) *specialized = g ? 1 : 0;
+        if (compileSeconds) *compileSeconds = (g && fresh) ? m->genSeconds : 0.0;
+    });
+}
+
+const char* mpart_codegen_error(mpart_map m) { return m ? m->genErr.c_str() : "null map handle"; }
 
 }  // extern "C"
