@@ -27,21 +27,24 @@ std::string term_product(const GenComp& c, unsigned t, unsigned* q) {
             *q = c.orders[i];
             continue;
         }
-        std::string f = "b" + std::to_string(c.dims[i]) + "[" + std::to_string(c.orders[i]) + "]";
+        std::string f = "b" + std::to_string(c.dims[i]) + "_" + std::to_string(c.orders[i]);
         expr = expr.empty() ? f : "(" + expr + " * " + f + ")";
     }
     return expr.empty() ? std::string("1.0") : expr;
 }
 
+// Off-diagonal basis values of the point as scalars b<d>_<k> (zero for an
+// inactive lane).
 void emit_basis(std::ostringstream& o, const GenSpec& s, unsigned ndims) {
     for (unsigned d = 0; d < ndims && d < s.dimDeg.size(); ++d) {
         const int deg = s.dimDeg[d];
         if (deg == 0) continue;
-        o << "    double b" << d << "[" << deg + 1 << "];\n";
-        o << "    basis_vals<GBK, " << deg << ">(M, active ? pt[" << d << " * prs] : 0.0, b" << d << ");\n";
-        o << "    if (!active) { ";
-        for (int k = 0; k <= deg; ++k) o << "b" << d << "[" << k << "] = 0.0; ";
-        o << "}\n";
+        o << "    double b" << d << "_1";
+        for (int k = 2; k <= deg; ++k) o << ", b" << d << "_" << k;
+        o << ";\n    {\n        double t[" << deg + 1 << "];\n";
+        o << "        basis_vals<GBK, " << deg << ">(M, active ? pt[" << d << " * prs] : 0.0, t);\n";
+        for (int k = 1; k <= deg; ++k) o << "        b" << d << "_" << k << " = active ? t[" << k << "] : 0.0;\n";
+        o << "    }\n";
     }
 }
 
@@ -54,7 +57,7 @@ void emit_basis(std::ostringstream& o, const GenSpec& s, unsigned ndims) {
 void emit_pin(std::ostringstream& o, const GenSpec& s, unsigned ndims, const char* indent) {
     for (unsigned d = 0; d < ndims && d < s.dimDeg.size(); ++d)
         for (int k = 1; k <= s.dimDeg[d]; ++k)
-            o << indent << "asm volatile(\"\" : \"+v\"(b" << d << "[" << k << "]));\n";
+            o << indent << "asm volatile(\"\" : \"+v\"(b" << d << "_" << k << "));\n";
 }
 
 void emit_coeff_sums(std::ostringstream& o, const GenComp& c) {
@@ -84,7 +87,7 @@ std::string generate_source(const GenSpec& s) {
     o << R"(
 extern "C" __global__ void __launch_bounds__(256) GFUSED_ATTR mpart_gen_fused(MapDev M, const double* __restrict__ w,
         const double* __restrict__ pts, long prs, long pcs, long N, const double* sens, long srs, long scs,
-        const double* lsens, long lss, int klMode, double* partialGrad, double* partialLoss) {
+        const double* lsens, long lss, int klMode, double* partial) {
     extern __shared__ double acc[];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -148,15 +151,17 @@ extern "C" __global__ void __launch_bounds__(256) GFUSED_ATTR mpart_gen_fused(Ma
     __shared__ double lossWave[4];
     if (lane == 0) lossWave[wave] = lw;
     __syncthreads();
+    // Partial row of the block: K gradient sums, then the loss sum.
+    double* row = partial + (long)blockIdx.x * (K + 1);
     if (tid == 0) {
         double l = 0.0;
         for (int i = 0; i < nwaves; ++i) l += lossWave[i];
-        partialLoss[blockIdx.x] = l;
+        row[K] = l;
     }
     for (int k = tid; k < K; k += blockDim.x) {
         double t = 0.0;
         for (int i = 0; i < nwaves; ++i) t += acc[(long)i * K + k];
-        partialGrad[(long)blockIdx.x * K + k] = t;
+        row[k] = t;
     }
 }
 )";

@@ -17,7 +17,13 @@ hipError_t launch_eval(int var, const Launch& L, const MapDev& M, const double* 
 }
 
 hipError_t launch_reduce_rows(const double* in, long nrows, long K, double scale, double* out, hipStream_t s) {
-    hipLaunchKernelGGL(k_reduce_rows, dim3((unsigned)((K + 31) / 32)), dim3(256), 0, s, in, nrows, K, scale, out);
+    return launch_reduce_cols(in, nrows, K, K, scale, out, nullptr, s);
+}
+hipError_t launch_reduce_cols(const double* in, long nrows, long rowStride, long ncols, double scale, double* out,
+                              double* outLast, hipStream_t s) {
+    if (ncols <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_reduce_cols, dim3((unsigned)((ncols + 7) / 8)), dim3(256), 0, s, in, nrows, rowStride, ncols,
+                       scale, out, outLast);
     return hipGetLastError();
 }
 hipError_t launch_kl_rows(const double* G, const double* LG, long N, long K, double* grad, hipStream_t s) {

@@ -38,6 +38,8 @@ hipError_t launch_fused(int var, const Launch& L, const MapDev& M, const double*
 int occupancy_fused(int var, int block, size_t smem);
 
 hipError_t launch_reduce_rows(const double* in, long nrows, long K, double scale, double* out, hipStream_t s);
+hipError_t launch_reduce_cols(const double* in, long nrows, long rowStride, long ncols, double scale, double* out,
+                              double* outLast, hipStream_t s);
 hipError_t launch_kl_rows(const double* G, const double* LG, long N, long K, double* grad, hipStream_t s);
 hipError_t launch_kl_loss(const double* T, long M, const double* ld, long N, double* partial, hipStream_t s);
 hipError_t launch_fill(double* p, long rows, long cols, long rs, long cs, double v, hipStream_t s);

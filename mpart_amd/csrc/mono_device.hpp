@@ -787,11 +787,19 @@ MPD_HD void comp_core(const MapDev& M, const CompDev& cd, const double* C, doubl
     }
 }
 // ---------------------------------------------------------------------------
-// Wavefront reduction of 16 per-lane values into per-row partial sums.
-// After the four exchange stages lane l holds, in v[0], the sum over its row
-// of 16 lanes of the values for term (l & 15).  Partners are l^8 (row_ror:8),
-// l^7 (row_half_mirror), l^3 (quad_perm 3,2,1,0) and l^1 (quad_perm 1,0,3,2):
-// each flips the deciding bit and leaves the higher bits unchanged.
+// Wavefront reduction of 16 per-lane values v[0..15] (one per term of a
+// chunk) into the 16 sums over the 64 lanes.
+//
+// Stages, each halving the number of values a lane holds and doubling the
+// lanes each value covers:
+//   A  v_permlane32_swap  (lanes l, l^32):   16 -> 8 values
+//   B  v_permlane16_swap  (lanes l, l^16):    8 -> 4
+//   C  DPP row_ror:8      (lanes l, l^8):     4 -> 2
+//   D  DPP row_half_mirror (lanes l, l^7):    2 -> 1
+//   E  DPP quad_perm      (lanes l^1, l^2):   sum of the remaining value
+// The swaps exchange whole register halves between two values, so A and B
+// need one add per value pair and no lane selects.  Afterwards every lane
+// holds the total of term (lane >> 2) & 15.
 template <int CTRL>
 __device__ __forceinline__ double dpp_mov(double v) {
     int lo = __double2loint(v);
@@ -801,31 +809,46 @@ __device__ __forceinline__ double dpp_mov(double v) {
     return __hiloint2double(hi, lo);
 }
 
-template <int H, int CTRL>
-__device__ __forceinline__ void bfly_stage(double* v, bool upper) {
-#pragma unroll
-    for (int i = 0; i < H; ++i) {
-        const double keep = upper ? v[H + i] : v[i];
-        const double send = upper ? v[i] : v[H + i];
-        v[i] = keep + dpp_mov<CTRL>(send);
-    }
+// a + b after exchanging lanes 32..63 of a with lanes 0..31 of b.
+__device__ __forceinline__ double swap32_add(double a, double b) {
+    const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)__double2loint(a), (unsigned)__double2loint(b), false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)__double2hiint(a), (unsigned)__double2hiint(b), false, false);
+    return __hiloint2double((int)hi[0], (int)lo[0]) + __hiloint2double((int)hi[1], (int)lo[1]);
 }
 
-__device__ __forceinline__ double row_reduce16(double* v, int lane) {
-    bfly_stage<8, 0x128>(v, (lane & 8) != 0);
-    bfly_stage<4, 0x141>(v, (lane & 4) != 0);
-    bfly_stage<2, 0x1B>(v, (lane & 2) != 0);
-    bfly_stage<1, 0xB1>(v, (lane & 1) != 0);
-    return v[0];
+// a + b after exchanging the odd 16-lane rows of a with the even rows of b.
+__device__ __forceinline__ double swap16_add(double a, double b) {
+    const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)__double2loint(a), (unsigned)__double2loint(b), false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)__double2hiint(a), (unsigned)__double2hiint(b), false, false);
+    return __hiloint2double((int)hi[0], (int)lo[0]) + __hiloint2double((int)hi[1], (int)lo[1]);
 }
 
-// Reduces 16 per-lane values over the wavefront and adds the 16 sums to
-// dst[0..nvalid) (lane i < nvalid adds the sum for value i).
+// The intermediate values are named scalars: selects between elements of a
+// local array can be turned into address selects, which forces the array
+// into scratch memory.
+__device__ __forceinline__ double wave_reduce16(const double* v, int lane) {
+    const double w0 = swap32_add(v[0], v[8]), w1 = swap32_add(v[1], v[9]);  // term i + 8*b5
+    const double w2 = swap32_add(v[2], v[10]), w3 = swap32_add(v[3], v[11]);
+    const double w4 = swap32_add(v[4], v[12]), w5 = swap32_add(v[5], v[13]);
+    const double w6 = swap32_add(v[6], v[14]), w7 = swap32_add(v[7], v[15]);
+    const double x0 = swap16_add(w0, w4), x1 = swap16_add(w1, w5);  // + 4*b4
+    const double x2 = swap16_add(w2, w6), x3 = swap16_add(w3, w7);
+    const bool b3 = (lane & 8) != 0;  // + 2*b3
+    const double y0 = (b3 ? x2 : x0) + dpp_mov<0x128>(b3 ? x0 : x2);
+    const double y1 = (b3 ? x3 : x1) + dpp_mov<0x128>(b3 ? x1 : x3);
+    const bool b2 = (lane & 4) != 0;  // + b2
+    double r = (b2 ? y1 : y0) + dpp_mov<0x141>(b2 ? y0 : y1);
+    r += dpp_mov<0xB1>(r);
+    r += dpp_mov<0x4E>(r);
+    return r;
+}
+
+// Reduces 16 per-lane values over the wavefront and adds sum i to dst[i]
+// for i < nvalid.
 __device__ __forceinline__ void reduce16_add(double* v, int lane, double* dst, int nvalid) {
-    double r = row_reduce16(v, lane);
-    r += __shfl_xor(r, 16, 64);
-    r += __shfl_xor(r, 32, 64);
-    if (lane < nvalid) dst[lane] += r;
+    const double r = wave_reduce16(v, lane);
+    const int u = lane >> 2;
+    if ((lane & 3) == 0 && u < nvalid) dst[u] += r;
 }
 
 __device__ __forceinline__ double wave_sum(double v) {

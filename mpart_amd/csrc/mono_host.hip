@@ -761,8 +761,7 @@ void do_fused(mpart_map_s& m, const mpart_cview& p, const double* sens, long srs
         }
         const long ntiles = std::max<long>(1, (N + gblock - 1) / gblock);
         const long grid = std::min<long>(ntiles, (long)g_numCU * m.genOcc);
-        m.partialGrad.ensure(std::max<size_t>(8, (size_t)grid * K * sizeof(double)));
-        m.partialLoss.ensure(std::max<size_t>(8, (size_t)grid * sizeof(double)));
+        m.partialGrad.ensure(std::max<size_t>(8, (size_t)grid * (K + 1) * sizeof(double)));
         if (N > 0) {
             const double* w = gen_coeffs(m, s);
             MapDev M = m.dev;
@@ -773,15 +772,15 @@ void do_fused(mpart_map_s& m, const mpart_cview& p, const double* sens, long srs
             const double* lp = lsens;
             long lss_ = lss;
             int kl = klMode;
-            double* pg = m.partialGrad.as<double>();
-            double* pl = m.partialLoss.as<double>();
-            void* args[] = {&M, &w, &pts, &prs, &pcs, &NN, &sp, &srs_, &scs_, &lp, &lss_, &kl, &pg, &pl};
+            double* part = m.partialGrad.as<double>();
+            void* args[] = {&M, &w, &pts, &prs, &pcs, &NN, &sp, &srs_, &scs_, &lp, &lss_, &kl, &part};
             gen_launch(g->fused, grid, gblock, gsmem, s, args, "launch specialized fused");
         }
+        // One pass over the partial rows: K gradient columns, then the loss column.
         const long rows = (N > 0) ? grid : 0;
-        hip_check(mpd::launch_reduce_rows(m.partialGrad.as<double>(), rows, K, scale, grad, s), "launch reduce");
-        if (loss)
-            hip_check(mpd::launch_reduce_rows(m.partialLoss.as<double>(), rows, 1, scale, loss, s), "launch reduce");
+        hip_check(mpd::launch_reduce_cols(m.partialGrad.as<double>(), rows, K + 1, loss ? K + 1 : K, scale, grad, loss,
+                                          s),
+                  "launch reduce");
         return;
     }
     int block = 256;

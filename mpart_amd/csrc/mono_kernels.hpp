@@ -385,8 +385,8 @@ __global__ void __launch_bounds__(256) k_inverse(MapDev M, const double* w, long
 // For each point n and component c it computes T_c, log g(d_c f), the
 // adjoint integrals S'_q on the forward tree, and
 //   G_q = s_c (phi~_q(0) + rho_q S'_q) + l_n rho_q B'_q(x_c) g'/g,
-// then reduces sum_n P_j(x_n) G_q(x_n) over the wavefront (row butterfly +
-// two cross-row shuffles) into per-wave LDS accumulators.  klMode uses
+// then reduces sum_n P_j(x_n) G_q(x_n) over the wavefront (wave_reduce16)
+// into per-wave LDS accumulators.  klMode uses
 // s_c = T_c and l_n = -1 (gradient of mean(0.5 |T|^2 - logdet) up to 1/N) and
 // also sums 0.5 |T|^2 - logdet.  Blocks loop over point tiles (persistent
 // grid) and write one partial gradient row per block.
@@ -454,11 +454,9 @@ __global__ void __launch_bounds__(256) k_fused(MapDev M, const double* w, long w
                         for (int e = 2; e <= M.nd; ++e) Pj *= tab[(long)rec[e] * stride];
                         v[u] = active ? Pj * Gq : 0.0;
                     }
-                    double rsum = row_reduce16(v, lane);
-                    rsum += __shfl_xor(rsum, 16, 64);
-                    rsum += __shfl_xor(rsum, 32, 64);
-                    if (lane < 16) {
-                        const uint32_t idx = M.terms[(long)(j0 + lane) * M.trec];
+                    const double rsum = wave_reduce16(v, lane);
+                    if ((lane & 3) == 0) {
+                        const uint32_t idx = M.terms[(long)(j0 + (lane >> 2)) * M.trec];
                         if (idx != DUMMY) {
                             if (ldsAcc) myAcc[cd.coeffOff + idx] += rsum;
                             else atomicAdd(gradGlobal + cd.coeffOff + idx, rsum);
@@ -488,20 +486,26 @@ __global__ void __launch_bounds__(256) k_fused(MapDev M, const double* w, long w
 }
 
 #ifdef MPD_UTILITY_KERNELS
-// Column sums of a (nrows x K) row-major block of partials: out[k] = scale * sum_r in[r*K + k].
-__global__ void __launch_bounds__(256) k_reduce_rows(const double* in, long nrows, long K, double scale, double* out) {
-    __shared__ double red[8][33];
-    const int kk = threadIdx.x & 31, rr = threadIdx.x >> 5;
-    const long k = (long)blockIdx.x * 32 + kk;
+// Column sums of a block of per-block partial rows: for c < ncols,
+// out[c] = scale * sum_r in[r * rowStride + c], except that the last column
+// goes to *outLast when outLast is given.  Eight columns per block, 32 lanes
+// over the rows, then a fixed-order sum of the 32 lane partials (the result
+// does not depend on the launch).
+__global__ void __launch_bounds__(256) k_reduce_cols(const double* in, long nrows, long rowStride, long ncols,
+                                                     double scale, double* out, double* outLast) {
+    __shared__ double red[32][9];
+    const int cc = threadIdx.x & 7, rl = threadIdx.x >> 3;
+    const long c = (long)blockIdx.x * 8 + cc;
     double s = 0.0;
-    if (k < K)
-        for (long r = rr; r < nrows; r += 8) s += in[r * K + k];
-    red[rr][kk] = s;
+    if (c < ncols)
+        for (long r = rl; r < nrows; r += 32) s += in[r * rowStride + c];
+    red[rl][cc] = s;
     __syncthreads();
-    if (rr == 0 && k < K) {
+    if (rl == 0 && c < ncols) {
         double t = 0.0;
-        for (int i = 0; i < 8; ++i) t += red[i][kk];
-        out[k] = scale * t;
+        for (int i = 0; i < 32; ++i) t += red[i][cc];
+        if (outLast && c == ncols - 1) *outLast = scale * t;
+        else out[c] = scale * t;
     }
 }
 
