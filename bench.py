@@ -1,23 +1,26 @@
 """Benchmark of the headline workload: fused forward+adjoint KL gradient.
 
-Workload (config C3): 8-D triangular map, total order 3 per component
-(494 coefficients), ProbabilistHermite / SoftPlus / adaptive Simpson
-(MapOptions defaults), N = 500,000 FP64 points from the banana recursion
-x1 ~ N(0,1), x_k = 0.5 x_{k-1}^2 + tanh(x_{k-2}) + 0.3 N(0,1) (seed 0), at
-the factory's initial coefficients (zeros).  The recursion explodes (x8 reaches
-6e48), so at any nonzero coefficient vector SoftPlus underflows for some
-points and the objective is +inf; the zero start is the configuration's only
+Workload: 8-D triangular map, total order 3 per component (494 coefficients),
+ProbabilistHermite / SoftPlus / adaptive Simpson (MapOptions defaults),
+N = 500,000 FP64 points from the banana recursion
+x1 ~ N(0,1), x_k = 0.5 x_{k-1}^2 + tanh(x_{k-2}) + 0.3 N(0,1) (x_0 := 0, seed 0),
+at the factory's initial coefficients (zeros).  The recursion explodes (x8
+reaches 6e48), so at any nonzero coefficient vector SoftPlus underflows for
+some points and the objective is +inf; the zero start is the workload's only
 finite point.  A second workload, reported beside it, uses the same map with
 x ~ N(0, I_8) and coefficients 0.1 N(0,1) (seed 1), where the adaptive
 quadrature subdivides.
 
 A step is one evaluation of loss = mean(0.5 |T|^2 - logdet) and its
-coefficient gradient with inputs resident in HBM.  "fused" is one launch over
-all components (mpart_kl_value_and_grad); "plain" is the reference's structure
+coefficient gradient with inputs resident in HBM and outputs preallocated.
+"fused" is one launch over all components plus one column reduction
+(mpart_kl_value_and_grad); "plain" is the reference's structure
 (per-component Evaluate / LogDeterminant / CoeffGrad / LogDeterminantCoeffGrad
-launches with stored Jacobians, then a row reduction).
+launches with stored Jacobians, then row reductions).
 
-Prints one JSON line.  --gpus must be 1.
+Prints one JSON line.  --gpus must be 1.  --profile runs only the headline
+fused steps followed by a 256 MiB elementwise copy (the calibration kernel
+for rocprofv3 FETCH_SIZE / WRITE_SIZE), for use under rocprofv3.
 """
 import argparse
 import json
@@ -32,67 +35,72 @@ sys.path.insert(0, ROOT)
 
 METRIC = "KL fwd+adjoint points/s, fused batched vs plain GPU path @1e-10 rel err"
 PEAK_FP64_TFLOPS = 78.6
-PEAK_HBM_GBS = 8000.0
 D, ORDER = 8, 3
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "kl_fused_c3_banana_traffic.json")
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def measure(tri, x, steps, warmup, plain_steps, do_plain):
-    """Times the fused step (wall clock around `steps` launches, bracketed by
-    synchronizations, plus HIP events on the launch stream) and the plain path."""
+def timed_fused(tri, x, steps, warmup):
+    """Wall clock around `steps` fused launches (bracketed by synchronizations)
+    and HIP events around each launch on the launch stream."""
     import torch
 
-    from mpart_amd.roofline import fused_flops
-
     stream = torch.cuda.current_stream()
-    N = x.shape[1]
+    loss = torch.zeros(1, dtype=torch.float64, device="cuda")
+    grad = torch.zeros(tri.numCoeffs, dtype=torch.float64, device="cuda")
     for _ in range(warmup):
-        tri.KLValueAndGrad(x)
-    torch.cuda.synchronize()
+        tri.KLValueAndGradImpl(x, loss, grad)
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(steps):
         ev[i][0].record(stream)
-        loss, grad = tri.KLValueAndGrad(x)
+        tri.KLValueAndGradImpl(x, loss, grad)
         ev[i][1].record(stream)
     torch.cuda.synchronize()
     ms_step = (time.perf_counter() - t0) * 1e3 / steps
     ev_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
-    fl = float(loss.item())
-    fg = grad.cpu().numpy()
-    log(f"fused {ms_step:.3f} ms/step (events {ev_ms:.3f} ms), loss {fl:.6g}")
+    return ms_step, ev_ms, float(loss.item()), grad.cpu().numpy()
 
+
+def timed_plain(tri, x, steps):
+    import torch
+
+    loss = torch.zeros(1, dtype=torch.float64, device="cuda")
+    grad = torch.zeros(tri.numCoeffs, dtype=torch.float64, device="cuda")
+    tri.KLValueAndGradImpl(x, loss, grad, plain=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        tri.KLValueAndGradImpl(x, loss, grad, plain=True)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps, float(loss.item()), grad.cpu().numpy()
+
+
+def measure(tri, x, steps, warmup, plain_steps, do_plain):
+    from mpart_amd.roofline import fused_flops
+
+    N = x.shape[1]
+    ms_step, ev_ms, fl, fg = timed_fused(tri, x, steps, warmup)
+    log(f"fused {ms_step:.3f} ms/step (events {ev_ms:.3f} ms), loss {fl:.6g}")
     res = {"ms_per_step": ms_step, "event_ms": ev_ms, "points_per_s": N / (ms_step * 1e-3), "loss": fl}
     if do_plain:
-        tri.KLValueAndGrad(x, plain=True)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(plain_steps):
-            lp, gp = tri.KLValueAndGrad(x, plain=True)
-        torch.cuda.synchronize()
-        pms = (time.perf_counter() - t0) * 1e3 / plain_steps
-        gp = gp.cpu().numpy()
+        pms, lp, gp = timed_plain(tri, x, plain_steps)
         res["plain_ms_per_step"] = pms
         res["plain_points_per_s"] = N / (pms * 1e-3)
         res["speedup_fused_vs_plain"] = pms / ms_step
-        res["fused_vs_plain_max_rel_err"] = float(max(abs(float(lp.item()) - fl) / abs(fl),
+        res["fused_vs_plain_max_rel_err"] = float(max(abs(lp - fl) / abs(fl),
                                                       np.max(np.abs(gp - fg)) / np.max(np.abs(gp))))
         log(f"plain {pms:.3f} ms/step")
         # The same fused pass through the precompiled generic kernel.
         tri.SetCodegen(False)
-        tri.KLValueAndGrad(x)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(plain_steps):
-            tri.KLValueAndGrad(x)
-        torch.cuda.synchronize()
-        res["generic_fused_ms_per_step"] = (time.perf_counter() - t0) * 1e3 / plain_steps
+        gms, _, _, _ = timed_fused(tri, x, plain_steps, 1)
         tri.SetCodegen(True)
-        log(f"generic fused {res['generic_fused_ms_per_step']:.3f} ms/step")
+        res["generic_fused_ms_per_step"] = gms
+        log(f"generic fused {gms:.3f} ms/step")
     msets = [tri.GetMultiIndexSet(c) for c in range(tri.NumComponents())]
     _, counts = tri.EvaluateCounted(x)
     counts = counts.cpu().numpy()
@@ -116,6 +124,35 @@ def oracle_check(x_np, coeffs, fused_grad_fn, n=2000):
     return float(max(abs(lg - lr) / abs(lr), np.max(np.abs(gg - gr)) / np.max(np.abs(gr))))
 
 
+def load_traffic():
+    """HBM bytes per fused launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary of this workload (None when absent)."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as f:
+        return json.load(f)
+
+
+def profile_mode(args):
+    """Headline fused steps, then the copy calibration kernel (for rocprofv3)."""
+    import torch
+
+    import mpart_amd as mt
+    from mpart_amd.data import banana_points
+
+    tri = mt.CreateTriangular(D, D, ORDER, mt.MapOptions())
+    tri.PrepareCodegen()
+    x = torch.from_numpy(banana_points(args.n, D, seed=0)).cuda()
+    tri.SetCoeffs(np.zeros(tri.numCoeffs))
+    ms, ev, fl, _ = timed_fused(tri, x, args.steps, args.warmup)
+    a = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.mul(a, 1.0, out=b)  # reads 256 MiB, writes 256 MiB
+    torch.cuda.synchronize()
+    print(json.dumps({"profile": "kl_fused_c3_banana", "ms_per_step": ms, "event_ms": ev, "loss": fl,
+                      "calibration": "torch.mul over 256 MiB (reads 262144 KiB, writes 262144 KiB)"}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -126,13 +163,17 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-second", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--profile", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path would shard points "
-              f"across GPUs with one all-reduce of the gradient; see DESIGN.md).", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: this build runs on a single GPU "
+              f"(the path would shard points across GPUs with one all-reduce of the gradient).", file=sys.stderr)
         sys.exit(2)
+    if args.profile:
+        profile_mode(args)
+        return
 
     import torch
 
@@ -144,12 +185,12 @@ def main():
     spec, compile_s, gen_err = tri.PrepareCodegen()
     log(f"specialized kernels: {spec} ({compile_s:.2f} s) {gen_err}")
 
-    # Headline: literal C3 (banana data, zero coefficients).
+    # Headline: banana data, zero coefficients.
     x_np = banana_points(N, D, seed=0)
     x = torch.from_numpy(x_np).cuda()
     c0 = np.zeros(tri.numCoeffs)
     tri.SetCoeffs(c0)
-    log(f"C3 literal: N={N}, coeffs={tri.numCoeffs}")
+    log(f"headline: N={N}, coeffs={tri.numCoeffs}")
     head, _ = measure(tri, x, args.steps, args.warmup, args.plain_steps, not args.no_plain)
 
     def gpu_kl(xs):
@@ -165,13 +206,14 @@ def main():
         c2 = 0.1 * s.normal(tri.numCoeffs)
         tri.SetCoeffs(c2)
         x2 = torch.from_numpy(x2_np).cuda()
-        log("C3 map, N(0,I) data, 0.1 N(0,1) coefficients")
+        log("same map, N(0,I) data, 0.1 N(0,1) coefficients")
         second, _ = measure(tri, x2, args.steps, args.warmup, args.plain_steps, not args.no_plain)
         second["oracle_max_rel_err_2000pts"] = oracle_check(x2_np, c2, gpu_kl)
         second["data"] = "x ~ N(0, I_8) (seed 1 stream), coefficients 0.1 N(0,1) from the same stream"
         del x2
 
     bytes_alg = 8.0 * D * N + 8.0 * (tri.numCoeffs + 1)
+    traffic = load_traffic()
     out = {
         "metric": METRIC,
         "value": head["points_per_s"],
@@ -185,18 +227,22 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic: banana recursion (seed 0), factory-initial (zero) coefficients",
-        "config": {"workload": "C3 KL fwd+adjoint: 8-D triangular map, total order 3 (494 coeffs), "
-                               "ProbabilistHermite/SoftPlus/AdaptiveSimpson, N=500000",
+        "config": {"workload": "KL fwd+adjoint, 8-D triangular map, total order 3 (494 coefficients), "
+                               "ProbabilistHermite/SoftPlus/AdaptiveSimpson, N=500000 banana points",
                    "N": N, "dim": D, "order": ORDER, "num_coeffs": tri.numCoeffs},
+        "roofline": {"bound": "mfma", "achieved": head["achieved_tflops"], "peak": PEAK_FP64_TFLOPS,
+                     "unit": "TFLOP/s", "frac": head["achieved_tflops"] / PEAK_FP64_TFLOPS,
+                     "traffic": traffic["bytes_per_launch"] if traffic else None,
+                     "note": "compute-bound FP64 on the vector ALUs (no MFMA: nothing is a dense contraction); "
+                             "MI355X FP64 peak 78.6 TFLOP/s; achieved = algorithmic flops of the launch "
+                             "(mpart_amd/roofline.py) / HIP-event time of the fused kernel + its reduction",
+                     "algorithmic_flops_per_launch": head["algorithmic_flops"],
+                     "algorithmic_bytes_per_launch": bytes_alg,
+                     "traffic_source": traffic["source"] if traffic else None},
+        "cpu_baseline": None,
         "fused": head,
         "specialized_kernels": {"active": spec, "compile_s": compile_s, "error": gen_err},
         "active_quadrature_workload": second,
-        "roofline": {"bound": "fp64-valu", "achieved": head["achieved_tflops"], "peak": PEAK_FP64_TFLOPS,
-                     "unit": "TFLOP/s", "frac": head["achieved_tflops"] / PEAK_FP64_TFLOPS, "traffic": None,
-                     "algorithmic_flops_per_launch": head["algorithmic_flops"],
-                     "algorithmic_bytes_per_launch": bytes_alg,
-                     "hbm_GBs": bytes_alg / (head["event_ms"] * 1e-3) / 1e9},
-        "cpu_baseline": None,
     }
     if not args.no_cpu:
         log("cpu baseline")
@@ -205,24 +251,27 @@ def main():
 
 
 def cpu_baseline(x_np, coeffs, budget_s):
-    """The oracle (CPU restatement of the reference path) on a bounded sample."""
+    """The oracle (CPU restatement of the reference path, OpenMP over points)
+    repeated over the whole headline workload until `budget_s` seconds have
+    passed (at least once)."""
     from oracle import oracle as orc
 
     threads = min(16, os.cpu_count() or 1)
     orc.set_threads(threads)
     r = orc.CreateTriangular(D, D, ORDER, orc.options())
     r.SetCoeffs(coeffs)
-    n = 2000
+    xs = np.ascontiguousarray(x_np)
+    reps = 0
     t0 = time.perf_counter()
-    r.KL(np.ascontiguousarray(x_np[:, :n]))
-    dt = time.perf_counter() - t0
-    n2 = int(min(x_np.shape[1], max(n, n * budget_s / max(dt, 1e-3) * 0.8)))
-    sample = np.ascontiguousarray(x_np[:, :n2])
-    t0 = time.perf_counter()
-    r.KL(sample)
-    dt = time.perf_counter() - t0
-    return {"value": n2 / dt, "unit": "points/s", "cores": threads, "kind": "port",
-            "sample": f"first {n2} of the {x_np.shape[1]} C3 points, one KL value+gradient ({dt:.1f} s)"}
+    while True:
+        r.KL(xs)
+        reps += 1
+        dt = time.perf_counter() - t0
+        if dt >= budget_s:
+            break
+    n = xs.shape[1]
+    return {"value": reps * n / dt, "unit": "points/s", "cores": threads, "kind": "port",
+            "sample": f"the whole workload ({n} points), KL value+gradient repeated {reps} times ({dt:.1f} s)"}
 
 
 if __name__ == "__main__":

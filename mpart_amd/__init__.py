@@ -423,6 +423,11 @@ class ConditionalMap:
             return out.cpu().numpy()
         return out
 
+    def KLValueAndGradImpl(self, pts, loss, grad, plain=False):
+        """KLValueAndGrad into preallocated device outputs (loss: 1 double, grad: numCoeffs)."""
+        fn = _lib.mpart_kl_value_and_grad_plain if plain else _lib.mpart_kl_value_and_grad
+        _check(fn(self._h, _view(pts), loss.data_ptr(), grad.data_ptr(), _stream()))
+
     def KLValueAndGrad(self, pts, plain=False):
         """Fused forward + adjoint of mean(0.5 |T|^2 - logdet) and its coefficient gradient."""
         torch = _torch()
