@@ -103,6 +103,19 @@ struct PosF {
     double g, gp;
 };
 
+// 1/d for d in [1, 2]: hardware reciprocal estimate refined by two Newton
+// steps (within an ulp of the correctly rounded quotient; no scaling needed
+// on this range).
+MPD_HD double recip_1_2(double d) {
+#ifdef __HIP_DEVICE_COMPILE__
+    double r = __builtin_amdgcn_rcp(d);
+    r = fma(r, fma(-d, r, 1.0), r);
+    return fma(r, fma(-d, r, 1.0), r);
+#else
+    return 1.0 / d;
+#endif
+}
+
 template <int PK>
 MPD_HD PosF<PK> posfunc(double z) {
     PosF<PK> r;
@@ -110,10 +123,13 @@ MPD_HD PosF<PK> posfunc(double z) {
         r.g = exp(z);
         r.gp = r.g;
     } else {
-        // log(1 + exp(-|z|)) + max(z, 0); derivative shares e = exp(-|z|).
+        // log(1 + exp(-|z|)) + max(z, 0); derivative shares e = exp(-|z|):
+        // g' = e / (1 + e) for z < 0, 1 / (1 + e) otherwise.  g' only enters
+        // derivatives, so its quotient uses the refined reciprocal.
         double e = exp(-fabs(z));
         r.g = log(1.0 + e) + fmax(z, 0.0);
-        r.gp = (z < 0) ? e / (e + 1.0) : 1.0 / (1.0 + e);
+        const double q = recip_1_2(1.0 + e);
+        r.gp = (z < 0) ? e * q : q;
     }
     return r;
 }
@@ -568,11 +584,12 @@ struct AsRec {
 // AdaptiveSimpson::Integrate (Quadrature.h:553-676): depth-first, left first,
 // accepted intervals add intFine in left-to-right order.
 template <int BK, int PK, int FL, int PM>
-__host__ __device__ double integrate_as(const QuadDev& Q, const DiagCoef<PM>& dc, AdjAcc<PM>& A, int* nevals) {
+__host__ __device__ double integrate_as(const QuadDev& Q, const DiagCoef<PM>& dc, AdjAcc<PM>& A, int* nevals,
+                                        const NodeV<FL>* at1) {
     AsRec<FL> stk[MAXSUB];
     int sp = 0;
     NodeV<FL> fa = eval_node<BK, PK, FL, PM>(dc, 0.0);
-    NodeV<FL> fb = eval_node<BK, PK, FL, PM>(dc, 1.0);
+    NodeV<FL> fb = at1 ? *at1 : eval_node<BK, PK, FL, PM>(dc, 1.0);
     NodeV<FL> fm = eval_node<BK, PK, FL, PM>(dc, 0.5);
     int ne = 3;
     double a = 0.0, b = 1.0, res = 0.0;
@@ -636,7 +653,8 @@ struct AccRec {
 // rule size when known at compile time (5 for quadPts = 5), or MAXNF with the
 // runtime size in Q.nf.
 template <int BK, int PK, int FL, int NF, int PM>
-__host__ __device__ double integrate_acc(const QuadDev& Q, const DiagCoef<PM>& dc, AdjAcc<PM>& A, int* nevals) {
+__host__ __device__ double integrate_acc(const QuadDev& Q, const DiagCoef<PM>& dc, AdjAcc<PM>& A, int* nevals,
+                                         const NodeV<FL>* at1) {
     AccRec<FL> stk[MAXSUB];
     int sp = 0;
     const int nf = (NF == MAXNF) ? Q.nf : NF;
@@ -644,7 +662,7 @@ __host__ __device__ double integrate_acc(const QuadDev& Q, const DiagCoef<PM>& d
     const int cRight = nc - 1, cMid = cRight / 2;
     const int fRight = nf - 1, fMid = fRight / 2;
     NodeV<FL> fa = eval_node<BK, PK, FL, PM>(dc, 0.0);
-    NodeV<FL> fb = eval_node<BK, PK, FL, PM>(dc, 1.0);
+    NodeV<FL> fb = at1 ? *at1 : eval_node<BK, PK, FL, PM>(dc, 1.0);
     int ne = 2;
     double a = 0.0, b = 1.0, res = 0.0;
     int level = 0;
@@ -726,11 +744,14 @@ __host__ __device__ double integrate_acc(const QuadDev& Q, const DiagCoef<PM>& d
     return res;
 }
 
+// at1, when given, is the integrand at t = 1 (computed by the caller, which
+// also needs g and g' at x_k); it is the value eval_node would produce there.
 template <int BK, int PK, int QK, int FL, int PM>
-MPD_HD double integrate(const QuadDev& Q, const DiagCoef<PM>& dc, AdjAcc<PM>& A, int* nevals) {
-    if constexpr (QK == QK_AS) return integrate_as<BK, PK, FL, PM>(Q, dc, A, nevals);
-    else if constexpr (QK == QK_ACC5) return integrate_acc<BK, PK, FL, 5, PM>(Q, dc, A, nevals);
-    else return integrate_acc<BK, PK, FL, MAXNF, PM>(Q, dc, A, nevals);
+MPD_HD double integrate(const QuadDev& Q, const DiagCoef<PM>& dc, AdjAcc<PM>& A, int* nevals,
+                        const NodeV<FL>* at1 = nullptr) {
+    if constexpr (QK == QK_AS) return integrate_as<BK, PK, FL, PM>(Q, dc, A, nevals, at1);
+    else if constexpr (QK == QK_ACC5) return integrate_acc<BK, PK, FL, 5, PM>(Q, dc, A, nevals, at1);
+    else return integrate_acc<BK, PK, FL, MAXNF, PM>(Q, dc, A, nevals, at1);
 }
 
 template <int PM>
@@ -757,18 +778,24 @@ MPD_HD void comp_core(const MapDev& M, const CompDev& cd, const double* C, doubl
     const double f0 = f_at_zero<PM>(M, cd, C);
     DiagCoef<PM> dc;
     make_diag<PM>(M, cd, C, xk, M.nugget, dc);
+    // g and g' at x_k (log-determinant) are the integrand's values at t = 1.
+    const double z = diag_eval<BK, false, PM>(dc, 1.0 * xk, nullptr);
+    const PosF<PK> gv = posfunc<PK>(z);
+    NodeV<FL_COEF> n1;
+    n1.F = dc.x * (gv.g + dc.nug);
+    n1.h = dc.x * gv.gp;
     AdjAcc<PM> A;
     zero_adj<PM>(A);
-    double T = integrate<BK, PK, QK, FL_COEF, PM>(M.quad, dc, A, nullptr) + f0;
+    double T = integrate<BK, PK, QK, FL_COEF, PM>(M.quad, dc, A, nullptr, &n1) + f0;
     if (M.nugget != 0.0) {
         // Evaluate drops the nugget (MonotoneComponent.h:287): its tree may differ.
         DiagCoef<PM> dv = dc;
         dv.nug = 0.0;
+        NodeV<FL_VAL> v1;
+        v1.F = dv.x * (gv.g + dv.nug);
         AdjAcc<PM> Av;
-        T = integrate<BK, PK, QK, FL_VAL, PM>(M.quad, dv, Av, nullptr) + f0;
+        T = integrate<BK, PK, QK, FL_VAL, PM>(M.quad, dv, Av, nullptr, &v1) + f0;
     }
-    const double z = diag_eval<BK, false, PM>(dc, xk, nullptr);
-    const PosF<PK> gv = posfunc<PK>(z);
     const double ldv = (gv.g <= 0) ? -__builtin_inf() : log(gv.g);
     const double r = gv.gp * (1.0 / gv.g);
     double s = s_in, l = l_in;
