@@ -871,11 +871,13 @@ __device__ __forceinline__ double wave_reduce16(const double* v, int lane) {
 }
 
 // Reduces 16 per-lane values over the wavefront and adds sum i to dst[i]
-// for i < nvalid.
+// for i < nvalid.  dst is in LDS and owned by this wave; the update is an LDS
+// atomic add without return, so the wave does not wait for it (the 16 writer
+// lanes hit distinct addresses, and one wave's updates apply in program order).
 __device__ __forceinline__ void reduce16_add(double* v, int lane, double* dst, int nvalid) {
     const double r = wave_reduce16(v, lane);
     const int u = lane >> 2;
-    if ((lane & 3) == 0 && u < nvalid) dst[u] += r;
+    if ((lane & 3) == 0 && u < nvalid) __hip_atomic_fetch_add(dst + u, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
