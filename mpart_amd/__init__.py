@@ -323,6 +323,12 @@ class ConditionalMap:
         _check(_lib.mpart_wrap_coeffs(self._h, coeffs.data_ptr(), coeffs.shape[0], coeffs.stride(0)))
         self._wrapped = coeffs
 
+    def torch(self, store_coeffs=True, return_logdet=False):
+        """torch.nn.Module view of the map (bindings/python/src/ConditionalMapBase.cpp:44-50)."""
+        from .torch import TorchConditionalMapBase
+
+        return TorchConditionalMapBase(self, store_coeffs, return_logdet)
+
     def CoeffMap(self):
         torch = _torch()
         out = torch.empty(self.numCoeffs, dtype=torch.float64, device="cuda")

@@ -138,5 +138,35 @@ def kl_loss(f, x, coeffs):
     return (0.5 * (y * y).sum(0) - ld).mean()
 
 
+class KLObjective(torch.autograd.Function):
+    """The KL objective mean(0.5 |T(x)|^2 - log det grad T(x)) of
+    FastGaussianReverseKLObjective (src/MapObjective.cpp:146-234) as one autograd
+    node: forward runs the fused forward+adjoint launch
+    (ConditionalMap.KLValueAndGradImpl) and keeps the coefficient gradient;
+    backward scales it.  Gradients with respect to the points are not formed."""
+
+    @staticmethod
+    def forward(ctx, x, coeffs, f):
+        if x.requires_grad:
+            raise ValueError("KLObjective differentiates with respect to the coefficients only")
+        cd = coeffs.detach().to(torch.float64).contiguous()
+        f.WrapCoeffs(cd)
+        loss = torch.zeros(1, dtype=torch.float64, device=x.device)
+        grad = torch.zeros(f.numCoeffs, dtype=torch.float64, device=x.device)
+        f.KLValueAndGradImpl(x.detach().to(torch.float64), loss, grad)
+        ctx.save_for_backward(grad)
+        return loss[0].to(coeffs.dtype)
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        (grad,) = ctx.saved_tensors
+        return None, (grad_out * grad).to(grad_out.dtype), None
+
+
+def kl_objective(f, x, coeffs):
+    """Fused KL objective; x is inputDim x N (a strided view is accepted)."""
+    return KLObjective.apply(x, coeffs, f)
+
+
 __all__ = ["MpartTorchAutograd", "TorchParameterizedFunctionBase", "TorchConditionalMapBase", "kl_loss",
-           "ConditionalMap"]
+           "KLObjective", "kl_objective", "ConditionalMap"]
