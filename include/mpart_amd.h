@@ -200,6 +200,37 @@ int mpart_set_plain_launches(mpart_map map, int plain);
  * are in use, 0 otherwise, and the compile time of the module in seconds (0
  * when it came from the cache).  mpart_codegen_error returns the reason of a
  * failed compilation ("" otherwise). */
+/* Batches of independent maps.  A batch runs one method over many maps (each
+ * with its own points, outputs and coefficients) in one persistent launch:
+ * every map's points are split into tiles, the tiles are ordered by
+ * estimated cost and the GPU's blocks take them from a shared counter.  No
+ * reference counterpart (the reference loops over maps and components on the
+ * host, src/TriangularMap.cpp:100-310); it is the batched form of the
+ * per-map EvaluateImpl / LogDeterminantImpl / CoeffGradImpl / InverseImpl.
+ *
+ * The maps must share MapOptions, must outlive the batch, and keep their own
+ * coefficients (read at call time).  Arrays hold one entry per map, in batch
+ * order.  Each method keeps its arguments on the device and uploads a call's
+ * arguments when they differ from the previous call of that method; under
+ * stream capture they must not differ (make the same calls once before
+ * capturing), so a captured graph -- e.g. Evaluate, CoeffGrad and Inverse in
+ * sequence -- replays with no host work.  A graph reads the arguments of each
+ * method at replay time: calling a method with other arguments between
+ * replays redirects the graph's launch of that method to them.  Batched Inverse follows ConditionalMapBase::Inverse per map: x1 has
+ * inDim - outDim rows for a triangular map, and all inDim rows (the last one
+ * the initial guess) for a single component. */
+typedef struct mpart_batch_s* mpart_batch;
+int mpart_create_batch(const mpart_map* maps, uint32_t n, mpart_batch* out);
+int mpart_destroy_batch(mpart_batch batch);
+int mpart_batch_size(mpart_batch batch, uint32_t* n);
+/* out and logdet may be NULL (not computed); so may out[i].ptr / logdet[i]. */
+int mpart_batch_evaluate(mpart_batch batch, const mpart_cview* pts, const mpart_view* out, double* const* logdet,
+                         const int64_t* logdetStride, mpart_stream stream);
+int mpart_batch_coeff_grad(mpart_batch batch, const mpart_cview* pts, const mpart_cview* sens, const mpart_view* out,
+                           mpart_stream stream);
+int mpart_batch_inverse(mpart_batch batch, const mpart_cview* x1, const mpart_cview* r, const mpart_view* out,
+                        double xtol, double ytol, mpart_stream stream);
+
 int mpart_set_codegen(mpart_map map, int enable);
 int mpart_codegen_prepare(mpart_map map, int32_t* specialized, double* compileSeconds);
 const char* mpart_codegen_error(mpart_map map);

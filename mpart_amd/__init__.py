@@ -88,6 +88,12 @@ _SIGS = {
     "mpart_coeff_vjp": [_vp, _CView, _CView, _vp, _i64, _vp, _vp],
     "mpart_set_plain_launches": [_vp, _int],
     "mpart_set_codegen": [_vp, _int],
+    "mpart_create_batch": [_MAPP, _u32, _MAPP],
+    "mpart_destroy_batch": [_vp],
+    "mpart_batch_size": [_vp, _U32P],
+    "mpart_batch_evaluate": [_vp, _vp, _vp, _vp, _vp, _vp],
+    "mpart_batch_coeff_grad": [_vp, _vp, _vp, _vp, _vp],
+    "mpart_batch_inverse": [_vp, _vp, _vp, _vp, _dbl, _dbl, _vp],
     "mpart_codegen_prepare": [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)],
 }
 for _name, _args in _SIGS.items():
@@ -498,5 +504,77 @@ def TriangularMap(components):
     return ConditionalMap(h.value)
 
 
+class MapBatch:
+    """Many independent maps (sharing MapOptions) run one method in one
+    persistent launch (``mpart_create_batch``).  Arguments are lists with one
+    device tensor per map; outputs are preallocated (``*Impl``) or allocated."""
+
+    def __init__(self, maps):
+        self.maps = list(maps)
+        arr = (ctypes.c_void_p * len(self.maps))(*[m.handle.value for m in self.maps])
+        h = ctypes.c_void_p()
+        _check(_lib.mpart_create_batch(arr, len(self.maps), ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            _lib.mpart_destroy_batch(h)
+            self._h = None
+
+    def __len__(self):
+        return len(self.maps)
+
+    @staticmethod
+    def _views(ts):
+        return (_CView * len(ts))(*[_view(t) for t in ts])
+
+    def EvaluateImpl(self, pts, outs=None, logdets=None):
+        pv = self._views(pts)
+        ov = self._views(outs) if outs is not None else None
+        if logdets is not None:
+            lp = (ctypes.c_void_p * len(logdets))(*[t.data_ptr() for t in logdets])
+            ls = (ctypes.c_int64 * len(logdets))(*[t.stride(0) for t in logdets])
+        else:
+            lp = ls = None
+        _check(_lib.mpart_batch_evaluate(self._h, ctypes.cast(pv, _vp), ctypes.cast(ov, _vp) if ov is not None else None,
+                                         ctypes.cast(lp, _vp) if lp is not None else None,
+                                         ctypes.cast(ls, _vp) if ls is not None else None, _stream()))
+
+    def CoeffGradImpl(self, pts, sens, outs):
+        _check(_lib.mpart_batch_coeff_grad(self._h, ctypes.cast(self._views(pts), _vp), ctypes.cast(self._views(sens), _vp),
+                                           ctypes.cast(self._views(outs), _vp), _stream()))
+
+    def InverseImpl(self, x1, r, outs, xtol=1e-6, ytol=1e-6):
+        _check(_lib.mpart_batch_inverse(self._h, ctypes.cast(self._views(x1), _vp), ctypes.cast(self._views(r), _vp),
+                                        ctypes.cast(self._views(outs), _vp), xtol, ytol, _stream()))
+
+    def Evaluate(self, pts, logdet=False):
+        torch = _torch()
+        outs = [torch.empty((m.outputDim, p.shape[1]), dtype=torch.float64, device=p.device) for m, p in zip(self.maps, pts)]
+        lds = [torch.empty(p.shape[1], dtype=torch.float64, device=p.device) for p in pts] if logdet else None
+        self.EvaluateImpl(pts, outs, lds)
+        return (outs, lds) if logdet else outs
+
+    def LogDeterminant(self, pts):
+        torch = _torch()
+        lds = [torch.empty(p.shape[1], dtype=torch.float64, device=p.device) for p in pts]
+        self.EvaluateImpl(pts, None, lds)
+        return lds
+
+    def CoeffGrad(self, pts, sens):
+        torch = _torch()
+        outs = [torch.empty((m.numCoeffs, p.shape[1]), dtype=torch.float64, device=p.device) for m, p in zip(self.maps, pts)]
+        self.CoeffGradImpl(pts, sens, outs)
+        return outs
+
+    def Inverse(self, x1, r, xtol=1e-6, ytol=1e-6):
+        torch = _torch()
+        outs = [torch.empty((m.outputDim, rr.shape[1]), dtype=torch.float64, device=rr.device) for m, rr in zip(self.maps, r)]
+        self.InverseImpl(x1, r, outs, xtol, ytol)
+        return outs
+
+
 __all__ = ["MapOptions", "BasisTypes", "PosFuncTypes", "QuadTypes", "FixedMultiIndexSet", "ConditionalMap",
-           "CreateComponent", "CreateTotalOrderComponent", "CreateTriangular", "TriangularMap", "Initialize"]
+           "CreateComponent", "CreateTotalOrderComponent", "CreateTriangular", "TriangularMap", "MapBatch",
+           "Initialize"]

@@ -37,6 +37,11 @@ hipError_t launch_fused(int var, const Launch& L, const MapDev& M, const double*
                         double* gradGlobal);
 int occupancy_fused(int var, int block, size_t smem);
 
+struct BatchIO;
+hipError_t launch_batch(int var, const Launch& L, const MapDev* maps, const BatchIO* io, const int* tileMap,
+                        const long* tileStart, int ntiles, unsigned* counter, int method, double xtol, double ytol);
+int occupancy_batch(int var, int block, size_t smem);
+
 hipError_t launch_reduce_rows(const double* in, long nrows, long K, double scale, double* out, hipStream_t s);
 hipError_t launch_reduce_cols(const double* in, long nrows, long rowStride, long ncols, double scale, double* out,
                               double* outLast, hipStream_t s);

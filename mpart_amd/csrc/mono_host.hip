@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/mpart_amd.h"
+#include "batch_io.hpp"
 #include "codegen.hpp"
 #include "launchers.hpp"
 
@@ -821,6 +822,121 @@ void do_fused(mpart_map_s& m, const mpart_cview& p, const double* sens, long srs
                   "launch reduce");
 }
 
+// ---------------------------------------------------------------------------
+// Batches of independent maps (one persistent launch per method).
+}  // namespace
+
+// Device-side arguments of one batched method (each method keeps its own, so a
+// graph can hold a sequence of different batched calls).
+struct BatchSlot {
+    DBuf dIO, dTileMap, dTileStart, dCounter;
+    bool valid = false;
+    std::vector<mpd::BatchIO> io;
+    std::vector<int> tileMap;
+    std::vector<long> tileStart;
+};
+
+struct mpart_batch_s {
+    std::vector<mpart_map> maps;
+    int variant = 0;
+    int block = 256;
+    size_t smem = 0;
+    DBuf dMaps;
+    BatchSlot slot[3];
+    int occ = 0;
+};
+
+namespace {
+
+mpart_batch_s* make_batch(const mpart_map* maps, uint32_t n) {
+    if (!maps || n == 0) throw ArgError("CreateBatch: no maps");
+    auto b = std::make_unique<mpart_batch_s>();
+    int pmClass = 0;
+    int maxSlots = 1;
+    std::vector<MapDev> devs;
+    for (uint32_t i = 0; i < n; ++i) {
+        check_map(maps[i]);
+        if (std::memcmp(&maps[i]->opts, &maps[0]->opts, sizeof(mpart_options)) != 0)
+            throw NotImpl("a batch needs maps with identical MapOptions");
+        pmClass = std::max(pmClass, maps[i]->variant / 1000);
+        maxSlots = std::max(maxSlots, maps[i]->dev.nslots);
+        if (maps[i]->inDim > (unsigned)mpd::BATCH_MAX_DIM)
+            throw NotImpl("batched maps are limited to input dimension " + std::to_string(mpd::BATCH_MAX_DIM));
+        b->maps.push_back(maps[i]);
+        devs.push_back(maps[i]->dev);
+    }
+    b->variant = pmClass * 1000 + maps[0]->variant % 1000;
+    int block = 256;
+    while (block > 64 && (size_t)maxSlots * block * sizeof(double) > LDS_LIMIT) block /= 2;
+    if ((size_t)maxSlots * block * sizeof(double) > LDS_LIMIT)
+        throw NotImpl("batched maps need a basis table that fits in LDS (at most " +
+                      std::to_string(LDS_LIMIT / (64 * sizeof(double))) + " table slots)");
+    b->block = block;
+    b->smem = (size_t)maxSlots * block * sizeof(double);
+    upload(b->dMaps, devs);
+    for (auto& sl : b->slot) sl.dCounter.ensure(256);
+    return b.release();
+}
+
+// Uploads the call's descriptors and tile list when they differ from the ones
+// on the device, then launches.  Under stream capture the descriptors must
+// already be on the device (run the same call once before capturing).
+void run_batch(mpart_batch_s& b, int method, std::vector<mpd::BatchIO>& io, double xtol, double ytol, hipStream_t s) {
+    const size_t n = b.maps.size();
+    // Tiles of `block` points, most expensive map first (cost ~ terms + quadrature per component).
+    std::vector<std::pair<double, std::pair<int, long>>> tiles;
+    for (size_t i = 0; i < n; ++i) {
+        const double cost = (double)b.maps[i]->numCoeffs + 40.0 * (double)b.maps[i]->comps.size();
+        for (long t0 = 0; t0 < io[i].N; t0 += b.block) tiles.push_back({cost, {(int)i, t0}});
+    }
+    std::stable_sort(tiles.begin(), tiles.end(), [](const auto& a, const auto& c) { return a.first > c.first; });
+    std::vector<int> tileMap(tiles.size());
+    std::vector<long> tileStart(tiles.size());
+    for (size_t t = 0; t < tiles.size(); ++t) {
+        tileMap[t] = tiles[t].second.first;
+        tileStart[t] = tiles[t].second.second;
+    }
+    BatchSlot& sl = b.slot[method];
+    const bool same = sl.valid && tileMap == sl.tileMap && tileStart == sl.tileStart && io.size() == sl.io.size() &&
+                      std::memcmp(io.data(), sl.io.data(), io.size() * sizeof(mpd::BatchIO)) == 0;
+    if (!same) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        hip_check(hipStreamIsCapturing(s, &cs), "hipStreamIsCapturing");
+        if (cs != hipStreamCaptureStatusNone)
+            throw ArgError("batched call: its arguments differ from the previous call of this method on this batch; "
+                           "run the call once before capturing it in a graph");
+        hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+        upload(sl.dIO, io);
+        upload(sl.dTileMap, tileMap);
+        upload(sl.dTileStart, tileStart);
+        sl.io = io;
+        sl.tileMap = tileMap;
+        sl.tileStart = tileStart;
+        sl.valid = true;
+    }
+    if (tiles.empty()) return;
+    if (g_numCU == 0) {
+        int dev = 0;
+        hip_check(hipGetDevice(&dev), "hipGetDevice");
+        hip_check(hipDeviceGetAttribute(&g_numCU, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
+    }
+    if (b.occ == 0) b.occ = mpd::occupancy_batch(b.variant, b.block, b.smem);
+    const long grid = std::min<long>((long)tiles.size(), (long)g_numCU * b.occ);
+    mpd::Launch L{dim3((unsigned)grid), dim3(b.block), b.smem, s};
+    hip_check(mpd::launch_batch(b.variant, L, b.dMaps.as<MapDev>(), sl.dIO.as<mpd::BatchIO>(), sl.dTileMap.as<int>(),
+                                sl.dTileStart.as<long>(), (int)tiles.size(), sl.dCounter.as<unsigned>(), method, xtol,
+                                ytol),
+              "launch batch");
+}
+
+mpd::BatchIO base_io(mpart_map m) {
+    check_coeffs(m);
+    mpd::BatchIO io{};
+    io.w = m->coeffs;
+    io.ws = m->coeffStride;
+    return io;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1128,5 +1244,119 @@ int mpart_codegen_prepare(mpart_map m, int32_t* specialized, double* compileSeco
 }
 
 const char* mpart_codegen_error(mpart_map m) { return m ? m->genErr.c_str() : "null map handle"; }
+
+int mpart_create_batch(const mpart_map* maps, uint32_t n, mpart_batch* out) {
+    return guarded([&] {
+        if (!out) throw ArgError("null argument");
+        *out = make_batch(maps, n);
+    });
+}
+
+int mpart_destroy_batch(mpart_batch b) {
+    return guarded([&] { delete b; });
+}
+
+int mpart_batch_size(mpart_batch b, uint32_t* n) {
+    return guarded([&] {
+        if (!b || !n) throw ArgError("null argument");
+        *n = (uint32_t)b->maps.size();
+    });
+}
+
+int mpart_batch_evaluate(mpart_batch b, const mpart_cview* pts, const mpart_view* out, double* const* logdet,
+                         const int64_t* logdetStride, mpart_stream s) {
+    return guarded([&] {
+        if (!b || !pts) throw ArgError("null argument");
+        std::vector<mpd::BatchIO> io;
+        for (size_t i = 0; i < b->maps.size(); ++i) {
+            mpart_map m = b->maps[i];
+            mpd::BatchIO d = base_io(m);
+            const std::string w = "batched Evaluate, map " + std::to_string(i);
+            check_view((w + ": pts").c_str(), pts[i].rows, pts[i].cols, m->inDim, pts[i].cols);
+            d.p = pts[i].ptr;
+            d.prs = pts[i].row_stride;
+            d.pcs = pts[i].col_stride;
+            d.N = pts[i].cols;
+            if (out && out[i].ptr) {
+                check_view((w + ": output").c_str(), out[i].rows, out[i].cols, m->outDim, pts[i].cols);
+                d.o = out[i].ptr;
+                d.ors = out[i].row_stride;
+                d.ocs = out[i].col_stride;
+            }
+            if (logdet && logdet[i]) {
+                if (!m->opts.contDeriv) throw NotImpl("LogDeterminant with contDeriv = false is not provided yet");
+                d.ld = logdet[i];
+                d.lds = logdetStride ? logdetStride[i] : 1;
+            }
+            io.push_back(d);
+        }
+        run_batch(*b, mpd::BM_EVAL, io, 0.0, 0.0, S(s));
+    });
+}
+
+int mpart_batch_coeff_grad(mpart_batch b, const mpart_cview* pts, const mpart_cview* sens, const mpart_view* out,
+                           mpart_stream s) {
+    return guarded([&] {
+        if (!b || !pts || !sens || !out) throw ArgError("null argument");
+        std::vector<mpd::BatchIO> io;
+        for (size_t i = 0; i < b->maps.size(); ++i) {
+            mpart_map m = b->maps[i];
+            mpd::BatchIO d = base_io(m);
+            const std::string w = "batched CoeffGrad, map " + std::to_string(i);
+            check_view((w + ": pts").c_str(), pts[i].rows, pts[i].cols, m->inDim, pts[i].cols);
+            check_view((w + ": sens").c_str(), sens[i].rows, sens[i].cols, m->outDim, pts[i].cols);
+            check_view((w + ": output").c_str(), out[i].rows, out[i].cols, m->numCoeffs, pts[i].cols);
+            d.p = pts[i].ptr;
+            d.prs = pts[i].row_stride;
+            d.pcs = pts[i].col_stride;
+            d.N = pts[i].cols;
+            d.s = sens[i].ptr;
+            d.srs = sens[i].row_stride;
+            d.scs = sens[i].col_stride;
+            d.o = out[i].ptr;
+            d.ors = out[i].row_stride;
+            d.ocs = out[i].col_stride;
+            io.push_back(d);
+        }
+        run_batch(*b, mpd::BM_COEFF_GRAD, io, 0.0, 0.0, S(s));
+    });
+}
+
+int mpart_batch_inverse(mpart_batch b, const mpart_cview* x1, const mpart_cview* r, const mpart_view* out, double xtol,
+                        double ytol, mpart_stream s) {
+    return guarded([&] {
+        if (!b || !x1 || !r || !out) throw ArgError("null argument");
+        if (xtol < 0 || ytol < 0) throw ArgError("Invalid tolerance given to Inverse.  Value must be non-negative");
+        if (ytol <= 2.220446049250313e-16 && xtol <= 2.220446049250313e-16)
+            throw ArgError("Invalid tolerances given to Inverse.  Either \"xtol\" or \"ytol\" must be nonzero");
+        std::vector<mpd::BatchIO> io;
+        for (size_t i = 0; i < b->maps.size(); ++i) {
+            mpart_map m = b->maps[i];
+            mpd::BatchIO d = base_io(m);
+            const std::string w = "batched Inverse, map " + std::to_string(i);
+            const long N = r[i].cols;
+            check_view((w + ": r").c_str(), r[i].rows, r[i].cols, m->outDim, N);
+            check_view((w + ": output").c_str(), out[i].rows, out[i].cols, m->outDim, N);
+            const long xrows = m->triangular ? (long)(m->inDim - m->outDim) : (long)m->inDim;
+            if (x1[i].rows != xrows) throw ArgError(w + ": x1 must have " + std::to_string(xrows) + " rows");
+            if (xrows > 0 && x1[i].cols != N && !(x1[i].cols == 1 && !m->triangular))
+                throw ArgError(w + ": x1 must have one column per target");
+            d.p = x1[i].ptr;
+            d.prs = x1[i].row_stride;
+            d.pcs = x1[i].col_stride;
+            d.pcols = x1[i].cols;
+            d.xrows = xrows;
+            d.N = N;
+            d.s = r[i].ptr;
+            d.srs = r[i].row_stride;
+            d.scs = r[i].col_stride;
+            d.o = out[i].ptr;
+            d.ors = out[i].row_stride;
+            d.ocs = out[i].col_stride;
+            io.push_back(d);
+        }
+        run_batch(*b, mpd::BM_INVERSE, io, xtol, ytol, S(s));
+    });
+}
 
 }  // extern "C"

@@ -288,34 +288,12 @@ __device__ double eval_T(const MapDev& M, const DiagCoef<PM>& dc0, double f0, do
     return integrate<BK, PK, QK, FL_VAL, PM>(M.quad, dc, A, nullptr) + f0;
 }
 
+// Bracket (FindBracket, RootFinding.h:20-80) and ITP (Find_x_ITP, 82-178) for
+// T(x) = yd with T given by its diagonal coefficients dc (x_k replaced by the
+// iterate) and f0 = f(x_<k, 0).  Returns NaN when no bracket is found.
 template <int BK, int PK, int QK, int PM>
-__global__ void __launch_bounds__(256) k_inverse(MapDev M, const double* w, long ws, const double* xs, long xrows,
-                                                 long xcols, long xrs, long xcs, const double* ys, long yss, long N,
-                                                 double* out, long os, int c, double xtol, double ytol, double* gtab) {
-    extern __shared__ double smem[];
-    const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
-    const long xi = (xcols == 1) ? 0 : n;
-    const double* pt = xs + xi * xcs;
-    for (long i = 0; i < xrows; ++i) {
-        if (isnan(pt[i * xrs])) {
-            out[n * os] = NAN;
-            return;
-        }
-    }
-    const double x0 = pt[(xrows - 1) * xrs];
-    const double yd = ys[n * yss];
-    long stride;
-    double* tab = lane_table(smem, gtab, stride);
-    const CompDev cd = M.comps[c];
-    fill_values<BK>(M, cd.dim - 1, pt, xrs, tab, (int)stride);
-    double C[PM + 1];
-    coeff_groups<PM>(M, cd, w + (long)cd.coeffOff * ws, ws, tab, (int)stride, C);
-    const double f0 = f_at_zero<PM>(M, cd, C);
-    DiagCoef<PM> dc;
-    make_diag<PM>(M, cd, C, x0, M.nugget, dc);
-
-    // FindBracket
+__device__ double inverse_solve(const MapDev& M, const DiagCoef<PM>& dc, double f0, double x0, double yd, double xtol,
+                                double ytol) {
     double xlb = x0, xub = x0;
     double ylb = eval_T<BK, PK, QK, PM>(M, dc, f0, xlb);
     double yub = ylb;
@@ -347,10 +325,7 @@ __global__ void __launch_bounds__(256) k_inverse(MapDev M, const double* w, long
             if (info == 0 && i >= maxIts) info = -2;
         }
     }
-    if ((info < 0) || ((ylb > yd) || (yub < yd))) {
-        out[n * os] = NAN;
-        return;
-    }
+    if ((info < 0) || ((ylb > yd) || (yub < yd))) return NAN;
     // ITP (k1 = 0.1, k2 = 2, n0 = 1)
     const double k1 = 0.1, k2 = 2.0, n0 = 1.0;
     const double nhalf = ceil(log2(0.5 * (xub - xlb) / xtol));
@@ -365,8 +340,7 @@ __global__ void __launch_bounds__(256) k_inverse(MapDev M, const double* w, long
         const double xc = xb - sigma * rho;
         const double yc = eval_T<BK, PK, QK, PM>(M, dc, f0, xc);
         if (fabs(yc - yd) < ytol) {
-            out[n * os] = xc;
-            return;
+            return xc;
         } else if (yc > yd) {
             xub = xc;
             yub = yc;
@@ -376,7 +350,36 @@ __global__ void __launch_bounds__(256) k_inverse(MapDev M, const double* w, long
         }
         if (((xub - xlb) < xtol) || ((yub - ylb) < ytol)) break;
     }
-    out[n * os] = 0.5 * (xub + xlb);
+    return 0.5 * (xub + xlb);
+}
+
+template <int BK, int PK, int QK, int PM>
+__global__ void __launch_bounds__(256) k_inverse(MapDev M, const double* w, long ws, const double* xs, long xrows,
+                                                 long xcols, long xrs, long xcs, const double* ys, long yss, long N,
+                                                 double* out, long os, int c, double xtol, double ytol, double* gtab) {
+    extern __shared__ double smem[];
+    const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    const long xi = (xcols == 1) ? 0 : n;
+    const double* pt = xs + xi * xcs;
+    for (long i = 0; i < xrows; ++i) {
+        if (isnan(pt[i * xrs])) {
+            out[n * os] = NAN;
+            return;
+        }
+    }
+    const double x0 = pt[(xrows - 1) * xrs];
+    const double yd = ys[n * yss];
+    long stride;
+    double* tab = lane_table(smem, gtab, stride);
+    const CompDev cd = M.comps[c];
+    fill_values<BK>(M, cd.dim - 1, pt, xrs, tab, (int)stride);
+    double C[PM + 1];
+    coeff_groups<PM>(M, cd, w + (long)cd.coeffOff * ws, ws, tab, (int)stride, C);
+    const double f0 = f_at_zero<PM>(M, cd, C);
+    DiagCoef<PM> dc;
+    make_diag<PM>(M, cd, C, x0, M.nugget, dc);
+    out[n * os] = inverse_solve<BK, PK, QK, PM>(M, dc, f0, x0, yd, xtol, ytol);
 }
 
 // ---------------------------------------------------------------------------

@@ -1,0 +1,177 @@
+"""Batched execution of many independent maps (one persistent launch per
+method) and its capture in a HIP graph.
+
+Maps: triangular maps with dimension and total order drawn from a seeded
+stream (a scaled-down version of the many-maps workload: dims 2..9, orders
+2..4, 100..700 points each) plus single components.  The batched kernel runs
+the same per-point arithmetic as the single-map generic kernels, so its
+results equal theirs bitwise; they are also checked against the oracle.
+"""
+import numpy as np
+import pytest
+
+import mpart_amd as mt
+from mpart_amd.data import Stream
+from oracle import oracle as orc
+
+from test_gpu_parity import close
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+def _case(name):
+    print(f"[gpu-case] {name}", flush=True)
+
+
+def _make_batch(nmaps=20, seed=11):
+    s = Stream(seed)
+    u = s.uniform(4 * nmaps).reshape(nmaps, 4)
+    maps, refs, xs, tri = [], [], [], []
+    for i in range(nmaps):
+        d = 2 + int(u[i, 0] * 8)
+        p = 2 + int(u[i, 1] * 3)
+        n = 100 + int(u[i, 2] * 600)
+        if i % 5 == 4:  # a single component among the triangular maps
+            g = mt.CreateTotalOrderComponent(d, p, mt.MapOptions())
+            r = orc.CreateComponent(orc.options(), d, p)
+        else:
+            g = mt.CreateTriangular(d, d, p, mt.MapOptions())
+            r = orc.CreateTriangular(d, d, p, orc.options())
+        c = 0.1 * s.normal(g.numCoeffs)
+        g.SetCoeffs(c)
+        r.SetCoeffs(c)
+        maps.append(g)
+        refs.append(r)
+        xs.append(torch.from_numpy(s.points(d, n)).cuda())
+        tri.append(i % 5 != 4)
+    return maps, refs, xs, tri
+
+
+@pytest.fixture(scope="module")
+def batch():
+    maps, refs, xs, tri = _make_batch()
+    return mt.MapBatch(maps), maps, refs, xs, tri
+
+
+def _as_dev(a):
+    return a if isinstance(a, torch.Tensor) else torch.from_numpy(a).cuda()
+
+
+def test_batched_evaluate_and_logdet(batch):
+    _case("batch_evaluate")
+    b, maps, refs, xs, _ = batch
+    ys, lds = b.Evaluate(xs, logdet=True)
+    for m, r, x, y, ld in zip(maps, refs, xs, ys, lds):
+        m.SetCodegen(False)  # the single-map generic kernels: same arithmetic, same bits
+        assert torch.equal(y, _as_dev(m.Evaluate(x)))
+        assert torch.equal(ld, _as_dev(m.LogDeterminant(x)))
+        m.SetCodegen(True)
+        xn = x.cpu().numpy()
+        close(y.cpu().numpy(), r.Evaluate(xn), what="batched Evaluate")
+        close(ld.cpu().numpy(), r.LogDeterminant(xn), what="batched LogDeterminant")
+
+
+def _sens(maps, xs, seed):
+    s = Stream(seed)
+    return [torch.from_numpy(s.normal(m.outputDim * x.shape[1]).reshape(m.outputDim, x.shape[1])).cuda()
+            for m, x in zip(maps, xs)]
+
+
+def test_batched_coeff_grad(batch):
+    _case("batch_coeff_grad")
+    b, maps, refs, xs, _ = batch
+    sens = _sens(maps, xs, 12)
+    outs = b.CoeffGrad(xs, sens)
+    for m, r, x, se, o in zip(maps, refs, xs, sens, outs):
+        m.SetCodegen(False)
+        single = _as_dev(m.CoeffGrad(x, se))
+        m.SetCodegen(True)
+        assert torch.equal(o, single)
+        close(o.cpu().numpy(), r.CoeffGrad(x.cpu().numpy(), se.cpu().numpy()), what="batched CoeffGrad")
+
+
+def _inverse_args(maps, refs, xs, tri):
+    x1s, rs = [], []
+    for m, r, x, t in zip(maps, refs, xs, tri):
+        rs.append(torch.from_numpy(r.Evaluate(x.cpu().numpy())).cuda())
+        # triangular map: no conditioning rows; component: all rows, the last one the initial guess
+        x1s.append(x[:0] if t else x)
+    return x1s, rs
+
+
+def test_batched_inverse(batch):
+    _case("batch_inverse")
+    b, maps, refs, xs, tri = batch
+    x1s, rs = _inverse_args(maps, refs, xs, tri)
+    outs = b.Inverse(x1s, rs)
+    for m, r, x, x1, rr, o, t in zip(maps, refs, xs, x1s, rs, outs, tri):
+        single = _as_dev(m.Inverse(x1, rr))
+        assert torch.equal(o, single)
+        ref = r.Inverse(x1.cpu().numpy(), rr.cpu().numpy())
+        got = o.cpu().numpy()
+        fin = np.isfinite(ref)
+        assert np.array_equal(fin, np.isfinite(got))
+        if t:
+            # initial guesses 0: agreement to the oracle's arithmetic
+            assert np.max(np.abs(got[fin] - ref[fin])) <= 1e-9
+        else:
+            # initial guess = the solution: agreement to the solver tolerance
+            assert np.max(np.abs(got[fin] - ref[fin])) <= 1e-5
+
+
+def test_batched_sequence_replays_in_one_graph(batch):
+    # Evaluate + LogDeterminant, CoeffGrad and Inverse captured in one graph and
+    # replayed three times with eager work in between; every replay must give
+    # the eager results.
+    _case("batch_graph")
+    b, maps, refs, xs, tri = batch
+    ys = [torch.empty((m.outputDim, x.shape[1]), dtype=torch.float64, device="cuda") for m, x in zip(maps, xs)]
+    lds = [torch.empty(x.shape[1], dtype=torch.float64, device="cuda") for x in xs]
+    sens = _sens(maps, xs, 13)
+    cgs = [torch.empty((m.numCoeffs, x.shape[1]), dtype=torch.float64, device="cuda") for m, x in zip(maps, xs)]
+    x1s, rs = _inverse_args(maps, refs, xs, tri)
+    xis = [torch.empty_like(r) for r in rs]
+
+    def seq():
+        b.EvaluateImpl(xs, ys, lds)
+        b.CoeffGradImpl(xs, sens, cgs)
+        b.InverseImpl(x1s, rs, xis)
+
+    stream = torch.cuda.Stream()
+    stream.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(stream):
+        seq()  # eager: uploads each method's arguments
+    torch.cuda.current_stream().wait_stream(stream)
+    torch.cuda.synchronize()
+    ref = [[t.clone() for t in ts] for ts in (ys, lds, cgs, xis)]
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        seq()
+    for rep in range(3):
+        for ts in (ys, lds, cgs, xis):
+            for t in ts:
+                t.fill_(-1.0)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        for ts, es in zip((ys, lds, cgs, xis), ref):
+            for a, e in zip(ts, es):
+                assert torch.equal(a, e), f"replay {rep}"
+        # eager work between replays: unrelated torch work, and an eager call
+        # with the captured arguments (a call with other arguments would
+        # redirect the graph, which reads the method's arguments at replay)
+        scratch = torch.randn(1 << 20, dtype=torch.float64, device="cuda")
+        (scratch * 2.0).sum().item()
+        b.EvaluateImpl(xs, ys, lds)
+        torch.cuda.synchronize()
+        for a, e in zip(ys, ref[0]):
+            assert torch.equal(a, e)
+
+
+def test_batch_rejects_mixed_options():
+    _case("batch_mixed")
+    a = mt.CreateTriangular(2, 2, 2, mt.MapOptions())
+    c = mt.CreateTriangular(2, 2, 2, mt.MapOptions(posFuncType=mt.PosFuncTypes.Exp))
+    with pytest.raises(NotImplementedError):
+        mt.MapBatch([a, c])
