@@ -124,6 +124,70 @@ def oracle_check(x_np, coeffs, fused_grad_fn, n=2000):
     return float(max(abs(lg - lr) / abs(lr), np.max(np.abs(gg - gr)) / np.max(np.abs(gr))))
 
 
+def many_maps(steps):
+    """256 independent triangular maps (dims 2..8, orders 2..4, 4096..16384
+    N(0, I) points each, coefficients 0.1 N(0,1); seed-21 stream): Evaluate +
+    LogDeterminant of every map as one batched launch replayed from a HIP graph,
+    against a host loop of per-map launches (generic kernels)."""
+    import torch
+
+    import mpart_amd as mt
+    from mpart_amd.data import Stream
+
+    s = Stream(21)
+    nm = 256
+    u = s.uniform(3 * nm).reshape(nm, 3)
+    maps, xs = [], []
+    for i in range(nm):
+        d = 2 + int(u[i, 0] * 7)
+        p = 2 + int(u[i, 1] * 3)
+        n = 4096 + int(u[i, 2] * (16384 - 4096))
+        m = mt.CreateTriangular(d, d, p, mt.MapOptions())
+        m.SetCoeffs(0.1 * s.normal(m.numCoeffs))
+        m.SetCodegen(False)
+        maps.append(m)
+        xs.append(torch.from_numpy(s.points(d, n)).cuda())
+    npts = sum(x.shape[1] for x in xs)
+    ys = [torch.empty((m.outputDim, x.shape[1]), dtype=torch.float64, device="cuda") for m, x in zip(maps, xs)]
+    lds = [torch.empty(x.shape[1], dtype=torch.float64, device="cuda") for x in xs]
+    b = mt.MapBatch(maps)
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        b.EvaluateImpl(xs, ys, lds)
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        b.EvaluateImpl(xs, ys, lds)
+    g.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.replay()
+    torch.cuda.synchronize()
+    tb = (time.perf_counter() - t0) * 1e3 / steps
+    y_b = [t.clone() for t in ys]
+    for m, x, y, ld in zip(maps, xs, ys, lds):
+        m.EvaluateImpl(x, y)
+        m.LogDeterminantImpl(x, ld)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for m, x, y, ld in zip(maps, xs, ys, lds):
+            m.EvaluateImpl(x, y)
+            m.LogDeterminantImpl(x, ld)
+    torch.cuda.synchronize()
+    tl = (time.perf_counter() - t0) * 1e3 / steps
+    same = all(torch.equal(a, c) for a, c in zip(y_b, ys))
+    log(f"many maps: batched graph {tb:.3f} ms, per-map loop {tl:.3f} ms, identical outputs {same}")
+    return {"maps": nm, "points": int(npts), "batched_graph_ms": tb, "per_map_loop_ms": tl,
+            "batched_points_per_s": npts / (tb * 1e-3), "speedup_batched_vs_loop": tl / tb,
+            "batched_equals_per_map_bitwise": bool(same),
+            "workload": "Evaluate + LogDeterminant of 256 triangular maps, dims 2..8, orders 2..4, "
+                        "4096..16384 N(0,I) points each"}
+
+
 def load_traffic():
     """HBM bytes per fused launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
     summary of this workload (None when absent)."""
@@ -163,6 +227,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-second", action="store_true")
+    ap.add_argument("--no-many", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--profile", action="store_true")
     args = ap.parse_args()
@@ -212,6 +277,10 @@ def main():
         second["data"] = "x ~ N(0, I_8) (seed 1 stream), coefficients 0.1 N(0,1) from the same stream"
         del x2
 
+    many = None
+    if not args.no_many:
+        many = many_maps(max(3, args.plain_steps))
+
     bytes_alg = 8.0 * D * N + 8.0 * (tri.numCoeffs + 1)
     traffic = load_traffic()
     out = {
@@ -243,6 +312,7 @@ def main():
         "fused": head,
         "specialized_kernels": {"active": spec, "compile_s": compile_s, "error": gen_err},
         "active_quadrature_workload": second,
+        "many_maps_workload": many,
     }
     if not args.no_cpu:
         log("cpu baseline")
