@@ -103,7 +103,7 @@ struct PosF {
     double g, gp;
 };
 
-// 1/d for d in [1, 2]: hardware reciprocal estimate refined by two Newton
+// 1/d for d in [1, 2.5]: hardware reciprocal estimate refined by two Newton
 // steps (within an ulp of the correctly rounded quotient; no scaling needed
 // on this range).
 MPD_HD double recip_1_2(double d) {
@@ -113,6 +113,41 @@ MPD_HD double recip_1_2(double d) {
     return fma(r, fma(-d, r, 1.0), r);
 #else
     return 1.0 / d;
+#endif
+}
+
+// log(x) for x in [1, 2] (the SoftPlus argument 1 + exp(-|z|)).
+// Method of fdlibm's __ieee754_log: x = 2^k m with m in [sqrt(1/2), sqrt(2)),
+// f = m - 1 (exact), s = f / (2 + f) and
+//   log(1 + f) = 2 atanh(s) = f - (f^2/2 - s (f^2/2 + R)),  R = sum_{j>=1} 2 s^(2j) / (2j + 1),
+// with the series cut after ten terms (|s| <= 0.1716: the remainder is below
+// 1e-18 of the result) and ln 2 split into a double and its remainder.  Error
+// within about an ulp, against roughly a hundred instructions for the general
+// double-double log.
+MPD_HD double log_1_2(double x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    const bool up = x > 1.4142135623730951;
+    const double m = up ? 0.5 * x : x;
+    const double k = up ? 1.0 : 0.0;
+    const double f = m - 1.0;
+    const double hfsq = 0.5 * f * f;
+    const double s = f * recip_1_2(2.0 + f);
+    const double z = s * s;
+    double R = 0.09523809523809523;  // 2/21
+    R = fma(R, z, 0.10526315789473684);  // 2/19
+    R = fma(R, z, 0.11764705882352941);  // 2/17
+    R = fma(R, z, 0.13333333333333333);  // 2/15
+    R = fma(R, z, 0.15384615384615385);  // 2/13
+    R = fma(R, z, 0.18181818181818182);  // 2/11
+    R = fma(R, z, 0.2222222222222222);   // 2/9
+    R = fma(R, z, 0.2857142857142857);   // 2/7
+    R = fma(R, z, 0.4);                  // 2/5
+    R = fma(R, z, 0.6666666666666666);   // 2/3
+    R *= z;
+    const double ln2Hi = 0.6931471805599453, ln2Lo = 2.3190468138462996e-17;
+    return k * ln2Hi - ((hfsq - (s * (hfsq + R) + k * ln2Lo)) - f);
+#else
+    return log(x);
 #endif
 }
 
@@ -127,7 +162,7 @@ MPD_HD PosF<PK> posfunc(double z) {
         // g' = e / (1 + e) for z < 0, 1 / (1 + e) otherwise.  g' only enters
         // derivatives, so its quotient uses the refined reciprocal.
         double e = exp(-fabs(z));
-        r.g = log(1.0 + e) + fmax(z, 0.0);
+        r.g = log_1_2(1.0 + e) + fmax(z, 0.0);
         const double q = recip_1_2(1.0 + e);
         r.gp = (z < 0) ? e * q : q;
     }
