@@ -199,7 +199,7 @@ extern "C" __global__ void __launch_bounds__(256) mpart_gen_eval(MapDev M, const
         if (ld) {
             const double z = diag_eval<GBK, false, GPM>(dc, xk, nullptr);
             const double g = posfunc<GPK>(z).g;
-            ldsum += (g <= 0) ? -__builtin_inf() : log(g);
+            ldsum += (g <= 0) ? -__builtin_inf() : log_pos(g);
         }
     }
     if (ld) {

@@ -53,7 +53,7 @@ __global__ void __launch_bounds__(256) k_batch(const MapDev* maps, const BatchIO
                 if (B.ld) {
                     const double z = diag_eval<BK, false, PM>(dc, xk, nullptr);
                     const double g = posfunc<PK>(z).g;
-                    ldsum += (g <= 0) ? -__builtin_inf() : log(g);
+                    ldsum += (g <= 0) ? -__builtin_inf() : log_pos(g);
                 }
             }
             if (B.ld) B.ld[n * B.lds] = ldsum;

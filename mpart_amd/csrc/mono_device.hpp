@@ -116,20 +116,15 @@ MPD_HD double recip_1_2(double d) {
 #endif
 }
 
-// log(x) for x in [1, 2] (the SoftPlus argument 1 + exp(-|z|)).
-// Method of fdlibm's __ieee754_log: x = 2^k m with m in [sqrt(1/2), sqrt(2)),
-// f = m - 1 (exact), s = f / (2 + f) and
+// Logarithms by the method of fdlibm's __ieee754_log: x = 2^k m with m in
+// [sqrt(1/2), sqrt(2)), f = m - 1 (exact), s = f / (2 + f) and
 //   log(1 + f) = 2 atanh(s) = f - (f^2/2 - s (f^2/2 + R)),  R = sum_{j>=1} 2 s^(2j) / (2j + 1),
 // with the series cut after ten terms (|s| <= 0.1716: the remainder is below
-// 1e-18 of the result) and ln 2 split into a double and its remainder.  Error
-// within about an ulp, against roughly a hundred instructions for the general
-// double-double log.
-MPD_HD double log_1_2(double x) {
-#ifdef __HIP_DEVICE_COMPILE__
-    const bool up = x > 1.4142135623730951;
-    const double m = up ? 0.5 * x : x;
-    const double k = up ? 1.0 : 0.0;
-    const double f = m - 1.0;
+// 1e-18 of the result) and ln 2 split into a head with 12 trailing zero bits
+// (k * head is exact) and a tail.  Within an ulp of the correctly rounded
+// logarithm, at about a quarter of the instructions of the general
+// double-double device log.
+__device__ __forceinline__ double log_reduced(double f, double k) {
     const double hfsq = 0.5 * f * f;
     const double s = f * recip_1_2(2.0 + f);
     const double z = s * s;
@@ -144,8 +139,32 @@ MPD_HD double log_1_2(double x) {
     R = fma(R, z, 0.4);                  // 2/5
     R = fma(R, z, 0.6666666666666666);   // 2/3
     R *= z;
-    const double ln2Hi = 0.6931471805599453, ln2Lo = 2.3190468138462996e-17;
+    const double ln2Hi = 0.693147180559663, ln2Lo = 2.8235290563031577e-13;
     return k * ln2Hi - ((hfsq - (s * (hfsq + R) + k * ln2Lo)) - f);
+}
+
+// log(x) for x in [1, 2] (the SoftPlus argument 1 + exp(-|z|)).
+MPD_HD double log_1_2(double x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    const bool up = x > 1.4142135623730951;
+    return log_reduced((up ? 0.5 * x : x) - 1.0, up ? 1.0 : 0.0);
+#else
+    return log(x);
+#endif
+}
+
+// log(x) for any x: the reduction above for positive normal numbers, the
+// general device log for zero, subnormals, infinities, NaN and negatives.
+MPD_HD double log_pos(double x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    if (!(x >= 2.2250738585072014e-308 && x <= 1.7976931348623157e308)) return log(x);
+    int e;
+    double m = frexp(x, &e);  // m in [0.5, 1)
+    if (m < 0.7071067811865476) {
+        m *= 2.0;
+        e -= 1;
+    }
+    return log_reduced(m - 1.0, (double)e);
 #else
     return log(x);
 #endif
@@ -831,7 +850,7 @@ MPD_HD void comp_core(const MapDev& M, const CompDev& cd, const double* C, doubl
         AdjAcc<PM> Av;
         T = integrate<BK, PK, QK, FL_VAL, PM>(M.quad, dv, Av, nullptr, &v1) + f0;
     }
-    const double ldv = (gv.g <= 0) ? -__builtin_inf() : log(gv.g);
+    const double ldv = (gv.g <= 0) ? -__builtin_inf() : log_pos(gv.g);
     const double r = gv.gp * (1.0 / gv.g);
     double s = s_in, l = l_in;
     if (klMode) {

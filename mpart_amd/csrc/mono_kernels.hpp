@@ -72,7 +72,7 @@ __global__ void __launch_bounds__(256) k_eval(MapDev M, const double* w, long ws
             make_diag<PM>(M, cd, C, xk, 0.0, dc);
             double z = diag_eval<BK, false, PM>(dc, xk, nullptr);
             double g = posfunc<PK>(z).g;
-            ldsum += (g <= 0) ? -__builtin_inf() : log(g);
+            ldsum += (g <= 0) ? -__builtin_inf() : log_pos(g);
         }
     }
     if (ld) {
