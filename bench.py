@@ -263,6 +263,9 @@ def main():
         return float(l.item()), g.cpu().numpy()
 
     head["oracle_max_rel_err_2000pts"] = oracle_check(x_np, c0, gpu_kl)
+    choice, trials = tri.FusedSchedule()
+    fused_schedule = {"choice": choice, "trial_ms": trials,
+                      "meaning": "bit 0: three-waves-per-SIMD kernel, bit 1: doubled persistent grid"}
 
     second = None
     if not args.no_second:
@@ -310,7 +313,8 @@ def main():
                      "traffic_source": traffic["source"] if traffic else None},
         "cpu_baseline": None,
         "fused": head,
-        "specialized_kernels": {"active": spec, "compile_s": compile_s, "error": gen_err},
+        "specialized_kernels": {"active": spec, "compile_s": compile_s, "error": gen_err,
+                                "fused_schedule": fused_schedule},
         "active_quadrature_workload": second,
         "many_maps_workload": many,
     }

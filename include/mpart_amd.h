@@ -235,6 +235,16 @@ int mpart_set_codegen(mpart_map map, int enable);
 int mpart_codegen_prepare(mpart_map map, int32_t* specialized, double* compileSeconds);
 const char* mpart_codegen_error(mpart_map map);
 
+/* Schedule of the specialized fused KL / VJP kernel.  The first such call on
+ * a map with at least 65536 points (outside stream capture) times four
+ * schedules -- {compiler-chosen occupancy, three waves per SIMD} x
+ * {persistent grid of one or two blocks per resident slot} -- twice each on
+ * the caller's stream and keeps the fastest for the map.  *choice is the
+ * schedule index (bit 0: three-waves kernel, bit 1: doubled grid) or -1
+ * before tuning; trialMs (4 doubles, may be NULL) receives the trial times in
+ * milliseconds.  No reference counterpart. */
+int mpart_fused_schedule(mpart_map map, int32_t* choice, double* trialMs);
+
 #ifdef __cplusplus
 }
 #endif

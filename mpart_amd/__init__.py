@@ -95,6 +95,7 @@ _SIGS = {
     "mpart_batch_coeff_grad": [_vp, _vp, _vp, _vp, _vp],
     "mpart_batch_inverse": [_vp, _vp, _vp, _vp, _dbl, _dbl, _vp],
     "mpart_codegen_prepare": [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)],
+    "mpart_fused_schedule": [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)],
 }
 for _name, _args in _SIGS.items():
     _f = getattr(_lib, _name)
@@ -354,6 +355,13 @@ class ConditionalMap:
         secs = ctypes.c_double(0.0)
         _check(_lib.mpart_codegen_prepare(self._h, ctypes.byref(spec), ctypes.byref(secs)))
         return bool(spec.value), float(secs.value), _lib.mpart_codegen_error(self._h).decode()
+
+    def FusedSchedule(self):
+        """(schedule index or -1 before tuning, trial times in ms) of the fused kernel."""
+        c = ctypes.c_int32(-1)
+        t = (ctypes.c_double * 4)()
+        _check(_lib.mpart_fused_schedule(self._h, ctypes.byref(c), t))
+        return int(c.value), list(t)
 
     # Impl methods (device tensors, preallocated outputs) --------------------
     def EvaluateImpl(self, pts, out):

@@ -80,12 +80,11 @@ std::string generate_source(const GenSpec& s) {
     const unsigned ncomp = unsigned(s.comps.size());
 
     // ---------------- fused forward + adjoint (KL / VJP) ----------------
-    if (s.fusedWavesPerEU > 0)
-        o << "#define GFUSED_ATTR __attribute__((amdgpu_waves_per_eu(" << s.fusedWavesPerEU << ")))\n";
-    else
-        o << "#define GFUSED_ATTR\n";
+    // The body is one device function; two kernels instantiate it with
+    // different occupancy targets (the compiler's choice, and three waves per
+    // SIMD), between which the host picks by timing the first calls.
     o << R"(
-extern "C" __global__ void __launch_bounds__(256) GFUSED_ATTR mpart_gen_fused(MapDev M, const double* __restrict__ w,
+__device__ __forceinline__ void fused_body(const MapDev& M, const double* __restrict__ w,
         const double* __restrict__ pts, long prs, long pcs, long N, const double* sens, long srs, long scs,
         const double* lsens, long lss, int klMode, double* partial) {
     extern __shared__ double acc[];
@@ -163,6 +162,15 @@ extern "C" __global__ void __launch_bounds__(256) GFUSED_ATTR mpart_gen_fused(Ma
         for (int i = 0; i < nwaves; ++i) t += acc[(long)i * K + k];
         row[k] = t;
     }
+}
+#define MPD_FUSED_ARGS MapDev M, const double* __restrict__ w, const double* __restrict__ pts, long prs, long pcs, \
+        long N, const double* sens, long srs, long scs, const double* lsens, long lss, int klMode, double* partial
+extern "C" __global__ void __launch_bounds__(256) mpart_gen_fused(MPD_FUSED_ARGS) {
+    fused_body(M, w, pts, prs, pcs, N, sens, srs, scs, lsens, lss, klMode, partial);
+}
+extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+mpart_gen_fused_w3(MPD_FUSED_ARGS) {
+    fused_body(M, w, pts, prs, pcs, N, sens, srs, scs, lsens, lss, klMode, partial);
 }
 )";
 
@@ -334,6 +342,7 @@ GenModule* get_module(const GenSpec& spec, std::string& err) {
         return nullptr;
     }
     if (hipModuleGetFunction(&m->fused, m->mod, "mpart_gen_fused") != hipSuccess ||
+        hipModuleGetFunction(&m->fused3, m->mod, "mpart_gen_fused_w3") != hipSuccess ||
         hipModuleGetFunction(&m->eval, m->mod, "mpart_gen_eval") != hipSuccess ||
         hipModuleGetFunction(&m->coef, m->mod, "mpart_gen_coef") != hipSuccess ||
         hipModuleGetFunction(&m->ldcoef, m->mod, "mpart_gen_ldcoef") != hipSuccess) {

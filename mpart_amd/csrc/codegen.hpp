@@ -29,12 +29,11 @@ struct GenSpec {
     unsigned numCoeffs;
     std::vector<int> dimDeg;  // largest off-diagonal order per coordinate
     std::vector<GenComp> comps;
-    int fusedWavesPerEU = 0;  // occupancy hint for the fused kernel (0: compiler default)
 };
 
 struct GenModule {
     hipModule_t mod = nullptr;
-    hipFunction_t fused = nullptr, eval = nullptr, coef = nullptr, ldcoef = nullptr;
+    hipFunction_t fused = nullptr, fused3 = nullptr, eval = nullptr, coef = nullptr, ldcoef = nullptr;
     double compileSeconds = 0.0;
 };
 

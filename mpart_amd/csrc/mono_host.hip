@@ -209,8 +209,10 @@ struct mpart_map_s {
     mpg::GenModule* gen = nullptr;
     double genSeconds = 0.0;
     std::string genErr;
-    int genOcc = 0;
+    int genOccK[2] = {0, 0};
     size_t genOccSmem = 0;
+    int fusedChoice = -1;  // tuned schedule of the fused kernel (-1: not yet)
+    double fusedTrialMs[4] = {0, 0, 0, 0};
 };
 
 namespace {
@@ -487,6 +489,7 @@ void check_view(const char* what, int64_t rows, int64_t cols, int64_t erows, int
 // ---------------------------------------------------------------------------
 // Runtime-specialized kernels.
 constexpr unsigned GEN_MAX_COEFFS = 4096;
+constexpr long FUSED_TUNE_MIN_POINTS = 65536;
 
 bool codegen_env_enabled() {
     static const bool on = [] {
@@ -753,32 +756,69 @@ void do_fused(mpart_map_s& m, const mpart_cview& p, const double* sens, long srs
         int gblock = 256;
         while (gblock > 64 && (size_t)(gblock / 64) * K * sizeof(double) > LDS_LIMIT) gblock /= 2;
         const size_t gsmem = (size_t)(gblock / 64) * K * sizeof(double);
-        if (m.genOcc == 0 || m.genOccSmem != gsmem) {
-            int nb = 0;
-            hip_check(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nb, g->fused, gblock, gsmem),
-                      "hipModuleOccupancyMaxActiveBlocksPerMultiprocessor");
-            m.genOcc = std::max(1, nb);
+        hipFunction_t fns[2] = {g->fused, g->fused3};
+        if (m.genOccSmem != gsmem) {
+            for (int k = 0; k < 2; ++k) {
+                int nb = 0;
+                hip_check(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fns[k], gblock, gsmem),
+                          "hipModuleOccupancyMaxActiveBlocksPerMultiprocessor");
+                m.genOccK[k] = std::max(1, nb);
+            }
             m.genOccSmem = gsmem;
         }
         const long ntiles = std::max<long>(1, (N + gblock - 1) / gblock);
-        const long grid = std::min<long>(ntiles, (long)g_numCU * m.genOcc);
-        m.partialGrad.ensure(std::max<size_t>(8, (size_t)grid * (K + 1) * sizeof(double)));
-        if (N > 0) {
-            const double* w = gen_coeffs(m, s);
-            MapDev M = m.dev;
-            const double* pts = p.ptr;
-            long prs = p.row_stride, pcs = p.col_stride, NN = N;
-            const double* sp = sens;
-            long srs_ = srs, scs_ = scs;
-            const double* lp = lsens;
-            long lss_ = lss;
-            int kl = klMode;
-            double* part = m.partialGrad.as<double>();
-            void* args[] = {&M, &w, &pts, &prs, &pcs, &NN, &sp, &srs_, &scs_, &lp, &lss_, &kl, &part};
-            gen_launch(g->fused, grid, gblock, gsmem, s, args, "launch specialized fused");
+        // Schedule cfg: kernel (cfg & 1: compiler occupancy / three waves per
+        // SIMD) and persistent-grid multiple ((cfg >> 1) + 1).
+        auto grid_of = [&](int cfg) {
+            return std::min<long>(ntiles, (long)g_numCU * m.genOccK[cfg & 1] * ((cfg >> 1) + 1));
+        };
+        long maxGrid = 1;
+        for (int cfg = 0; cfg < 4; ++cfg) maxGrid = std::max(maxGrid, grid_of(cfg));
+        m.partialGrad.ensure(std::max<size_t>(8, (size_t)maxGrid * (K + 1) * sizeof(double)));
+        const double* w = gen_coeffs(m, s);
+        MapDev M = m.dev;
+        const double* pts = p.ptr;
+        long prs = p.row_stride, pcs = p.col_stride, NN = N;
+        const double* sp = sens;
+        long srs_ = srs, scs_ = scs;
+        const double* lp = lsens;
+        long lss_ = lss;
+        int kl = klMode;
+        double* part = m.partialGrad.as<double>();
+        void* args[] = {&M, &w, &pts, &prs, &pcs, &NN, &sp, &srs_, &scs_, &lp, &lss_, &kl, &part};
+        auto launch_cfg = [&](int cfg) {
+            gen_launch(fns[cfg & 1], grid_of(cfg), gblock, gsmem, s, args, "launch specialized fused");
+        };
+        // Autotuning: on the first call with enough points (and not under
+        // stream capture) every schedule runs twice and the faster one is kept.
+        if (m.fusedChoice < 0 && N >= FUSED_TUNE_MIN_POINTS) {
+            hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+            hip_check(hipStreamIsCapturing(s, &cs), "hipStreamIsCapturing");
+            if (cs == hipStreamCaptureStatusNone) {
+                hipEvent_t e0, e1;
+                hip_check(hipEventCreate(&e0), "hipEventCreate");
+                hip_check(hipEventCreate(&e1), "hipEventCreate");
+                int best = 0;
+                for (int cfg = 0; cfg < 4; ++cfg) {
+                    launch_cfg(cfg);
+                    hip_check(hipEventRecord(e0, s), "hipEventRecord");
+                    launch_cfg(cfg);
+                    hip_check(hipEventRecord(e1, s), "hipEventRecord");
+                    hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+                    float ms = 0.f;
+                    hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+                    m.fusedTrialMs[cfg] = ms;
+                    if (ms < m.fusedTrialMs[best]) best = cfg;
+                }
+                (void)hipEventDestroy(e0);
+                (void)hipEventDestroy(e1);
+                m.fusedChoice = best;
+            }
         }
+        const int cfg = m.fusedChoice < 0 ? 0 : m.fusedChoice;
+        if (N > 0) launch_cfg(cfg);
         // One pass over the partial rows: K gradient columns, then the loss column.
-        const long rows = (N > 0) ? grid : 0;
+        const long rows = (N > 0) ? grid_of(cfg) : 0;
         hip_check(mpd::launch_reduce_cols(m.partialGrad.as<double>(), rows, K + 1, loss ? K + 1 : K, scale, grad, loss,
                                           s),
                   "launch reduce");
@@ -1244,6 +1284,15 @@ int mpart_codegen_prepare(mpart_map m, int32_t* specialized, double* compileSeco
 }
 
 const char* mpart_codegen_error(mpart_map m) { return m ? m->genErr.c_str() : "null map handle"; }
+
+int mpart_fused_schedule(mpart_map m, int32_t* choice, double* trialMs) {
+    return guarded([&] {
+        check_map(m);
+        if (choice) *choice = m->fusedChoice;
+        if (trialMs)
+            for (int i = 0; i < 4; ++i) trialMs[i] = m->fusedTrialMs[i];
+    });
+}
 
 int mpart_create_batch(const mpart_map* maps, uint32_t n, mpart_batch* out) {
     return guarded([&] {

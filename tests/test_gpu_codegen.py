@@ -123,3 +123,23 @@ def test_large_map_falls_back_to_generic():
     g.SetCoeffs(c)
     r.SetCoeffs(c)
     close(g.Evaluate(x), r.Evaluate(x), what="Evaluate")
+
+
+def test_fused_schedule_autotuning():
+    torch = pytest.importorskip("torch")
+    _case("codegen_autotune")
+    g = mt.CreateTriangular(8, 8, 3, mt.MapOptions())
+    s = Stream(17)
+    x = torch.from_numpy(s.points(8, 70000)).cuda()
+    g.SetCoeffs(0.1 * s.normal(g.numCoeffs))
+    assert g.FusedSchedule()[0] == -1
+    l1, g1 = g.KLValueAndGrad(x)
+    choice, trials = g.FusedSchedule()
+    print(f"[autotune] choice {choice}, trial ms {['%.3f' % t for t in trials]}", flush=True)
+    assert 0 <= choice <= 3 and all(t > 0 for t in trials)
+    lp, gp = g.KLValueAndGrad(x, plain=True)
+    close(l1.item(), lp.item(), rtol=1e-12, what="loss")
+    close(g1.cpu().numpy(), gp.cpu().numpy(), rtol=1e-12, what="grad")
+    l2, g2 = g.KLValueAndGrad(x)  # the tuned schedule, deterministic from here on
+    l3, g3 = g.KLValueAndGrad(x)
+    assert torch.equal(g2, g3) and torch.equal(l2, l3)
