@@ -25,6 +25,12 @@ hipError_t launch_coeff_grad(int var, const Launch& L, const MapDev& M, const do
 hipError_t launch_logdet_coeff_grad(int var, const Launch& L, const MapDev& M, const double* w, long ws,
                                     const double* pts, long prs, long pcs, long N, double* out, long ors, long ocs,
                                     int c, double* gtab);
+hipError_t launch_logdet_discrete(int var, const Launch& L, const MapDev& M, const double* w, long ws,
+                                  const double* pts, long prs, long pcs, long N, double* ld, long lds, int ldAcc,
+                                  int cb, int ce, double* gtab);
+hipError_t launch_logdet_coeff_grad_discrete(int var, const Launch& L, const MapDev& M, const double* w, long ws,
+                                             const double* pts, long prs, long pcs, long N, double* out, long ors,
+                                             long ocs, int c, double* gtab);
 hipError_t launch_input_grad(int var, const Launch& L, const MapDev& M, const double* w, long ws, const double* pts,
                              long prs, long pcs, long N, const double* sens, long srs, long scs, double* out, long ors,
                              long ocs, int c, int mode, double* gtab);

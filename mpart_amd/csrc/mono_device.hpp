@@ -43,7 +43,9 @@ constexpr uint32_t DUMMY = 0xFFFFFFFFu;
 enum BasisKind { BK_PH = 0, BK_HF = 1 };
 enum PosKind { PK_EXP = 0, PK_SP = 1 };
 enum QuadKind { QK_AS = 0, QK_ACC5 = 1, QK_ACC = 2 };
-enum Flavor { FL_VAL = 0, FL_COEF = 1, FL_INPUT = 2 };
+// Integrand flavours: value only; coefficient adjoint; input adjoint (and the
+// x_k-derivative U); discrete-derivative mode (U and its coefficient adjoint).
+enum Flavor { FL_VAL = 0, FL_COEF = 1, FL_INPUT = 2, FL_DCOEF = 3 };
 
 // Nested Clenshaw-Curtis pair and adaptive parameters.
 struct QuadDev {
@@ -168,6 +170,27 @@ MPD_HD double log_pos(double x) {
 #else
     return log(x);
 #endif
+}
+
+// g'' as well (SecondDerivative, MParT/PositiveBijectors.h:25-28, 48-50):
+// SoftPlus g'' = g' (1 - g'), Exp g'' = exp.
+template <int PK>
+struct PosF2 {
+    double g, gp, gpp;
+};
+
+template <int PK>
+MPD_HD PosF<PK> posfunc(double z);
+
+template <int PK>
+MPD_HD PosF2<PK> posfunc2(double z) {
+    const PosF<PK> a = posfunc<PK>(z);
+    PosF2<PK> r;
+    r.g = a.g;
+    r.gp = a.gp;
+    if constexpr (PK == PK_EXP) r.gpp = a.g;
+    else r.gpp = a.gp * (1.0 - a.gp);
+    return r;
 }
 
 template <int PK>
@@ -589,15 +612,27 @@ template <>
 struct NodeV<FL_INPUT> {
     double F, h, g1;
 };
+template <>
+struct NodeV<FL_DCOEF> {
+    double F, gp, gpp, g1;
+};
 
 template <int BK, int PK, int FL, int PM>
 MPD_HD NodeV<FL> eval_node(const DiagCoef<PM>& dc, double t) {
     double z = diag_eval<BK, false, PM>(dc, t * dc.x, nullptr);
-    PosF<PK> gv = posfunc<PK>(z);
     NodeV<FL> v;
-    v.F = dc.x * (gv.g + dc.nug);
-    if constexpr (FL >= FL_COEF) v.h = dc.x * gv.gp;
-    if constexpr (FL == FL_INPUT) v.g1 = gv.g + dc.nug;
+    if constexpr (FL == FL_DCOEF) {
+        const PosF2<PK> gv = posfunc2<PK>(z);
+        v.F = dc.x * (gv.g + dc.nug);
+        v.gp = gv.gp;
+        v.gpp = gv.gpp;
+        v.g1 = gv.g + dc.nug;
+    } else {
+        PosF<PK> gv = posfunc<PK>(z);
+        v.F = dc.x * (gv.g + dc.nug);
+        if constexpr (FL >= FL_COEF) v.h = dc.x * gv.gp;
+        if constexpr (FL == FL_INPUT) v.g1 = gv.g + dc.nug;
+    }
     return v;
 }
 
@@ -610,7 +645,26 @@ struct AdjAcc {
 // Adds w * v(t) at one node to the adjoint accumulators.
 template <int BK, int FL, int PM>
 MPD_HD void adj_node(const DiagCoef<PM>& dc, double t, double w, const NodeV<FL>& v, AdjAcc<PM>& A) {
-    if constexpr (FL >= FL_COEF) {
+    if constexpr (FL == FL_DCOEF) {
+        // Diagonal and MixedCoeff integrand modes (MonotoneIntegrand.h:148-165,
+        // 193-202): U += (g + nugget) + x t g' d2f, and
+        // S'_q += (g' + x t g'' d2f) B'_q + x t g' B''_q at y = t x.
+        const double y = t * dc.x;
+        double z2;
+        diag_eval<BK, true, PM>(dc, y, &z2);
+        double bq[PM + 1], bq2[PM + 1];
+        diag_basis<BK, PM>(dc.p, y, bq);
+        diag_basis2<BK, PM>(dc.p, y, bq2);
+        const double xt = dc.x * t;
+        A.U += w * (v.g1 + xt * v.gp * z2);
+        const double a = v.gp + xt * z2 * v.gpp;
+        const double b = xt * v.gp;
+#pragma unroll
+        for (int q = 1; q <= PM; ++q) {
+            if (q > dc.p) break;
+            A.S[q] += w * (a * bq[q] + b * bq2[q]);
+        }
+    } else if constexpr (FL >= FL_COEF) {
         double bq[PM + 1];
         const double y = t * dc.x;
         diag_basis<BK, PM>(dc.p, y, bq);

@@ -592,8 +592,23 @@ void do_evaluate(mpart_map_s& m, const mpart_cview& p, const mpart_view& o, int3
 void do_logdet(mpart_map_s& m, const mpart_cview& p, double* out, long os, hipStream_t s) {
     check_coeffs(&m);
     check_view("LogDeterminant: pts", p.rows, p.cols, m.inDim, p.cols);
-    if (!m.opts.contDeriv) throw NotImpl("LogDeterminant with contDeriv = false is not provided yet");
     if (p.cols == 0) return;
+    if (!m.opts.contDeriv) {
+        // DiscreteDerivative (MonotoneComponent.h:586-638), generic kernels.
+        Shape sh = shape_for(m, p.cols, s);
+        if (m.plain) {
+            for (int c = 0; c < m.dev.ncomp; ++c)
+                hip_check(mpd::launch_logdet_discrete(m.variant, sh.L, m.dev, m.coeffs, m.coeffStride, p.ptr,
+                                                      p.row_stride, p.col_stride, p.cols, out, os, c > 0 ? 1 : 0, c,
+                                                      c + 1, sh.gtab),
+                          "launch logdet (discrete)");
+        } else {
+            hip_check(mpd::launch_logdet_discrete(m.variant, sh.L, m.dev, m.coeffs, m.coeffStride, p.ptr, p.row_stride,
+                                                  p.col_stride, p.cols, out, os, 0, 0, m.dev.ncomp, sh.gtab),
+                      "launch logdet (discrete)");
+        }
+        return;
+    }
     if (mpg::GenModule* g = gen_for(m)) {
         const double* w = gen_coeffs(m, s);
         if (m.plain) {
@@ -652,8 +667,17 @@ void do_logdet_coeff_grad(mpart_map_s& m, const mpart_cview& p, const mpart_view
     check_coeffs(&m);
     check_view("LogDeterminantCoeffGrad: pts", p.rows, p.cols, m.inDim, p.cols);
     check_view("LogDeterminantCoeffGrad: output", o.rows, o.cols, m.numCoeffs, p.cols);
-    if (!m.opts.contDeriv) throw NotImpl("LogDeterminantCoeffGrad with contDeriv = false is not provided yet");
     if (p.cols == 0) return;
+    if (!m.opts.contDeriv) {
+        // DiscreteMixedJacobian / DiscreteDerivative (MonotoneComponent.h:185-206, 928-980).
+        Shape sh = shape_for(m, p.cols, s);
+        for (int c = 0; c < m.dev.ncomp; ++c)
+            hip_check(mpd::launch_logdet_coeff_grad_discrete(m.variant, sh.L, m.dev, m.coeffs, m.coeffStride, p.ptr,
+                                                             p.row_stride, p.col_stride, p.cols, o.ptr, o.row_stride,
+                                                             o.col_stride, c, sh.gtab),
+                      "launch logdet_coeff_grad (discrete)");
+        return;
+    }
     if (mpg::GenModule* g = gen_for(m)) {
         const double* w = gen_coeffs(m, s);
         MapDev M = m.dev;
@@ -860,6 +884,44 @@ void do_fused(mpart_map_s& m, const mpart_cview& p, const double* sens, long srs
     if (loss)
         hip_check(mpd::launch_reduce_rows(m.partialLoss.as<double>(), N > 0 ? grid : 0, 1, scale, loss, s),
                   "launch reduce");
+}
+
+// The KL objective with the reference's launch structure
+// (src/MapObjective.cpp:107-215): per-component Evaluate, LogDeterminant,
+// CoeffGrad(sens = T) and LogDeterminantCoeffGrad with stored Jacobians,
+// then row reductions.
+void do_kl_plain(mpart_map_s& m, const mpart_cview& pts, double* loss, double* grad, hipStream_t st) {
+    check_coeffs(&m);
+    const long N = pts.cols, K = m.numCoeffs, Mo = m.outDim;
+    check_view("KL: pts", pts.rows, pts.cols, m.inDim, N);
+    if (N == 0) throw ArgError("KL: no points");
+    m.wT.ensure((size_t)Mo * N * sizeof(double));
+    m.wLd.ensure((size_t)N * sizeof(double));
+    m.wG.ensure((size_t)K * N * sizeof(double));
+    m.wLG.ensure((size_t)K * N * sizeof(double));
+    const bool plainSave = m.plain;
+    m.plain = true;
+    mpart_view T{m.wT.as<double>(), Mo, N, N, 1};
+    mpart_cview Tc{m.wT.as<double>(), Mo, N, N, 1};
+    mpart_view G{m.wG.as<double>(), K, N, N, 1};
+    mpart_view LG{m.wLG.as<double>(), K, N, N, 1};
+    try {
+        do_evaluate(m, pts, T, nullptr, st);
+        do_logdet(m, pts, m.wLd.as<double>(), 1, st);
+        do_coeff_grad(m, pts, Tc, G, st);
+        do_logdet_coeff_grad(m, pts, LG, st);
+    } catch (...) {
+        m.plain = plainSave;
+        throw;
+    }
+    m.plain = plainSave;
+    hip_check(mpd::launch_kl_rows(m.wG.as<double>(), m.wLG.as<double>(), N, K, grad, st), "launch kl_rows");
+    const long nb = (N + 255) / 256;
+    m.partialLoss.ensure(std::max<size_t>(8, (size_t)nb * sizeof(double)));
+    hip_check(mpd::launch_kl_loss(m.wT.as<double>(), Mo, m.wLd.as<double>(), N, m.partialLoss.as<double>(), st),
+              "launch kl_loss");
+    hip_check(mpd::launch_reduce_rows(m.partialLoss.as<double>(), nb, 1, 1.0 / (double)N, loss, st),
+              "launch reduce");
 }
 
 // ---------------------------------------------------------------------------
@@ -1204,7 +1266,12 @@ int mpart_kl_value_and_grad(mpart_map m, mpart_cview pts, double* loss, double* 
         check_map(m);
         check_view("KL: pts", pts.rows, pts.cols, m->inDim, pts.cols);
         if (pts.cols == 0) throw ArgError("KL: no points");
-        if (!m->opts.contDeriv) throw NotImpl("KL with contDeriv = false is not provided yet");
+        if (!m->opts.contDeriv) {
+            // The fused pass implements the continuous derivative; the
+            // discrete-derivative objective runs the per-method kernels.
+            do_kl_plain(*m, pts, loss, grad, S(s));
+            return;
+        }
         do_fused(*m, pts, nullptr, 0, 0, nullptr, 0, 1, loss, grad, 1.0 / (double)pts.cols, S(s));
     });
 }
@@ -1223,39 +1290,7 @@ int mpart_coeff_vjp(mpart_map m, mpart_cview pts, mpart_cview sens, const double
 int mpart_kl_value_and_grad_plain(mpart_map m, mpart_cview pts, double* loss, double* grad, mpart_stream s) {
     return guarded([&] {
         check_map(m);
-        check_coeffs(m);
-        const long N = pts.cols, K = m->numCoeffs, Mo = m->outDim;
-        check_view("KL: pts", pts.rows, pts.cols, m->inDim, N);
-        if (N == 0) throw ArgError("KL: no points");
-        if (!m->opts.contDeriv) throw NotImpl("KL with contDeriv = false is not provided yet");
-        hipStream_t st = S(s);
-        m->wT.ensure((size_t)Mo * N * sizeof(double));
-        m->wLd.ensure((size_t)N * sizeof(double));
-        m->wG.ensure((size_t)K * N * sizeof(double));
-        m->wLG.ensure((size_t)K * N * sizeof(double));
-        const bool plainSave = m->plain;
-        m->plain = true;
-        mpart_view T{m->wT.as<double>(), Mo, N, N, 1};
-        mpart_cview Tc{m->wT.as<double>(), Mo, N, N, 1};
-        mpart_view G{m->wG.as<double>(), K, N, N, 1};
-        mpart_view LG{m->wLG.as<double>(), K, N, N, 1};
-        try {
-            do_evaluate(*m, pts, T, nullptr, st);
-            do_logdet(*m, pts, m->wLd.as<double>(), 1, st);
-            do_coeff_grad(*m, pts, Tc, G, st);
-            do_logdet_coeff_grad(*m, pts, LG, st);
-        } catch (...) {
-            m->plain = plainSave;
-            throw;
-        }
-        m->plain = plainSave;
-        hip_check(mpd::launch_kl_rows(m->wG.as<double>(), m->wLG.as<double>(), N, K, grad, st), "launch kl_rows");
-        const long nb = (N + 255) / 256;
-        m->partialLoss.ensure(std::max<size_t>(8, (size_t)nb * sizeof(double)));
-        hip_check(mpd::launch_kl_loss(m->wT.as<double>(), Mo, m->wLd.as<double>(), N, m->partialLoss.as<double>(), st),
-                  "launch kl_loss");
-        hip_check(mpd::launch_reduce_rows(m->partialLoss.as<double>(), nb, 1, 1.0 / (double)N, loss, st),
-                  "launch reduce");
+        do_kl_plain(*m, pts, loss, grad, S(s));
     });
 }
 

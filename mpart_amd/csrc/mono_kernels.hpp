@@ -157,6 +157,73 @@ __global__ void __launch_bounds__(256) k_logdet_coeff_grad(MapDev M, const doubl
 }
 
 // ---------------------------------------------------------------------------
+// Discrete-derivative mode (contDeriv = false).  LogDeterminant uses
+// DiscreteDerivative (MonotoneComponent.h:586-638): dT/dx_k as the integral
+// U = int_0^1 (g + nugget) + x t g' d2f dt on the tree of x (g + nugget);
+// logdet is the sum of log U over components [cb, ce).
+template <int BK, int PK, int QK, int PM>
+__global__ void __launch_bounds__(256) k_logdet_discrete(MapDev M, const double* w, long ws, PtsArg P, double* ld,
+                                                         long lds, int ldAccumulate, int cb, int ce, double* gtab) {
+    extern __shared__ double smem[];
+    const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= P.N) return;
+    long stride;
+    double* tab = lane_table(smem, gtab, stride);
+    const double* pt = P.p + n * P.cs;
+    fill_values<BK>(M, table_dims(M, cb, ce), pt, P.rs, tab, (int)stride);
+    double ldsum = 0.0;
+    for (int c = cb; c < ce; ++c) {
+        const CompDev cd = M.comps[c];
+        double C[PM + 1];
+        coeff_groups<PM>(M, cd, w + (long)cd.coeffOff * ws, ws, tab, (int)stride, C);
+        DiagCoef<PM> dc;
+        make_diag<PM>(M, cd, C, pt[(long)(cd.dim - 1) * P.rs], M.nugget, dc);
+        AdjAcc<PM> A;
+        zero_adj<PM>(A);
+        integrate<BK, PK, QK, FL_INPUT, PM>(M.quad, dc, A, nullptr);
+        ldsum += (A.U <= 0) ? -__builtin_inf() : log_pos(A.U);
+    }
+    if (ldAccumulate) ld[n * lds] += ldsum;
+    else ld[n * lds] = ldsum;
+}
+
+// LogDeterminantCoeffGrad of one component without contDeriv
+// (DiscreteMixedJacobian + DiscreteDerivative, MonotoneComponent.h:185-206,
+// 928-980): P_j rho_q S'_q * (1 / U) with S'_q of the MixedCoeff integrand.
+template <int BK, int PK, int QK, int PM>
+__global__ void __launch_bounds__(256) k_logdet_coeff_grad_discrete(MapDev M, const double* w, long ws, PtsArg P,
+                                                                    OutArg out, int c, double* gtab) {
+    extern __shared__ double smem[];
+    const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= P.N) return;
+    long stride;
+    double* tab = lane_table(smem, gtab, stride);
+    const double* pt = P.p + n * P.cs;
+    const CompDev cd = M.comps[c];
+    fill_values<BK>(M, cd.dim - 1, pt, P.rs, tab, (int)stride);
+    double C[PM + 1];
+    coeff_groups<PM>(M, cd, w + (long)cd.coeffOff * ws, ws, tab, (int)stride, C);
+    DiagCoef<PM> dc;
+    make_diag<PM>(M, cd, C, pt[(long)(cd.dim - 1) * P.rs], M.nugget, dc);
+    AdjAcc<PM> A;
+    zero_adj<PM>(A);
+    integrate<BK, PK, QK, FL_DCOEF, PM>(M.quad, dc, A, nullptr);
+    const double inv = 1.0 / A.U;
+    double G[PM + 1];
+#pragma unroll
+    for (int q = 0; q <= PM; ++q) G[q] = (q <= cd.p) ? M.rho[q] * A.S[q] : 0.0;
+    double* o = out.p + (long)cd.coeffOff * out.rs + n * out.cs;
+    struct Op {
+        double* o;
+        long rs;
+        const double* G;
+        double inv;
+        __device__ void operator()(uint32_t idx, double Pj, int q) { o[(long)idx * rs] = (Pj * G[q]) * inv; }
+    } op{o, out.rs, G, inv};
+    visit_terms<PM>(M, cd, tab, (int)stride, op);
+}
+
+// ---------------------------------------------------------------------------
 // Input gradients of one component, accumulated into out (inDim x N):
 //  mode 0: Gradient      sens_n * dT/dx  (InputJacobian, MonotoneComponent.h:148-164, 758-818)
 //  mode 1: LogDeterminantInputGrad       (ContinuousMixedInputJacobian, 208-232, 877-926)

@@ -272,3 +272,32 @@ def test_c3_sample_kl(variant):
     close(gf, gr, what="C3 grad")
     close(lp, lr, what="C3 loss plain")
     close(gp, gr, what="C3 grad plain")
+
+
+DISCRETE_CASES = [
+    dict(contDeriv=False),
+    dict(contDeriv=False, posFuncType="Exp", basisType="HermiteFunctions"),
+    dict(contDeriv=False, basisNorm=False, nugget=0.05),
+    dict(contDeriv=False, quadType="AdaptiveClenshawCurtis", quadAbsTol=1e-8, quadRelTol=1e-8),
+]
+
+
+@pytest.mark.parametrize("ci", range(len(DISCRETE_CASES)))
+def test_discrete_derivative_mode(ci):
+    # contDeriv = false: LogDeterminant from DiscreteDerivative and its
+    # coefficient gradient from DiscreteMixedJacobian (MonotoneComponent.h:586-638, 928-980).
+    case = DISCRETE_CASES[ci]
+    _print_case("discrete", case=case)
+    g, r, x, _ = _pair(case, 3, 3, 950 + ci, n=600, triangular=True, coeff_scale=0.1)
+    close(g.Evaluate(x), r.Evaluate(x), what="Evaluate")
+    close(g.LogDeterminant(x), r.LogDeterminant(x), what="LogDeterminant (discrete)")
+    close(g.LogDeterminantCoeffGrad(x), r.LogDeterminantCoeffGrad(x), what="LogDeterminantCoeffGrad (discrete)")
+    g.SetPlainLaunches(True)
+    close(g.LogDeterminant(x), r.LogDeterminant(x), what="LogDeterminant (discrete, per component)")
+    g.SetPlainLaunches(False)
+    lr, gr = r.KL(x)
+    lf, gf = g.KLValueAndGrad(x)
+    close(lf, lr, what="KL loss (discrete)")
+    close(gf, gr, what="KL grad (discrete)")
+    with pytest.raises(ValueError):
+        g.LogDeterminantInputGrad(x)
