@@ -188,6 +188,54 @@ def many_maps(steps):
                         "4096..16384 N(0,I) points each"}
 
 
+def single_component_workload(steps):
+    """One component, dim 6, total order 4 (210 terms), ProbabilistHermite /
+    SoftPlus / adaptive Clenshaw-Curtis (quadPts 5, tolerances 1e-8);
+    N = 1,000,000 points x ~ N(0, I_6), coefficients ~ N(0, 0.05^2), targets
+    r ~ N(0, 1) from one seed-0 stream.  Times Evaluate, LogDeterminant,
+    CoeffGrad (sens = 1, 210 x N output) and Inverse (x_{1:5} fixed, x_6 as
+    the initial guess) with preallocated outputs."""
+    import torch
+
+    import mpart_amd as mt
+    from mpart_amd.data import Stream
+
+    opts = mt.MapOptions(quadType=mt.QuadTypes.AdaptiveClenshawCurtis, quadPts=5, quadAbsTol=1e-8, quadRelTol=1e-8)
+    g = mt.CreateTotalOrderComponent(6, 4, opts)
+    s = Stream(0)
+    N = 1_000_000
+    x = torch.from_numpy(s.points(6, N)).cuda()
+    g.SetCoeffs(0.05 * s.normal(g.numCoeffs))
+    r = torch.from_numpy(s.normal(N)[None, :].copy()).cuda()
+    g.PrepareCodegen()
+    y = torch.empty((1, N), dtype=torch.float64, device="cuda")
+    ld = torch.empty(N, dtype=torch.float64, device="cuda")
+    sens = torch.ones((1, N), dtype=torch.float64, device="cuda")
+    cg = torch.empty((g.numCoeffs, N), dtype=torch.float64, device="cuda")
+    xi = torch.empty((1, N), dtype=torch.float64, device="cuda")
+    calls = {
+        "evaluate": lambda: g.EvaluateImpl(x, y),
+        "log_determinant": lambda: g.LogDeterminantImpl(x, ld),
+        "coeff_grad": lambda: g.CoeffGradImpl(x, sens, cg),
+        "inverse": lambda: g.InverseImpl(x, r, xi),
+    }
+    out = {"workload": "one component, dim 6, total order 4 (210 terms), AdaptiveClenshawCurtis tol 1e-8, "
+                       "N=1000000", "N": N}
+    for name, fn in calls.items():
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        out[name + "_ms"] = ms
+        out[name + "_points_per_s"] = N / (ms * 1e-3)
+        log(f"single component {name}: {ms:.3f} ms")
+    del cg
+    return out
+
+
 def load_traffic():
     """HBM bytes per fused launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
     summary of this workload (None when absent)."""
@@ -228,6 +276,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-second", action="store_true")
     ap.add_argument("--no-many", action="store_true")
+    ap.add_argument("--no-single", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--profile", action="store_true")
     args = ap.parse_args()
@@ -283,6 +332,9 @@ def main():
     many = None
     if not args.no_many:
         many = many_maps(max(3, args.plain_steps))
+    single = None
+    if not args.no_single:
+        single = single_component_workload(max(3, args.plain_steps))
 
     bytes_alg = 8.0 * D * N + 8.0 * (tri.numCoeffs + 1)
     traffic = load_traffic()
@@ -317,6 +369,7 @@ def main():
                                 "fused_schedule": fused_schedule},
         "active_quadrature_workload": second,
         "many_maps_workload": many,
+        "single_component_workload": single,
     }
     if not args.no_cpu:
         log("cpu baseline")
