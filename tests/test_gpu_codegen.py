@@ -143,3 +143,39 @@ def test_fused_schedule_autotuning():
     l2, g2 = g.KLValueAndGrad(x)  # the tuned schedule, deterministic from here on
     l3, g3 = g.KLValueAndGrad(x)
     assert torch.equal(g2, g3) and torch.equal(l2, l3)
+
+
+def test_fused_kl_step_replays_in_a_graph():
+    # One training step's objective + gradient captured once and replayed with
+    # new coefficients written into the wrapped (static) coefficient tensor.
+    torch = pytest.importorskip("torch")
+    _case("codegen_graph_step")
+    g = mt.CreateTriangular(8, 8, 3, mt.MapOptions())
+    s = Stream(23)
+    x = torch.from_numpy(s.points(8, 100000)).cuda()
+    coeffs = torch.from_numpy(0.05 * s.normal(g.numCoeffs)).cuda()
+    g.WrapCoeffs(coeffs)
+    loss = torch.zeros(1, dtype=torch.float64, device="cuda")
+    grad = torch.zeros(g.numCoeffs, dtype=torch.float64, device="cuda")
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        g.KLValueAndGradImpl(x, loss, grad)  # compiles, tunes and sizes the work buffers
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        g.KLValueAndGradImpl(x, loss, grad)
+    for step in range(3):
+        new = torch.from_numpy((0.05 + 0.01 * step) * Stream(30 + step).normal(g.numCoeffs)).cuda()
+        coeffs.copy_(new)
+        loss.fill_(-1.0)
+        grad.fill_(-1.0)
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        le = torch.zeros_like(loss)
+        ge = torch.zeros_like(grad)
+        g.KLValueAndGradImpl(x, le, ge)  # eager, same buffers and schedule
+        torch.cuda.synchronize()
+        assert torch.equal(loss, le) and torch.equal(grad, ge), f"step {step}"
