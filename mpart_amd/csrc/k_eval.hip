@@ -26,6 +26,17 @@ hipError_t launch_reduce_cols(const double* in, long nrows, long rowStride, long
                        scale, out, outLast);
     return hipGetLastError();
 }
+hipError_t launch_colsum(const double* in, long nrows, long rowStride, long ncols, double scale, double* out,
+                         double* outLast, double* work, hipStream_t s) {
+    if (ncols <= 0) return hipSuccess;
+    const long nchunks = std::max<long>(1, std::min<long>(COLSUM_MAX_CHUNKS, (nrows + 15) / 16));
+    const long chunk = std::max<long>(1, (nrows + nchunks - 1) / nchunks);
+    hipLaunchKernelGGL(k_colsum_chunks, dim3((unsigned)((ncols + 63) / 64), (unsigned)nchunks), dim3(256), 0, s, in,
+                       nrows, rowStride, ncols, chunk, work);
+    hipLaunchKernelGGL(k_colsum_final, dim3((unsigned)((ncols + 255) / 256)), dim3(256), 0, s, work, nchunks, ncols,
+                       scale, out, outLast);
+    return hipGetLastError();
+}
 hipError_t launch_kl_rows(const double* G, const double* LG, long N, long K, double* grad, hipStream_t s) {
     hipLaunchKernelGGL(k_kl_rows, dim3((unsigned)K), dim3(256), 0, s, G, LG, N, K, grad);
     return hipGetLastError();

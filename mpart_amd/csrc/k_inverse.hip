@@ -1,14 +1,48 @@
 #include "launchers.hpp"
-#include "mono_kernels.hpp"
+#include "mono_inverse.hpp"
 
 namespace mpd {
 
-hipError_t launch_inverse(int var, const Launch& L, const MapDev& M, const double* w, long ws, const double* xs,
-                          long xrows, long xcols, long xrs, long xcs, const double* ys, long yss, long N, double* out,
-                          long os, int c, double xtol, double ytol, double* gtab) {
-#define CALL(B, Q, R, PM)                                                                                               \
-    hipLaunchKernelGGL((k_inverse<B, Q, R, PM>), L.grid, L.block, L.smem, L.stream, M, w, ws, xs, xrows, xcols, xrs, xcs, \
-                       ys, yss, N, out, os, c, xtol, ytol, gtab)
+__global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
+
+template <int B, int Q, int R, int PM>
+static int inverse_occupancy(int block) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_inverse_queue<B, Q, R, PM>, block, 0) != hipSuccess)
+        return 1;
+    return nb > 0 ? nb : 1;
+}
+
+int occupancy_inverse(int var, int block) {
+#define CALL(B, Q, R, PM) return inverse_occupancy<B, Q, R, PM>(block)
+    switch (var) {
+        MPD_CASES_PM(0, 4, CALL)
+        MPD_CASES_PM(1, 12, CALL)
+        default: return 1;
+    }
+#undef CALL
+}
+
+int inverse_record_rows(int var) {
+#define CALL(B, Q, R, PM) return inv_rec_rows<PM>()
+    switch (var) {
+        MPD_CASES_PM(0, 4, CALL)
+        MPD_CASES_PM(1, 12, CALL)
+        default: return inv_rec_rows<MAXP>();
+    }
+#undef CALL
+}
+
+hipError_t launch_inverse(int var, const Launch& L, dim3 qgrid, dim3 qblock, const MapDev& M, const double* w, long ws,
+                          const double* xs, long xrows, long xcols, long xrs, long xcs, const double* ys, long yss,
+                          long N, double* out, long os, int c, double xtol, double ytol, double* gtab, double* rec,
+                          unsigned long long* counter) {
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, L.stream, counter);
+#define CALL(B, Q, R, PM)                                                                                          \
+    hipLaunchKernelGGL((k_inverse_setup<B, Q, R, PM>), L.grid, L.block, L.smem, L.stream, M, w, ws, xs, xrows, xcols, \
+                       xrs, xcs, ys, yss, N, c, gtab, rec);                                                        \
+    hipLaunchKernelGGL((k_inverse_queue<B, Q, R, PM>), qgrid, qblock, 0, L.stream, M, rec, N, out, os, c, xtol, ytol, \
+                       counter)
     MPD_DISPATCH(var, CALL)
 #undef CALL
     return hipGetLastError();

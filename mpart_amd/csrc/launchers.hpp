@@ -34,9 +34,15 @@ hipError_t launch_logdet_coeff_grad_discrete(int var, const Launch& L, const Map
 hipError_t launch_input_grad(int var, const Launch& L, const MapDev& M, const double* w, long ws, const double* pts,
                              long prs, long pcs, long N, const double* sens, long srs, long scs, double* out, long ors,
                              long ocs, int c, int mode, double* gtab);
-hipError_t launch_inverse(int var, const Launch& L, const MapDev& M, const double* w, long ws, const double* xs,
-                          long xrows, long xcols, long xrs, long xcs, const double* ys, long yss, long N, double* out,
-                          long os, int c, double xtol, double ytol, double* gtab);
+// Inverse of component c: a setup pass (shape L) writes per-point records to
+// rec, then the persistent solve pass (qgrid x qblock) takes points from the
+// counter at rec + records.
+hipError_t launch_inverse(int var, const Launch& L, dim3 qgrid, dim3 qblock, const MapDev& M, const double* w, long ws,
+                          const double* xs, long xrows, long xcols, long xrs, long xcs, const double* ys, long yss,
+                          long N, double* out, long os, int c, double xtol, double ytol, double* gtab, double* rec,
+                          unsigned long long* counter);
+int occupancy_inverse(int var, int block);
+int inverse_record_rows(int var);
 hipError_t launch_fused(int var, const Launch& L, const MapDev& M, const double* w, long ws, const double* pts,
                         long prs, long pcs, long N, const double* sens, long srs, long scs, const double* lsens,
                         long lss, int klMode, double* partialGrad, double* partialLoss, double* gtab,
@@ -51,6 +57,11 @@ int occupancy_batch(int var, int block, size_t smem);
 hipError_t launch_reduce_rows(const double* in, long nrows, long K, double scale, double* out, hipStream_t s);
 hipError_t launch_reduce_cols(const double* in, long nrows, long rowStride, long ncols, double scale, double* out,
                               double* outLast, hipStream_t s);
+// Column sums of many partial rows in two fixed-order stages; work holds
+// COLSUM_MAX_CHUNKS * ncols doubles.
+constexpr long COLSUM_MAX_CHUNKS = 64;
+hipError_t launch_colsum(const double* in, long nrows, long rowStride, long ncols, double scale, double* out,
+                         double* outLast, double* work, hipStream_t s);
 hipError_t launch_kl_rows(const double* G, const double* LG, long N, long K, double* grad, hipStream_t s);
 hipError_t launch_kl_loss(const double* T, long M, const double* ld, long N, double* partial, hipStream_t s);
 hipError_t launch_fill(double* p, long rows, long cols, long rs, long cs, double v, hipStream_t s);

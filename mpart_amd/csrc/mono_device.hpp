@@ -118,6 +118,17 @@ MPD_HD double recip_1_2(double d) {
 #endif
 }
 
+// x / 6, correctly rounded, in three operations.  q = RN(x RN(1/6)) is within
+// 1.5 ulp of x/6, so the remainder r = x - 6q is exact (a multiple of 2 ulp(q)
+// below 20 ulp(q)) and q + r RN(1/6) differs from x/6 by under 2^-50 ulp.  For
+// normal x, x/6 is an integer multiple of ulp(x/6)/3, hence at least ulp/6 from
+// every rounding midpoint, so the fused correction rounds to RN(x/6) exactly.
+MPD_HD double div6(double x) {
+    const double c = 1.0 / 6.0;
+    const double q = x * c;
+    return fma(fma(-q, 6.0, x), c, q);
+}
+
 // Logarithms by the method of fdlibm's __ieee754_log: x = 2^k m with m in
 // [sqrt(1/2), sqrt(2)), f = m - 1 (exact), s = f / (2 + f) and
 //   log(1 + f) = 2 atanh(s) = f - (f^2/2 - s (f^2/2 + R)),  R = sum_{j>=1} 2 s^(2j) / (2j + 1),
@@ -709,15 +720,15 @@ __host__ __device__ double integrate_as(const QuadDev& Q, const DiagCoef<PM>& dc
         NodeV<FL> flm = eval_node<BK, PK, FL, PM>(dc, lm);
         NodeV<FL> frm = eval_node<BK, PK, FL, PM>(dc, rm);
         ne += 2;
-        const double coarse = ((b - a) / 6.0) * (fa.F + 4.0 * fm.F + fb.F);
-        double fine = ((m - a) / 6.0) * (fa.F + 4.0 * flm.F + fm.F);
-        fine += ((b - m) / 6.0) * (fm.F + 4.0 * frm.F + fb.F);
+        const double coarse = div6(b - a) * (fa.F + 4.0 * fm.F + fb.F);
+        const double wl = div6(m - a), wr = div6(b - m);
+        double fine = wl * (fa.F + 4.0 * flm.F + fm.F);
+        fine += wr * (fm.F + 4.0 * frm.F + fb.F);
         const double err = fabs(fine - coarse);
         const double tol = fmax(Q.relTol * fabs(coarse), Q.absTol);
         if (((err < tol) || (level == Q.maxSub - 1)) && (level >= Q.minSub)) {
             res += fine;
             if constexpr (FL >= FL_COEF) {
-                const double wl = (m - a) / 6.0, wr = (b - m) / 6.0;
                 adj_node<BK, FL, PM>(dc, a, wl, fa, A);
                 adj_node<BK, FL, PM>(dc, lm, 4.0 * wl, flm, A);
                 adj_node<BK, FL, PM>(dc, m, wl + wr, fm, A);

@@ -202,6 +202,7 @@ struct mpart_map_s {
 
     // work buffers
     DBuf partialGrad, partialLoss, gtab, wT, wLd, wG, wLG, wFull, wCoeffs;
+    DBuf wInv;  // inverse: per-point records + the point counter
 
     // runtime-specialized kernels (codegen.hip)
     bool genEnabled = true;
@@ -718,6 +719,29 @@ void do_input_grad(mpart_map_s& m, const mpart_cview& p, const mpart_cview* sens
                   "launch input_grad");
 }
 
+// Inverse of component c: a lane-per-point setup pass writes each point's
+// record, then a persistent pass solves with per-lane scheduling (lanes take
+// new points as their solves end).
+void inverse_component(mpart_map_s& m, const Shape& sh, const double* xs, long xrows, long xcols, long xrs, long xcs,
+                       const double* ys, long yss, long N, double* out, long os, int c, double xtol, double ytol) {
+    if (g_numCU == 0) {
+        int dev = 0;
+        hip_check(hipGetDevice(&dev), "hipGetDevice");
+        hip_check(hipDeviceGetAttribute(&g_numCU, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
+    }
+    const size_t recBytes = (size_t)mpd::inverse_record_rows(m.variant) * N * sizeof(double);
+    m.wInv.ensure(recBytes + 256);
+    double* rec = m.wInv.as<double>();
+    auto* counter = reinterpret_cast<unsigned long long*>(m.wInv.as<char>() + recBytes);
+    const int block = 256;
+    const long grid = std::min<long>((N + block - 1) / block,
+                                     (long)g_numCU * mpd::occupancy_inverse(m.variant, block));
+    hip_check(mpd::launch_inverse(m.variant, sh.L, dim3((unsigned)std::max<long>(1, grid)), dim3(block), m.dev,
+                                  m.coeffs, m.coeffStride, xs, xrows, xcols, xrs, xcs, ys, yss, N, out, os, c, xtol,
+                                  ytol, sh.gtab, rec, counter),
+              "launch inverse");
+}
+
 void do_inverse(mpart_map_s& m, const mpart_cview& x1, const mpart_cview& r, const mpart_view& o, double xtol,
                 double ytol, hipStream_t s) {
     check_coeffs(&m);
@@ -737,10 +761,8 @@ void do_inverse(mpart_map_s& m, const mpart_cview& x1, const mpart_cview& r, con
                            std::to_string(x1.cols) + ", but the number of y points is " + std::to_string(N) +
                            ".  If the number of xs is not 1 then it must match the number of ys.");
         Shape sh = shape_for(m, N, s);
-        hip_check(mpd::launch_inverse(m.variant, sh.L, m.dev, m.coeffs, m.coeffStride, x1.ptr, x1.rows, x1.cols,
-                                      x1.row_stride, x1.col_stride, r.ptr, r.col_stride, N, o.ptr, o.col_stride, 0,
-                                      xtol, ytol, sh.gtab),
-                  "launch inverse");
+        inverse_component(m, sh, x1.ptr, x1.rows, x1.cols, x1.row_stride, x1.col_stride, r.ptr, r.col_stride, N,
+                          o.ptr, o.col_stride, 0, xtol, ytol);
         return;
     }
     // TriangularMap::InverseImpl (src/TriangularMap.cpp:147-184): components
@@ -755,10 +777,8 @@ void do_inverse(mpart_map_s& m, const mpart_cview& x1, const mpart_cview& r, con
     Shape sh = shape_for(m, N, s);
     for (int c = 0; c < m.dev.ncomp; ++c) {
         const long dimc = m.comps[c].dim;
-        hip_check(mpd::launch_inverse(m.variant, sh.L, m.dev, m.coeffs, m.coeffStride, full, dimc, N, N, 1,
-                                      r.ptr + (long)c * r.row_stride, r.col_stride, N, full + (extra + c) * N, 1, c,
-                                      xtol, ytol, sh.gtab),
-                  "launch inverse");
+        inverse_component(m, sh, full, dimc, N, N, 1, r.ptr + (long)c * r.row_stride, r.col_stride, N,
+                          full + (extra + c) * N, 1, c, xtol, ytol);
     }
     hip_check(mpd::launch_copy(full + extra * N, op, N, N, 1, o.ptr, o.row_stride, o.col_stride, s), "launch copy");
 }
@@ -798,7 +818,8 @@ void do_fused(mpart_map_s& m, const mpart_cview& p, const double* sens, long srs
         };
         long maxGrid = 1;
         for (int cfg = 0; cfg < 4; ++cfg) maxGrid = std::max(maxGrid, grid_of(cfg));
-        m.partialGrad.ensure(std::max<size_t>(8, (size_t)maxGrid * (K + 1) * sizeof(double)));
+        // Partial rows of the widest schedule, then the reduction's chunk rows.
+        m.partialGrad.ensure((size_t)(maxGrid + mpd::COLSUM_MAX_CHUNKS) * (K + 1) * sizeof(double));
         const double* w = gen_coeffs(m, s);
         MapDev M = m.dev;
         const double* pts = p.ptr;
@@ -841,10 +862,10 @@ void do_fused(mpart_map_s& m, const mpart_cview& p, const double* sens, long srs
         }
         const int cfg = m.fusedChoice < 0 ? 0 : m.fusedChoice;
         if (N > 0) launch_cfg(cfg);
-        // One pass over the partial rows: K gradient columns, then the loss column.
+        // Column sums of the partial rows: K gradient columns, then the loss column.
         const long rows = (N > 0) ? grid_of(cfg) : 0;
-        hip_check(mpd::launch_reduce_cols(m.partialGrad.as<double>(), rows, K + 1, loss ? K + 1 : K, scale, grad, loss,
-                                          s),
+        hip_check(mpd::launch_colsum(part, rows, K + 1, loss ? K + 1 : K, scale, grad, loss, part + maxGrid * (K + 1),
+                                     s),
                   "launch reduce");
         return;
     }

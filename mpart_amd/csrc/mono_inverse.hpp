@@ -1,0 +1,409 @@
+// mono_inverse.hpp -- inverse of one component with per-lane solve scheduling.
+//
+// MonotoneComponent::InverseImpl (MonotoneComponent.h:331-443) solves
+// T(x_<k, x) = r for every point with a bracket search (FindBracket,
+// RootFinding.h:20-90) and ITP iterations (InverseSingleBracket, 126-176),
+// each step an adaptive quadrature.  Per point the work varies by two orders
+// of magnitude (a few hundred to tens of thousands of integrand evaluations
+// on random targets), so a lane-per-point kernel that runs each wavefront
+// until its slowest lane is done spends most of its time masked.
+//
+// Here a lane advances its own solve one quadrature interval at a time: each
+// loop iteration every busy lane evaluates the integrand at the nodes of its
+// current interval (independent evaluations, so they overlap), applies the
+// interval's acceptance test and, when its quadrature is complete, advances
+// its bracket / ITP state.  A lane whose solve ends writes the root and takes
+// the next point from a wavefront-local range of point indices, refilled 64
+// at a time from a global counter.  The nodes, the accumulations and every
+// comparison are those of integrate_as / integrate_acc and inverse_solve, so
+// the results are those of the sequential solver bit for bit; only the
+// interleaving differs.
+//
+// The per-point setup (basis values, C_q sums, f(x_<k, 0)) runs in a separate
+// lane-per-point pass that stores a record per point, so that refilling a
+// lane costs a few loads.
+#pragma once
+#include "mono_kernels.hpp"
+
+namespace mpd {
+
+// Per-point record, structure of arrays with N doubles per row:
+// rows 0..PM: C_q rho_q (DiagCoef::cd), PM+1: f(x_<k, 0), PM+2: the initial
+// guess (NaN when any input coordinate is NaN), PM+3: the target r.
+template <int PM>
+constexpr int inv_rec_rows() {
+    return PM + 4;
+}
+
+template <int BK, int PK, int QK, int PM>
+__global__ void __launch_bounds__(256) k_inverse_setup(MapDev M, const double* w, long ws, const double* xs,
+                                                       long xrows, long xcols, long xrs, long xcs, const double* ys,
+                                                       long yss, long N, int c, double* gtab, double* rec) {
+    extern __shared__ double smem[];
+    const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    const long xi = (xcols == 1) ? 0 : n;
+    const double* pt = xs + xi * xcs;
+    bool bad = false;
+    for (long i = 0; i < xrows; ++i) bad |= isnan(pt[i * xrs]);
+    double* r = rec + n;
+    r[(PM + 3) * N] = ys[n * yss];
+    if (bad) {
+        r[(PM + 2) * N] = NAN;
+        return;
+    }
+    const double x0 = pt[(xrows - 1) * xrs];
+    long stride;
+    double* tab = lane_table(smem, gtab, stride);
+    const CompDev cd = M.comps[c];
+    fill_values<BK>(M, cd.dim - 1, pt, xrs, tab, (int)stride);
+    double C[PM + 1];
+    coeff_groups<PM>(M, cd, w + (long)cd.coeffOff * ws, ws, tab, (int)stride, C);
+    DiagCoef<PM> dc;
+    make_diag<PM>(M, cd, C, x0, M.nugget, dc);
+    for (int q = 0; q <= PM; ++q) r[q * N] = dc.cd[q];
+    r[(PM + 1) * N] = f_at_zero<PM>(M, cd, C);
+    r[(PM + 2) * N] = x0;
+}
+
+// Saved interval of the depth-first quadrature (fm is unused by
+// Clenshaw-Curtis).
+struct InvStk {
+    double a, b, fa, fm, fb;
+    int level;
+};
+
+// Solver phases.
+enum { IP_START = 0, IP_DOWN = 1, IP_UP = 2, IP_ITP = 3 };
+
+constexpr int INV_CHUNK = 64;  // point indices taken from the global counter at a time
+
+template <int BK, int PK, int QK, int PM>
+__global__ void __launch_bounds__(256) k_inverse_queue(MapDev M, const double* __restrict__ rec, long N, double* out,
+                                                       long os, int c, double xtol, double ytol,
+                                                       unsigned long long* counter) {
+    const QuadDev& Q = M.quad;
+    const int p = M.comps[c].p;
+    // Clenshaw-Curtis node table of one interval, in the order integrate_acc
+    // visits the nodes after the midpoint: position (cp or fp), coarse weight
+    // index (-1: fine-only node) and fine weight index.
+    __shared__ double nPos[2 * MAXNF];
+    __shared__ int nCw[2 * MAXNF], nFw[2 * MAXNF];
+    __shared__ double sCw[MAXNF], sFw[MAXNF];
+    const int nf = (QK == QK_ACC5) ? 5 : Q.nf;
+    const int nc = (nf - 1) / 2 + 1;
+    const int cRight = nc - 1, cMid = cRight / 2;
+    const int fRight = nf - 1, fMid = fRight / 2;
+    const int nNodes = cRight - 1 + (nf - 1) / 2;  // per interval, midpoint included
+    if (QK != QK_AS) {
+        if (threadIdx.x == 0) {
+            int j = 1;
+            for (int i = 1; i < cMid; ++i, ++j) {
+                nPos[j] = Q.cp[i];
+                nCw[j] = i;
+                nFw[j] = 2 * i;
+            }
+            for (int ii = cMid + 1; ii < cRight; ++ii, ++j) {
+                nPos[j] = Q.cp[ii];
+                nCw[j] = ii;
+                nFw[j] = 2 * ii;
+            }
+            for (int i = 1; i < nf; i += 2, ++j) {
+                nPos[j] = Q.fp[i];
+                nCw[j] = -1;
+                nFw[j] = i;
+            }
+        }
+        for (int i = threadIdx.x; i < MAXNF; i += blockDim.x) {
+            sCw[i] = Q.cw[i];
+            sFw[i] = Q.fw[i];
+        }
+        __syncthreads();
+    }
+    const int lane = threadIdx.x & 63;
+    const double nug = M.nugget;
+
+    // Point and solver state.
+    long n = -1;
+    bool busy = false, finished = false;
+    DiagCoef<PM> dc;
+    dc.p = p;
+    dc.nug = nug;
+    dc.x = 0.0;
+#pragma unroll
+    for (int q = 0; q <= PM; ++q) dc.cd[q] = 0.0;
+    double f0 = 0.0, yd = 0.0, x0 = 0.0;
+    int phase = IP_START, bi = 0, it = 0;
+    double step = 1.0, xlb = 0.0, xub = 0.0, ylb = 0.0, yub = 0.0, nhalf = 0.0, xc = 0.0;
+    // Quadrature state.
+    double qa = 0.0, qb = 1.0, res = 0.0, fa = 0.0, fb = 0.0, fm = 0.0;
+    int level = 0, sp = 0, k = 0;
+    InvStk stk[MAXSUB];
+    // Wavefront-uniform range of point indices not yet handed to a lane.
+    long qNext = 0, qEnd = 0;
+    bool exhausted = false;
+
+    auto start_quad = [&](double x) {
+        dc.x = x;
+        qa = 0.0;
+        qb = 1.0;
+        res = 0.0;
+        level = 0;
+        sp = 0;
+        k = (QK == QK_AS) ? -2 : -1;
+    };
+
+    while (true) {
+        // Refill lanes without a point.
+        while (true) {
+            const bool need = !busy && !finished;
+            const unsigned long long mask = __ballot(need);
+            if (mask == 0) break;
+            const int cnt = __popcll(mask);
+            const int rank = __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+            const long avail = qEnd - qNext;
+            long base = 0;
+            if (!exhausted && cnt > avail) {
+                const int first = __builtin_ctzll(mask);
+                unsigned long long b = 0;
+                if (lane == first) b = atomicAdd(counter, (unsigned long long)INV_CHUNK);
+                const unsigned lo = __builtin_amdgcn_readlane((unsigned)b, first);
+                const unsigned hi = __builtin_amdgcn_readlane((unsigned)(b >> 32), first);
+                base = (long)(((unsigned long long)hi << 32) | lo);
+            }
+            if (need) {
+                long idx = N;
+                if (rank < avail) idx = qNext + rank;
+                else if (!exhausted) idx = base + (rank - avail);
+                if (idx >= N) {
+                    finished = true;
+                } else {
+                    n = idx;
+                    x0 = rec[(PM + 2) * N + n];
+                    if (isnan(x0)) {
+                        out[n * os] = NAN;  // stays without a point: refilled on the next pass
+                    } else {
+#pragma unroll
+                        for (int q = 0; q <= PM; ++q) dc.cd[q] = rec[q * N + n];
+                        f0 = rec[(PM + 1) * N + n];
+                        yd = rec[(PM + 3) * N + n];
+                        busy = true;
+                        phase = IP_START;
+                        start_quad(x0);
+                    }
+                }
+            }
+            if (!exhausted && cnt > avail) {
+                qNext = base + (cnt - avail);
+                qEnd = base + INV_CHUNK;
+                if (base >= N) exhausted = true;
+            } else {
+                qNext += cnt;
+            }
+        }
+        if (__ballot(busy) == 0) break;
+        if (!busy) continue;
+
+        // One step of this lane's quadrature: the integrand at every node of
+        // its current interval (E evaluations, independent of each other),
+        // then the interval's acceptance test.  Starting a quadrature takes
+        // one step for F(0) and F(1) (and, for Simpson, one for F(1/2)).
+        bool qdone = false;
+        if constexpr (QK == QK_AS) {
+            const double m = 0.5 * (qa + qb);
+            const double t0 = (k == -2) ? 0.0 : (k == -1) ? 0.5 : 0.5 * (qa + m);
+            const double t1 = (k == -2) ? 1.0 : (k == -1) ? 0.5 : 0.5 * (m + qb);
+            const double F0 = eval_node<BK, PK, FL_VAL, PM>(dc, t0).F;
+            const double F1 = eval_node<BK, PK, FL_VAL, PM>(dc, t1).F;
+            if (k == -2) {
+                fa = F0;
+                fb = F1;
+                k = -1;
+            } else if (k == -1) {
+                fm = F0;
+                k = 0;
+            } else {
+                const double a = qa, b = qb;
+                const double flm = F0, frm = F1;
+                const double coarse_ = div6(b - a) * (fa + 4.0 * fm + fb);
+                const double wl = div6(m - a), wr = div6(b - m);
+                double fine_ = wl * (fa + 4.0 * flm + fm);
+                fine_ += wr * (fm + 4.0 * frm + fb);
+                const double err = fabs(fine_ - coarse_);
+                const double tol = fmax(Q.relTol * fabs(coarse_), Q.absTol);
+                if (((err < tol) || (level == Q.maxSub - 1)) && (level >= Q.minSub)) {
+                    res += fine_;
+                    if (sp == 0) {
+                        qdone = true;
+                    } else {
+                        --sp;
+                        qa = stk[sp].a;
+                        qb = stk[sp].b;
+                        level = stk[sp].level;
+                        fa = stk[sp].fa;
+                        fm = stk[sp].fm;
+                        fb = stk[sp].fb;
+                    }
+                } else {
+                    stk[sp].a = m;
+                    stk[sp].b = b;
+                    stk[sp].level = level + 1;
+                    stk[sp].fa = fm;
+                    stk[sp].fm = frm;
+                    stk[sp].fb = fb;
+                    ++sp;
+                    qb = m;
+                    fb = fm;
+                    fm = flm;
+                    ++level;
+                }
+            }
+        } else {
+            constexpr int NF = (QK == QK_ACC5) ? 5 : MAXNF;
+            const double mid = 0.5 * (qa + qb);
+            const double scale = 0.5 * (qb - qa);
+            const bool start = k < 0;
+            double Fv[NF];
+#pragma unroll
+            for (int j = 0; j < NF; ++j) {
+                if (j >= nNodes) break;
+                const double t = start ? (j == 0 ? 0.0 : 1.0) : (j == 0 ? mid : mid + scale * nPos[j]);
+                Fv[j] = (start && j > 1) ? 0.0 : eval_node<BK, PK, FL_VAL, PM>(dc, t).F;
+            }
+            if (start) {
+                fa = Fv[0];
+                fb = Fv[1];
+                k = 0;
+            } else {
+                fm = Fv[0];
+                double coarse_ = scale * sCw[0] * fa;
+                coarse_ += scale * sCw[cMid] * fm;
+                coarse_ += scale * sCw[cRight] * fb;
+                double fine_ = scale * sFw[0] * fa;
+                fine_ += scale * sFw[fMid] * fm;
+                fine_ += scale * sFw[fRight] * fb;
+#pragma unroll
+                for (int j = 1; j < NF; ++j) {
+                    if (j >= nNodes) break;
+                    const int ci = nCw[j];
+                    if (ci >= 0) coarse_ += scale * sCw[ci] * Fv[j];
+                    fine_ += scale * sFw[nFw[j]] * Fv[j];
+                }
+                const double err = fabs(fine_ - coarse_);
+                const double tol = fmax(Q.relTol * fabs(coarse_), Q.absTol);
+                if (((err < tol) || (level == Q.maxSub - 1)) && (level >= Q.minSub)) {
+                    res += fine_;
+                    if (sp == 0) {
+                        qdone = true;
+                    } else {
+                        --sp;
+                        qa = stk[sp].a;
+                        qb = stk[sp].b;
+                        level = stk[sp].level;
+                        fa = stk[sp].fa;
+                        fb = stk[sp].fb;
+                    }
+                } else {
+                    stk[sp].a = mid;
+                    stk[sp].b = qb;
+                    stk[sp].level = level + 1;
+                    stk[sp].fa = fm;
+                    stk[sp].fb = fb;
+                    ++sp;
+                    qb = mid;
+                    fb = fm;
+                    ++level;
+                }
+            }
+        }
+        if (!qdone) continue;
+
+        // T at the current iterate; advance the bracket search or ITP.
+        const double y = res + f0;
+        bool fail = false, done = false;
+        double root = 0.0;
+        if (phase == IP_START) {
+            // InverseSingleBracket and FindBracket evaluate T(x0) three times;
+            // the values are identical.
+            xlb = xub = x0;
+            ylb = yub = y;
+            step = 1.0;
+            bi = 0;
+            if (ylb > yd) {
+                phase = IP_DOWN;
+                xlb = xub - step;
+                start_quad(xlb);
+            } else {
+                phase = IP_UP;
+                xub = xlb + step;
+                start_quad(xub);
+            }
+        } else if (phase == IP_DOWN || phase == IP_UP) {
+            bool bracketed = false;
+            if (phase == IP_DOWN) ylb = y;
+            else yub = y;
+            if ((fabs((yub - ylb) / (xub - xlb)) < 1e-12) && ((xub - xlb) > 10)) {
+                fail = true;
+            } else if (phase == IP_DOWN && ylb > yd) {
+                yub = ylb;
+                xub = xlb;
+                step *= 2.0;
+                if (++bi >= 128) fail = true;
+                else {
+                    xlb = xub - step;
+                    start_quad(xlb);
+                }
+            } else if (phase == IP_UP && yub < yd) {
+                ylb = yub;
+                xlb = xub;
+                step *= 2.0;
+                if (++bi >= 128) fail = true;
+                else {
+                    xub = xlb + step;
+                    start_quad(xub);
+                }
+            } else {
+                bracketed = true;
+            }
+            if (bracketed) {
+                if ((ylb > yd) || (yub < yd)) {
+                    fail = true;
+                } else {
+                    phase = IP_ITP;
+                    it = 0;
+                    nhalf = itp_nhalf(xlb, xub, xtol);
+                    xc = itp_point(xlb, xub, ylb, yub, nhalf, it, xtol);
+                    start_quad(xc);
+                }
+            }
+        } else {
+            const double yc = y;
+            if (fabs(yc - yd) < ytol) {
+                done = true;
+                root = xc;
+            } else {
+                if (yc > yd) {
+                    xub = xc;
+                    yub = yc;
+                } else {
+                    xlb = xc;
+                    ylb = yc;
+                }
+                if (((xub - xlb) < xtol) || ((yub - ylb) < ytol) || (unsigned)(it + 1) >= ITP_MAX_ITS) {
+                    done = true;
+                    root = 0.5 * (xub + xlb);
+                } else {
+                    ++it;
+                    xc = itp_point(xlb, xub, ylb, yub, nhalf, it, xtol);
+                    start_quad(xc);
+                }
+            }
+        }
+        if (fail || done) {
+            out[n * os] = fail ? NAN : root;
+            busy = false;
+        }
+    }
+}
+
+}  // namespace mpd

@@ -355,6 +355,31 @@ __device__ double eval_T(const MapDev& M, const DiagCoef<PM>& dc0, double f0, do
     return integrate<BK, PK, QK, FL_VAL, PM>(M.quad, dc, A, nullptr) + f0;
 }
 
+// One ITP iterate (Find_x_ITP, RootFinding.h:92-106; k1 = 0.1, k2 = 2, n0 = 1):
+// interpolation xf, truncation towards the bisection point xb by
+// delta = min(k1 (xub - xlb)^2, |xb - xf|), projection onto the ball of
+// radius xtol 2^(nhalf + n0 - it) - (xub - xlb)/2 around xb.  The powers are
+// a product and a power of two (pow(w, 2) and pow(2, k) are exactly these in
+// a correctly rounded libm).
+constexpr unsigned ITP_MAX_ITS = 10000;
+__device__ __forceinline__ double itp_nhalf(double xlb, double xub, double xtol) {
+    return ceil(log2(0.5 * (xub - xlb) / xtol));
+}
+__device__ __forceinline__ double itp_point(double xlb, double xub, double ylb, double yub, double nhalf, int it,
+                                            double xtol) {
+    const double k1 = 0.1, n0 = 1.0;
+    const double xb = 0.5 * (xub + xlb);
+    double xf = (xub * ylb - xlb * yub) / (ylb - yub);
+    const double sigma = ((xb - xf) > 0) ? 1.0 : -1.0;
+    const double w = xub - xlb;
+    const double delta = fmin(k1 * (w * w), fabs(xb - xf));
+    xf += delta * sigma;
+    const double e = nhalf + n0 - (double)it;
+    const double p2 = e < -2000.0 ? 0.0 : (e > 2000.0 ? __builtin_inf() : ldexp(1.0, (int)e));
+    const double rho = fmin(xtol * p2 - 0.5 * (xub - xlb), fabs(xf - xb));
+    return xb - sigma * rho;
+}
+
 // Bracket (FindBracket, RootFinding.h:20-80) and ITP (Find_x_ITP, 82-178) for
 // T(x) = yd with T given by its diagonal coefficients dc (x_k replaced by the
 // iterate) and f0 = f(x_<k, 0).  Returns NaN when no bracket is found.
@@ -393,18 +418,9 @@ __device__ double inverse_solve(const MapDev& M, const DiagCoef<PM>& dc, double 
         }
     }
     if ((info < 0) || ((ylb > yd) || (yub < yd))) return NAN;
-    // ITP (k1 = 0.1, k2 = 2, n0 = 1)
-    const double k1 = 0.1, k2 = 2.0, n0 = 1.0;
-    const double nhalf = ceil(log2(0.5 * (xub - xlb) / xtol));
-    const unsigned maxIts = 10000;
-    for (unsigned it = 0; it < maxIts; ++it) {
-        const double xb = 0.5 * (xub + xlb);
-        double xf = (xub * ylb - xlb * yub) / (ylb - yub);
-        const double sigma = ((xb - xf) > 0) ? 1.0 : -1.0;
-        const double delta = fmin(k1 * pow((xub - xlb), k2), fabs(xb - xf));
-        xf += delta * sigma;
-        const double rho = fmin(xtol * pow(2.0, nhalf + n0 - (double)it) - 0.5 * (xub - xlb), fabs(xf - xb));
-        const double xc = xb - sigma * rho;
+    const double nhalf = itp_nhalf(xlb, xub, xtol);
+    for (unsigned it = 0; it < ITP_MAX_ITS; ++it) {
+        const double xc = itp_point(xlb, xub, ylb, yub, nhalf, (int)it, xtol);
         const double yc = eval_T<BK, PK, QK, PM>(M, dc, f0, xc);
         if (fabs(yc - yd) < ytol) {
             return xc;
@@ -418,35 +434,6 @@ __device__ double inverse_solve(const MapDev& M, const DiagCoef<PM>& dc, double 
         if (((xub - xlb) < xtol) || ((yub - ylb) < ytol)) break;
     }
     return 0.5 * (xub + xlb);
-}
-
-template <int BK, int PK, int QK, int PM>
-__global__ void __launch_bounds__(256) k_inverse(MapDev M, const double* w, long ws, const double* xs, long xrows,
-                                                 long xcols, long xrs, long xcs, const double* ys, long yss, long N,
-                                                 double* out, long os, int c, double xtol, double ytol, double* gtab) {
-    extern __shared__ double smem[];
-    const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
-    const long xi = (xcols == 1) ? 0 : n;
-    const double* pt = xs + xi * xcs;
-    for (long i = 0; i < xrows; ++i) {
-        if (isnan(pt[i * xrs])) {
-            out[n * os] = NAN;
-            return;
-        }
-    }
-    const double x0 = pt[(xrows - 1) * xrs];
-    const double yd = ys[n * yss];
-    long stride;
-    double* tab = lane_table(smem, gtab, stride);
-    const CompDev cd = M.comps[c];
-    fill_values<BK>(M, cd.dim - 1, pt, xrs, tab, (int)stride);
-    double C[PM + 1];
-    coeff_groups<PM>(M, cd, w + (long)cd.coeffOff * ws, ws, tab, (int)stride, C);
-    const double f0 = f_at_zero<PM>(M, cd, C);
-    DiagCoef<PM> dc;
-    make_diag<PM>(M, cd, C, x0, M.nugget, dc);
-    out[n * os] = inverse_solve<BK, PK, QK, PM>(M, dc, f0, x0, yd, xtol, ytol);
 }
 
 // ---------------------------------------------------------------------------
@@ -577,6 +564,41 @@ __global__ void __launch_bounds__(256) k_reduce_cols(const double* in, long nrow
         if (outLast && c == ncols - 1) *outLast = scale * t;
         else out[c] = scale * t;
     }
+}
+
+// Two-stage column sums of a row-major partial matrix, for many rows.  Stage
+// one: block (x = group of 64 columns, y = chunk of rows) sums its chunk with
+// one lane per column (coalesced 512-byte rows per wave); its four waves take
+// every fourth row and are combined in wave order.  Stage two sums the chunk
+// rows of each column in order.  Both orders are fixed, so results repeat
+// bitwise for a given launch shape.
+__global__ void __launch_bounds__(256) k_colsum_chunks(const double* in, long nrows, long rowStride, long ncols,
+                                                       long chunk, double* work) {
+    __shared__ double red[4][64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const long c = (long)blockIdx.x * 64 + lane;
+    const long r0 = (long)blockIdx.y * chunk;
+    const long r1 = r0 + chunk < nrows ? r0 + chunk : nrows;
+    double s = 0.0;
+    if (c < ncols) {
+#pragma unroll 4
+        for (long r = r0 + wv; r < r1; r += 4) s += in[r * rowStride + c];
+    }
+    red[wv][lane] = s;
+    __syncthreads();
+    if (wv == 0 && c < ncols)
+        work[(long)blockIdx.y * ncols + c] = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+}
+
+__global__ void __launch_bounds__(256) k_colsum_final(const double* work, long nchunks, long ncols, double scale,
+                                                      double* out, double* outLast) {
+    const long c = (long)blockIdx.x * 256 + threadIdx.x;
+    if (c >= ncols) return;
+    double t = 0.0;
+#pragma unroll 8
+    for (long i = 0; i < nchunks; ++i) t += work[i * ncols + c];
+    if (outLast && c == ncols - 1) *outLast = scale * t;
+    else out[c] = scale * t;
 }
 
 // Plain KL reduction (FastGaussianReverseKLObjective, MapObjective.cpp:146-215):
