@@ -66,9 +66,15 @@ __global__ void __launch_bounds__(256) k_inverse_setup(MapDev M, const double* w
     r[(PM + 2) * N] = x0;
 }
 
-// Saved interval of the depth-first quadrature (fm is unused by
-// Clenshaw-Curtis).
+// Saved interval of the depth-first quadrature (Simpson also keeps the
+// midpoint value).
+template <int QK>
 struct InvStk {
+    double a, b, fa, fb;
+    int level;
+};
+template <>
+struct InvStk<QK_AS> {
     double a, b, fa, fm, fb;
     int level;
 };
@@ -138,7 +144,11 @@ __global__ void __launch_bounds__(256) k_inverse_queue(MapDev M, const double* _
     // Quadrature state.
     double qa = 0.0, qb = 1.0, res = 0.0, fa = 0.0, fb = 0.0, fm = 0.0;
     int level = 0, sp = 0, k = 0;
-    InvStk stk[MAXSUB];
+    // Depth-first stack: the top entry in registers, the others in scratch
+    // (stk[0 .. sp-2]), so that popping does not wait for memory; the load
+    // that refills the top completes while the next interval is evaluated.
+    InvStk<QK> stk[MAXSUB];
+    InvStk<QK> top{};
     // Wavefront-uniform range of point indices not yet handed to a lane.
     long qNext = 0, qEnd = 0;
     bool exhausted = false;
@@ -236,21 +246,22 @@ __global__ void __launch_bounds__(256) k_inverse_queue(MapDev M, const double* _
                     if (sp == 0) {
                         qdone = true;
                     } else {
-                        --sp;
-                        qa = stk[sp].a;
-                        qb = stk[sp].b;
-                        level = stk[sp].level;
-                        fa = stk[sp].fa;
-                        fm = stk[sp].fm;
-                        fb = stk[sp].fb;
+                        qa = top.a;
+                        qb = top.b;
+                        level = top.level;
+                        fa = top.fa;
+                        fm = top.fm;
+                        fb = top.fb;
+                        if (--sp > 0) top = stk[sp - 1];
                     }
                 } else {
-                    stk[sp].a = m;
-                    stk[sp].b = b;
-                    stk[sp].level = level + 1;
-                    stk[sp].fa = fm;
-                    stk[sp].fm = frm;
-                    stk[sp].fb = fb;
+                    if (sp > 0) stk[sp - 1] = top;
+                    top.a = m;
+                    top.b = b;
+                    top.level = level + 1;
+                    top.fa = fm;
+                    top.fm = frm;
+                    top.fb = fb;
                     ++sp;
                     qb = m;
                     fb = fm;
@@ -296,19 +307,20 @@ __global__ void __launch_bounds__(256) k_inverse_queue(MapDev M, const double* _
                     if (sp == 0) {
                         qdone = true;
                     } else {
-                        --sp;
-                        qa = stk[sp].a;
-                        qb = stk[sp].b;
-                        level = stk[sp].level;
-                        fa = stk[sp].fa;
-                        fb = stk[sp].fb;
+                        qa = top.a;
+                        qb = top.b;
+                        level = top.level;
+                        fa = top.fa;
+                        fb = top.fb;
+                        if (--sp > 0) top = stk[sp - 1];
                     }
                 } else {
-                    stk[sp].a = mid;
-                    stk[sp].b = qb;
-                    stk[sp].level = level + 1;
-                    stk[sp].fa = fm;
-                    stk[sp].fb = fb;
+                    if (sp > 0) stk[sp - 1] = top;
+                    top.a = mid;
+                    top.b = qb;
+                    top.level = level + 1;
+                    top.fa = fm;
+                    top.fb = fb;
                     ++sp;
                     qb = mid;
                     fb = fm;
