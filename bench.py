@@ -20,7 +20,9 @@ launches with stored Jacobians, then row reductions).
 
 Prints one JSON line.  --gpus must be 1.  --profile runs only the headline
 fused steps followed by a 256 MiB elementwise copy (the calibration kernel
-for rocprofv3 FETCH_SIZE / WRITE_SIZE), for use under rocprofv3.
+for rocprofv3 FETCH_SIZE / WRITE_SIZE), for use under rocprofv3;
+--profile-inverse runs only three Inverse calls of the single-component
+workload.
 """
 import argparse
 import json
@@ -188,13 +190,8 @@ def many_maps(steps):
                         "4096..16384 N(0,I) points each"}
 
 
-def single_component_workload(steps):
-    """One component, dim 6, total order 4 (210 terms), ProbabilistHermite /
-    SoftPlus / adaptive Clenshaw-Curtis (quadPts 5, tolerances 1e-8);
-    N = 1,000,000 points x ~ N(0, I_6), coefficients ~ N(0, 0.05^2), targets
-    r ~ N(0, 1) from one seed-0 stream.  Times Evaluate, LogDeterminant,
-    CoeffGrad (sens = 1, 210 x N output) and Inverse (x_{1:5} fixed, x_6 as
-    the initial guess) with preallocated outputs."""
+def single_component_setup():
+    """The single-component workload's map, points, targets (see below)."""
     import torch
 
     import mpart_amd as mt
@@ -207,6 +204,36 @@ def single_component_workload(steps):
     x = torch.from_numpy(s.points(6, N)).cuda()
     g.SetCoeffs(0.05 * s.normal(g.numCoeffs))
     r = torch.from_numpy(s.normal(N)[None, :].copy()).cuda()
+    return g, x, r, N
+
+
+def profile_inverse_mode(args):
+    """Three Inverse calls of the single-component workload (for rocprofv3)."""
+    import torch
+
+    g, x, r, N = single_component_setup()
+    xi = torch.empty((1, N), dtype=torch.float64, device="cuda")
+    times = []
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        g.InverseImpl(x, r, xi)
+        torch.cuda.synchronize()
+        times.append((time.perf_counter() - t0) * 1e3)
+    print(json.dumps({"profile": "inverse_c2_random_targets", "ms": times,
+                      "nan": int(torch.isnan(xi).sum().item())}), flush=True)
+
+
+def single_component_workload(steps):
+    """One component, dim 6, total order 4 (210 terms), ProbabilistHermite /
+    SoftPlus / adaptive Clenshaw-Curtis (quadPts 5, tolerances 1e-8);
+    N = 1,000,000 points x ~ N(0, I_6), coefficients ~ N(0, 0.05^2), targets
+    r ~ N(0, 1) from one seed-0 stream.  Times Evaluate, LogDeterminant,
+    CoeffGrad (sens = 1, 210 x N output) and Inverse (x_{1:5} fixed, x_6 as
+    the initial guess) with preallocated outputs."""
+    import torch
+
+    g, x, r, N = single_component_setup()
     g.PrepareCodegen()
     y = torch.empty((1, N), dtype=torch.float64, device="cuda")
     ld = torch.empty(N, dtype=torch.float64, device="cuda")
@@ -261,7 +288,9 @@ def profile_mode(args):
     b = torch.empty_like(a)
     torch.mul(a, 1.0, out=b)  # reads 256 MiB, writes 256 MiB
     torch.cuda.synchronize()
+    choice, trials = tri.FusedSchedule()
     print(json.dumps({"profile": "kl_fused_c3_banana", "ms_per_step": ms, "event_ms": ev, "loss": fl,
+                      "fused_schedule": choice, "trial_ms": trials,
                       "calibration": "torch.mul over 256 MiB (reads 262144 KiB, writes 262144 KiB)"}), flush=True)
 
 
@@ -279,6 +308,7 @@ def main():
     ap.add_argument("--no-single", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--profile", action="store_true")
+    ap.add_argument("--profile-inverse", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -287,6 +317,9 @@ def main():
         sys.exit(2)
     if args.profile:
         profile_mode(args)
+        return
+    if args.profile_inverse:
+        profile_inverse_mode(args)
         return
 
     import torch
@@ -314,7 +347,7 @@ def main():
     head["oracle_max_rel_err_2000pts"] = oracle_check(x_np, c0, gpu_kl)
     choice, trials = tri.FusedSchedule()
     fused_schedule = {"choice": choice, "trial_ms": trials,
-                      "meaning": "bit 0: three-waves-per-SIMD kernel, bit 1: doubled persistent grid"}
+                      "meaning": "bit 0: three-waves-per-SIMD kernel, bit 1: doubled persistent grid; trial -1: not tried (that kernel spills registers)"}
 
     second = None
     if not args.no_second:
@@ -336,8 +369,13 @@ def main():
     if not args.no_single:
         single = single_component_workload(max(3, args.plain_steps))
 
-    bytes_alg = 8.0 * D * N + 8.0 * (tri.numCoeffs + 1)
     traffic = load_traffic()
+    # HBM bytes of the schedule the autotuner chose (measured per schedule; a
+    # schedule without a measurement reports null)
+    tsched = None
+    if traffic and fused_schedule["choice"] is not None:
+        tsched = traffic["schedules"].get(str(fused_schedule["choice"]))
+    bytes_alg = tsched["algorithmic_bytes"] if tsched else 8.0 * D * N + 8.0 * tri.numCoeffs
     out = {
         "metric": METRIC,
         "value": head["points_per_s"],
@@ -356,12 +394,13 @@ def main():
                    "N": N, "dim": D, "order": ORDER, "num_coeffs": tri.numCoeffs},
         "roofline": {"bound": "mfma", "achieved": head["achieved_tflops"], "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": head["achieved_tflops"] / PEAK_FP64_TFLOPS,
-                     "traffic": traffic["bytes_per_launch"] if traffic else None,
+                     "traffic": tsched["bytes_per_launch"] if tsched else None,
                      "note": "compute-bound FP64 on the vector ALUs (no MFMA: nothing is a dense contraction); "
                              "MI355X FP64 peak 78.6 TFLOP/s; achieved = algorithmic flops of the launch "
                              "(mpart_amd/roofline.py) / HIP-event time of the fused kernel + its reduction",
                      "algorithmic_flops_per_launch": head["algorithmic_flops"],
                      "algorithmic_bytes_per_launch": bytes_alg,
+                     "algorithmic_bytes_note": "points 8*D*N + coefficients 8*K + partial rows 8*(K+1) per block",
                      "traffic_source": traffic["source"] if traffic else None},
         "cpu_baseline": None,
         "fused": head,

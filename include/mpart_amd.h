@@ -242,7 +242,8 @@ const char* mpart_codegen_error(mpart_map map);
  * the caller's stream and keeps the fastest for the map.  *choice is the
  * schedule index (bit 0: three-waves kernel, bit 1: doubled grid) or -1
  * before tuning; trialMs (4 doubles, may be NULL) receives the trial times in
- * milliseconds.  No reference counterpart. */
+ * milliseconds, -1 for a schedule left out because its kernel would spill
+ * registers.  No reference counterpart. */
 int mpart_fused_schedule(mpart_map map, int32_t* choice, double* trialMs);
 
 #ifdef __cplusplus

@@ -212,6 +212,7 @@ struct mpart_map_s {
     std::string genErr;
     int genOccK[2] = {0, 0};
     size_t genOccSmem = 0;
+    bool genW3Spills = false;  // the three-waves kernel needs more private memory (register spills)
     int fusedChoice = -1;  // tuned schedule of the fused kernel (-1: not yet)
     double fusedTrialMs[4] = {0, 0, 0, 0};
 };
@@ -808,6 +809,11 @@ void do_fused(mpart_map_s& m, const mpart_cview& p, const double* sens, long srs
                           "hipModuleOccupancyMaxActiveBlocksPerMultiprocessor");
                 m.genOccK[k] = std::max(1, nb);
             }
+            int priv[2] = {0, 0};
+            for (int k = 0; k < 2; ++k)
+                hip_check(hipFuncGetAttribute(&priv[k], HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, fns[k]),
+                          "hipFuncGetAttribute");
+            m.genW3Spills = priv[1] > priv[0];
             m.genOccSmem = gsmem;
         }
         const long ntiles = std::max<long>(1, (N + gblock - 1) / gblock);
@@ -836,6 +842,8 @@ void do_fused(mpart_map_s& m, const mpart_cview& p, const double* sens, long srs
         };
         // Autotuning: on the first call with enough points (and not under
         // stream capture) every schedule runs twice and the faster one is kept.
+        // The three-waves kernel is left out when its register cap makes it
+        // spill (the spill traffic goes to memory); its trials report -1.
         if (m.fusedChoice < 0 && N >= FUSED_TUNE_MIN_POINTS) {
             hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
             hip_check(hipStreamIsCapturing(s, &cs), "hipStreamIsCapturing");
@@ -845,6 +853,10 @@ void do_fused(mpart_map_s& m, const mpart_cview& p, const double* sens, long srs
                 hip_check(hipEventCreate(&e1), "hipEventCreate");
                 int best = 0;
                 for (int cfg = 0; cfg < 4; ++cfg) {
+                    if ((cfg & 1) && m.genW3Spills) {
+                        m.fusedTrialMs[cfg] = -1.0;
+                        continue;
+                    }
                     launch_cfg(cfg);
                     hip_check(hipEventRecord(e0, s), "hipEventRecord");
                     launch_cfg(cfg);

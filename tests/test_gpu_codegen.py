@@ -136,7 +136,9 @@ def test_fused_schedule_autotuning():
     l1, g1 = g.KLValueAndGrad(x)
     choice, trials = g.FusedSchedule()
     print(f"[autotune] choice {choice}, trial ms {['%.3f' % t for t in trials]}", flush=True)
-    assert 0 <= choice <= 3 and all(t > 0 for t in trials)
+    # schedules whose kernel would spill registers are not tried (-1)
+    assert 0 <= choice <= 3 and trials[choice] > 0
+    assert trials[0] > 0 and trials[2] > 0 and all(t > 0 or t == -1.0 for t in trials)
     lp, gp = g.KLValueAndGrad(x, plain=True)
     close(l1.item(), lp.item(), rtol=1e-12, what="loss")
     close(g1.cpu().numpy(), gp.cpu().numpy(), rtol=1e-12, what="grad")
