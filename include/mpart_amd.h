@@ -169,6 +169,37 @@ int mpart_kl_value_and_grad(mpart_map map, mpart_cview pts, double* loss, double
  * then a row reduction (src/MapObjective.cpp:107-215). */
 int mpart_kl_value_and_grad_plain(mpart_map map, mpart_cview pts, double* loss, double* grad, mpart_stream stream);
 
+/* TrainOptions (MParT/TrainMap.h:19-40): NLopt-style stopping rules.  The
+ * reference's opt_alg names an NLopt algorithm (default "LD_SLSQP"); here the
+ * one optimizer is L-BFGS (MPART_TRAIN_LBFGS) with a strong-Wolfe line search;
+ * `history` is its memory.  mpart_default_train_options() gives the reference
+ * defaults (ftol_rel = ftol_abs = 1e-3, xtol_rel = xtol_abs = 1e-4,
+ * maxeval = 1000, stopval = -inf, maxtime = inf) and history = 10. */
+#define MPART_TRAIN_LBFGS 0
+typedef struct mpart_train_options {
+    int32_t algorithm;
+    double stopval;
+    double ftol_rel, ftol_abs;
+    double xtol_rel, xtol_abs;
+    int32_t maxeval;
+    double maxtime;
+    int32_t verbose;
+    int32_t history;
+} mpart_train_options;
+void mpart_default_train_options(mpart_train_options* opts);
+
+/* TrainMap(map, CreateGaussianKLObjective(train), options) (src/TrainMap.cpp:168-216):
+ * minimizes the standard-Gaussian KL objective over the map's coefficients,
+ * a triangular map one component at a time on its rows of `train`
+ * (TrainMap_Triangular_StandardNormalDensity, 141-166).  Every objective
+ * evaluation is one fused KL pass on the device; the optimizer runs on the
+ * host.  *error receives the sum of the components' final objective values
+ * (mean(T_i^2 / 2 - log dT_i/dx_i), without the Gaussian normalization
+ * constant), *evals the number of objective evaluations; the trained
+ * coefficients stay in the map.  train is a device view (inputDim x N). */
+int mpart_train_map(mpart_map map, mpart_cview train, const mpart_train_options* opts, double* error, int32_t* evals,
+                    mpart_stream stream);
+
 /* Fused vector-Jacobian product used by the torch backward
  * (bindings/python/package/torch_helpers.py:91-107):
  * grad = sum_n [ CoeffGrad(sens)(:, n) + logdetSens[n] LogDeterminantCoeffGrad(:, n) ].

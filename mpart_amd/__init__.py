@@ -15,6 +15,7 @@ numpy inputs are copied to the GPU and results copied back; torch CUDA tensors
 stay on the device.
 """
 import ctypes
+import math
 import enum
 import os
 
@@ -48,6 +49,21 @@ class _COptions(ctypes.Structure):
         ("quadAbsTol", ctypes.c_double),
         ("quadRelTol", ctypes.c_double),
         ("nugget", ctypes.c_double),
+    ]
+
+
+class _CTrainOptions(ctypes.Structure):
+    _fields_ = [
+        ("algorithm", ctypes.c_int32),
+        ("stopval", ctypes.c_double),
+        ("ftol_rel", ctypes.c_double),
+        ("ftol_abs", ctypes.c_double),
+        ("xtol_rel", ctypes.c_double),
+        ("xtol_abs", ctypes.c_double),
+        ("maxeval", ctypes.c_int32),
+        ("maxtime", ctypes.c_double),
+        ("verbose", ctypes.c_int32),
+        ("history", ctypes.c_int32),
     ]
 
 
@@ -96,6 +112,8 @@ _SIGS = {
     "mpart_batch_inverse": [_vp, _vp, _vp, _vp, _dbl, _dbl, _vp],
     "mpart_codegen_prepare": [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)],
     "mpart_fused_schedule": [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)],
+    "mpart_train_map": [_vp, _CView, ctypes.POINTER(_CTrainOptions), ctypes.POINTER(ctypes.c_double),
+                        ctypes.POINTER(ctypes.c_int32), _vp],
 }
 for _name, _args in _SIGS.items():
     _f = getattr(_lib, _name)
@@ -107,9 +125,12 @@ _lib.mpart_default_options.argtypes = [_OPTP]
 _lib.mpart_default_options.restype = None
 _lib.mpart_codegen_error.argtypes = [_vp]
 _lib.mpart_codegen_error.restype = ctypes.c_char_p
+_lib.mpart_default_train_options.argtypes = [ctypes.POINTER(_CTrainOptions)]
+_lib.mpart_default_train_options.restype = None
 
 LIB_PATH = _LIB_PATH
-C_EXPORTS = tuple(_SIGS) + ("mpart_last_error", "mpart_default_options", "mpart_codegen_error")
+C_EXPORTS = tuple(_SIGS) + ("mpart_last_error", "mpart_default_options", "mpart_codegen_error",
+                            "mpart_default_train_options")
 
 
 def _check(rc):
@@ -586,3 +607,106 @@ class MapBatch:
 __all__ = ["MapOptions", "BasisTypes", "PosFuncTypes", "QuadTypes", "FixedMultiIndexSet", "ConditionalMap",
            "CreateComponent", "CreateTotalOrderComponent", "CreateTriangular", "TriangularMap", "MapBatch",
            "Initialize"]
+
+
+# ---------------------------------------------------------------------------
+# Training (TrainOptions, MParT/TrainMap.h:19-66; CreateGaussianKLObjective,
+# src/MapObjective.cpp:41-52; TrainMap, src/TrainMap.cpp:168-216).
+
+_TRAIN_ALGS = ("LD_LBFGS", "LD_SLSQP")
+
+
+class TrainOptions:
+    """Fields and defaults of the reference's TrainOptions.  The optimizer is
+    L-BFGS with a strong-Wolfe line search; opt_alg accepts "LD_LBFGS" and
+    the reference's default "LD_SLSQP" (both run L-BFGS, another quasi-Newton
+    method on the same unconstrained problem) and rejects other NLopt names.
+    ``history`` (L-BFGS memory) has no reference counterpart."""
+
+    def __init__(self, **kw):
+        self.opt_alg = "LD_SLSQP"
+        self.opt_stopval = -math.inf
+        self.opt_ftol_rel = 1e-3
+        self.opt_ftol_abs = 1e-3
+        self.opt_xtol_rel = 1e-4
+        self.opt_xtol_abs = 1e-4
+        self.opt_maxeval = 1000
+        self.opt_maxtime = math.inf
+        self.verbose = 0
+        self.history = 10
+        for k, v in kw.items():
+            if not hasattr(self, k):
+                raise AttributeError(f"TrainOptions has no field {k}")
+            setattr(self, k, v)
+
+    def String(self):
+        return "\n".join(f"{k} = {getattr(self, k)}" for k in (
+            "opt_alg", "opt_stopval", "opt_ftol_rel", "opt_ftol_abs", "opt_xtol_rel", "opt_xtol_abs",
+            "opt_maxeval", "opt_maxtime", "verbose"))
+
+    __str__ = String
+
+    def _c(self):
+        if self.opt_alg not in _TRAIN_ALGS:
+            raise NotImplementedError(f"TrainOptions: optimizer {self.opt_alg!r} is not available (L-BFGS only)")
+        o = _CTrainOptions()
+        _lib.mpart_default_train_options(ctypes.byref(o))
+        o.stopval, o.ftol_rel, o.ftol_abs = self.opt_stopval, self.opt_ftol_rel, self.opt_ftol_abs
+        o.xtol_rel, o.xtol_abs = self.opt_xtol_rel, self.opt_xtol_abs
+        o.maxeval, o.maxtime, o.verbose, o.history = int(self.opt_maxeval), self.opt_maxtime, int(self.verbose), \
+            int(self.history)
+        return o
+
+
+class KLObjective:
+    """KL divergence to the standard Gaussian (KLObjective with a
+    GaussianSamplerDensity, src/MapObjective.cpp:41-91): the objective of a map
+    on samples X is mean_n(|T(x_n)|^2 / 2 + d log(2 pi) / 2 - log det dT(x_n)),
+    d = the map's output dimension.  Samples are kept on the device."""
+
+    def __init__(self, train, test=None, dim=0):
+        self.train, _ = _as_dev(train)
+        self.test = _as_dev(test)[0] if test is not None else None
+        self.dim = int(dim) if dim else int(self.train.shape[0])
+
+    def _value(self, X, map):
+        loss, _ = map.KLValueAndGrad(X)
+        return float(loss.item()) + 0.5 * map.outputDim * math.log(2.0 * math.pi)
+
+    def TrainError(self, map):
+        return self._value(self.train, map)
+
+    def TestError(self, map):
+        if self.test is None:
+            raise RuntimeError("No test dataset given!")
+        return self._value(self.test, map)
+
+    def ObjectivePlusCoeffGrad(self, map, X=None):
+        """(objective, coefficient gradient) on X (default: the training samples)."""
+        X = self.train if X is None else _as_dev(X)[0]
+        loss, grad = map.KLValueAndGrad(X)
+        return float(loss.item()) + 0.5 * map.outputDim * math.log(2.0 * math.pi), grad
+
+
+def CreateGaussianKLObjective(train, test=None, dim=0):
+    """ObjectiveFactory::CreateGaussianKLObjective(train[, test], dim)."""
+    if isinstance(test, int) and dim == 0:
+        test, dim = None, test
+    return KLObjective(train, test, dim)
+
+
+def TrainMap(map, objective, options=None):
+    """Trains `map` in place on the objective's training samples and returns
+    the sum of the per-component objective values (without the Gaussian
+    normalization constant), as the reference's TrainMap returns for a
+    triangular map under the Gaussian KL objective."""
+    if not isinstance(objective, KLObjective):
+        raise NotImplementedError("TrainMap: only the Gaussian KL objective is available")
+    if objective.dim != map.outputDim:
+        raise ValueError(f"TrainMap: objective dimension {objective.dim} != map output dimension {map.outputDim}")
+    opts = (options or TrainOptions())._c()
+    err, nev = ctypes.c_double(), ctypes.c_int32()
+    _check(_lib.mpart_train_map(map.handle, _view(objective.train), ctypes.byref(opts), ctypes.byref(err),
+                                ctypes.byref(nev), _stream()))
+    map.last_train_evaluations = nev.value
+    return err.value

@@ -8,6 +8,7 @@
 #include <cstdlib>
 #include <cstdint>
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
@@ -1072,6 +1073,280 @@ mpd::BatchIO base_io(mpart_map m) {
     return io;
 }
 
+// ---------------------------------------------------------------------------
+// Training.  TrainMap (src/TrainMap.cpp:168-216) trains a triangular map under
+// the standard-Gaussian KL objective one component at a time
+// (TrainMap_Triangular_StandardNormalDensity, 141-166): component i sees rows
+// [0, inputDim_i) of the samples through FastGaussianReverseKLObjective
+// (src/MapObjective.cpp:93-234), L_i = mean(T_i^2 / 2 - log dT_i/dx_i), and the
+// returned error is the sum of the L_i at the end.  Here every objective
+// evaluation is one fused KL pass (value + coefficient gradient).  The
+// optimizer is L-BFGS (Nocedal & Wright, Numerical Optimization, 2nd ed.,
+// Algorithm 7.4, memory `history`) with a strong-Wolfe line search
+// (Algorithms 3.5 / 3.6, c1 = 1e-4, c2 = 0.9, cubic interpolation) in place of
+// NLopt, stopped by NLopt's rules (TrainOptions, MParT/TrainMap.h:19-40):
+// f <= stopval; |df| < ftol_abs or |df| < ftol_rel |f|; every |dx_i| < xtol_abs
+// or < xtol_rel |x_i|; maxeval objective evaluations; maxtime seconds.
+
+// Value and coefficient gradient of the KL objective of one map at host
+// coefficients w (written into the map's coefficient storage).
+struct KLObjectiveEval {
+    mpart_map_s& m;
+    mpart_cview pts;
+    hipStream_t s;
+    DBuf dev;  // [w (K) | loss | grad (K)]
+    std::vector<double> host;
+    int evals = 0;
+    KLObjectiveEval(mpart_map_s& m_, const mpart_cview& p, hipStream_t st) : m(m_), pts(p), s(st) {
+        dev.ensure((2 * (size_t)m.numCoeffs + 1) * sizeof(double));
+        host.resize((size_t)m.numCoeffs + 1);
+    }
+    double operator()(const std::vector<double>& w, std::vector<double>& g) {
+        const long K = m.numCoeffs;
+        double* d = dev.as<double>();
+        hip_check(hipMemcpyAsync(d, w.data(), K * sizeof(double), hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+        hip_check(mpd::launch_copy(d, 1, K, 0, 1, m.coeffs, 0, m.coeffStride, s), "launch copy");
+        if (m.opts.contDeriv) do_fused(m, pts, nullptr, 0, 0, nullptr, 0, 1, d + K, d + K + 1, 1.0 / (double)pts.cols, s);
+        else do_kl_plain(m, pts, d + K, d + K + 1, s);
+        hip_check(hipMemcpyAsync(host.data(), d + K, (K + 1) * sizeof(double), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+        hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+        ++evals;
+        g.assign(host.begin() + 1, host.end());
+        const double f = host[0];
+        return std::isnan(f) ? std::numeric_limits<double>::infinity() : f;
+    }
+};
+
+double dot(const std::vector<double>& a, const std::vector<double>& b) {
+    double t = 0.0;
+    for (size_t i = 0; i < a.size(); ++i) t += a[i] * b[i];
+    return t;
+}
+
+// Minimizer of the cubic interpolating (a, fa, da) and (b, fb, db), kept in
+// [lo + 0.1 (hi - lo), hi - 0.1 (hi - lo)] of the interval they span.
+double cubic_step(double a, double fa, double da, double b, double fb, double db) {
+    const double lo = std::min(a, b), hi = std::max(a, b);
+    const double d1 = da + db - 3.0 * (fa - fb) / (a - b);
+    const double r = d1 * d1 - da * db;
+    double t = 0.5 * (lo + hi);
+    if (r >= 0.0 && std::isfinite(r)) {
+        const double d2 = std::copysign(std::sqrt(r), b - a);
+        t = b - (b - a) * (db + d2 - d1) / (db - da + 2.0 * d2);
+    }
+    if (!std::isfinite(t)) t = 0.5 * (lo + hi);
+    return std::min(std::max(t, lo + 0.1 * (hi - lo)), hi - 0.1 * (hi - lo));
+}
+
+struct TrainResult {
+    double f;
+    int evals;
+    const char* status;
+};
+
+TrainResult lbfgs_minimize(KLObjectiveEval& fg, std::vector<double>& x, const mpart_train_options& o) {
+    const auto t0 = std::chrono::steady_clock::now();
+    auto elapsed = [&] { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
+    const size_t n = x.size();
+    const int hist = std::max(1, (int)o.history);
+    std::vector<double> g(n), d(n), xn(n), gn(n);
+    double f = fg(x, g);
+    if (!std::isfinite(f)) return {f, fg.evals, "objective not finite at the initial coefficients"};
+    if (f <= o.stopval) return {f, fg.evals, "stopval reached"};
+    std::vector<std::vector<double>> S, Y;
+    std::vector<double> rho;
+    auto budget = [&] { return fg.evals < o.maxeval && elapsed() < o.maxtime; };
+    for (int iter = 0;; ++iter) {
+        if (!budget()) return {f, fg.evals, fg.evals >= o.maxeval ? "maxeval reached" : "maxtime reached"};
+        // Two-loop recursion: d = -H g.
+        for (size_t i = 0; i < n; ++i) d[i] = -g[i];
+        const int mcur = (int)S.size();
+        std::vector<double> alpha(mcur);
+        for (int j = mcur - 1; j >= 0; --j) {
+            alpha[j] = rho[j] * dot(S[j], d);
+            for (size_t i = 0; i < n; ++i) d[i] -= alpha[j] * Y[j][i];
+        }
+        if (mcur > 0) {
+            const double gamma = dot(S.back(), Y.back()) / dot(Y.back(), Y.back());
+            for (size_t i = 0; i < n; ++i) d[i] *= gamma;
+        }
+        for (int j = 0; j < mcur; ++j) {
+            const double b = rho[j] * dot(Y[j], d);
+            for (size_t i = 0; i < n; ++i) d[i] += (alpha[j] - b) * S[j][i];
+        }
+        double gtd = dot(g, d);
+        if (!(gtd < 0.0)) {  // not a descent direction: restart from steepest descent
+            S.clear();
+            Y.clear();
+            rho.clear();
+            for (size_t i = 0; i < n; ++i) d[i] = -g[i];
+            gtd = dot(g, d);
+            if (!(gtd < 0.0)) return {f, fg.evals, "zero gradient"};
+        }
+        double gnorm1 = 0.0;
+        for (double v : g) gnorm1 += std::fabs(v);
+        double step = (mcur == 0) ? std::min(1.0, 1.0 / gnorm1) : 1.0;
+        // Strong-Wolfe line search (bracketing, then zoom).
+        const double c1 = 1e-4, c2 = 0.9;
+        double aPrev = 0.0, fPrev = f, dPrev = gtd;
+        double aLo = 0.0, fLo = f, dLo = gtd, aHi = 0.0, fHi = 0.0, dHi = 0.0;
+        bool zoom = false, found = false;
+        double fNew = f, aNew = 0.0;
+        std::vector<double> gLo = g;
+        auto eval_at = [&](double a, double& fa, double& da) {
+            for (size_t i = 0; i < n; ++i) xn[i] = x[i] + a * d[i];
+            fa = fg(xn, gn);
+            da = dot(gn, d);
+        };
+        for (int ls = 0; ls < 25 && budget(); ++ls) {
+            double fa, da;
+            eval_at(step, fa, da);
+            if (fa > f + c1 * step * gtd || (ls > 0 && fa >= fPrev)) {
+                aLo = aPrev; fLo = fPrev; dLo = dPrev;
+                aHi = step; fHi = fa; dHi = da;
+                zoom = true;
+                break;
+            }
+            if (std::fabs(da) <= -c2 * gtd) {
+                found = true; aNew = step; fNew = fa;
+                break;
+            }
+            if (da >= 0.0) {
+                aLo = step; fLo = fa; dLo = da; gLo = gn;
+                aHi = aPrev; fHi = fPrev; dHi = dPrev;
+                zoom = true;
+                break;
+            }
+            aPrev = step; fPrev = fa; dPrev = da;
+            aLo = step; fLo = fa; dLo = da; gLo = gn;
+            step *= 2.0;
+        }
+        if (zoom) {
+            for (int z = 0; z < 25 && budget(); ++z) {
+                if (std::fabs(aHi - aLo) * std::sqrt(dot(d, d)) < 1e-12) break;
+                const double a = cubic_step(aLo, fLo, dLo, aHi, std::isfinite(fHi) ? fHi : fLo + 1e300, dHi);
+                double fa, da;
+                eval_at(a, fa, da);
+                if (fa > f + c1 * a * gtd || fa >= fLo) {
+                    aHi = a; fHi = fa; dHi = da;
+                } else {
+                    if (std::fabs(da) <= -c2 * gtd) {
+                        found = true; aNew = a; fNew = fa;
+                        break;
+                    }
+                    if (da * (aHi - aLo) >= 0.0) {
+                        aHi = aLo; fHi = fLo; dHi = dLo;
+                    }
+                    aLo = a; fLo = fa; dLo = da; gLo = gn;
+                }
+            }
+        }
+        if (!found) {
+            // Best point with sufficient decrease seen by the search, if any.
+            if (aLo > 0.0 && fLo < f) {
+                aNew = aLo;
+                fNew = fLo;
+                gn = gLo;
+            } else {
+                return {f, fg.evals, budget() ? "line search made no progress" : "maxeval reached"};
+            }
+        }
+        // Accept the step; gn holds the gradient at x + aNew d.
+        for (size_t i = 0; i < n; ++i) xn[i] = x[i] + aNew * d[i];
+        bool xconv = true;
+        std::vector<double> sv(n), yv(n);
+        for (size_t i = 0; i < n; ++i) {
+            sv[i] = xn[i] - x[i];
+            yv[i] = gn[i] - g[i];
+            const double dx = std::fabs(sv[i]);
+            if (!(dx < o.xtol_abs || dx < o.xtol_rel * std::fabs(xn[i]))) xconv = false;
+        }
+        const double df = std::fabs(fNew - f);
+        const bool fconv = df < o.ftol_abs || df < o.ftol_rel * std::fabs(fNew);
+        x = xn;
+        g = gn;
+        f = fNew;
+        if (o.verbose > 1) printf("  iteration %d: f = %.12g, step %.3g, evaluations %d\n", iter, f, aNew, fg.evals);
+        if (f <= o.stopval) return {f, fg.evals, "stopval reached"};
+        if (fconv) return {f, fg.evals, "ftol reached"};
+        if (xconv) return {f, fg.evals, "xtol reached"};
+        const double sy = dot(sv, yv);
+        if (sy > 1e-10 * std::sqrt(dot(sv, sv) * dot(yv, yv))) {
+            if ((int)S.size() == hist) {
+                S.erase(S.begin());
+                Y.erase(Y.begin());
+                rho.erase(rho.begin());
+            }
+            S.push_back(sv);
+            Y.push_back(yv);
+            rho.push_back(1.0 / sy);
+        }
+    }
+}
+
+// One map (a component, or any map trained as a whole): returns the final
+// objective value and leaves the trained coefficients in the map.
+TrainResult train_one(mpart_map_s& m, const mpart_cview& pts, const mpart_train_options& o, hipStream_t s) {
+    check_coeffs(&m);
+    const long K = m.numCoeffs;
+    std::vector<double> x(K);
+    DBuf tmp;
+    tmp.ensure(std::max<long>(1, K) * sizeof(double));
+    hip_check(mpd::launch_copy(m.coeffs, 1, K, 0, m.coeffStride, tmp.as<double>(), 0, 1, s), "launch copy");
+    hip_check(hipMemcpyAsync(x.data(), tmp.as<double>(), K * sizeof(double), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    KLObjectiveEval fg(m, pts, s);
+    const TrainResult r = lbfgs_minimize(fg, x, o);
+    // The map keeps the accepted iterate (the last evaluation may have been a
+    // rejected trial point of the line search).
+    double* d = fg.dev.as<double>();
+    hip_check(hipMemcpyAsync(d, x.data(), K * sizeof(double), hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+    hip_check(mpd::launch_copy(d, 1, K, 0, 1, m.coeffs, 0, m.coeffStride, s), "launch copy");
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    return r;
+}
+
+void do_train(mpart_map_s& m, const mpart_cview& pts, const mpart_train_options& o, double* error, int32_t* evals,
+              hipStream_t s) {
+    if (o.algorithm != MPART_TRAIN_LBFGS) throw NotImpl("TrainMap: only the L-BFGS optimizer is available");
+    if (o.maxeval <= 0) throw ArgError("TrainMap: opt_maxeval must be positive");
+    if (o.history <= 0) throw ArgError("TrainMap: history must be positive");
+    check_view("TrainMap: samples", pts.rows, pts.cols, m.inDim, pts.cols);
+    if (pts.cols == 0) throw ArgError("TrainMap: no samples");
+    check_coeffs(&m);
+    double total = 0.0;
+    int nev = 0;
+    if (m.outDim == 1 && m.comps.size() == 1) {
+        const TrainResult r = train_one(m, pts, o, s);
+        if (o.verbose) printf("Optimization result: %s\nOptimization error: %.12g\nOptimization evaluations: %d\n", r.status, r.f, r.evals);
+        total = r.f;
+        nev = r.evals;
+    } else {
+        if (o.verbose) printf("Detected standard normal reference density with Triangular Map\n");
+        for (size_t i = 0; i < m.comps.size(); ++i) {
+            std::unique_ptr<mpart_map_s> comp(make_map(m.opts, std::vector<MultiSet>{m.comps[i]}, false));
+            const long nt = comp->numCoeffs;
+            double* block = m.coeffs + (long)m.coeffOff[i] * m.coeffStride;
+            hip_check(mpd::launch_copy(block, 1, nt, 0, m.coeffStride, comp->ownCoeffs.as<double>(), 0, 1, s), "launch copy");
+            mpart_cview sub = pts;
+            sub.rows = comp->inDim;
+            if (o.verbose) printf("Training component %zu...\n", i);
+            const TrainResult r = train_one(*comp, sub, o, s);
+            hip_check(mpd::launch_copy(comp->coeffs, 1, nt, 0, comp->coeffStride, block, 0, m.coeffStride, s),
+                      "launch copy");
+            hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");  // before the component is freed
+            if (o.verbose)
+                printf("Optimization result: %s\nOptimization error: %.12g\nOptimization evaluations: %d\nComponent %zu trained.\n\n",
+                       r.status, r.f, r.evals, i);
+            total += r.f;
+            nev += r.evals;
+        }
+        hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    }
+    if (error) *error = total;
+    if (evals) *evals = nev;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1317,6 +1592,29 @@ int mpart_coeff_vjp(mpart_map m, mpart_cview pts, mpart_cview sens, const double
         if (sens.ptr) check_view("vjp: sens", sens.rows, sens.cols, m->outDim, pts.cols);
         if (!m->opts.contDeriv) throw NotImpl("contDeriv = false is not provided yet");
         do_fused(*m, pts, sens.ptr, sens.row_stride, sens.col_stride, lsens, lss, 0, nullptr, grad, 1.0, S(s));
+    });
+}
+
+void mpart_default_train_options(mpart_train_options* o) {
+    if (!o) return;
+    o->algorithm = MPART_TRAIN_LBFGS;
+    o->stopval = -std::numeric_limits<double>::infinity();
+    o->ftol_rel = 1e-3;
+    o->ftol_abs = 1e-3;
+    o->xtol_rel = 1e-4;
+    o->xtol_abs = 1e-4;
+    o->maxeval = 1000;
+    o->maxtime = std::numeric_limits<double>::infinity();
+    o->verbose = 0;
+    o->history = 10;
+}
+
+int mpart_train_map(mpart_map m, mpart_cview train, const mpart_train_options* opts, double* error, int32_t* evals,
+                    mpart_stream s) {
+    return guarded([&] {
+        check_map(m);
+        if (!opts) throw ArgError("null argument");
+        do_train(*m, train, *opts, error, evals, S(s));
     });
 }
 
