@@ -44,6 +44,8 @@ struct Options {
     double quadAbsTol;
     double quadRelTol;
     double nugget;
+    double basisLB;  // LinearizedBasis bounds (MapOptions.h:76-77); +-inf: not linearized
+    double basisUB;
 };
 
 // Derivative flags, MParT/DerivativeFlags.h:7-15.
@@ -202,6 +204,39 @@ struct OrthoPoly {
                 d2[k] /= n;
             }
     }
+    // Derivative(order, x) (OrthogonalPolynomial.h:192-226): upward recurrence.
+    double derivative(unsigned order, double x) const {
+        if (order == 0) return 0.0;
+        if (order == 1) return normalize ? phi1_deriv(x) / norm(1) : phi1_deriv(x);
+        double lag2_val, lag1_val = phi0(x), next_val = phi1(x);
+        double lag2_deriv, lag1_deriv = 0.0, next_deriv = phi1_deriv(x);
+        for (unsigned i = 2; i <= order; ++i) {
+            double a = ak(i), b = bk(i), c = ck(i);
+            lag2_val = lag1_val;
+            lag1_val = next_val;
+            next_val = (a * x + b) * lag1_val - c * lag2_val;
+            lag2_deriv = lag1_deriv;
+            lag1_deriv = next_deriv;
+            next_deriv = a * lag1_val + (a * x + b) * lag1_deriv - c * lag2_deriv;
+        }
+        return normalize ? next_deriv / norm(order) : next_deriv;
+    }
+    // Evaluate(order, x) (OrthogonalPolynomial.h:160-190): downward Clenshaw sum.
+    double evaluate(unsigned order, double x) const {
+        double nrm = normalize ? norm(order) : 1.0;
+        if (order == 0) return phi0(x) / nrm;
+        if (order == 1) return phi1(x) / nrm;
+        double yk2 = 0.0, yk1 = 0.0, yk = 1.0;
+        for (int k = (int)order - 1; k >= 0; k--) {
+            yk2 = yk1;
+            yk1 = yk;
+            double alpha = ak(k + 1) * x + bk(k + 1);
+            double beta = -ck(k + 2);
+            yk = alpha * yk1 + beta * yk2;
+        }
+        double beta = -ck(2);
+        return (yk1 * phi1(x) + beta * phi0(x) * yk2) / nrm;
+    }
 };
 
 // Hermite functions, MParT/HermiteFunction.h:22-104: phi_0 = 1, phi_1 = x and
@@ -248,6 +283,14 @@ struct HermiteFn {
             }
         }
     }
+    // Derivative(order, x) (HermiteFunction.h:117-129).
+    double derivative(unsigned order, double x) const {
+        if (order == 0) return 0.0;
+        if (order == 1) return 1.0;
+        const double scaling = std::pow(std::pow(2, order - 2) * factorial_u32(order - 2) * std::sqrt(M_PI), -0.5);
+        const double expPart = std::exp(-0.5 * x * x);
+        return scaling * (-x * expPart * base.evaluate(order - 2, x) + expPart * base.derivative(order - 2, x));
+    }
     void eval_second(double* v, double* d, double* d2, unsigned maxOrder, double x) const {
         eval_derivs(v, d, maxOrder, x);
         d2[0] = 0.0;
@@ -263,14 +306,55 @@ struct Basis1D {
     int type = PROB_HERMITE;
     OrthoPoly poly;
     HermiteFn hf;
-    void eval_all(double* o, unsigned m, double x) const {
+    // LinearizedBasis (MParT/LinearizedBasis.h:17-160): linear outside [lb, ub]
+    // with the value and slope of the basis at the nearer bound.
+    bool lin = false;
+    double lb = -INFINITY, ub = INFINITY;
+
+    void raw_all(double* o, unsigned m, double x) const {
         if (type == HERMITE_FN) hf.eval_all(o, m, x); else poly.eval_all(o, m, x);
     }
-    void eval_derivs(double* v, double* d, unsigned m, double x) const {
+    void raw_derivs(double* v, double* d, unsigned m, double x) const {
         if (type == HERMITE_FN) hf.eval_derivs(v, d, m, x); else poly.eval_derivs(v, d, m, x);
     }
-    void eval_second(double* v, double* d, double* d2, unsigned m, double x) const {
+    void raw_second(double* v, double* d, double* d2, unsigned m, double x) const {
         if (type == HERMITE_FN) hf.eval_second(v, d, d2, m, x); else poly.eval_second(v, d, d2, m, x);
+    }
+    double raw_derivative(unsigned k, double x) const {
+        return type == HERMITE_FN ? hf.derivative(k, x) : poly.derivative(k, x);
+    }
+    // LinearizedBasis::EvaluateAll (LinearizedBasis.h:41-61).
+    void eval_all(double* o, unsigned m, double x) const {
+        if (lin && x < lb) {
+            raw_all(o, m, lb);
+            for (unsigned i = 0; i <= m; ++i) o[i] += raw_derivative(i, lb) * (x - lb);
+        } else if (lin && x > ub) {
+            raw_all(o, m, ub);
+            for (unsigned i = 0; i <= m; ++i) o[i] += raw_derivative(i, ub) * (x - ub);
+        } else {
+            raw_all(o, m, x);
+        }
+    }
+    // LinearizedBasis::EvaluateDerivatives (63-87).
+    void eval_derivs(double* v, double* d, unsigned m, double x) const {
+        if (lin && x < lb) {
+            raw_derivs(v, d, m, lb);
+            for (unsigned i = 0; i <= m; ++i) v[i] += d[i] * (x - lb);
+        } else if (lin && x > ub) {
+            raw_derivs(v, d, m, ub);
+            for (unsigned i = 0; i <= m; ++i) v[i] += d[i] * (x - ub);
+        } else {
+            raw_derivs(v, d, m, x);
+        }
+    }
+    // LinearizedBasis::EvaluateSecondDerivatives (90-109).
+    void eval_second(double* v, double* d, double* d2, unsigned m, double x) const {
+        if (lin && (x < lb || x > ub)) {
+            eval_derivs(v, d, m, x);
+            for (unsigned i = 0; i <= m; ++i) d2[i] = 0.0;
+        } else {
+            raw_second(v, d, d2, m, x);
+        }
     }
 };
 
@@ -599,6 +683,7 @@ struct Quad {
         cc_rule(nf, fWts.data(), fPts.data());
     }
     size_t workspace(unsigned fdim) const {
+        if (type == QUAD_CC) return fdim;
         if (type == QUAD_AS) return (2 * maxSub + 5) * fdim + 2 * maxSub;
         return (maxSub + 5) * fdim + 2 * maxSub;
     }
@@ -774,9 +859,34 @@ struct Quad {
         }
     }
 
+    // ClenshawCurtisQuadrature::Integrate (Quadrature.h:279-307) with the rule
+    // of GetRule(quadPts) (212-258).
+    std::vector<double> ccPts, ccWts;
+    void init_cc(unsigned quadPts) {
+        ccPts.resize(quadPts);
+        ccWts.resize(quadPts);
+        cc_rule(quadPts, ccWts.data(), ccPts.data());
+    }
+    template <class F>
+    void integrate_cc(double* ws, const F& f, unsigned fdim, double lb, double ub, double* res) const {
+        double midpoint_tol = 15.0 * std::numeric_limits<double>::epsilon();
+        if ((ub - lb) < midpoint_tol) {
+            f(0.5 * (ub + lb), res);
+            for (unsigned i = 0; i < fdim; ++i) res[i] *= (ub - lb);
+            return;
+        }
+        for (unsigned j = 0; j < fdim; ++j) res[j] = 0.0;
+        double* fval = ws;
+        for (unsigned i = 0; i < ccPts.size(); ++i) {
+            f(0.5 * (ub + lb + (ub - lb) * ccPts[i]), fval);
+            for (unsigned j = 0; j < fdim; ++j) res[j] += 0.5 * (ub - lb) * ccWts[i] * fval[j];
+        }
+    }
+
     template <class F>
     void integrate(double* ws, const F& f, unsigned fdim, double lb, double ub, double* res) const {
         if (type == QUAD_AS) integrate_as(ws, f, fdim, lb, ub, res);
+        else if (type == QUAD_CC) integrate_cc(ws, f, fdim, lb, ub, res);
         else integrate_acc(ws, f, fdim, lb, ub, res);
     }
 };
@@ -1181,6 +1291,10 @@ static Component* make_component(const Options* o, const MSet& ms) {
     c->ex.basis.type = o->basisType;
     c->ex.basis.poly.phys = (o->basisType == PHYS_HERMITE);
     c->ex.basis.poly.normalize = (o->basisType != HERMITE_FN) && o->basisNorm;
+    // isLinearized = !isinf(basisLB) || !isinf(basisUB) (MParT/MapFactory.h:226)
+    c->ex.basis.lin = !std::isinf(o->basisLB) || !std::isinf(o->basisUB);
+    c->ex.basis.lb = o->basisLB;
+    c->ex.basis.ub = o->basisUB;
     c->ex.init();
     c->g.type = o->posFuncType;
     c->quad.type = o->quadType;
@@ -1189,6 +1303,7 @@ static Component* make_component(const Options* o, const MSet& ms) {
     c->quad.absTol = o->quadAbsTol;
     c->quad.relTol = o->quadRelTol;
     if (o->quadType == QUAD_ACC) c->quad.init_acc(o->quadPts);
+    if (o->quadType == QUAD_CC) c->quad.init_cc(o->quadPts);
     c->contDeriv = o->contDeriv != 0;
     c->nugget = o->nugget;
     c->inDim = ms.dim;
@@ -1263,6 +1378,22 @@ int orc_basis_eval(int basisType, int normalize, int mode, unsigned maxOrder, do
     return 0;
 }
 
+// The same through LinearizedBasis with bounds [lb, ub].
+int orc_basis_eval_lin(int basisType, int normalize, int mode, unsigned maxOrder, double x, double lb, double ub,
+                       double* v, double* d, double* d2) {
+    Basis1D b;
+    b.type = basisType;
+    b.poly.phys = (basisType == PHYS_HERMITE);
+    b.poly.normalize = (basisType != HERMITE_FN) && normalize;
+    b.lin = true;
+    b.lb = lb;
+    b.ub = ub;
+    if (mode == 0) b.eval_all(v, maxOrder, x);
+    else if (mode == 1) b.eval_derivs(v, d, maxOrder, x);
+    else b.eval_second(v, d, d2, maxOrder, x);
+    return 0;
+}
+
 int orc_posfunc_eval(int type, double x, double* g, double* dg, double* d2g) {
     PosFunc p;
     p.type = type;
@@ -1292,6 +1423,7 @@ int orc_quad_test(int quadType, unsigned level, unsigned maxSub, double absTol, 
         cc_rule(nc, q.cWts.data(), q.cPts.data());
         cc_rule(nf, q.fWts.data(), q.fPts.data());
     }
+    if (quadType == QUAD_CC) q.init_cc(level);  // level = number of points for the fixed rule
     long ne = 0;
     auto f = [&](double x, double* out) {
         ne++;

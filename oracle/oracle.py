@@ -8,6 +8,7 @@ Arrays are numpy float64.  Point matrices are D x N (a column per point), as in
 the reference's ``StridedMatrix`` convention; any strides are accepted.
 """
 import ctypes
+import math
 import os
 import subprocess
 
@@ -34,12 +35,14 @@ class COptions(ctypes.Structure):
         ("quadAbsTol", ctypes.c_double),
         ("quadRelTol", ctypes.c_double),
         ("nugget", ctypes.c_double),
+        ("basisLB", ctypes.c_double),
+        ("basisUB", ctypes.c_double),
     ]
 
 
 def options(basisType="ProbabilistHermite", posFuncType="SoftPlus", quadType="AdaptiveSimpson",
             quadAbsTol=1e-6, quadRelTol=1e-6, quadMaxSub=30, quadMinSub=0, quadPts=5,
-            contDeriv=True, basisNorm=True, nugget=0.0):
+            contDeriv=True, basisNorm=True, nugget=0.0, basisLB=-math.inf, basisUB=math.inf):
     """MapOptions with the reference defaults (MParT/MapOptions.h:61-123)."""
     o = COptions()
     o.basisType = BASIS[basisType]
@@ -53,6 +56,8 @@ def options(basisType="ProbabilistHermite", posFuncType="SoftPlus", quadType="Ad
     o.quadAbsTol = quadAbsTol
     o.quadRelTol = quadRelTol
     o.nugget = nugget
+    o.basisLB = basisLB
+    o.basisUB = basisUB
     return o
 
 
@@ -83,6 +88,7 @@ def _declare(L):
         "orc_total_order_size": [u, u, c.POINTER(u), c.POINTER(u)],
         "orc_total_order_fill": [u, u, U, U, U, U],
         "orc_basis_eval": [i, i, i, u, d, D, D, D],
+        "orc_basis_eval_lin": [i, i, i, u, d, d, d, D, D, D],
         "orc_posfunc_eval": [i, d, Dp, Dp, Dp],
         "orc_cc_rule": [u, D, D],
         "orc_quad_test": [i, u, u, d, d, i, d, d, Dp, Lp],
@@ -157,6 +163,16 @@ def basis_eval(basis, x, maxOrder, mode=2, normalize=True):
     d = np.zeros(maxOrder + 1)
     d2 = np.zeros(maxOrder + 1)
     lib().orc_basis_eval(BASIS[basis], int(normalize), mode, maxOrder, ctypes.c_double(x), _dp(v), _dp(d), _dp(d2))
+    return v, d, d2
+
+
+def basis_eval_lin(basis, x, maxOrder, lb, ub, mode=2, normalize=True):
+    """LinearizedBasis(basis, lb, ub) values / derivatives / second derivatives at x."""
+    v = np.zeros(maxOrder + 1)
+    d = np.zeros(maxOrder + 1)
+    d2 = np.zeros(maxOrder + 1)
+    lib().orc_basis_eval_lin(BASIS[basis], int(normalize), mode, maxOrder, ctypes.c_double(x), ctypes.c_double(lb),
+                             ctypes.c_double(ub), _dp(v), _dp(d), _dp(d2))
     return v, d, d2
 
 
