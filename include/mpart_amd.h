@@ -46,7 +46,10 @@ extern "C" {
 #define MPART_QUAD_ADAPTIVE_CLENSHAW_CURTIS 2
 
 /* Replaces mpart::MapOptions (MParT/MapOptions.h:58-123); defaults from
- * mpart_default_options() are the reference defaults. */
+ * mpart_default_options() are the reference defaults.  A finite basisLB or
+ * basisUB selects the LinearizedBasis of the factory (MParT/MapFactory.h:226,
+ * MParT/LinearizedBasis.h); quadType ClenshawCurtis is the fixed rule of
+ * quadPts points (at most 17 here). */
 typedef struct mpart_options {
     int32_t basisType;
     int32_t posFuncType;
@@ -59,6 +62,8 @@ typedef struct mpart_options {
     double quadAbsTol;
     double quadRelTol;
     double nugget;
+    double basisLB;
+    double basisUB;
 } mpart_options;
 
 typedef struct mpart_map_s* mpart_map;

@@ -49,6 +49,8 @@ class _COptions(ctypes.Structure):
         ("quadAbsTol", ctypes.c_double),
         ("quadRelTol", ctypes.c_double),
         ("nugget", ctypes.c_double),
+        ("basisLB", ctypes.c_double),
+        ("basisUB", ctypes.c_double),
     ]
 
 
@@ -170,13 +172,15 @@ class QuadTypes(enum.IntEnum):
 class MapOptions:
     """Options of MParT/MapOptions.h:58-123 with the same defaults."""
 
-    _FIELDS = ("basisType", "posFuncType", "quadType", "quadAbsTol", "quadRelTol", "quadMaxSub", "quadMinSub",
-               "quadPts", "contDeriv", "basisNorm", "nugget")
+    _FIELDS = ("basisType", "basisLB", "basisUB", "posFuncType", "quadType", "quadAbsTol", "quadRelTol",
+               "quadMaxSub", "quadMinSub", "quadPts", "contDeriv", "basisNorm", "nugget")
 
     def __init__(self, **kw):
         c = _COptions()
         _lib.mpart_default_options(ctypes.byref(c))
         self.basisType = BasisTypes(c.basisType)
+        self.basisLB = c.basisLB
+        self.basisUB = c.basisUB
         self.posFuncType = PosFuncTypes(c.posFuncType)
         self.quadType = QuadTypes(c.quadType)
         self.quadAbsTol = c.quadAbsTol
@@ -205,6 +209,8 @@ class MapOptions:
         c.quadAbsTol = float(self.quadAbsTol)
         c.quadRelTol = float(self.quadRelTol)
         c.nugget = float(self.nugget)
+        c.basisLB = float(self.basisLB)
+        c.basisUB = float(self.basisUB)
         return c
 
     def __repr__(self):

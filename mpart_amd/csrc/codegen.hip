@@ -74,7 +74,7 @@ void emit_coeff_sums(std::ostringstream& o, const GenComp& c) {
 
 std::string generate_source(const GenSpec& s) {
     std::ostringstream o;
-    o << "#include \"mono_device.hpp\"\nusing namespace mpd;\n";
+    o << "#define MPD_LINEARIZED " << (s.lin ? 1 : 0) << "\n#include \"mono_device.hpp\"\nusing namespace mpd;\n";
     o << "#define GBK " << s.bk << "\n#define GPK " << s.pk << "\n#define GQK " << s.qk << "\n#define GPM " << s.pm
       << "\n";
     const unsigned ncomp = unsigned(s.comps.size());
@@ -261,7 +261,7 @@ extern "C" __global__ void __launch_bounds__(256) mpart_gen_ldcoef(MapDev M, con
     const PosF<GPK> gv = posfunc<GPK>(z);
     const double inv = 1.0 / gv.g;
     double bq[GPM + 1];
-    diag_basis<GBK, GPM>(cd.p, xk, bq);
+    diag_basis<GBK, GPM>(dc, xk, bq);
 #pragma unroll
     for (int q = 0; q <= GPM; ++q) G[q] = (q <= cd.p) ? M.rho[q] * bq[q] : 0.0;
 )";

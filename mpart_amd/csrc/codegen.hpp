@@ -25,6 +25,7 @@ struct GenComp {
 
 struct GenSpec {
     int bk, pk, qk, pm;
+    int lin = 0;  // LinearizedBasis bounds in use
     unsigned D;
     unsigned numCoeffs;
     std::vector<int> dimDeg;  // largest off-diagonal order per coordinate

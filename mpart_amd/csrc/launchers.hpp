@@ -71,21 +71,22 @@ hipError_t launch_copy(const double* src, long rows, long cols, long srs, long s
 }  // namespace mpd
 
 // Expands CALL(BK, PK, QK, PM) for the variant id
-//   var = pmClass * 1000 + bk * 100 + pk * 10 + qk,  PM = 4 (pmClass 0) or 12 (pmClass 1);
-// returns hipErrorInvalidValue for an unknown id.
-#define MPD_CASES_PM(PMC, PMV, CALL)                   \
-    case PMC * 1000 + 0: CALL(0, 0, 0, PMV); break;     \
-    case PMC * 1000 + 1: CALL(0, 0, 1, PMV); break;     \
-    case PMC * 1000 + 2: CALL(0, 0, 2, PMV); break;     \
-    case PMC * 1000 + 10: CALL(0, 1, 0, PMV); break;    \
-    case PMC * 1000 + 11: CALL(0, 1, 1, PMV); break;    \
-    case PMC * 1000 + 12: CALL(0, 1, 2, PMV); break;    \
-    case PMC * 1000 + 100: CALL(1, 0, 0, PMV); break;   \
-    case PMC * 1000 + 101: CALL(1, 0, 1, PMV); break;   \
-    case PMC * 1000 + 102: CALL(1, 0, 2, PMV); break;   \
-    case PMC * 1000 + 110: CALL(1, 1, 0, PMV); break;   \
-    case PMC * 1000 + 111: CALL(1, 1, 1, PMV); break;   \
-    case PMC * 1000 + 112: CALL(1, 1, 2, PMV); break;
+//   var = pmClass * 1000 + bk * 100 + pk * 10 + qk,  PM = 4 (pmClass 0) or 12 (pmClass 1),
+// bk: ProbabilistHermite 0, HermiteFunctions 1, PhysicistHermite 2; pk: Exp 0,
+// SoftPlus 1; qk: adaptive Simpson 0, adaptive Clenshaw-Curtis 3/5 1, adaptive
+// Clenshaw-Curtis (run-time rule) 2, fixed Clenshaw-Curtis 3.
+#define MPD_CASES_Q(PMC, PMV, B, P, CALL)                     \
+    case PMC * 1000 + B * 100 + P * 10 + 0: CALL(B, P, 0, PMV); break; \
+    case PMC * 1000 + B * 100 + P * 10 + 1: CALL(B, P, 1, PMV); break; \
+    case PMC * 1000 + B * 100 + P * 10 + 2: CALL(B, P, 2, PMV); break; \
+    case PMC * 1000 + B * 100 + P * 10 + 3: CALL(B, P, 3, PMV); break;
+#define MPD_CASES_PM(PMC, PMV, CALL)      \
+    MPD_CASES_Q(PMC, PMV, 0, 0, CALL)    \
+    MPD_CASES_Q(PMC, PMV, 0, 1, CALL)    \
+    MPD_CASES_Q(PMC, PMV, 1, 0, CALL)    \
+    MPD_CASES_Q(PMC, PMV, 1, 1, CALL)    \
+    MPD_CASES_Q(PMC, PMV, 2, 0, CALL)    \
+    MPD_CASES_Q(PMC, PMV, 2, 1, CALL)
 
 #define MPD_DISPATCH(var, CALL)              \
     switch (var) {                           \

@@ -40,14 +40,33 @@ constexpr int MAXNF = 17;       // largest nested Clenshaw-Curtis fine rule
 constexpr int MAXORD = 16;      // largest off-diagonal degree in the basis table
 constexpr uint32_t DUMMY = 0xFFFFFFFFu;
 
-enum BasisKind { BK_PH = 0, BK_HF = 1 };
+enum BasisKind { BK_PH = 0, BK_HF = 1, BK_PHYS = 2 };
 enum PosKind { PK_EXP = 0, PK_SP = 1 };
-enum QuadKind { QK_AS = 0, QK_ACC5 = 1, QK_ACC = 2 };
+enum QuadKind { QK_AS = 0, QK_ACC5 = 1, QK_ACC = 2, QK_CC = 3 };
+
+// Orthogonal-polynomial bases p_k = (a x + 0) p_{k-1} - c_k p_{k-2} with
+// p_0 = 1, p_1 = a x (OrthogonalPolynomial.h:25-156): ProbabilistHermite
+// a = 1, c_k = k - 1 (301-306); PhysicistHermite a = 2, c_k = 2 (k - 1)
+// (319-324).  p_k' = a k p_{k-1} and p_k'' = a^2 k (k-1) p_{k-2} for both.
+template <int BK>
+constexpr double poly_a() {
+    return BK == BK_PHYS ? 2.0 : 1.0;
+}
+
+// LinearizedBasis (MParT/LinearizedBasis.h): outside [lb, ub] every 1-D basis
+// function continues linearly with its value and slope at the nearer bound.
+// The bounds are run-time data (MapDev::lin/lb/ub, DiagCoef::lb/ub); kernels
+// generated for a map without bounds define MPD_LINEARIZED 0 and drop the
+// checks.
+#ifndef MPD_LINEARIZED
+#define MPD_LINEARIZED 1
+#endif
 // Integrand flavours: value only; coefficient adjoint; input adjoint (and the
 // x_k-derivative U); discrete-derivative mode (U and its coefficient adjoint).
 enum Flavor { FL_VAL = 0, FL_COEF = 1, FL_INPUT = 2, FL_DCOEF = 3 };
 
-// Nested Clenshaw-Curtis pair and adaptive parameters.
+// Nested Clenshaw-Curtis pair and adaptive parameters; the fixed
+// Clenshaw-Curtis rule (QK_CC) uses fw / fp with nf points.
 struct QuadDev {
     double absTol, relTol;
     int maxSub, minSub;
@@ -95,6 +114,8 @@ struct MapDev {
     double hfS[MAXORD + 1];    // (2^i i!)^(-1/2)
     double hfPi;               // pi^(-1/4)
     double hfSqrt2;            // sqrt(2)
+    int lin;                   // LinearizedBasis bounds in use
+    double lb, ub;             // its bounds (-inf / +inf when unused)
     QuadDev quad;
 };
 
@@ -235,23 +256,45 @@ struct DiagCoef {
     int p;
     double x;   // x_k
     double nug; // nugget added to g inside the integrand
+    double lb, ub;  // LinearizedBasis bounds (+-inf when unused)
 };
+
+// The argument of the diagonal basis derivatives: a linearized basis has the
+// slope of its nearer bound outside [lb, ub] (NaN stays NaN).
+template <int PM>
+MPD_HD double lin_arg(const DiagCoef<PM>& dc, double y) {
+#if MPD_LINEARIZED
+    return (y < dc.lb) ? dc.lb : ((y > dc.ub) ? dc.ub : y);
+#else
+    return y;
+#endif
+}
+template <int PM>
+MPD_HD bool lin_outside(const DiagCoef<PM>& dc, double y) {
+#if MPD_LINEARIZED
+    return (y < dc.lb) || (y > dc.ub);
+#else
+    return false;
+#endif
+}
 
 // d_k f at y (and optionally the second derivative).
 template <int BK, bool SECOND, int PM>
-MPD_HD double diag_eval(const DiagCoef<PM>& dc, double y, double* z2out) {
+MPD_HD double diag_eval(const DiagCoef<PM>& dc, double y0, double* z2out) {
     const int p = dc.p;
-    if constexpr (BK == BK_PH) {
-        // He_0 = 1, He_1 = y, He_r = y He_{r-1} - (r-1) He_{r-2}
-        double hm2 = 0.0, hm1 = 1.0;  // He_{r-2}, He_{r-1}
+    const double y = lin_arg(dc, y0);
+    if constexpr (BK != BK_HF) {
+        // p_0 = 1, p_1 = a y, p_r = a y p_{r-1} - a (r-1) p_{r-2};  p'_{r+1} = a (r+1) p_r
+        constexpr double A = poly_a<BK>();
+        double hm2 = 0.0, hm1 = 1.0;  // p_{r-2}, p_{r-1}
         double z = 0.0, z2 = 0.0;
 #pragma unroll
         for (int r = 0; r < PM; ++r) {
             if (r >= p) break;
-            double hr = (r == 0) ? 1.0 : ((r == 1) ? y : y * hm1 - (r - 1.0) * hm2);
-            z += (r + 1.0) * dc.cd[r + 1] * hr;
+            double hr = (r == 0) ? 1.0 : ((r == 1) ? A * y : (A * y) * hm1 - (A * (r - 1.0)) * hm2);
+            z += (A * (r + 1.0)) * dc.cd[r + 1] * hr;
             if constexpr (SECOND) {
-                if (r + 2 <= p) z2 += (r + 2.0) * (r + 1.0) * dc.cd[r + 2] * hr;
+                if (r + 2 <= p) z2 += (A * A * (r + 2.0) * (r + 1.0)) * dc.cd[r + 2] * hr;
             }
             if (r >= 1) {
                 hm2 = hm1;
@@ -260,7 +303,7 @@ MPD_HD double diag_eval(const DiagCoef<PM>& dc, double y, double* z2out) {
                 hm1 = hr;
             }
         }
-        if constexpr (SECOND) *z2out = z2;
+        if constexpr (SECOND) *z2out = lin_outside(dc, y0) ? 0.0 : z2;
         return z;
     } else {
         double z = (p >= 1) ? dc.cd[1] : 0.0;
@@ -281,22 +324,25 @@ MPD_HD double diag_eval(const DiagCoef<PM>& dc, double y, double* z2out) {
             hkm2 = hkm1;
             hkm1 = hk;
         }
-        if constexpr (SECOND) *z2out = ex * s2;
+        if constexpr (SECOND) *z2out = lin_outside(dc, y0) ? 0.0 : ex * s2;
         return z + ex * s;
     }
 }
 
 // B'_q(y) for q = 0..p (raw derivative basis; phi~'_q = rho_q B'_q).
 template <int BK, int PM>
-MPD_HD void diag_basis(int p, double y, double* bq) {
+MPD_HD void diag_basis(const DiagCoef<PM>& dc, double y0, double* bq) {
+    const int p = dc.p;
+    const double y = lin_arg(dc, y0);
     bq[0] = 0.0;
-    if constexpr (BK == BK_PH) {
+    if constexpr (BK != BK_HF) {
+        constexpr double A = poly_a<BK>();
         double hm2 = 0.0, hm1 = 1.0;
 #pragma unroll
         for (int r = 0; r < PM; ++r) {
             if (r >= p) break;
-            double hr = (r == 0) ? 1.0 : ((r == 1) ? y : y * hm1 - (r - 1.0) * hm2);
-            bq[r + 1] = (r + 1.0) * hr;
+            double hr = (r == 0) ? 1.0 : ((r == 1) ? A * y : (A * y) * hm1 - (A * (r - 1.0)) * hm2);
+            bq[r + 1] = (A * (r + 1.0)) * hr;
             if (r >= 1) {
                 hm2 = hm1;
                 hm1 = hr;
@@ -321,19 +367,27 @@ MPD_HD void diag_basis(int p, double y, double* bq) {
     }
 }
 
-// B''_q(y) (raw second-derivative basis, same folding as B'_q).
+// B''_q(y) (raw second-derivative basis, same folding as B'_q); zero outside
+// the bounds of a linearized basis.
 template <int BK, int PM>
-MPD_HD void diag_basis2(int p, double y, double* bq) {
+MPD_HD void diag_basis2(const DiagCoef<PM>& dc, double y, double* bq) {
+    const int p = dc.p;
     bq[0] = 0.0;
     if (p >= 1) bq[1] = 0.0;
-    if constexpr (BK == BK_PH) {
-        // He''_q = q (q-1) He_{q-2}
+    if (lin_outside(dc, y)) {
+#pragma unroll
+        for (int q = 2; q <= PM; ++q) bq[q] = 0.0;
+        return;
+    }
+    if constexpr (BK != BK_HF) {
+        // p''_q = a^2 q (q-1) p_{q-2}
+        constexpr double A = poly_a<BK>();
         double hm2 = 0.0, hm1 = 1.0;
 #pragma unroll
         for (int r = 0; r < PM - 1; ++r) {
             if (r + 2 > p) break;
-            double hr = (r == 0) ? 1.0 : ((r == 1) ? y : y * hm1 - (r - 1.0) * hm2);
-            bq[r + 2] = (r + 2.0) * (r + 1.0) * hr;
+            double hr = (r == 0) ? 1.0 : ((r == 1) ? A * y : (A * y) * hm1 - (A * (r - 1.0)) * hm2);
+            bq[r + 2] = (A * A * (r + 2.0) * (r + 1.0)) * hr;
             if (r >= 1) {
                 hm2 = hm1;
                 hm1 = hr;
@@ -356,6 +410,74 @@ MPD_HD void diag_basis2(int p, double y, double* bq) {
     }
 }
 
+// Normalized value and first derivative of the order-o basis function
+// (o >= 1) at x, with the arithmetic of fill_values_derivs (used at the bound
+// of a linearized basis).
+template <int BK>
+MPD_HD void basis_vd(const MapDev& M, int o, double x, double& v, double& d) {
+    if constexpr (BK != BK_HF) {
+        constexpr double A = poly_a<BK>();
+        double vm2 = 1.0, vm1 = A * x, dm2 = 0.0, dm1 = A;
+        for (int k = 2; k <= o; ++k) {
+            const double vv = (A * x + 0.0) * vm1 - (A * (k - 1.0)) * vm2;
+            const double dd = A * vm1 + (A * x + 0.0) * dm1 - (A * (k - 1.0)) * dm2;
+            vm2 = vm1;
+            vm1 = vv;
+            dm2 = dm1;
+            dm1 = dd;
+        }
+        if (M.normalize) {
+            v = vm1 / M.norm[o];
+            d = dm1 / M.norm[o];
+        } else {
+            v = vm1;
+            d = dm1;
+        }
+    } else {
+        if (o == 1) {
+            v = x;
+            d = 1.0;
+            return;
+        }
+        const int k = o - 2;
+        const double base = M.hfPi * exp(-0.5 * x * x);
+        double hm1 = 0.0, hm2 = 0.0, dm1 = 0.0, dm2 = 0.0, h = 1.0, dh = 0.0;
+        for (int j = 0; j <= k; ++j) {
+            if (j == 0) {
+                h = 1.0;
+                dh = 0.0;
+            } else if (j == 1) {
+                h = 2.0 * x;
+                dh = 2.0;
+            } else {
+                h = (2.0 * x + 0.0) * hm1 - 2.0 * (j - 1.0) * hm2;
+                dh = 2.0 * hm1 + (2.0 * x + 0.0) * dm1 - 2.0 * (j - 1.0) * dm2;
+            }
+            hm2 = hm1;
+            hm1 = h;
+            dm2 = dm1;
+            dm1 = dh;
+        }
+        const double scale = (k == 0) ? base : base * M.hfS[k];
+        v = h * scale;
+        d = (dh - x * h) * scale;
+    }
+}
+
+// Whether x lies outside the bounds of a linearized basis (and which bound).
+MPD_HD bool lin_bound(const MapDev& M, double x, double& c) {
+#if MPD_LINEARIZED
+    if (M.lin && (x < M.lb || x > M.ub)) {
+        c = (x < M.lb) ? M.lb : M.ub;
+        return true;
+    }
+#endif
+    (void)M;
+    (void)x;
+    (void)c;
+    return false;
+}
+
 // ---------------------------------------------------------------------------
 // Off-diagonal basis table.  Slot 0 holds 1.0; coordinate d occupies slots
 // dimSlot[d] .. dimSlot[d]+dimDeg[d]-1 for orders 1..dimDeg[d].  The table of a
@@ -370,10 +492,21 @@ __host__ __device__ void fill_values(const MapDev& M, int ndims, const double* p
         if (deg == 0) continue;
         const double x = pt[d * prs];
         double* out = tab + (long)M.dimSlot[d] * stride - stride;  // out[o*stride], o >= 1
-        if constexpr (BK == BK_PH) {
-            double vm2 = 1.0, vm1 = x;
+        double c;
+        if (lin_bound(M, x, c)) {
+            // LinearizedBasis::EvaluateAll (LinearizedBasis.h:41-61)
             for (int o = 1; o <= deg; ++o) {
-                double v = (o == 1) ? x : (1.0 * x + 0.0) * vm1 - (o - 1.0) * vm2;
+                double v, dv;
+                basis_vd<BK>(M, o, c, v, dv);
+                out[o * stride] = v + dv * (x - c);
+            }
+            continue;
+        }
+        if constexpr (BK != BK_HF) {
+            constexpr double A = poly_a<BK>();
+            double vm2 = 1.0, vm1 = A * x;
+            for (int o = 1; o <= deg; ++o) {
+                double v = (o == 1) ? A * x : (A * x + 0.0) * vm1 - (A * (o - 1.0)) * vm2;
                 if (o >= 2) {
                     vm2 = vm1;
                     vm1 = v;
@@ -408,11 +541,22 @@ __host__ __device__ void fill_values(const MapDev& M, int ndims, const double* p
 template <int BK, int DEG>
 MPD_HD void basis_vals(const MapDev& M, double x, double* b) {
     b[0] = 1.0;
-    if constexpr (BK == BK_PH) {
-        double vm2 = 1.0, vm1 = x;
+    double c;
+    if (lin_bound(M, x, c)) {
 #pragma unroll
         for (int o = 1; o <= DEG; ++o) {
-            double v = (o == 1) ? x : (1.0 * x + 0.0) * vm1 - (o - 1.0) * vm2;
+            double v, dv;
+            basis_vd<BK>(M, o, c, v, dv);
+            b[o] = v + dv * (x - c);
+        }
+        return;
+    }
+    if constexpr (BK != BK_HF) {
+        constexpr double A = poly_a<BK>();
+        double vm2 = 1.0, vm1 = A * x;
+#pragma unroll
+        for (int o = 1; o <= DEG; ++o) {
+            double v = (o == 1) ? A * x : (A * x + 0.0) * vm1 - (A * (o - 1.0)) * vm2;
             if (o >= 2) {
                 vm2 = vm1;
                 vm1 = v;
@@ -449,16 +593,28 @@ __host__ __device__ void fill_values_derivs(const MapDev& M, int ndims, const do
         const double x = pt[d * prs];
         double* out = tab + (long)M.dimSlot[d] * stride - stride;
         double* dout = dtab + (long)M.dimSlot[d] * stride - stride;
-        if constexpr (BK == BK_PH) {
-            double vm2 = 1.0, vm1 = x, dm2 = 0.0, dm1 = 1.0;
+        double c;
+        if (lin_bound(M, x, c)) {
+            // LinearizedBasis::EvaluateDerivatives (LinearizedBasis.h:63-87)
+            for (int o = 1; o <= deg; ++o) {
+                double v, dv;
+                basis_vd<BK>(M, o, c, v, dv);
+                out[o * stride] = v + dv * (x - c);
+                dout[o * stride] = dv;
+            }
+            continue;
+        }
+        if constexpr (BK != BK_HF) {
+            constexpr double A = poly_a<BK>();
+            double vm2 = 1.0, vm1 = A * x, dm2 = 0.0, dm1 = A;
             for (int o = 1; o <= deg; ++o) {
                 double v, dv;
                 if (o == 1) {
-                    v = x;
-                    dv = 1.0;
+                    v = A * x;
+                    dv = A;
                 } else {
-                    v = (1.0 * x + 0.0) * vm1 - (o - 1.0) * vm2;
-                    dv = 1.0 * vm1 + (1.0 * x + 0.0) * dm1 - (o - 1.0) * dm2;
+                    v = (A * x + 0.0) * vm1 - (A * (o - 1.0)) * vm2;
+                    dv = A * vm1 + (A * x + 0.0) * dm1 - (A * (o - 1.0)) * dm2;
                     vm2 = vm1;
                     vm1 = v;
                     dm2 = dm1;
@@ -586,6 +742,8 @@ MPD_HD void make_diag(const MapDev& M, const CompDev& c, const double* C, double
     dc.p = c.p;
     dc.x = xk;
     dc.nug = nug;
+    dc.lb = M.lb;
+    dc.ub = M.ub;
 #pragma unroll
     for (int q = 0; q <= PM; ++q) dc.cd[q] = (q <= c.p) ? C[q] * M.rho[q] : 0.0;
 }
@@ -664,8 +822,8 @@ MPD_HD void adj_node(const DiagCoef<PM>& dc, double t, double w, const NodeV<FL>
         double z2;
         diag_eval<BK, true, PM>(dc, y, &z2);
         double bq[PM + 1], bq2[PM + 1];
-        diag_basis<BK, PM>(dc.p, y, bq);
-        diag_basis2<BK, PM>(dc.p, y, bq2);
+        diag_basis<BK, PM>(dc, y, bq);
+        diag_basis2<BK, PM>(dc, y, bq2);
         const double xt = dc.x * t;
         A.U += w * (v.g1 + xt * v.gp * z2);
         const double a = v.gp + xt * z2 * v.gpp;
@@ -678,7 +836,7 @@ MPD_HD void adj_node(const DiagCoef<PM>& dc, double t, double w, const NodeV<FL>
     } else if constexpr (FL >= FL_COEF) {
         double bq[PM + 1];
         const double y = t * dc.x;
-        diag_basis<BK, PM>(dc.p, y, bq);
+        diag_basis<BK, PM>(dc, y, bq);
         const double wh = w * v.h;
 #pragma unroll
         for (int q = 1; q <= PM; ++q) {
@@ -863,6 +1021,26 @@ __host__ __device__ double integrate_acc(const QuadDev& Q, const DiagCoef<PM>& d
     return res;
 }
 
+// ClenshawCurtisQuadrature::Integrate (Quadrature.h:279-307) on [0, 1] with
+// the fixed rule of Q.nf points (nodes 0.5 (1 + p_i), weights 0.5 w_i, summed
+// in rule order); the last node is t = 1.
+template <int BK, int PK, int FL, int PM>
+MPD_HD double integrate_cc(const QuadDev& Q, const DiagCoef<PM>& dc, AdjAcc<PM>& A, int* nevals,
+                           const NodeV<FL>* at1) {
+    const int n = Q.nf;
+    double res = 0.0;
+    for (int i = 0; i < MAXNF; ++i) {
+        if (i >= n) break;
+        const double t = 0.5 * (1.0 + 0.0 + (1.0 - 0.0) * Q.fp[i]);
+        const NodeV<FL> v = (at1 && Q.fp[i] == 1.0) ? *at1 : eval_node<BK, PK, FL, PM>(dc, t);
+        const double w = 0.5 * (1.0 - 0.0) * Q.fw[i];
+        res += w * v.F;
+        if constexpr (FL >= FL_COEF) adj_node<BK, FL, PM>(dc, t, w, v, A);
+    }
+    if (nevals) *nevals += n;
+    return res;
+}
+
 // at1, when given, is the integrand at t = 1 (computed by the caller, which
 // also needs g and g' at x_k); it is the value eval_node would produce there.
 template <int BK, int PK, int QK, int FL, int PM>
@@ -870,6 +1048,7 @@ MPD_HD double integrate(const QuadDev& Q, const DiagCoef<PM>& dc, AdjAcc<PM>& A,
                         const NodeV<FL>* at1 = nullptr) {
     if constexpr (QK == QK_AS) return integrate_as<BK, PK, FL, PM>(Q, dc, A, nevals, at1);
     else if constexpr (QK == QK_ACC5) return integrate_acc<BK, PK, FL, 5, PM>(Q, dc, A, nevals, at1);
+    else if constexpr (QK == QK_CC) return integrate_cc<BK, PK, FL, PM>(Q, dc, A, nevals, at1);
     else return integrate_acc<BK, PK, FL, MAXNF, PM>(Q, dc, A, nevals, at1);
 }
 
@@ -924,7 +1103,7 @@ MPD_HD void comp_core(const MapDev& M, const CompDev& cd, const double* C, doubl
         loss += 0.5 * T * T - ldv;
     }
     double bq[PM + 1];
-    diag_basis<BK, PM>(cd.p, xk, bq);
+    diag_basis<BK, PM>(dc, xk, bq);
     G[0] = s;
 #pragma unroll
     for (int q = 1; q <= PM; ++q) {

@@ -222,10 +222,15 @@ namespace {
 
 void build_plan(mpart_map_s& m) {
     const mpart_options& o = m.opts;
-    if (o.basisType == MPART_BASIS_PHYSICIST_HERMITE)
-        throw NotImpl("basisType PhysicistHermite is not provided by this library");
-    if (o.quadType == MPART_QUAD_CLENSHAW_CURTIS)
-        throw NotImpl("quadType ClenshawCurtis (non-adaptive) is not provided by this library");
+    if (o.basisType != MPART_BASIS_PROBABILIST_HERMITE && o.basisType != MPART_BASIS_PHYSICIST_HERMITE &&
+        o.basisType != MPART_BASIS_HERMITE_FUNCTIONS)
+        throw ArgError("unknown basisType");
+    if (o.quadType != MPART_QUAD_CLENSHAW_CURTIS && o.quadType != MPART_QUAD_ADAPTIVE_SIMPSON &&
+        o.quadType != MPART_QUAD_ADAPTIVE_CLENSHAW_CURTIS)
+        throw ArgError("unknown quadType");
+    // isLinearized = !isinf(basisLB) || !isinf(basisUB) (MParT/MapFactory.h:226)
+    const bool lin = !std::isinf(o.basisLB) || !std::isinf(o.basisUB);
+    if (lin && !(o.basisLB < o.basisUB)) throw ArgError("LinearizedBasis needs basisLB < basisUB");
     if (o.posFuncType != MPART_POSFUNC_EXP && o.posFuncType != MPART_POSFUNC_SOFTPLUS)
         throw ArgError("unknown posFuncType");
     if (o.quadMaxSub < 1 || o.quadMaxSub > (unsigned)mpd::MAXSUB)
@@ -234,7 +239,9 @@ void build_plan(mpart_map_s& m) {
         throw ArgError("quadMinSub must be smaller than quadMaxSub (the subdivision depth is bounded by quadMaxSub)");
     if (o.nugget < 0) throw ArgError("nugget must be non-negative");
 
-    const int bk = (o.basisType == MPART_BASIS_HERMITE_FUNCTIONS) ? mpd::BK_HF : mpd::BK_PH;
+    const int bk = (o.basisType == MPART_BASIS_HERMITE_FUNCTIONS)   ? mpd::BK_HF
+                   : (o.basisType == MPART_BASIS_PHYSICIST_HERMITE) ? mpd::BK_PHYS
+                                                                     : mpd::BK_PH;
     const int pk = (o.posFuncType == MPART_POSFUNC_EXP) ? mpd::PK_EXP : mpd::PK_SP;
     MapDev& M = m.dev;
     std::memset(&M, 0, sizeof(M));
@@ -245,6 +252,38 @@ void build_plan(mpart_map_s& m) {
     M.quad.maxSub = (int)o.quadMaxSub;
     M.quad.minSub = (int)o.quadMinSub;
     int qk = mpd::QK_AS;
+    // Clenshaw-Curtis rule (MParT/Quadrature.h:212-258).
+    auto rule = [](unsigned n, double* w, double* p) {
+        if (n == 1) {
+            p[0] = 0.0;
+            w[0] = 2.0;
+            return;
+        }
+        for (unsigned i = 0; i < n; ++i) p[i] = std::cos((n - (i + 1)) * M_PI / (n - 1));
+        p[0] = -1.0;
+        if (n % 2 == 1) p[(n - 1) / 2] = 0.0;
+        p[n - 1] = 1.0;
+        for (unsigned i = 0; i < n; ++i) {
+            w[i] = 1.0;
+            double theta = i * M_PI / (n - 1);
+            for (unsigned j = 0; j < (n - 1) / 2; ++j) {
+                double b = (2 * (j + 1) == (n - 1)) ? 1.0 : 2.0;
+                w[i] = w[i] - b * std::cos(2.0 * (j + 1) * theta) / (4 * (j + 1) * (j + 1) - 1);
+            }
+        }
+        w[0] = w[0] / (n - 1.0);
+        for (unsigned i = 1; i < (n - 1); ++i) w[i] = 2.0 * w[i] / (n - 1.0);
+        w[n - 1] = w[n - 1] / (n - 1.0);
+    };
+    if (o.quadType == MPART_QUAD_CLENSHAW_CURTIS) {
+        // ClenshawCurtisQuadrature(quadPts) (src/MapFactoryImpl5.cpp:17)
+        if (o.quadPts < 1) throw ArgError("ClenshawCurtis needs quadPts >= 1");
+        if (o.quadPts > (unsigned)mpd::MAXNF)
+            throw NotImpl("ClenshawCurtis with quadPts > " + std::to_string(mpd::MAXNF) + " is not provided");
+        rule(o.quadPts, M.quad.fw, M.quad.fp);
+        M.quad.nf = (int)o.quadPts;
+        qk = mpd::QK_CC;
+    }
     if (o.quadType == MPART_QUAD_ADAPTIVE_CLENSHAW_CURTIS) {
         if (o.quadPts < 3) throw ArgError("AdaptiveClenshawCurtis needs quadPts >= 4");
         // level = floor(log2(quadPts - 2)) (src/MapFactoryImpl4.cpp:18)
@@ -252,24 +291,6 @@ void build_plan(mpart_map_s& m) {
         if (level < 1) throw ArgError("AdaptiveClenshawCurtis needs a nested level >= 1 (quadPts >= 4)");
         const unsigned nc = (unsigned)std::pow(2, level) + 1, nf = (unsigned)std::pow(2, level + 1) + 1;
         if (nf > (unsigned)mpd::MAXNF) throw NotImpl("AdaptiveClenshawCurtis with quadPts > 17 is not provided");
-        // Clenshaw-Curtis rule (MParT/Quadrature.h:212-258).
-        auto rule = [](unsigned n, double* w, double* p) {
-            for (unsigned i = 0; i < n; ++i) p[i] = std::cos((n - (i + 1)) * M_PI / (n - 1));
-            p[0] = -1.0;
-            if (n % 2 == 1) p[(n - 1) / 2] = 0.0;
-            p[n - 1] = 1.0;
-            for (unsigned i = 0; i < n; ++i) {
-                w[i] = 1.0;
-                double theta = i * M_PI / (n - 1);
-                for (unsigned j = 0; j < (n - 1) / 2; ++j) {
-                    double b = (2 * (j + 1) == (n - 1)) ? 1.0 : 2.0;
-                    w[i] = w[i] - b * std::cos(2.0 * (j + 1) * theta) / (4 * (j + 1) * (j + 1) - 1);
-                }
-            }
-            w[0] = w[0] / (n - 1.0);
-            for (unsigned i = 1; i < (n - 1); ++i) w[i] = 2.0 * w[i] / (n - 1.0);
-            w[n - 1] = w[n - 1] / (n - 1.0);
-        };
         rule(nc, M.quad.cw, M.quad.cp);
         rule(nf, M.quad.fw, M.quad.fp);
         M.quad.nc = (int)nc;
@@ -279,8 +300,16 @@ void build_plan(mpart_map_s& m) {
     m.variant = bk * 100 + pk * 10 + qk;
 
     // Basis constants.
-    M.normalize = (bk == mpd::BK_PH && o.basisNorm) ? 1 : 0;
-    for (int k = 0; k <= mpd::MAXORD; ++k) M.norm[k] = std::sqrt(std::sqrt(2.0 * M_PI) * factorial_u32(k));
+    M.normalize = (bk != mpd::BK_HF && o.basisNorm) ? 1 : 0;
+    // Normalization sqrt(sqrt(2 pi) k!) (OrthogonalPolynomial.h:297) or
+    // sqrt(sqrt(pi) 2^k k!) (313-315), with the 32-bit factorial.
+    for (int k = 0; k <= mpd::MAXORD; ++k)
+        M.norm[k] = (bk == mpd::BK_PHYS)
+                        ? std::sqrt(std::sqrt(M_PI) * std::pow(2.0, static_cast<double>(k)) * factorial_u32(k))
+                        : std::sqrt(std::sqrt(2.0 * M_PI) * factorial_u32(k));
+    M.lin = lin ? 1 : 0;
+    M.lb = lin ? o.basisLB : -std::numeric_limits<double>::infinity();
+    M.ub = lin ? o.basisUB : std::numeric_limits<double>::infinity();
     M.hfPi = std::pow(M_PI, -0.25);
     M.hfSqrt2 = std::sqrt(2.0);
     {
@@ -295,10 +324,11 @@ void build_plan(mpart_map_s& m) {
     // phi~_q(0) and the derivative folding rho_q.
     M.phi0[0] = 1.0;
     M.rho[0] = 0.0;
-    if (bk == mpd::BK_PH) {
-        double hm2 = 1.0, hm1 = 0.0;  // He_0(0), He_1(0)
+    if (bk != mpd::BK_HF) {
+        const double A = (bk == mpd::BK_PHYS) ? 2.0 : 1.0;
+        double hm2 = 1.0, hm1 = 0.0;  // p_0(0), p_1(0)
         for (int q = 1; q <= mpd::MAXP; ++q) {
-            double h = (q == 1) ? 0.0 : (1.0 * 0.0 + 0.0) * hm1 - (q - 1.0) * hm2;
+            double h = (q == 1) ? 0.0 : (A * 0.0 + 0.0) * hm1 - (A * (q - 1.0)) * hm2;
             if (q >= 2) {
                 hm2 = hm1;
                 hm1 = h;
@@ -318,6 +348,18 @@ void build_plan(mpart_map_s& m) {
         for (int q = 1; q <= mpd::MAXP; ++q) {
             M.phi0[q] = v[q];
             M.rho[q] = (q == 1) ? 1.0 : M.hfPi * M.hfS[q - 2];
+        }
+    }
+    // A linearized basis whose bounds exclude 0 continues linearly there
+    // (LinearizedBasis::EvaluateAll at 0).
+    if (lin && (0.0 < M.lb || 0.0 > M.ub)) {
+        const double c = (0.0 < M.lb) ? M.lb : M.ub;
+        for (int q = 1; q <= mpd::MAXP; ++q) {
+            double v, d;
+            if (bk == mpd::BK_HF) mpd::basis_vd<mpd::BK_HF>(M, q, c, v, d);
+            else if (bk == mpd::BK_PHYS) mpd::basis_vd<mpd::BK_PHYS>(M, q, c, v, d);
+            else mpd::basis_vd<mpd::BK_PH>(M, q, c, v, d);
+            M.phi0[q] = v + d * (0.0 - c);
         }
     }
     M.nugget = o.nugget;
@@ -514,6 +556,7 @@ mpg::GenModule* gen_for(mpart_map_s& m) {
     spec.bk = (m.variant / 100) % 10;
     spec.pk = (m.variant / 10) % 10;
     spec.qk = m.variant % 10;
+    spec.lin = m.dev.lin;
     spec.pm = std::max(1, m.dev.pmax);
     spec.D = m.inDim;
     spec.numCoeffs = m.numCoeffs;
@@ -1376,6 +1419,8 @@ void mpart_default_options(mpart_options* o) {
     o->quadAbsTol = 1e-6;
     o->quadRelTol = 1e-6;
     o->nugget = 0.0;
+    o->basisLB = -std::numeric_limits<double>::infinity();
+    o->basisUB = std::numeric_limits<double>::infinity();
 }
 
 int mpart_total_order_set(uint32_t dim, uint32_t order, uint32_t* nterms, uint32_t* nnz, uint32_t* nzStarts,

@@ -101,7 +101,7 @@ __global__ void __launch_bounds__(256) k_inverse_queue(MapDev M, const double* _
     const int cRight = nc - 1, cMid = cRight / 2;
     const int fRight = nf - 1, fMid = fRight / 2;
     const int nNodes = cRight - 1 + (nf - 1) / 2;  // per interval, midpoint included
-    if (QK != QK_AS) {
+    if (QK == QK_ACC5 || QK == QK_ACC) {
         if (threadIdx.x == 0) {
             int j = 1;
             for (int i = 1; i < cMid; ++i, ++j) {
@@ -138,6 +138,8 @@ __global__ void __launch_bounds__(256) k_inverse_queue(MapDev M, const double* _
     dc.x = 0.0;
 #pragma unroll
     for (int q = 0; q <= PM; ++q) dc.cd[q] = 0.0;
+    dc.lb = M.lb;
+    dc.ub = M.ub;
     double f0 = 0.0, yd = 0.0, x0 = 0.0;
     int phase = IP_START, bi = 0, it = 0;
     double step = 1.0, xlb = 0.0, xub = 0.0, ylb = 0.0, yub = 0.0, nhalf = 0.0, xc = 0.0;
@@ -219,7 +221,18 @@ __global__ void __launch_bounds__(256) k_inverse_queue(MapDev M, const double* _
         // then the interval's acceptance test.  Starting a quadrature takes
         // one step for F(0) and F(1) (and, for Simpson, one for F(1/2)).
         bool qdone = false;
-        if constexpr (QK == QK_AS) {
+        if constexpr (QK == QK_CC) {
+            // The fixed rule: one step evaluates every node (integrate_cc).
+            double r = 0.0;
+#pragma unroll
+            for (int i = 0; i < MAXNF; ++i) {
+                if (i >= Q.nf) break;
+                const double t = 0.5 * (1.0 + 0.0 + (1.0 - 0.0) * Q.fp[i]);
+                r += (0.5 * (1.0 - 0.0) * Q.fw[i]) * eval_node<BK, PK, FL_VAL, PM>(dc, t).F;
+            }
+            res = r;
+            qdone = true;
+        } else if constexpr (QK == QK_AS) {
             const double m = 0.5 * (qa + qb);
             const double t0 = (k == -2) ? 0.0 : (k == -1) ? 0.5 : 0.5 * (qa + m);
             const double t1 = (k == -2) ? 1.0 : (k == -1) ? 0.5 : 0.5 * (m + qb);

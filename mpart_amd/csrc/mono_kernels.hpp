@@ -141,7 +141,7 @@ __global__ void __launch_bounds__(256) k_logdet_coeff_grad(MapDev M, const doubl
     const PosF<PK> gv = posfunc<PK>(z);
     const double inv = 1.0 / gv.g;
     double bq[PM + 1];
-    diag_basis<BK, PM>(cd.p, xk, bq);
+    diag_basis<BK, PM>(dc, xk, bq);
     double G[PM + 1];
 #pragma unroll
     for (int q = 0; q <= PM; ++q) G[q] = (q <= cd.p) ? M.rho[q] * bq[q] : 0.0;
@@ -327,7 +327,7 @@ __global__ void __launch_bounds__(256) k_input_grad(MapDev M, const double* w, l
         const PosF<PK> gv = posfunc<PK>(z);
         const double inv = 1.0 / gv.g;
         double bq[PM + 1];
-        diag_basis<BK, PM>(cd.p, xk, bq);
+        diag_basis<BK, PM>(dc, xk, bq);
         for (int i = 0; i < k; ++i) {
             double Dq[PM + 1];
             input_groups<PM>(M, cd, wc, ws, tab, dtab, (int)stride, i, Dq);

@@ -68,6 +68,16 @@ CASES = [
     dict(quadType="AdaptiveClenshawCurtis", quadPts=9, posFuncType="Exp"),
     dict(basisNorm=False, nugget=0.05),
     dict(quadMaxSub=6, quadAbsTol=1e-10, quadRelTol=1e-10),
+    # the remaining factory keys (MParT/MapFactory.h:218-245, src/MapFactoryImpl1-18.cpp)
+    dict(basisType="PhysicistHermite"),
+    dict(basisType="PhysicistHermite", posFuncType="Exp", quadType="AdaptiveClenshawCurtis", quadAbsTol=1e-8,
+         quadRelTol=1e-8),
+    dict(quadType="ClenshawCurtis", quadPts=9),
+    dict(basisType="HermiteFunctions", posFuncType="Exp", quadType="ClenshawCurtis", quadPts=6),
+    dict(basisLB=-1.5, basisUB=1.5),
+    dict(basisType="HermiteFunctions", basisLB=-1.0, basisUB=2.0, quadType="AdaptiveClenshawCurtis",
+         quadAbsTol=1e-8, quadRelTol=1e-8),
+    dict(basisType="PhysicistHermite", basisLB=0.5, basisUB=3.0, quadType="ClenshawCurtis", quadPts=7),
 ]
 
 
