@@ -216,6 +216,22 @@ class MapOptions:
     def __repr__(self):
         return "MapOptions(" + ", ".join(f"{k}={getattr(self, k)!r}" for k in self._FIELDS) + ")"
 
+    def Serialize(self, filename):
+        """Writes the options as the reference's cereal binary archive
+        (bindings/python/src/MapOptions.cpp:74-78)."""
+        from . import serialization
+        with open(filename, "wb") as f:
+            f.write(serialization.options_to_bytes(self))
+
+    def Deserialize(self, filename):
+        from . import serialization
+        with open(filename, "rb") as f:
+            serialization.options_from_bytes(f.read(), self)
+        self.basisType = BasisTypes(self.basisType)
+        self.posFuncType = PosFuncTypes(self.posFuncType)
+        self.quadType = QuadTypes(self.quadType)
+        return self
+
 
 class FixedMultiIndexSet:
     """Compressed multi-index set (MParT/MultiIndices/FixedMultiIndexSet.h).
@@ -262,6 +278,21 @@ class FixedMultiIndexSet:
         for k in range(self.nzStarts[i], self.nzStarts[i + 1]):
             m[self.nzDims[k]] = self.nzOrders[k]
         return m
+
+    def Serialize(self, filename):
+        """Writes the set as the reference's cereal binary archive
+        (bindings/python/src/MultiIndex.cpp:243-248)."""
+        from . import serialization
+        with open(filename, "wb") as f:
+            f.write(serialization.multiset_to_bytes(self))
+        return self
+
+    def Deserialize(self, filename):
+        from . import serialization
+        with open(filename, "rb") as f:
+            self.dim, self.nzStarts, self.nzDims, self.nzOrders = serialization.multiset_from_bytes(f.read())
+        self.dim = int(self.dim)
+        return self
 
 
 def _read_multiset(h, i):
@@ -362,6 +393,13 @@ class ConditionalMap:
         from .torch import TorchConditionalMapBase
 
         return TorchConditionalMapBase(self, store_coeffs, return_logdet)
+
+    def Serialize(self, filename):
+        """inputDim, outputDim, numCoeffs and the coefficients as the
+        reference's cereal binary archive (bindings/python/src/ParameterizedFunctionBase.cpp:51-56)."""
+        from . import serialization
+        with open(filename, "wb") as f:
+            f.write(serialization.map_to_bytes(self.inputDim, self.outputDim, self.CoeffMap()))
 
     def CoeffMap(self):
         torch = _torch()
@@ -716,3 +754,11 @@ def TrainMap(map, objective, options=None):
                                 ctypes.byref(nev), _stream()))
     map.last_train_evaluations = nev.value
     return err.value
+
+
+def DeserializeMap(filename):
+    """(inputDim, outputDim, coefficients) from a file written by Serialize
+    (bindings/python/src/Serialization.cpp:22-31)."""
+    from . import serialization
+    with open(filename, "rb") as f:
+        return serialization.map_from_bytes(f.read())
