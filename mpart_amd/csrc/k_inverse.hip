@@ -8,7 +8,8 @@ __global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
 template <int B, int Q, int R, int PM>
 static int inverse_occupancy(int block) {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_inverse_queue<B, Q, R, PM>, block, 0) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_inverse_queue<B, Q, R, PM, false>, block, 0) !=
+        hipSuccess)
         return 1;
     return nb > 0 ? nb : 1;
 }
@@ -41,8 +42,12 @@ hipError_t launch_inverse(int var, const Launch& L, dim3 qgrid, dim3 qblock, con
 #define CALL(B, Q, R, PM)                                                                                          \
     hipLaunchKernelGGL((k_inverse_setup<B, Q, R, PM>), L.grid, L.block, L.smem, L.stream, M, w, ws, xs, xrows, xcols, \
                        xrs, xcs, ys, yss, N, c, gtab, rec);                                                        \
-    hipLaunchKernelGGL((k_inverse_queue<B, Q, R, PM>), qgrid, qblock, 0, L.stream, M, rec, N, out, os, c, xtol, ytol, \
-                       counter)
+    if (M.lin)                                                                                                   \
+        hipLaunchKernelGGL((k_inverse_queue<B, Q, R, PM, true>), qgrid, qblock, 0, L.stream, M, rec, N, out, os, c, \
+                           xtol, ytol, counter);                                                                 \
+    else                                                                                                         \
+        hipLaunchKernelGGL((k_inverse_queue<B, Q, R, PM, false>), qgrid, qblock, 0, L.stream, M, rec, N, out, os, \
+                           c, xtol, ytol, counter)
     MPD_DISPATCH(var, CALL)
 #undef CALL
     return hipGetLastError();

@@ -256,7 +256,8 @@ struct DiagCoef {
     int p;
     double x;   // x_k
     double nug; // nugget added to g inside the integrand
-    double lb, ub;  // LinearizedBasis bounds (+-inf when unused)
+    int lin;        // LinearizedBasis in use (the same for every lane)
+    double lb, ub;  // its bounds
 };
 
 // The argument of the diagonal basis derivatives: a linearized basis has the
@@ -264,6 +265,7 @@ struct DiagCoef {
 template <int PM>
 MPD_HD double lin_arg(const DiagCoef<PM>& dc, double y) {
 #if MPD_LINEARIZED
+    if (!dc.lin) return y;
     return (y < dc.lb) ? dc.lb : ((y > dc.ub) ? dc.ub : y);
 #else
     return y;
@@ -272,7 +274,7 @@ MPD_HD double lin_arg(const DiagCoef<PM>& dc, double y) {
 template <int PM>
 MPD_HD bool lin_outside(const DiagCoef<PM>& dc, double y) {
 #if MPD_LINEARIZED
-    return (y < dc.lb) || (y > dc.ub);
+    return dc.lin && ((y < dc.lb) || (y > dc.ub));
 #else
     return false;
 #endif
@@ -742,6 +744,7 @@ MPD_HD void make_diag(const MapDev& M, const CompDev& c, const double* C, double
     dc.p = c.p;
     dc.x = xk;
     dc.nug = nug;
+    dc.lin = M.lin;
     dc.lb = M.lb;
     dc.ub = M.ub;
 #pragma unroll

@@ -84,7 +84,9 @@ enum { IP_START = 0, IP_DOWN = 1, IP_UP = 2, IP_ITP = 3 };
 
 constexpr int INV_CHUNK = 64;  // point indices taken from the global counter at a time
 
-template <int BK, int PK, int QK, int PM>
+// LIN: the map has LinearizedBasis bounds.  Without them the kernel is built
+// with the bound checks folded away (they cost ~15 % of the integrand work).
+template <int BK, int PK, int QK, int PM, bool LIN>
 __global__ void __launch_bounds__(256) k_inverse_queue(MapDev M, const double* __restrict__ rec, long N, double* out,
                                                        long os, int c, double xtol, double ytol,
                                                        unsigned long long* counter) {
@@ -138,6 +140,7 @@ __global__ void __launch_bounds__(256) k_inverse_queue(MapDev M, const double* _
     dc.x = 0.0;
 #pragma unroll
     for (int q = 0; q <= PM; ++q) dc.cd[q] = 0.0;
+    dc.lin = LIN ? M.lin : 0;
     dc.lb = M.lb;
     dc.ub = M.ub;
     double f0 = 0.0, yd = 0.0, x0 = 0.0;
