@@ -174,6 +174,15 @@ int mpart_kl_value_and_grad(mpart_map map, mpart_cview pts, double* loss, double
  * then a row reduction (src/MapObjective.cpp:107-215). */
 int mpart_kl_value_and_grad_plain(mpart_map map, mpart_cview pts, double* loss, double* grad, mpart_stream stream);
 
+/* Input gradient of the same objective, per point and fused over the
+ * components: out (inputDim x N) = scale * grad_x [ |T(x_n)|^2 / 2 -
+ * log det grad T(x_n) ], the point gradient the torch autograd path forms as
+ * GradientImpl(pts, sens = T) - LogDeterminantInputGradImpl(pts)
+ * (MParT/ParameterizedFunctionBase.h:140-160, MParT/ConditionalMapBase.h:148-159;
+ * bindings/python/package/torch_helpers.py:91-107), in one launch without
+ * storing T.  scale = 1/N gives the gradient of the mean. */
+int mpart_kl_input_grad(mpart_map map, mpart_cview pts, double scale, mpart_view out, mpart_stream stream);
+
 /* TrainOptions (MParT/TrainMap.h:19-40): NLopt-style stopping rules.  The
  * reference's opt_alg names an NLopt algorithm (default "LD_SLSQP"); here the
  * one optimizer is L-BFGS (MPART_TRAIN_LBFGS) with a strong-Wolfe line search;

@@ -104,6 +104,7 @@ _SIGS = {
     "mpart_kl_value_and_grad": [_vp, _CView, _vp, _vp, _vp],
     "mpart_kl_value_and_grad_plain": [_vp, _CView, _vp, _vp, _vp],
     "mpart_coeff_vjp": [_vp, _CView, _CView, _vp, _i64, _vp, _vp],
+    "mpart_kl_input_grad": [_vp, _CView, _dbl, _CView, _vp],
     "mpart_set_plain_launches": [_vp, _int],
     "mpart_set_codegen": [_vp, _int],
     "mpart_create_batch": [_MAPP, _u32, _MAPP],
@@ -525,6 +526,21 @@ class ConditionalMap:
             torch.cuda.current_stream().synchronize()
             return float(loss.item()), grad.cpu().numpy()
         return loss, grad
+
+    def KLInputGradImpl(self, pts, out, scale=1.0):
+        """out (inputDim x N) = scale * grad_x [0.5 |T(x)|^2 - log det grad T(x)], per point, fused."""
+        _check(_lib.mpart_kl_input_grad(self._h, _view(pts), float(scale), _view(out), _stream()))
+
+    def KLInputGrad(self, pts, scale=1.0):
+        """Gradient(pts, T(pts)) - LogDeterminantInputGrad(pts), times scale, in one launch."""
+        torch = _torch()
+        p, host = _as_dev(pts)
+        out = torch.empty((self.inputDim, p.shape[1]), dtype=torch.float64, device="cuda")
+        self.KLInputGradImpl(p, out, scale)
+        if host:
+            torch.cuda.current_stream().synchronize()
+            return out.cpu().numpy()
+        return out
 
     def CoeffVJP(self, pts, sens=None, logdet_sens=None, out=None):
         """sum_n CoeffGrad(sens)(:, n) + logdet_sens[n] LogDeterminantCoeffGrad(:, n), fused."""

@@ -16,4 +16,16 @@ hipError_t launch_input_grad(int var, const Launch& L, const MapDev& M, const do
     return hipGetLastError();
 }
 
+hipError_t launch_kl_input_grad(int var, const Launch& L, const MapDev& M, const double* w, long ws, const double* pts,
+                                long prs, long pcs, long N, double scale, double* out, long ors, long ocs,
+                                double* gtab) {
+    PtsArg P{pts, prs, pcs, N};
+    OutArg O{out, ors, ocs};
+#define CALL(B, Q, R, PM) \
+    hipLaunchKernelGGL((k_kl_input_grad<B, Q, R, PM>), L.grid, L.block, L.smem, L.stream, M, w, ws, P, scale, O, gtab)
+    MPD_DISPATCH(var, CALL)
+#undef CALL
+    return hipGetLastError();
+}
+
 }  // namespace mpd

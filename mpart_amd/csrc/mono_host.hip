@@ -764,6 +764,19 @@ void do_input_grad(mpart_map_s& m, const mpart_cview& p, const mpart_cview* sens
                   "launch input_grad");
 }
 
+// KL input gradient, every component in one launch.
+void do_kl_input_grad(mpart_map_s& m, const mpart_cview& p, double scale, const mpart_view& o, hipStream_t s) {
+    check_coeffs(&m);
+    check_view("KL input gradient: pts", p.rows, p.cols, m.inDim, p.cols);
+    check_view("KL input gradient: output", o.rows, o.cols, m.inDim, p.cols);
+    if (!m.opts.contDeriv) throw ArgError("Discrete derivative version is not implemented yet (To Do)");
+    if (p.cols == 0) return;
+    Shape sh = shape_for(m, p.cols, s, 2);
+    hip_check(mpd::launch_kl_input_grad(m.variant, sh.L, m.dev, m.coeffs, m.coeffStride, p.ptr, p.row_stride,
+                                        p.col_stride, p.cols, scale, o.ptr, o.row_stride, o.col_stride, sh.gtab),
+              "launch kl_input_grad");
+}
+
 // Inverse of component c: a lane-per-point setup pass writes each point's
 // record, then a persistent pass solves with per-lane scheduling (lanes take
 // new points as their solves end).
@@ -1637,6 +1650,13 @@ int mpart_coeff_vjp(mpart_map m, mpart_cview pts, mpart_cview sens, const double
         if (sens.ptr) check_view("vjp: sens", sens.rows, sens.cols, m->outDim, pts.cols);
         if (!m->opts.contDeriv) throw NotImpl("contDeriv = false is not provided yet");
         do_fused(*m, pts, sens.ptr, sens.row_stride, sens.col_stride, lsens, lss, 0, nullptr, grad, 1.0, S(s));
+    });
+}
+
+int mpart_kl_input_grad(mpart_map m, mpart_cview pts, double scale, mpart_view out, mpart_stream s) {
+    return guarded([&] {
+        check_map(m);
+        do_kl_input_grad(*m, pts, scale, out, S(s));
     });
 }
 

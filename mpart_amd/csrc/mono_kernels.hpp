@@ -344,6 +344,77 @@ __global__ void __launch_bounds__(256) k_input_grad(MapDev M, const double* w, l
 }
 
 // ---------------------------------------------------------------------------
+// Input gradient of the KL objective, all components in one pass:
+//   out(:, n) = scale * grad_x [ |T(x_n)|^2 / 2 - log det grad T(x_n) ],
+// i.e. GradientImpl with sens = T (InputJacobian, MonotoneComponent.h:148-164,
+// 758-818) minus LogDeterminantInputGradImpl (ContinuousMixedInputJacobian,
+// 208-232, 877-926), summed over components, without storing T.  T is the
+// Evaluate value (no nugget, 287); the input-adjoint integral carries the
+// nugget (434-style, as the reference's Jacobians do).
+template <int BK, int PK, int QK, int PM>
+__global__ void __launch_bounds__(256) k_kl_input_grad(MapDev M, const double* w, long ws, PtsArg P, double scale,
+                                                       OutArg out, double* gtab) {
+    extern __shared__ double smem[];
+    const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= P.N) return;
+    long stride;
+    double* tab;
+    if (gtab) {
+        stride = (long)gridDim.x * blockDim.x;
+        tab = gtab + n;
+    } else {
+        stride = blockDim.x;
+        tab = smem + threadIdx.x;
+    }
+    double* dtab = tab + (long)M.nslots * stride;
+    const double* pt = P.p + n * P.cs;
+    fill_values_derivs<BK>(M, table_dims(M, 0, M.ncomp), pt, P.rs, tab, dtab, (int)stride);
+    double* o = out.p + n * out.cs;
+    for (int d = 0; d < M.D; ++d) o[(long)d * out.rs] = 0.0;
+    for (int c = 0; c < M.ncomp; ++c) {
+        const CompDev cd = M.comps[c];
+        const int k = cd.dim - 1;
+        const double* wc = w + (long)cd.coeffOff * ws;
+        double C[PM + 1];
+        coeff_groups<PM>(M, cd, wc, ws, tab, (int)stride, C);
+        const double f0 = f_at_zero<PM>(M, cd, C);
+        const double xk = pt[(long)k * P.rs];
+        DiagCoef<PM> dc;
+        make_diag<PM>(M, cd, C, xk, M.nugget, dc);
+        AdjAcc<PM> A;
+        zero_adj<PM>(A);
+        double T = integrate<BK, PK, QK, FL_INPUT, PM>(M.quad, dc, A, nullptr) + f0;
+        if (M.nugget != 0.0) {
+            DiagCoef<PM> dv;
+            make_diag<PM>(M, cd, C, xk, 0.0, dv);
+            AdjAcc<PM> Av;
+            T = integrate<BK, PK, QK, FL_VAL, PM>(M.quad, dv, Av, nullptr) + f0;
+        }
+        DiagCoef<PM> dl;
+        make_diag<PM>(M, cd, C, xk, 0.0, dl);
+        double z2;
+        const double z = diag_eval<BK, true, PM>(dl, xk, &z2);
+        const PosF<PK> gv = posfunc<PK>(z);
+        const double inv = 1.0 / gv.g;
+        double bq[PM + 1];
+        diag_basis<BK, PM>(dl, xk, bq);
+        for (int i = 0; i < k; ++i) {
+            double Dq[PM + 1];
+            input_groups<PM>(M, cd, wc, ws, tab, dtab, (int)stride, i, Dq);
+            double v0 = 0.0, v1 = 0.0;
+#pragma unroll
+            for (int q = 0; q <= PM; ++q) {
+                if (q > cd.p) break;
+                v0 += Dq[q] * (M.phi0[q] + M.rho[q] * A.S[q]);
+                v1 += Dq[q] * (M.rho[q] * bq[q]);
+            }
+            o[(long)i * out.rs] += scale * (v0 * T - (v1 * gv.gp) * inv);
+        }
+        o[(long)k * out.rs] += scale * (A.U * T - (z2 * gv.gp) * inv);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Inverse of one component (MonotoneComponent::InverseImpl, MonotoneComponent.h:331-443;
 // RootFinding.h:20-178).  xs has xrows rows and 1 or N columns; its last row
 // holds the initial guess.  out may alias that row.

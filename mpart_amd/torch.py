@@ -143,24 +143,36 @@ class KLObjective(torch.autograd.Function):
     FastGaussianReverseKLObjective (src/MapObjective.cpp:146-234) as one autograd
     node: forward runs the fused forward+adjoint launch
     (ConditionalMap.KLValueAndGradImpl) and keeps the coefficient gradient;
-    backward scales it.  Gradients with respect to the points are not formed."""
+    when the points require a gradient it also runs the fused input-adjoint
+    launch (ConditionalMap.KLInputGradImpl), the per-point gradient
+    (Gradient(x, T) - LogDeterminantInputGrad(x)) / N that
+    MpartTorchAutograd's backward would form from two launches and T.
+    backward scales both by the incoming sensitivity."""
 
     @staticmethod
     def forward(ctx, x, coeffs, f):
-        if x.requires_grad:
-            raise ValueError("KLObjective differentiates with respect to the coefficients only")
         cd = coeffs.detach().to(torch.float64).contiguous()
         f.WrapCoeffs(cd)
+        xd = x.detach().to(torch.float64)
         loss = torch.zeros(1, dtype=torch.float64, device=x.device)
         grad = torch.zeros(f.numCoeffs, dtype=torch.float64, device=x.device)
-        f.KLValueAndGradImpl(x.detach().to(torch.float64), loss, grad)
-        ctx.save_for_backward(grad)
+        f.KLValueAndGradImpl(xd, loss, grad)
+        xgrad = None
+        if ctx.needs_input_grad[0]:
+            xgrad = torch.empty((f.inputDim, xd.shape[1]), dtype=torch.float64, device=x.device)
+            f.KLInputGradImpl(xd, xgrad, 1.0 / xd.shape[1])
+        ctx.save_for_backward(grad, xgrad)
+        ctx.x_dtype = x.dtype
         return loss[0].to(coeffs.dtype)
 
     @staticmethod
     def backward(ctx, grad_out):
-        (grad,) = ctx.saved_tensors
-        return None, (grad_out * grad).to(grad_out.dtype), None
+        grad, xgrad = ctx.saved_tensors
+        gx = None
+        if xgrad is not None:
+            gx = (grad_out.to(torch.float64) * xgrad).to(ctx.x_dtype)
+        gc = (grad_out * grad).to(grad_out.dtype) if ctx.needs_input_grad[1] else None
+        return gx, gc, None
 
 
 def kl_objective(f, x, coeffs):
