@@ -263,6 +263,43 @@ def single_component_workload(steps):
     return out
 
 
+def input_grad_workload(tri, x, steps):
+    """Point gradient of the KL objective, grad_x [0.5 |T|^2 - logdet] / N, on the
+    headline map and the given points: the fused launch (KLInputGradImpl)
+    against the unfused sequence the reference's torch wrapper runs (Evaluate,
+    Gradient with sens = T, LogDeterminantInputGrad, then the difference)."""
+    import torch
+
+    D_, N = x.shape
+    y = torch.empty((tri.outputDim, N), dtype=torch.float64, device="cuda")
+    g1 = torch.empty((D_, N), dtype=torch.float64, device="cuda")
+    g2 = torch.empty((D_, N), dtype=torch.float64, device="cuda")
+    gf = torch.empty((D_, N), dtype=torch.float64, device="cuda")
+
+    def fused():
+        tri.KLInputGradImpl(x, gf, 1.0 / N)
+
+    def unfused():
+        tri.EvaluateImpl(x, y)
+        tri.GradientImpl(x, y, g1)
+        tri.LogDeterminantInputGradImpl(x, g2)
+        torch.sub(g1, g2, out=g1).mul_(1.0 / N)
+
+    out = {"N": N}
+    for name, fn in (("fused", fused), ("unfused", unfused)):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        out[name + "_ms"] = (time.perf_counter() - t0) * 1e3 / steps
+        log(f"KL input gradient {name}: {out[name + '_ms']:.3f} ms")
+    out["speedup"] = out["unfused_ms"] / out["fused_ms"]
+    out["max_rel_diff"] = ((gf - g1).abs().max() / g1.abs().max()).item()
+    return out
+
+
 def load_traffic():
     """HBM bytes per fused launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
     summary of this workload (None when absent)."""
@@ -306,6 +343,7 @@ def main():
     ap.add_argument("--no-second", action="store_true")
     ap.add_argument("--no-many", action="store_true")
     ap.add_argument("--no-single", action="store_true")
+    ap.add_argument("--no-input-grad", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--profile", action="store_true")
     ap.add_argument("--profile-inverse", action="store_true")
@@ -360,6 +398,8 @@ def main():
         second, _ = measure(tri, x2, args.steps, args.warmup, args.plain_steps, not args.no_plain)
         second["oracle_max_rel_err_2000pts"] = oracle_check(x2_np, c2, gpu_kl)
         second["data"] = "x ~ N(0, I_8) (seed 1 stream), coefficients 0.1 N(0,1) from the same stream"
+        if not args.no_input_grad:
+            second["kl_input_grad"] = input_grad_workload(tri, x2, max(3, args.plain_steps))
         del x2
 
     many = None

@@ -60,6 +60,38 @@ void emit_pin(std::ostringstream& o, const GenSpec& s, unsigned ndims, const cha
             o << indent << "asm volatile(\"\" : \"+v\"(b" << d << "_" << k << "));\n";
 }
 
+// The same with first derivatives db<d>_<k> (FillCache1 with Input flags).
+void emit_basis_derivs(std::ostringstream& o, const GenSpec& s, unsigned ndims) {
+    for (unsigned d = 0; d < ndims && d < s.dimDeg.size(); ++d) {
+        const int deg = s.dimDeg[d];
+        if (deg == 0) continue;
+        o << "    double b" << d << "_1, db" << d << "_1";
+        for (int k = 2; k <= deg; ++k) o << ", b" << d << "_" << k << ", db" << d << "_" << k;
+        o << ";\n    {\n        double t[" << deg + 1 << "], dt[" << deg + 1 << "];\n";
+        o << "        basis_vals_derivs<GBK, " << deg << ">(M, pt[" << d << " * prs], t, dt);\n";
+        for (int k = 1; k <= deg; ++k)
+            o << "        b" << d << "_" << k << " = t[" << k << "]; db" << d << "_" << k << " = dt[" << k << "];\n";
+        o << "    }\n";
+    }
+}
+
+void emit_pin_derivs(std::ostringstream& o, const GenSpec& s, unsigned ndims, const char* indent) {
+    emit_pin(o, s, ndims, indent);
+    for (unsigned d = 0; d < ndims && d < s.dimDeg.size(); ++d)
+        for (int k = 1; k <= s.dimDeg[d]; ++k)
+            o << indent << "asm volatile(\"\" : \"+v\"(db" << d << "_" << k << "));\n";
+}
+
+// d/dx_{dims[j]} of the off-diagonal product of term t (entry j of the term).
+std::string term_deriv(const GenComp& c, unsigned t, unsigned j) {
+    std::string expr = "db" + std::to_string(c.dims[j]) + "_" + std::to_string(c.orders[j]);
+    for (unsigned i = c.starts[t]; i < c.starts[t + 1]; ++i) {
+        if (i == j || c.dims[i] == c.dim - 1) continue;
+        expr = "(" + expr + " * b" + std::to_string(c.dims[i]) + "_" + std::to_string(c.orders[i]) + ")";
+    }
+    return expr;
+}
+
 void emit_coeff_sums(std::ostringstream& o, const GenComp& c) {
     for (unsigned q = 0; q <= c.p; ++q) o << "        C[" << q << "] = 0.0;\n";
     const unsigned nt = unsigned(c.starts.size()) - 1;
@@ -285,6 +317,52 @@ extern "C" __global__ void __launch_bounds__(256) mpart_gen_ldcoef(MapDev M, con
         }
         o << "    }\n}\n";
     }
+
+    // ---------------- KL input gradient (all components) ----------------
+    o << R"(
+extern "C" __global__ void __launch_bounds__(256) mpart_gen_klx(MapDev M, const double* __restrict__ w,
+        const double* __restrict__ pts, long prs, long pcs, long N, double scale, double* out, long ors, long ocs) {
+    const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    const double* pt = pts + n * pcs;
+)";
+    emit_basis_derivs(o, s, s.D - 1);
+    for (unsigned d = 0; d < s.D; ++d) o << "    double gx" << d << " = 0.0;\n";
+    o << "#pragma unroll 1\n    for (int c = 0; c < " << ncomp << "; ++c) {\n";
+    emit_pin_derivs(o, s, s.D - 1, "        ");
+    o << "        const CompDev cd = M.comps[c];\n        double C[GPM + 1];\n        switch (c) {\n";
+    for (unsigned c = 0; c < ncomp; ++c) {
+        o << "        case " << c << ": {\n";
+        emit_coeff_sums(o, s.comps[c]);
+        o << "        break; }\n";
+    }
+    o << R"(        }
+        double G[GPM + 1], gk;
+        klx_core<GBK, GPK, GQK, GPM>(M, cd, C, pt[(long)(cd.dim - 1) * prs], G, gk);
+)";
+    emit_pin_derivs(o, s, s.D - 1, "        ");
+    o << "        switch (c) {\n";
+    for (unsigned c = 0; c < ncomp; ++c) {
+        const GenComp& gc = s.comps[c];
+        const unsigned nt = unsigned(gc.starts.size()) - 1;
+        o << "        case " << c << ": {\n";
+        for (unsigned t = 0; t < nt; ++t) {
+            unsigned q;
+            term_product(gc, t, &q);
+            bool any = false;
+            for (unsigned j = gc.starts[t]; j < gc.starts[t + 1]; ++j) {
+                if (gc.dims[j] == gc.dim - 1) continue;
+                if (!any) o << "            { const double a = w[" << gc.coeffOff + t << "] * G[" << q << "];\n";
+                any = true;
+                o << "              gx" << gc.dims[j] << " += a * " << term_deriv(gc, t, j) << ";\n";
+            }
+            if (any) o << "            }\n";
+        }
+        o << "            gx" << gc.dim - 1 << " += gk;\n        break; }\n";
+    }
+    o << "        }\n    }\n    double* o = out + n * ocs;\n";
+    for (unsigned d = 0; d < s.D; ++d) o << "    o[" << d << "L * ors] = scale * gx" << d << ";\n";
+    o << "}\n";
     return o.str();
 }
 
@@ -345,7 +423,8 @@ GenModule* get_module(const GenSpec& spec, std::string& err) {
         hipModuleGetFunction(&m->fused3, m->mod, "mpart_gen_fused_w3") != hipSuccess ||
         hipModuleGetFunction(&m->eval, m->mod, "mpart_gen_eval") != hipSuccess ||
         hipModuleGetFunction(&m->coef, m->mod, "mpart_gen_coef") != hipSuccess ||
-        hipModuleGetFunction(&m->ldcoef, m->mod, "mpart_gen_ldcoef") != hipSuccess) {
+        hipModuleGetFunction(&m->ldcoef, m->mod, "mpart_gen_ldcoef") != hipSuccess ||
+        hipModuleGetFunction(&m->klx, m->mod, "mpart_gen_klx") != hipSuccess) {
         err = "hipModuleGetFunction failed";
         return nullptr;
     }

@@ -582,8 +582,82 @@ MPD_HD void basis_vals(const MapDev& M, double x, double* b) {
 }
 
 // Values and first derivatives as EvaluateDerivatives computes them
-// (OrthogonalPolynomial.h:87-118, HermiteFunction.h:45-80); used by the
-// input-gradient paths (FillCache1 with DerivativeFlags::Input).
+// (OrthogonalPolynomial.h:87-118, HermiteFunction.h:45-80) of orders 1..deg
+// of one coordinate, into out[o * stride] and dout[o * stride].
+template <int BK>
+MPD_HD void values_derivs_1d(const MapDev& M, int deg, double x, double* out, double* dout, long stride) {
+    double c;
+    if (lin_bound(M, x, c)) {
+        // LinearizedBasis::EvaluateDerivatives (LinearizedBasis.h:63-87)
+        for (int o = 1; o <= deg; ++o) {
+            double v, dv;
+            basis_vd<BK>(M, o, c, v, dv);
+            out[o * stride] = v + dv * (x - c);
+            dout[o * stride] = dv;
+        }
+        return;
+    }
+    if constexpr (BK != BK_HF) {
+        constexpr double A = poly_a<BK>();
+        double vm2 = 1.0, vm1 = A * x, dm2 = 0.0, dm1 = A;
+        for (int o = 1; o <= deg; ++o) {
+            double v, dv;
+            if (o == 1) {
+                v = A * x;
+                dv = A;
+            } else {
+                v = (A * x + 0.0) * vm1 - (A * (o - 1.0)) * vm2;
+                dv = A * vm1 + (A * x + 0.0) * dm1 - (A * (o - 1.0)) * dm2;
+                vm2 = vm1;
+                vm1 = v;
+                dm2 = dm1;
+                dm1 = dv;
+            }
+            if (M.normalize) {
+                double n = M.norm[o];
+                v /= n;
+                dv /= n;
+            }
+            out[o * stride] = v;
+            dout[o * stride] = dv;
+        }
+    } else {
+        // phi_1 = x (derivative 1); phi_{k+2}: physicist H_k and H'_k scaled
+        // by pi^-1/4 e^{-x^2/2} (2^k k!)^-1/2, derivative (H'_k - x H_k) * scale.
+        const double base = M.hfPi * exp(-0.5 * x * x);
+        double hm1 = 0.0, hm2 = 0.0, dm1 = 0.0, dm2 = 0.0;
+        for (int o = 1; o <= deg; ++o) {
+            if (o == 1) {
+                out[o * stride] = x;
+                dout[o * stride] = 1.0;
+                continue;
+            }
+            int k = o - 2;
+            double h, dh;
+            if (k == 0) {
+                h = 1.0;
+                dh = 0.0;
+            } else if (k == 1) {
+                h = 2.0 * x;
+                dh = 2.0;
+            } else {
+                h = (2.0 * x + 0.0) * hm1 - 2.0 * (k - 1.0) * hm2;
+                dh = 2.0 * hm1 + (2.0 * x + 0.0) * dm1 - 2.0 * (k - 1.0) * dm2;
+            }
+            hm2 = hm1;
+            hm1 = h;
+            dm2 = dm1;
+            dm1 = dh;
+            double scale = (k == 0) ? base : base * M.hfS[k];
+            double dd = dh - x * h;
+            out[o * stride] = h * scale;
+            dout[o * stride] = dd * scale;
+        }
+    }
+}
+
+// Value and derivative tables of coordinates 0..ndims-1 (FillCache1 with
+// DerivativeFlags::Input); used by the input-gradient paths.
 template <int BK>
 __host__ __device__ void fill_values_derivs(const MapDev& M, int ndims, const double* pt, long prs, double* tab, double* dtab,
                                    int stride) {
@@ -592,78 +666,18 @@ __host__ __device__ void fill_values_derivs(const MapDev& M, int ndims, const do
     for (int d = 0; d < ndims; ++d) {
         const int deg = M.dimDeg[d];
         if (deg == 0) continue;
-        const double x = pt[d * prs];
-        double* out = tab + (long)M.dimSlot[d] * stride - stride;
-        double* dout = dtab + (long)M.dimSlot[d] * stride - stride;
-        double c;
-        if (lin_bound(M, x, c)) {
-            // LinearizedBasis::EvaluateDerivatives (LinearizedBasis.h:63-87)
-            for (int o = 1; o <= deg; ++o) {
-                double v, dv;
-                basis_vd<BK>(M, o, c, v, dv);
-                out[o * stride] = v + dv * (x - c);
-                dout[o * stride] = dv;
-            }
-            continue;
-        }
-        if constexpr (BK != BK_HF) {
-            constexpr double A = poly_a<BK>();
-            double vm2 = 1.0, vm1 = A * x, dm2 = 0.0, dm1 = A;
-            for (int o = 1; o <= deg; ++o) {
-                double v, dv;
-                if (o == 1) {
-                    v = A * x;
-                    dv = A;
-                } else {
-                    v = (A * x + 0.0) * vm1 - (A * (o - 1.0)) * vm2;
-                    dv = A * vm1 + (A * x + 0.0) * dm1 - (A * (o - 1.0)) * dm2;
-                    vm2 = vm1;
-                    vm1 = v;
-                    dm2 = dm1;
-                    dm1 = dv;
-                }
-                if (M.normalize) {
-                    double n = M.norm[o];
-                    v /= n;
-                    dv /= n;
-                }
-                out[o * stride] = v;
-                dout[o * stride] = dv;
-            }
-        } else {
-            // phi_1 = x (derivative 1); phi_{k+2}: physicist H_k and H'_k scaled
-            // by pi^-1/4 e^{-x^2/2} (2^k k!)^-1/2, derivative (H'_k - x H_k) * scale.
-            const double base = M.hfPi * exp(-0.5 * x * x);
-            double hm1 = 0.0, hm2 = 0.0, dm1 = 0.0, dm2 = 0.0;
-            for (int o = 1; o <= deg; ++o) {
-                if (o == 1) {
-                    out[o * stride] = x;
-                    dout[o * stride] = 1.0;
-                    continue;
-                }
-                int k = o - 2;
-                double h, dh;
-                if (k == 0) {
-                    h = 1.0;
-                    dh = 0.0;
-                } else if (k == 1) {
-                    h = 2.0 * x;
-                    dh = 2.0;
-                } else {
-                    h = (2.0 * x + 0.0) * hm1 - 2.0 * (k - 1.0) * hm2;
-                    dh = 2.0 * hm1 + (2.0 * x + 0.0) * dm1 - 2.0 * (k - 1.0) * dm2;
-                }
-                hm2 = hm1;
-                hm1 = h;
-                dm2 = dm1;
-                dm1 = dh;
-                double scale = (k == 0) ? base : base * M.hfS[k];
-                double dd = dh - x * h;
-                out[o * stride] = h * scale;
-                dout[o * stride] = dd * scale;
-            }
-        }
+        values_derivs_1d<BK>(M, deg, pt[d * prs], tab + (long)M.dimSlot[d] * stride - stride,
+                             dtab + (long)M.dimSlot[d] * stride - stride, stride);
     }
+}
+
+// The same values and derivatives into b[1..DEG], db[1..DEG] (b[0] = 1,
+// db[0] = 0), for generated kernels that keep them in registers.
+template <int BK, int DEG>
+MPD_HD void basis_vals_derivs(const MapDev& M, double x, double* b, double* db) {
+    b[0] = 1.0;
+    db[0] = 0.0;
+    values_derivs_1d<BK>(M, DEG, x, b, db, 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -1113,6 +1127,46 @@ MPD_HD void comp_core(const MapDev& M, const CompDev& cd, const double* C, doubl
         if (q > cd.p) break;
         G[q] = s * (M.phi0[q] + M.rho[q] * A.S[q]) + (l * r) * (M.rho[q] * bq[q]);
     }
+}
+// Per-component factors of the KL input gradient at one point,
+//   d/dx [T_c^2 / 2 - log g(d_c f)] = sum_q G_q dC_q/dx_i  (i < c)  and  gk  (i = c),
+// with G_q = T_c (phi~_q(0) + rho_q S_q) - (g'/g) rho_q B'_q(x_c) and
+// gk = T_c U - (g'/g) d^2_c f (InputJacobian with sens = T_c,
+// MonotoneComponent.h:758-818, minus ContinuousMixedInputJacobian, 877-926).
+// T_c is the Evaluate value (no nugget, 287); S_q and U carry the nugget as
+// the reference's input Jacobian does.  G[0..PM] is written.
+template <int BK, int PK, int QK, int PM>
+MPD_HD void klx_core(const MapDev& M, const CompDev& cd, const double* C, double xk, double* G, double& gk) {
+    const double f0 = f_at_zero<PM>(M, cd, C);
+    DiagCoef<PM> dc;
+    make_diag<PM>(M, cd, C, xk, M.nugget, dc);
+    // g and g' at x_k are the integrand's values at t = 1.
+    double z2;
+    const double z = diag_eval<BK, true, PM>(dc, 1.0 * xk, &z2);
+    const PosF<PK> gv = posfunc<PK>(z);
+    NodeV<FL_INPUT> n1;
+    n1.F = dc.x * (gv.g + dc.nug);
+    n1.h = dc.x * gv.gp;
+    n1.g1 = gv.g + dc.nug;
+    AdjAcc<PM> A;
+    zero_adj<PM>(A);
+    double T = integrate<BK, PK, QK, FL_INPUT, PM>(M.quad, dc, A, nullptr, &n1) + f0;
+    if (M.nugget != 0.0) {
+        DiagCoef<PM> dv = dc;
+        dv.nug = 0.0;
+        NodeV<FL_VAL> v1;
+        v1.F = dv.x * gv.g;
+        AdjAcc<PM> Av;
+        T = integrate<BK, PK, QK, FL_VAL, PM>(M.quad, dv, Av, nullptr, &v1) + f0;
+    }
+    const double r = gv.gp * (1.0 / gv.g);
+    double bq[PM + 1];
+    diag_basis<BK, PM>(dc, xk, bq);
+    G[0] = T;
+#pragma unroll
+    for (int q = 1; q <= PM; ++q)
+        G[q] = (q <= cd.p) ? T * (M.phi0[q] + M.rho[q] * A.S[q]) - r * (M.rho[q] * bq[q]) : 0.0;
+    gk = A.U * T - r * z2;
 }
 // ---------------------------------------------------------------------------
 // Wavefront reduction of 16 per-lane values v[0..15] (one per term of a

@@ -771,6 +771,17 @@ void do_kl_input_grad(mpart_map_s& m, const mpart_cview& p, double scale, const 
     check_view("KL input gradient: output", o.rows, o.cols, m.inDim, p.cols);
     if (!m.opts.contDeriv) throw ArgError("Discrete derivative version is not implemented yet (To Do)");
     if (p.cols == 0) return;
+    if (mpg::GenModule* g = gen_for(m)) {
+        const double* w = gen_coeffs(m, s);
+        MapDev M = m.dev;
+        const double* pts = p.ptr;
+        long prs = p.row_stride, pcs = p.col_stride, N = p.cols;
+        double* op = o.ptr;
+        long ors = o.row_stride, ocs = o.col_stride;
+        void* args[] = {&M, &w, &pts, &prs, &pcs, &N, &scale, &op, &ors, &ocs};
+        gen_launch(g->klx, (N + 255) / 256, 256, 0, s, args, "launch specialized kl_input_grad");
+        return;
+    }
     Shape sh = shape_for(m, p.cols, s, 2);
     hip_check(mpd::launch_kl_input_grad(m.variant, sh.L, m.dev, m.coeffs, m.coeffStride, p.ptr, p.row_stride,
                                         p.col_stride, p.cols, scale, o.ptr, o.row_stride, o.col_stride, sh.gtab),

@@ -377,40 +377,20 @@ __global__ void __launch_bounds__(256) k_kl_input_grad(MapDev M, const double* w
         const double* wc = w + (long)cd.coeffOff * ws;
         double C[PM + 1];
         coeff_groups<PM>(M, cd, wc, ws, tab, (int)stride, C);
-        const double f0 = f_at_zero<PM>(M, cd, C);
-        const double xk = pt[(long)k * P.rs];
-        DiagCoef<PM> dc;
-        make_diag<PM>(M, cd, C, xk, M.nugget, dc);
-        AdjAcc<PM> A;
-        zero_adj<PM>(A);
-        double T = integrate<BK, PK, QK, FL_INPUT, PM>(M.quad, dc, A, nullptr) + f0;
-        if (M.nugget != 0.0) {
-            DiagCoef<PM> dv;
-            make_diag<PM>(M, cd, C, xk, 0.0, dv);
-            AdjAcc<PM> Av;
-            T = integrate<BK, PK, QK, FL_VAL, PM>(M.quad, dv, Av, nullptr) + f0;
-        }
-        DiagCoef<PM> dl;
-        make_diag<PM>(M, cd, C, xk, 0.0, dl);
-        double z2;
-        const double z = diag_eval<BK, true, PM>(dl, xk, &z2);
-        const PosF<PK> gv = posfunc<PK>(z);
-        const double inv = 1.0 / gv.g;
-        double bq[PM + 1];
-        diag_basis<BK, PM>(dl, xk, bq);
+        double G[PM + 1], gk;
+        klx_core<BK, PK, QK, PM>(M, cd, C, pt[(long)k * P.rs], G, gk);
         for (int i = 0; i < k; ++i) {
             double Dq[PM + 1];
             input_groups<PM>(M, cd, wc, ws, tab, dtab, (int)stride, i, Dq);
-            double v0 = 0.0, v1 = 0.0;
+            double v = 0.0;
 #pragma unroll
             for (int q = 0; q <= PM; ++q) {
                 if (q > cd.p) break;
-                v0 += Dq[q] * (M.phi0[q] + M.rho[q] * A.S[q]);
-                v1 += Dq[q] * (M.rho[q] * bq[q]);
+                v += Dq[q] * G[q];
             }
-            o[(long)i * out.rs] += scale * (v0 * T - (v1 * gv.gp) * inv);
+            o[(long)i * out.rs] += scale * v;
         }
-        o[(long)k * out.rs] += scale * (A.U * T - (z2 * gv.gp) * inv);
+        o[(long)k * out.rs] += scale * gk;
     }
 }
 

@@ -61,10 +61,16 @@ def test_triangular_matches_oracle(in_dim, out_dim, order, case):
     r = orc.CreateTriangular(in_dim, out_dim, order, ref_opts(**case))
     r.SetCoeffs(c)
     ref = _oracle_kl_input_grad(r, x)
-    _close(g.KLInputGrad(x), ref)
+    spec = g.KLInputGrad(x)
+    _close(spec, ref)
     # the scale multiplies every entry, and a strided view of the points is read in place
     xd = torch.from_numpy(np.ascontiguousarray(x.T)).cuda().T
     _close(g.KLInputGrad(xd, scale=0.25).cpu().numpy(), 0.25 * ref)
+    # the generic kernel (term records in memory, not specialized code)
+    g.SetCodegen(False)
+    gen = g.KLInputGrad(x)
+    _close(gen, ref)
+    _close(gen, spec, rel=1e-13)
 
 
 def test_large_table_in_global_memory():
