@@ -183,6 +183,15 @@ int mpart_kl_value_and_grad_plain(mpart_map map, mpart_cview pts, double* loss, 
  * storing T.  scale = 1/N gives the gradient of the mean. */
 int mpart_kl_input_grad(mpart_map map, mpart_cview pts, double scale, mpart_view out, mpart_stream stream);
 
+/* Input-side vector-Jacobian product used by the torch backward
+ * (bindings/python/package/torch_helpers.py:91-107), per point and fused over
+ * the components: out(:, n) = Gradient(sens)(:, n) + logdetSens[n]
+ * LogDeterminantInputGrad(:, n) (MParT/ParameterizedFunctionBase.h:140-160,
+ * MParT/ConditionalMapBase.h:148-159) in one launch.  sens.ptr or logdetSens
+ * may be NULL (that term is left out). */
+int mpart_input_vjp(mpart_map map, mpart_cview pts, mpart_cview sens, const double* logdetSens, int64_t lstride,
+                    mpart_view out, mpart_stream stream);
+
 /* TrainOptions (MParT/TrainMap.h:19-40): NLopt-style stopping rules.  The
  * reference's opt_alg names an NLopt algorithm (default "LD_SLSQP"); here the
  * one optimizer is L-BFGS (MPART_TRAIN_LBFGS) with a strong-Wolfe line search;

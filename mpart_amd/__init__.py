@@ -105,6 +105,7 @@ _SIGS = {
     "mpart_kl_value_and_grad_plain": [_vp, _CView, _vp, _vp, _vp],
     "mpart_coeff_vjp": [_vp, _CView, _CView, _vp, _i64, _vp, _vp],
     "mpart_kl_input_grad": [_vp, _CView, _dbl, _CView, _vp],
+    "mpart_input_vjp": [_vp, _CView, _CView, _vp, _i64, _CView, _vp],
     "mpart_set_plain_launches": [_vp, _int],
     "mpart_set_codegen": [_vp, _int],
     "mpart_create_batch": [_MAPP, _u32, _MAPP],
@@ -537,6 +538,25 @@ class ConditionalMap:
         p, host = _as_dev(pts)
         out = torch.empty((self.inputDim, p.shape[1]), dtype=torch.float64, device="cuda")
         self.KLInputGradImpl(p, out, scale)
+        if host:
+            torch.cuda.current_stream().synchronize()
+            return out.cpu().numpy()
+        return out
+
+    def InputVJPImpl(self, pts, sens, logdet_sens, out):
+        """out (inputDim x N) = Gradient(sens) + logdet_sens[n] LogDeterminantInputGrad, fused
+        (sens and logdet_sens may be None)."""
+        ls_ptr = logdet_sens.data_ptr() if logdet_sens is not None else None
+        ls_stride = logdet_sens.stride(0) if logdet_sens is not None else 0
+        _check(_lib.mpart_input_vjp(self._h, _view(pts), _view(sens), ls_ptr, ls_stride, _view(out), _stream()))
+
+    def InputVJP(self, pts, sens=None, logdet_sens=None):
+        torch = _torch()
+        p, host = _as_dev(pts)
+        s = _as_dev(sens)[0] if sens is not None else None
+        ls = _as_dev(logdet_sens)[0].contiguous() if logdet_sens is not None else None
+        out = torch.empty((self.inputDim, p.shape[1]), dtype=torch.float64, device="cuda")
+        self.InputVJPImpl(p, s, ls, out)
         if host:
             torch.cuda.current_stream().synchronize()
             return out.cpu().numpy()

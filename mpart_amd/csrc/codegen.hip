@@ -318,13 +318,19 @@ extern "C" __global__ void __launch_bounds__(256) mpart_gen_ldcoef(MapDev M, con
         o << "    }\n}\n";
     }
 
-    // ---------------- KL input gradient (all components) ----------------
+    // ---------------- input VJP / KL input gradient (all components) ----------------
+    // The body is a template on KL mode (sensitivities T and -1 known at
+    // compile time), instantiated by two kernels.
     o << R"(
-extern "C" __global__ void __launch_bounds__(256) mpart_gen_klx(MapDev M, const double* __restrict__ w,
-        const double* __restrict__ pts, long prs, long pcs, long N, double scale, double* out, long ors, long ocs) {
+template <bool KL>
+__device__ __forceinline__ void ivjp_body(const MapDev& M, const double* __restrict__ w,
+        const double* __restrict__ pts, long prs, long pcs, long N, const double* sens, long srs,
+        long scs, const double* lsens, long lss, double lval, double scale, double* out, long ors, long ocs) {
     const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
     const double* pt = pts + n * pcs;
+    const bool hasL = lsens != nullptr || lval != 0.0;
+    const double lv = lsens ? lsens[n * lss] : lval;
 )";
     emit_basis_derivs(o, s, s.D - 1);
     for (unsigned d = 0; d < s.D; ++d) o << "    double gx" << d << " = 0.0;\n";
@@ -337,8 +343,10 @@ extern "C" __global__ void __launch_bounds__(256) mpart_gen_klx(MapDev M, const 
         o << "        break; }\n";
     }
     o << R"(        }
+        const double sv = sens ? sens[(long)cd.outRow * srs + n * scs] : 0.0;
         double G[GPM + 1], gk;
-        klx_core<GBK, GPK, GQK, GPM>(M, cd, C, pt[(long)(cd.dim - 1) * prs], G, gk);
+        ivjp_core<GBK, GPK, GQK, GPM, KL>(M, cd, C, pt[(long)(cd.dim - 1) * prs], sens != nullptr, sv, hasL, lv,
+                                          G, gk);
 )";
     emit_pin_derivs(o, s, s.D - 1, "        ");
     o << "        switch (c) {\n";
@@ -362,7 +370,17 @@ extern "C" __global__ void __launch_bounds__(256) mpart_gen_klx(MapDev M, const 
     }
     o << "        }\n    }\n    double* o = out + n * ocs;\n";
     for (unsigned d = 0; d < s.D; ++d) o << "    o[" << d << "L * ors] = scale * gx" << d << ";\n";
-    o << "}\n";
+    o << R"(}
+#define MPD_IVJP_ARGS MapDev M, const double* __restrict__ w, const double* __restrict__ pts, long prs, long pcs, \
+        long N, const double* sens, long srs, long scs, const double* lsens, long lss, double lval, double scale, \
+        double* out, long ors, long ocs
+extern "C" __global__ void __launch_bounds__(256) mpart_gen_ivjp(MPD_IVJP_ARGS) {
+    ivjp_body<false>(M, w, pts, prs, pcs, N, sens, srs, scs, lsens, lss, lval, scale, out, ors, ocs);
+}
+extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) mpart_gen_klx(MPD_IVJP_ARGS) {
+    ivjp_body<true>(M, w, pts, prs, pcs, N, nullptr, 0, 0, nullptr, 0, -1.0, scale, out, ors, ocs);
+}
+)";
     return o.str();
 }
 
@@ -424,6 +442,7 @@ GenModule* get_module(const GenSpec& spec, std::string& err) {
         hipModuleGetFunction(&m->eval, m->mod, "mpart_gen_eval") != hipSuccess ||
         hipModuleGetFunction(&m->coef, m->mod, "mpart_gen_coef") != hipSuccess ||
         hipModuleGetFunction(&m->ldcoef, m->mod, "mpart_gen_ldcoef") != hipSuccess ||
+        hipModuleGetFunction(&m->ivjp, m->mod, "mpart_gen_ivjp") != hipSuccess ||
         hipModuleGetFunction(&m->klx, m->mod, "mpart_gen_klx") != hipSuccess) {
         err = "hipModuleGetFunction failed";
         return nullptr;

@@ -34,7 +34,7 @@ struct GenSpec {
 
 struct GenModule {
     hipModule_t mod = nullptr;
-    hipFunction_t fused = nullptr, fused3 = nullptr, eval = nullptr, coef = nullptr, ldcoef = nullptr, klx = nullptr;
+    hipFunction_t fused = nullptr, fused3 = nullptr, eval = nullptr, coef = nullptr, ldcoef = nullptr, ivjp = nullptr, klx = nullptr;
     double compileSeconds = 0.0;
 };
 

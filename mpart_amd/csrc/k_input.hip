@@ -16,13 +16,15 @@ hipError_t launch_input_grad(int var, const Launch& L, const MapDev& M, const do
     return hipGetLastError();
 }
 
-hipError_t launch_kl_input_grad(int var, const Launch& L, const MapDev& M, const double* w, long ws, const double* pts,
-                                long prs, long pcs, long N, double scale, double* out, long ors, long ocs,
-                                double* gtab) {
+hipError_t launch_input_vjp(int var, const Launch& L, const MapDev& M, const double* w, long ws, const double* pts,
+                            long prs, long pcs, long N, int klMode, const double* sens, long srs, long scs,
+                            const double* lsens, long lss, double lval, double scale, double* out, long ors, long ocs,
+                            double* gtab) {
     PtsArg P{pts, prs, pcs, N};
     OutArg O{out, ors, ocs};
-#define CALL(B, Q, R, PM) \
-    hipLaunchKernelGGL((k_kl_input_grad<B, Q, R, PM>), L.grid, L.block, L.smem, L.stream, M, w, ws, P, scale, O, gtab)
+#define CALL(B, Q, R, PM)                                                                                          \
+    hipLaunchKernelGGL((k_input_vjp<B, Q, R, PM>), L.grid, L.block, L.smem, L.stream, M, w, ws, P, klMode, sens, \
+                       srs, scs, lsens, lss, lval, scale, O, gtab)
     MPD_DISPATCH(var, CALL)
 #undef CALL
     return hipGetLastError();

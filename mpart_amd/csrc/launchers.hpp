@@ -34,9 +34,12 @@ hipError_t launch_logdet_coeff_grad_discrete(int var, const Launch& L, const Map
 hipError_t launch_input_grad(int var, const Launch& L, const MapDev& M, const double* w, long ws, const double* pts,
                              long prs, long pcs, long N, const double* sens, long srs, long scs, double* out, long ors,
                              long ocs, int c, int mode, double* gtab);
-hipError_t launch_kl_input_grad(int var, const Launch& L, const MapDev& M, const double* w, long ws, const double* pts,
-                                long prs, long pcs, long N, double scale, double* out, long ors, long ocs,
-                                double* gtab);
+// Input VJP of all components (klMode: sensitivities T and -1 for the KL
+// objective); lsens == nullptr uses lval for every point.
+hipError_t launch_input_vjp(int var, const Launch& L, const MapDev& M, const double* w, long ws, const double* pts,
+                            long prs, long pcs, long N, int klMode, const double* sens, long srs, long scs,
+                            const double* lsens, long lss, double lval, double scale, double* out, long ors, long ocs,
+                            double* gtab);
 // Inverse of component c: a setup pass (shape L) writes per-point records to
 // rec, then the persistent solve pass (qgrid x qblock) takes points from the
 // counter at rec + records.

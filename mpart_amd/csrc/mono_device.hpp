@@ -1128,45 +1128,59 @@ MPD_HD void comp_core(const MapDev& M, const CompDev& cd, const double* C, doubl
         G[q] = s * (M.phi0[q] + M.rho[q] * A.S[q]) + (l * r) * (M.rho[q] * bq[q]);
     }
 }
-// Per-component factors of the KL input gradient at one point,
-//   d/dx [T_c^2 / 2 - log g(d_c f)] = sum_q G_q dC_q/dx_i  (i < c)  and  gk  (i = c),
-// with G_q = T_c (phi~_q(0) + rho_q S_q) - (g'/g) rho_q B'_q(x_c) and
-// gk = T_c U - (g'/g) d^2_c f (InputJacobian with sens = T_c,
-// MonotoneComponent.h:758-818, minus ContinuousMixedInputJacobian, 877-926).
-// T_c is the Evaluate value (no nugget, 287); S_q and U carry the nugget as
-// the reference's input Jacobian does.  G[0..PM] is written.
-template <int BK, int PK, int QK, int PM>
-MPD_HD void klx_core(const MapDev& M, const CompDev& cd, const double* C, double xk, double* G, double& gk) {
-    const double f0 = f_at_zero<PM>(M, cd, C);
+// Per-component factors of an input vector-Jacobian product at one point,
+//   d/dx [s T_c + l log g(d_c f)] = sum_q G_q dC_q/dx_i  (i < c)  and  gk  (i = c),
+// with G_q = s (phi~_q(0) + rho_q S_q) + l (g'/g) rho_q B'_q(x_c) and
+// gk = s U + l (g'/g) d^2_c f (InputJacobian, MonotoneComponent.h:148-164,
+// 758-818, and ContinuousMixedInputJacobian, 208-232, 877-926).  S_q and U
+// carry the nugget as the reference's input Jacobian does.  In KL mode
+// s = T_c (the Evaluate value, without the nugget, 287) and l = -1, the
+// gradient of T_c^2 / 2 - log g.  Without s (hasS false) the quadrature is
+// skipped; without l (hasL false) the log-determinant term is left out.
+// G[0..PM] is written.
+template <int BK, int PK, int QK, int PM, bool KL>
+MPD_HD void ivjp_core(const MapDev& M, const CompDev& cd, const double* C, double xk, bool hasS, double s_in,
+                      bool hasL, double l_in, double* G, double& gk) {
+    constexpr bool klMode = KL;
     DiagCoef<PM> dc;
     make_diag<PM>(M, cd, C, xk, M.nugget, dc);
     // g and g' at x_k are the integrand's values at t = 1.
     double z2;
     const double z = diag_eval<BK, true, PM>(dc, 1.0 * xk, &z2);
     const PosF<PK> gv = posfunc<PK>(z);
-    NodeV<FL_INPUT> n1;
-    n1.F = dc.x * (gv.g + dc.nug);
-    n1.h = dc.x * gv.gp;
-    n1.g1 = gv.g + dc.nug;
     AdjAcc<PM> A;
     zero_adj<PM>(A);
-    double T = integrate<BK, PK, QK, FL_INPUT, PM>(M.quad, dc, A, nullptr, &n1) + f0;
-    if (M.nugget != 0.0) {
-        DiagCoef<PM> dv = dc;
-        dv.nug = 0.0;
-        NodeV<FL_VAL> v1;
-        v1.F = dv.x * gv.g;
-        AdjAcc<PM> Av;
-        T = integrate<BK, PK, QK, FL_VAL, PM>(M.quad, dv, Av, nullptr, &v1) + f0;
+    double s = 0.0, l = hasL ? l_in : 0.0;
+    if (klMode || hasS) {
+        const double f0 = f_at_zero<PM>(M, cd, C);
+        NodeV<FL_INPUT> n1;
+        n1.F = dc.x * (gv.g + dc.nug);
+        n1.h = dc.x * gv.gp;
+        n1.g1 = gv.g + dc.nug;
+        double T = integrate<BK, PK, QK, FL_INPUT, PM>(M.quad, dc, A, nullptr, &n1) + f0;
+        if constexpr (KL) {
+            if (M.nugget != 0.0) {
+                DiagCoef<PM> dv = dc;
+                dv.nug = 0.0;
+                NodeV<FL_VAL> v1;
+                v1.F = dv.x * gv.g;
+                AdjAcc<PM> Av;
+                T = integrate<BK, PK, QK, FL_VAL, PM>(M.quad, dv, Av, nullptr, &v1) + f0;
+            }
+            s = T;
+            l = -1.0;
+        } else {
+            s = s_in;
+        }
     }
-    const double r = gv.gp * (1.0 / gv.g);
+    const double lr = (klMode || hasL) ? l * (gv.gp * (1.0 / gv.g)) : 0.0;
     double bq[PM + 1];
     diag_basis<BK, PM>(dc, xk, bq);
-    G[0] = T;
+    G[0] = s;
 #pragma unroll
     for (int q = 1; q <= PM; ++q)
-        G[q] = (q <= cd.p) ? T * (M.phi0[q] + M.rho[q] * A.S[q]) - r * (M.rho[q] * bq[q]) : 0.0;
-    gk = A.U * T - r * z2;
+        G[q] = (q <= cd.p) ? s * (M.phi0[q] + M.rho[q] * A.S[q]) + lr * (M.rho[q] * bq[q]) : 0.0;
+    gk = s * A.U + lr * z2;
 }
 // ---------------------------------------------------------------------------
 // Wavefront reduction of 16 per-lane values v[0..15] (one per term of a

@@ -744,6 +744,9 @@ void do_logdet_coeff_grad(mpart_map_s& m, const mpart_cview& p, const mpart_view
                   "launch logdet_coeff_grad");
 }
 
+void do_input_vjp(mpart_map_s& m, const mpart_cview& p, int klMode, const mpart_cview* sens, const double* lsens,
+                  long lss, double lval, double scale, const mpart_view& o, hipStream_t s, const char* nm);
+
 void do_input_grad(mpart_map_s& m, const mpart_cview& p, const mpart_cview* sens, const mpart_view& o, int mode,
                    hipStream_t s) {
     check_coeffs(&m);
@@ -754,6 +757,11 @@ void do_input_grad(mpart_map_s& m, const mpart_cview& p, const mpart_cview* sens
     if (mode == 1 && !m.opts.contDeriv)
         throw ArgError("Discrete derivative version is not implemented yet (To Do)");
     if (p.cols == 0) return;
+    if (!m.plain && gen_for(m)) {
+        // all components in one specialized launch
+        do_input_vjp(m, p, 0, sens, nullptr, 0, mode == 1 ? 1.0 : 0.0, 1.0, o, s, nm);
+        return;
+    }
     hip_check(mpd::launch_fill(o.ptr, o.rows, o.cols, o.row_stride, o.col_stride, 0.0, s), "launch fill");
     Shape sh = shape_for(m, p.cols, s, 2);
     for (int c = 0; c < m.dev.ncomp; ++c)
@@ -764,13 +772,20 @@ void do_input_grad(mpart_map_s& m, const mpart_cview& p, const mpart_cview* sens
                   "launch input_grad");
 }
 
-// KL input gradient, every component in one launch.
-void do_kl_input_grad(mpart_map_s& m, const mpart_cview& p, double scale, const mpart_view& o, hipStream_t s) {
+// Input VJP of every component in one launch: out = scale * grad_x [sum_c
+// sens_c T_c + l logdet] with l = lsens[n] (or lval when lsens is null); in KL
+// mode sens = T and l = -1.
+void do_input_vjp(mpart_map_s& m, const mpart_cview& p, int klMode, const mpart_cview* sens, const double* lsens,
+                  long lss, double lval, double scale, const mpart_view& o, hipStream_t s, const char* nm) {
     check_coeffs(&m);
-    check_view("KL input gradient: pts", p.rows, p.cols, m.inDim, p.cols);
-    check_view("KL input gradient: output", o.rows, o.cols, m.inDim, p.cols);
-    if (!m.opts.contDeriv) throw ArgError("Discrete derivative version is not implemented yet (To Do)");
+    check_view((std::string(nm) + ": pts").c_str(), p.rows, p.cols, m.inDim, p.cols);
+    if (sens) check_view((std::string(nm) + ": sens").c_str(), sens->rows, sens->cols, m.outDim, p.cols);
+    check_view((std::string(nm) + ": output").c_str(), o.rows, o.cols, m.inDim, p.cols);
+    if ((klMode || lsens || lval != 0.0) && !m.opts.contDeriv)
+        throw ArgError("Discrete derivative version is not implemented yet (To Do)");
     if (p.cols == 0) return;
+    const double* sp = sens ? sens->ptr : nullptr;
+    long srs = sens ? sens->row_stride : 0, scs = sens ? sens->col_stride : 0;
     if (mpg::GenModule* g = gen_for(m)) {
         const double* w = gen_coeffs(m, s);
         MapDev M = m.dev;
@@ -778,14 +793,15 @@ void do_kl_input_grad(mpart_map_s& m, const mpart_cview& p, double scale, const 
         long prs = p.row_stride, pcs = p.col_stride, N = p.cols;
         double* op = o.ptr;
         long ors = o.row_stride, ocs = o.col_stride;
-        void* args[] = {&M, &w, &pts, &prs, &pcs, &N, &scale, &op, &ors, &ocs};
-        gen_launch(g->klx, (N + 255) / 256, 256, 0, s, args, "launch specialized kl_input_grad");
+        void* args[] = {&M, &w, &pts, &prs, &pcs, &N, &sp, &srs, &scs, &lsens, &lss, &lval, &scale, &op, &ors, &ocs};
+        gen_launch(klMode ? g->klx : g->ivjp, (N + 255) / 256, 256, 0, s, args, "launch specialized input_vjp");
         return;
     }
     Shape sh = shape_for(m, p.cols, s, 2);
-    hip_check(mpd::launch_kl_input_grad(m.variant, sh.L, m.dev, m.coeffs, m.coeffStride, p.ptr, p.row_stride,
-                                        p.col_stride, p.cols, scale, o.ptr, o.row_stride, o.col_stride, sh.gtab),
-              "launch kl_input_grad");
+    hip_check(mpd::launch_input_vjp(m.variant, sh.L, m.dev, m.coeffs, m.coeffStride, p.ptr, p.row_stride,
+                                    p.col_stride, p.cols, klMode, sp, srs, scs, lsens, lss, lval, scale, o.ptr,
+                                    o.row_stride, o.col_stride, sh.gtab),
+              "launch input_vjp");
 }
 
 // Inverse of component c: a lane-per-point setup pass writes each point's
@@ -1667,7 +1683,16 @@ int mpart_coeff_vjp(mpart_map m, mpart_cview pts, mpart_cview sens, const double
 int mpart_kl_input_grad(mpart_map m, mpart_cview pts, double scale, mpart_view out, mpart_stream s) {
     return guarded([&] {
         check_map(m);
-        do_kl_input_grad(*m, pts, scale, out, S(s));
+        do_input_vjp(*m, pts, 1, nullptr, nullptr, 0, 0.0, scale, out, S(s), "KL input gradient");
+    });
+}
+
+int mpart_input_vjp(mpart_map m, mpart_cview pts, mpart_cview sens, const double* logdetSens, int64_t lstride,
+                    mpart_view out, mpart_stream s) {
+    return guarded([&] {
+        check_map(m);
+        do_input_vjp(*m, pts, 0, sens.ptr ? &sens : nullptr, logdetSens, (long)lstride, 0.0, 1.0, out, S(s),
+                     "InputVJP");
     });
 }
 

@@ -344,16 +344,17 @@ __global__ void __launch_bounds__(256) k_input_grad(MapDev M, const double* w, l
 }
 
 // ---------------------------------------------------------------------------
-// Input gradient of the KL objective, all components in one pass:
-//   out(:, n) = scale * grad_x [ |T(x_n)|^2 / 2 - log det grad T(x_n) ],
-// i.e. GradientImpl with sens = T (InputJacobian, MonotoneComponent.h:148-164,
-// 758-818) minus LogDeterminantInputGradImpl (ContinuousMixedInputJacobian,
-// 208-232, 877-926), summed over components, without storing T.  T is the
-// Evaluate value (no nugget, 287); the input-adjoint integral carries the
-// nugget (434-style, as the reference's Jacobians do).
+// Input vector-Jacobian product, all components in one pass:
+//   out(:, n) = scale * grad_x [ sum_c s_c T_c(x_n) + l log det grad T(x_n) ],
+// s_c = sens(c, n) (absent: 0), l = lsens[n] (absent: lval), i.e.
+// GradientImpl(sens) + l LogDeterminantInputGradImpl (InputJacobian,
+// MonotoneComponent.h:148-164, 758-818; ContinuousMixedInputJacobian,
+// 208-232, 877-926) summed over the components.  In KL mode s_c = T_c and
+// l = -1: the point gradient of |T|^2 / 2 - log det, without storing T.
 template <int BK, int PK, int QK, int PM>
-__global__ void __launch_bounds__(256) k_kl_input_grad(MapDev M, const double* w, long ws, PtsArg P, double scale,
-                                                       OutArg out, double* gtab) {
+__global__ void __launch_bounds__(256) k_input_vjp(MapDev M, const double* w, long ws, PtsArg P, int klMode,
+                                                   const double* sens, long srs, long scs, const double* lsens,
+                                                   long lss, double lval, double scale, OutArg out, double* gtab) {
     extern __shared__ double smem[];
     const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= P.N) return;
@@ -371,14 +372,20 @@ __global__ void __launch_bounds__(256) k_kl_input_grad(MapDev M, const double* w
     fill_values_derivs<BK>(M, table_dims(M, 0, M.ncomp), pt, P.rs, tab, dtab, (int)stride);
     double* o = out.p + n * out.cs;
     for (int d = 0; d < M.D; ++d) o[(long)d * out.rs] = 0.0;
+    const bool hasL = lsens != nullptr || lval != 0.0;
+    const double lv = lsens ? lsens[n * lss] : lval;
     for (int c = 0; c < M.ncomp; ++c) {
         const CompDev cd = M.comps[c];
         const int k = cd.dim - 1;
         const double* wc = w + (long)cd.coeffOff * ws;
         double C[PM + 1];
         coeff_groups<PM>(M, cd, wc, ws, tab, (int)stride, C);
+        const double sv = sens ? sens[(long)cd.outRow * srs + n * scs] : 0.0;
         double G[PM + 1], gk;
-        klx_core<BK, PK, QK, PM>(M, cd, C, pt[(long)k * P.rs], G, gk);
+        if (klMode)
+            ivjp_core<BK, PK, QK, PM, true>(M, cd, C, pt[(long)k * P.rs], false, 0.0, true, -1.0, G, gk);
+        else
+            ivjp_core<BK, PK, QK, PM, false>(M, cd, C, pt[(long)k * P.rs], sens != nullptr, sv, hasL, lv, G, gk);
         for (int i = 0; i < k; ++i) {
             double Dq[PM + 1];
             input_groups<PM>(M, cd, wc, ws, tab, dtab, (int)stride, i, Dq);

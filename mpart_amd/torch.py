@@ -5,9 +5,11 @@
 map output (bitwise equal to ``Evaluate``) and optionally the log-determinant;
 ``backward`` returns per-point input gradients and coefficient gradients summed
 over the points.  Unlike the reference (which copies to host tensors), all work
-stays on the GPU, and the coefficient gradient of both outputs is one fused
+stays on the GPU: the coefficient gradient of both outputs is one fused
 forward+adjoint launch (``ConditionalMap.CoeffVJP``) whose quadrature nodes are
-recomputed rather than stored.
+recomputed rather than stored, and the input gradient of both outputs is
+another (``ConditionalMap.InputVJPImpl``) in place of the reference's
+Gradient and LogDeterminantInputGrad calls.
 """
 import torch
 
@@ -47,12 +49,8 @@ class MpartTorchAutograd(torch.autograd.Function):
         grad = None
         if input.requires_grad:
             grad = torch.zeros((f.inputDim, x.shape[1]), dtype=torch.float64, device=x.device)
-            if sens is not None:
-                f.GradientImpl(x, sens, grad)
-            if lsens is not None:
-                g2 = torch.empty((f.inputDim, x.shape[1]), dtype=torch.float64, device=x.device)
-                f.LogDeterminantInputGradImpl(x, g2)
-                grad += g2 * lsens[None, :]
+            if sens is not None or lsens is not None:
+                f.InputVJPImpl(x, sens, lsens, grad)
             grad = grad.to(input.dtype)
 
         coeff_grad = None

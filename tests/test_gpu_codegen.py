@@ -56,10 +56,11 @@ def test_specialized_vs_generic(ci):
         y, ne = g.EvaluateCounted(x)
         res[mode] = dict(
             y=y, ne=ne, ld=g.LogDeterminant(x), cg=g.CoeffGrad(x, sens), lcg=g.LogDeterminantCoeffGrad(x),
-            kl=g.KLValueAndGrad(x), klp=g.KLValueAndGrad(x, plain=True))
+            kl=g.KLValueAndGrad(x), klp=g.KLValueAndGrad(x, plain=True), gr=g.Gradient(x, sens),
+            lig=g.LogDeterminantInputGrad(x) if CASES[ci].get("contDeriv", True) else np.zeros(1))
     a, b = res[True], res[False]
     assert np.array_equal(a["ne"], b["ne"])
-    for k in ("y", "ld", "cg", "lcg"):
+    for k in ("y", "ld", "cg", "lcg", "gr", "lig"):
         close(a[k], b[k], rtol=1e-12, what=k)
     for k in ("kl", "klp"):
         close(a[k][0], b[k][0], rtol=1e-12, what=k + " loss")
