@@ -267,7 +267,9 @@ def input_grad_workload(tri, x, steps):
     """Point gradient of the KL objective, grad_x [0.5 |T|^2 - logdet] / N, on the
     headline map and the given points: the fused launch (KLInputGradImpl)
     against the unfused sequence the reference's torch wrapper runs (Evaluate,
-    Gradient with sens = T, LogDeterminantInputGrad, then the difference)."""
+    Gradient with sens = T, LogDeterminantInputGrad, then the difference);
+    and the loss with both gradients from one pass (KLValueAndGradsImpl)
+    against the loss with the coefficient gradient only."""
     import torch
 
     D_, N = x.shape
@@ -276,8 +278,18 @@ def input_grad_workload(tri, x, steps):
     g2 = torch.empty((D_, N), dtype=torch.float64, device="cuda")
     gf = torch.empty((D_, N), dtype=torch.float64, device="cuda")
 
+    loss = torch.zeros(1, dtype=torch.float64, device="cuda")
+    grad = torch.zeros(tri.numCoeffs, dtype=torch.float64, device="cuda")
+    g1p = torch.empty((D_, N), dtype=torch.float64, device="cuda")
+
     def fused():
         tri.KLInputGradImpl(x, gf, 1.0 / N)
+
+    def coeff_only():
+        tri.KLValueAndGradImpl(x, loss, grad)
+
+    def one_pass():
+        tri.KLValueAndGradsImpl(x, loss, grad, g1p)
 
     def unfused():
         tri.EvaluateImpl(x, y)
@@ -286,7 +298,8 @@ def input_grad_workload(tri, x, steps):
         torch.sub(g1, g2, out=g1).mul_(1.0 / N)
 
     out = {"N": N}
-    for name, fn in (("fused", fused), ("unfused", unfused)):
+    for name, fn in (("fused", fused), ("unfused", unfused), ("kl_coeff_grad", coeff_only),
+                     ("kl_both_grads_one_pass", one_pass)):
         fn()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -297,6 +310,7 @@ def input_grad_workload(tri, x, steps):
         log(f"KL input gradient {name}: {out[name + '_ms']:.3f} ms")
     out["speedup"] = out["unfused_ms"] / out["fused_ms"]
     out["max_rel_diff"] = ((gf - g1).abs().max() / g1.abs().max()).item()
+    out["one_pass_max_rel_diff"] = ((g1p - g1).abs().max() / g1.abs().max()).item()
     return out
 
 

@@ -183,6 +183,21 @@ int mpart_kl_value_and_grad_plain(mpart_map map, mpart_cview pts, double* loss, 
  * storing T.  scale = 1/N gives the gradient of the mean. */
 int mpart_kl_input_grad(mpart_map map, mpart_cview pts, double scale, mpart_view out, mpart_stream stream);
 
+/* KL value, coefficient gradient and per-point input gradient in one pass:
+ * as mpart_kl_value_and_grad, and gradInputs (inputDim x N; ptr may be NULL)
+ * receives grad_x of the mean objective, (Gradient(x, T) -
+ * LogDeterminantInputGrad(x)) / N per point, computed from the same
+ * quadrature as the coefficient gradient (reference pieces as for
+ * mpart_kl_value_and_grad and mpart_kl_input_grad). */
+int mpart_kl_value_and_grads(mpart_map map, mpart_cview pts, double* loss, double* gradCoeffs,
+                             mpart_view gradInputs, mpart_stream stream);
+
+/* Both sides of the torch backward (bindings/python/package/torch_helpers.py:78-107)
+ * in one pass: grad as mpart_coeff_vjp, and gradInputs (inputDim x N; ptr may
+ * be NULL) as mpart_input_vjp, for the same sens / logdetSens. */
+int mpart_vjp(mpart_map map, mpart_cview pts, mpart_cview sens, const double* logdetSens, int64_t lstride,
+              double* grad, mpart_view gradInputs, mpart_stream stream);
+
 /* Input-side vector-Jacobian product used by the torch backward
  * (bindings/python/package/torch_helpers.py:91-107), per point and fused over
  * the components: out(:, n) = Gradient(sens)(:, n) + logdetSens[n]

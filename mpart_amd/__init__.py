@@ -106,6 +106,8 @@ _SIGS = {
     "mpart_coeff_vjp": [_vp, _CView, _CView, _vp, _i64, _vp, _vp],
     "mpart_kl_input_grad": [_vp, _CView, _dbl, _CView, _vp],
     "mpart_input_vjp": [_vp, _CView, _CView, _vp, _i64, _CView, _vp],
+    "mpart_kl_value_and_grads": [_vp, _CView, _vp, _vp, _CView, _vp],
+    "mpart_vjp": [_vp, _CView, _CView, _vp, _i64, _vp, _CView, _vp],
     "mpart_set_plain_launches": [_vp, _int],
     "mpart_set_codegen": [_vp, _int],
     "mpart_create_batch": [_MAPP, _u32, _MAPP],
@@ -527,6 +529,19 @@ class ConditionalMap:
             torch.cuda.current_stream().synchronize()
             return float(loss.item()), grad.cpu().numpy()
         return loss, grad
+
+    def KLValueAndGradsImpl(self, pts, loss, grad, grad_inputs):
+        """Loss, coefficient gradient and (grad_inputs, inputDim x N, may be None)
+        the per-point input gradient of the mean KL objective, from one pass."""
+        _check(_lib.mpart_kl_value_and_grads(self._h, _view(pts), loss.data_ptr(), grad.data_ptr(),
+                                             _view(grad_inputs), _stream()))
+
+    def VJPImpl(self, pts, sens, logdet_sens, grad, grad_inputs):
+        """CoeffVJP into grad and InputVJP into grad_inputs (may be None), from one pass."""
+        ls_ptr = logdet_sens.data_ptr() if logdet_sens is not None else None
+        ls_stride = logdet_sens.stride(0) if logdet_sens is not None else 0
+        _check(_lib.mpart_vjp(self._h, _view(pts), _view(sens), ls_ptr, ls_stride, grad.data_ptr(),
+                              _view(grad_inputs), _stream()))
 
     def KLInputGradImpl(self, pts, out, scale=1.0):
         """out (inputDim x N) = scale * grad_x [0.5 |T(x)|^2 - log det grad T(x)], per point, fused."""

@@ -114,19 +114,24 @@ std::string generate_source(const GenSpec& s) {
     // ---------------- fused forward + adjoint (KL / VJP) ----------------
     // The body is one device function; two kernels instantiate it with
     // different occupancy targets (the compiler's choice, and three waves per
-    // SIMD), between which the host picks by timing the first calls.
-    o << R"(
-__device__ __forceinline__ void fused_body(const MapDev& M, const double* __restrict__ w,
+    // SIMD), between which the host picks by timing the first calls.  A
+    // second body (fused_body_in) also forms the per-point input gradient of
+    // the same sensitivities from the same quadrature (the input-adjoint
+    // integrand adds U to the coefficient adjoint S_q).
+    for (int in = 0; in < 2; ++in) {
+        o << "\n__device__ __forceinline__ void " << (in ? "fused_body_in" : "fused_body") << R"((const MapDev& M, const double* __restrict__ w,
         const double* __restrict__ pts, long prs, long pcs, long N, const double* sens, long srs, long scs,
-        const double* lsens, long lss, int klMode, double* partial) {
+        const double* lsens, long lss, int klMode, double* partial)";
+        if (in) o << ", double gscale, double* gin, long gors, long gocs";
+        o << R"() {
     extern __shared__ double acc[];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
     const int nwaves = blockDim.x >> 6;
 )";
-    o << "    const int K = " << s.numCoeffs << ";\n";
-    o << R"(    for (int i = tid; i < nwaves * K; i += blockDim.x) acc[i] = 0.0;
+        o << "    const int K = " << s.numCoeffs << ";\n";
+        o << R"(    for (int i = tid; i < nwaves * K; i += blockDim.x) acc[i] = 0.0;
     __syncthreads();
     double* myAcc = acc + (long)wave * K;
     double lossLane = 0.0;
@@ -137,47 +142,77 @@ __device__ __forceinline__ void fused_body(const MapDev& M, const double* __rest
         const long n = active ? n0 : N - 1;
         const double* pt = pts + n * pcs;
 )";
-    emit_basis(o, s, s.D - 1);
-    o << "#pragma unroll 1\n        for (int c = 0; c < " << ncomp << "; ++c) {\n";
-    emit_pin(o, s, s.D - 1, "        ");
-    o << "        const CompDev cd = M.comps[c];\n        double C[GPM + 1];\n        switch (c) {\n";
-    for (unsigned c = 0; c < ncomp; ++c) {
-        o << "        case " << c << ": {\n";
-        emit_coeff_sums(o, s.comps[c]);
-        o << "        break; }\n";
-    }
-    o << R"(        }
+        if (in) {
+            emit_basis_derivs(o, s, s.D - 1);
+            for (unsigned d = 0; d < s.D; ++d) o << "    double gx" << d << " = 0.0;\n";
+        } else {
+            emit_basis(o, s, s.D - 1);
+        }
+        o << "#pragma unroll 1\n        for (int c = 0; c < " << ncomp << "; ++c) {\n";
+        if (in) emit_pin_derivs(o, s, s.D - 1, "        ");
+        else emit_pin(o, s, s.D - 1, "        ");
+        o << "        const CompDev cd = M.comps[c];\n        double C[GPM + 1];\n        switch (c) {\n";
+        for (unsigned c = 0; c < ncomp; ++c) {
+            o << "        case " << c << ": {\n";
+            emit_coeff_sums(o, s.comps[c]);
+            o << "        break; }\n";
+        }
+        o << R"(        }
         const double xk = pt[(long)(cd.dim - 1) * prs];
         const double sv = (!klMode && sens) ? sens[(long)cd.outRow * srs + n * scs] : 0.0;
         const double lv = (!klMode && lsens) ? lsens[n * lss] : 0.0;
         double G[GPM + 1];
-        comp_core<GBK, GPK, GQK, GPM>(M, cd, C, xk, active, klMode, sv, lv, G, lossLane);
 )";
-    emit_pin(o, s, s.D - 1, "        ");
-    o << "        switch (c) {\n";
-    for (unsigned c = 0; c < ncomp; ++c) {
-        const GenComp& gc = s.comps[c];
-        const unsigned nt = unsigned(gc.starts.size()) - 1;
-        o << "        case " << c << ": {\n";
-        for (unsigned t0 = 0; t0 < nt; t0 += 16) {
-            const unsigned nv = std::min(16u, nt - t0);
-            o << "            { double v[16];\n";
-            for (unsigned u = 0; u < 16; ++u) {
-                if (u < nv) {
-                    unsigned q;
-                    std::string P = term_product(gc, t0 + u, &q);
-                    o << "              v[" << u << "] = G[" << q << "] * " << P << ";\n";
-                } else {
-                    o << "              v[" << u << "] = 0.0;\n";
+        if (in)
+            o << "        double gk;\n        comp_core<GBK, GPK, GQK, GPM, true>(M, cd, C, xk, active, klMode, sv, lv, G, "
+                 "lossLane, &gk);\n";
+        else
+            o << "        comp_core<GBK, GPK, GQK, GPM>(M, cd, C, xk, active, klMode, sv, lv, G, lossLane);\n";
+        if (in) emit_pin_derivs(o, s, s.D - 1, "        ");
+        else emit_pin(o, s, s.D - 1, "        ");
+        o << "        switch (c) {\n";
+        for (unsigned c = 0; c < ncomp; ++c) {
+            const GenComp& gc = s.comps[c];
+            const unsigned nt = unsigned(gc.starts.size()) - 1;
+            o << "        case " << c << ": {\n";
+            for (unsigned t0 = 0; t0 < nt; t0 += 16) {
+                const unsigned nv = std::min(16u, nt - t0);
+                o << "            { double v[16];\n";
+                for (unsigned u = 0; u < 16; ++u) {
+                    if (u < nv) {
+                        unsigned q;
+                        std::string P = term_product(gc, t0 + u, &q);
+                        o << "              v[" << u << "] = G[" << q << "] * " << P << ";\n";
+                    } else {
+                        o << "              v[" << u << "] = 0.0;\n";
+                    }
                 }
+                o << "              reduce16_add(v, lane, myAcc + " << gc.coeffOff + t0 << ", " << nv << "); }\n";
             }
-            o << "              reduce16_add(v, lane, myAcc + " << gc.coeffOff + t0 << ", " << nv << "); }\n";
+            if (in) {
+                for (unsigned t = 0; t < nt; ++t) {
+                    unsigned q;
+                    term_product(gc, t, &q);
+                    bool any = false;
+                    for (unsigned j = gc.starts[t]; j < gc.starts[t + 1]; ++j) {
+                        if (gc.dims[j] == gc.dim - 1) continue;
+                        if (!any) o << "            { const double a = w[" << gc.coeffOff + t << "] * G[" << q << "];\n";
+                        any = true;
+                        o << "              gx" << gc.dims[j] << " += a * " << term_deriv(gc, t, j) << ";\n";
+                    }
+                    if (any) o << "            }\n";
+                }
+                o << "            gx" << gc.dim - 1 << " += gk;\n";
+            }
+            o << "        break; }\n";
         }
-        o << "        break; }\n";
-    }
-    o << R"(        }
+        o << "        }\n        }\n";
+        if (in) {
+            o << "        if (active) {\n            double* go = gin + n * gocs;\n";
+            for (unsigned d = 0; d < s.D; ++d) o << "            go[" << d << "L * gors] = gscale * gx" << d << ";\n";
+            o << "        }\n";
         }
-    }
+        o << R"(    }
     const double lw = wave_sum(lossLane);
     __shared__ double lossWave[4];
     if (lane == 0) lossWave[wave] = lw;
@@ -195,7 +230,9 @@ __device__ __forceinline__ void fused_body(const MapDev& M, const double* __rest
         row[k] = t;
     }
 }
-#define MPD_FUSED_ARGS MapDev M, const double* __restrict__ w, const double* __restrict__ pts, long prs, long pcs, \
+)";
+    }
+    o << R"(#define MPD_FUSED_ARGS MapDev M, const double* __restrict__ w, const double* __restrict__ pts, long prs, long pcs, \
         long N, const double* sens, long srs, long scs, const double* lsens, long lss, int klMode, double* partial
 extern "C" __global__ void __launch_bounds__(256) mpart_gen_fused(MPD_FUSED_ARGS) {
     fused_body(M, w, pts, prs, pcs, N, sens, srs, scs, lsens, lss, klMode, partial);
@@ -203,6 +240,10 @@ extern "C" __global__ void __launch_bounds__(256) mpart_gen_fused(MPD_FUSED_ARGS
 extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 mpart_gen_fused_w3(MPD_FUSED_ARGS) {
     fused_body(M, w, pts, prs, pcs, N, sens, srs, scs, lsens, lss, klMode, partial);
+}
+extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) mpart_gen_fused_in(MPD_FUSED_ARGS, double gscale, double* gin,
+                                                                     long gors, long gocs) {
+    fused_body_in(M, w, pts, prs, pcs, N, sens, srs, scs, lsens, lss, klMode, partial, gscale, gin, gors, gocs);
 }
 )";
 
@@ -443,7 +484,8 @@ GenModule* get_module(const GenSpec& spec, std::string& err) {
         hipModuleGetFunction(&m->coef, m->mod, "mpart_gen_coef") != hipSuccess ||
         hipModuleGetFunction(&m->ldcoef, m->mod, "mpart_gen_ldcoef") != hipSuccess ||
         hipModuleGetFunction(&m->ivjp, m->mod, "mpart_gen_ivjp") != hipSuccess ||
-        hipModuleGetFunction(&m->klx, m->mod, "mpart_gen_klx") != hipSuccess) {
+        hipModuleGetFunction(&m->klx, m->mod, "mpart_gen_klx") != hipSuccess ||
+        hipModuleGetFunction(&m->fusedIn, m->mod, "mpart_gen_fused_in") != hipSuccess) {
         err = "hipModuleGetFunction failed";
         return nullptr;
     }

@@ -34,7 +34,8 @@ struct GenSpec {
 
 struct GenModule {
     hipModule_t mod = nullptr;
-    hipFunction_t fused = nullptr, fused3 = nullptr, eval = nullptr, coef = nullptr, ldcoef = nullptr, ivjp = nullptr, klx = nullptr;
+    hipFunction_t fused = nullptr, fused3 = nullptr, eval = nullptr, coef = nullptr, ldcoef = nullptr, ivjp = nullptr, klx = nullptr,
+                  fusedIn = nullptr;
     double compileSeconds = 0.0;
 };
 

@@ -1084,24 +1084,31 @@ MPD_HD void zero_adj(AdjAcc<PM>& A) {
 // with s = T_c and l = -1 in KL mode (then loss += 0.5 T_c^2 - logdet), or the
 // given sensitivities otherwise.  G[0..p] is written; G is zero for an
 // inactive lane.
-template <int BK, int PK, int QK, int PM>
+// With IN, the integrand also carries the input adjoint U and gk receives
+// the x_c entry of the input gradient, s U + l (g'/g) d^2_c f (the other
+// entries are G_q dC_q/dx_i, as for the input VJP below).
+template <int BK, int PK, int QK, int PM, bool IN = false>
 MPD_HD void comp_core(const MapDev& M, const CompDev& cd, const double* C, double xk, bool active, int klMode,
-                      double s_in, double l_in, double* G, double& loss) {
+                      double s_in, double l_in, double* G, double& loss, double* gk = nullptr) {
+    constexpr int FLV = IN ? FL_INPUT : FL_COEF;
 #pragma unroll
     for (int q = 0; q <= PM; ++q) G[q] = 0.0;
+    if constexpr (IN) *gk = 0.0;
     if (!active) return;
     const double f0 = f_at_zero<PM>(M, cd, C);
     DiagCoef<PM> dc;
     make_diag<PM>(M, cd, C, xk, M.nugget, dc);
     // g and g' at x_k (log-determinant) are the integrand's values at t = 1.
-    const double z = diag_eval<BK, false, PM>(dc, 1.0 * xk, nullptr);
+    double z2 = 0.0;
+    const double z = diag_eval<BK, IN, PM>(dc, 1.0 * xk, IN ? &z2 : nullptr);
     const PosF<PK> gv = posfunc<PK>(z);
-    NodeV<FL_COEF> n1;
+    NodeV<FLV> n1;
     n1.F = dc.x * (gv.g + dc.nug);
     n1.h = dc.x * gv.gp;
+    if constexpr (IN) n1.g1 = gv.g + dc.nug;
     AdjAcc<PM> A;
     zero_adj<PM>(A);
-    double T = integrate<BK, PK, QK, FL_COEF, PM>(M.quad, dc, A, nullptr, &n1) + f0;
+    double T = integrate<BK, PK, QK, FLV, PM>(M.quad, dc, A, nullptr, &n1) + f0;
     if (M.nugget != 0.0) {
         // Evaluate drops the nugget (MonotoneComponent.h:287): its tree may differ.
         DiagCoef<PM> dv = dc;
@@ -1127,6 +1134,7 @@ MPD_HD void comp_core(const MapDev& M, const CompDev& cd, const double* C, doubl
         if (q > cd.p) break;
         G[q] = s * (M.phi0[q] + M.rho[q] * A.S[q]) + (l * r) * (M.rho[q] * bq[q]);
     }
+    if constexpr (IN) *gk = s * A.U + (l * r) * z2;
 }
 // Per-component factors of an input vector-Jacobian product at one point,
 //   d/dx [s T_c + l log g(d_c f)] = sum_q G_q dC_q/dx_i  (i < c)  and  gk  (i = c),

@@ -212,6 +212,7 @@ struct mpart_map_s {
     double genSeconds = 0.0;
     std::string genErr;
     int genOccK[2] = {0, 0};
+    int genOccIn = 0;  // resident blocks per CU of the fused kernel with point gradients
     size_t genOccSmem = 0;
     bool genW3Spills = false;  // the three-waves kernel needs more private memory (register spills)
     int fusedChoice = -1;  // tuned schedule of the fused kernel (-1: not yet)
@@ -870,9 +871,18 @@ void do_inverse(mpart_map_s& m, const mpart_cview& x1, const mpart_cview& r, con
 
 // Launch shape of the fused kernel: persistent grid, LDS accumulators when
 // they fit, otherwise global atomics.
+void do_input_vjp(mpart_map_s& m, const mpart_cview& p, int klMode, const mpart_cview* sens, const double* lsens,
+                  long lss, double lval, double scale, const mpart_view& o, hipStream_t s, const char* nm);
+
+// Fused forward + adjoint: the coefficient gradient (and in KL mode the loss)
+// summed over the points, scaled by `scale`; with gin, also the per-point
+// input gradient of the same sensitivities times gscale, from the same
+// quadrature (specialized kernels) or by a second launch (generic kernels).
 void do_fused(mpart_map_s& m, const mpart_cview& p, const double* sens, long srs, long scs, const double* lsens,
-              long lss, int klMode, double* loss, double* grad, double scale, hipStream_t s) {
+              long lss, int klMode, double* loss, double* grad, double scale, hipStream_t s,
+              const mpart_view* gin = nullptr, double gscale = 1.0) {
     check_coeffs(&m);
+    if (gin) check_view("input gradient output", gin->rows, gin->cols, m.inDim, p.cols);
     const long N = p.cols;
     const long K = m.numCoeffs;
     if (g_numCU == 0) {
@@ -924,6 +934,31 @@ void do_fused(mpart_map_s& m, const mpart_cview& p, const double* sens, long srs
         auto launch_cfg = [&](int cfg) {
             gen_launch(fns[cfg & 1], grid_of(cfg), gblock, gsmem, s, args, "launch specialized fused");
         };
+        if (gin) {
+            // One kernel for both gradients; persistent grid of two blocks per
+            // resident slot (the schedule the tuner picks for the fused kernel).
+            if (m.genOccIn == 0) {
+                int nb = 0;
+                hip_check(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nb, g->fusedIn, gblock, gsmem),
+                          "hipModuleOccupancyMaxActiveBlocksPerMultiprocessor");
+                m.genOccIn = std::max(1, nb);
+            }
+            const long gridIn = std::min<long>(ntiles, (long)g_numCU * m.genOccIn * 2);
+            m.partialGrad.ensure((size_t)(std::max(maxGrid, gridIn) + mpd::COLSUM_MAX_CHUNKS) * (K + 1) *
+                                 sizeof(double));
+            part = m.partialGrad.as<double>();
+            double gs = gscale;
+            double* gp = gin->ptr;
+            long gors = gin->row_stride, gocs = gin->col_stride;
+            void* argsIn[] = {&M, &w, &pts, &prs, &pcs, &NN, &sp, &srs_, &scs_, &lp, &lss_, &kl, &part,
+                              &gs, &gp, &gors, &gocs};
+            const long rows = (N > 0) ? gridIn : 0;
+            if (N > 0) gen_launch(g->fusedIn, gridIn, gblock, gsmem, s, argsIn, "launch specialized fused (inputs)");
+            hip_check(mpd::launch_colsum(part, rows, K + 1, loss ? K + 1 : K, scale, grad, loss,
+                                         part + std::max(maxGrid, gridIn) * (K + 1), s),
+                      "launch reduce");
+            return;
+        }
         // Autotuning: on the first call with enough points (and not under
         // stream capture) every schedule runs twice and the faster one is kept.
         // The three-waves kernel is left out when its register cap makes it
@@ -1001,6 +1036,10 @@ void do_fused(mpart_map_s& m, const mpart_cview& p, const double* sens, long srs
     if (loss)
         hip_check(mpd::launch_reduce_rows(m.partialLoss.as<double>(), N > 0 ? grid : 0, 1, scale, loss, s),
                   "launch reduce");
+    if (gin) {
+        mpart_cview sv{sens, m.outDim, N, srs, scs};
+        do_input_vjp(m, p, klMode, sens ? &sv : nullptr, lsens, lss, 0.0, gscale, *gin, s, "input gradient");
+    }
 }
 
 // The KL objective with the reference's launch structure
@@ -1677,6 +1716,35 @@ int mpart_coeff_vjp(mpart_map m, mpart_cview pts, mpart_cview sens, const double
         if (sens.ptr) check_view("vjp: sens", sens.rows, sens.cols, m->outDim, pts.cols);
         if (!m->opts.contDeriv) throw NotImpl("contDeriv = false is not provided yet");
         do_fused(*m, pts, sens.ptr, sens.row_stride, sens.col_stride, lsens, lss, 0, nullptr, grad, 1.0, S(s));
+    });
+}
+
+int mpart_kl_value_and_grads(mpart_map m, mpart_cview pts, double* loss, double* grad, mpart_view gradInputs,
+                             mpart_stream s) {
+    return guarded([&] {
+        check_map(m);
+        check_view("KL: pts", pts.rows, pts.cols, m->inDim, pts.cols);
+        if (pts.cols == 0) throw ArgError("KL: no points");
+        if (!gradInputs.ptr) {
+            if (!m->opts.contDeriv) do_kl_plain(*m, pts, loss, grad, S(s));
+            else do_fused(*m, pts, nullptr, 0, 0, nullptr, 0, 1, loss, grad, 1.0 / (double)pts.cols, S(s));
+            return;
+        }
+        if (!m->opts.contDeriv) throw ArgError("Discrete derivative version is not implemented yet (To Do)");
+        do_fused(*m, pts, nullptr, 0, 0, nullptr, 0, 1, loss, grad, 1.0 / (double)pts.cols, S(s), &gradInputs,
+                 1.0 / (double)pts.cols);
+    });
+}
+
+int mpart_vjp(mpart_map m, mpart_cview pts, mpart_cview sens, const double* lsens, int64_t lss, double* grad,
+              mpart_view gradInputs, mpart_stream s) {
+    return guarded([&] {
+        check_map(m);
+        check_view("vjp: pts", pts.rows, pts.cols, m->inDim, pts.cols);
+        if (sens.ptr) check_view("vjp: sens", sens.rows, sens.cols, m->outDim, pts.cols);
+        if (!m->opts.contDeriv) throw NotImpl("contDeriv = false is not provided yet");
+        do_fused(*m, pts, sens.ptr, sens.row_stride, sens.col_stride, lsens, lss, 0, nullptr, grad, 1.0, S(s),
+                 gradInputs.ptr ? &gradInputs : nullptr, 1.0);
     });
 }
 

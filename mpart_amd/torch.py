@@ -46,16 +46,23 @@ class MpartTorchAutograd(torch.autograd.Function):
         sens = output_sens.to(torch.float64) if output_sens is not None else None
         lsens = logdet_sens.to(torch.float64).contiguous() if logdet_sens is not None else None
 
+        want_c = coeffs is not None and coeffs.requires_grad
         grad = None
         if input.requires_grad:
             grad = torch.zeros((f.inputDim, x.shape[1]), dtype=torch.float64, device=x.device)
-            if sens is not None or lsens is not None:
-                f.InputVJPImpl(x, sens, lsens, grad)
-            grad = grad.to(input.dtype)
-
         coeff_grad = None
-        if coeffs is not None and coeffs.requires_grad:
-            coeff_grad = f.CoeffVJP(x, sens, lsens).to(coeffs.dtype)
+        if want_c:
+            coeff_grad = torch.zeros(f.numCoeffs, dtype=torch.float64, device=x.device)
+        if sens is not None or lsens is not None:
+            if want_c:
+                # both gradients from one fused pass
+                f.VJPImpl(x, sens, lsens, coeff_grad, grad)
+            elif grad is not None:
+                f.InputVJPImpl(x, sens, lsens, grad)
+        if grad is not None:
+            grad = grad.to(input.dtype)
+        if coeff_grad is not None:
+            coeff_grad = coeff_grad.to(coeffs.dtype)
         return grad, coeff_grad, None, None
 
 
@@ -139,13 +146,13 @@ def kl_loss(f, x, coeffs):
 class KLObjective(torch.autograd.Function):
     """The KL objective mean(0.5 |T(x)|^2 - log det grad T(x)) of
     FastGaussianReverseKLObjective (src/MapObjective.cpp:146-234) as one autograd
-    node: forward runs the fused forward+adjoint launch
-    (ConditionalMap.KLValueAndGradImpl) and keeps the coefficient gradient;
-    when the points require a gradient it also runs the fused input-adjoint
-    launch (ConditionalMap.KLInputGradImpl), the per-point gradient
+    node: forward runs the fused forward+adjoint pass
+    (ConditionalMap.KLValueAndGradsImpl) and keeps the coefficient gradient
+    and, when the points require a gradient, the per-point gradient
     (Gradient(x, T) - LogDeterminantInputGrad(x)) / N that
-    MpartTorchAutograd's backward would form from two launches and T.
-    backward scales both by the incoming sensitivity."""
+    MpartTorchAutograd's backward would form from separate calls and T; both
+    come from the same quadrature.  backward scales them by the incoming
+    sensitivity."""
 
     @staticmethod
     def forward(ctx, x, coeffs, f):
@@ -154,11 +161,11 @@ class KLObjective(torch.autograd.Function):
         xd = x.detach().to(torch.float64)
         loss = torch.zeros(1, dtype=torch.float64, device=x.device)
         grad = torch.zeros(f.numCoeffs, dtype=torch.float64, device=x.device)
-        f.KLValueAndGradImpl(xd, loss, grad)
         xgrad = None
         if ctx.needs_input_grad[0]:
             xgrad = torch.empty((f.inputDim, xd.shape[1]), dtype=torch.float64, device=x.device)
-            f.KLInputGradImpl(xd, xgrad, 1.0 / xd.shape[1])
+        # one pass for the loss and both gradients
+        f.KLValueAndGradsImpl(xd, loss, grad, xgrad)
         ctx.save_for_backward(grad, xgrad)
         ctx.x_dtype = x.dtype
         return loss[0].to(coeffs.dtype)

@@ -183,3 +183,62 @@ def test_torch_backward_input_gradient_is_one_fused_call():
     r.SetCoeffs(c)
     ref = r.Gradient(xn, w.cpu().numpy()) + v.cpu().numpy()[None, :] * r.LogDeterminantInputGrad(xn)
     _close(x.grad.cpu().numpy(), ref)
+
+
+@pytest.mark.parametrize("ci", [0, 1, 2, 6, 10, 12])
+def test_one_pass_value_and_both_gradients(ci):
+    # KLValueAndGradsImpl / VJPImpl (one specialized kernel, or the generic
+    # fused kernel followed by the input VJP) against the separate entry points.
+    _case("kl_value_and_grads", case=CASES[ci])
+    s = Stream(1800 + ci)
+    g = mt.CreateTriangular(5, 4, 3, gpu_opts(**CASES[ci]))
+    g.SetCoeffs(0.1 * s.normal(g.numCoeffs))
+    n = 3000
+    x = torch.from_numpy(s.points(5, n)).cuda()
+    sens = torch.from_numpy(s.normal(4 * n).reshape(4, n)).cuda()
+    ls = torch.from_numpy(s.normal(n)).cuda()
+    for codegen in (True, False):
+        g.SetCodegen(codegen)
+        loss0, grad0 = g.KLValueAndGrad(x)
+        gx0 = g.KLInputGrad(x, scale=1.0 / n)
+        loss = torch.zeros(1, dtype=torch.float64, device="cuda")
+        grad = torch.zeros(g.numCoeffs, dtype=torch.float64, device="cuda")
+        gx = torch.full((5, n), 7.0, dtype=torch.float64, device="cuda")
+        g.KLValueAndGradsImpl(x, loss, grad, gx)
+        _close(loss.cpu().numpy(), loss0.cpu().numpy(), rel=1e-13)
+        _close(grad.cpu().numpy(), grad0.cpu().numpy(), rel=1e-13)
+        _close(gx.cpu().numpy(), gx0.cpu().numpy(), rel=1e-13)
+        # without the input output it is the plain fused pass
+        g.KLValueAndGradsImpl(x, loss, grad, None)
+        _close(grad.cpu().numpy(), grad0.cpu().numpy(), rel=1e-13)
+        # VJP: both sides of the torch backward
+        for sv, lv in ((sens, ls), (sens, None), (None, ls)):
+            cg0 = g.CoeffVJP(x, sv, lv)
+            gi0 = g.InputVJP(x, sv, lv)
+            cg = torch.zeros(g.numCoeffs, dtype=torch.float64, device="cuda")
+            gi = torch.full((5, n), 7.0, dtype=torch.float64, device="cuda")
+            g.VJPImpl(x, sv, lv, cg, gi)
+            _close(cg.cpu().numpy(), cg0.cpu().numpy(), rel=1e-13)
+            _close(gi.cpu().numpy(), gi0.cpu().numpy(), rel=1e-13)
+
+
+def test_one_pass_large_batch_and_graph_free_reuse():
+    # Enough points for a persistent grid that loops over many tiles, a
+    # point-major strided view, and a second call reusing the buffers.
+    _case("kl_value_and_grads_large")
+    s = Stream(1900)
+    g = mt.CreateTriangular(8, 8, 3, mt.MapOptions())
+    g.SetCoeffs(0.1 * s.normal(g.numCoeffs))
+    n = 300_000
+    xpm = torch.from_numpy(np.ascontiguousarray(s.points(8, n).T)).cuda()
+    x = xpm.T
+    gx0 = g.KLInputGrad(x, scale=1.0 / n)
+    loss0, grad0 = g.KLValueAndGrad(x)
+    loss = torch.zeros(1, dtype=torch.float64, device="cuda")
+    grad = torch.zeros(g.numCoeffs, dtype=torch.float64, device="cuda")
+    gxt = torch.zeros((n, 8), dtype=torch.float64, device="cuda")
+    for _ in range(2):
+        g.KLValueAndGradsImpl(x, loss, grad, gxt.T)
+        _close(gxt.T.cpu().numpy(), gx0.cpu().numpy(), rel=1e-13)
+        _close(grad.cpu().numpy(), grad0.cpu().numpy(), rel=1e-13)
+        _close(loss.cpu().numpy(), loss0.cpu().numpy(), rel=1e-13)
