@@ -118,6 +118,8 @@ _SIGS = {
     "mpart_batch_inverse": [_vp, _vp, _vp, _vp, _dbl, _dbl, _vp],
     "mpart_codegen_prepare": [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)],
     "mpart_fused_schedule": [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)],
+    "mpart_set_point_group": [_vp, ctypes.c_int32],
+    "mpart_eval_schedule": [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)],
     "mpart_train_map": [_vp, _CView, ctypes.POINTER(_CTrainOptions), ctypes.POINTER(ctypes.c_double),
                         ctypes.POINTER(ctypes.c_int32), _vp],
 }
@@ -424,6 +426,18 @@ class ConditionalMap:
         secs = ctypes.c_double(0.0)
         _check(_lib.mpart_codegen_prepare(self._h, ctypes.byref(spec), ctypes.byref(secs)))
         return bool(spec.value), float(secs.value), _lib.mpart_codegen_error(self._h).decode()
+
+    def SetPointGroup(self, lanes=0):
+        """Lanes per point of the generic Evaluate / LogDeterminant / fused pass (1, 8, 64; 0 = autotune)."""
+        _check(_lib.mpart_set_point_group(self._h, int(lanes)))
+
+    def EvalSchedule(self):
+        """((lanes for Evaluate, LogDeterminant, fused pass), trial ms for lanes 1, 8, 64 of each)
+        of the generic kernels (0: not tuned)."""
+        lanes = (ctypes.c_int32 * 3)()
+        ms = (ctypes.c_double * 9)()
+        _check(_lib.mpart_eval_schedule(self._h, lanes, ms))
+        return (lanes[0], lanes[1], lanes[2]), [list(ms[0:3]), list(ms[3:6]), list(ms[6:9])]
 
     def FusedSchedule(self):
         """(schedule index or -1 before tuning, trial times in ms) of the fused kernel."""

@@ -19,6 +19,17 @@ struct Launch {
 hipError_t launch_eval(int var, const Launch& L, const MapDev& M, const double* w, long ws, const double* pts, long prs,
                        long pcs, long N, double* out, long ors, long ocs, double* ld, long lds, int ldAcc, int* nevals,
                        int cb, int ce, double* gtab);
+// Group-per-point Evaluate / LogDeterminant (G = 8 or 64 lanes per point;
+// L.smem = block / G tables of nslots doubles).
+hipError_t launch_eval_grp(int var, int G, const Launch& L, const MapDev& M, const double* w, long ws,
+                           const double* pts, long prs, long pcs, long N, double* out, long ors, long ocs, double* ld,
+                           long lds, int ldAcc, int* nevals, int cb, int ce);
+// Group-per-point fused pass: one partial row of K + 1 per wavefront of the
+// grid (rows = grid * block / 64).
+hipError_t launch_fused_grp(int var, int G, const Launch& L, const MapDev& M, const double* w, long ws,
+                            const double* pts, long prs, long pcs, long N, const double* sens, long srs, long scs,
+                            const double* lsens, long lss, int klMode, double* rows);
+int occupancy_fused_grp(int var, int G, int block, size_t smem);
 hipError_t launch_coeff_grad(int var, const Launch& L, const MapDev& M, const double* w, long ws, const double* pts,
                              long prs, long pcs, long N, const double* sens, long srs, long scs, double* out, long ors,
                              long ocs, int c, double* gtab);

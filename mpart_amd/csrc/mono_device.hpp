@@ -486,53 +486,57 @@ MPD_HD bool lin_bound(const MapDev& M, double x, double& c) {
 // lane lives at tab[slot * stride].
 //
 // Values: EvaluateAll (OrthogonalPolynomial.h:25-42, HermiteFunction.h:22-43).
+// Values of orders 1..deg of one coordinate into out[o * stride].
+template <int BK>
+MPD_HD void values_1d(const MapDev& M, int deg, double x, double* out, long stride) {
+    double c;
+    if (lin_bound(M, x, c)) {
+        // LinearizedBasis::EvaluateAll (LinearizedBasis.h:41-61)
+        for (int o = 1; o <= deg; ++o) {
+            double v, dv;
+            basis_vd<BK>(M, o, c, v, dv);
+            out[o * stride] = v + dv * (x - c);
+        }
+        return;
+    }
+    if constexpr (BK != BK_HF) {
+        constexpr double A = poly_a<BK>();
+        double vm2 = 1.0, vm1 = A * x;
+        for (int o = 1; o <= deg; ++o) {
+            double v = (o == 1) ? A * x : (A * x + 0.0) * vm1 - (A * (o - 1.0)) * vm2;
+            if (o >= 2) {
+                vm2 = vm1;
+                vm1 = v;
+            }
+            out[o * stride] = M.normalize ? v / M.norm[o] : v;
+        }
+    } else {
+        // phi_1 = x, phi_2 = pi^-1/4 exp(-x^2/2), phi_3 = sqrt(2) x phi_2,
+        // phi_{i+2} = (x phi_{i+1} - sqrt((i-1)/2) phi_i) / sqrt(i/2)
+        double pm1 = 0.0, pm2 = 0.0;
+        for (int o = 1; o <= deg; ++o) {
+            double v;
+            if (o == 1) v = x;
+            else if (o == 2) v = M.hfPi * exp(-0.5 * x * x);
+            else if (o == 3) v = M.hfSqrt2 * x * pm1;
+            else {
+                int i = o - 2;
+                v = (x * pm1 - M.hfA[i] * pm2) / M.hfB[i];
+            }
+            pm2 = pm1;
+            pm1 = v;
+            out[o * stride] = v;
+        }
+    }
+}
+
 template <int BK>
 __host__ __device__ void fill_values(const MapDev& M, int ndims, const double* pt, long prs, double* tab, int stride) {
     tab[0] = 1.0;
     for (int d = 0; d < ndims; ++d) {
         const int deg = M.dimDeg[d];
         if (deg == 0) continue;
-        const double x = pt[d * prs];
-        double* out = tab + (long)M.dimSlot[d] * stride - stride;  // out[o*stride], o >= 1
-        double c;
-        if (lin_bound(M, x, c)) {
-            // LinearizedBasis::EvaluateAll (LinearizedBasis.h:41-61)
-            for (int o = 1; o <= deg; ++o) {
-                double v, dv;
-                basis_vd<BK>(M, o, c, v, dv);
-                out[o * stride] = v + dv * (x - c);
-            }
-            continue;
-        }
-        if constexpr (BK != BK_HF) {
-            constexpr double A = poly_a<BK>();
-            double vm2 = 1.0, vm1 = A * x;
-            for (int o = 1; o <= deg; ++o) {
-                double v = (o == 1) ? A * x : (A * x + 0.0) * vm1 - (A * (o - 1.0)) * vm2;
-                if (o >= 2) {
-                    vm2 = vm1;
-                    vm1 = v;
-                }
-                out[o * stride] = M.normalize ? v / M.norm[o] : v;
-            }
-        } else {
-            // phi_1 = x, phi_2 = pi^-1/4 exp(-x^2/2), phi_3 = sqrt(2) x phi_2,
-            // phi_{i+2} = (x phi_{i+1} - sqrt((i-1)/2) phi_i) / sqrt(i/2)
-            double pm1 = 0.0, pm2 = 0.0;
-            for (int o = 1; o <= deg; ++o) {
-                double v;
-                if (o == 1) v = x;
-                else if (o == 2) v = M.hfPi * exp(-0.5 * x * x);
-                else if (o == 3) v = M.hfSqrt2 * x * pm1;
-                else {
-                    int i = o - 2;
-                    v = (x * pm1 - M.hfA[i] * pm2) / M.hfB[i];
-                }
-                pm2 = pm1;
-                pm1 = v;
-                out[o * stride] = v;
-            }
-        }
+        values_1d<BK>(M, deg, pt[d * prs], tab + (long)M.dimSlot[d] * stride - stride, stride);
     }
 }
 

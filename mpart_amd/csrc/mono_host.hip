@@ -216,6 +216,14 @@ struct mpart_map_s {
     size_t genOccSmem = 0;
     bool genW3Spills = false;  // the three-waves kernel needs more private memory (register spills)
     int fusedChoice = -1;  // tuned schedule of the fused kernel (-1: not yet)
+    // Point schedule of the generic Evaluate / LogDeterminant: lanes per point
+    // (1, 8 or 64) for [Evaluate (with or without log-det), log-det only];
+    // 0: not tuned yet.  pointGroupForced > 0 overrides the tuner.
+    int evalGroup[2] = {0, 0};
+    double evalTrialMs[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    int pointGroupForced = 0;
+    int fusedGroup = 0;  // the same for the generic fused pass
+    double fusedGroupTrialMs[3] = {0, 0, 0};
     double fusedTrialMs[4] = {0, 0, 0, 0};
 };
 
@@ -606,6 +614,68 @@ void gen_eval(mpart_map_s& m, mpg::GenModule* g, const double* w, const mpart_cv
     gen_launch(g->eval, (N + 255) / 256, 256, 0, s, args, "launch specialized evaluate");
 }
 
+constexpr long EVAL_TUNE_MIN_POINTS = 16384;
+
+// Generic Evaluate and/or LogDeterminant of components [cb, ce) with the
+// map's point schedule: a lane per point (k_eval) or a group of 8 or 64 lanes
+// per point (k_eval_grp).  The first call with enough points (outputs
+// overwritten, not under stream capture) times the three schedules with HIP
+// events and keeps the fastest for this kind of call.
+void generic_eval(mpart_map_s& m, const mpart_cview& p, double* out, long ors, long ocs, double* ld, long lds,
+                  int ldAcc, int32_t* nevals, int cb, int ce, hipStream_t s) {
+    const long N = p.cols;
+    auto run = [&](int G) {
+        if (G > 1) {
+            const size_t tabBytes = (size_t)m.dev.nslots * sizeof(double);
+            int block = 256;
+            while (block > G && (size_t)(block / G) * tabBytes > LDS_LIMIT) block /= 2;
+            if ((size_t)(block / G) * tabBytes <= LDS_LIMIT) {
+                const long per = block / G;
+                mpd::Launch L{dim3((unsigned)((N + per - 1) / per)), dim3(block), (size_t)per * tabBytes, s};
+                hip_check(mpd::launch_eval_grp(m.variant, G, L, m.dev, m.coeffs, m.coeffStride, p.ptr, p.row_stride,
+                                               p.col_stride, N, out, ors, ocs, ld, lds, ldAcc, nevals, cb, ce),
+                          "launch evaluate (group per point)");
+                return;
+            }
+        }
+        Shape sh = shape_for(m, N, s);
+        hip_check(mpd::launch_eval(m.variant, sh.L, m.dev, m.coeffs, m.coeffStride, p.ptr, p.row_stride,
+                                   p.col_stride, N, out, ors, ocs, ld, lds, ldAcc, nevals, cb, ce, sh.gtab),
+                  "launch evaluate");
+    };
+    const int kind = out ? 0 : 1;
+    static const int kGroups[3] = {1, 8, 64};
+    if (m.pointGroupForced > 0) {
+        run(m.pointGroupForced);
+        return;
+    }
+    if (m.evalGroup[kind] == 0 && N >= EVAL_TUNE_MIN_POINTS && !ldAcc && !m.plain) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        hip_check(hipStreamIsCapturing(s, &cs), "hipStreamIsCapturing");
+        if (cs == hipStreamCaptureStatusNone) {
+            hipEvent_t e0, e1;
+            hip_check(hipEventCreate(&e0), "hipEventCreate");
+            hip_check(hipEventCreate(&e1), "hipEventCreate");
+            int best = 0;
+            for (int i = 0; i < 3; ++i) {
+                run(kGroups[i]);
+                hip_check(hipEventRecord(e0, s), "hipEventRecord");
+                run(kGroups[i]);
+                hip_check(hipEventRecord(e1, s), "hipEventRecord");
+                hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+                float ms = 0.f;
+                hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+                m.evalTrialMs[kind][i] = ms;
+                if (ms < m.evalTrialMs[kind][best]) best = i;
+            }
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            m.evalGroup[kind] = kGroups[best];
+        }
+    }
+    run(m.evalGroup[kind] > 0 && !m.plain ? m.evalGroup[kind] : 1);
+}
+
 void do_evaluate(mpart_map_s& m, const mpart_cview& p, const mpart_view& o, int32_t* nevals, hipStream_t s) {
     check_coeffs(&m);
     check_view("Evaluate: pts", p.rows, p.cols, m.inDim, p.cols);
@@ -621,18 +691,15 @@ void do_evaluate(mpart_map_s& m, const mpart_cview& p, const mpart_view& o, int3
         }
         return;
     }
-    Shape sh = shape_for(m, p.cols, s);
     if (m.plain) {
+        Shape sh = shape_for(m, p.cols, s);
         for (int c = 0; c < m.dev.ncomp; ++c)
             hip_check(mpd::launch_eval(m.variant, sh.L, m.dev, m.coeffs, m.coeffStride, p.ptr, p.row_stride,
                                        p.col_stride, p.cols, o.ptr, o.row_stride, o.col_stride, nullptr, 0, 0,
                                        nevals, c, c + 1, sh.gtab),
                       "launch evaluate");
     } else {
-        hip_check(mpd::launch_eval(m.variant, sh.L, m.dev, m.coeffs, m.coeffStride, p.ptr, p.row_stride, p.col_stride,
-                                   p.cols, o.ptr, o.row_stride, o.col_stride, nullptr, 0, 0, nevals, 0,
-                                   m.dev.ncomp, sh.gtab),
-                  "launch evaluate");
+        generic_eval(m, p, o.ptr, o.row_stride, o.col_stride, nullptr, 0, 0, nevals, 0, m.dev.ncomp, s);
     }
 }
 
@@ -666,17 +733,15 @@ void do_logdet(mpart_map_s& m, const mpart_cview& p, double* out, long os, hipSt
         }
         return;
     }
-    Shape sh = shape_for(m, p.cols, s);
     if (m.plain) {
+        Shape sh = shape_for(m, p.cols, s);
         for (int c = 0; c < m.dev.ncomp; ++c)
             hip_check(mpd::launch_eval(m.variant, sh.L, m.dev, m.coeffs, m.coeffStride, p.ptr, p.row_stride,
                                        p.col_stride, p.cols, nullptr, 0, 0, out, os, c > 0 ? 1 : 0, nullptr, c, c + 1,
                                        sh.gtab),
                       "launch logdet");
     } else {
-        hip_check(mpd::launch_eval(m.variant, sh.L, m.dev, m.coeffs, m.coeffStride, p.ptr, p.row_stride, p.col_stride,
-                                   p.cols, nullptr, 0, 0, out, os, 0, nullptr, 0, m.dev.ncomp, sh.gtab),
-                  "launch logdet");
+        generic_eval(m, p, nullptr, 0, 0, out, os, 0, nullptr, 0, m.dev.ncomp, s);
     }
 }
 
@@ -1000,42 +1065,109 @@ void do_fused(mpart_map_s& m, const mpart_cview& p, const double* sens, long srs
                   "launch reduce");
         return;
     }
-    int block = 256;
-    size_t tabLane = (size_t)m.dev.nslots * sizeof(double);
-    auto smem_for = [&](int b, bool ldsAcc) { return (ldsAcc ? (size_t)(b / 64) * K * sizeof(double) : 0) + tabLane * b; };
-    bool ldsAcc = true;
-    while (block > 64 && smem_for(block, true) > LDS_LIMIT) block /= 2;
-    if (smem_for(block, true) > LDS_LIMIT) ldsAcc = false;
-    block = 256;
-    while (block > 64 && smem_for(block, ldsAcc) > LDS_LIMIT) block /= 2;
-    const bool tabLds = smem_for(block, ldsAcc) <= LDS_LIMIT;
-    const size_t smem = tabLds ? smem_for(block, ldsAcc) : (ldsAcc ? (size_t)(block / 64) * K * sizeof(double) : 0);
-    const long ntiles = std::max<long>(1, (N + block - 1) / block);
-    const int occ = mpd::occupancy_fused(m.variant, block, smem);
-    const long grid = std::min<long>(ntiles, (long)g_numCU * occ);
-    mpd::Launch L{dim3((unsigned)grid), dim3(block), smem, s};
-    double* gtab = nullptr;
-    if (!tabLds) {
-        m.gtab.ensure(tabLane * (size_t)grid * block);
-        gtab = m.gtab.as<double>();
-    }
-    m.partialGrad.ensure(std::max<size_t>(8, (size_t)grid * K * sizeof(double)));
-    m.partialLoss.ensure(std::max<size_t>(8, (size_t)grid * sizeof(double)));
-    double* gradGlobal = nullptr;
-    if (!ldsAcc) {
-        gradGlobal = m.partialGrad.as<double>();
-        hip_check(mpd::launch_fill(gradGlobal, 1, K, K, 1, 0.0, s), "launch fill");
-    }
-    if (N > 0)
-        hip_check(mpd::launch_fused(m.variant, L, m.dev, m.coeffs, m.coeffStride, p.ptr, p.row_stride, p.col_stride, N,
-                                    sens, srs, scs, lsens, lss, klMode, m.partialGrad.as<double>(),
-                                    m.partialLoss.as<double>(), gtab, gradGlobal),
-                  "launch fused");
-    const long rows = (N > 0) ? (ldsAcc ? grid : 1) : 0;
-    hip_check(mpd::launch_reduce_rows(m.partialGrad.as<double>(), rows, K, scale, grad, s), "launch reduce");
-    if (loss)
-        hip_check(mpd::launch_reduce_rows(m.partialLoss.as<double>(), N > 0 ? grid : 0, 1, scale, loss, s),
+    // Generic kernels: a lane per point (k_fused) or a group of 8 or 64 lanes
+    // per point (k_fused_grp), tuned like the generic Evaluate.
+    auto run_lane = [&]() {
+        int block = 256;
+        size_t tabLane = (size_t)m.dev.nslots * sizeof(double);
+        auto smem_for = [&](int b, bool ldsAcc) {
+            return (ldsAcc ? (size_t)(b / 64) * K * sizeof(double) : 0) + tabLane * b;
+        };
+        bool ldsAcc = true;
+        while (block > 64 && smem_for(block, true) > LDS_LIMIT) block /= 2;
+        if (smem_for(block, true) > LDS_LIMIT) ldsAcc = false;
+        block = 256;
+        while (block > 64 && smem_for(block, ldsAcc) > LDS_LIMIT) block /= 2;
+        const bool tabLds = smem_for(block, ldsAcc) <= LDS_LIMIT;
+        const size_t smem =
+            tabLds ? smem_for(block, ldsAcc) : (ldsAcc ? (size_t)(block / 64) * K * sizeof(double) : 0);
+        const long ntiles = std::max<long>(1, (N + block - 1) / block);
+        const int occ = mpd::occupancy_fused(m.variant, block, smem);
+        const long grid = std::min<long>(ntiles, (long)g_numCU * occ);
+        mpd::Launch L{dim3((unsigned)grid), dim3(block), smem, s};
+        double* gtab = nullptr;
+        if (!tabLds) {
+            m.gtab.ensure(tabLane * (size_t)grid * block);
+            gtab = m.gtab.as<double>();
+        }
+        m.partialGrad.ensure(std::max<size_t>(8, (size_t)grid * K * sizeof(double)));
+        m.partialLoss.ensure(std::max<size_t>(8, (size_t)grid * sizeof(double)));
+        double* gradGlobal = nullptr;
+        if (!ldsAcc) {
+            gradGlobal = m.partialGrad.as<double>();
+            hip_check(mpd::launch_fill(gradGlobal, 1, K, K, 1, 0.0, s), "launch fill");
+        }
+        if (N > 0)
+            hip_check(mpd::launch_fused(m.variant, L, m.dev, m.coeffs, m.coeffStride, p.ptr, p.row_stride,
+                                        p.col_stride, N, sens, srs, scs, lsens, lss, klMode,
+                                        m.partialGrad.as<double>(), m.partialLoss.as<double>(), gtab, gradGlobal),
+                      "launch fused");
+        const long rows = (N > 0) ? (ldsAcc ? grid : 1) : 0;
+        hip_check(mpd::launch_reduce_rows(m.partialGrad.as<double>(), rows, K, scale, grad, s), "launch reduce");
+        if (loss)
+            hip_check(mpd::launch_reduce_rows(m.partialLoss.as<double>(), N > 0 ? grid : 0, 1, scale, loss, s),
+                      "launch reduce");
+    };
+    // false when the group kernel does not fit (the caller then runs run_lane)
+    auto run_grp = [&](int G) -> bool {
+        const size_t tabBytes = (size_t)m.dev.nslots * sizeof(double);
+        const int block = 256;
+        const size_t smem = (size_t)(block / G) * tabBytes;
+        if (smem > LDS_LIMIT) return false;
+        const long pointsPerBlock = block / G;
+        const long nblocks = std::max<long>(1, (N + pointsPerBlock - 1) / pointsPerBlock);
+        const int occ = mpd::occupancy_fused_grp(m.variant, G, block, smem);
+        // one partial row of K + 1 doubles per wavefront, at most 2 GiB of rows
+        const long rowBytes = (long)(K + 1) * (long)sizeof(double) * (block / 64);
+        const long maxBlocks = std::max<long>(1, (2L << 30) / rowBytes);
+        const long grid = std::min<long>(std::min<long>(nblocks, (long)g_numCU * occ), maxBlocks);
+        const long nrows = grid * (block / 64);
+        m.partialGrad.ensure((size_t)(nrows + mpd::COLSUM_MAX_CHUNKS) * (K + 1) * sizeof(double));
+        double* rowsp = m.partialGrad.as<double>();
+        mpd::Launch L{dim3((unsigned)grid), dim3(block), smem, s};
+        if (N > 0)
+            hip_check(mpd::launch_fused_grp(m.variant, G, L, m.dev, m.coeffs, m.coeffStride, p.ptr, p.row_stride,
+                                            p.col_stride, N, sens, srs, scs, lsens, lss, klMode, rowsp),
+                      "launch fused (group per point)");
+        hip_check(mpd::launch_colsum(rowsp, N > 0 ? nrows : 0, K + 1, loss ? K + 1 : K, scale, grad, loss,
+                                     rowsp + nrows * (K + 1), s),
                   "launch reduce");
+        return true;
+    };
+    auto run = [&](int G) {
+        if (G > 1 && run_grp(G)) return;
+        run_lane();
+    };
+    static const int kGroups[3] = {1, 8, 64};
+    if (m.pointGroupForced > 0) {
+        run(m.pointGroupForced);
+    } else {
+        if (m.fusedGroup == 0 && N >= EVAL_TUNE_MIN_POINTS) {
+            hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+            hip_check(hipStreamIsCapturing(s, &cs), "hipStreamIsCapturing");
+            if (cs == hipStreamCaptureStatusNone) {
+                hipEvent_t e0, e1;
+                hip_check(hipEventCreate(&e0), "hipEventCreate");
+                hip_check(hipEventCreate(&e1), "hipEventCreate");
+                int best = 0;
+                for (int i = 0; i < 3; ++i) {
+                    run(kGroups[i]);
+                    hip_check(hipEventRecord(e0, s), "hipEventRecord");
+                    run(kGroups[i]);
+                    hip_check(hipEventRecord(e1, s), "hipEventRecord");
+                    hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+                    float ms = 0.f;
+                    hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+                    m.fusedGroupTrialMs[i] = ms;
+                    if (ms < m.fusedGroupTrialMs[best]) best = i;
+                }
+                (void)hipEventDestroy(e0);
+                (void)hipEventDestroy(e1);
+                m.fusedGroup = kGroups[best];
+            }
+        }
+        run(m.fusedGroup > 0 ? m.fusedGroup : 1);
+    }
     if (gin) {
         mpart_cview sv{sens, m.outDim, N, srs, scs};
         do_input_vjp(m, p, klMode, sens ? &sv : nullptr, lsens, lss, 0.0, gscale, *gin, s, "input gradient");
@@ -1819,6 +1951,28 @@ int mpart_codegen_prepare(mpart_map m, int32_t* specialized, double* compileSeco
 }
 
 const char* mpart_codegen_error(mpart_map m) { return m ? m->genErr.c_str() : "null map handle"; }
+
+int mpart_set_point_group(mpart_map m, int32_t lanes) {
+    return guarded([&] {
+        check_map(m);
+        if (lanes != 0 && lanes != 1 && lanes != 8 && lanes != 64) throw ArgError("lanes per point must be 0, 1, 8 or 64");
+        m->pointGroupForced = lanes;
+    });
+}
+
+int mpart_eval_schedule(mpart_map m, int32_t* lanes, double* trialMs) {
+    return guarded([&] {
+        check_map(m);
+        for (int k = 0; k < 2; ++k) {
+            if (lanes) lanes[k] = m->evalGroup[k];
+            if (trialMs)
+                for (int i = 0; i < 3; ++i) trialMs[k * 3 + i] = m->evalTrialMs[k][i];
+        }
+        if (lanes) lanes[2] = m->fusedGroup;
+        if (trialMs)
+            for (int i = 0; i < 3; ++i) trialMs[6 + i] = m->fusedGroupTrialMs[i];
+    });
+}
 
 int mpart_fused_schedule(mpart_map m, int32_t* choice, double* trialMs) {
     return guarded([&] {

@@ -82,6 +82,189 @@ __global__ void __launch_bounds__(256) k_eval(MapDev M, const double* w, long ws
 }
 
 // ---------------------------------------------------------------------------
+// Group-per-point schedule for maps with many terms per component: G lanes
+// (G = 8 or 64) share one point.  The point's basis table sits once in LDS
+// (filled by the group, one coordinate per lane), the lanes split each
+// group of terms (C_q by a butterfly sum, bitwise the same in every lane of
+// the group), and every lane of the group runs the same quadrature, so the
+// group stays converged.  Same outputs as k_eval up to the summation order
+// of C_q.
+
+// Orders all LDS traffic of the wavefront before what follows.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int G>
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+    for (int off = G / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+template <int BK, int G>
+__device__ __forceinline__ void group_fill_values(const MapDev& M, int ndims, const double* pt, long prs, double* tab,
+                                                  int gl) {
+    if (gl == 0) tab[0] = 1.0;
+    for (int d = gl; d < ndims; d += G) {
+        const int deg = M.dimDeg[d];
+        if (deg > 0) values_1d<BK>(M, deg, pt[d * prs], tab + M.dimSlot[d] - 1, 1);
+    }
+    wave_lds_sync();
+}
+
+template <int ND, int PM, int G>
+__device__ __forceinline__ void group_coeff_groups_nd(const MapDev& M, const CompDev& c, const double* w, long ws,
+                                                      const double* tab, int gl, double* C) {
+#pragma unroll
+    for (int q = 0; q <= PM; ++q) {
+        if (q > c.p) break;
+        const GroupDev g = M.groups[c.groupOff + q];
+        double acc = 0.0;
+        for (int j = g.start + gl; j < g.start + g.count; j += G) {
+            const uint32_t* rec = M.terms + (long)j * (ND + 1);
+            double P = tab[rec[1]];
+#pragma unroll
+            for (int e = 2; e <= ND; ++e) P *= tab[rec[e]];
+            acc += w[(long)rec[0] * ws] * P;
+        }
+        C[q] = group_sum<G>(acc);
+    }
+}
+
+template <int PM, int G>
+__device__ __forceinline__ void group_coeff_groups(const MapDev& M, const CompDev& c, const double* w, long ws,
+                                                   const double* tab, int gl, double* C) {
+    switch (M.nd) {
+        case 1: group_coeff_groups_nd<1, PM, G>(M, c, w, ws, tab, gl, C); break;
+        case 2: group_coeff_groups_nd<2, PM, G>(M, c, w, ws, tab, gl, C); break;
+        case 3: group_coeff_groups_nd<3, PM, G>(M, c, w, ws, tab, gl, C); break;
+        case 4: group_coeff_groups_nd<4, PM, G>(M, c, w, ws, tab, gl, C); break;
+        case 5: group_coeff_groups_nd<5, PM, G>(M, c, w, ws, tab, gl, C); break;
+        case 6: group_coeff_groups_nd<6, PM, G>(M, c, w, ws, tab, gl, C); break;
+        case 7: group_coeff_groups_nd<7, PM, G>(M, c, w, ws, tab, gl, C); break;
+        default: group_coeff_groups_nd<8, PM, G>(M, c, w, ws, tab, gl, C); break;
+    }
+}
+
+template <int BK, int PK, int QK, int PM, int G>
+__global__ void __launch_bounds__(256) k_eval_grp(MapDev M, const double* w, long ws, PtsArg P, OutArg out, double* ld,
+                                                  long lds, int ldAccumulate, int* nevals, int cb, int ce) {
+    extern __shared__ double smem[];
+    const int gl = threadIdx.x & (G - 1);
+    const int grp = threadIdx.x / G;
+    const long n = (long)blockIdx.x * (blockDim.x / G) + grp;
+    if (n >= P.N) return;  // the whole group leaves together
+    double* tab = smem + (long)grp * M.nslots;
+    const double* pt = P.p + n * P.cs;
+    group_fill_values<BK, G>(M, table_dims(M, cb, ce), pt, P.rs, tab, gl);
+    double ldsum = 0.0;
+    for (int c = cb; c < ce; ++c) {
+        const CompDev cd = M.comps[c];
+        double C[PM + 1];
+        group_coeff_groups<PM, G>(M, cd, w + (long)cd.coeffOff * ws, ws, tab, gl, C);
+        const double xk = pt[(long)(cd.dim - 1) * P.rs];
+        DiagCoef<PM> dc;
+        if (out.p) {
+            make_diag<PM>(M, cd, C, xk, 0.0, dc);
+            AdjAcc<PM> A;
+            int ne = 0;
+            double I = integrate<BK, PK, QK, FL_VAL, PM>(M.quad, dc, A, nevals ? &ne : nullptr);
+            if (gl == 0) {
+                out.p[(long)cd.outRow * out.rs + n * out.cs] = I + f_at_zero<PM>(M, cd, C);
+                if (nevals) nevals[(long)c * P.N + n] = ne;
+            }
+        }
+        if (ld) {
+            make_diag<PM>(M, cd, C, xk, 0.0, dc);
+            double z = diag_eval<BK, false, PM>(dc, xk, nullptr);
+            double g = posfunc<PK>(z).g;
+            ldsum += (g <= 0) ? -__builtin_inf() : log_pos(g);
+        }
+    }
+    if (ld && gl == 0) {
+        if (ldAccumulate) ld[n * lds] += ldsum;
+        else ld[n * lds] = ldsum;
+    }
+}
+
+// Fused forward + adjoint (as k_fused) with G lanes per point.  A wavefront
+// takes 64 / G points per step from a static tile sequence; after each
+// component's quadrature the lanes split the terms, sum P_j G_q over the
+// wave's points by a butterfly across the groups, and the lanes of the first
+// group add the sums to the wave's own partial row (K gradient sums, then the
+// loss), so every row has one writer and the order of additions is fixed.
+// The additions are hardware FP64 atomics on ordinary (coarse-grained) device
+// memory, which the wave does not wait for.
+template <int BK, int PK, int QK, int PM, int G>
+__global__ void __launch_bounds__(256) k_fused_grp(MapDev M, const double* w, long ws, PtsArg P, const double* sens,
+                                                   long srs, long scs, const double* lsens, long lss, int klMode,
+                                                   double* rows) {
+    extern __shared__ double smem[];
+    constexpr int GPW = 64 / G;  // points per wavefront step
+    const int lane = threadIdx.x & 63;
+    const int gl = threadIdx.x & (G - 1);
+    const int gw = lane / G;
+    const long waveId = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const long nwavesTotal = (long)gridDim.x * (blockDim.x >> 6);
+    const int K = M.numCoeffs;
+    double* tab = smem + (long)(threadIdx.x / G) * M.nslots;
+    double* row = rows + waveId * (long)(K + 1);
+    for (int j = lane; j <= K; j += 64) row[j] = 0.0;
+    double lossLane = 0.0;
+    const int ndims = table_dims(M, 0, M.ncomp);
+    const long ntiles = (P.N + GPW - 1) / GPW;
+    for (long tile = waveId; tile < ntiles; tile += nwavesTotal) {
+        const long n0 = tile * GPW + gw;
+        const bool active = n0 < P.N;
+        const long n = active ? n0 : P.N - 1;
+        const double* pt = P.p + n * P.cs;
+        wave_lds_sync();
+        group_fill_values<BK, G>(M, ndims, pt, P.rs, tab, gl);
+        for (int c = 0; c < M.ncomp; ++c) {
+            const CompDev cd = M.comps[c];
+            const double* wc = w + (long)cd.coeffOff * ws;
+            double C[PM + 1];
+            group_coeff_groups<PM, G>(M, cd, wc, ws, tab, gl, C);
+            const double xk = pt[(long)(cd.dim - 1) * P.rs];
+            const double sv = (!klMode && sens) ? sens[(long)cd.outRow * srs + n * scs] : 0.0;
+            const double lv = (!klMode && lsens) ? lsens[n * lss] : 0.0;
+            double Gq[PM + 1];
+            double lp = 0.0;
+            comp_core<BK, PK, QK, PM>(M, cd, C, xk, active, klMode, sv, lv, Gq, lp);
+            if (gl == 0) lossLane += lp;
+#pragma unroll
+            for (int q = 0; q <= PM; ++q) {
+                if (q > cd.p) break;
+                const GroupDev g = M.groups[cd.groupOff + q];
+                const int end = g.start + g.count;
+                for (int j0 = g.start; j0 < end; j0 += G) {
+                    const int j = j0 + gl;
+                    double v = 0.0;
+                    uint32_t idx = 0;
+                    if (j < end) {
+                        const uint32_t* rec = M.terms + (long)j * M.trec;
+                        double Pj = tab[rec[1]];
+                        for (int e = 2; e <= M.nd; ++e) Pj *= tab[rec[e]];
+                        v = Pj * Gq[q];
+                        idx = rec[0];
+                    }
+#pragma unroll
+                    for (int off = G; off < 64; off <<= 1) v += __shfl_xor(v, off, 64);
+                    // hardware add at L2, no return value: the wave does not
+                    // wait for the row; one writer per row keeps the order fixed
+                    if (gw == 0 && j < end) (void)unsafeAtomicAdd(row + cd.coeffOff + idx, v);
+                }
+            }
+        }
+    }
+    const double l = wave_sum(lossLane);
+    if (lane == 0) row[K] = l;
+}
+
+// ---------------------------------------------------------------------------
 // CoeffGrad of one component (CoeffJacobian + sensitivity scaling,
 // MonotoneComponent.h:166-182, 685-741): rows coeffOff.. of out receive
 //   sens_n * P_j (phi~_q(0) + rho_q S'_q).
