@@ -314,6 +314,58 @@ def input_grad_workload(tri, x, steps):
     return out
 
 
+def uneven_map_workload(steps):
+    """32-D triangular map, component k of total order max(1, 6 - k // 6)
+    (7 to 7 315 terms, 62 572 coefficients), HermiteFunctions / Exp / adaptive
+    Simpson (tolerances 1e-6); N = 200 000 points with Student-t(5)
+    coordinates (seed 3), coefficients 0.01 N(0, 1) (seed 5).  Above the
+    specialization limit, so the generic kernels run with the autotuned
+    point schedule (lanes per point).  Times Evaluate, LogDeterminant and the
+    fused KL value + gradient; the plain KL path (per-component launches with
+    the two K x N Jacobians stored) on the first 20 000 points."""
+    import torch
+
+    import mpart_amd as mt
+    from mpart_amd.data import Stream, student_t_points
+
+    orders = [max(1, 6 - k // 6) for k in range(32)]
+    o = mt.MapOptions(basisType=mt.BasisTypes.HermiteFunctions, posFuncType=mt.PosFuncTypes.Exp)
+    g = mt.TriangularMap([mt.CreateTotalOrderComponent(k + 1, orders[k], o) for k in range(32)])
+    g.SetCoeffs(0.01 * Stream(5).normal(g.numCoeffs))
+    N = 200_000
+    x = torch.from_numpy(student_t_points(32, N, nu=5, seed=3)).cuda()
+    y = torch.empty((32, N), dtype=torch.float64, device="cuda")
+    ld = torch.empty(N, dtype=torch.float64, device="cuda")
+    loss = torch.zeros(1, dtype=torch.float64, device="cuda")
+    grad = torch.zeros(g.numCoeffs, dtype=torch.float64, device="cuda")
+    xs = x[:, :20_000].contiguous()
+    calls = {
+        "evaluate": lambda: g.EvaluateImpl(x, y),
+        "log_determinant": lambda: g.LogDeterminantImpl(x, ld),
+        "kl_fused": lambda: g.KLValueAndGradImpl(x, loss, grad),
+        "kl_fused_20000": lambda: g.KLValueAndGradImpl(xs, loss, grad),
+        "kl_plain_20000": lambda: g.KLValueAndGradImpl(xs, loss, grad, plain=True),
+    }
+    out = {"workload": "32-D triangular, orders max(1, 6 - k // 6), 62572 coefficients, HermiteFunctions/Exp/"
+                       "AdaptiveSimpson, N=200000 Student-t(5)", "N": N, "num_coeffs": g.numCoeffs}
+    for name, fn in calls.items():
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        out[name + "_ms"] = ms
+        log(f"uneven 32-D map {name}: {ms:.2f} ms")
+    out["kl_fused_points_per_s"] = N / (out["kl_fused_ms"] * 1e-3)
+    out["fused_vs_plain_20000"] = out["kl_plain_20000_ms"] / out["kl_fused_20000_ms"]
+    lanes, trials = g.EvalSchedule()
+    out["lanes_per_point"] = {"evaluate": lanes[0], "log_determinant": lanes[1], "fused": lanes[2],
+                              "trial_ms_lanes_1_8_64": trials}
+    return out
+
+
 def load_traffic():
     """HBM bytes per fused launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
     summary of this workload (None when absent)."""
@@ -358,6 +410,7 @@ def main():
     ap.add_argument("--no-many", action="store_true")
     ap.add_argument("--no-single", action="store_true")
     ap.add_argument("--no-input-grad", action="store_true")
+    ap.add_argument("--no-uneven", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--profile", action="store_true")
     ap.add_argument("--profile-inverse", action="store_true")
@@ -422,6 +475,9 @@ def main():
     single = None
     if not args.no_single:
         single = single_component_workload(max(3, args.plain_steps))
+    uneven = None
+    if not args.no_uneven:
+        uneven = uneven_map_workload(2)
 
     traffic = load_traffic()
     # HBM bytes of the schedule the autotuner chose (measured per schedule; a
@@ -463,6 +519,7 @@ def main():
         "active_quadrature_workload": second,
         "many_maps_workload": many,
         "single_component_workload": single,
+        "uneven_map_workload": uneven,
     }
     if not args.no_cpu:
         log("cpu baseline")

@@ -240,22 +240,35 @@ __global__ void __launch_bounds__(256) k_fused_grp(MapDev M, const double* w, lo
                 if (q > cd.p) break;
                 const GroupDev g = M.groups[cd.groupOff + q];
                 const int end = g.start + g.count;
-                for (int j0 = g.start; j0 < end; j0 += G) {
-                    const int j = j0 + gl;
-                    double v = 0.0;
-                    uint32_t idx = 0;
-                    if (j < end) {
-                        const uint32_t* rec = M.terms + (long)j * M.trec;
-                        double Pj = tab[rec[1]];
-                        for (int e = 2; e <= M.nd; ++e) Pj *= tab[rec[e]];
-                        v = Pj * Gq[q];
-                        idx = rec[0];
+                // four chunks of G terms per step, so their loads overlap
+                constexpr int U = 4;
+                for (int j0 = g.start; j0 < end; j0 += U * G) {
+                    double v[U];
+                    uint32_t idx[U];
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        const int j = j0 + u * G + gl;
+                        v[u] = 0.0;
+                        idx[u] = 0;
+                        if (j < end) {
+                            const uint32_t* rec = M.terms + (long)j * M.trec;
+                            double Pj = tab[rec[1]];
+                            for (int e = 2; e <= M.nd; ++e) Pj *= tab[rec[e]];
+                            v[u] = Pj * Gq[q];
+                            idx[u] = rec[0];
+                        }
                     }
 #pragma unroll
-                    for (int off = G; off < 64; off <<= 1) v += __shfl_xor(v, off, 64);
+                    for (int u = 0; u < U; ++u) {
+#pragma unroll
+                        for (int off = G; off < 64; off <<= 1) v[u] += __shfl_xor(v[u], off, 64);
+                    }
                     // hardware add at L2, no return value: the wave does not
                     // wait for the row; one writer per row keeps the order fixed
-                    if (gw == 0 && j < end) (void)unsafeAtomicAdd(row + cd.coeffOff + idx, v);
+#pragma unroll
+                    for (int u = 0; u < U; ++u)
+                        if (gw == 0 && j0 + u * G + gl < end)
+                            (void)unsafeAtomicAdd(row + cd.coeffOff + idx[u], v[u]);
                 }
             }
         }
