@@ -104,3 +104,34 @@ def test_scheduled_triangular_inverse_equals_sequential():
     seq = mt.MapBatch([g]).Inverse([empty], [y])[0]
     assert torch.equal(torch.isnan(single), torch.isnan(seq))
     assert torch.equal(torch.nan_to_num(single), torch.nan_to_num(seq))
+
+
+@pytest.mark.parametrize("ci", [0, 1, 4, 10, 12])
+def test_large_scheduled_inverse_equals_sequential(ci):
+    # 30 000 points with wide targets over several option cases: every solve
+    # is bitwise the sequential one, NaN inputs give NaN.
+    _case("inverse_large", case=CASES[ci])
+    g, c, x, tgt = _component(CASES[ci], 980 + ci, 30_000, nan_at=[(0, 7), (2, 29_999)])
+    xd = torch.from_numpy(x).cuda()
+    td = torch.from_numpy(tgt).cuda()
+    out = torch.full_like(td, -7.0)
+    g.InverseImpl(xd, td, out)
+    seq = mt.MapBatch([g]).Inverse([xd], [td])[0]
+    assert not bool((out == -7.0).any()), "a point was not solved"
+    assert torch.equal(torch.isnan(out), torch.isnan(seq))
+    assert torch.equal(torch.nan_to_num(out), torch.nan_to_num(seq))
+    assert bool(torch.isnan(out[0, [7, 29_999]]).all())
+
+
+def test_large_triangular_inverse_equals_sequential():
+    _case("inverse_triangular_large")
+    s = Stream(990)
+    g = mt.CreateTriangular(3, 3, 3, mt.MapOptions())
+    g.SetCoeffs(0.1 * s.normal(g.numCoeffs))
+    n = 40_000
+    y = torch.from_numpy(2.0 * s.normal(3 * n).reshape(3, n)).cuda()
+    empty = torch.zeros((0, n), dtype=torch.float64, device="cuda")
+    single = g.Inverse(empty, y)
+    seq = mt.MapBatch([g]).Inverse([empty], [y])[0]
+    assert torch.equal(torch.isnan(single), torch.isnan(seq))
+    assert torch.equal(torch.nan_to_num(single), torch.nan_to_num(seq))
