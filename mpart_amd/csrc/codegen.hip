@@ -102,9 +102,77 @@ void emit_coeff_sums(std::ostringstream& o, const GenComp& c) {
     }
 }
 
+// Input VJP / KL input gradient kernels (all components).
+void emit_input_vjp(std::ostringstream& o, const GenSpec& s) {
+    const unsigned ncomp = unsigned(s.comps.size());
+    // ---------------- input VJP / KL input gradient (all components) ----------------
+    // The body is a template on KL mode (sensitivities T and -1 known at
+    // compile time), instantiated by two kernels.
+    o << R"(
+template <bool KL>
+__device__ __forceinline__ void ivjp_body(const MapDev& M, const double* __restrict__ w,
+        const double* __restrict__ pts, long prs, long pcs, long N, const double* sens, long srs,
+        long scs, const double* lsens, long lss, double lval, double scale, double* out, long ors, long ocs) {
+    const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    const double* pt = pts + n * pcs;
+    const bool hasL = lsens != nullptr || lval != 0.0;
+    const double lv = lsens ? lsens[n * lss] : lval;
+)";
+    emit_basis_derivs(o, s, s.D - 1);
+    for (unsigned d = 0; d < s.D; ++d) o << "    double gx" << d << " = 0.0;\n";
+    o << "#pragma unroll 1\n    for (int c = 0; c < " << ncomp << "; ++c) {\n";
+    emit_pin_derivs(o, s, s.D - 1, "        ");
+    o << "        const CompDev cd = M.comps[c];\n        double C[GPM + 1];\n        switch (c) {\n";
+    for (unsigned c = 0; c < ncomp; ++c) {
+        o << "        case " << c << ": {\n";
+        emit_coeff_sums(o, s.comps[c]);
+        o << "        break; }\n";
+    }
+    o << R"(        }
+        const double sv = sens ? sens[(long)cd.outRow * srs + n * scs] : 0.0;
+        double G[GPM + 1], gk;
+        ivjp_core<GBK, GPK, GQK, GPM, KL>(M, cd, C, pt[(long)(cd.dim - 1) * prs], sens != nullptr, sv, hasL, lv,
+                                          G, gk);
+)";
+    emit_pin_derivs(o, s, s.D - 1, "        ");
+    o << "        switch (c) {\n";
+    for (unsigned c = 0; c < ncomp; ++c) {
+        const GenComp& gc = s.comps[c];
+        const unsigned nt = unsigned(gc.starts.size()) - 1;
+        o << "        case " << c << ": {\n";
+        for (unsigned t = 0; t < nt; ++t) {
+            unsigned q;
+            term_product(gc, t, &q);
+            bool any = false;
+            for (unsigned j = gc.starts[t]; j < gc.starts[t + 1]; ++j) {
+                if (gc.dims[j] == gc.dim - 1) continue;
+                if (!any) o << "            { const double a = w[" << gc.coeffOff + t << "] * G[" << q << "];\n";
+                any = true;
+                o << "              gx" << gc.dims[j] << " += a * " << term_deriv(gc, t, j) << ";\n";
+            }
+            if (any) o << "            }\n";
+        }
+        o << "            gx" << gc.dim - 1 << " += gk;\n        break; }\n";
+    }
+    o << "        }\n    }\n    double* o = out + n * ocs;\n";
+    for (unsigned d = 0; d < s.D; ++d) o << "    o[" << d << "L * ors] = scale * gx" << d << ";\n";
+    o << R"(}
+#define MPD_IVJP_ARGS MapDev M, const double* __restrict__ w, const double* __restrict__ pts, long prs, long pcs, \
+        long N, const double* sens, long srs, long scs, const double* lsens, long lss, double lval, double scale, \
+        double* out, long ors, long ocs
+extern "C" __global__ void __launch_bounds__(256) mpart_gen_ivjp(MPD_IVJP_ARGS) {
+    ivjp_body<false>(M, w, pts, prs, pcs, N, sens, srs, scs, lsens, lss, lval, scale, out, ors, ocs);
+}
+extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) mpart_gen_klx(MPD_IVJP_ARGS) {
+    ivjp_body<true>(M, w, pts, prs, pcs, N, nullptr, 0, 0, nullptr, 0, -1.0, scale, out, ors, ocs);
+}
+)";
+}
+
 }  // namespace
 
-std::string generate_source(const GenSpec& s) {
+std::string generate_source(const GenSpec& s, int part) {
     std::ostringstream o;
     o << "#define MPD_LINEARIZED " << (s.lin ? 1 : 0) << "\n#include \"mono_device.hpp\"\nusing namespace mpd;\n";
     o << "#define GBK " << s.bk << "\n#define GPK " << s.pk << "\n#define GQK " << s.qk << "\n#define GPM " << s.pm
@@ -119,6 +187,7 @@ std::string generate_source(const GenSpec& s) {
     // the same sensitivities from the same quadrature (the input-adjoint
     // integrand adds U to the coefficient adjoint S_q).
     for (int in = 0; in < 2; ++in) {
+        if (in != part) continue;
         o << "\n__device__ __forceinline__ void " << (in ? "fused_body_in" : "fused_body") << R"((const MapDev& M, const double* __restrict__ w,
         const double* __restrict__ pts, long prs, long pcs, long N, const double* sens, long srs, long scs,
         const double* lsens, long lss, int klMode, double* partial)";
@@ -234,16 +303,22 @@ std::string generate_source(const GenSpec& s) {
     }
     o << R"(#define MPD_FUSED_ARGS MapDev M, const double* __restrict__ w, const double* __restrict__ pts, long prs, long pcs, \
         long N, const double* sens, long srs, long scs, const double* lsens, long lss, int klMode, double* partial
-extern "C" __global__ void __launch_bounds__(256) mpart_gen_fused(MPD_FUSED_ARGS) {
+)";
+    if (part == 1) {
+        o << R"(extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+mpart_gen_fused_in(MPD_FUSED_ARGS, double gscale, double* gin, long gors, long gocs) {
+    fused_body_in(M, w, pts, prs, pcs, N, sens, srs, scs, lsens, lss, klMode, partial, gscale, gin, gors, gocs);
+}
+)";
+        emit_input_vjp(o, s);
+        return o.str();
+    }
+    o << R"(extern "C" __global__ void __launch_bounds__(256) mpart_gen_fused(MPD_FUSED_ARGS) {
     fused_body(M, w, pts, prs, pcs, N, sens, srs, scs, lsens, lss, klMode, partial);
 }
 extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 mpart_gen_fused_w3(MPD_FUSED_ARGS) {
     fused_body(M, w, pts, prs, pcs, N, sens, srs, scs, lsens, lss, klMode, partial);
-}
-extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) mpart_gen_fused_in(MPD_FUSED_ARGS, double gscale, double* gin,
-                                                                     long gors, long gocs) {
-    fused_body_in(M, w, pts, prs, pcs, N, sens, srs, scs, lsens, lss, klMode, partial, gscale, gin, gors, gocs);
 }
 )";
 
@@ -359,69 +434,6 @@ extern "C" __global__ void __launch_bounds__(256) mpart_gen_ldcoef(MapDev M, con
         o << "    }\n}\n";
     }
 
-    // ---------------- input VJP / KL input gradient (all components) ----------------
-    // The body is a template on KL mode (sensitivities T and -1 known at
-    // compile time), instantiated by two kernels.
-    o << R"(
-template <bool KL>
-__device__ __forceinline__ void ivjp_body(const MapDev& M, const double* __restrict__ w,
-        const double* __restrict__ pts, long prs, long pcs, long N, const double* sens, long srs,
-        long scs, const double* lsens, long lss, double lval, double scale, double* out, long ors, long ocs) {
-    const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
-    const double* pt = pts + n * pcs;
-    const bool hasL = lsens != nullptr || lval != 0.0;
-    const double lv = lsens ? lsens[n * lss] : lval;
-)";
-    emit_basis_derivs(o, s, s.D - 1);
-    for (unsigned d = 0; d < s.D; ++d) o << "    double gx" << d << " = 0.0;\n";
-    o << "#pragma unroll 1\n    for (int c = 0; c < " << ncomp << "; ++c) {\n";
-    emit_pin_derivs(o, s, s.D - 1, "        ");
-    o << "        const CompDev cd = M.comps[c];\n        double C[GPM + 1];\n        switch (c) {\n";
-    for (unsigned c = 0; c < ncomp; ++c) {
-        o << "        case " << c << ": {\n";
-        emit_coeff_sums(o, s.comps[c]);
-        o << "        break; }\n";
-    }
-    o << R"(        }
-        const double sv = sens ? sens[(long)cd.outRow * srs + n * scs] : 0.0;
-        double G[GPM + 1], gk;
-        ivjp_core<GBK, GPK, GQK, GPM, KL>(M, cd, C, pt[(long)(cd.dim - 1) * prs], sens != nullptr, sv, hasL, lv,
-                                          G, gk);
-)";
-    emit_pin_derivs(o, s, s.D - 1, "        ");
-    o << "        switch (c) {\n";
-    for (unsigned c = 0; c < ncomp; ++c) {
-        const GenComp& gc = s.comps[c];
-        const unsigned nt = unsigned(gc.starts.size()) - 1;
-        o << "        case " << c << ": {\n";
-        for (unsigned t = 0; t < nt; ++t) {
-            unsigned q;
-            term_product(gc, t, &q);
-            bool any = false;
-            for (unsigned j = gc.starts[t]; j < gc.starts[t + 1]; ++j) {
-                if (gc.dims[j] == gc.dim - 1) continue;
-                if (!any) o << "            { const double a = w[" << gc.coeffOff + t << "] * G[" << q << "];\n";
-                any = true;
-                o << "              gx" << gc.dims[j] << " += a * " << term_deriv(gc, t, j) << ";\n";
-            }
-            if (any) o << "            }\n";
-        }
-        o << "            gx" << gc.dim - 1 << " += gk;\n        break; }\n";
-    }
-    o << "        }\n    }\n    double* o = out + n * ocs;\n";
-    for (unsigned d = 0; d < s.D; ++d) o << "    o[" << d << "L * ors] = scale * gx" << d << ";\n";
-    o << R"(}
-#define MPD_IVJP_ARGS MapDev M, const double* __restrict__ w, const double* __restrict__ pts, long prs, long pcs, \
-        long N, const double* sens, long srs, long scs, const double* lsens, long lss, double lval, double scale, \
-        double* out, long ors, long ocs
-extern "C" __global__ void __launch_bounds__(256) mpart_gen_ivjp(MPD_IVJP_ARGS) {
-    ivjp_body<false>(M, w, pts, prs, pcs, N, sens, srs, scs, lsens, lss, lval, scale, out, ors, ocs);
-}
-extern "C" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) mpart_gen_klx(MPD_IVJP_ARGS) {
-    ivjp_body<true>(M, w, pts, prs, pcs, N, nullptr, 0, 0, nullptr, 0, -1.0, scale, out, ors, ocs);
-}
-)";
     return o.str();
 }
 
@@ -458,8 +470,8 @@ bool compile_source(const std::string& src, const std::string& arch, std::vector
     return true;
 }
 
-GenModule* get_module(const GenSpec& spec, std::string& err) {
-    const std::string src = generate_source(spec);
+GenModule* get_module(const GenSpec& spec, std::string& err, int part) {
+    const std::string src = generate_source(spec, part);
     std::lock_guard<std::mutex> lk(g_mu);
     auto it = g_cache.find(src);
     if (it != g_cache.end()) return it->second.get();
@@ -478,14 +490,16 @@ GenModule* get_module(const GenSpec& spec, std::string& err) {
         err = "hipModuleLoadData failed";
         return nullptr;
     }
-    if (hipModuleGetFunction(&m->fused, m->mod, "mpart_gen_fused") != hipSuccess ||
-        hipModuleGetFunction(&m->fused3, m->mod, "mpart_gen_fused_w3") != hipSuccess ||
-        hipModuleGetFunction(&m->eval, m->mod, "mpart_gen_eval") != hipSuccess ||
-        hipModuleGetFunction(&m->coef, m->mod, "mpart_gen_coef") != hipSuccess ||
-        hipModuleGetFunction(&m->ldcoef, m->mod, "mpart_gen_ldcoef") != hipSuccess ||
-        hipModuleGetFunction(&m->ivjp, m->mod, "mpart_gen_ivjp") != hipSuccess ||
-        hipModuleGetFunction(&m->klx, m->mod, "mpart_gen_klx") != hipSuccess ||
-        hipModuleGetFunction(&m->fusedIn, m->mod, "mpart_gen_fused_in") != hipSuccess) {
+    const bool ok =
+        part == 0 ? (hipModuleGetFunction(&m->fused, m->mod, "mpart_gen_fused") == hipSuccess &&
+                     hipModuleGetFunction(&m->fused3, m->mod, "mpart_gen_fused_w3") == hipSuccess &&
+                     hipModuleGetFunction(&m->eval, m->mod, "mpart_gen_eval") == hipSuccess &&
+                     hipModuleGetFunction(&m->coef, m->mod, "mpart_gen_coef") == hipSuccess &&
+                     hipModuleGetFunction(&m->ldcoef, m->mod, "mpart_gen_ldcoef") == hipSuccess)
+                  : (hipModuleGetFunction(&m->ivjp, m->mod, "mpart_gen_ivjp") == hipSuccess &&
+                     hipModuleGetFunction(&m->klx, m->mod, "mpart_gen_klx") == hipSuccess &&
+                     hipModuleGetFunction(&m->fusedIn, m->mod, "mpart_gen_fused_in") == hipSuccess);
+    if (!ok) {
         err = "hipModuleGetFunction failed";
         return nullptr;
     }

@@ -39,7 +39,11 @@ struct GenModule {
     double compileSeconds = 0.0;
 };
 
-std::string generate_source(const GenSpec& spec);
+// part 0: the fused pass (KL / VJP), Evaluate / LogDeterminant, CoeffGrad,
+// LogDeterminantCoeffGrad; part 1: the kernels with input gradients (input
+// VJP, KL input gradient, fused pass with input gradients).  Part 1 is
+// compiled only when one of them is first used.
+std::string generate_source(const GenSpec& spec, int part);
 
 // hipRTC compilation of generated source for `arch` (e.g. "gfx950") into a code
 // object; false with `err` set (including the compiler log) on failure.
@@ -47,6 +51,6 @@ bool compile_source(const std::string& src, const std::string& arch, std::vector
 
 // Returns the compiled module for this structure (cached process-wide), or
 // nullptr with `err` set when compilation or loading fails.
-GenModule* get_module(const GenSpec& spec, std::string& err);
+GenModule* get_module(const GenSpec& spec, std::string& err, int part);
 
 }  // namespace mpg

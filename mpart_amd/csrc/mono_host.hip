@@ -208,6 +208,8 @@ struct mpart_map_s {
     // runtime-specialized kernels (codegen.hip)
     bool genEnabled = true;
     bool genTried = false;
+    bool genInTried = false;
+    mpg::GenModule* genIn = nullptr;
     mpg::GenModule* gen = nullptr;
     double genSeconds = 0.0;
     std::string genErr;
@@ -553,14 +555,7 @@ bool codegen_env_enabled() {
     return on;
 }
 
-mpg::GenModule* gen_for(mpart_map_s& m) {
-    if (!m.genEnabled || !codegen_env_enabled()) return nullptr;
-    if (m.genTried) return m.gen;
-    m.genTried = true;
-    if (m.numCoeffs > GEN_MAX_COEFFS) {
-        m.genErr = "more than " + std::to_string(GEN_MAX_COEFFS) + " coefficients: generic kernels";
-        return nullptr;
-    }
+mpg::GenSpec gen_spec(const mpart_map_s& m) {
     mpg::GenSpec spec;
     spec.bk = (m.variant / 100) % 10;
     spec.pk = (m.variant / 10) % 10;
@@ -583,10 +578,33 @@ mpg::GenModule* gen_for(mpart_map_s& m) {
         g.orders = s.orders;
         spec.comps.push_back(std::move(g));
     }
+    return spec;
+}
+
+mpg::GenModule* gen_for(mpart_map_s& m) {
+    if (!m.genEnabled || !codegen_env_enabled()) return nullptr;
+    if (m.genTried) return m.gen;
+    m.genTried = true;
+    if (m.numCoeffs > GEN_MAX_COEFFS) {
+        m.genErr = "more than " + std::to_string(GEN_MAX_COEFFS) + " coefficients: generic kernels";
+        return nullptr;
+    }
     const auto t0 = std::chrono::steady_clock::now();
-    m.gen = mpg::get_module(spec, m.genErr);
+    m.gen = mpg::get_module(gen_spec(m), m.genErr, 0);
     m.genSeconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return m.gen;
+}
+
+// The specialized kernels with input gradients, a second module compiled on
+// their first use (nullptr: generic kernels).
+mpg::GenModule* gen_in_for(mpart_map_s& m) {
+    if (!gen_for(m)) return nullptr;
+    if (m.genInTried) return m.genIn;
+    m.genInTried = true;
+    std::string err;
+    m.genIn = mpg::get_module(gen_spec(m), err, 1);
+    if (!m.genIn) m.genErr = err;
+    return m.genIn;
 }
 
 // Coefficients as a contiguous vector (the specialized kernels index w[i]).
@@ -852,7 +870,7 @@ void do_input_vjp(mpart_map_s& m, const mpart_cview& p, int klMode, const mpart_
     if (p.cols == 0) return;
     const double* sp = sens ? sens->ptr : nullptr;
     long srs = sens ? sens->row_stride : 0, scs = sens ? sens->col_stride : 0;
-    if (mpg::GenModule* g = gen_for(m)) {
+    if (mpg::GenModule* g = gen_in_for(m)) {
         const double* w = gen_coeffs(m, s);
         MapDev M = m.dev;
         const double* pts = p.ptr;
@@ -999,12 +1017,13 @@ void do_fused(mpart_map_s& m, const mpart_cview& p, const double* sens, long srs
         auto launch_cfg = [&](int cfg) {
             gen_launch(fns[cfg & 1], grid_of(cfg), gblock, gsmem, s, args, "launch specialized fused");
         };
-        if (gin) {
+        mpg::GenModule* gi = gin ? gen_in_for(m) : nullptr;
+        if (gi) {
             // One kernel for both gradients; persistent grid of two blocks per
             // resident slot (the schedule the tuner picks for the fused kernel).
             if (m.genOccIn == 0) {
                 int nb = 0;
-                hip_check(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nb, g->fusedIn, gblock, gsmem),
+                hip_check(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nb, gi->fusedIn, gblock, gsmem),
                           "hipModuleOccupancyMaxActiveBlocksPerMultiprocessor");
                 m.genOccIn = std::max(1, nb);
             }
@@ -1018,7 +1037,7 @@ void do_fused(mpart_map_s& m, const mpart_cview& p, const double* sens, long srs
             void* argsIn[] = {&M, &w, &pts, &prs, &pcs, &NN, &sp, &srs_, &scs_, &lp, &lss_, &kl, &part,
                               &gs, &gp, &gors, &gocs};
             const long rows = (N > 0) ? gridIn : 0;
-            if (N > 0) gen_launch(g->fusedIn, gridIn, gblock, gsmem, s, argsIn, "launch specialized fused (inputs)");
+            if (N > 0) gen_launch(gi->fusedIn, gridIn, gblock, gsmem, s, argsIn, "launch specialized fused (inputs)");
             hip_check(mpd::launch_colsum(part, rows, K + 1, loss ? K + 1 : K, scale, grad, loss,
                                          part + std::max(maxGrid, gridIn) * (K + 1), s),
                       "launch reduce");
@@ -1063,6 +1082,11 @@ void do_fused(mpart_map_s& m, const mpart_cview& p, const double* sens, long srs
         hip_check(mpd::launch_colsum(part, rows, K + 1, loss ? K + 1 : K, scale, grad, loss, part + maxGrid * (K + 1),
                                      s),
                   "launch reduce");
+        if (gin) {
+            // the input-gradient module did not build: a second (generic) launch
+            mpart_cview sv{sens, m.outDim, N, srs, scs};
+            do_input_vjp(m, p, klMode, sens ? &sv : nullptr, lsens, lss, 0.0, gscale, *gin, s, "input gradient");
+        }
         return;
     }
     // Generic kernels: a lane per point (k_fused) or a group of 8 or 64 lanes
