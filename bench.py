@@ -397,6 +397,54 @@ def profile_mode(args):
                       "calibration": "torch.mul over 256 MiB (reads 262144 KiB, writes 262144 KiB)"}), flush=True)
 
 
+def profile_grads_mode(args):
+    """For rocprofv3: on the 8-D order-3 map with N(0, I) data (seed 1) and
+    0.1 N(0, 1) coefficients, N = 500 000: three KL point-gradient launches
+    (KLInputGradImpl) and three one-pass loss + both gradients
+    (KLValueAndGradsImpl); then on the 32-D uneven map (orders
+    max(1, 6 - k // 6), HermiteFunctions / Exp, Student-t(5), 20 000 points)
+    Evaluate, LogDeterminant and the fused KL once each with 1 and with 8
+    lanes per point."""
+    import torch
+
+    import mpart_amd as mt
+    from mpart_amd.data import Stream, student_t_points
+
+    tri = mt.CreateTriangular(D, D, ORDER, mt.MapOptions())
+    s = Stream(1)
+    x = torch.from_numpy(s.points(D, args.n)).cuda()
+    tri.SetCoeffs(0.1 * s.normal(tri.numCoeffs))
+    gx = torch.empty((D, args.n), dtype=torch.float64, device="cuda")
+    loss = torch.zeros(1, dtype=torch.float64, device="cuda")
+    grad = torch.zeros(tri.numCoeffs, dtype=torch.float64, device="cuda")
+    tri.KLInputGradImpl(x, gx, 1.0 / args.n)
+    tri.KLValueAndGradsImpl(x, loss, grad, gx)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        tri.KLInputGradImpl(x, gx, 1.0 / args.n)
+    for _ in range(3):
+        tri.KLValueAndGradsImpl(x, loss, grad, gx)
+    torch.cuda.synchronize()
+    orders = [max(1, 6 - k // 6) for k in range(32)]
+    o = mt.MapOptions(basisType=mt.BasisTypes.HermiteFunctions, posFuncType=mt.PosFuncTypes.Exp)
+    g = mt.TriangularMap([mt.CreateTotalOrderComponent(k + 1, orders[k], o) for k in range(32)])
+    g.SetCoeffs(0.01 * Stream(5).normal(g.numCoeffs))
+    n = 20_000
+    xu = torch.from_numpy(student_t_points(32, n, nu=5, seed=3)).cuda()
+    y = torch.empty((32, n), dtype=torch.float64, device="cuda")
+    ld = torch.empty(n, dtype=torch.float64, device="cuda")
+    lossu = torch.zeros(1, dtype=torch.float64, device="cuda")
+    gradu = torch.zeros(g.numCoeffs, dtype=torch.float64, device="cuda")
+    for lanes in (1, 8):
+        g.SetPointGroup(lanes)
+        g.EvaluateImpl(xu, y)
+        g.LogDeterminantImpl(xu, ld)
+        g.KLValueAndGradImpl(xu, lossu, gradu)
+    torch.cuda.synchronize()
+    print(json.dumps({"profile": "kl_input_grad_c3_normal + uneven_map_group_kernels", "loss": float(loss.item())}),
+          flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -414,6 +462,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--profile", action="store_true")
     ap.add_argument("--profile-inverse", action="store_true")
+    ap.add_argument("--profile-grads", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -425,6 +474,9 @@ def main():
         return
     if args.profile_inverse:
         profile_inverse_mode(args)
+        return
+    if args.profile_grads:
+        profile_grads_mode(args)
         return
 
     import torch
