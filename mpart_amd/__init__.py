@@ -656,7 +656,8 @@ class MapBatch:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h is not None and h.value:
+        # at interpreter exit the module globals may already be gone
+        if h is not None and h.value and _lib is not None:
             _lib.mpart_destroy_batch(h)
             self._h = None
 
