@@ -126,11 +126,11 @@ def oracle_check(x_np, coeffs, fused_grad_fn, n=2000):
     return float(max(abs(lg - lr) / abs(lr), np.max(np.abs(gg - gr)) / np.max(np.abs(gr))))
 
 
-def many_maps(steps):
-    """256 independent triangular maps (dims 2..8, orders 2..4, 4096..16384
-    N(0, I) points each, coefficients 0.1 N(0,1); seed-21 stream): Evaluate +
-    LogDeterminant of every map as one batched launch replayed from a HIP graph,
-    against a host loop of per-map launches (generic kernels)."""
+def many_maps(steps, dmax=8, pmax=4, nmax=16384):
+    """256 independent triangular maps (dims 2..dmax, orders 2..pmax,
+    4096..nmax N(0, I) points each, coefficients 0.1 N(0,1); seed-21 stream):
+    Evaluate + LogDeterminant of every map as one batched launch replayed from
+    a HIP graph, against a host loop of per-map launches (generic kernels)."""
     import torch
 
     import mpart_amd as mt
@@ -141,9 +141,9 @@ def many_maps(steps):
     u = s.uniform(3 * nm).reshape(nm, 3)
     maps, xs = [], []
     for i in range(nm):
-        d = 2 + int(u[i, 0] * 7)
-        p = 2 + int(u[i, 1] * 3)
-        n = 4096 + int(u[i, 2] * (16384 - 4096))
+        d = 2 + int(u[i, 0] * (dmax - 1))
+        p = 2 + int(u[i, 1] * (pmax - 1))
+        n = 4096 + int(u[i, 2] * (nmax - 4096))
         m = mt.CreateTriangular(d, d, p, mt.MapOptions())
         m.SetCoeffs(0.1 * s.normal(m.numCoeffs))
         m.SetCodegen(False)
@@ -182,12 +182,15 @@ def many_maps(steps):
     torch.cuda.synchronize()
     tl = (time.perf_counter() - t0) * 1e3 / steps
     same = all(torch.equal(a, c) for a, c in zip(y_b, ys))
-    log(f"many maps: batched graph {tb:.3f} ms, per-map loop {tl:.3f} ms, identical outputs {same}")
-    return {"maps": nm, "points": int(npts), "batched_graph_ms": tb, "per_map_loop_ms": tl,
+    maxabs = max((a - c).abs().max().item() for a, c in zip(y_b, ys))
+    log(f"many maps (dims 2..{dmax}, orders 2..{pmax}, N <= {nmax}): batched graph {tb:.3f} ms, "
+        f"per-map loop {tl:.3f} ms, identical outputs {same}, max abs diff {maxabs:.2e}")
+    return {"maps": nm, "points": int(npts), "num_coeffs": int(sum(m.numCoeffs for m in maps)),
+            "batched_graph_ms": tb, "per_map_loop_ms": tl,
             "batched_points_per_s": npts / (tb * 1e-3), "speedup_batched_vs_loop": tl / tb,
-            "batched_equals_per_map_bitwise": bool(same),
-            "workload": "Evaluate + LogDeterminant of 256 triangular maps, dims 2..8, orders 2..4, "
-                        "4096..16384 N(0,I) points each"}
+            "batched_equals_per_map_bitwise": bool(same), "max_abs_diff_batched_vs_loop": maxabs,
+            "workload": f"Evaluate + LogDeterminant of 256 triangular maps, dims 2..{dmax}, orders 2..{pmax}, "
+                        f"4096..{nmax} N(0,I) points each"}
 
 
 def single_component_setup():
@@ -522,8 +525,12 @@ def main():
         del x2
 
     many = None
+    many_full = None
     if not args.no_many:
         many = many_maps(max(3, args.plain_steps))
+        # the full ranges (dims 2..16, orders 2..5, up to 65 536 points); the
+        # loop's first call tunes each large map's lanes per point
+        many_full = many_maps(2, dmax=16, pmax=5, nmax=65536)
     single = None
     if not args.no_single:
         single = single_component_workload(max(3, args.plain_steps))
@@ -570,6 +577,7 @@ def main():
                                 "fused_schedule": fused_schedule},
         "active_quadrature_workload": second,
         "many_maps_workload": many,
+        "many_maps_full_ranges_workload": many_full,
         "single_component_workload": single,
         "uneven_map_workload": uneven,
     }
