@@ -292,6 +292,18 @@ typedef struct mpart_batch_s* mpart_batch;
 int mpart_create_batch(const mpart_map* maps, uint32_t n, mpart_batch* out);
 int mpart_destroy_batch(mpart_batch batch);
 int mpart_batch_size(mpart_batch batch, uint32_t* n);
+/* Schedule of the batched Evaluate / LogDeterminant: 1 (a lane per point for
+ * every map) or 8 (mixed: maps with many terms -- at least 4096 coefficients,
+ * or a per-lane basis table that does not fit 256 lanes of LDS -- take a
+ * group of 8 lanes per point sharing its table, the others a lane per point,
+ * in the same launch).  mpart_batch_set_point_group forces it (0: autotune, the
+ * default: the first uncaptured call with >= 65536 points in all times both
+ * and keeps the faster; the choice is frozen once a call has been captured).
+ * mpart_batch_eval_schedule reports it (0: not tuned yet) and the two trial
+ * times in ms.  No counterpart in the reference. */
+int mpart_batch_set_point_group(mpart_batch batch, int32_t lanes);
+int mpart_batch_eval_schedule(mpart_batch batch, int32_t* lanes, double* trialMs);
+
 /* out and logdet may be NULL (not computed); so may out[i].ptr / logdet[i]. */
 int mpart_batch_evaluate(mpart_batch batch, const mpart_cview* pts, const mpart_view* out, double* const* logdet,
                          const int64_t* logdetStride, mpart_stream stream);

@@ -116,6 +116,8 @@ _SIGS = {
     "mpart_batch_evaluate": [_vp, _vp, _vp, _vp, _vp, _vp],
     "mpart_batch_coeff_grad": [_vp, _vp, _vp, _vp, _vp],
     "mpart_batch_inverse": [_vp, _vp, _vp, _vp, _dbl, _dbl, _vp],
+    "mpart_batch_set_point_group": [_vp, ctypes.c_int32],
+    "mpart_batch_eval_schedule": [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)],
     "mpart_codegen_prepare": [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)],
     "mpart_fused_schedule": [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)],
     "mpart_set_point_group": [_vp, ctypes.c_int32],
@@ -660,6 +662,18 @@ class MapBatch:
         if h is not None and h.value and _lib is not None:
             _lib.mpart_destroy_batch(h)
             self._h = None
+
+    def SetPointGroup(self, lanes=0):
+        """Batched Evaluate / LogDeterminant schedule: 1 (a lane per point), 8 (maps with many
+        terms take 8 lanes per point), 0 = autotune."""
+        _check(_lib.mpart_batch_set_point_group(self._h, int(lanes)))
+
+    def EvalSchedule(self):
+        """(lanes per point or 0 before tuning, [trial ms with 1 lane, with 8 lanes])."""
+        lanes = ctypes.c_int32(0)
+        ms = (ctypes.c_double * 2)()
+        _check(_lib.mpart_batch_eval_schedule(self._h, ctypes.byref(lanes), ms))
+        return int(lanes.value), list(ms)
 
     def __len__(self):
         return len(self.maps)

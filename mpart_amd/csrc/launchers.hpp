@@ -70,6 +70,13 @@ struct BatchIO;
 hipError_t launch_batch(int var, const Launch& L, const MapDev* maps, const BatchIO* io, const int* tileMap,
                         const long* tileStart, int ntiles, unsigned* counter, int method, double xtol, double ytol);
 int occupancy_batch(int var, int block, size_t smem);
+// Batched Evaluate / LogDeterminant with a schedule per tile (tileLanes: 1, a
+// lane per point on block points, or 8 lanes per point on block / 8 points
+// with tables of grpSlots doubles).
+hipError_t launch_batch_eval_mixed(int var, const Launch& L, const MapDev* maps, const BatchIO* io,
+                                   const int* tileMap, const long* tileStart, const int* tileLanes, int ntiles,
+                                   unsigned* counter, int grpSlots);
+int occupancy_batch_eval_grp(int var, int block, size_t smem);
 
 hipError_t launch_reduce_rows(const double* in, long nrows, long K, double scale, double* out, hipStream_t s);
 hipError_t launch_reduce_cols(const double* in, long nrows, long rowStride, long ncols, double scale, double* out,

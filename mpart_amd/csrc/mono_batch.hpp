@@ -122,4 +122,68 @@ __global__ void __launch_bounds__(256) k_batch(const MapDev* maps, const BatchIO
     }
 }
 
+// Batched Evaluate / LogDeterminant with a schedule per tile: tileLanes[t] = 1
+// (blockDim points, a lane per point, the per-lane table of at most
+// laneSlots doubles) or G (blockDim / G points of a map with many terms, G
+// lanes per point as k_eval_grp, one table of at most grpSlots doubles per
+// group).  The LDS holds the larger of the two layouts.
+template <int BK, int PK, int QK, int PM, int G>
+__global__ void __launch_bounds__(256) k_batch_eval_mixed(const MapDev* maps, const BatchIO* io, const int* tileMap,
+                                                          const long* tileStart, const int* tileLanes, int ntiles,
+                                                          unsigned* counter, int grpSlots) {
+    extern __shared__ double smem[];
+    __shared__ int sTile;
+    const int tid = threadIdx.x;
+    const int gl = tid & (G - 1);
+    const int grp = tid / G;
+    while (true) {
+        if (tid == 0) sTile = (int)atomicAdd(counter, 1u);
+        __syncthreads();
+        const int t = sTile;
+        __syncthreads();
+        if (t >= ntiles) break;
+        const int mi = tileMap[t];
+        const MapDev& M = maps[mi];
+        const BatchIO& B = io[mi];
+        const bool group = tileLanes[t] > 1;  // uniform over the block
+        const long n = tileStart[t] + (group ? grp : tid);
+        if (n >= B.N) continue;
+        const double* pt = B.p + n * B.pcs;
+        double* tab;
+        int stride;
+        if (group) {
+            tab = smem + (long)grp * grpSlots;
+            stride = 1;
+            group_fill_values<BK, G>(M, M.D - 1, pt, B.prs, tab, gl);
+        } else {
+            tab = smem + tid;
+            stride = blockDim.x;
+            fill_values<BK>(M, M.D - 1, pt, B.prs, tab, stride);
+        }
+        double ldsum = 0.0;
+        for (int c = 0; c < M.ncomp; ++c) {
+            const CompDev cd = M.comps[c];
+            double C[PM + 1];
+            if (group) group_coeff_groups<PM, G>(M, cd, B.w + (long)cd.coeffOff * B.ws, B.ws, tab, gl, C);
+            else coeff_groups<PM>(M, cd, B.w + (long)cd.coeffOff * B.ws, B.ws, tab, stride, C);
+            const double xk = pt[(long)(cd.dim - 1) * B.prs];
+            DiagCoef<PM> dc;
+            make_diag<PM>(M, cd, C, xk, 0.0, dc);
+            if (B.o) {
+                AdjAcc<PM> A;
+                const double I = integrate<BK, PK, QK, FL_VAL, PM>(M.quad, dc, A, nullptr);
+                if (!group || gl == 0) B.o[(long)cd.outRow * B.ors + n * B.ocs] = I + f_at_zero<PM>(M, cd, C);
+            }
+            if (B.ld) {
+                const double z = diag_eval<BK, false, PM>(dc, xk, nullptr);
+                const double g = posfunc<PK>(z).g;
+                ldsum += (g <= 0) ? -__builtin_inf() : log_pos(g);
+            }
+        }
+        if (B.ld && (!group || gl == 0)) B.ld[n * B.lds] = ldsum;
+        // tables are rewritten for the next tile only after the block-wide
+        // barrier at the top of the loop
+    }
+}
+
 }  // namespace mpd

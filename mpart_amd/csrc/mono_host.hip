@@ -1243,10 +1243,10 @@ void do_kl_plain(mpart_map_s& m, const mpart_cview& pts, double* loss, double* g
 // Device-side arguments of one batched method (each method keeps its own, so a
 // graph can hold a sequence of different batched calls).
 struct BatchSlot {
-    DBuf dIO, dTileMap, dTileStart, dCounter;
+    DBuf dIO, dTileMap, dTileStart, dTileLanes, dCounter;
     bool valid = false;
     std::vector<mpd::BatchIO> io;
-    std::vector<int> tileMap;
+    std::vector<int> tileMap, tileLanes;
     std::vector<long> tileStart;
 };
 
@@ -1258,6 +1258,19 @@ struct mpart_batch_s {
     DBuf dMaps;
     BatchSlot slot[3];
     int occ = 0;
+    // Mixed schedule of the batched Evaluate / LogDeterminant: lanes per point
+    // of each map (8 for maps with many terms, else 1), the largest table of
+    // the 8-lane maps, the LDS bytes (0: the layout does not fit), occupancy,
+    // and the tuner's choice (1: a lane per point everywhere, 8: the mixed
+    // schedule; 0: not tuned; never changed once a call has been captured,
+    // since a captured graph reads the tile list of its schedule).
+    std::vector<int> mapLanes;
+    int grpSlots = 1;
+    size_t smemGrp = 0;
+    int occGrp = 0;
+    int evalGroup = 0;
+    double evalTrialMs[2] = {0, 0};
+    bool captured = false;
 };
 
 namespace {
@@ -1287,6 +1300,22 @@ mpart_batch_s* make_batch(const mpart_map* maps, uint32_t n) {
                       std::to_string(LDS_LIMIT / (64 * sizeof(double))) + " table slots)");
     b->block = block;
     b->smem = (size_t)maxSlots * block * sizeof(double);
+    // maps above the specialization limit, or whose per-lane table does not
+    // fit 256 lanes, take 8 lanes per point in the mixed schedule
+    int laneSlots = 1, grpSlots = 1;
+    bool anyGroup = false;
+    for (uint32_t i = 0; i < n; ++i) {
+        const bool big = maps[i]->numCoeffs >= GEN_MAX_COEFFS ||
+                         (size_t)maps[i]->dev.nslots * 256 * sizeof(double) > LDS_LIMIT;
+        b->mapLanes.push_back(big ? 8 : 1);
+        anyGroup = anyGroup || big;
+        if (big) grpSlots = std::max(grpSlots, maps[i]->dev.nslots);
+        else laneSlots = std::max(laneSlots, maps[i]->dev.nslots);
+    }
+    b->grpSlots = grpSlots;
+    const size_t mixedSmem =
+        std::max((size_t)256 * laneSlots, (size_t)(256 / 8) * grpSlots) * sizeof(double);
+    b->smemGrp = (anyGroup && mixedSmem <= LDS_LIMIT) ? mixedSmem : 0;
     upload(b->dMaps, devs);
     for (auto& sl : b->slot) sl.dCounter.ensure(256);
     return b.release();
@@ -1295,27 +1324,35 @@ mpart_batch_s* make_batch(const mpart_map* maps, uint32_t n) {
 // Uploads the call's descriptors and tile list when they differ from the ones
 // on the device, then launches.  Under stream capture the descriptors must
 // already be on the device (run the same call once before capturing).
-void run_batch(mpart_batch_s& b, int method, std::vector<mpd::BatchIO>& io, double xtol, double ytol, hipStream_t s) {
+void run_batch(mpart_batch_s& b, int method, std::vector<mpd::BatchIO>& io, double xtol, double ytol, hipStream_t s,
+               int lanes = 1) {
     const size_t n = b.maps.size();
-    // Tiles of `block` points, most expensive map first (cost ~ terms + quadrature per component).
+    // mixed schedule: Evaluate / LogDeterminant only, and only when its layout fits
+    const bool mixed = lanes > 1 && method == mpd::BM_EVAL && b.smemGrp > 0;
+    // Tiles, most expensive map first (cost ~ terms + quadrature per component);
+    // a tile has 256 / lanes points in the mixed schedule, else block points.
     std::vector<std::pair<double, std::pair<int, long>>> tiles;
     for (size_t i = 0; i < n; ++i) {
         const double cost = (double)b.maps[i]->numCoeffs + 40.0 * (double)b.maps[i]->comps.size();
-        for (long t0 = 0; t0 < io[i].N; t0 += b.block) tiles.push_back({cost, {(int)i, t0}});
+        const long tilePoints = mixed ? 256 / b.mapLanes[i] : b.block;
+        for (long t0 = 0; t0 < io[i].N; t0 += tilePoints) tiles.push_back({cost, {(int)i, t0}});
     }
     std::stable_sort(tiles.begin(), tiles.end(), [](const auto& a, const auto& c) { return a.first > c.first; });
-    std::vector<int> tileMap(tiles.size());
+    std::vector<int> tileMap(tiles.size()), tileLanes(tiles.size());
     std::vector<long> tileStart(tiles.size());
     for (size_t t = 0; t < tiles.size(); ++t) {
         tileMap[t] = tiles[t].second.first;
         tileStart[t] = tiles[t].second.second;
+        tileLanes[t] = mixed ? b.mapLanes[tileMap[t]] : 1;
     }
     BatchSlot& sl = b.slot[method];
-    const bool same = sl.valid && tileMap == sl.tileMap && tileStart == sl.tileStart && io.size() == sl.io.size() &&
+    const bool same = sl.valid && tileMap == sl.tileMap && tileStart == sl.tileStart && tileLanes == sl.tileLanes &&
+                      io.size() == sl.io.size() &&
                       std::memcmp(io.data(), sl.io.data(), io.size() * sizeof(mpd::BatchIO)) == 0;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    hip_check(hipStreamIsCapturing(s, &cs), "hipStreamIsCapturing");
+    if (cs != hipStreamCaptureStatusNone) b.captured = true;
     if (!same) {
-        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-        hip_check(hipStreamIsCapturing(s, &cs), "hipStreamIsCapturing");
         if (cs != hipStreamCaptureStatusNone)
             throw ArgError("batched call: its arguments differ from the previous call of this method on this batch; "
                            "run the call once before capturing it in a graph");
@@ -1323,9 +1360,11 @@ void run_batch(mpart_batch_s& b, int method, std::vector<mpd::BatchIO>& io, doub
         upload(sl.dIO, io);
         upload(sl.dTileMap, tileMap);
         upload(sl.dTileStart, tileStart);
+        upload(sl.dTileLanes, tileLanes);
         sl.io = io;
         sl.tileMap = tileMap;
         sl.tileStart = tileStart;
+        sl.tileLanes = tileLanes;
         sl.valid = true;
     }
     if (tiles.empty()) return;
@@ -1333,6 +1372,17 @@ void run_batch(mpart_batch_s& b, int method, std::vector<mpd::BatchIO>& io, doub
         int dev = 0;
         hip_check(hipGetDevice(&dev), "hipGetDevice");
         hip_check(hipDeviceGetAttribute(&g_numCU, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
+    }
+    if (mixed) {
+        if (b.occGrp == 0) b.occGrp = mpd::occupancy_batch_eval_grp(b.variant, 256, b.smemGrp);
+        const long grid = std::min<long>((long)tiles.size(), (long)g_numCU * b.occGrp);
+        mpd::Launch L{dim3((unsigned)grid), dim3(256), b.smemGrp, s};
+        hip_check(mpd::launch_batch_eval_mixed(b.variant, L, b.dMaps.as<MapDev>(), sl.dIO.as<mpd::BatchIO>(),
+                                               sl.dTileMap.as<int>(), sl.dTileStart.as<long>(),
+                                               sl.dTileLanes.as<int>(), (int)tiles.size(),
+                                               sl.dCounter.as<unsigned>(), b.grpSlots),
+                  "launch batch (mixed lanes per point)");
+        return;
     }
     if (b.occ == 0) b.occ = mpd::occupancy_batch(b.variant, b.block, b.smem);
     const long grid = std::min<long>((long)tiles.size(), (long)g_numCU * b.occ);
@@ -2025,6 +2075,27 @@ int mpart_batch_size(mpart_batch b, uint32_t* n) {
     });
 }
 
+constexpr long BATCH_TUNE_MIN_POINTS = 65536;
+
+int mpart_batch_set_point_group(mpart_batch b, int32_t lanes) {
+    return guarded([&] {
+        if (!b) throw ArgError("null argument");
+        if (lanes != 0 && lanes != 1 && lanes != 8) throw ArgError("lanes per point must be 0, 1 or 8");
+        if (b->captured && lanes != b->evalGroup)
+            throw ArgError("the batch has been captured in a graph: its schedule can no longer change");
+        b->evalGroup = lanes;
+    });
+}
+
+int mpart_batch_eval_schedule(mpart_batch b, int32_t* lanes, double* trialMs) {
+    return guarded([&] {
+        if (!b) throw ArgError("null argument");
+        if (lanes) *lanes = b->evalGroup;
+        if (trialMs)
+            for (int i = 0; i < 2; ++i) trialMs[i] = b->evalTrialMs[i];
+    });
+}
+
 int mpart_batch_evaluate(mpart_batch b, const mpart_cview* pts, const mpart_view* out, double* const* logdet,
                          const int64_t* logdetStride, mpart_stream s) {
     return guarded([&] {
@@ -2052,7 +2123,33 @@ int mpart_batch_evaluate(mpart_batch b, const mpart_cview* pts, const mpart_view
             }
             io.push_back(d);
         }
-        run_batch(*b, mpd::BM_EVAL, io, 0.0, 0.0, S(s));
+        // Lanes per point (1 or 8): timed on the first uncaptured call with
+        // enough points, before any capture, then kept.
+        long total = 0;
+        for (auto& d : io) total += d.N;
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        hip_check(hipStreamIsCapturing(S(s), &cs), "hipStreamIsCapturing");
+        if (b->evalGroup == 0 && !b->captured && cs == hipStreamCaptureStatusNone && b->smemGrp > 0 &&
+            total >= BATCH_TUNE_MIN_POINTS) {
+            hipEvent_t e0, e1;
+            hip_check(hipEventCreate(&e0), "hipEventCreate");
+            hip_check(hipEventCreate(&e1), "hipEventCreate");
+            const int lanes[2] = {1, 8};
+            for (int i = 0; i < 2; ++i) {
+                run_batch(*b, mpd::BM_EVAL, io, 0.0, 0.0, S(s), lanes[i]);
+                hip_check(hipEventRecord(e0, S(s)), "hipEventRecord");
+                run_batch(*b, mpd::BM_EVAL, io, 0.0, 0.0, S(s), lanes[i]);
+                hip_check(hipEventRecord(e1, S(s)), "hipEventRecord");
+                hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+                float ms = 0.f;
+                hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+                b->evalTrialMs[i] = ms;
+            }
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            b->evalGroup = b->evalTrialMs[1] < b->evalTrialMs[0] ? 8 : 1;
+        }
+        run_batch(*b, mpd::BM_EVAL, io, 0.0, 0.0, S(s), b->evalGroup > 0 ? b->evalGroup : 1);
     });
 }
 

@@ -175,3 +175,77 @@ def test_batch_rejects_mixed_options():
     c = mt.CreateTriangular(2, 2, 2, mt.MapOptions(posFuncType=mt.PosFuncTypes.Exp))
     with pytest.raises(NotImplementedError):
         mt.MapBatch([a, c])
+
+
+def test_batched_evaluate_group_schedule():
+    # The mixed schedule: maps with many terms take 8 lanes per point (a
+    # shared basis table per point, the terms split over the lanes), the
+    # others a lane per point, in one launch.  Same values as the per-map
+    # kernels up to the order of the C_q sums, checked against the oracle.
+    _case("batch_eval_group")
+    maps, refs, xs, tri = _make_batch(nmaps=12, seed=13)
+    s = Stream(14)
+    for d, p in ((11, 4), (9, 5)):  # 4367 and 5004 coefficients: 8 lanes per point
+        g = mt.CreateTriangular(d, d, p, mt.MapOptions())
+        r = orc.CreateTriangular(d, d, p, orc.options())
+        c = 0.05 * s.normal(g.numCoeffs)
+        g.SetCoeffs(c)
+        r.SetCoeffs(c)
+        maps.append(g)
+        refs.append(r)
+        xs.append(torch.from_numpy(s.points(d, 333)).cuda())
+    b = mt.MapBatch(maps)
+    b.SetPointGroup(8)
+    ys = [torch.empty((m.outputDim, x.shape[1]), dtype=torch.float64, device="cuda") for m, x in zip(maps, xs)]
+    lds = [torch.empty(x.shape[1], dtype=torch.float64, device="cuda") for x in xs]
+    b.EvaluateImpl(xs, ys, lds)
+    torch.cuda.synchronize()
+    for m, r, x, y, ld in zip(maps, refs, xs, ys, lds):
+        xn = x.cpu().numpy()
+        close(y.cpu().numpy(), r.Evaluate(xn), what="batched Evaluate, 8 lanes per point")
+        close(ld.cpu().numpy(), r.LogDeterminant(xn), what="batched LogDeterminant, 8 lanes per point")
+        close(y.cpu().numpy(), m.Evaluate(x).cpu().numpy(), rtol=1e-12, what="group vs per-map")
+    assert b.EvalSchedule()[0] == 8
+
+
+def test_batched_evaluate_tuner_and_capture_freeze():
+    # Enough points to tune (>= 65536 in all); after a captured call the
+    # schedule cannot change.
+    _case("batch_eval_tuner")
+    s = Stream(17)
+    maps, xs = [], []
+    for i in range(6):
+        d, p = 4 + 2 * i, 3 if i % 2 else 4
+        m = mt.CreateTriangular(d, d, p, mt.MapOptions())
+        m.SetCoeffs(0.1 * s.normal(m.numCoeffs))
+        m.SetCodegen(False)
+        maps.append(m)
+        xs.append(torch.from_numpy(s.points(d, 12000)).cuda())
+    b = mt.MapBatch(maps)
+    ys = [torch.empty((m.outputDim, x.shape[1]), dtype=torch.float64, device="cuda") for m, x in zip(maps, xs)]
+    lds = [torch.empty(x.shape[1], dtype=torch.float64, device="cuda") for x in xs]
+    b.EvaluateImpl(xs, ys, lds)
+    torch.cuda.synchronize()
+    lanes, trials = b.EvalSchedule()
+    print(f"[batch] tuned lanes {lanes} trials {trials}", flush=True)
+    assert lanes in (1, 8) and all(t > 0 for t in trials)
+    for m, x, y, ld in zip(maps, xs, ys, lds):
+        close(y.cpu().numpy(), m.Evaluate(x).cpu().numpy(), rtol=1e-12, what="tuned batch vs per-map")
+        close(ld.cpu().numpy(), m.LogDeterminant(x).cpu().numpy(), rtol=1e-12, what="tuned logdet vs per-map")
+    y0 = [t.clone() for t in ys]
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        b.EvaluateImpl(xs, ys, lds)
+    torch.cuda.current_stream().wait_stream(side)
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        b.EvaluateImpl(xs, ys, lds)
+    for _ in range(3):
+        for t in ys:
+            t.zero_()
+        g.replay()
+        torch.cuda.synchronize()
+        assert all(torch.equal(a, c) for a, c in zip(ys, y0))
+    with pytest.raises(ValueError):
+        b.SetPointGroup(9 - lanes)  # the other schedule
