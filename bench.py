@@ -189,7 +189,7 @@ def many_maps(steps, dmax=8, pmax=4, nmax=16384):
             "batched_graph_ms": tb, "per_map_loop_ms": tl,
             "batched_points_per_s": npts / (tb * 1e-3), "speedup_batched_vs_loop": tl / tb,
             "batched_equals_per_map_bitwise": bool(same), "max_abs_diff_batched_vs_loop": maxabs,
-            "batched_lanes_per_point": b.EvalSchedule()[0], "batched_trial_ms_lanes_1_8": b.EvalSchedule()[1],
+            "batched_schedule": b.EvalSchedule()[0], "batched_trial_ms_schedules_1_4": b.EvalSchedule()[1],
             "workload": f"Evaluate + LogDeterminant of 256 triangular maps, dims 2..{dmax}, orders 2..{pmax}, "
                         f"4096..{nmax} N(0,I) points each"}
 

@@ -293,9 +293,9 @@ int mpart_create_batch(const mpart_map* maps, uint32_t n, mpart_batch* out);
 int mpart_destroy_batch(mpart_batch batch);
 int mpart_batch_size(mpart_batch batch, uint32_t* n);
 /* Schedule of the batched Evaluate / LogDeterminant: 1 (a lane per point for
- * every map) or 8 (mixed: maps with many terms -- at least 4096 coefficients,
+ * every map) or 4 (mixed: maps with many terms -- at least 4096 coefficients,
  * or a per-lane basis table that does not fit 256 lanes of LDS -- take a
- * group of 8 lanes per point sharing its table, the others a lane per point,
+ * group of 4 lanes per point sharing its table, the others a lane per point,
  * in the same launch).  mpart_batch_set_point_group forces it (0: autotune, the
  * default: the first uncaptured call with >= 65536 points in all times both
  * and keeps the faster; the choice is frozen once a call has been captured).

@@ -664,12 +664,12 @@ class MapBatch:
             self._h = None
 
     def SetPointGroup(self, lanes=0):
-        """Batched Evaluate / LogDeterminant schedule: 1 (a lane per point), 8 (maps with many
-        terms take 8 lanes per point), 0 = autotune."""
+        """Batched Evaluate / LogDeterminant schedule: 1 (a lane per point), 4 (maps with many
+        terms take 4 lanes per point), 0 = autotune."""
         _check(_lib.mpart_batch_set_point_group(self._h, int(lanes)))
 
     def EvalSchedule(self):
-        """(lanes per point or 0 before tuning, [trial ms with 1 lane, with 8 lanes])."""
+        """(schedule: 1, 4, or 0 before tuning; [trial ms of schedule 1, of schedule 4])."""
         lanes = ctypes.c_int32(0)
         ms = (ctypes.c_double * 2)()
         _check(_lib.mpart_batch_eval_schedule(self._h, ctypes.byref(lanes), ms))

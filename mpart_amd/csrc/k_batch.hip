@@ -36,7 +36,7 @@ hipError_t launch_batch(int var, const Launch& L, const MapDev* maps, const Batc
 template <int B, int Q, int R, int PM>
 static int batch_grp_occupancy(int block, size_t smem) {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch_eval_mixed<B, Q, R, PM, 8>, block, smem) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch_eval_mixed<B, Q, R, PM, 4>, block, smem) !=
         hipSuccess)
         return 1;
     return nb > 0 ? nb : 1;
@@ -57,7 +57,7 @@ hipError_t launch_batch_eval_mixed(int var, const Launch& L, const MapDev* maps,
                                    unsigned* counter, int grpSlots) {
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, L.stream, counter);
 #define CALL(B, Q, R, PM)                                                                                       \
-    hipLaunchKernelGGL((k_batch_eval_mixed<B, Q, R, PM, 8>), L.grid, L.block, L.smem, L.stream, maps, io, tileMap, \
+    hipLaunchKernelGGL((k_batch_eval_mixed<B, Q, R, PM, 4>), L.grid, L.block, L.smem, L.stream, maps, io, tileMap, \
                        tileStart, tileLanes, ntiles, counter, grpSlots)
     MPD_DISPATCH(var, CALL)
 #undef CALL

@@ -71,7 +71,7 @@ hipError_t launch_batch(int var, const Launch& L, const MapDev* maps, const Batc
                         const long* tileStart, int ntiles, unsigned* counter, int method, double xtol, double ytol);
 int occupancy_batch(int var, int block, size_t smem);
 // Batched Evaluate / LogDeterminant with a schedule per tile (tileLanes: 1, a
-// lane per point on block points, or 8 lanes per point on block / 8 points
+// lane per point on block points, or 4 lanes per point on block / 4 points
 // with tables of grpSlots doubles).
 hipError_t launch_batch_eval_mixed(int var, const Launch& L, const MapDev* maps, const BatchIO* io,
                                    const int* tileMap, const long* tileStart, const int* tileLanes, int ntiles,

@@ -1259,9 +1259,9 @@ struct mpart_batch_s {
     BatchSlot slot[3];
     int occ = 0;
     // Mixed schedule of the batched Evaluate / LogDeterminant: lanes per point
-    // of each map (8 for maps with many terms, else 1), the largest table of
-    // the 8-lane maps, the LDS bytes (0: the layout does not fit), occupancy,
-    // and the tuner's choice (1: a lane per point everywhere, 8: the mixed
+    // of each map (4 for maps with many terms, else 1), the largest table of
+    // the 4-lane maps, the LDS bytes (0: the layout does not fit), occupancy,
+    // and the tuner's choice (1: a lane per point everywhere, 4: the mixed
     // schedule; 0: not tuned; never changed once a call has been captured,
     // since a captured graph reads the tile list of its schedule).
     std::vector<int> mapLanes;
@@ -1301,20 +1301,20 @@ mpart_batch_s* make_batch(const mpart_map* maps, uint32_t n) {
     b->block = block;
     b->smem = (size_t)maxSlots * block * sizeof(double);
     // maps above the specialization limit, or whose per-lane table does not
-    // fit 256 lanes, take 8 lanes per point in the mixed schedule
+    // fit 256 lanes, take 4 lanes per point in the mixed schedule
     int laneSlots = 1, grpSlots = 1;
     bool anyGroup = false;
     for (uint32_t i = 0; i < n; ++i) {
         const bool big = maps[i]->numCoeffs >= GEN_MAX_COEFFS ||
                          (size_t)maps[i]->dev.nslots * 256 * sizeof(double) > LDS_LIMIT;
-        b->mapLanes.push_back(big ? 8 : 1);
+        b->mapLanes.push_back(big ? 4 : 1);
         anyGroup = anyGroup || big;
         if (big) grpSlots = std::max(grpSlots, maps[i]->dev.nslots);
         else laneSlots = std::max(laneSlots, maps[i]->dev.nslots);
     }
     b->grpSlots = grpSlots;
     const size_t mixedSmem =
-        std::max((size_t)256 * laneSlots, (size_t)(256 / 8) * grpSlots) * sizeof(double);
+        std::max((size_t)256 * laneSlots, (size_t)(256 / 4) * grpSlots) * sizeof(double);
     b->smemGrp = (anyGroup && mixedSmem <= LDS_LIMIT) ? mixedSmem : 0;
     upload(b->dMaps, devs);
     for (auto& sl : b->slot) sl.dCounter.ensure(256);
@@ -2080,7 +2080,7 @@ constexpr long BATCH_TUNE_MIN_POINTS = 65536;
 int mpart_batch_set_point_group(mpart_batch b, int32_t lanes) {
     return guarded([&] {
         if (!b) throw ArgError("null argument");
-        if (lanes != 0 && lanes != 1 && lanes != 8) throw ArgError("lanes per point must be 0, 1 or 8");
+        if (lanes != 0 && lanes != 1 && lanes != 4) throw ArgError("lanes per point must be 0, 1 or 4");
         if (b->captured && lanes != b->evalGroup)
             throw ArgError("the batch has been captured in a graph: its schedule can no longer change");
         b->evalGroup = lanes;
@@ -2134,7 +2134,7 @@ int mpart_batch_evaluate(mpart_batch b, const mpart_cview* pts, const mpart_view
             hipEvent_t e0, e1;
             hip_check(hipEventCreate(&e0), "hipEventCreate");
             hip_check(hipEventCreate(&e1), "hipEventCreate");
-            const int lanes[2] = {1, 8};
+            const int lanes[2] = {1, 4};
             for (int i = 0; i < 2; ++i) {
                 run_batch(*b, mpd::BM_EVAL, io, 0.0, 0.0, S(s), lanes[i]);
                 hip_check(hipEventRecord(e0, S(s)), "hipEventRecord");
@@ -2147,7 +2147,7 @@ int mpart_batch_evaluate(mpart_batch b, const mpart_cview* pts, const mpart_view
             }
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);
-            b->evalGroup = b->evalTrialMs[1] < b->evalTrialMs[0] ? 8 : 1;
+            b->evalGroup = b->evalTrialMs[1] < b->evalTrialMs[0] ? 4 : 1;
         }
         run_batch(*b, mpd::BM_EVAL, io, 0.0, 0.0, S(s), b->evalGroup > 0 ? b->evalGroup : 1);
     });

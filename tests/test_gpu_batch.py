@@ -178,14 +178,14 @@ def test_batch_rejects_mixed_options():
 
 
 def test_batched_evaluate_group_schedule():
-    # The mixed schedule: maps with many terms take 8 lanes per point (a
+    # The mixed schedule: maps with many terms take 4 lanes per point (a
     # shared basis table per point, the terms split over the lanes), the
     # others a lane per point, in one launch.  Same values as the per-map
     # kernels up to the order of the C_q sums, checked against the oracle.
     _case("batch_eval_group")
     maps, refs, xs, tri = _make_batch(nmaps=12, seed=13)
     s = Stream(14)
-    for d, p in ((11, 4), (9, 5)):  # 4367 and 5004 coefficients: 8 lanes per point
+    for d, p in ((11, 4), (9, 5)):  # 4367 and 5004 coefficients: 4 lanes per point
         g = mt.CreateTriangular(d, d, p, mt.MapOptions())
         r = orc.CreateTriangular(d, d, p, orc.options())
         c = 0.05 * s.normal(g.numCoeffs)
@@ -195,17 +195,17 @@ def test_batched_evaluate_group_schedule():
         refs.append(r)
         xs.append(torch.from_numpy(s.points(d, 333)).cuda())
     b = mt.MapBatch(maps)
-    b.SetPointGroup(8)
+    b.SetPointGroup(4)
     ys = [torch.empty((m.outputDim, x.shape[1]), dtype=torch.float64, device="cuda") for m, x in zip(maps, xs)]
     lds = [torch.empty(x.shape[1], dtype=torch.float64, device="cuda") for x in xs]
     b.EvaluateImpl(xs, ys, lds)
     torch.cuda.synchronize()
     for m, r, x, y, ld in zip(maps, refs, xs, ys, lds):
         xn = x.cpu().numpy()
-        close(y.cpu().numpy(), r.Evaluate(xn), what="batched Evaluate, 8 lanes per point")
-        close(ld.cpu().numpy(), r.LogDeterminant(xn), what="batched LogDeterminant, 8 lanes per point")
+        close(y.cpu().numpy(), r.Evaluate(xn), what="batched Evaluate, mixed schedule")
+        close(ld.cpu().numpy(), r.LogDeterminant(xn), what="batched LogDeterminant, mixed schedule")
         close(y.cpu().numpy(), m.Evaluate(x).cpu().numpy(), rtol=1e-12, what="group vs per-map")
-    assert b.EvalSchedule()[0] == 8
+    assert b.EvalSchedule()[0] == 4
 
 
 def test_batched_evaluate_tuner_and_capture_freeze():
@@ -228,7 +228,7 @@ def test_batched_evaluate_tuner_and_capture_freeze():
     torch.cuda.synchronize()
     lanes, trials = b.EvalSchedule()
     print(f"[batch] tuned lanes {lanes} trials {trials}", flush=True)
-    assert lanes in (1, 8) and all(t > 0 for t in trials)
+    assert lanes in (1, 4) and all(t > 0 for t in trials)
     for m, x, y, ld in zip(maps, xs, ys, lds):
         close(y.cpu().numpy(), m.Evaluate(x).cpu().numpy(), rtol=1e-12, what="tuned batch vs per-map")
         close(ld.cpu().numpy(), m.LogDeterminant(x).cpu().numpy(), rtol=1e-12, what="tuned logdet vs per-map")
@@ -248,4 +248,4 @@ def test_batched_evaluate_tuner_and_capture_freeze():
         torch.cuda.synchronize()
         assert all(torch.equal(a, c) for a, c in zip(ys, y0))
     with pytest.raises(ValueError):
-        b.SetPointGroup(9 - lanes)  # the other schedule
+        b.SetPointGroup(5 - lanes)  # the other schedule
