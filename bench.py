@@ -366,7 +366,7 @@ def uneven_map_workload(steps):
     out["fused_vs_plain_20000"] = out["kl_plain_20000_ms"] / out["kl_fused_20000_ms"]
     lanes, trials = g.EvalSchedule()
     out["lanes_per_point"] = {"evaluate": lanes[0], "log_determinant": lanes[1], "fused": lanes[2],
-                              "trial_ms_lanes_1_8_64": trials}
+                              "trial_ms_lanes_1_4_8_64": trials}
     return out
 
 

@@ -318,13 +318,13 @@ const char* mpart_codegen_error(mpart_map map);
 
 /* Point schedule of the generic (non-specialized) Evaluate / LogDeterminant
  * kernels: lanes per point, 1 (a lane per point, the basis table per lane),
- * 8 or 64 (a group of lanes per point sharing one table and splitting the
+ * 4, 8 or 64 (a group of lanes per point sharing one table and splitting the
  * terms).  mpart_set_point_group forces it (0: autotune, the default: the
- * first call with >= 16384 points times the three and keeps the fastest).
+ * first call with >= 16384 points times the four and keeps the fastest).
  * The generic fused KL / VJP pass has the same three schedules and its own
  * tuning.  mpart_eval_schedule reports the choice for [Evaluate,
  * LogDeterminant only, fused pass] in lanes[3] (0: not tuned yet) and the
- * trial times in ms, three per kind, in trialMs[9].  No
+ * trial times in ms, four per kind (lanes 1, 4, 8, 64), in trialMs[12].  No
  * counterpart in the reference, which runs a thread per point
  * (MParT/Utilities/KokkosHelpers.h:17-28). */
 int mpart_set_point_group(mpart_map map, int32_t lanes);

@@ -430,16 +430,16 @@ class ConditionalMap:
         return bool(spec.value), float(secs.value), _lib.mpart_codegen_error(self._h).decode()
 
     def SetPointGroup(self, lanes=0):
-        """Lanes per point of the generic Evaluate / LogDeterminant / fused pass (1, 8, 64; 0 = autotune)."""
+        """Lanes per point of the generic Evaluate / LogDeterminant / fused pass (1, 4, 8, 64; 0 = autotune)."""
         _check(_lib.mpart_set_point_group(self._h, int(lanes)))
 
     def EvalSchedule(self):
-        """((lanes for Evaluate, LogDeterminant, fused pass), trial ms for lanes 1, 8, 64 of each)
+        """((lanes for Evaluate, LogDeterminant, fused pass), trial ms for lanes 1, 4, 8, 64 of each)
         of the generic kernels (0: not tuned)."""
         lanes = (ctypes.c_int32 * 3)()
-        ms = (ctypes.c_double * 9)()
+        ms = (ctypes.c_double * 12)()
         _check(_lib.mpart_eval_schedule(self._h, lanes, ms))
-        return (lanes[0], lanes[1], lanes[2]), [list(ms[0:3]), list(ms[3:6]), list(ms[6:9])]
+        return (lanes[0], lanes[1], lanes[2]), [list(ms[0:4]), list(ms[4:8]), list(ms[8:12])]
 
     def FusedSchedule(self):
         """(schedule index or -1 before tuning, trial times in ms) of the fused kernel."""

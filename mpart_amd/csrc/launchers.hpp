@@ -19,7 +19,7 @@ struct Launch {
 hipError_t launch_eval(int var, const Launch& L, const MapDev& M, const double* w, long ws, const double* pts, long prs,
                        long pcs, long N, double* out, long ors, long ocs, double* ld, long lds, int ldAcc, int* nevals,
                        int cb, int ce, double* gtab);
-// Group-per-point Evaluate / LogDeterminant (G = 8 or 64 lanes per point;
+// Group-per-point Evaluate / LogDeterminant (G = 4, 8 or 64 lanes per point;
 // L.smem = block / G tables of nslots doubles).
 hipError_t launch_eval_grp(int var, int G, const Launch& L, const MapDev& M, const double* w, long ws,
                            const double* pts, long prs, long pcs, long N, double* out, long ors, long ocs, double* ld,

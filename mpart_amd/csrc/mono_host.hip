@@ -219,13 +219,13 @@ struct mpart_map_s {
     bool genW3Spills = false;  // the three-waves kernel needs more private memory (register spills)
     int fusedChoice = -1;  // tuned schedule of the fused kernel (-1: not yet)
     // Point schedule of the generic Evaluate / LogDeterminant: lanes per point
-    // (1, 8 or 64) for [Evaluate (with or without log-det), log-det only];
+    // (1, 4, 8 or 64) for [Evaluate (with or without log-det), log-det only];
     // 0: not tuned yet.  pointGroupForced > 0 overrides the tuner.
     int evalGroup[2] = {0, 0};
-    double evalTrialMs[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    double evalTrialMs[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
     int pointGroupForced = 0;
     int fusedGroup = 0;  // the same for the generic fused pass
-    double fusedGroupTrialMs[3] = {0, 0, 0};
+    double fusedGroupTrialMs[4] = {0, 0, 0, 0};
     double fusedTrialMs[4] = {0, 0, 0, 0};
 };
 
@@ -635,9 +635,9 @@ void gen_eval(mpart_map_s& m, mpg::GenModule* g, const double* w, const mpart_cv
 constexpr long EVAL_TUNE_MIN_POINTS = 16384;
 
 // Generic Evaluate and/or LogDeterminant of components [cb, ce) with the
-// map's point schedule: a lane per point (k_eval) or a group of 8 or 64 lanes
-// per point (k_eval_grp).  The first call with enough points (outputs
-// overwritten, not under stream capture) times the three schedules with HIP
+// map's point schedule: a lane per point (k_eval) or a group of 4, 8 or 64
+// lanes per point (k_eval_grp).  The first call with enough points (outputs
+// overwritten, not under stream capture) times the four schedules with HIP
 // events and keeps the fastest for this kind of call.
 void generic_eval(mpart_map_s& m, const mpart_cview& p, double* out, long ors, long ocs, double* ld, long lds,
                   int ldAcc, int32_t* nevals, int cb, int ce, hipStream_t s) {
@@ -662,7 +662,7 @@ void generic_eval(mpart_map_s& m, const mpart_cview& p, double* out, long ors, l
                   "launch evaluate");
     };
     const int kind = out ? 0 : 1;
-    static const int kGroups[3] = {1, 8, 64};
+    static const int kGroups[4] = {1, 4, 8, 64};
     if (m.pointGroupForced > 0) {
         run(m.pointGroupForced);
         return;
@@ -675,7 +675,7 @@ void generic_eval(mpart_map_s& m, const mpart_cview& p, double* out, long ors, l
             hip_check(hipEventCreate(&e0), "hipEventCreate");
             hip_check(hipEventCreate(&e1), "hipEventCreate");
             int best = 0;
-            for (int i = 0; i < 3; ++i) {
+            for (int i = 0; i < 4; ++i) {
                 run(kGroups[i]);
                 hip_check(hipEventRecord(e0, s), "hipEventRecord");
                 run(kGroups[i]);
@@ -1089,7 +1089,7 @@ void do_fused(mpart_map_s& m, const mpart_cview& p, const double* sens, long srs
         }
         return;
     }
-    // Generic kernels: a lane per point (k_fused) or a group of 8 or 64 lanes
+    // Generic kernels: a lane per point (k_fused) or a group of 4, 8 or 64 lanes
     // per point (k_fused_grp), tuned like the generic Evaluate.
     auto run_lane = [&]() {
         int block = 256;
@@ -1162,7 +1162,7 @@ void do_fused(mpart_map_s& m, const mpart_cview& p, const double* sens, long srs
         if (G > 1 && run_grp(G)) return;
         run_lane();
     };
-    static const int kGroups[3] = {1, 8, 64};
+    static const int kGroups[4] = {1, 4, 8, 64};
     if (m.pointGroupForced > 0) {
         run(m.pointGroupForced);
     } else {
@@ -1174,7 +1174,7 @@ void do_fused(mpart_map_s& m, const mpart_cview& p, const double* sens, long srs
                 hip_check(hipEventCreate(&e0), "hipEventCreate");
                 hip_check(hipEventCreate(&e1), "hipEventCreate");
                 int best = 0;
-                for (int i = 0; i < 3; ++i) {
+                for (int i = 0; i < 4; ++i) {
                     run(kGroups[i]);
                     hip_check(hipEventRecord(e0, s), "hipEventRecord");
                     run(kGroups[i]);
@@ -2029,7 +2029,8 @@ const char* mpart_codegen_error(mpart_map m) { return m ? m->genErr.c_str() : "n
 int mpart_set_point_group(mpart_map m, int32_t lanes) {
     return guarded([&] {
         check_map(m);
-        if (lanes != 0 && lanes != 1 && lanes != 8 && lanes != 64) throw ArgError("lanes per point must be 0, 1, 8 or 64");
+        if (lanes != 0 && lanes != 1 && lanes != 4 && lanes != 8 && lanes != 64)
+            throw ArgError("lanes per point must be 0, 1, 4, 8 or 64");
         m->pointGroupForced = lanes;
     });
 }
@@ -2040,11 +2041,11 @@ int mpart_eval_schedule(mpart_map m, int32_t* lanes, double* trialMs) {
         for (int k = 0; k < 2; ++k) {
             if (lanes) lanes[k] = m->evalGroup[k];
             if (trialMs)
-                for (int i = 0; i < 3; ++i) trialMs[k * 3 + i] = m->evalTrialMs[k][i];
+                for (int i = 0; i < 4; ++i) trialMs[k * 4 + i] = m->evalTrialMs[k][i];
         }
         if (lanes) lanes[2] = m->fusedGroup;
         if (trialMs)
-            for (int i = 0; i < 3; ++i) trialMs[6 + i] = m->fusedGroupTrialMs[i];
+            for (int i = 0; i < 4; ++i) trialMs[8 + i] = m->fusedGroupTrialMs[i];
     });
 }
 

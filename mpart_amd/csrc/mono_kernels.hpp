@@ -83,7 +83,7 @@ __global__ void __launch_bounds__(256) k_eval(MapDev M, const double* w, long ws
 
 // ---------------------------------------------------------------------------
 // Group-per-point schedule for maps with many terms per component: G lanes
-// (G = 8 or 64) share one point.  The point's basis table sits once in LDS
+// (G = 4, 8 or 64) share one point.  The point's basis table sits once in LDS
 // (filled by the group, one coordinate per lane), the lanes split each
 // group of terms (C_q by a butterfly sum, bitwise the same in every lane of
 // the group), and every lane of the group runs the same quadrature, so the
