@@ -266,6 +266,10 @@ def many_maps(steps, dmax=8, pmax=4, nmax=16384, coeff_grad=True, inverse=True):
         res["sequence"]["inverse_roundtrip_max_abs_err_finite"] = max(e.max().item() if e.numel() else 0.0 for e in ex)
         res["sequence"]["inverse_points_err_above_1e-6"] = int(sum((e > 1e-6).sum().item() for e in ex))
         res["sequence"]["inverse_points_err_above_1e-3"] = int(sum((e > 1e-3).sum().item() for e in ex))
+        sched, trial = b.InverseSchedule()
+        res["sequence"]["inverse_schedule"] = {"choice": sched, "trial_ms_schedules_1_2": trial,
+                                               "meaning": "1: per-lane solve scheduling, one pass per component "
+                                                          "index; 2: a lane per point, all components in one launch"}
     log(f"many maps sequence ({res['sequence']['methods']}): batched graph {ts:.3f} ms, per-map loop {tls:.3f} ms, "
         f"identical {seq_same}; {json.dumps({k: v for k, v in res['sequence'].items() if k.startswith(('inverse', 'coeff'))})}")
     return res

@@ -311,6 +311,19 @@ int mpart_batch_coeff_grad(mpart_batch batch, const mpart_cview* pts, const mpar
                            mpart_stream stream);
 int mpart_batch_inverse(mpart_batch batch, const mpart_cview* x1, const mpart_cview* r, const mpart_view* out,
                         double xtol, double ytol, mpart_stream stream);
+/* Schedule of the batched Inverse: 1 runs, component by component, one setup
+ * pass and one persistent solve pass over the points of every map, lanes
+ * taking new points as their solves end; 2 solves each point to the end in
+ * its lane (a lane per point, all components in one launch).  0 (the
+ * default) autotunes: the first uncaptured call with >= 65536 points in all
+ * times both and keeps the faster (until then 2; frozen once an Inverse call
+ * of the batch has been captured).  Both take the reference's steps (MonotoneComponent::InverseImpl,
+ * MParT/MonotoneComponent.h:331-443); their roots can differ in the last bits
+ * where the compiled arithmetic differs.  mpart_batch_inverse_schedule reports
+ * the schedule (0: not tuned yet) and the trial times in ms of 1 and 2.
+ * No counterpart in the reference. */
+int mpart_batch_set_inverse_schedule(mpart_batch batch, int32_t schedule);
+int mpart_batch_inverse_schedule(mpart_batch batch, int32_t* schedule, double* trialMs);
 
 int mpart_set_codegen(mpart_map map, int enable);
 int mpart_codegen_prepare(mpart_map map, int32_t* specialized, double* compileSeconds);

@@ -117,6 +117,8 @@ _SIGS = {
     "mpart_batch_coeff_grad": [_vp, _vp, _vp, _vp, _vp],
     "mpart_batch_inverse": [_vp, _vp, _vp, _vp, _dbl, _dbl, _vp],
     "mpart_batch_set_point_group": [_vp, ctypes.c_int32],
+    "mpart_batch_set_inverse_schedule": [_vp, ctypes.c_int32],
+    "mpart_batch_inverse_schedule": [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)],
     "mpart_batch_eval_schedule": [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)],
     "mpart_codegen_prepare": [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)],
     "mpart_fused_schedule": [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)],
@@ -667,6 +669,18 @@ class MapBatch:
         """Batched Evaluate / LogDeterminant schedule: 1 (a lane per point), 4 (maps with many
         terms take 4 lanes per point), 0 = autotune."""
         _check(_lib.mpart_batch_set_point_group(self._h, int(lanes)))
+
+    def SetInverseSchedule(self, schedule=0):
+        """Batched Inverse schedule: 1 per-lane solve scheduling over every map's
+        points, component by component; 2 a lane per point; 0 autotune."""
+        _check(_lib.mpart_batch_set_inverse_schedule(self._h, int(schedule)))
+
+    def InverseSchedule(self):
+        """(schedule: 1, 2, or 0 before tuning; [trial ms of schedule 1, of schedule 2])."""
+        sched = ctypes.c_int32(0)
+        ms = (ctypes.c_double * 2)()
+        _check(_lib.mpart_batch_inverse_schedule(self._h, ctypes.byref(sched), ms))
+        return int(sched.value), list(ms)
 
     def EvalSchedule(self):
         """(schedule: 1, 4, or 0 before tuning; [trial ms of schedule 1, of schedule 4])."""

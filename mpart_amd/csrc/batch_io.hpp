@@ -5,6 +5,7 @@
 namespace mpd {
 
 constexpr int BATCH_MAX_DIM = 64;  // inverse keeps a point in a per-lane array
+constexpr int INV_CHUNK = 64;      // point indices the inverse queue takes from its counter at a time
 
 enum BatchMethod { BM_EVAL = 0, BM_COEFF_GRAD = 1, BM_INVERSE = 2 };
 
@@ -23,6 +24,26 @@ struct BatchIO {
     long lds;
     const double* s;  // CoeffGrad sensitivities (outDim x N) or Inverse targets r (outDim x N)
     long srs, scs;
+};
+
+// One map's share of a batched Inverse pass over component c (device copy).
+// The pass's points are numbered in one index space in which every map
+// starts at a multiple of the queue chunk, so a chunk never spans two maps.
+struct InvSeg {
+    int map;          // index into the batch's MapDev array
+    int p;            // last-coordinate order of component c
+    long base;        // first index of this map in the pass's index space
+    long N;           // points of this map
+    const double* w;  // coefficients
+    long ws;
+    const double* x;  // inputs of component c (the last row holds the initial guess)
+    long xrows, xcols, xrs, xcs;
+    const double* y;  // targets r of component c
+    long yss;
+    double* o;        // root of component c, point n at o[n * os]
+    long os;
+    double* o2;       // second copy of the root (the caller's output) or null
+    long os2;
 };
 
 }  // namespace mpd

@@ -8,7 +8,7 @@ __global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
 template <int B, int Q, int R, int PM>
 static int inverse_occupancy(int block) {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_inverse_queue<B, Q, R, PM, false>, block, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_inverse_queue<B, Q, R, PM, false, false>, block, 0) !=
         hipSuccess)
         return 1;
     return nb > 0 ? nb : 1;
@@ -43,11 +43,11 @@ hipError_t launch_inverse(int var, const Launch& L, dim3 qgrid, dim3 qblock, con
     hipLaunchKernelGGL((k_inverse_setup<B, Q, R, PM>), L.grid, L.block, L.smem, L.stream, M, w, ws, xs, xrows, xcols, \
                        xrs, xcs, ys, yss, N, c, gtab, rec);                                                        \
     if (M.lin)                                                                                                   \
-        hipLaunchKernelGGL((k_inverse_queue<B, Q, R, PM, true>), qgrid, qblock, 0, L.stream, M, rec, N, out, os, c, \
-                           xtol, ytol, counter);                                                                 \
+        hipLaunchKernelGGL((k_inverse_queue<B, Q, R, PM, true, false>), qgrid, qblock, 0, L.stream, M, rec, N, out, \
+                           os, c, xtol, ytol, counter, nullptr, nullptr);                                        \
     else                                                                                                         \
-        hipLaunchKernelGGL((k_inverse_queue<B, Q, R, PM, false>), qgrid, qblock, 0, L.stream, M, rec, N, out, os, \
-                           c, xtol, ytol, counter)
+        hipLaunchKernelGGL((k_inverse_queue<B, Q, R, PM, false, false>), qgrid, qblock, 0, L.stream, M, rec, N, out, \
+                           os, c, xtol, ytol, counter, nullptr, nullptr)
     MPD_DISPATCH(var, CALL)
 #undef CALL
     return hipGetLastError();

@@ -1,0 +1,47 @@
+#include "launchers.hpp"
+#include "mono_inverse.hpp"
+
+namespace mpd {
+
+__global__ void k_zero_u64_batch(unsigned long long* p) { *p = 0ull; }
+
+template <int B, int Q, int R, int PM>
+static int inverse_batch_occupancy(int block) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_inverse_queue<B, Q, R, PM, false, true>, block, 0) !=
+        hipSuccess)
+        return 1;
+    return nb > 0 ? nb : 1;
+}
+
+int occupancy_inverse_batch(int var, int block) {
+#define CALL(B, Q, R, PM) return inverse_batch_occupancy<B, Q, R, PM>(block)
+    switch (var) {
+        MPD_CASES_PM(0, 4, CALL)
+        MPD_CASES_PM(1, 12, CALL)
+        default: return 1;
+    }
+#undef CALL
+}
+
+hipError_t launch_inverse_batch(int var, const Launch& L, int ntiles, dim3 qgrid, dim3 qblock, const MapDev& M0,
+                                const MapDev* maps, const InvSeg* segs, const int* tileSeg, const long* tileStart,
+                                const int* chunkSeg, long Ntot, int c, double xtol, double ytol, double* rec,
+                                unsigned long long* counter) {
+    if (ntiles <= 0 || Ntot <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_zero_u64_batch, dim3(1), dim3(1), 0, L.stream, counter);
+#define CALL(B, Q, R, PM)                                                                                           \
+    hipLaunchKernelGGL((k_inverse_setup_batch<B, Q, R, PM>), dim3((unsigned)ntiles), L.block, L.smem, L.stream,     \
+                       maps, segs, tileSeg, tileStart, Ntot, c, rec);                                               \
+    if (M0.lin)                                                                                                   \
+        hipLaunchKernelGGL((k_inverse_queue<B, Q, R, PM, true, true>), qgrid, qblock, 0, L.stream, M0, rec, Ntot,   \
+                           nullptr, 0, c, xtol, ytol, counter, segs, chunkSeg);                                   \
+    else                                                                                                          \
+        hipLaunchKernelGGL((k_inverse_queue<B, Q, R, PM, false, true>), qgrid, qblock, 0, L.stream, M0, rec, Ntot,  \
+                           nullptr, 0, c, xtol, ytol, counter, segs, chunkSeg)
+    MPD_DISPATCH(var, CALL)
+#undef CALL
+    return hipGetLastError();
+}
+
+}  // namespace mpd

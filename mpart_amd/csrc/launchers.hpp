@@ -60,6 +60,15 @@ hipError_t launch_inverse(int var, const Launch& L, dim3 qgrid, dim3 qblock, con
                           unsigned long long* counter);
 int occupancy_inverse(int var, int block);
 int inverse_record_rows(int var);
+// Batched Inverse of component c over many maps: a counter reset, the setup
+// pass (ntiles blocks of L.block points, L.smem bytes of basis tables), then
+// the persistent solve pass over the Ntot indices of the segments.
+struct InvSeg;
+hipError_t launch_inverse_batch(int var, const Launch& L, int ntiles, dim3 qgrid, dim3 qblock, const MapDev& M0,
+                                const MapDev* maps, const InvSeg* segs, const int* tileSeg, const long* tileStart,
+                                const int* chunkSeg, long Ntot, int c, double xtol, double ytol, double* rec,
+                                unsigned long long* counter);
+int occupancy_inverse_batch(int var, int block);
 hipError_t launch_fused(int var, const Launch& L, const MapDev& M, const double* w, long ws, const double* pts,
                         long prs, long pcs, long N, const double* sens, long srs, long scs, const double* lsens,
                         long lss, int klMode, double* partialGrad, double* partialLoss, double* gtab,

@@ -1250,6 +1250,22 @@ struct BatchSlot {
     std::vector<long> tileStart;
 };
 
+// Batched Inverse: per component index c, one pass over the maps that have a
+// component c (segments), with its setup tiles and queue chunks; built and
+// uploaded by a call whose arguments differ from the previous one.
+struct InvPlan {
+    struct Pass {
+        long seg0, nseg, tile0, ntiles, chunk0, Ntot;
+    };
+    bool valid = false;
+    std::vector<mpd::BatchIO> io;
+    std::vector<Pass> passes;
+    std::vector<long> fullOff;  // per map: offset of its point buffer (-1: none)
+    long fullElems = 0;
+    long maxNtot = 0;
+    DBuf dSegs, dTileSeg, dTileStart, dChunkSeg, dCounters, wFull, wRec;
+};
+
 struct mpart_batch_s {
     std::vector<mpart_map> maps;
     int variant = 0;
@@ -1264,6 +1280,13 @@ struct mpart_batch_s {
     // and the tuner's choice (1: a lane per point everywhere, 4: the mixed
     // schedule; 0: not tuned; never changed once a call has been captured,
     // since a captured graph reads the tile list of its schedule).
+    InvPlan inv;
+    // Batched Inverse schedule: 1 per-lane solve scheduling (component passes),
+    // 2 a lane per point; 0 autotune (timed on the first uncaptured call with
+    // enough points; untuned calls take 2).  Frozen once a call is captured.
+    int invSchedule = 0;
+    double invTrialMs[2] = {0, 0};
+    bool invCaptured = false;  // an Inverse call has been captured (its schedule is frozen)
     std::vector<int> mapLanes;
     int grpSlots = 1;
     size_t smemGrp = 0;
@@ -1391,6 +1414,165 @@ void run_batch(mpart_batch_s& b, int method, std::vector<mpd::BatchIO>& io, doub
                                 sl.dTileStart.as<long>(), (int)tiles.size(), sl.dCounter.as<unsigned>(), method, xtol,
                                 ytol),
               "launch batch");
+}
+
+// Batched Inverse with per-lane solve scheduling (the single-map queue solver
+// run over the points of all maps): component by component, a setup pass and
+// a persistent solve pass over every map that has that component.  A
+// triangular map solves into a zero-initialized point buffer (its x1 rows
+// copied in) as TriangularMap::InverseImpl does (src/TriangularMap.cpp:147-184);
+// each root is written there and to the caller's output.  The per-point
+// arithmetic is that of the single-map path, so the results are equal bit for
+// bit.
+void batch_inverse(mpart_batch_s& b, const std::vector<mpd::BatchIO>& io, double xtol, double ytol, hipStream_t s) {
+    InvPlan& P = b.inv;
+    const size_t nm = b.maps.size();
+    const bool same = P.valid && io.size() == P.io.size() &&
+                      std::memcmp(io.data(), P.io.data(), io.size() * sizeof(mpd::BatchIO)) == 0;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    hip_check(hipStreamIsCapturing(s, &cs), "hipStreamIsCapturing");
+    if (cs != hipStreamCaptureStatusNone) b.captured = true;
+    if (!same) {
+        if (cs != hipStreamCaptureStatusNone)
+            throw ArgError("batched call: its arguments differ from the previous call of this method on this batch; "
+                           "run the call once before capturing it in a graph");
+        hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+        P.valid = false;
+        P.passes.clear();
+        P.fullOff.assign(nm, -1);
+        P.fullElems = 0;
+        P.maxNtot = 0;
+        size_t ncompMax = 0;
+        for (size_t i = 0; i < nm; ++i) {
+            ncompMax = std::max(ncompMax, b.maps[i]->comps.size());
+            if (b.maps[i]->triangular && io[i].N > 0) {
+                P.fullOff[i] = P.fullElems;
+                P.fullElems += (long)b.maps[i]->inDim * io[i].N;
+            }
+        }
+        std::vector<mpd::InvSeg> segs;
+        std::vector<long> xOff, oOff;  // per segment: offsets in the point buffers (-1: caller memory)
+        std::vector<int> tileSeg, chunkSeg;
+        std::vector<long> tileStart;
+        const long tileN = b.block;
+        for (size_t c = 0; c < ncompMax; ++c) {
+            InvPlan::Pass ps{(long)segs.size(), 0, (long)tileSeg.size(), 0, (long)chunkSeg.size(), 0};
+            for (size_t i = 0; i < nm; ++i) {
+                mpart_map m = b.maps[i];
+                const mpd::BatchIO& d = io[i];
+                if (c >= m->comps.size() || d.N <= 0) continue;
+                const MultiSet& ms = m->comps[c];
+                mpd::InvSeg g{};
+                g.map = (int)i;
+                g.p = (int)ms.maxDeg[ms.dim - 1];
+                g.base = ps.Ntot;
+                g.N = d.N;
+                g.w = d.w;
+                g.ws = d.ws;
+                g.y = d.s + (long)c * d.srs;
+                g.yss = d.scs;
+                if (m->triangular) {
+                    // the solved coordinates follow the x1 rows in the point buffer
+                    xOff.push_back(P.fullOff[i]);
+                    oOff.push_back(P.fullOff[i] + (d.xrows + (long)c) * d.N);
+                    g.xrows = (long)ms.dim;
+                    g.xcols = d.N;
+                    g.xrs = d.N;
+                    g.xcs = 1;
+                    g.os = 1;
+                    g.o2 = d.o + (long)c * d.ors;
+                    g.os2 = d.ocs;
+                } else {
+                    xOff.push_back(-1);
+                    oOff.push_back(-1);
+                    g.x = d.p;
+                    g.xrows = d.xrows;
+                    g.xcols = d.pcols;
+                    g.xrs = d.prs;
+                    g.xcs = d.pcs;
+                    g.o = d.o;
+                    g.os = d.ocs;
+                    g.o2 = nullptr;
+                    g.os2 = 0;
+                }
+                const int si = (int)segs.size() - (int)ps.seg0;
+                for (long t0 = 0; t0 < d.N; t0 += tileN) {
+                    tileSeg.push_back(si);
+                    tileStart.push_back(t0);
+                }
+                const long padded = (d.N + mpd::INV_CHUNK - 1) / mpd::INV_CHUNK * mpd::INV_CHUNK;
+                for (long k = 0; k < padded / mpd::INV_CHUNK; ++k) chunkSeg.push_back(si);
+                ps.Ntot += padded;
+                segs.push_back(g);
+            }
+            ps.nseg = (long)segs.size() - ps.seg0;
+            ps.ntiles = (long)tileSeg.size() - ps.tile0;
+            P.maxNtot = std::max(P.maxNtot, ps.Ntot);
+            P.passes.push_back(ps);
+        }
+        // device buffers, then the point-buffer offsets become addresses
+        P.wFull.ensure(std::max<long>(P.fullElems, 1) * sizeof(double));
+        const int rows = mpd::inverse_record_rows(b.variant);
+        P.wRec.ensure((size_t)rows * std::max<long>(P.maxNtot, 1) * sizeof(double));
+        P.dCounters.ensure(std::max<size_t>(P.passes.size(), 1) * sizeof(unsigned long long));
+        double* fullBuf = P.wFull.as<double>();
+        for (size_t k = 0; k < segs.size(); ++k) {
+            if (xOff[k] >= 0) segs[k].x = fullBuf + xOff[k];
+            if (oOff[k] >= 0) segs[k].o = fullBuf + oOff[k];
+        }
+        // host-side checks of the tables the kernels index
+        for (const auto& ps : P.passes) {
+            if (ps.Ntot % mpd::INV_CHUNK != 0) throw std::runtime_error("batched Inverse: unpadded pass");
+            for (long k = 0; k < ps.Ntot / mpd::INV_CHUNK; ++k) {
+                const int si = chunkSeg[ps.chunk0 + k];
+                if (si < 0 || si >= ps.nseg) throw std::runtime_error("batched Inverse: chunk table out of range");
+                const mpd::InvSeg& g = segs[ps.seg0 + si];
+                const long padded = (g.N + mpd::INV_CHUNK - 1) / mpd::INV_CHUNK * mpd::INV_CHUNK;
+                if (k * mpd::INV_CHUNK < g.base || k * mpd::INV_CHUNK >= g.base + padded)
+                    throw std::runtime_error("batched Inverse: chunk outside its segment");
+            }
+            for (long t = 0; t < ps.ntiles; ++t) {
+                const int si = tileSeg[ps.tile0 + t];
+                if (si < 0 || si >= ps.nseg || tileStart[ps.tile0 + t] < 0 ||
+                    tileStart[ps.tile0 + t] >= segs[ps.seg0 + si].N)
+                    throw std::runtime_error("batched Inverse: tile table out of range");
+            }
+        }
+        upload(P.dSegs, segs);
+        upload(P.dTileSeg, tileSeg);
+        upload(P.dTileStart, tileStart);
+        upload(P.dChunkSeg, chunkSeg);
+        P.io = io;
+        P.valid = true;
+    }
+    if (g_numCU == 0) {
+        int dev = 0;
+        hip_check(hipGetDevice(&dev), "hipGetDevice");
+        hip_check(hipDeviceGetAttribute(&g_numCU, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
+    }
+    // zeroed point buffers with the x1 rows copied in
+    double* full = P.wFull.as<double>();
+    if (P.fullElems > 0) hip_check(mpd::launch_fill(full, 1, P.fullElems, P.fullElems, 1, 0.0, s), "launch fill");
+    for (size_t i = 0; i < nm; ++i) {
+        if (P.fullOff[i] < 0 || io[i].xrows == 0) continue;
+        hip_check(mpd::launch_copy(io[i].p, io[i].xrows, io[i].N, io[i].prs, io[i].pcs, full + P.fullOff[i], io[i].N, 1,
+                                   s),
+                  "launch copy");
+    }
+    const int qblock = 256;
+    const long qcap = (long)g_numCU * mpd::occupancy_inverse_batch(b.variant, qblock);
+    const auto* segs = P.dSegs.as<mpd::InvSeg>();
+    for (size_t c = 0; c < P.passes.size(); ++c) {
+        const InvPlan::Pass& ps = P.passes[c];
+        const long qgrid = std::max<long>(1, std::min<long>((ps.Ntot + qblock - 1) / qblock, qcap));
+        mpd::Launch L{dim3(1), dim3(b.block), b.smem, s};
+        hip_check(mpd::launch_inverse_batch(b.variant, L, (int)ps.ntiles, dim3((unsigned)qgrid), dim3(qblock),
+                                            b.maps[0]->dev, b.dMaps.as<MapDev>(), segs + ps.seg0,
+                                            P.dTileSeg.as<int>() + ps.tile0, P.dTileStart.as<long>() + ps.tile0,
+                                            P.dChunkSeg.as<int>() + ps.chunk0, ps.Ntot, (int)c, xtol, ytol,
+                                            P.wRec.as<double>(), P.dCounters.as<unsigned long long>() + c),
+                  "launch batched inverse");
+    }
 }
 
 mpd::BatchIO base_io(mpart_map m) {
@@ -2078,6 +2260,25 @@ int mpart_batch_size(mpart_batch b, uint32_t* n) {
 
 constexpr long BATCH_TUNE_MIN_POINTS = 65536;
 
+int mpart_batch_set_inverse_schedule(mpart_batch b, int32_t schedule) {
+    return guarded([&] {
+        if (!b) throw ArgError("null argument");
+        if (schedule < 0 || schedule > 2) throw ArgError("the inverse schedule must be 0, 1 or 2");
+        if (b->invCaptured && schedule != b->invSchedule)
+            throw ArgError("the batch's Inverse has been captured in a graph: its schedule can no longer change");
+        b->invSchedule = schedule;
+    });
+}
+
+int mpart_batch_inverse_schedule(mpart_batch b, int32_t* schedule, double* trialMs) {
+    return guarded([&] {
+        if (!b) throw ArgError("null argument");
+        if (schedule) *schedule = b->invSchedule;
+        if (trialMs)
+            for (int i = 0; i < 2; ++i) trialMs[i] = b->invTrialMs[i];
+    });
+}
+
 int mpart_batch_set_point_group(mpart_batch b, int32_t lanes) {
     return guarded([&] {
         if (!b) throw ArgError("null argument");
@@ -2215,7 +2416,34 @@ int mpart_batch_inverse(mpart_batch b, const mpart_cview* x1, const mpart_cview*
             d.ocs = out[i].col_stride;
             io.push_back(d);
         }
-        run_batch(*b, mpd::BM_INVERSE, io, xtol, ytol, S(s));
+        auto run = [&](int sched) {
+            if (sched == 1) batch_inverse(*b, io, xtol, ytol, S(s));
+            else run_batch(*b, mpd::BM_INVERSE, io, xtol, ytol, S(s));
+        };
+        long total = 0;
+        for (auto& d : io) total += d.N;
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        hip_check(hipStreamIsCapturing(S(s), &cs), "hipStreamIsCapturing");
+        if (cs != hipStreamCaptureStatusNone) b->invCaptured = true;
+        if (b->invSchedule == 0 && !b->invCaptured && total >= BATCH_TUNE_MIN_POINTS) {
+            hipEvent_t e0, e1;
+            hip_check(hipEventCreate(&e0), "hipEventCreate");
+            hip_check(hipEventCreate(&e1), "hipEventCreate");
+            for (int i = 0; i < 2; ++i) {
+                run(i + 1);  // untimed: builds and uploads the schedule's tables
+                hip_check(hipEventRecord(e0, S(s)), "hipEventRecord");
+                run(i + 1);
+                hip_check(hipEventRecord(e1, S(s)), "hipEventRecord");
+                hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+                float ms = 0.f;
+                hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+                b->invTrialMs[i] = ms;
+            }
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            b->invSchedule = b->invTrialMs[0] < b->invTrialMs[1] ? 1 : 2;
+        }
+        run(b->invSchedule == 1 ? 1 : 2);
     });
 }
 

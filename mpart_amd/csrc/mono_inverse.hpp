@@ -23,6 +23,7 @@
 // lane-per-point pass that stores a record per point, so that refilling a
 // lane costs a few loads.
 #pragma once
+#include "batch_io.hpp"
 #include "mono_kernels.hpp"
 
 namespace mpd {
@@ -35,6 +36,31 @@ constexpr int inv_rec_rows() {
     return PM + 4;
 }
 
+// Record of one point: pt is the point's input column (xrows entries, stride
+// xrs; the last one the initial guess), r its record column (rows R apart).
+template <int BK, int PK, int QK, int PM>
+__device__ __forceinline__ void inv_setup_point(const MapDev& M, const double* w, long ws, const double* pt,
+                                                long xrows, long xrs, double yv, int c, double* tab, long stride,
+                                                double* r, long R) {
+    bool bad = false;
+    for (long i = 0; i < xrows; ++i) bad |= isnan(pt[i * xrs]);
+    r[(PM + 3) * R] = yv;
+    if (bad) {
+        r[(PM + 2) * R] = NAN;
+        return;
+    }
+    const double x0 = pt[(xrows - 1) * xrs];
+    const CompDev cd = M.comps[c];
+    fill_values<BK>(M, cd.dim - 1, pt, xrs, tab, (int)stride);
+    double C[PM + 1];
+    coeff_groups<PM>(M, cd, w + (long)cd.coeffOff * ws, ws, tab, (int)stride, C);
+    DiagCoef<PM> dc;
+    make_diag<PM>(M, cd, C, x0, M.nugget, dc);
+    for (int q = 0; q <= PM; ++q) r[q * R] = dc.cd[q];
+    r[(PM + 1) * R] = f_at_zero<PM>(M, cd, C);
+    r[(PM + 2) * R] = x0;
+}
+
 template <int BK, int PK, int QK, int PM>
 __global__ void __launch_bounds__(256) k_inverse_setup(MapDev M, const double* w, long ws, const double* xs,
                                                        long xrows, long xcols, long xrs, long xcs, const double* ys,
@@ -43,27 +69,28 @@ __global__ void __launch_bounds__(256) k_inverse_setup(MapDev M, const double* w
     const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
     const long xi = (xcols == 1) ? 0 : n;
-    const double* pt = xs + xi * xcs;
-    bool bad = false;
-    for (long i = 0; i < xrows; ++i) bad |= isnan(pt[i * xrs]);
-    double* r = rec + n;
-    r[(PM + 3) * N] = ys[n * yss];
-    if (bad) {
-        r[(PM + 2) * N] = NAN;
-        return;
-    }
-    const double x0 = pt[(xrows - 1) * xrs];
     long stride;
     double* tab = lane_table(smem, gtab, stride);
-    const CompDev cd = M.comps[c];
-    fill_values<BK>(M, cd.dim - 1, pt, xrs, tab, (int)stride);
-    double C[PM + 1];
-    coeff_groups<PM>(M, cd, w + (long)cd.coeffOff * ws, ws, tab, (int)stride, C);
-    DiagCoef<PM> dc;
-    make_diag<PM>(M, cd, C, x0, M.nugget, dc);
-    for (int q = 0; q <= PM; ++q) r[q * N] = dc.cd[q];
-    r[(PM + 1) * N] = f_at_zero<PM>(M, cd, C);
-    r[(PM + 2) * N] = x0;
+    inv_setup_point<BK, PK, QK, PM>(M, w, ws, xs + xi * xcs, xrows, xrs, ys[n * yss], c, tab, stride, rec + n, N);
+}
+
+// Batched setup: block b covers points tileStart[b] .. + blockDim.x - 1 of
+// segment tileSeg[b]; records go to the pass's index space (rows R apart).
+template <int BK, int PK, int QK, int PM>
+__global__ void __launch_bounds__(256) k_inverse_setup_batch(const MapDev* __restrict__ maps,
+                                                             const InvSeg* __restrict__ segs,
+                                                             const int* __restrict__ tileSeg,
+                                                             const long* __restrict__ tileStart, long R, int c,
+                                                             double* rec) {
+    extern __shared__ double smem[];
+    const InvSeg& S = segs[tileSeg[blockIdx.x]];
+    const long n = tileStart[blockIdx.x] + threadIdx.x;
+    if (n >= S.N) return;
+    const long xi = (S.xcols == 1) ? 0 : n;
+    long stride;
+    double* tab = lane_table(smem, nullptr, stride);
+    inv_setup_point<BK, PK, QK, PM>(maps[S.map], S.w, S.ws, S.x + xi * S.xcs, S.xrows, S.xrs, S.y[n * S.yss], c, tab,
+                                    stride, rec + S.base + n, R);
 }
 
 // Saved interval of the depth-first quadrature (Simpson also keeps the
@@ -82,16 +109,21 @@ struct InvStk<QK_AS> {
 // Solver phases.
 enum { IP_START = 0, IP_DOWN = 1, IP_UP = 2, IP_ITP = 3 };
 
-constexpr int INV_CHUNK = 64;  // point indices taken from the global counter at a time
 
 // LIN: the map has LinearizedBasis bounds.  Without them the kernel is built
 // with the bound checks folded away (they cost ~15 % of the integrand work).
-template <int BK, int PK, int QK, int PM, bool LIN>
+// BATCH: the points of many maps (segs, one per map; chunkSeg: segment of
+// each chunk of INV_CHUNK indices), each map starting at a chunk boundary;
+// M then only supplies the options every map of a batch shares (quadrature,
+// nugget, basis bounds), and each lane takes p and its output rows from the
+// segment of its point.  Indices past a map's points are skipped.
+template <int BK, int PK, int QK, int PM, bool LIN, bool BATCH>
 __global__ void __launch_bounds__(256) k_inverse_queue(MapDev M, const double* __restrict__ rec, long N, double* out,
                                                        long os, int c, double xtol, double ytol,
-                                                       unsigned long long* counter) {
+                                                       unsigned long long* counter, const InvSeg* __restrict__ segs,
+                                                       const int* __restrict__ chunkSeg) {
     const QuadDev& Q = M.quad;
-    const int p = M.comps[c].p;
+    const int p = BATCH ? 0 : M.comps[c].p;
     // Clenshaw-Curtis node table of one interval, in the order integrate_acc
     // visits the nodes after the midpoint: position (cp or fp), coarse weight
     // index (-1: fine-only node) and fine weight index.
@@ -133,6 +165,8 @@ __global__ void __launch_bounds__(256) k_inverse_queue(MapDev M, const double* _
 
     // Point and solver state.
     long n = -1;
+    double* op = nullptr;   // BATCH: the lane's output rows
+    double* op2 = nullptr;
     bool busy = false, finished = false;
     DiagCoef<PM> dc;
     dc.p = p;
@@ -190,13 +224,31 @@ __global__ void __launch_bounds__(256) k_inverse_queue(MapDev M, const double* _
                 long idx = N;
                 if (rank < avail) idx = qNext + rank;
                 else if (!exhausted) idx = base + (rank - avail);
+                bool skip = false;
                 if (idx >= N) {
                     finished = true;
-                } else {
+                } else if constexpr (BATCH) {
+                    const InvSeg& S = segs[chunkSeg[idx / INV_CHUNK]];
+                    const long local = idx - S.base;
+                    if (local >= S.N) {
+                        skip = true;  // padding after the map's points: stays without a point
+                    } else {
+                        op = S.o + local * S.os;
+                        op2 = S.o2 ? S.o2 + local * S.os2 : nullptr;
+                        dc.p = S.p;
+                    }
+                }
+                if (idx < N && !skip) {
                     n = idx;
                     x0 = rec[(PM + 2) * N + n];
                     if (isnan(x0)) {
-                        out[n * os] = NAN;  // stays without a point: refilled on the next pass
+                        // stays without a point: refilled on the next pass
+                        if constexpr (BATCH) {
+                            *op = NAN;
+                            if (op2) *op2 = NAN;
+                        } else {
+                            out[n * os] = NAN;
+                        }
                     } else {
 #pragma unroll
                         for (int q = 0; q <= PM; ++q) dc.cd[q] = rec[q * N + n];
@@ -428,7 +480,13 @@ __global__ void __launch_bounds__(256) k_inverse_queue(MapDev M, const double* _
             }
         }
         if (fail || done) {
-            out[n * os] = fail ? NAN : root;
+            const double v = fail ? NAN : root;
+            if constexpr (BATCH) {
+                *op = v;
+                if (op2) *op2 = v;
+            } else {
+                out[n * os] = v;
+            }
             busy = false;
         }
     }

@@ -249,3 +249,37 @@ def test_batched_evaluate_tuner_and_capture_freeze():
         assert all(torch.equal(a, c) for a, c in zip(ys, y0))
     with pytest.raises(ValueError):
         b.SetPointGroup(5 - lanes)  # the other schedule
+
+
+def test_batched_inverse_schedule_tuner():
+    # >= 65536 points in all: the first call times both inverse schedules and
+    # keeps the faster; its roots equal those of a batch forced to that schedule.
+    _case("batch_inverse_tuner")
+    s = Stream(31)
+    maps, xs, ys = [], [], []
+    for d, p in ((3, 2), (4, 3), (5, 2), (2, 4)):
+        m = mt.CreateTriangular(d, d, p, mt.MapOptions())
+        m.SetCoeffs(0.1 * s.normal(m.numCoeffs))
+        x = torch.from_numpy(s.points(d, 20_000)).cuda()
+        maps.append(m)
+        xs.append(x)
+        ys.append(_as_dev(m.Evaluate(x)))
+    x1s = [x[:0] for x in xs]
+    b = mt.MapBatch(maps)
+    assert b.InverseSchedule()[0] == 0
+    got = b.Inverse(x1s, ys)
+    sched, trial = b.InverseSchedule()
+    assert sched in (1, 2) and min(trial) > 0.0
+    forced = mt.MapBatch(maps)
+    forced.SetInverseSchedule(sched)
+    ref = forced.Inverse(x1s, ys)
+    for a, c in zip(got, ref):
+        assert torch.equal(torch.isnan(a), torch.isnan(c))
+        assert torch.equal(torch.nan_to_num(a), torch.nan_to_num(c))
+    # both schedules solve to the same tolerance: T(x) is recovered where T is well conditioned
+    other = mt.MapBatch(maps)
+    other.SetInverseSchedule(3 - sched)
+    for a, c in zip(got, other.Inverse(x1s, ys)):
+        fin = torch.isfinite(a) & torch.isfinite(c)
+        assert torch.equal(torch.isfinite(a), torch.isfinite(c))
+        assert (a[fin] - c[fin]).abs().max().item() <= 1e-9

@@ -1,6 +1,8 @@
 """Inverse with per-lane solve scheduling (the single-map path) against the
-lane-per-point sequential solver (the batched path, which runs bracket + ITP
-for one point per lane from start to end) and the oracle.
+lane-per-point sequential solver (the batched path's schedule 2, which runs
+bracket + ITP for one point per lane from start to end) and the oracle; the
+batched queue path (schedule 1: the same solver over the points of many maps)
+against the oracle.
 
 Both paths evaluate the same nodes in the same accumulation order and take
 the same branches, so their roots agree bit for bit; against the oracle the
@@ -38,6 +40,13 @@ def _component(case, seed, n, dim=3, order=3, scale=0.2, nan_at=()):
     return g, c, x, tgt
 
 
+def _sequential(g):
+    # the lane-per-point solver: each point solved to the end in its lane
+    b = mt.MapBatch([g])
+    b.SetInverseSchedule(2)
+    return b
+
+
 @pytest.mark.parametrize("ci", range(len(CASES)))
 def test_scheduled_inverse_equals_sequential_and_oracle(ci):
     _case("inverse_scheduled", case=CASES[ci])
@@ -45,7 +54,7 @@ def test_scheduled_inverse_equals_sequential_and_oracle(ci):
     xd = torch.from_numpy(x).cuda()
     td = torch.from_numpy(tgt).cuda()
     single = g.Inverse(xd, td)
-    seq = mt.MapBatch([g]).Inverse([xd], [td])[0]
+    seq = _sequential(g).Inverse([xd], [td])[0]
     assert torch.equal(torch.isnan(single), torch.isnan(seq))
     assert torch.equal(torch.nan_to_num(single), torch.nan_to_num(seq))
     r = orc.CreateComponent(ref_opts(**CASES[ci]), 3, 3)
@@ -56,6 +65,12 @@ def test_scheduled_inverse_equals_sequential_and_oracle(ci):
     assert np.array_equal(fin, np.isfinite(got))
     assert np.isnan(got[0, [5, 17, 650]]).all()
     assert np.max(np.abs(got[fin] - ref[fin])) <= 1e-9
+    # the batched queue (per-lane p and outputs): the oracle's roots to 1e-9
+    bqb = mt.MapBatch([g])
+    bqb.SetInverseSchedule(1)
+    bq = bqb.Inverse([xd], [td])[0].cpu().numpy()
+    assert np.array_equal(fin, np.isfinite(bq))
+    assert np.max(np.abs(bq[fin] - ref[fin])) <= 1e-9
 
 
 def test_scheduled_inverse_one_column_x1():
@@ -82,7 +97,7 @@ def test_scheduled_inverse_many_refills(ci):
     td = torch.from_numpy(tgt).cuda()
     out = torch.full_like(td, -7.0)
     g.InverseImpl(xd, td, out)
-    seq = mt.MapBatch([g]).Inverse([xd], [td])[0]
+    seq = _sequential(g).Inverse([xd], [td])[0]
     assert not bool((out == -7.0).any()), "a point was not solved"
     assert torch.equal(torch.isnan(out), torch.isnan(seq))
     assert torch.equal(torch.nan_to_num(out), torch.nan_to_num(seq))
@@ -101,7 +116,7 @@ def test_scheduled_triangular_inverse_equals_sequential():
     y = torch.from_numpy(2.0 * s.normal(4 * n).reshape(4, n)).cuda()
     empty = torch.zeros((0, n), dtype=torch.float64, device="cuda")
     single = g.Inverse(empty, y)
-    seq = mt.MapBatch([g]).Inverse([empty], [y])[0]
+    seq = _sequential(g).Inverse([empty], [y])[0]
     assert torch.equal(torch.isnan(single), torch.isnan(seq))
     assert torch.equal(torch.nan_to_num(single), torch.nan_to_num(seq))
 
@@ -116,7 +131,7 @@ def test_large_scheduled_inverse_equals_sequential(ci):
     td = torch.from_numpy(tgt).cuda()
     out = torch.full_like(td, -7.0)
     g.InverseImpl(xd, td, out)
-    seq = mt.MapBatch([g]).Inverse([xd], [td])[0]
+    seq = _sequential(g).Inverse([xd], [td])[0]
     assert not bool((out == -7.0).any()), "a point was not solved"
     assert torch.equal(torch.isnan(out), torch.isnan(seq))
     assert torch.equal(torch.nan_to_num(out), torch.nan_to_num(seq))
@@ -132,6 +147,6 @@ def test_large_triangular_inverse_equals_sequential():
     y = torch.from_numpy(2.0 * s.normal(3 * n).reshape(3, n)).cuda()
     empty = torch.zeros((0, n), dtype=torch.float64, device="cuda")
     single = g.Inverse(empty, y)
-    seq = mt.MapBatch([g]).Inverse([empty], [y])[0]
+    seq = _sequential(g).Inverse([empty], [y])[0]
     assert torch.equal(torch.isnan(single), torch.isnan(seq))
     assert torch.equal(torch.nan_to_num(single), torch.nan_to_num(seq))
