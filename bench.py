@@ -24,7 +24,10 @@ for rocprofv3 FETCH_SIZE / WRITE_SIZE), for use under rocprofv3;
 --profile-inverse runs only three Inverse calls of the single-component
 workload; --profile-many runs only the many-maps workload at dims 2..8,
 orders 2..4 (Evaluate + LogDeterminant graph, then the Evaluate +
-LogDeterminant + CoeffGrad + Inverse graph, each beside its per-map loop).
+LogDeterminant + CoeffGrad + Inverse graph, each beside its per-map loop);
+--profile-inverse-many runs only the batched Inverse of the many-maps
+workload at the full ranges (dims 2..16, orders 2..5), once with each
+schedule.
 """
 import argparse
 import json
@@ -273,6 +276,44 @@ def many_maps(steps, dmax=8, pmax=4, nmax=16384, coeff_grad=True, inverse=True):
     log(f"many maps sequence ({res['sequence']['methods']}): batched graph {ts:.3f} ms, per-map loop {tls:.3f} ms, "
         f"identical {seq_same}; {json.dumps({k: v for k, v in res['sequence'].items() if k.startswith(('inverse', 'coeff'))})}")
     return res
+
+
+def profile_inverse_many_mode():
+    """Batched Inverse of the many-maps workload at the full ranges (the maps
+    and points of many_maps(dmax=16, pmax=5, nmax=65536), targets T(x)), once
+    with each schedule (1: per-lane solve scheduling, 2: a lane per point)."""
+    import torch
+
+    import mpart_amd as mt
+    from mpart_amd.data import Stream
+
+    s = Stream(21)
+    nm, dmax, pmax, nmax = 256, 16, 5, 65536
+    u = s.uniform(3 * nm).reshape(nm, 3)
+    maps, xs = [], []
+    for i in range(nm):
+        d = 2 + int(u[i, 0] * (dmax - 1))
+        p = 2 + int(u[i, 1] * (pmax - 1))
+        n = 4096 + int(u[i, 2] * (nmax - 4096))
+        m = mt.CreateTriangular(d, d, p, mt.MapOptions())
+        m.SetCoeffs(0.1 * s.normal(m.numCoeffs))
+        m.SetCodegen(False)
+        maps.append(m)
+        xs.append(torch.from_numpy(s.points(d, n)).cuda())
+    b = mt.MapBatch(maps)
+    ys = b.Evaluate(xs)
+    x1s = [x[:0] for x in xs]
+    xis = [torch.empty_like(x) for x in xs]
+    out = {}
+    for sched in (1, 2):
+        b.SetInverseSchedule(sched)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b.InverseImpl(x1s, ys, xis)
+        torch.cuda.synchronize()
+        out[f"schedule_{sched}_ms"] = (time.perf_counter() - t0) * 1e3
+        log(f"inverse schedule {sched}: {out[f'schedule_{sched}_ms']:.1f} ms")
+    print(json.dumps({"profile": "inverse_many_maps_full_ranges", **out}), flush=True)
 
 
 def single_component_setup():
@@ -549,6 +590,7 @@ def main():
     ap.add_argument("--profile-inverse", action="store_true")
     ap.add_argument("--profile-grads", action="store_true")
     ap.add_argument("--profile-many", action="store_true")
+    ap.add_argument("--profile-inverse-many", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -563,6 +605,9 @@ def main():
         return
     if args.profile_grads:
         profile_grads_mode(args)
+        return
+    if args.profile_inverse_many:
+        profile_inverse_many_mode()
         return
     if args.profile_many:
         print(json.dumps({"profile": "many_maps_sequence", **many_maps(args.steps)}), flush=True)
