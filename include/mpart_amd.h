@@ -285,7 +285,10 @@ int mpart_set_plain_launches(mpart_map map, int plain);
  * capturing), so a captured graph -- e.g. Evaluate, CoeffGrad and Inverse in
  * sequence -- replays with no host work.  A graph reads the arguments of each
  * method at replay time: calling a method with other arguments between
- * replays redirects the graph's launch of that method to them.  Batched Inverse follows ConditionalMapBase::Inverse per map: x1 has
+ * replays redirects the graph's launch of that method to them; with the
+ * per-lane Inverse schedule (1, below) such a call is rejected instead, since
+ * the captured launches hold that call's table offsets and counts.  Batched
+ * Inverse follows ConditionalMapBase::Inverse per map: x1 has
  * inDim - outDim rows for a triangular map, and all inDim rows (the last one
  * the initial guess) for a single component. */
 typedef struct mpart_batch_s* mpart_batch;

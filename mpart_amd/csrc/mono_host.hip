@@ -1436,6 +1436,11 @@ void batch_inverse(mpart_batch_s& b, const std::vector<mpd::BatchIO>& io, double
         if (cs != hipStreamCaptureStatusNone)
             throw ArgError("batched call: its arguments differ from the previous call of this method on this batch; "
                            "run the call once before capturing it in a graph");
+        // a captured graph holds the pass offsets and counts of this plan as
+        // launch arguments: a new plan would leave it indexing the new tables
+        if (b.invCaptured)
+            throw ArgError("batched Inverse: this batch's Inverse has been captured in a graph, so its arguments can "
+                           "no longer change; use another batch for other arguments");
         hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
         P.valid = false;
         P.passes.clear();

@@ -283,3 +283,44 @@ def test_batched_inverse_schedule_tuner():
         fin = torch.isfinite(a) & torch.isfinite(c)
         assert torch.equal(torch.isfinite(a), torch.isfinite(c))
         assert (a[fin] - c[fin]).abs().max().item() <= 1e-9
+
+
+def test_captured_queue_inverse_rejects_other_arguments():
+    # The per-lane Inverse schedule bakes its plan's offsets into the captured
+    # launches: once captured, a call with other arguments is refused on the
+    # host (before any launch), and the graph still replays to the eager roots.
+    _case("batch_inverse_capture_guard")
+    s = Stream(41)
+    maps, xs = [], []
+    for d, p in ((3, 2), (4, 2)):
+        m = mt.CreateTriangular(d, d, p, mt.MapOptions())
+        m.SetCoeffs(0.1 * s.normal(m.numCoeffs))
+        maps.append(m)
+        xs.append(torch.from_numpy(s.points(d, 3000)).cuda())
+    b = mt.MapBatch(maps)
+    b.SetInverseSchedule(1)
+    ys = b.Evaluate(xs)
+    x1s = [x[:0] for x in xs]
+    outs = [torch.empty_like(x) for x in xs]
+    stream = torch.cuda.Stream()
+    stream.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(stream):
+        b.InverseImpl(x1s, ys, outs)
+    torch.cuda.current_stream().wait_stream(stream)
+    torch.cuda.synchronize()
+    ref = [o.clone() for o in outs]
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        b.InverseImpl(x1s, ys, outs)
+    other = [torch.empty_like(x) for x in xs]
+    with pytest.raises(ValueError, match="captured"):
+        b.InverseImpl(x1s, ys, other)
+    with pytest.raises(ValueError, match="captured"):
+        b.SetInverseSchedule(2)
+    for o in outs:
+        o.fill_(-1.0)
+    torch.cuda.synchronize()
+    g.replay()
+    torch.cuda.synchronize()
+    for o, e in zip(outs, ref):
+        assert torch.equal(torch.nan_to_num(o), torch.nan_to_num(e))
