@@ -324,3 +324,43 @@ def test_captured_queue_inverse_rejects_other_arguments():
     torch.cuda.synchronize()
     for o, e in zip(outs, ref):
         assert torch.equal(torch.nan_to_num(o), torch.nan_to_num(e))
+
+
+def test_queue_inverse_mixed_order_classes():
+    # Orders 2..5 in one batch: the batch runs every map with the kernels of the
+    # largest last-coordinate order class, the per-map path with each map's own;
+    # a single component with a one-column x1 rides along.  Forced per-lane
+    # schedule against the per-map roots and the oracle (1e-9).
+    _case("batch_inverse_queue_mixed_orders")
+    s = Stream(51)
+    maps, refs, xs, x1s, ys, tri = [], [], [], [], [], []
+    for d, p, t in ((2, 5, True), (3, 2, True), (4, 3, True), (3, 4, False), (2, 2, True)):
+        if t:
+            g = mt.CreateTriangular(d, d, p, mt.MapOptions())
+            r = orc.CreateTriangular(d, d, p, orc.options())
+        else:
+            g = mt.CreateTotalOrderComponent(d, p, mt.MapOptions())
+            r = orc.CreateComponent(orc.options(), d, p)
+        c = 0.1 * s.normal(g.numCoeffs)
+        g.SetCoeffs(c)
+        r.SetCoeffs(c)
+        n = 700 + 97 * len(maps)
+        x = torch.from_numpy(s.points(d, n)).cuda()
+        maps.append(g)
+        refs.append(r)
+        xs.append(x)
+        ys.append(torch.from_numpy(r.Evaluate(x.cpu().numpy())).cuda())
+        x1s.append(x[:0] if t else x[:, :1].contiguous())
+        tri.append(t)
+    b = mt.MapBatch(maps)
+    b.SetInverseSchedule(1)
+    outs = b.Inverse(x1s, ys)
+    for m, r, x1, y, o, t in zip(maps, refs, x1s, ys, outs, tri):
+        single = _as_dev(m.Inverse(x1, y)).cpu().numpy()
+        ref = r.Inverse(x1.cpu().numpy(), y.cpu().numpy())
+        got = o.cpu().numpy()
+        fin = np.isfinite(ref)
+        assert np.array_equal(fin, np.isfinite(got))
+        assert np.array_equal(np.isfinite(single), np.isfinite(got))
+        assert np.max(np.abs(got[fin] - ref[fin]), initial=0.0) <= 1e-9
+        assert np.max(np.abs(got[fin] - single[fin]), initial=0.0) <= 1e-9
