@@ -440,6 +440,65 @@ def input_grad_workload(tri, x, steps):
     return out
 
 
+def training_workload(tri, x, steps=50):
+    """KL training of the 8-D order-3 map with torch.optim.LBFGS (`steps`
+    iterations, strong-Wolfe line search, from the factory's zero
+    coefficients) on the points x: through the fused objective (one fused
+    pass per evaluation, mpart_amd.torch.kl_objective) and through the
+    reference's torch structure (Evaluate + LogDeterminant forward, the
+    coefficient VJP backward, mpart_amd.torch.kl_loss).  Each path runs once
+    untimed (first calls compile, tune and size buffers), then timed; the
+    time inside objective evaluations (forward + backward, synchronized) is
+    reported beside the wall time, which adds the optimizer's host work."""
+    import torch
+
+    from mpart_amd.torch import kl_loss, kl_objective
+
+    zeros = torch.zeros(tri.numCoeffs, dtype=torch.float64, device="cuda")
+
+    def run(fn):
+        coeffs = torch.nn.Parameter(zeros.clone())
+        opt = torch.optim.LBFGS([coeffs], lr=1.0, max_iter=steps, max_eval=int(steps * 1.25) + 1, history_size=20,
+                                tolerance_grad=1e-12, tolerance_change=1e-14, line_search_fn="strong_wolfe")
+        hist, evals = [], []
+
+        def closure():
+            opt.zero_grad()
+            t0 = time.perf_counter()
+            loss = fn(tri, x, coeffs)
+            loss.backward()
+            hist.append(float(loss.detach()))  # synchronizes
+            evals.append(time.perf_counter() - t0)
+            return loss
+
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        opt.step(closure)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, hist, sum(evals)
+
+    out = {"steps": steps, "N": int(x.shape[1])}
+    for name, fn in (("fused", kl_objective), ("unfused", kl_loss)):
+        run(fn)  # untimed
+        wall, hist, ev = run(fn)
+        out[f"{name}_s"] = wall
+        out[f"{name}_objective_s"] = ev
+        out[f"{name}_evaluations"] = len(hist)
+        out[f"{name}_final_loss"] = hist[-1]
+        out[f"_{name}_hist"] = hist
+    hf, hu = out.pop("_fused_hist"), out.pop("_unfused_hist")
+    out["speedup_fused_vs_unfused_wall"] = out["unfused_s"] / out["fused_s"]
+    out["speedup_fused_vs_unfused_objective"] = out["unfused_objective_s"] / out["fused_objective_s"]
+    n = min(len(hf), len(hu))
+    out["objective_history_max_rel_diff"] = max(abs(a - b) / max(abs(b), 1e-300) for a, b in zip(hf[:n], hu[:n]))
+    tri.SetCoeffs(zeros)
+    log(f"training ({steps} L-BFGS steps, N={x.shape[1]}): fused {out['fused_s'] * 1e3:.1f} ms "
+        f"(objective {out['fused_objective_s'] * 1e3:.1f} ms, {out['fused_evaluations']} evaluations), unfused "
+        f"{out['unfused_s'] * 1e3:.1f} ms (objective {out['unfused_objective_s'] * 1e3:.1f} ms), "
+        f"history max rel diff {out['objective_history_max_rel_diff']:.1e}")
+    return out
+
+
 def uneven_map_workload(steps):
     """32-D triangular map, component k of total order max(1, 6 - k // 6)
     (7 to 7 315 terms, 62 572 coefficients), HermiteFunctions / Exp / adaptive
@@ -585,6 +644,7 @@ def main():
     ap.add_argument("--no-single", action="store_true")
     ap.add_argument("--no-input-grad", action="store_true")
     ap.add_argument("--no-uneven", action="store_true")
+    ap.add_argument("--no-train", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--profile", action="store_true")
     ap.add_argument("--profile-inverse", action="store_true")
@@ -653,6 +713,8 @@ def main():
         second["data"] = "x ~ N(0, I_8) (seed 1 stream), coefficients 0.1 N(0,1) from the same stream"
         if not args.no_input_grad:
             second["kl_input_grad"] = input_grad_workload(tri, x2, max(3, args.plain_steps))
+        if not args.no_train:
+            second["training"] = training_workload(tri, x2)
         del x2
 
     many = None
