@@ -19,6 +19,15 @@
  * (ptr, (rows, cols), (rowStride, colStride)) tuples
  * (MParT/Utilities/ArrayConversions.h:102-130); the fields are 64-bit here.
  * Calls are asynchronous on `stream` (a hipStream_t; NULL is the null stream).
+ *
+ * Graph capture.  A call on a map may be captured in a HIP graph (run it once
+ * eagerly first, so its kernels are compiled, tuned and its work buffers
+ * sized).  The captured launches hold the addresses of the map's work
+ * buffers, so once a call on a map has been captured the map never frees a
+ * work buffer while it lives: a later call that needs larger buffers
+ * allocates new ones and keeps the old ones until mpart_destroy.  Replays
+ * and eager calls on one map must not overlap in time (they share the work
+ * buffers); keep them on one stream or order them with events.
  */
 #ifndef MPART_AMD_H
 #define MPART_AMD_H
@@ -231,10 +240,14 @@ void mpart_default_train_options(mpart_train_options* opts);
  * a triangular map one component at a time on its rows of `train`
  * (TrainMap_Triangular_StandardNormalDensity, 141-166).  Every objective
  * evaluation is one fused KL pass on the device; the optimizer runs on the
- * host.  *error receives the sum of the components' final objective values
- * (mean(T_i^2 / 2 - log dT_i/dx_i), without the Gaussian normalization
- * constant), *evals the number of objective evaluations; the trained
- * coefficients stay in the map.  train is a device view (inputDim x N). */
+ * host.  For a triangular map *error receives the sum of the components'
+ * final objective values (mean(T_i^2 / 2 - log dT_i/dx_i), without the
+ * Gaussian normalization constant, as FastGaussianReverseKLObjective returns
+ * them); a lone component is trained as a whole under the KLObjective
+ * (184-199), whose value includes 0.5 d log(2 pi) with d its output
+ * dimension, and *error is that value.  *evals is the number of objective
+ * evaluations; the trained coefficients stay in the map.  train is a device
+ * view (inputDim x N). */
 int mpart_train_map(mpart_map map, mpart_cview train, const mpart_train_options* opts, double* error, int32_t* evals,
                     mpart_stream stream);
 
@@ -283,11 +296,14 @@ int mpart_set_plain_launches(mpart_map map, int plain);
  * arguments when they differ from the previous call of that method; under
  * stream capture they must not differ (make the same calls once before
  * capturing), so a captured graph -- e.g. Evaluate, CoeffGrad and Inverse in
- * sequence -- replays with no host work.  A graph reads the arguments of each
- * method at replay time: calling a method with other arguments between
- * replays redirects the graph's launch of that method to them; with the
- * per-lane Inverse schedule (1, below) such a call is rejected instead, since
- * the captured launches hold that call's table offsets and counts.  Batched
+ * sequence -- replays with no host work.  A captured launch holds the tables,
+ * tile counts and grid of its call, so once a method has been captured on a
+ * batch its arguments are frozen: a later call of that method with other
+ * arguments (other pointers, sizes or strides, or a map whose coefficients
+ * were re-wrapped elsewhere) returns MPART_INVALID_ARGUMENT before launching
+ * anything, and the graph keeps replaying the captured call; use another
+ * batch for other arguments.  Calls with the captured arguments stay allowed
+ * between replays.  Batched
  * Inverse follows ConditionalMapBase::Inverse per map: x1 has
  * inDim - outDim rows for a triangular map, and all inDim rows (the last one
  * the initial guess) for a single component. */

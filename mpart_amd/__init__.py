@@ -16,6 +16,7 @@ stay on the device.
 """
 import ctypes
 import math
+import warnings
 import enum
 import os
 
@@ -757,9 +758,12 @@ _TRAIN_ALGS = ("LD_LBFGS", "LD_SLSQP")
 class TrainOptions:
     """Fields and defaults of the reference's TrainOptions.  The optimizer is
     L-BFGS with a strong-Wolfe line search; opt_alg accepts "LD_LBFGS" and
-    the reference's default "LD_SLSQP" (both run L-BFGS, another quasi-Newton
-    method on the same unconstrained problem) and rejects other NLopt names.
-    ``history`` (L-BFGS memory) has no reference counterpart."""
+    the reference's default "LD_SLSQP" and rejects other NLopt names.  NLopt's
+    SLSQP is not provided: "LD_SLSQP" runs L-BFGS (another quasi-Newton method
+    on the same unconstrained problem, so the optimum agrees to the stopping
+    tolerances but the iterates differ), and training with it issues a
+    RuntimeWarning that says so.  ``history`` (L-BFGS memory) has no reference
+    counterpart."""
 
     def __init__(self, **kw):
         self.opt_alg = "LD_SLSQP"
@@ -787,6 +791,9 @@ class TrainOptions:
     def _c(self):
         if self.opt_alg not in _TRAIN_ALGS:
             raise NotImplementedError(f"TrainOptions: optimizer {self.opt_alg!r} is not available (L-BFGS only)")
+        if self.opt_alg == "LD_SLSQP":
+            warnings.warn("TrainOptions: opt_alg 'LD_SLSQP' (NLopt's SLSQP) is not provided; training runs L-BFGS "
+                          "instead (set opt_alg='LD_LBFGS' to silence this)", RuntimeWarning, stacklevel=3)
         o = _CTrainOptions()
         _lib.mpart_default_train_options(ctypes.byref(o))
         o.stopval, o.ftol_rel, o.ftol_abs = self.opt_stopval, self.opt_ftol_rel, self.opt_ftol_abs
@@ -834,10 +841,12 @@ def CreateGaussianKLObjective(train, test=None, dim=0):
 
 
 def TrainMap(map, objective, options=None):
-    """Trains `map` in place on the objective's training samples and returns
-    the sum of the per-component objective values (without the Gaussian
-    normalization constant), as the reference's TrainMap returns for a
-    triangular map under the Gaussian KL objective."""
+    """Trains `map` in place on the objective's training samples
+    (src/TrainMap.cpp:168-216).  A triangular map is trained one component at
+    a time and the sum of the per-component objective values is returned
+    (without the Gaussian normalization constant, as the reference's
+    FastGaussianReverseKLObjective gives them); a lone component is trained as
+    a whole and the KL objective's value (with 0.5 d log(2 pi)) is returned."""
     if not isinstance(objective, KLObjective):
         raise NotImplementedError("TrainMap: only the Gaussian KL objective is available")
     if objective.dim != map.outputDim:

@@ -884,6 +884,10 @@ struct AsRec {
 template <int BK, int PK, int FL, int PM>
 __host__ __device__ double integrate_as(const QuadDev& Q, const DiagCoef<PM>& dc, AdjAcc<PM>& A, int* nevals,
                                         const NodeV<FL>* at1) {
+    // The interval sums, node positions and the acceptance test are computed
+    // as written, without FMA contraction, like the reference's x86-64 build:
+    // a fused sum rounds differently and can flip a subdivision decision.
+#pragma clang fp contract(off)
     AsRec<FL> stk[MAXSUB];
     int sp = 0;
     NodeV<FL> fa = eval_node<BK, PK, FL, PM>(dc, 0.0);
@@ -953,6 +957,7 @@ struct AccRec {
 template <int BK, int PK, int FL, int NF, int PM>
 __host__ __device__ double integrate_acc(const QuadDev& Q, const DiagCoef<PM>& dc, AdjAcc<PM>& A, int* nevals,
                                          const NodeV<FL>* at1) {
+#pragma clang fp contract(off)  // as in integrate_as
     AccRec<FL> stk[MAXSUB];
     int sp = 0;
     const int nf = (NF == MAXNF) ? Q.nf : NF;
@@ -1048,6 +1053,7 @@ __host__ __device__ double integrate_acc(const QuadDev& Q, const DiagCoef<PM>& d
 template <int BK, int PK, int FL, int PM>
 MPD_HD double integrate_cc(const QuadDev& Q, const DiagCoef<PM>& dc, AdjAcc<PM>& A, int* nevals,
                            const NodeV<FL>* at1) {
+#pragma clang fp contract(off)  // as in integrate_as
     const int n = Q.nf;
     double res = 0.0;
     for (int i = 0; i < MAXNF; ++i) {

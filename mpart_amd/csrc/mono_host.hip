@@ -145,16 +145,28 @@ MultiSet csr_set(unsigned dim, unsigned nterms, const uint32_t* starts, const ui
 }
 
 // ---------------------------------------------------------------------------
-// Device buffer.
+// Device buffer.  Grown on demand.  Once `keep` is set (a call of the owner
+// has been captured in a graph, whose launches hold this buffer's address),
+// growing retires the old allocation instead of freeing it: it stays valid
+// until the owner is destroyed, so the graph never reads freed memory.
 struct DBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    bool keep = false;
+    std::vector<void*> retired;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
     ~DBuf() {
         if (p) (void)hipFree(p);
+        for (void* q : retired) (void)hipFree(q);
     }
     void ensure(size_t b) {
         if (b <= bytes) return;
-        if (p) hip_check(hipFree(p), "hipFree");
+        if (p) {
+            if (keep) retired.push_back(p);
+            else hip_check(hipFree(p), "hipFree");
+        }
         p = nullptr;
         bytes = 0;
         hip_check(hipMalloc(&p, b), "hipMalloc");
@@ -204,6 +216,18 @@ struct mpart_map_s {
     // work buffers
     DBuf partialGrad, partialLoss, gtab, wT, wLd, wG, wLG, wFull, wCoeffs;
     DBuf wInv;  // inverse: per-point records + the point counter
+    // A call on this map has been captured in a graph: from then on the work
+    // buffers are never freed while the map lives (a larger eager call
+    // allocates new ones and retires the old ones, which the graph reads).
+    bool captured = false;
+    void note_capture(hipStream_t s) {
+        if (captured) return;
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs == hipStreamCaptureStatusNone) return;
+        captured = true;
+        for (DBuf* b : {&ownCoeffs, &partialGrad, &partialLoss, &gtab, &wT, &wLd, &wG, &wLG, &wFull, &wCoeffs, &wInv})
+            b->keep = true;
+    }
 
     // runtime-specialized kernels (codegen.hip)
     bool genEnabled = true;
@@ -1245,6 +1269,7 @@ void do_kl_plain(mpart_map_s& m, const mpart_cview& pts, double* loss, double* g
 struct BatchSlot {
     DBuf dIO, dTileMap, dTileStart, dTileLanes, dCounter;
     bool valid = false;
+    bool captured = false;  // a call of this method has been captured: its arguments are frozen
     std::vector<mpd::BatchIO> io;
     std::vector<int> tileMap, tileLanes;
     std::vector<long> tileStart;
@@ -1374,11 +1399,16 @@ void run_batch(mpart_batch_s& b, int method, std::vector<mpd::BatchIO>& io, doub
                       std::memcmp(io.data(), sl.io.data(), io.size() * sizeof(mpd::BatchIO)) == 0;
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     hip_check(hipStreamIsCapturing(s, &cs), "hipStreamIsCapturing");
-    if (cs != hipStreamCaptureStatusNone) b.captured = true;
     if (!same) {
         if (cs != hipStreamCaptureStatusNone)
             throw ArgError("batched call: its arguments differ from the previous call of this method on this batch; "
                            "run the call once before capturing it in a graph");
+        // A captured launch holds this method's tables, tile count and grid:
+        // new arguments would re-upload (and possibly reallocate) the tables
+        // under it, so they are refused before anything is launched.
+        if (sl.captured)
+            throw ArgError("batched call: this method has been captured in a graph on this batch, so its arguments "
+                           "can no longer change; use another batch for other arguments");
         hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
         upload(sl.dIO, io);
         upload(sl.dTileMap, tileMap);
@@ -1389,6 +1419,10 @@ void run_batch(mpart_batch_s& b, int method, std::vector<mpd::BatchIO>& io, doub
         sl.tileStart = tileStart;
         sl.tileLanes = tileLanes;
         sl.valid = true;
+    }
+    if (cs != hipStreamCaptureStatusNone) {
+        b.captured = true;
+        sl.captured = true;
     }
     if (tiles.empty()) return;
     if (g_numCU == 0) {
@@ -1612,7 +1646,9 @@ struct KLObjectiveEval {
     DBuf dev;  // [w (K) | loss | grad (K)]
     std::vector<double> host;
     int evals = 0;
-    KLObjectiveEval(mpart_map_s& m_, const mpart_cview& p, hipStream_t st) : m(m_), pts(p), s(st) {
+    double offset = 0.0;  // constant added to the objective value (the Gaussian normalization, if any)
+    KLObjectiveEval(mpart_map_s& m_, const mpart_cview& p, hipStream_t st, double off = 0.0)
+        : m(m_), pts(p), s(st), offset(off) {
         dev.ensure((2 * (size_t)m.numCoeffs + 1) * sizeof(double));
         host.resize((size_t)m.numCoeffs + 1);
     }
@@ -1627,7 +1663,7 @@ struct KLObjectiveEval {
         hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
         ++evals;
         g.assign(host.begin() + 1, host.end());
-        const double f = host[0];
+        const double f = host[0] + offset;
         return std::isnan(f) ? std::numeric_limits<double>::infinity() : f;
     }
 };
@@ -1801,7 +1837,8 @@ TrainResult lbfgs_minimize(KLObjectiveEval& fg, std::vector<double>& x, const mp
 
 // One map (a component, or any map trained as a whole): returns the final
 // objective value and leaves the trained coefficients in the map.
-TrainResult train_one(mpart_map_s& m, const mpart_cview& pts, const mpart_train_options& o, hipStream_t s) {
+TrainResult train_one(mpart_map_s& m, const mpart_cview& pts, const mpart_train_options& o, hipStream_t s,
+                      double offset = 0.0) {
     check_coeffs(&m);
     const long K = m.numCoeffs;
     std::vector<double> x(K);
@@ -1810,7 +1847,7 @@ TrainResult train_one(mpart_map_s& m, const mpart_cview& pts, const mpart_train_
     hip_check(mpd::launch_copy(m.coeffs, 1, K, 0, m.coeffStride, tmp.as<double>(), 0, 1, s), "launch copy");
     hip_check(hipMemcpyAsync(x.data(), tmp.as<double>(), K * sizeof(double), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
     hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
-    KLObjectiveEval fg(m, pts, s);
+    KLObjectiveEval fg(m, pts, s, offset);
     const TrainResult r = lbfgs_minimize(fg, x, o);
     // The map keeps the accepted iterate (the last evaluation may have been a
     // rejected trial point of the line search).
@@ -1831,10 +1868,16 @@ void do_train(mpart_map_s& m, const mpart_cview& pts, const mpart_train_options&
     check_coeffs(&m);
     double total = 0.0;
     int nev = 0;
-    if (m.outDim == 1 && m.comps.size() == 1) {
-        const TrainResult r = train_one(m, pts, o, s);
-        if (o.verbose) printf("Optimization result: %s\nOptimization error: %.12g\nOptimization evaluations: %d\n", r.status, r.f, r.evals);
+    if (!m.triangular) {
+        // A lone component goes through the generic NLopt path with the
+        // KLObjective (src/TrainMap.cpp:184-199, src/MapObjective.cpp:55-91),
+        // whose value carries the Gaussian normalization 0.5 d log(2 pi)
+        // (GaussianSamplerDensity, d = the output dimension), so that the
+        // optimizer sees the same values as NLopt would (stopval and
+        // ftol_rel compare against them).
+        const TrainResult r = train_one(m, pts, o, s, 0.5 * (double)m.outDim * std::log(2.0 * M_PI));
         total = r.f;
+        if (o.verbose) printf("Optimization result: %s\nOptimization error: %.12g\nOptimization evaluations: %d\n", r.status, total, r.evals);
         nev = r.evals;
     } else {
         if (o.verbose) printf("Detected standard normal reference density with Triangular Map\n");
@@ -2000,6 +2043,7 @@ int mpart_component_multiset(mpart_map m, uint32_t i, uint32_t* dim, uint32_t* n
 int mpart_set_coeffs(mpart_map m, const double* c, int64_t len, int64_t stride, mpart_stream s) {
     return guarded([&] {
         check_map(m);
+        m->note_capture(S(s));
         if (len != m->numCoeffs)
             throw ArgError("SetCoeffs: expected " + std::to_string(m->numCoeffs) + " coefficients, got " +
                            std::to_string(len));
@@ -2024,6 +2068,7 @@ int mpart_wrap_coeffs(mpart_map m, double* c, int64_t len, int64_t stride) {
 int mpart_get_coeffs(mpart_map m, double* out, mpart_stream s) {
     return guarded([&] {
         check_map(m);
+        m->note_capture(S(s));
         check_coeffs(m);
         hip_check(mpd::launch_copy(m->coeffs, 1, m->numCoeffs, 0, m->coeffStride, out, 0, 1, S(s)), "launch copy");
     });
@@ -2032,6 +2077,7 @@ int mpart_get_coeffs(mpart_map m, double* out, mpart_stream s) {
 int mpart_evaluate(mpart_map m, mpart_cview pts, mpart_view out, mpart_stream s) {
     return guarded([&] {
         check_map(m);
+        m->note_capture(S(s));
         do_evaluate(*m, pts, out, nullptr, S(s));
     });
 }
@@ -2039,6 +2085,7 @@ int mpart_evaluate(mpart_map m, mpart_cview pts, mpart_view out, mpart_stream s)
 int mpart_evaluate_counted(mpart_map m, mpart_cview pts, mpart_view out, int32_t* nevals, mpart_stream s) {
     return guarded([&] {
         check_map(m);
+        m->note_capture(S(s));
         do_evaluate(*m, pts, out, nevals, S(s));
     });
 }
@@ -2046,6 +2093,7 @@ int mpart_evaluate_counted(mpart_map m, mpart_cview pts, mpart_view out, int32_t
 int mpart_log_determinant(mpart_map m, mpart_cview pts, double* out, int64_t stride, mpart_stream s) {
     return guarded([&] {
         check_map(m);
+        m->note_capture(S(s));
         do_logdet(*m, pts, out, stride, S(s));
     });
 }
@@ -2053,6 +2101,7 @@ int mpart_log_determinant(mpart_map m, mpart_cview pts, double* out, int64_t str
 int mpart_gradient(mpart_map m, mpart_cview pts, mpart_cview sens, mpart_view out, mpart_stream s) {
     return guarded([&] {
         check_map(m);
+        m->note_capture(S(s));
         do_input_grad(*m, pts, &sens, out, 0, S(s));
     });
 }
@@ -2060,6 +2109,7 @@ int mpart_gradient(mpart_map m, mpart_cview pts, mpart_cview sens, mpart_view ou
 int mpart_coeff_grad(mpart_map m, mpart_cview pts, mpart_cview sens, mpart_view out, mpart_stream s) {
     return guarded([&] {
         check_map(m);
+        m->note_capture(S(s));
         do_coeff_grad(*m, pts, sens, out, S(s));
     });
 }
@@ -2067,6 +2117,7 @@ int mpart_coeff_grad(mpart_map m, mpart_cview pts, mpart_cview sens, mpart_view 
 int mpart_log_determinant_coeff_grad(mpart_map m, mpart_cview pts, mpart_view out, mpart_stream s) {
     return guarded([&] {
         check_map(m);
+        m->note_capture(S(s));
         do_logdet_coeff_grad(*m, pts, out, S(s));
     });
 }
@@ -2074,6 +2125,7 @@ int mpart_log_determinant_coeff_grad(mpart_map m, mpart_cview pts, mpart_view ou
 int mpart_log_determinant_input_grad(mpart_map m, mpart_cview pts, mpart_view out, mpart_stream s) {
     return guarded([&] {
         check_map(m);
+        m->note_capture(S(s));
         do_input_grad(*m, pts, nullptr, out, 1, S(s));
     });
 }
@@ -2082,6 +2134,7 @@ int mpart_inverse(mpart_map m, mpart_cview x1, mpart_cview r, mpart_view out, do
                   mpart_stream s) {
     return guarded([&] {
         check_map(m);
+        m->note_capture(S(s));
         do_inverse(*m, x1, r, out, xtol, ytol, S(s));
     });
 }
@@ -2089,6 +2142,7 @@ int mpart_inverse(mpart_map m, mpart_cview x1, mpart_cview r, mpart_view out, do
 int mpart_kl_value_and_grad(mpart_map m, mpart_cview pts, double* loss, double* grad, mpart_stream s) {
     return guarded([&] {
         check_map(m);
+        m->note_capture(S(s));
         check_view("KL: pts", pts.rows, pts.cols, m->inDim, pts.cols);
         if (pts.cols == 0) throw ArgError("KL: no points");
         if (!m->opts.contDeriv) {
@@ -2105,6 +2159,7 @@ int mpart_coeff_vjp(mpart_map m, mpart_cview pts, mpart_cview sens, const double
                     mpart_stream s) {
     return guarded([&] {
         check_map(m);
+        m->note_capture(S(s));
         check_view("vjp: pts", pts.rows, pts.cols, m->inDim, pts.cols);
         if (sens.ptr) check_view("vjp: sens", sens.rows, sens.cols, m->outDim, pts.cols);
         if (!m->opts.contDeriv) throw NotImpl("contDeriv = false is not provided yet");
@@ -2116,6 +2171,7 @@ int mpart_kl_value_and_grads(mpart_map m, mpart_cview pts, double* loss, double*
                              mpart_stream s) {
     return guarded([&] {
         check_map(m);
+        m->note_capture(S(s));
         check_view("KL: pts", pts.rows, pts.cols, m->inDim, pts.cols);
         if (pts.cols == 0) throw ArgError("KL: no points");
         if (!gradInputs.ptr) {
@@ -2133,6 +2189,7 @@ int mpart_vjp(mpart_map m, mpart_cview pts, mpart_cview sens, const double* lsen
               mpart_view gradInputs, mpart_stream s) {
     return guarded([&] {
         check_map(m);
+        m->note_capture(S(s));
         check_view("vjp: pts", pts.rows, pts.cols, m->inDim, pts.cols);
         if (sens.ptr) check_view("vjp: sens", sens.rows, sens.cols, m->outDim, pts.cols);
         if (!m->opts.contDeriv) throw NotImpl("contDeriv = false is not provided yet");
@@ -2144,6 +2201,7 @@ int mpart_vjp(mpart_map m, mpart_cview pts, mpart_cview sens, const double* lsen
 int mpart_kl_input_grad(mpart_map m, mpart_cview pts, double scale, mpart_view out, mpart_stream s) {
     return guarded([&] {
         check_map(m);
+        m->note_capture(S(s));
         do_input_vjp(*m, pts, 1, nullptr, nullptr, 0, 0.0, scale, out, S(s), "KL input gradient");
     });
 }
@@ -2152,6 +2210,7 @@ int mpart_input_vjp(mpart_map m, mpart_cview pts, mpart_cview sens, const double
                     mpart_view out, mpart_stream s) {
     return guarded([&] {
         check_map(m);
+        m->note_capture(S(s));
         do_input_vjp(*m, pts, 0, sens.ptr ? &sens : nullptr, logdetSens, (long)lstride, 0.0, 1.0, out, S(s),
                      "InputVJP");
     });
@@ -2175,6 +2234,7 @@ int mpart_train_map(mpart_map m, mpart_cview train, const mpart_train_options* o
                     mpart_stream s) {
     return guarded([&] {
         check_map(m);
+        m->note_capture(S(s));
         if (!opts) throw ArgError("null argument");
         do_train(*m, train, *opts, error, evals, S(s));
     });
@@ -2183,6 +2243,7 @@ int mpart_train_map(mpart_map m, mpart_cview train, const mpart_train_options* o
 int mpart_kl_value_and_grad_plain(mpart_map m, mpart_cview pts, double* loss, double* grad, mpart_stream s) {
     return guarded([&] {
         check_map(m);
+        m->note_capture(S(s));
         do_kl_plain(*m, pts, loss, grad, S(s));
     });
 }

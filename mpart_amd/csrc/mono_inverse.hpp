@@ -278,6 +278,7 @@ __global__ void __launch_bounds__(256) k_inverse_queue(MapDev M, const double* _
         bool qdone = false;
         if constexpr (QK == QK_CC) {
             // The fixed rule: one step evaluates every node (integrate_cc).
+#pragma clang fp contract(off)  // sums and node positions as integrate_cc forms them
             double r = 0.0;
 #pragma unroll
             for (int i = 0; i < MAXNF; ++i) {
@@ -288,6 +289,7 @@ __global__ void __launch_bounds__(256) k_inverse_queue(MapDev M, const double* _
             res = r;
             qdone = true;
         } else if constexpr (QK == QK_AS) {
+#pragma clang fp contract(off)  // as in integrate_as
             const double m = 0.5 * (qa + qb);
             const double t0 = (k == -2) ? 0.0 : (k == -1) ? 0.5 : 0.5 * (qa + m);
             const double t1 = (k == -2) ? 1.0 : (k == -1) ? 0.5 : 0.5 * (m + qb);
@@ -338,6 +340,7 @@ __global__ void __launch_bounds__(256) k_inverse_queue(MapDev M, const double* _
                 }
             }
         } else {
+#pragma clang fp contract(off)  // as in integrate_acc
             constexpr int NF = (QK == QK_ACC5) ? 5 : MAXNF;
             const double mid = 0.5 * (qa + qb);
             const double scale = 0.5 * (qb - qa);

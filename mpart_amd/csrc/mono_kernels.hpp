@@ -621,6 +621,7 @@ __device__ __forceinline__ double itp_nhalf(double xlb, double xub, double xtol)
 }
 __device__ __forceinline__ double itp_point(double xlb, double xub, double ylb, double yub, double nhalf, int it,
                                             double xtol) {
+#pragma clang fp contract(off)  // the iterate as RootFinding.h:96-104 writes it, unfused
     const double k1 = 0.1, n0 = 1.0;
     const double xb = 0.5 * (xub + xlb);
     double xf = (xub * ylb - xlb * yub) / (ylb - yub);

@@ -364,3 +364,80 @@ def test_queue_inverse_mixed_order_classes():
         assert np.array_equal(np.isfinite(single), np.isfinite(got))
         assert np.max(np.abs(got[fin] - ref[fin]), initial=0.0) <= 1e-9
         assert np.max(np.abs(got[fin] - single[fin]), initial=0.0) <= 1e-9
+
+
+def _graph_of(fn):
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        fn()  # eager first: tunes, sizes and uploads
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        fn()
+    return g
+
+
+def test_batch_arguments_frozen_after_capture():
+    # A captured batched launch holds its tables, tile count and grid: a later
+    # call of that method with other (here larger) arguments is refused before
+    # anything is launched, and the graph keeps replaying the captured call.
+    _case("batch_capture_frozen")
+    maps, _, xs, _ = _make_batch(nmaps=6, seed=41)
+    b = mt.MapBatch(maps)
+    ys = [torch.empty((m.outputDim, x.shape[1]), dtype=torch.float64, device="cuda") for m, x in zip(maps, xs)]
+    g = _graph_of(lambda: b.EvaluateImpl(xs, ys))
+    torch.cuda.synchronize()
+    ref = [y.clone() for y in ys]
+    s = Stream(42)
+    big = [torch.from_numpy(s.points(x.shape[0], 4 * x.shape[1] + 1000)).cuda() for x in xs]
+    ybig = [torch.empty((m.outputDim, x.shape[1]), dtype=torch.float64, device="cuda") for m, x in zip(maps, big)]
+    with pytest.raises(ValueError, match="captured"):
+        b.EvaluateImpl(big, ybig)
+    for y in ys:
+        y.fill_(-1.0)
+    torch.cuda.synchronize()
+    g.replay()
+    torch.cuda.synchronize()
+    for a, e in zip(ys, ref):
+        assert torch.equal(a, e)
+    # the captured arguments are still accepted eagerly
+    b.EvaluateImpl(xs, ys)
+    torch.cuda.synchronize()
+    for a, e in zip(ys, ref):
+        assert torch.equal(a, e)
+    # another method of the same batch is not frozen until it is captured
+    cgs = [torch.empty((m.numCoeffs, x.shape[1]), dtype=torch.float64, device="cuda") for m, x in zip(maps, big)]
+    sens = [torch.ones((m.outputDim, x.shape[1]), dtype=torch.float64, device="cuda") for m, x in zip(maps, big)]
+    b.CoeffGradImpl(big, sens, cgs)
+    torch.cuda.synchronize()
+
+
+def test_map_buffers_outlive_capture():
+    # A captured single-map Inverse holds the map's work buffers (per-point
+    # records, the triangular point buffer).  A later eager call with more
+    # points needs larger buffers: the map allocates new ones and keeps the
+    # captured ones, so the graph still replays to the captured results.
+    _case("map_capture_buffers")
+    m = mt.CreateTriangular(3, 3, 2, mt.MapOptions())
+    s = Stream(43)
+    m.SetCoeffs(0.1 * s.normal(m.numCoeffs))
+    x = torch.from_numpy(s.points(3, 3000)).cuda()
+    y = m.Evaluate(x)
+    x1 = x[:0]
+    xi = torch.empty_like(x)
+    g = _graph_of(lambda: m.InverseImpl(x1, y, xi))
+    torch.cuda.synchronize()
+    ref = xi.clone()
+    xb = torch.from_numpy(s.points(3, 60000)).cuda()
+    yb = m.Evaluate(xb)
+    xib = m.Inverse(xb[:0], yb)
+    torch.cuda.synchronize()
+    assert torch.isfinite(xib).any()
+    xi.fill_(-1.0)
+    torch.cuda.synchronize()
+    for _ in range(2):
+        g.replay()
+        torch.cuda.synchronize()
+        assert bool(((xi == ref) | (torch.isnan(xi) & torch.isnan(ref))).all())

@@ -79,8 +79,11 @@ def test_train_component_and_budget():
     r = orc.CreateComponent(orc.options(), 3, 3)
     r.SetCoeffs(g2.CoeffMap())
     T = r.Evaluate(x)[0]
-    ref = float(np.mean(0.5 * T * T - r.LogDeterminant(x)))
+    # a lone component is trained under the KL objective, whose value carries
+    # the Gaussian normalization 0.5 d log(2 pi), d = 1 (src/TrainMap.cpp:184-199)
+    ref = float(np.mean(0.5 * T * T - r.LogDeterminant(x))) + 0.5 * np.log(2.0 * np.pi)
     assert abs(err2 - ref) <= 1e-10 * abs(ref)
+    assert abs(err2 - obj.TrainError(g2)) <= 1e-10 * abs(ref)
 
 
 def test_train_rejects_bad_arguments():

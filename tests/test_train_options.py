@@ -22,6 +22,9 @@ def test_python_train_options():
     t = mt.TrainOptions()
     assert t.opt_alg == "LD_SLSQP" and t.opt_maxeval == 1000 and t.opt_ftol_rel == 1e-3
     c = mt.TrainOptions(opt_alg="LD_LBFGS", opt_maxeval=7, history=4)._c()
+    # the reference default LD_SLSQP runs L-BFGS here, and says so
+    with pytest.warns(RuntimeWarning, match="L-BFGS"):
+        mt.TrainOptions()._c()
     assert c.maxeval == 7 and c.history == 4 and c.xtol_abs == 1e-4
     with pytest.raises(NotImplementedError):
         mt.TrainOptions(opt_alg="LN_NELDERMEAD")._c()
