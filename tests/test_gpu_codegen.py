@@ -39,10 +39,12 @@ def test_specialized_module_is_used():
     spec, secs, err = g.PrepareCodegen()
     print(f"[codegen] C3 compile {secs:.2f} s", flush=True)
     assert spec and err == ""
-    # A second map of the same structure reuses the compiled module.
+    # A second map of the same structure reuses the compiled module (an
+    # in-process lookup; the first compile may itself have been served from
+    # the compiler's on-disk cache of an earlier process, so it can be as fast).
     g2 = mt.CreateTriangular(8, 8, 3, mt.MapOptions())
     spec2, secs2, _ = g2.PrepareCodegen()
-    assert spec2 and secs2 < secs
+    assert spec2 and secs2 < max(secs, 0.05)
 
 
 @pytest.mark.parametrize("ci", range(len(CASES)))
