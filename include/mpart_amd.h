@@ -32,6 +32,7 @@
 #ifndef MPART_AMD_H
 #define MPART_AMD_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -176,6 +177,26 @@ int mpart_inverse(mpart_map map, mpart_cview x1, mpart_cview r, mpart_view out, 
  * grad = mean_n(sum_i T_i dT_i/dw - dlogdet/dw).  One launch over every
  * component; no per-point Jacobian is stored.  loss and grad are device pointers. */
 int mpart_kl_value_and_grad(mpart_map map, mpart_cview pts, double* loss, double* grad, mpart_stream stream);
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): mpart_kl_value_and_grad with every work buffer (partial
+ * rows, reduction chunks, a contiguous copy of strided coefficients) in a
+ * fixed device array of MPART_NOALLOC_SCRATCH_BYTES that the library holds
+ * per GPU until the process ends.  The call allocates, frees and compiles
+ * nothing: the map's specialized kernels run only if an earlier call (or
+ * mpart_codegen_prepare) compiled them, on the schedule the autotuner picked
+ * then, otherwise the generic kernel runs; the persistent grid shrinks when
+ * its partial rows would not fit (results then differ from
+ * mpart_kl_value_and_grad's in the last bits, by summation order).  Maps
+ * whose work does not fit are refused with MPART_INVALID_ARGUMENT (more than
+ * about 2 million coefficients, far beyond the specialization limit), maps
+ * whose basis table needs a global buffer and contDeriv = false with
+ * MPART_NOT_IMPLEMENTED.  Every call on a GPU shares that GPU's scratch:
+ * calls that overlap in time on one GPU (other streams, other threads) race
+ * and give wrong results; serialize them. */
+#define MPART_NOALLOC_SCRATCH_BYTES ((size_t)16 << 20)
+int mpart_kl_value_and_grad_noalloc(mpart_map map, mpart_cview pts, double* loss, double* grad, mpart_stream stream);
 
 /* The same objective computed the way the reference structures it: per
  * component Evaluate, LogDeterminant, CoeffGrad(sens = T) and

@@ -103,6 +103,7 @@ _SIGS = {
     "mpart_log_determinant_input_grad": [_vp, _CView, _CView, _vp],
     "mpart_inverse": [_vp, _CView, _CView, _CView, _dbl, _dbl, _vp],
     "mpart_kl_value_and_grad": [_vp, _CView, _vp, _vp, _vp],
+    "mpart_kl_value_and_grad_noalloc": [_vp, _CView, _vp, _vp, _vp],
     "mpart_kl_value_and_grad_plain": [_vp, _CView, _vp, _vp, _vp],
     "mpart_coeff_vjp": [_vp, _CView, _CView, _vp, _i64, _vp, _vp],
     "mpart_kl_input_grad": [_vp, _CView, _dbl, _CView, _vp],
@@ -548,6 +549,12 @@ class ConditionalMap:
             torch.cuda.current_stream().synchronize()
             return float(loss.item()), grad.cpu().numpy()
         return loss, grad
+
+    def KLValueAndGradNoAllocImpl(self, pts, loss, grad):
+        """KLValueAndGradImpl with every work buffer in the library's fixed
+        per-GPU scratch: allocates nothing (no reference counterpart)."""
+        _check(_lib.mpart_kl_value_and_grad_noalloc(self._h, _view(pts), loss.data_ptr(), grad.data_ptr(),
+                                                    _stream()))
 
     def KLValueAndGradsImpl(self, pts, loss, grad, grad_inputs):
         """Loss, coefficient gradient and (grad_inputs, inputDim x N, may be None)

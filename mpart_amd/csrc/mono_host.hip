@@ -195,6 +195,35 @@ int g_numCU = 0;
 }  // namespace
 
 // ---------------------------------------------------------------------------
+// Scratch of mpart_kl_value_and_grad_noalloc: a fixed device array, one copy
+// per GPU, held until the process ends.  Host code never passes `scratch`
+// itself (in host code the name stands for a host-side shadow, not a device
+// address): each device's address comes from hipGetSymbolAddress, called once
+// per device with that device current, and is kept in g_scratchDev.
+static __device__ __align__(256) double scratch[MPART_NOALLOC_SCRATCH_BYTES / sizeof(double)];
+
+namespace {
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratchMu;
+double* g_scratchDev[SCRATCH_MAX_DEVICES] = {};
+
+// Device address of the current device's scratch (set up on first use).
+double* noalloc_scratch() {
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) throw RtError("device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratchMu);
+    if (!g_scratchDev[dev]) {
+        void* ptr = nullptr;
+        hip_check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (!ptr) throw RtError("hipGetSymbolAddress(scratch) returned no address");
+        g_scratchDev[dev] = static_cast<double*>(ptr);
+    }
+    return g_scratchDev[dev];
+}
+}  // namespace
+
+// ---------------------------------------------------------------------------
 struct mpart_map_s {
     mpart_options opts{};
     bool triangular = false;
@@ -976,6 +1005,60 @@ void do_inverse(mpart_map_s& m, const mpart_cview& x1, const mpart_cview& r, con
     hip_check(mpd::launch_copy(full + extra * N, op, N, N, 1, o.ptr, o.row_stride, o.col_stride, s), "launch copy");
 }
 
+// Block size, LDS bytes and resident blocks per CU of the specialized fused
+// kernels (per-wave LDS accumulators of K doubles; K <= 4096 keeps them within
+// 64 KiB at 128 lanes).  Queries only: nothing is allocated.
+void gen_fused_shape(mpart_map_s& m, mpg::GenModule* g, int& gblock, size_t& gsmem) {
+    const long K = m.numCoeffs;
+    gblock = 256;
+    while (gblock > 64 && (size_t)(gblock / 64) * K * sizeof(double) > LDS_LIMIT) gblock /= 2;
+    gsmem = (size_t)(gblock / 64) * K * sizeof(double);
+    if (m.genOccSmem == gsmem) return;
+    hipFunction_t fns[2] = {g->fused, g->fused3};
+    for (int k = 0; k < 2; ++k) {
+        int nb = 0;
+        hip_check(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fns[k], gblock, gsmem),
+                  "hipModuleOccupancyMaxActiveBlocksPerMultiprocessor");
+        m.genOccK[k] = std::max(1, nb);
+    }
+    int priv[2] = {0, 0};
+    for (int k = 0; k < 2; ++k)
+        hip_check(hipFuncGetAttribute(&priv[k], HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, fns[k]), "hipFuncGetAttribute");
+    m.genW3Spills = priv[1] > priv[0];
+    m.genOccSmem = gsmem;
+}
+
+// Launch shape of the generic lane-per-point fused kernel: block size, LDS
+// bytes, whether the per-wave gradient accumulators (ldsAcc) and the basis
+// table (tabLds) fit in LDS, and the persistent grid.
+struct LaneFusedShape {
+    int block;
+    size_t smem, tabLane;
+    bool ldsAcc, tabLds;
+    long grid;
+};
+LaneFusedShape lane_fused_shape(const mpart_map_s& m, long N) {
+    const long K = m.numCoeffs;
+    LaneFusedShape f{};
+    f.tabLane = (size_t)m.dev.nslots * sizeof(double);
+    auto smem_for = [&](int b, bool ldsAcc) {
+        return (ldsAcc ? (size_t)(b / 64) * K * sizeof(double) : 0) + f.tabLane * b;
+    };
+    int block = 256;
+    f.ldsAcc = true;
+    while (block > 64 && smem_for(block, true) > LDS_LIMIT) block /= 2;
+    if (smem_for(block, true) > LDS_LIMIT) f.ldsAcc = false;
+    block = 256;
+    while (block > 64 && smem_for(block, f.ldsAcc) > LDS_LIMIT) block /= 2;
+    f.block = block;
+    f.tabLds = smem_for(block, f.ldsAcc) <= LDS_LIMIT;
+    f.smem = f.tabLds ? smem_for(block, f.ldsAcc) : (f.ldsAcc ? (size_t)(block / 64) * K * sizeof(double) : 0);
+    const long ntiles = std::max<long>(1, (N + block - 1) / block);
+    const int occ = mpd::occupancy_fused(m.variant, block, f.smem);
+    f.grid = std::min<long>(ntiles, (long)g_numCU * occ);
+    return f;
+}
+
 // Launch shape of the fused kernel: persistent grid, LDS accumulators when
 // they fit, otherwise global atomics.
 void do_input_vjp(mpart_map_s& m, const mpart_cview& p, int klMode, const mpart_cview* sens, const double* lsens,
@@ -998,25 +1081,10 @@ void do_fused(mpart_map_s& m, const mpart_cview& p, const double* sens, long srs
         hip_check(hipDeviceGetAttribute(&g_numCU, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
     }
     if (mpg::GenModule* g = gen_for(m)) {
-        // Per-wave LDS accumulators of K doubles; K <= 4096 keeps them within 64 KiB at 128 lanes.
         int gblock = 256;
-        while (gblock > 64 && (size_t)(gblock / 64) * K * sizeof(double) > LDS_LIMIT) gblock /= 2;
-        const size_t gsmem = (size_t)(gblock / 64) * K * sizeof(double);
+        size_t gsmem = 0;
+        gen_fused_shape(m, g, gblock, gsmem);
         hipFunction_t fns[2] = {g->fused, g->fused3};
-        if (m.genOccSmem != gsmem) {
-            for (int k = 0; k < 2; ++k) {
-                int nb = 0;
-                hip_check(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fns[k], gblock, gsmem),
-                          "hipModuleOccupancyMaxActiveBlocksPerMultiprocessor");
-                m.genOccK[k] = std::max(1, nb);
-            }
-            int priv[2] = {0, 0};
-            for (int k = 0; k < 2; ++k)
-                hip_check(hipFuncGetAttribute(&priv[k], HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, fns[k]),
-                          "hipFuncGetAttribute");
-            m.genW3Spills = priv[1] > priv[0];
-            m.genOccSmem = gsmem;
-        }
         const long ntiles = std::max<long>(1, (N + gblock - 1) / gblock);
         // Schedule cfg: kernel (cfg & 1: compiler occupancy / three waves per
         // SIMD) and persistent-grid multiple ((cfg >> 1) + 1).
@@ -1116,26 +1184,13 @@ void do_fused(mpart_map_s& m, const mpart_cview& p, const double* sens, long srs
     // Generic kernels: a lane per point (k_fused) or a group of 4, 8 or 64 lanes
     // per point (k_fused_grp), tuned like the generic Evaluate.
     auto run_lane = [&]() {
-        int block = 256;
-        size_t tabLane = (size_t)m.dev.nslots * sizeof(double);
-        auto smem_for = [&](int b, bool ldsAcc) {
-            return (ldsAcc ? (size_t)(b / 64) * K * sizeof(double) : 0) + tabLane * b;
-        };
-        bool ldsAcc = true;
-        while (block > 64 && smem_for(block, true) > LDS_LIMIT) block /= 2;
-        if (smem_for(block, true) > LDS_LIMIT) ldsAcc = false;
-        block = 256;
-        while (block > 64 && smem_for(block, ldsAcc) > LDS_LIMIT) block /= 2;
-        const bool tabLds = smem_for(block, ldsAcc) <= LDS_LIMIT;
-        const size_t smem =
-            tabLds ? smem_for(block, ldsAcc) : (ldsAcc ? (size_t)(block / 64) * K * sizeof(double) : 0);
-        const long ntiles = std::max<long>(1, (N + block - 1) / block);
-        const int occ = mpd::occupancy_fused(m.variant, block, smem);
-        const long grid = std::min<long>(ntiles, (long)g_numCU * occ);
-        mpd::Launch L{dim3((unsigned)grid), dim3(block), smem, s};
+        const LaneFusedShape f = lane_fused_shape(m, N);
+        const long grid = f.grid;
+        const bool ldsAcc = f.ldsAcc;
+        mpd::Launch L{dim3((unsigned)grid), dim3(f.block), f.smem, s};
         double* gtab = nullptr;
-        if (!tabLds) {
-            m.gtab.ensure(tabLane * (size_t)grid * block);
+        if (!f.tabLds) {
+            m.gtab.ensure(f.tabLane * (size_t)grid * f.block);
             gtab = m.gtab.as<double>();
         }
         m.partialGrad.ensure(std::max<size_t>(8, (size_t)grid * K * sizeof(double)));
@@ -1220,6 +1275,90 @@ void do_fused(mpart_map_s& m, const mpart_cview& p, const double* sens, long srs
         mpart_cview sv{sens, m.outDim, N, srs, scs};
         do_input_vjp(m, p, klMode, sens ? &sv : nullptr, lsens, lss, 0.0, gscale, *gin, s, "input gradient");
     }
+}
+
+// The fused KL value and gradient (do_fused in KL mode) with every work
+// buffer in `scr` (cap doubles, the no-allocation scratch): no device memory
+// is allocated or freed and nothing is compiled.  The specialized kernels run
+// when an earlier call (or mpart_codegen_prepare) compiled them, on the schedule
+// the autotuner picked then (the default schedule if none was tuned); otherwise
+// the generic lane-per-point kernel runs.  The persistent grid is reduced when
+// its partial rows would not fit in `cap`.
+void do_fused_noalloc(mpart_map_s& m, const mpart_cview& p, double* loss, double* grad, double scale, hipStream_t s,
+                      double* scr, long cap) {
+    check_coeffs(&m);
+    const long N = p.cols, K = m.numCoeffs;
+    if (g_numCU == 0) {
+        int dev = 0;
+        hip_check(hipGetDevice(&dev), "hipGetDevice");
+        hip_check(hipDeviceGetAttribute(&g_numCU, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
+    }
+    const std::string tooBig = "KL without allocation: the map's " + std::to_string(K) +
+                               " coefficients need more than the " + std::to_string(MPART_NOALLOC_SCRATCH_BYTES) +
+                               "-byte scratch";
+    mpg::GenModule* g = (m.genEnabled && codegen_env_enabled() && m.genTried) ? m.gen : nullptr;
+    if (g) {
+        int gblock = 256;
+        size_t gsmem = 0;
+        gen_fused_shape(m, g, gblock, gsmem);
+        const int cfg = (m.fusedChoice < 0) ? 0 : m.fusedChoice;
+        const long ntiles = std::max<long>(1, (N + gblock - 1) / gblock);
+        long grid = std::min<long>(ntiles, (long)g_numCU * m.genOccK[cfg & 1] * ((cfg >> 1) + 1));
+        // Layout: [partial rows: grid x (K + 1)][reduction chunks: <= 64 x (K + 1)] ...
+        // [contiguous copy of strided coefficients: K, at the end].
+        const long extra = (m.coeffStride == 1) ? 0 : K;
+        long rmax = (cap - extra) / (K + 1) - mpd::COLSUM_MAX_CHUNKS;
+        if (rmax < 1) {
+            if (2 * (K + 1) + extra > cap) throw ArgError(tooBig);
+            rmax = 1;
+        }
+        grid = std::min(grid, rmax);
+        const double* w = m.coeffs;
+        if (extra) {
+            double* wc = scr + (cap - K);
+            hip_check(mpd::launch_copy(m.coeffs, 1, K, 0, m.coeffStride, wc, 0, 1, s), "launch copy");
+            w = wc;
+        }
+        MapDev M = m.dev;
+        const double* pts = p.ptr;
+        long prs = p.row_stride, pcs = p.col_stride, NN = N;
+        const double* sp = nullptr;
+        long srs = 0, scs = 0;
+        const double* lp = nullptr;
+        long lss = 0;
+        int kl = 1;
+        double* part = scr;
+        void* args[] = {&M, &w, &pts, &prs, &pcs, &NN, &sp, &srs, &scs, &lp, &lss, &kl, &part};
+        hipFunction_t fn = (cfg & 1) ? g->fused3 : g->fused;
+        if (N > 0) gen_launch(fn, grid, gblock, gsmem, s, args, "launch specialized fused");
+        hip_check(mpd::launch_colsum(part, N > 0 ? grid : 0, K + 1, K + 1, scale, grad, loss, part + grid * (K + 1), s),
+                  "launch reduce");
+        return;
+    }
+    const LaneFusedShape f = lane_fused_shape(m, N);
+    if (!f.tabLds)
+        throw NotImpl("KL without allocation: the map's per-lane basis table does not fit in LDS (it would need a "
+                      "global table)");
+    // Layout: [gradient rows: grid x K (LDS accumulators) or one row of K
+    // (global atomics)][loss partials: grid].
+    long grid = f.grid;
+    if (f.ldsAcc) grid = std::min(grid, cap / (K + 1));
+    else grid = std::min(grid, cap - K);
+    if (grid < 1) throw ArgError(tooBig);
+    double* pg = scr;
+    double* pl = scr + (f.ldsAcc ? grid * K : K);
+    double* gradGlobal = nullptr;
+    if (!f.ldsAcc) {
+        gradGlobal = pg;
+        hip_check(mpd::launch_fill(gradGlobal, 1, K, K, 1, 0.0, s), "launch fill");
+    }
+    mpd::Launch L{dim3((unsigned)grid), dim3(f.block), f.smem, s};
+    if (N > 0)
+        hip_check(mpd::launch_fused(m.variant, L, m.dev, m.coeffs, m.coeffStride, p.ptr, p.row_stride, p.col_stride,
+                                    N, nullptr, 0, 0, nullptr, 0, 1, pg, pl, nullptr, gradGlobal),
+                  "launch fused");
+    hip_check(mpd::launch_reduce_rows(pg, N > 0 ? (f.ldsAcc ? grid : 1) : 0, K, scale, grad, s), "launch reduce");
+    hip_check(mpd::launch_reduce_rows(pl, N > 0 ? grid : 0, 1, scale, loss, s), "launch reduce");
 }
 
 // The KL objective with the reference's launch structure
@@ -2152,6 +2291,21 @@ int mpart_kl_value_and_grad(mpart_map m, mpart_cview pts, double* loss, double* 
             return;
         }
         do_fused(*m, pts, nullptr, 0, 0, nullptr, 0, 1, loss, grad, 1.0 / (double)pts.cols, S(s));
+    });
+}
+
+int mpart_kl_value_and_grad_noalloc(mpart_map m, mpart_cview pts, double* loss, double* grad, mpart_stream s) {
+    return guarded([&] {
+        check_map(m);
+        check_view("KL: pts", pts.rows, pts.cols, m->inDim, pts.cols);
+        if (pts.cols == 0) throw ArgError("KL: no points");
+        if (!loss || !grad) throw ArgError("KL: null output");
+        if (!m->opts.contDeriv)
+            throw NotImpl("KL without allocation: contDeriv = false runs the per-method kernels, which need work "
+                          "buffers");
+        double* scr = noalloc_scratch();
+        do_fused_noalloc(*m, pts, loss, grad, 1.0 / (double)pts.cols, S(s), scr,
+                         (long)(MPART_NOALLOC_SCRATCH_BYTES / sizeof(double)));
     });
 }
 

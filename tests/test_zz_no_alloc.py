@@ -1,0 +1,131 @@
+"""The KL entry point that allocates nothing (mpart_kl_value_and_grad_noalloc),
+called through the C ABI, against mpart_kl_value_and_grad on the same inputs.
+
+Each case makes two calls in a row, the second with fewer points, and
+compares both with the allocating entry point elementwise at 1e-10 relative
+(tests/parity.py).  Cases: the 8-D order-3 triangular map (494 coefficients)
+on its specialized kernels, with the schedule the allocating call tuned;
+the same with strided wrapped coefficients (copied into the scratch); a 3-D
+map on the generic kernel (specialization off); device memory unchanged by a
+call on a fresh map; the discrete-derivative mode refused.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import mpart_amd as mt
+from mpart_amd.data import Stream
+
+from parity import check
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+def _case(name, **kw):
+    print(f"[gpu-case] {name} " + " ".join(f"{k}={v}" for k, v in kw.items()), flush=True)
+
+
+def _kl(m, x, noalloc):
+    loss = torch.full((1,), -1.0, dtype=torch.float64, device="cuda")
+    grad = torch.full((m.numCoeffs,), -1.0, dtype=torch.float64, device="cuda")
+    fn = mt._lib.mpart_kl_value_and_grad_noalloc if noalloc else mt._lib.mpart_kl_value_and_grad
+    rc = fn(m._h, mt._view(x), ctypes.c_void_p(loss.data_ptr()), ctypes.c_void_p(grad.data_ptr()), mt._stream())
+    assert rc == 0, mt._lib.mpart_last_error().decode()
+    return loss, grad
+
+
+def _two_calls(m, x1, x2, what):
+    """No-allocation calls on x1 then x2 (fewer points), then the allocating
+    entry point on both; elementwise comparison."""
+    a1 = _kl(m, x1, True)
+    a2 = _kl(m, x2, True)
+    torch.cuda.synchronize()
+    a1 = [t.cpu().numpy() for t in a1]
+    a2 = [t.cpu().numpy() for t in a2]
+    for x, a, tag in ((x1, a1, "first call"), (x2, a2, "second call, fewer points")):
+        r = [t.cpu().numpy() for t in _kl(m, x, False)]
+        torch.cuda.synchronize()
+        check(a[0], r[0], what=f"{what}: loss, {tag} (N={x.shape[1]})")
+        check(a[1], r[1], what=f"{what}: gradient, {tag} (N={x.shape[1]})")
+        print(f"[no-alloc] {what}, {tag}: bitwise equal {np.array_equal(a[0], r[0]) and np.array_equal(a[1], r[1])}",
+              flush=True)
+
+
+def _c3(seed):
+    m = mt.CreateTriangular(8, 8, 3, mt.MapOptions())
+    s = Stream(seed)
+    c = 0.1 * s.normal(m.numCoeffs)
+    x1 = torch.from_numpy(s.points(8, 200_000)).cuda()
+    x2 = torch.from_numpy(s.points(8, 30_000)).cuda()
+    return m, c, x1, x2
+
+
+def test_noalloc_specialized():
+    _case("noalloc_specialized", N1=200_000, N2=30_000)
+    m, c, x1, x2 = _c3(51)
+    m.SetCoeffs(c)
+    spec, _, err = m.PrepareCodegen()
+    assert spec, err
+    _kl(m, x1, False)  # tunes the fused schedule, which the no-allocation call then uses
+    torch.cuda.synchronize()
+    _two_calls(m, x1, x2, "8-D map, specialized kernels")
+
+
+def test_noalloc_strided_coefficients():
+    _case("noalloc_strided_coefficients", N1=200_000, N2=30_000)
+    m, c, x1, x2 = _c3(52)
+    store = torch.zeros(2 * m.numCoeffs, dtype=torch.float64, device="cuda")
+    store[::2] = torch.from_numpy(c).cuda()
+    m.WrapCoeffs(store[::2])
+    spec, _, err = m.PrepareCodegen()
+    assert spec, err
+    _two_calls(m, x1, x2, "8-D map, strided coefficients")
+
+
+def test_noalloc_generic():
+    _case("noalloc_generic", N1=20_000, N2=3_000)
+    m = mt.CreateTriangular(3, 3, 4, mt.MapOptions())
+    m.SetCodegen(False)
+    s = Stream(53)
+    m.SetCoeffs(0.1 * s.normal(m.numCoeffs))
+    x1 = torch.from_numpy(s.points(3, 20_000)).cuda()
+    x2 = torch.from_numpy(s.points(3, 3_000)).cuda()
+    _two_calls(m, x1, x2, "3-D map, generic kernel")
+
+
+def test_noalloc_allocates_nothing():
+    _case("noalloc_memory")
+    m, c, x1, _ = _c3(54)
+    m.SetCoeffs(c)
+    assert m.PrepareCodegen()[0]
+    _kl(m, x1[:, :1000], True)  # the scratch is set up for this device
+    torch.cuda.synchronize()
+    fresh, c2, x, _ = _c3(55)
+    fresh.SetCoeffs(c2)
+    assert fresh.PrepareCodegen()[0]
+    loss = torch.zeros(1, dtype=torch.float64, device="cuda")
+    grad = torch.zeros(fresh.numCoeffs, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    free0, _ = torch.cuda.mem_get_info()
+    rc = mt._lib.mpart_kl_value_and_grad_noalloc(fresh._h, mt._view(x), ctypes.c_void_p(loss.data_ptr()),
+                                                 ctypes.c_void_p(grad.data_ptr()), mt._stream())
+    torch.cuda.synchronize()
+    free1, _ = torch.cuda.mem_get_info()
+    assert rc == 0, mt._lib.mpart_last_error().decode()
+    # the allocating entry point would size ~4 MB of partial rows on this map
+    print(f"[no-alloc] device free memory before {free0} after {free1}", flush=True)
+    assert free1 >= free0 - (1 << 20)
+    assert torch.isfinite(grad).all()
+
+
+def test_noalloc_refuses_discrete_derivative():
+    _case("noalloc_discrete")
+    m = mt.CreateTriangular(2, 2, 2, mt.MapOptions(contDeriv=False))
+    x = torch.zeros((2, 10), dtype=torch.float64, device="cuda")
+    loss = torch.zeros(1, dtype=torch.float64, device="cuda")
+    grad = torch.zeros(m.numCoeffs, dtype=torch.float64, device="cuda")
+    rc = mt._lib.mpart_kl_value_and_grad_noalloc(m._h, mt._view(x), ctypes.c_void_p(loss.data_ptr()),
+                                                 ctypes.c_void_p(grad.data_ptr()), mt._stream())
+    assert rc == -4, rc
