@@ -872,12 +872,47 @@ MPD_HD void adj_node(const DiagCoef<PM>& dc, double t, double w, const NodeV<FL>
     }
 }
 
+// Depth-first stack of pending right halves for the adaptive rules.  All
+// intervals are dyadic subintervals of [0, 1]: a pending interval is fully
+// described by its level (width 2^-level) once its left end is known, and in
+// depth-first order its left end is the right end of the leaf accepted just
+// before it is popped, whose right-end integrand value is the pending
+// interval's left-end value (the same computed value, propagated by copy).
+// So an entry keeps only the level and the values the left end does not
+// give: the midpoint and right end (Simpson) or the right end
+// (Clenshaw-Curtis).  The top entry lives in registers; deeper entries go to
+// the private array, written only when a push covers a top entry that exists
+// in registers alone and read only when a pop needs an entry held in memory.
 template <int FL>
-struct AsRec {
-    double a, b;
+struct AsEnt {
+    NodeV<FL> fm, fb;
     int level;
-    NodeV<FL> fa, fm, fb;
 };
+template <int FL>
+struct AccEnt {
+    NodeV<FL> fb;
+    int level;
+};
+
+// Push and pop of that stack: stk[0 .. sp-2] in the private array, entry
+// sp-1 in `top` (inReg) and/or stk[sp-1] (inMem).  `top` is a variable of its
+// own, apart from the dynamically indexed array, so that it stays in registers.
+template <class E>
+__host__ __device__ __forceinline__ void qpush(E* stk, E& top, int& sp, bool& inReg, bool& inMem, const E& e) {
+    if (sp > 0 && !inMem) stk[sp - 1] = top;
+    top = e;
+    inReg = true;
+    inMem = false;
+    ++sp;
+}
+template <class E>
+__host__ __device__ __forceinline__ E qpop(const E* stk, E& top, int& sp, bool& inReg, bool& inMem) {
+    if (!inReg) top = stk[sp - 1];
+    --sp;
+    inReg = false;
+    inMem = true;
+    return top;
+}
 
 // AdaptiveSimpson::Integrate (Quadrature.h:553-676): depth-first, left first,
 // accepted intervals add intFine in left-to-right order.
@@ -888,8 +923,10 @@ __host__ __device__ double integrate_as(const QuadDev& Q, const DiagCoef<PM>& dc
     // as written, without FMA contraction, like the reference's x86-64 build:
     // a fused sum rounds differently and can flip a subdivision decision.
 #pragma clang fp contract(off)
-    AsRec<FL> stk[MAXSUB];
+    AsEnt<FL> stk[MAXSUB];
+    AsEnt<FL> top;
     int sp = 0;
+    bool inReg = false, inMem = false;
     NodeV<FL> fa = eval_node<BK, PK, FL, PM>(dc, 0.0);
     NodeV<FL> fb = at1 ? *at1 : eval_node<BK, PK, FL, PM>(dc, 1.0);
     NodeV<FL> fm = eval_node<BK, PK, FL, PM>(dc, 0.5);
@@ -919,21 +956,16 @@ __host__ __device__ double integrate_as(const QuadDev& Q, const DiagCoef<PM>& dc
                 adj_node<BK, FL, PM>(dc, b, wr, fb, A);
             }
             if (sp == 0) break;
-            --sp;
-            a = stk[sp].a;
-            b = stk[sp].b;
-            level = stk[sp].level;
-            fa = stk[sp].fa;
-            fm = stk[sp].fm;
-            fb = stk[sp].fb;
+            const AsEnt<FL> e = qpop(stk, top, sp, inReg, inMem);
+            // the next interval starts where this one ends
+            a = b;
+            fa = fb;
+            level = e.level;
+            b = a + ldexp(1.0, -level);
+            fm = e.fm;
+            fb = e.fb;
         } else {
-            stk[sp].a = m;
-            stk[sp].b = b;
-            stk[sp].level = level + 1;
-            stk[sp].fa = fm;
-            stk[sp].fm = frm;
-            stk[sp].fb = fb;
-            ++sp;
+            qpush(stk, top, sp, inReg, inMem, AsEnt<FL>{frm, fb, level + 1});
             b = m;
             fb = fm;
             fm = flm;
@@ -944,13 +976,6 @@ __host__ __device__ double integrate_as(const QuadDev& Q, const DiagCoef<PM>& dc
     return res;
 }
 
-template <int FL>
-struct AccRec {
-    double a, b;
-    int level;
-    NodeV<FL> fa, fb;
-};
-
 // AdaptiveClenshawCurtis::Integrate (Quadrature.h:819-968).  NF is the fine
 // rule size when known at compile time (5 for quadPts = 5), or MAXNF with the
 // runtime size in Q.nf.
@@ -958,8 +983,10 @@ template <int BK, int PK, int FL, int NF, int PM>
 __host__ __device__ double integrate_acc(const QuadDev& Q, const DiagCoef<PM>& dc, AdjAcc<PM>& A, int* nevals,
                                          const NodeV<FL>* at1) {
 #pragma clang fp contract(off)  // as in integrate_as
-    AccRec<FL> stk[MAXSUB];
+    AccEnt<FL> stk[MAXSUB];
+    AccEnt<FL> top;
     int sp = 0;
+    bool inReg = false, inMem = false;
     const int nf = (NF == MAXNF) ? Q.nf : NF;
     const int nc = (nf - 1) / 2 + 1;
     const int cRight = nc - 1, cMid = cRight / 2;
@@ -1025,19 +1052,14 @@ __host__ __device__ double integrate_acc(const QuadDev& Q, const DiagCoef<PM>& d
                 }
             }
             if (sp == 0) break;
-            --sp;
-            a = stk[sp].a;
-            b = stk[sp].b;
-            level = stk[sp].level;
-            fa = stk[sp].fa;
-            fb = stk[sp].fb;
+            const AccEnt<FL> e = qpop(stk, top, sp, inReg, inMem);
+            a = b;
+            fa = fb;
+            level = e.level;
+            b = a + ldexp(1.0, -level);
+            fb = e.fb;
         } else {
-            stk[sp].a = mid;
-            stk[sp].b = b;
-            stk[sp].level = level + 1;
-            stk[sp].fa = fm;
-            stk[sp].fb = fb;
-            ++sp;
+            qpush(stk, top, sp, inReg, inMem, AccEnt<FL>{fb, level + 1});
             b = mid;
             fb = fm;
             ++level;

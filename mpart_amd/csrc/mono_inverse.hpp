@@ -93,16 +93,18 @@ __global__ void __launch_bounds__(256) k_inverse_setup_batch(const MapDev* __res
                                     stride, rec + S.base + n, R);
 }
 
-// Saved interval of the depth-first quadrature (Simpson also keeps the
-// midpoint value).
+// Pending right half of the depth-first quadrature (the entries of
+// integrate_as / integrate_acc: the interval's left end and its value are
+// those of the leaf accepted before it is popped, its width 2^-level; Simpson
+// also keeps the midpoint value).
 template <int QK>
 struct InvStk {
-    double a, b, fa, fb;
+    double fb;
     int level;
 };
 template <>
 struct InvStk<QK_AS> {
-    double a, b, fa, fm, fb;
+    double fm, fb;
     int level;
 };
 
@@ -186,8 +188,11 @@ __global__ void __launch_bounds__(256) k_inverse_queue(MapDev M, const double* _
     // Depth-first stack: the top entry in registers, the others in scratch
     // (stk[0 .. sp-2]), so that popping does not wait for memory; the load
     // that refills the top completes while the next interval is evaluated.
+    // topInMem: the top was refilled from stk[sp-1] and is unchanged, so a
+    // push need not store it again.
     InvStk<QK> stk[MAXSUB];
     InvStk<QK> top{};
+    bool topInMem = false;
     // Wavefront-uniform range of point indices not yet handed to a lane.
     long qNext = 0, qEnd = 0;
     bool exhausted = false;
@@ -316,22 +321,24 @@ __global__ void __launch_bounds__(256) k_inverse_queue(MapDev M, const double* _
                     if (sp == 0) {
                         qdone = true;
                     } else {
-                        qa = top.a;
-                        qb = top.b;
+                        // the next interval starts where this one ends
+                        qa = b;
+                        fa = fb;
                         level = top.level;
-                        fa = top.fa;
+                        qb = qa + ldexp(1.0, -level);
                         fm = top.fm;
                         fb = top.fb;
-                        if (--sp > 0) top = stk[sp - 1];
+                        if (--sp > 0) {
+                            top = stk[sp - 1];
+                            topInMem = true;
+                        }
                     }
                 } else {
-                    if (sp > 0) stk[sp - 1] = top;
-                    top.a = m;
-                    top.b = b;
+                    if (sp > 0 && !topInMem) stk[sp - 1] = top;
                     top.level = level + 1;
-                    top.fa = fm;
                     top.fm = frm;
                     top.fb = fb;
+                    topInMem = false;
                     ++sp;
                     qb = m;
                     fb = fm;
@@ -378,20 +385,21 @@ __global__ void __launch_bounds__(256) k_inverse_queue(MapDev M, const double* _
                     if (sp == 0) {
                         qdone = true;
                     } else {
-                        qa = top.a;
-                        qb = top.b;
+                        qa = qb;
+                        fa = fb;
                         level = top.level;
-                        fa = top.fa;
+                        qb = qa + ldexp(1.0, -level);
                         fb = top.fb;
-                        if (--sp > 0) top = stk[sp - 1];
+                        if (--sp > 0) {
+                            top = stk[sp - 1];
+                            topInMem = true;
+                        }
                     }
                 } else {
-                    if (sp > 0) stk[sp - 1] = top;
-                    top.a = mid;
-                    top.b = qb;
+                    if (sp > 0 && !topInMem) stk[sp - 1] = top;
                     top.level = level + 1;
-                    top.fa = fm;
                     top.fb = fb;
+                    topInMem = false;
                     ++sp;
                     qb = mid;
                     fb = fm;
