@@ -302,7 +302,12 @@ int mpart_set_plain_launches(mpart_map map, int plain);
  * the first call) and reports *specialized = 1 when the specialized kernels
  * are in use, 0 otherwise, and the compile time of the module in seconds (0
  * when it came from the cache).  mpart_codegen_error returns the reason of a
- * failed compilation ("" otherwise). */
+ * failed compilation ("" otherwise).  When the environment variable
+ * MPART_CODEGEN_CACHE names a writable directory, every compiled module's
+ * code object is stored there (keyed by hashes of the generated source, the
+ * device header, the target and the options) and later processes load it
+ * instead of compiling: they skip the compile time and run the identical
+ * binary, whichever compiler their own hipRTC would load. */
 /* Batches of independent maps.  A batch runs one method over many maps (each
  * with its own points, outputs and coefficients) in one persistent launch:
  * every map's points are split into tiles, the tiles are ordered by
@@ -393,6 +398,17 @@ int mpart_eval_schedule(mpart_map map, int32_t* lanes, double* trialMs);
  * milliseconds, -1 for a schedule left out because its kernel would spill
  * registers.  No reference counterpart. */
 int mpart_fused_schedule(mpart_map map, int32_t* choice, double* trialMs);
+
+/* Resources of one of the map's specialized (hipRTC-compiled) kernels, as the
+ * HIP runtime reports them (hipFuncGetAttribute): registers per lane,
+ * private (scratch) bytes per lane and static LDS bytes.  kernel is one of
+ * "fused", "fused_w3", "eval", "coef", "ldcoef" (first module) or
+ * "fused_in", "ivjp", "klx" (module with input gradients);
+ * MPART_INVALID_ARGUMENT when that kernel is not compiled.  The compiler is
+ * the one the process's hipRTC loads, so a map's kernels can differ between
+ * processes that load different ROCm runtimes.  No reference counterpart. */
+int mpart_codegen_kernel_info(mpart_map map, const char* kernel, int32_t* vgprs, int32_t* scratchBytes,
+                              int32_t* ldsBytes);
 
 #ifdef __cplusplus
 }

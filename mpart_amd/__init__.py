@@ -104,6 +104,7 @@ _SIGS = {
     "mpart_inverse": [_vp, _CView, _CView, _CView, _dbl, _dbl, _vp],
     "mpart_kl_value_and_grad": [_vp, _CView, _vp, _vp, _vp],
     "mpart_kl_value_and_grad_noalloc": [_vp, _CView, _vp, _vp, _vp],
+    "mpart_codegen_kernel_info": [_vp, ctypes.c_char_p, _vp, _vp, _vp],
     "mpart_kl_value_and_grad_plain": [_vp, _CView, _vp, _vp, _vp],
     "mpart_coeff_vjp": [_vp, _CView, _CView, _vp, _i64, _vp, _vp],
     "mpart_kl_input_grad": [_vp, _CView, _dbl, _CView, _vp],
@@ -444,6 +445,13 @@ class ConditionalMap:
         ms = (ctypes.c_double * 12)()
         _check(_lib.mpart_eval_schedule(self._h, lanes, ms))
         return (lanes[0], lanes[1], lanes[2]), [list(ms[0:4]), list(ms[4:8]), list(ms[8:12])]
+
+    def KernelInfo(self, kernel="fused"):
+        """(registers, scratch bytes, LDS bytes) of a compiled specialized kernel."""
+        v, sc, lds = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        _check(_lib.mpart_codegen_kernel_info(self._h, kernel.encode(), ctypes.byref(v), ctypes.byref(sc),
+                                              ctypes.byref(lds)))
+        return int(v.value), int(sc.value), int(lds.value)
 
     def FusedSchedule(self):
         """(schedule index or -1 before tuning, trial times in ms) of the fused kernel."""

@@ -3,7 +3,12 @@
 
 #include <hip/hiprtc.h>
 
+#include <unistd.h>
+
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <initializer_list>
 #include <memory>
 #include <mutex>
 #include <sstream>
@@ -440,6 +445,59 @@ extern "C" __global__ void __launch_bounds__(256) mpart_gen_ldcoef(MapDev M, con
 namespace {
 std::mutex g_mu;
 std::unordered_map<std::string, std::unique_ptr<GenModule>> g_cache;
+
+const char* const kCompileOpts[] = {"-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+
+// 64-bit FNV-1a over several byte strings (separated by a zero byte), from
+// the offset basis `h`.
+uint64_t fnv1a(std::initializer_list<const std::string*> parts, uint64_t h) {
+    for (const std::string* s : parts) {
+        for (unsigned char c : *s) h = (h ^ c) * 1099511628211ull;
+        h = (h ^ 0xffu) * 1099511628211ull;
+    }
+    return h;
+}
+
+// Path of a module's code object in the directory MPART_CODEGEN_CACHE names
+// ("" when unset): the name carries two independent 64-bit hashes of the
+// generated source, the embedded device header, the target and the options.
+std::string cache_path(const std::string& src, const std::string& arch) {
+    const char* dir = std::getenv("MPART_CODEGEN_CACHE");
+    if (!dir || !*dir) return "";
+    const std::string hdr(kDeviceHeader);
+    std::string opts;
+    for (const char* o : kCompileOpts) opts += std::string(o) + " ";
+    const uint64_t h1 = fnv1a({&src, &hdr, &arch, &opts}, 14695981039346656037ull);
+    const uint64_t h2 = fnv1a({&opts, &arch, &hdr, &src}, 0x9e3779b97f4a7c15ull);
+    char name[96];
+    std::snprintf(name, sizeof(name), "/mpart_gen_%016llx%016llx_", (unsigned long long)h1, (unsigned long long)h2);
+    return std::string(dir) + name + arch + ".co";
+}
+
+bool read_cached(const std::string& path, std::vector<char>& code) {
+    if (path.empty()) return false;
+    std::FILE* f = std::fopen(path.c_str(), "rb");
+    if (!f) return false;
+    std::vector<char> buf;
+    char tmp[1 << 16];
+    size_t n;
+    while ((n = std::fread(tmp, 1, sizeof(tmp), f)) > 0) buf.insert(buf.end(), tmp, tmp + n);
+    std::fclose(f);
+    // an ELF code object
+    if (buf.size() < 64 || buf[0] != 0x7f || buf[1] != 'E' || buf[2] != 'L' || buf[3] != 'F') return false;
+    code.swap(buf);
+    return true;
+}
+
+void write_cached(const std::string& path, const std::vector<char>& code) {
+    if (path.empty()) return;
+    const std::string tmp = path + ".tmp" + std::to_string((long)getpid());
+    std::FILE* f = std::fopen(tmp.c_str(), "wb");
+    if (!f) return;
+    const bool ok = std::fwrite(code.data(), 1, code.size(), f) == code.size();
+    if (std::fclose(f) == 0 && ok) std::rename(tmp.c_str(), path.c_str());
+    else std::remove(tmp.c_str());
+}
 }  // namespace
 
 bool compile_source(const std::string& src, const std::string& arch, std::vector<char>& code, std::string& err) {
@@ -451,8 +509,8 @@ bool compile_source(const std::string& src, const std::string& arch, std::vector
         return false;
     }
     const std::string archOpt = "--offload-arch=" + arch;
-    const char* opts[] = {archOpt.c_str(), "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
-                          "-packed-fp32-ops"};
+    const char* opts[7] = {archOpt.c_str()};
+    for (int i = 0; i < 6; ++i) opts[i + 1] = kCompileOpts[i];
     if (hiprtcCompileProgram(prog, 7, opts) != HIPRTC_SUCCESS) {
         size_t ls = 0;
         hiprtcGetProgramLogSize(prog, &ls);
@@ -483,8 +541,16 @@ GenModule* get_module(const GenSpec& spec, std::string& err, int part) {
         const std::string a = prop.gcnArchName;
         arch = a.substr(0, a.find(':'));
     }
+    // A code object compiled earlier (by this or another process) is reused
+    // from MPART_CODEGEN_CACHE when that names a directory; a process then
+    // runs exactly the binary the first one compiled, whatever compiler its
+    // own hipRTC would load.
     std::vector<char> code;
-    if (!compile_source(src, arch, code, err)) return nullptr;
+    const std::string cpath = cache_path(src, arch);
+    if (!read_cached(cpath, code)) {
+        if (!compile_source(src, arch, code, err)) return nullptr;
+        write_cached(cpath, code);
+    }
     auto m = std::make_unique<GenModule>();
     if (hipModuleLoadData(&m->mod, code.data()) != hipSuccess) {
         err = "hipModuleLoadData failed";

@@ -2460,6 +2460,35 @@ int mpart_fused_schedule(mpart_map m, int32_t* choice, double* trialMs) {
     });
 }
 
+int mpart_codegen_kernel_info(mpart_map m, const char* kernel, int32_t* vgprs, int32_t* scratchBytes,
+                              int32_t* ldsBytes) {
+    return guarded([&] {
+        check_map(m);
+        if (!kernel) throw ArgError("null kernel name");
+        const std::string k(kernel);
+        mpg::GenModule* g = m->gen;
+        mpg::GenModule* gi = m->genIn;
+        hipFunction_t f = nullptr;
+        if (k == "fused") f = g ? g->fused : nullptr;
+        else if (k == "fused_w3") f = g ? g->fused3 : nullptr;
+        else if (k == "eval") f = g ? g->eval : nullptr;
+        else if (k == "coef") f = g ? g->coef : nullptr;
+        else if (k == "ldcoef") f = g ? g->ldcoef : nullptr;
+        else if (k == "fused_in") f = gi ? gi->fusedIn : nullptr;
+        else if (k == "ivjp") f = gi ? gi->ivjp : nullptr;
+        else if (k == "klx") f = gi ? gi->klx : nullptr;
+        else throw ArgError("unknown specialized kernel '" + k + "'");
+        if (!f) throw ArgError("specialized kernel '" + k + "' is not compiled for this map");
+        int v = 0, sc = 0, lds = 0;
+        hip_check(hipFuncGetAttribute(&v, HIP_FUNC_ATTRIBUTE_NUM_REGS, f), "hipFuncGetAttribute");
+        hip_check(hipFuncGetAttribute(&sc, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, f), "hipFuncGetAttribute");
+        hip_check(hipFuncGetAttribute(&lds, HIP_FUNC_ATTRIBUTE_SHARED_SIZE_BYTES, f), "hipFuncGetAttribute");
+        if (vgprs) *vgprs = v;
+        if (scratchBytes) *scratchBytes = sc;
+        if (ldsBytes) *ldsBytes = lds;
+    });
+}
+
 int mpart_create_batch(const mpart_map* maps, uint32_t n, mpart_batch* out) {
     return guarded([&] {
         if (!out) throw ArgError("null argument");
