@@ -435,16 +435,18 @@ class ConditionalMap:
         return bool(spec.value), float(secs.value), _lib.mpart_codegen_error(self._h).decode()
 
     def SetPointGroup(self, lanes=0):
-        """Lanes per point of the generic Evaluate / LogDeterminant / fused pass (1, 4, 8, 64; 0 = autotune)."""
+        """Lanes per point of the generic Evaluate / LogDeterminant / fused pass (1, 4, 8, 64; 0 = autotune;
+        2 = the fused pass with the components split over the wavefronts of a block)."""
         _check(_lib.mpart_set_point_group(self._h, int(lanes)))
 
     def EvalSchedule(self):
-        """((lanes for Evaluate, LogDeterminant, fused pass), trial ms for lanes 1, 4, 8, 64 of each)
-        of the generic kernels (0: not tuned)."""
+        """((lanes for Evaluate, LogDeterminant, fused pass), trial ms of the generic kernels: lanes 1, 4, 8,
+        64 for each Evaluate kind, lanes 1, 4, 8, 64 and the split schedule (code 2) for the fused pass)
+        (0: not tuned)."""
         lanes = (ctypes.c_int32 * 3)()
-        ms = (ctypes.c_double * 12)()
+        ms = (ctypes.c_double * 13)()
         _check(_lib.mpart_eval_schedule(self._h, lanes, ms))
-        return (lanes[0], lanes[1], lanes[2]), [list(ms[0:4]), list(ms[4:8]), list(ms[8:12])]
+        return (lanes[0], lanes[1], lanes[2]), [list(ms[0:4]), list(ms[4:8]), list(ms[8:13])]
 
     def KernelInfo(self, kernel="fused"):
         """(registers, scratch bytes, LDS bytes) of a compiled specialized kernel."""

@@ -236,6 +236,11 @@ struct mpart_map_s {
     // device plan
     MapDev dev{};
     DBuf dComps, dGroups, dTerms, dSlotDim, dDimSlot, dDimDeg;
+    // Component-split fused schedule: [waveStart[0..4] | component order]
+    DBuf dSplit;
+    int splitOcc[2] = {0, 0};   // resident blocks per CU of the two builds
+    int splitMinB = 0;          // build the tuner chose (1 or 2; 0: not timed, 1 is used)
+    double splitTrialMs[2] = {0, 0};
 
     // coefficients
     DBuf ownCoeffs;
@@ -278,7 +283,9 @@ struct mpart_map_s {
     double evalTrialMs[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
     int pointGroupForced = 0;
     int fusedGroup = 0;  // the same for the generic fused pass
-    double fusedGroupTrialMs[4] = {0, 0, 0, 0};
+    // trials of the generic fused pass: lanes per point 1, 4, 8, 64, then the
+    // component-split schedule
+    double fusedGroupTrialMs[5] = {0, 0, 0, 0, 0};
     double fusedTrialMs[4] = {0, 0, 0, 0};
 };
 
@@ -500,6 +507,31 @@ void build_plan(mpart_map_s& m) {
         }
         comps.push_back(cd);
     }
+    // Component-split schedule: the components over the four wavefronts of a
+    // block, longest first to the least loaded wavefront (cost ~ terms, plus
+    // the quadrature).
+    {
+        std::vector<int> order(comps.size());
+        for (size_t c = 0; c < comps.size(); ++c) order[c] = (int)c;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int a, int b) { return comps[a].nterms > comps[b].nterms; });
+        std::vector<std::vector<int>> per(mpd::FUSED_SPLIT_WAVES);
+        std::vector<double> load(mpd::FUSED_SPLIT_WAVES, 0.0);
+        for (int c : order) {
+            const int wv = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            per[wv].push_back(c);
+            load[wv] += 2.0 * comps[c].nterms + 60.0;
+        }
+        std::vector<int> plan(mpd::FUSED_SPLIT_WAVES + 1 + comps.size());
+        int k = mpd::FUSED_SPLIT_WAVES + 1;
+        for (int wv = 0; wv < mpd::FUSED_SPLIT_WAVES; ++wv) {
+            plan[wv] = k - (mpd::FUSED_SPLIT_WAVES + 1);
+            std::sort(per[wv].begin(), per[wv].end());
+            for (int c : per[wv]) plan[k++] = c;
+        }
+        plan[mpd::FUSED_SPLIT_WAVES] = (int)comps.size();
+        upload(m.dSplit, plan);
+    }
     upload(m.dComps, comps);
     upload(m.dGroups, groups);
     upload(m.dTerms, terms);
@@ -686,6 +718,8 @@ void gen_eval(mpart_map_s& m, mpg::GenModule* g, const double* w, const mpart_cv
 }
 
 constexpr long EVAL_TUNE_MIN_POINTS = 16384;
+// Point-schedule code of the component-split fused pass (mono_split.hpp).
+constexpr int FUSED_SPLIT = 2;
 
 // Generic Evaluate and/or LogDeterminant of components [cb, ce) with the
 // map's point schedule: a lane per point (k_eval) or a group of 4, 8 or 64
@@ -696,6 +730,7 @@ void generic_eval(mpart_map_s& m, const mpart_cview& p, double* out, long ors, l
                   int ldAcc, int32_t* nevals, int cb, int ce, hipStream_t s) {
     const long N = p.cols;
     auto run = [&](int G) {
+        if (G == FUSED_SPLIT) G = 1;  // the split schedule is the fused pass's only
         if (G > 1) {
             const size_t tabBytes = (size_t)m.dev.nslots * sizeof(double);
             int block = 256;
@@ -1237,11 +1272,39 @@ void do_fused(mpart_map_s& m, const mpart_cview& p, const double* sens, long srs
                   "launch reduce");
         return true;
     };
+    // Components split over the wavefronts of a block (k_fused_split): false
+    // when the tile's table does not fit or the map has one component.
+    auto run_split = [&](int minb) -> bool {
+        const size_t smem = (size_t)m.dev.nslots * mpd::FUSED_SPLIT_TILE * sizeof(double);
+        if (smem > LDS_LIMIT || m.comps.size() < 2) return false;
+        if (minb != 2) minb = 1;
+        int& occ = m.splitOcc[minb - 1];
+        if (occ == 0) occ = mpd::occupancy_fused_split(m.variant, minb, smem);
+        const long ntiles = std::max<long>(1, (N + mpd::FUSED_SPLIT_TILE - 1) / mpd::FUSED_SPLIT_TILE);
+        const long grid = std::min<long>(ntiles, (long)g_numCU * occ);
+        m.partialGrad.ensure((size_t)(grid + mpd::COLSUM_MAX_CHUNKS) * (K + 1) * sizeof(double));
+        double* rowsp = m.partialGrad.as<double>();
+        const int* plan = m.dSplit.as<int>();
+        mpd::Launch L{dim3((unsigned)grid), dim3(mpd::FUSED_SPLIT_WAVES * 64), smem, s};
+        if (N > 0)
+            hip_check(mpd::launch_fused_split(m.variant, minb, L, m.dev, m.coeffs, m.coeffStride, p.ptr, p.row_stride,
+                                              p.col_stride, N, sens, srs, scs, lsens, lss, klMode,
+                                              plan + mpd::FUSED_SPLIT_WAVES + 1, plan, rowsp),
+                      "launch fused (components split over wavefronts)");
+        hip_check(mpd::launch_colsum(rowsp, N > 0 ? grid : 0, K + 1, loss ? K + 1 : K, scale, grad, loss,
+                                     rowsp + grid * (K + 1), s),
+                  "launch reduce");
+        return true;
+    };
     auto run = [&](int G) {
+        if (G == FUSED_SPLIT) {
+            if (run_split(m.splitMinB)) return;
+            G = 1;
+        }
         if (G > 1 && run_grp(G)) return;
         run_lane();
     };
-    static const int kGroups[4] = {1, 4, 8, 64};
+    static const int kGroups[5] = {1, 4, 8, 64, FUSED_SPLIT};
     if (m.pointGroupForced > 0) {
         run(m.pointGroupForced);
     } else {
@@ -1252,8 +1315,21 @@ void do_fused(mpart_map_s& m, const mpart_cview& p, const double* sens, long srs
                 hipEvent_t e0, e1;
                 hip_check(hipEventCreate(&e0), "hipEventCreate");
                 hip_check(hipEventCreate(&e1), "hipEventCreate");
+                // the split schedule's two builds (one or two wavefronts per
+                // SIMD) first; the faster represents it below
+                for (int b = 0; b < 2; ++b) {
+                    if (!run_split(b + 1)) break;
+                    hip_check(hipEventRecord(e0, s), "hipEventRecord");
+                    run_split(b + 1);
+                    hip_check(hipEventRecord(e1, s), "hipEventRecord");
+                    hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+                    float ms = 0.f;
+                    hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+                    m.splitTrialMs[b] = ms;
+                    m.splitMinB = (b == 0 || ms < m.splitTrialMs[0]) ? b + 1 : 1;
+                }
                 int best = 0;
-                for (int i = 0; i < 4; ++i) {
+                for (int i = 0; i < 5; ++i) {
                     run(kGroups[i]);
                     hip_check(hipEventRecord(e0, s), "hipEventRecord");
                     run(kGroups[i]);
@@ -2431,8 +2507,8 @@ const char* mpart_codegen_error(mpart_map m) { return m ? m->genErr.c_str() : "n
 int mpart_set_point_group(mpart_map m, int32_t lanes) {
     return guarded([&] {
         check_map(m);
-        if (lanes != 0 && lanes != 1 && lanes != 4 && lanes != 8 && lanes != 64)
-            throw ArgError("lanes per point must be 0, 1, 4, 8 or 64");
+        if (lanes != 0 && lanes != 1 && lanes != FUSED_SPLIT && lanes != 4 && lanes != 8 && lanes != 64)
+            throw ArgError("lanes per point must be 0, 1, 2 (fused pass split over wavefronts), 4, 8 or 64");
         m->pointGroupForced = lanes;
     });
 }
@@ -2447,7 +2523,7 @@ int mpart_eval_schedule(mpart_map m, int32_t* lanes, double* trialMs) {
         }
         if (lanes) lanes[2] = m->fusedGroup;
         if (trialMs)
-            for (int i = 0; i < 4; ++i) trialMs[8 + i] = m->fusedGroupTrialMs[i];
+            for (int i = 0; i < 5; ++i) trialMs[8 + i] = m->fusedGroupTrialMs[i];
     });
 }
 

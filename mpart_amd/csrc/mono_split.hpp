@@ -1,0 +1,103 @@
+// mono_split.hpp -- fused forward + adjoint for maps with many terms, with the
+// components of a map split over the wavefronts of a block.
+//
+// A block of four wavefronts takes a tile of 64 points at a time.  The tile's
+// basis table (every coordinate's values phi_1..phi_deg, slot-major: entry
+// (slot, point) at tab[slot * 64 + point]) sits once in LDS, filled by all
+// four wavefronts, so it costs 8 * nslots bytes per point instead of per lane.
+// Each wavefront then runs a lane per point over the components assigned to
+// it (a host-side partition of the components balanced by term count): the
+// C_q sums, the component's quadrature and the gradient pass, whose term loop
+// is uniform over the wavefront (every lane reads the same term record, so
+// the records come through the scalar cache).  No quadrature is replicated,
+// unlike the lanes-per-point schedules, and no wavefront waits on another
+// except at the tile boundary.  Gradient sums go to the block's own partial
+// row with return-less atomics: a coefficient belongs to one component and a
+// component to one wavefront, so every row entry has a single writer and the
+// order of its additions is fixed.
+#pragma once
+#include "mono_kernels.hpp"
+
+namespace mpd {
+
+constexpr int SPLIT_TILE = 64;   // points per tile
+constexpr int SPLIT_WAVES = 4;   // wavefronts per block
+
+// MINB: resident blocks per CU the compiler must allow (1, or 2 to cap the
+// registers at two wavefronts per SIMD at the cost of spills).
+template <int BK, int PK, int QK, int PM, int MINB>
+__global__ void __launch_bounds__(256, MINB) k_fused_split(MapDev M, const double* w, long ws, PtsArg P, const double* sens,
+                                                     long srs, long scs, const double* lsens, long lss, int klMode,
+                                                     const int* compOrder, const int* waveStart, double* rows) {
+    extern __shared__ double tab[];
+    __shared__ double lossWave[SPLIT_WAVES];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int K = M.numCoeffs;
+    double* row = rows + (long)blockIdx.x * (K + 1);
+    for (int j = threadIdx.x; j <= K; j += blockDim.x) row[j] = 0.0;
+    __threadfence();  // the zeros are in memory before any wavefront adds to the row
+    __syncthreads();
+    const int ndims = M.D - 1;
+    const int c0 = waveStart[wave], c1 = waveStart[wave + 1];
+    double lossLane = 0.0;
+    const long ntiles = (P.N + SPLIT_TILE - 1) / SPLIT_TILE;
+    for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const long n0 = tile * SPLIT_TILE + lane;
+        const bool active = n0 < P.N;
+        const long n = active ? n0 : P.N - 1;
+        const double* pt = P.p + n * P.cs;
+        __syncthreads();  // the previous tile's table is no longer read
+        if (wave == 0) tab[lane] = 1.0;
+        for (int d = wave; d < ndims; d += SPLIT_WAVES) {
+            const int deg = M.dimDeg[d];
+            if (deg > 0) values_1d<BK>(M, deg, pt[d * P.rs], tab + (long)(M.dimSlot[d] - 1) * SPLIT_TILE + lane, SPLIT_TILE);
+        }
+        __syncthreads();
+        const double* mytab = tab + lane;
+        for (int i = c0; i < c1; ++i) {
+            const CompDev cd = M.comps[compOrder[i]];
+            const double* wc = w + (long)cd.coeffOff * ws;
+            double C[PM + 1];
+            coeff_groups<PM>(M, cd, wc, ws, mytab, SPLIT_TILE, C);
+            const double xk = pt[(long)(cd.dim - 1) * P.rs];
+            const double sv = (!klMode && sens) ? sens[(long)cd.outRow * srs + n * scs] : 0.0;
+            const double lv = (!klMode && lsens) ? lsens[n * lss] : 0.0;
+            double G[PM + 1];
+            comp_core<BK, PK, QK, PM>(M, cd, C, xk, active, klMode, sv, lv, G, lossLane);
+            // Gradient pass: chunks of 16 term records per group, each chunk
+            // reduced over the tile's points by the wavefront.
+#pragma unroll
+            for (int q = 0; q <= PM; ++q) {
+                if (q > cd.p) break;
+                const GroupDev g = M.groups[cd.groupOff + q];
+                const double Gq = G[q];
+                for (int j0 = g.start; j0 < g.start + g.countPad; j0 += 16) {
+                    double v[16];
+#pragma unroll
+                    for (int u = 0; u < 16; ++u) {
+                        const uint32_t* rec = M.terms + (long)(j0 + u) * M.trec;
+                        double Pj = mytab[(long)rec[1] * SPLIT_TILE];
+                        for (int e = 2; e <= M.nd; ++e) Pj *= mytab[(long)rec[e] * SPLIT_TILE];
+                        v[u] = active ? Pj * Gq : 0.0;
+                    }
+                    const double rsum = wave_reduce16(v, lane);
+                    if ((lane & 3) == 0) {
+                        const uint32_t idx = M.terms[(long)(j0 + (lane >> 2)) * M.trec];
+                        if (idx != DUMMY) (void)unsafeAtomicAdd(row + cd.coeffOff + idx, rsum);
+                    }
+                }
+            }
+        }
+    }
+    const double l = wave_sum(lossLane);
+    if (lane == 0) lossWave[wave] = l;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int i = 0; i < SPLIT_WAVES; ++i) t += lossWave[i];
+        row[K] = t;
+    }
+}
+
+}  // namespace mpd

@@ -88,11 +88,12 @@ def test_point_group_argument_checked():
 
 
 @pytest.mark.parametrize("ci", [0, 1, 2, 6, 9, 13])
-@pytest.mark.parametrize("lanes", [4, 8, 64])
+@pytest.mark.parametrize("lanes", [2, 4, 8, 64])
 def test_group_fused_vs_oracle(ci, lanes):
     # KL value + gradient and the coefficient VJP of the generic fused pass
-    # with 8 / 64 lanes per point, against the oracle and the lane-per-point
-    # pass; 701 points leave part of the last wavefront step without a point.
+    # with 4 / 8 / 64 lanes per point, or (2) the components split over the
+    # wavefronts of a block, against the oracle and the lane-per-point pass;
+    # 701 points leave part of the last wavefront step (tile) without a point.
     _case("point_group_fused", case=CASES[ci], lanes=lanes)
     s = Stream(2100 + ci)
     g = mt.CreateTriangular(4, 4, 3, gpu_opts(**CASES[ci]))
@@ -132,7 +133,7 @@ def test_uneven_32d_map_fused_schedules_and_tuner():
     r.SetCoeffs(c)
     x = np.ascontiguousarray(student_t_points(32, 150, nu=5, seed=9))
     lr, gr = r.KL(x)
-    for lanes in (1, 4, 8, 64):
+    for lanes in (1, 2, 4, 8, 64):
         g.SetPointGroup(lanes)
         lf, gf = g.KLValueAndGrad(x)
         close(lf, lr, what=f"KL loss lanes={lanes}")
@@ -142,7 +143,7 @@ def test_uneven_32d_map_fused_schedules_and_tuner():
     lt, gt = g.KLValueAndGrad(xl)
     (_, _, lf_), trials = g.EvalSchedule()
     print(f"[point_group] tuned lanes fused={lf_} trials={trials[2]}", flush=True)
-    assert lf_ in (1, 4, 8, 64)
+    assert lf_ in (1, 2, 4, 8, 64) and len(trials[2]) == 5 and all(t > 0 for t in trials[2])
     g.SetPointGroup(1)
     l1, g1 = g.KLValueAndGrad(xl)
     close(lt.cpu().numpy(), l1.cpu().numpy(), rtol=1e-12, what="tuned KL loss vs lane per point")
