@@ -560,11 +560,10 @@ def uneven_map_workload(steps):
     out["fused_vs_plain_20000"] = out["kl_plain_20000_ms"] / out["kl_fused_20000_ms"]
     lanes, trials = g.EvalSchedule()
     out["lanes_per_point"] = {"evaluate": lanes[0], "log_determinant": lanes[1], "fused": lanes[2],
-                              "trial_ms": {"evaluate_lanes_1_4_8_64": trials[0],
-                                           "log_determinant_lanes_1_4_8_64": trials[1],
-                                           "fused_lanes_1_4_8_64_split": trials[2]},
-                              "meaning": "lanes per point; 2 for the fused pass: components split over the "
-                                         "wavefronts of a block"}
+                              "trial_ms_lanes_1_4_8_64_split": {"evaluate": trials[0],
+                                                                "log_determinant": trials[1],
+                                                                "fused": trials[2]},
+                              "meaning": "lanes per point; 2: components split over the wavefronts of a block"}
     return out
 
 

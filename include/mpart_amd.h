@@ -379,17 +379,15 @@ const char* mpart_codegen_error(mpart_map map);
  * 4, 8 or 64 (a group of lanes per point sharing one table and splitting the
  * terms).  mpart_set_point_group forces it (0: autotune, the default: the
  * first call with >= 16384 points times the four and keeps the fastest).
- * The generic fused KL / VJP pass has the same schedules plus a fifth, code
- * 2: the components split over the four wavefronts of a block, which share
- * one basis table for a tile of 64 points and run a lane per point over
- * their own components (for maps with several components); it has its own
- * tuning.  Forcing 2 selects that schedule for the fused pass and a lane per
- * point for Evaluate / LogDeterminant.  mpart_eval_schedule reports the
+ * A fifth schedule, code 2, splits the components over the four wavefronts
+ * of a block, which share one basis table for a tile of 64 points and run a
+ * lane per point over their own components (maps with several components,
+ * all components in one call).  The generic fused KL / VJP pass has the same
+ * five schedules and its own tuning.  mpart_eval_schedule reports the
  * choice for [Evaluate, LogDeterminant only, fused pass] in lanes[3] (0: not
- * tuned yet) and the trial times in ms in trialMs[13]: four for each
- * Evaluate kind (lanes 1, 4, 8, 64), then five for the fused pass (1, 4, 8,
- * 64, split).  No counterpart in the reference, which runs a thread per
- * point (MParT/Utilities/KokkosHelpers.h:17-28). */
+ * tuned yet) and the trial times in ms in trialMs[15], five per kind (lanes
+ * 1, 4, 8, 64, then the split schedule).  No counterpart in the reference,
+ * which runs a thread per point (MParT/Utilities/KokkosHelpers.h:17-28). */
 int mpart_set_point_group(mpart_map map, int32_t lanes);
 int mpart_eval_schedule(mpart_map map, int32_t* lanes, double* trialMs);
 

@@ -436,17 +436,16 @@ class ConditionalMap:
 
     def SetPointGroup(self, lanes=0):
         """Lanes per point of the generic Evaluate / LogDeterminant / fused pass (1, 4, 8, 64; 0 = autotune;
-        2 = the fused pass with the components split over the wavefronts of a block)."""
+        2 = the components split over the wavefronts of a block)."""
         _check(_lib.mpart_set_point_group(self._h, int(lanes)))
 
     def EvalSchedule(self):
-        """((lanes for Evaluate, LogDeterminant, fused pass), trial ms of the generic kernels: lanes 1, 4, 8,
-        64 for each Evaluate kind, lanes 1, 4, 8, 64 and the split schedule (code 2) for the fused pass)
-        (0: not tuned)."""
+        """((lanes for Evaluate, LogDeterminant, fused pass), trial ms of the generic kernels for each: lanes
+        1, 4, 8, 64, then the split schedule (code 2)) (0: not tuned)."""
         lanes = (ctypes.c_int32 * 3)()
-        ms = (ctypes.c_double * 13)()
+        ms = (ctypes.c_double * 15)()
         _check(_lib.mpart_eval_schedule(self._h, lanes, ms))
-        return (lanes[0], lanes[1], lanes[2]), [list(ms[0:4]), list(ms[4:8]), list(ms[8:13])]
+        return (lanes[0], lanes[1], lanes[2]), [list(ms[0:5]), list(ms[5:10]), list(ms[10:15])]
 
     def KernelInfo(self, kernel="fused"):
         """(registers, scratch bytes, LDS bytes) of a compiled specialized kernel."""

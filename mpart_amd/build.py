@@ -1,6 +1,7 @@
 """Builds mpart_amd/libmpart_amd.so from mpart_amd/csrc/*.hip for gfx950.
 
-Usage: ``python mpart_amd/build.py`` (also called by ``__graft_entry__.build()``).
+Usage: ``python mpart_amd/build.py [--incremental]`` (also called by
+``__graft_entry__.build()``, which compiles everything).
 Objects go to mpart_amd/_build/; the translation units compile in parallel.
 """
 import concurrent.futures as cf
@@ -54,12 +55,27 @@ def _compile(src):
     return obj
 
 
-def build(verbose=True, jobs=None):
+def _stale(src):
+    """An object is stale when it is missing or older than its source or any
+    header of csrc/ (headers are shared by every translation unit)."""
+    obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
+    if not os.path.exists(obj):
+        return True
+    t = os.path.getmtime(obj)
+    deps = [os.path.join(CSRC, src)] + [os.path.join(CSRC, f) for f in os.listdir(CSRC)
+                                        if f.endswith((".hpp", ".inc"))]
+    deps.append(os.path.join(HERE, "..", "include", "mpart_amd.h"))
+    return any(os.path.getmtime(d) > t for d in deps if os.path.exists(d))
+
+
+def build(verbose=True, jobs=None, incremental=False):
     os.makedirs(BUILD, exist_ok=True)
     embed_device_header()
     jobs = jobs or min(len(SOURCES), max(1, (os.cpu_count() or 4)))
+    todo = [src for src in SOURCES if not incremental or _stale(src)]
     with cf.ThreadPoolExecutor(jobs) as ex:
-        objs = list(ex.map(_compile, SOURCES))
+        list(ex.map(_compile, todo))
+    objs = [os.path.join(BUILD, os.path.splitext(src)[0] + ".o") for src in SOURCES]
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", tmp, *objs, f"-L{ROCM_LIB}", "-lhiprtc"]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -72,5 +88,7 @@ def build(verbose=True, jobs=None):
 
 
 if __name__ == "__main__":
-    build(verbose=True)
+    # --incremental recompiles only stale objects (for development; the
+    # default, and __graft_entry__.build(), compiles everything)
+    build(verbose=True, incremental="--incremental" in sys.argv)
     sys.exit(0)

@@ -41,6 +41,11 @@ hipError_t launch_fused_split(int var, int minb, const Launch& L, const MapDev& 
                               const double* lsens, long lss, int klMode, const int* compOrder, const int* waveStart,
                               double* rows);
 int occupancy_fused_split(int var, int minb, size_t smem);
+// Evaluate / LogDeterminant of every component with the same split.
+hipError_t launch_eval_split(int var, const Launch& L, const MapDev& M, const double* w, long ws, const double* pts,
+                             long prs, long pcs, long N, double* out, long ors, long ocs, double* ld, long lds,
+                             int ldAccumulate, int* nevals, const int* compOrder, const int* waveStart);
+int occupancy_eval_split(int var, size_t smem);
 hipError_t launch_coeff_grad(int var, const Launch& L, const MapDev& M, const double* w, long ws, const double* pts,
                              long prs, long pcs, long N, const double* sens, long srs, long scs, double* out, long ors,
                              long ocs, int c, double* gtab);

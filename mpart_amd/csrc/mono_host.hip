@@ -192,6 +192,16 @@ unsigned factorial_u32(unsigned d) {
 
 int g_numCU = 0;
 
+// Compute units of the current device (queried once).
+int num_cu() {
+    if (g_numCU == 0) {
+        int dev = 0;
+        hip_check(hipGetDevice(&dev), "hipGetDevice");
+        hip_check(hipDeviceGetAttribute(&g_numCU, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
+    }
+    return g_numCU;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -280,7 +290,8 @@ struct mpart_map_s {
     // (1, 4, 8 or 64) for [Evaluate (with or without log-det), log-det only];
     // 0: not tuned yet.  pointGroupForced > 0 overrides the tuner.
     int evalGroup[2] = {0, 0};
-    double evalTrialMs[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+    double evalTrialMs[2][5] = {{0, 0, 0, 0, 0}, {0, 0, 0, 0, 0}};
+    int evalSplitOcc = 0;
     int pointGroupForced = 0;
     int fusedGroup = 0;  // the same for the generic fused pass
     // trials of the generic fused pass: lanes per point 1, 4, 8, 64, then the
@@ -730,7 +741,25 @@ void generic_eval(mpart_map_s& m, const mpart_cview& p, double* out, long ors, l
                   int ldAcc, int32_t* nevals, int cb, int ce, hipStream_t s) {
     const long N = p.cols;
     auto run = [&](int G) {
-        if (G == FUSED_SPLIT) G = 1;  // the split schedule is the fused pass's only
+        if (G == FUSED_SPLIT) {
+            // components split over the wavefronts of a block: every component
+            // at once only, and only when the tile's table fits
+            const size_t smem = (size_t)m.dev.nslots * mpd::FUSED_SPLIT_TILE * sizeof(double);
+            if (cb == 0 && ce == (int)m.comps.size() && m.comps.size() >= 2 && smem <= LDS_LIMIT &&
+                m.dev.nslots >= mpd::FUSED_SPLIT_WAVES) {
+                if (m.evalSplitOcc == 0) m.evalSplitOcc = mpd::occupancy_eval_split(m.variant, smem);
+                const long ntiles = std::max<long>(1, (N + mpd::FUSED_SPLIT_TILE - 1) / mpd::FUSED_SPLIT_TILE);
+                const long grid = std::min<long>(ntiles, (long)num_cu() * m.evalSplitOcc);
+                const int* plan = m.dSplit.as<int>();
+                mpd::Launch L{dim3((unsigned)grid), dim3(mpd::FUSED_SPLIT_WAVES * 64), smem, s};
+                hip_check(mpd::launch_eval_split(m.variant, L, m.dev, m.coeffs, m.coeffStride, p.ptr, p.row_stride,
+                                                 p.col_stride, N, out, ors, ocs, ld, lds, ldAcc, nevals,
+                                                 plan + mpd::FUSED_SPLIT_WAVES + 1, plan),
+                          "launch evaluate (components split over wavefronts)");
+                return;
+            }
+            G = 1;
+        }
         if (G > 1) {
             const size_t tabBytes = (size_t)m.dev.nslots * sizeof(double);
             int block = 256;
@@ -750,7 +779,7 @@ void generic_eval(mpart_map_s& m, const mpart_cview& p, double* out, long ors, l
                   "launch evaluate");
     };
     const int kind = out ? 0 : 1;
-    static const int kGroups[4] = {1, 4, 8, 64};
+    static const int kGroups[5] = {1, 4, 8, 64, FUSED_SPLIT};
     if (m.pointGroupForced > 0) {
         run(m.pointGroupForced);
         return;
@@ -763,7 +792,7 @@ void generic_eval(mpart_map_s& m, const mpart_cview& p, double* out, long ors, l
             hip_check(hipEventCreate(&e0), "hipEventCreate");
             hip_check(hipEventCreate(&e1), "hipEventCreate");
             int best = 0;
-            for (int i = 0; i < 4; ++i) {
+            for (int i = 0; i < 5; ++i) {
                 run(kGroups[i]);
                 hip_check(hipEventRecord(e0, s), "hipEventRecord");
                 run(kGroups[i]);
@@ -1276,12 +1305,13 @@ void do_fused(mpart_map_s& m, const mpart_cview& p, const double* sens, long srs
     // when the tile's table does not fit or the map has one component.
     auto run_split = [&](int minb) -> bool {
         const size_t smem = (size_t)m.dev.nslots * mpd::FUSED_SPLIT_TILE * sizeof(double);
-        if (smem > LDS_LIMIT || m.comps.size() < 2) return false;
+        // (the kernel's static LDS is the four wavefronts' loss sums)
+        if (smem + mpd::FUSED_SPLIT_WAVES * sizeof(double) > LDS_LIMIT || m.comps.size() < 2) return false;
         if (minb != 2) minb = 1;
         int& occ = m.splitOcc[minb - 1];
         if (occ == 0) occ = mpd::occupancy_fused_split(m.variant, minb, smem);
         const long ntiles = std::max<long>(1, (N + mpd::FUSED_SPLIT_TILE - 1) / mpd::FUSED_SPLIT_TILE);
-        const long grid = std::min<long>(ntiles, (long)g_numCU * occ);
+        const long grid = std::min<long>(ntiles, (long)num_cu() * occ);
         m.partialGrad.ensure((size_t)(grid + mpd::COLSUM_MAX_CHUNKS) * (K + 1) * sizeof(double));
         double* rowsp = m.partialGrad.as<double>();
         const int* plan = m.dSplit.as<int>();
@@ -2519,11 +2549,11 @@ int mpart_eval_schedule(mpart_map m, int32_t* lanes, double* trialMs) {
         for (int k = 0; k < 2; ++k) {
             if (lanes) lanes[k] = m->evalGroup[k];
             if (trialMs)
-                for (int i = 0; i < 4; ++i) trialMs[k * 4 + i] = m->evalTrialMs[k][i];
+                for (int i = 0; i < 5; ++i) trialMs[k * 5 + i] = m->evalTrialMs[k][i];
         }
         if (lanes) lanes[2] = m->fusedGroup;
         if (trialMs)
-            for (int i = 0; i < 5; ++i) trialMs[8 + i] = m->fusedGroupTrialMs[i];
+            for (int i = 0; i < 5; ++i) trialMs[10 + i] = m->fusedGroupTrialMs[i];
     });
 }
 

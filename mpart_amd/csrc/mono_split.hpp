@@ -101,3 +101,79 @@ __global__ void __launch_bounds__(256, MINB) k_fused_split(MapDev M, const doubl
 }
 
 }  // namespace mpd
+
+namespace mpd {
+
+// Evaluate and/or LogDeterminant of every component with the same split: a
+// block of four wavefronts per tile of 64 points, one shared basis table,
+// each wavefront a lane per point over its own components.  T goes straight
+// to the output row of the component; the log-determinant terms of the four
+// wavefronts are added in wavefront order at the end of the tile, so that sum
+// runs in another order than the component order of the lane-per-point
+// kernels (same value up to rounding).  The sums pass through the table's
+// first rows once the tile's components are done, so the kernel needs no LDS
+// beyond the table.
+template <int BK, int PK, int QK, int PM>
+__global__ void __launch_bounds__(256) k_eval_split(MapDev M, const double* w, long ws, PtsArg P, OutArg out,
+                                                    double* ld, long lds, int ldAccumulate, int* nevals,
+                                                    const int* compOrder, const int* waveStart) {
+    extern __shared__ double tab[];  // nslots >= SPLIT_WAVES (the host checks)
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int ndims = M.D - 1;
+    const int c0 = waveStart[wave], c1 = waveStart[wave + 1];
+    const long ntiles = (P.N + SPLIT_TILE - 1) / SPLIT_TILE;
+    for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const long n0 = tile * SPLIT_TILE + lane;
+        const bool active = n0 < P.N;
+        const long n = active ? n0 : P.N - 1;
+        const double* pt = P.p + n * P.cs;
+        __syncthreads();  // the previous tile's table and log-det terms are no longer read
+        if (wave == 0) tab[lane] = 1.0;
+        for (int d = wave; d < ndims; d += SPLIT_WAVES) {
+            const int deg = M.dimDeg[d];
+            if (deg > 0) values_1d<BK>(M, deg, pt[d * P.rs], tab + (long)(M.dimSlot[d] - 1) * SPLIT_TILE + lane, SPLIT_TILE);
+        }
+        __syncthreads();
+        const double* mytab = tab + lane;
+        double ldsum = 0.0;
+        for (int i = c0; i < c1; ++i) {
+            const int c = compOrder[i];
+            const CompDev cd = M.comps[c];
+            double C[PM + 1];
+            coeff_groups<PM>(M, cd, w + (long)cd.coeffOff * ws, ws, mytab, SPLIT_TILE, C);
+            const double xk = pt[(long)(cd.dim - 1) * P.rs];
+            DiagCoef<PM> dc;
+            if (out.p) {
+                make_diag<PM>(M, cd, C, xk, 0.0, dc);
+                AdjAcc<PM> A;
+                int ne = 0;
+                const double I = integrate<BK, PK, QK, FL_VAL, PM>(M.quad, dc, A, nevals ? &ne : nullptr);
+                if (active) {
+                    out.p[(long)cd.outRow * out.rs + n * out.cs] = I + f_at_zero<PM>(M, cd, C);
+                    if (nevals) nevals[(long)c * P.N + n] = ne;
+                }
+            }
+            if (ld) {
+                make_diag<PM>(M, cd, C, xk, 0.0, dc);
+                const double z = diag_eval<BK, false, PM>(dc, xk, nullptr);
+                const double g = posfunc<PK>(z).g;
+                ldsum += (g <= 0) ? -__builtin_inf() : log_pos(g);
+            }
+        }
+        if (ld) {
+            // the table is no longer read: its first rows take the wavefronts' sums
+            __syncthreads();
+            tab[wave * SPLIT_TILE + lane] = ldsum;
+            __syncthreads();
+            if (wave == 0 && active) {
+                double s = tab[lane];
+                for (int v = 1; v < SPLIT_WAVES; ++v) s += tab[v * SPLIT_TILE + lane];
+                if (ldAccumulate) ld[n * lds] += s;
+                else ld[n * lds] = s;
+            }
+        }
+    }
+}
+
+}  // namespace mpd
