@@ -191,8 +191,7 @@ int mpart_kl_value_and_grad(mpart_map map, mpart_cview pts, double* loss, double
  * mpart_kl_value_and_grad's in the last bits, by summation order).  Maps
  * whose work does not fit are refused with MPART_INVALID_ARGUMENT (more than
  * about 2 million coefficients, far beyond the specialization limit), maps
- * whose basis table needs a global buffer and contDeriv = false with
- * MPART_NOT_IMPLEMENTED.  Every call on a GPU shares that GPU's scratch:
+ * whose basis table needs a global buffer with MPART_NOT_IMPLEMENTED.  Every call on a GPU shares that GPU's scratch:
  * calls that overlap in time on one GPU (other streams, other threads) race
  * and give wrong results; serialize them. */
 #define MPART_NOALLOC_SCRATCH_BYTES ((size_t)16 << 20)

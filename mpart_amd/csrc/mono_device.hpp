@@ -114,6 +114,7 @@ struct MapDev {
     double hfS[MAXORD + 1];    // (2^i i!)^(-1/2)
     double hfPi;               // pi^(-1/4)
     double hfSqrt2;            // sqrt(2)
+    int discrete;              // contDeriv = false: log-determinants use the discrete derivative
     int lin;                   // LinearizedBasis bounds in use
     double lb, ub;             // its bounds (-inf / +inf when unused)
     QuadDev quad;
@@ -1121,7 +1122,8 @@ MPD_HD void zero_adj(AdjAcc<PM>& A) {
 // entries are G_q dC_q/dx_i, as for the input VJP below).
 template <int BK, int PK, int QK, int PM, bool IN = false>
 MPD_HD void comp_core(const MapDev& M, const CompDev& cd, const double* C, double xk, bool active, int klMode,
-                      double s_in, double l_in, double* G, double& loss, double* gk = nullptr) {
+                      double s_in, double l_in, double* G, double& loss, double* gk = nullptr,
+                      double* Tout = nullptr) {
     constexpr int FLV = IN ? FL_INPUT : FL_COEF;
 #pragma unroll
     for (int q = 0; q <= PM; ++q) G[q] = 0.0;
@@ -1150,6 +1152,7 @@ MPD_HD void comp_core(const MapDev& M, const CompDev& cd, const double* C, doubl
         AdjAcc<PM> Av;
         T = integrate<BK, PK, QK, FL_VAL, PM>(M.quad, dv, Av, nullptr, &v1) + f0;
     }
+    if (Tout) *Tout = T;
     const double ldv = (gv.g <= 0) ? -__builtin_inf() : log_pos(gv.g);
     const double r = gv.gp * (1.0 / gv.g);
     double s = s_in, l = l_in;
@@ -1168,6 +1171,43 @@ MPD_HD void comp_core(const MapDev& M, const CompDev& cd, const double* C, doubl
     }
     if constexpr (IN) *gk = s * A.U + (l * r) * z2;
 }
+// The same with the discrete derivative (contDeriv = false): the
+// log-determinant term is log D with D the derivative of the quadrature
+// approximation of T in x_c (DiscreteDerivative, MonotoneComponent.h:586-638:
+// the Diagonal integrand, nugget included), and its coefficient gradient
+// (DiscreteMixedJacobian, 928-980: the MixedCoeff integrand) is
+// rho_q S''_q / D, both from one quadrature of the FL_DCOEF integrand; T and
+// its adjoint come from comp_core's own quadrature.  G_q = s (phi~_q(0) +
+// rho_q S'_q) + l rho_q S''_q / D; in KL mode s = T_c, l = -1.
+template <int BK, int PK, int QK, int PM>
+MPD_HD void comp_core_discrete(const MapDev& M, const CompDev& cd, const double* C, double xk, bool active,
+                               int klMode, double s_in, double l_in, double* G, double& loss) {
+    double lossT = 0.0, T = 0.0;
+    // T and phi~_q(0) + rho_q S'_q: comp_core with s = 1 and no log-det term
+    comp_core<BK, PK, QK, PM>(M, cd, C, xk, active, 0, 1.0, 0.0, G, lossT, nullptr, &T);
+    if (!active) return;
+    DiagCoef<PM> dc;
+    make_diag<PM>(M, cd, C, xk, M.nugget, dc);
+    AdjAcc<PM> A;
+    zero_adj<PM>(A);
+    integrate<BK, PK, QK, FL_DCOEF, PM>(M.quad, dc, A, nullptr);
+    const double D = A.U;
+    double s = s_in, l = l_in;
+    if (klMode) {
+        s = T;
+        l = -1.0;
+        loss += 0.5 * T * T - ((D <= 0) ? -__builtin_inf() : log_pos(D));
+    }
+#pragma unroll
+    for (int q = 0; q <= PM; ++q) G[q] *= s;
+    const double li = l / D;
+#pragma unroll
+    for (int q = 1; q <= PM; ++q) {
+        if (q > cd.p) break;
+        G[q] += li * (M.rho[q] * A.S[q]);
+    }
+}
+
 // Per-component factors of an input vector-Jacobian product at one point,
 //   d/dx [s T_c + l log g(d_c f)] = sum_q G_q dC_q/dx_i  (i < c)  and  gk  (i = c),
 // with G_q = s (phi~_q(0) + rho_q S_q) + l (g'/g) rho_q B'_q(x_c) and

@@ -389,6 +389,7 @@ void build_plan(mpart_map_s& m) {
         M.norm[k] = (bk == mpd::BK_PHYS)
                         ? std::sqrt(std::sqrt(M_PI) * std::pow(2.0, static_cast<double>(k)) * factorial_u32(k))
                         : std::sqrt(std::sqrt(2.0 * M_PI) * factorial_u32(k));
+    M.discrete = o.contDeriv ? 0 : 1;
     M.lin = lin ? 1 : 0;
     M.lb = lin ? o.basisLB : -std::numeric_limits<double>::infinity();
     M.ub = lin ? o.basisUB : std::numeric_limits<double>::infinity();
@@ -1144,7 +1145,8 @@ void do_fused(mpart_map_s& m, const mpart_cview& p, const double* sens, long srs
         hip_check(hipGetDevice(&dev), "hipGetDevice");
         hip_check(hipDeviceGetAttribute(&g_numCU, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
     }
-    if (mpg::GenModule* g = gen_for(m)) {
+    // the specialized kernels implement the continuous derivative only
+    if (mpg::GenModule* g = m.dev.discrete ? nullptr : gen_for(m)) {
         int gblock = 256;
         size_t gsmem = 0;
         gen_fused_shape(m, g, gblock, gsmem);
@@ -1327,6 +1329,8 @@ void do_fused(mpart_map_s& m, const mpart_cview& p, const double* sens, long srs
         return true;
     };
     auto run = [&](int G) {
+        // the discrete derivative: the lane-per-point and split kernels only
+        if (m.dev.discrete && G != FUSED_SPLIT) G = 1;
         if (G == FUSED_SPLIT) {
             if (run_split(m.splitMinB)) return;
             G = 1;
@@ -1402,7 +1406,8 @@ void do_fused_noalloc(mpart_map_s& m, const mpart_cview& p, double* loss, double
     const std::string tooBig = "KL without allocation: the map's " + std::to_string(K) +
                                " coefficients need more than the " + std::to_string(MPART_NOALLOC_SCRATCH_BYTES) +
                                "-byte scratch";
-    mpg::GenModule* g = (m.genEnabled && codegen_env_enabled() && m.genTried) ? m.gen : nullptr;
+    // (the specialized kernels implement the continuous derivative only)
+    mpg::GenModule* g = (m.genEnabled && codegen_env_enabled() && m.genTried && !m.dev.discrete) ? m.gen : nullptr;
     if (g) {
         int gblock = 256;
         size_t gsmem = 0;
@@ -1902,8 +1907,7 @@ struct KLObjectiveEval {
         double* d = dev.as<double>();
         hip_check(hipMemcpyAsync(d, w.data(), K * sizeof(double), hipMemcpyHostToDevice, s), "hipMemcpyAsync");
         hip_check(mpd::launch_copy(d, 1, K, 0, 1, m.coeffs, 0, m.coeffStride, s), "launch copy");
-        if (m.opts.contDeriv) do_fused(m, pts, nullptr, 0, 0, nullptr, 0, 1, d + K, d + K + 1, 1.0 / (double)pts.cols, s);
-        else do_kl_plain(m, pts, d + K, d + K + 1, s);
+        do_fused(m, pts, nullptr, 0, 0, nullptr, 0, 1, d + K, d + K + 1, 1.0 / (double)pts.cols, s);
         hip_check(hipMemcpyAsync(host.data(), d + K, (K + 1) * sizeof(double), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
         hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
         ++evals;
@@ -2390,12 +2394,6 @@ int mpart_kl_value_and_grad(mpart_map m, mpart_cview pts, double* loss, double* 
         m->note_capture(S(s));
         check_view("KL: pts", pts.rows, pts.cols, m->inDim, pts.cols);
         if (pts.cols == 0) throw ArgError("KL: no points");
-        if (!m->opts.contDeriv) {
-            // The fused pass implements the continuous derivative; the
-            // discrete-derivative objective runs the per-method kernels.
-            do_kl_plain(*m, pts, loss, grad, S(s));
-            return;
-        }
         do_fused(*m, pts, nullptr, 0, 0, nullptr, 0, 1, loss, grad, 1.0 / (double)pts.cols, S(s));
     });
 }
@@ -2406,9 +2404,6 @@ int mpart_kl_value_and_grad_noalloc(mpart_map m, mpart_cview pts, double* loss, 
         check_view("KL: pts", pts.rows, pts.cols, m->inDim, pts.cols);
         if (pts.cols == 0) throw ArgError("KL: no points");
         if (!loss || !grad) throw ArgError("KL: null output");
-        if (!m->opts.contDeriv)
-            throw NotImpl("KL without allocation: contDeriv = false runs the per-method kernels, which need work "
-                          "buffers");
         double* scr = noalloc_scratch();
         do_fused_noalloc(*m, pts, loss, grad, 1.0 / (double)pts.cols, S(s), scr,
                          (long)(MPART_NOALLOC_SCRATCH_BYTES / sizeof(double)));
@@ -2422,7 +2417,6 @@ int mpart_coeff_vjp(mpart_map m, mpart_cview pts, mpart_cview sens, const double
         m->note_capture(S(s));
         check_view("vjp: pts", pts.rows, pts.cols, m->inDim, pts.cols);
         if (sens.ptr) check_view("vjp: sens", sens.rows, sens.cols, m->outDim, pts.cols);
-        if (!m->opts.contDeriv) throw NotImpl("contDeriv = false is not provided yet");
         do_fused(*m, pts, sens.ptr, sens.row_stride, sens.col_stride, lsens, lss, 0, nullptr, grad, 1.0, S(s));
     });
 }
@@ -2435,8 +2429,7 @@ int mpart_kl_value_and_grads(mpart_map m, mpart_cview pts, double* loss, double*
         check_view("KL: pts", pts.rows, pts.cols, m->inDim, pts.cols);
         if (pts.cols == 0) throw ArgError("KL: no points");
         if (!gradInputs.ptr) {
-            if (!m->opts.contDeriv) do_kl_plain(*m, pts, loss, grad, S(s));
-            else do_fused(*m, pts, nullptr, 0, 0, nullptr, 0, 1, loss, grad, 1.0 / (double)pts.cols, S(s));
+            do_fused(*m, pts, nullptr, 0, 0, nullptr, 0, 1, loss, grad, 1.0 / (double)pts.cols, S(s));
             return;
         }
         if (!m->opts.contDeriv) throw ArgError("Discrete derivative version is not implemented yet (To Do)");
@@ -2452,7 +2445,8 @@ int mpart_vjp(mpart_map m, mpart_cview pts, mpart_cview sens, const double* lsen
         m->note_capture(S(s));
         check_view("vjp: pts", pts.rows, pts.cols, m->inDim, pts.cols);
         if (sens.ptr) check_view("vjp: sens", sens.rows, sens.cols, m->outDim, pts.cols);
-        if (!m->opts.contDeriv) throw NotImpl("contDeriv = false is not provided yet");
+        if (!m->opts.contDeriv && gradInputs.ptr)
+            throw ArgError("Discrete derivative version is not implemented yet (To Do)");
         do_fused(*m, pts, sens.ptr, sens.row_stride, sens.col_stride, lsens, lss, 0, nullptr, grad, 1.0, S(s),
                  gradInputs.ptr ? &gradInputs : nullptr, 1.0);
     });

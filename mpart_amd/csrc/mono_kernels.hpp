@@ -750,7 +750,8 @@ __global__ void __launch_bounds__(256) k_fused(MapDev M, const double* w, long w
             const double sv = (!klMode && sens) ? sens[(long)cd.outRow * srs + n * scs] : 0.0;
             const double lv = (!klMode && lsens) ? lsens[n * lss] : 0.0;
             double G[PM + 1];
-            comp_core<BK, PK, QK, PM>(M, cd, C, xk, active, klMode, sv, lv, G, lossLane);
+            if (M.discrete) comp_core_discrete<BK, PK, QK, PM>(M, cd, C, xk, active, klMode, sv, lv, G, lossLane);
+            else comp_core<BK, PK, QK, PM>(M, cd, C, xk, active, klMode, sv, lv, G, lossLane);
             // Gradient pass: chunks of 16 term records per group.
 #pragma unroll
             for (int q = 0; q <= PM; ++q) {

@@ -64,7 +64,8 @@ __global__ void __launch_bounds__(256, MINB) k_fused_split(MapDev M, const doubl
             const double sv = (!klMode && sens) ? sens[(long)cd.outRow * srs + n * scs] : 0.0;
             const double lv = (!klMode && lsens) ? lsens[n * lss] : 0.0;
             double G[PM + 1];
-            comp_core<BK, PK, QK, PM>(M, cd, C, xk, active, klMode, sv, lv, G, lossLane);
+            if (M.discrete) comp_core_discrete<BK, PK, QK, PM>(M, cd, C, xk, active, klMode, sv, lv, G, lossLane);
+            else comp_core<BK, PK, QK, PM>(M, cd, C, xk, active, klMode, sv, lv, G, lossLane);
             // Gradient pass: chunks of 16 term records per group, each chunk
             // reduced over the tile's points by the wavefront.
 #pragma unroll

@@ -305,9 +305,26 @@ def test_discrete_derivative_mode(ci):
     g.SetPlainLaunches(True)
     close(g.LogDeterminant(x), r.LogDeterminant(x), what="LogDeterminant (discrete, per component)")
     g.SetPlainLaunches(False)
+    # the KL objective in one fused launch (the generic kernel with the
+    # discrete derivative), with each schedule it can take, and the plain path
     lr, gr = r.KL(x)
-    lf, gf = g.KLValueAndGrad(x)
-    close(lf, lr, what="KL loss (discrete)")
-    close(gf, gr, what="KL grad (discrete)")
+    for lanes in (1, 2):
+        g.SetPointGroup(lanes)
+        lf, gf = g.KLValueAndGrad(x)
+        close(lf, lr, what=f"KL loss (discrete, fused, schedule {lanes})")
+        close(gf, gr, what=f"KL grad (discrete, fused, schedule {lanes})")
+    g.SetPointGroup(0)
+    lp, gp = g.KLValueAndGrad(x, plain=True)
+    close(lp, lr, what="KL loss (discrete, plain)")
+    close(gp, gr, what="KL grad (discrete, plain)")
+    # the coefficient VJP of the torch backward: sens . dT/dw + lsens . dlogdet/dw
+    torch = pytest.importorskip("torch")
+    s = Stream(990 + ci)
+    sens = s.normal(3 * x.shape[1]).reshape(3, x.shape[1])
+    ls = s.normal(x.shape[1])
+    xd = torch.from_numpy(np.ascontiguousarray(x)).cuda()
+    v = g.CoeffVJP(xd, torch.from_numpy(sens).cuda(), torch.from_numpy(ls).cuda()).cpu().numpy()
+    ref = r.CoeffGrad(np.ascontiguousarray(x), sens).sum(axis=1) + r.LogDeterminantCoeffGrad(np.ascontiguousarray(x)) @ ls
+    close(v, ref, what="coefficient VJP (discrete)")
     with pytest.raises(ValueError):
         g.LogDeterminantInputGrad(x)

@@ -120,12 +120,12 @@ def test_noalloc_allocates_nothing():
     assert torch.isfinite(grad).all()
 
 
-def test_noalloc_refuses_discrete_derivative():
-    _case("noalloc_discrete")
-    m = mt.CreateTriangular(2, 2, 2, mt.MapOptions(contDeriv=False))
-    x = torch.zeros((2, 10), dtype=torch.float64, device="cuda")
-    loss = torch.zeros(1, dtype=torch.float64, device="cuda")
-    grad = torch.zeros(m.numCoeffs, dtype=torch.float64, device="cuda")
-    rc = mt._lib.mpart_kl_value_and_grad_noalloc(m._h, mt._view(x), ctypes.c_void_p(loss.data_ptr()),
-                                                 ctypes.c_void_p(grad.data_ptr()), mt._stream())
-    assert rc == -4, rc
+def test_noalloc_discrete_derivative():
+    # contDeriv = false: the generic fused kernel with the discrete derivative
+    _case("noalloc_discrete", N1=20_000, N2=3_000)
+    m = mt.CreateTriangular(3, 3, 3, mt.MapOptions(contDeriv=False))
+    s = Stream(56)
+    m.SetCoeffs(0.1 * s.normal(m.numCoeffs))
+    x1 = torch.from_numpy(s.points(3, 20_000)).cuda()
+    x2 = torch.from_numpy(s.points(3, 3_000)).cuda()
+    _two_calls(m, x1, x2, "3-D map, discrete derivative")
