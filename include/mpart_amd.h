@@ -20,6 +20,18 @@
  * (MParT/Utilities/ArrayConversions.h:102-130); the fields are 64-bit here.
  * Calls are asynchronous on `stream` (a hipStream_t; NULL is the null stream).
  *
+ * Host memory.  The reference's ConditionalMapBase methods take HostSpace
+ * views as well as device views (src/ConditionalMapBase.cpp:7-81).  Here the
+ * matrix and vector arguments of mpart_evaluate, mpart_log_determinant,
+ * mpart_gradient, mpart_coeff_grad, mpart_log_determinant_coeff_grad,
+ * mpart_log_determinant_input_grad, mpart_inverse and mpart_kl_value_and_grad
+ * may each lie in host memory (pageable or pinned; any strides): such an
+ * input is packed and copied to a temporary device buffer, such an output is
+ * computed into one and copied back, and the call then returns only once the
+ * stream has finished it (a synchronous call).  Results equal the device-path
+ * results bitwise.  Host memory is refused while the stream is being captured
+ * in a graph.  Every other entry point takes device memory only.
+ *
  * Graph capture.  A call on a map may be captured in a HIP graph (run it once
  * eagerly first, so its kernels are compiled, tuned and its work buffers
  * sized).  The captured launches hold the addresses of the map's work
@@ -91,6 +103,12 @@ typedef struct mpart_view {
 typedef void* mpart_stream; /* hipStream_t */
 
 const char* mpart_last_error(void);
+
+/* 1 when the library stages `p` through a device buffer (host memory: pageable,
+ * pinned or unknown to HIP), 0 when kernels read it in place (device or managed
+ * memory, or NULL).  No counterpart in the reference, whose Kokkos views carry
+ * their memory space in their type. */
+int mpart_is_host_memory(const void* p);
 
 /* Replaces mpart::Initialize (MParT/Initialization.h:78-116): idempotent. */
 int mpart_init(void);

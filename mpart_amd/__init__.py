@@ -105,6 +105,7 @@ _SIGS = {
     "mpart_kl_value_and_grad": [_vp, _CView, _vp, _vp, _vp],
     "mpart_kl_value_and_grad_noalloc": [_vp, _CView, _vp, _vp, _vp],
     "mpart_codegen_kernel_info": [_vp, ctypes.c_char_p, _vp, _vp, _vp],
+    "mpart_is_host_memory": [_vp],
     "mpart_kl_value_and_grad_plain": [_vp, _CView, _vp, _vp, _vp],
     "mpart_coeff_vjp": [_vp, _CView, _CView, _vp, _i64, _vp, _vp],
     "mpart_kl_input_grad": [_vp, _CView, _dbl, _CView, _vp],

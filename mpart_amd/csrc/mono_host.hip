@@ -8,6 +8,7 @@
 #include <cstdlib>
 #include <cstdint>
 #include <cstring>
+#include <functional>
 #include <limits>
 #include <memory>
 #include <mutex>
@@ -201,6 +202,107 @@ int num_cu() {
     }
     return g_numCU;
 }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host memory at the boundary.  The reference's ConditionalMapBase methods take
+// host (Kokkos::HostSpace) views as well as device ones and copy what they need
+// (src/ConditionalMapBase.cpp:7-81, src/ParameterizedFunctionBase.cpp:24-33).
+// Here a pointer HIP does not know as device memory (pageable or pinned host
+// memory) is staged: inputs are packed and copied to the device before the
+// call, outputs are computed into device buffers and copied back, after which
+// the call has completed (it synchronizes the stream).  Device pointers pass
+// through untouched, so device calls stay asynchronous.
+namespace {
+
+// Only memory the kernels can read in place stays as it is: device and
+// managed allocations.  Everything else is staged: pageable memory (reported
+// as an error, or as hipMemoryTypeUnregistered, depending on the runtime),
+// pinned host memory and any type this code does not know.
+bool host_memory(const void* p) {
+    if (!p) return false;
+    hipPointerAttribute_t a{};
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();  // pageable memory HIP never saw: clear the error it recorded
+        return true;
+    }
+    return !(a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeArray || a.type == hipMemoryTypeManaged);
+}
+
+class Staging {
+   public:
+    explicit Staging(hipStream_t s) : s_(s) {}
+    mpart_cview in(const mpart_cview& v) {
+        if (v.rows <= 0 || v.cols <= 0 || !host_memory(v.ptr)) return v;  // the callee checks the shape
+        std::vector<double> h((size_t)(v.rows * v.cols));
+        for (int64_t i = 0; i < v.rows; ++i)
+            for (int64_t j = 0; j < v.cols; ++j) h[(size_t)(i * v.cols + j)] = v.ptr[i * v.row_stride + j * v.col_stride];
+        double* d = upload_vec(h);
+        return {d, v.rows, v.cols, v.cols, 1};
+    }
+    const double* in_vec(const double* p, int64_t n, int64_t stride) {
+        if (n <= 0 || !host_memory(p)) return p;
+        std::vector<double> h((size_t)n);
+        for (int64_t i = 0; i < n; ++i) h[(size_t)i] = p[i * stride];
+        return upload_vec(h);
+    }
+    mpart_view out(const mpart_view& v) {
+        if (v.rows <= 0 || v.cols <= 0 || !host_memory(v.ptr)) return v;
+        double* d = device((size_t)(v.rows * v.cols));
+        const mpart_view hv = v;
+        back_.push_back([this, hv, d] {
+            std::vector<double> h((size_t)(hv.rows * hv.cols));
+            hip_check(hipMemcpy(h.data(), d, h.size() * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy");
+            for (int64_t i = 0; i < hv.rows; ++i)
+                for (int64_t j = 0; j < hv.cols; ++j) hv.ptr[i * hv.row_stride + j * hv.col_stride] = h[(size_t)(i * hv.cols + j)];
+        });
+        return {d, v.rows, v.cols, v.cols, 1};
+    }
+    // a vector output (n entries with a stride); `read` copies its current
+    // values in first (for outputs the call accumulates into)
+    double* out_vec(double* p, int64_t n, int64_t stride, bool read = false) {
+        if (n <= 0 || !host_memory(p)) return p;
+        double* d = device((size_t)n);
+        if (read) hip_check(hipMemcpy(d, in_vec(p, n, stride), n * sizeof(double), hipMemcpyDeviceToDevice), "hipMemcpy");
+        back_.push_back([p, n, stride, d] {
+            std::vector<double> h((size_t)n);
+            hip_check(hipMemcpy(h.data(), d, h.size() * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy");
+            for (int64_t i = 0; i < n; ++i) p[i * stride] = h[(size_t)i];
+        });
+        return d;
+    }
+    bool staged() const { return !bufs_.empty(); }
+    // after the call: wait for it and copy host outputs back
+    void finish() {
+        if (bufs_.empty()) return;
+        hip_check(hipStreamSynchronize(s_), "hipStreamSynchronize");
+        for (auto& f : back_) f();
+    }
+
+   private:
+    double* device(size_t n) {
+        if (bufs_.empty()) {
+            // staging copies synchronously and frees its buffers when the call
+            // returns, which a graph being captured on the stream cannot take
+            hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+            hip_check(hipStreamIsCapturing(s_, &cs), "hipStreamIsCapturing");
+            if (cs != hipStreamCaptureStatusNone)
+                throw ArgError("host memory cannot be passed while the stream is being captured; pass device memory");
+        }
+        bufs_.push_back(std::make_unique<DBuf>());
+        bufs_.back()->ensure(std::max<size_t>(8, n * sizeof(double)));
+        return bufs_.back()->as<double>();
+    }
+    double* upload_vec(const std::vector<double>& h) {
+        double* d = device(h.size());
+        hip_check(hipMemcpy(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
+        return d;
+    }
+    hipStream_t s_;
+    std::vector<std::unique_ptr<DBuf>> bufs_;
+    std::vector<std::function<void()>> back_;
+};
 
 }  // namespace
 
@@ -2161,6 +2263,8 @@ extern "C" {
 
 const char* mpart_last_error(void) { return g_err.c_str(); }
 
+int mpart_is_host_memory(const void* p) { return host_memory(p) ? 1 : 0; }
+
 int mpart_init(void) {
     return guarded([&] {
         static std::once_flag once;
@@ -2327,7 +2431,9 @@ int mpart_evaluate(mpart_map m, mpart_cview pts, mpart_view out, mpart_stream s)
     return guarded([&] {
         check_map(m);
         m->note_capture(S(s));
-        do_evaluate(*m, pts, out, nullptr, S(s));
+        Staging st(S(s));
+        do_evaluate(*m, st.in(pts), st.out(out), nullptr, S(s));
+        st.finish();
     });
 }
 
@@ -2343,7 +2449,11 @@ int mpart_log_determinant(mpart_map m, mpart_cview pts, double* out, int64_t str
     return guarded([&] {
         check_map(m);
         m->note_capture(S(s));
-        do_logdet(*m, pts, out, stride, S(s));
+        Staging st(S(s));
+        const mpart_cview p = st.in(pts);
+        double* o = st.out_vec(out, pts.cols, stride);
+        do_logdet(*m, p, o, o != out ? 1 : stride, S(s));
+        st.finish();
     });
 }
 
@@ -2351,7 +2461,10 @@ int mpart_gradient(mpart_map m, mpart_cview pts, mpart_cview sens, mpart_view ou
     return guarded([&] {
         check_map(m);
         m->note_capture(S(s));
-        do_input_grad(*m, pts, &sens, out, 0, S(s));
+        Staging st(S(s));
+        const mpart_cview sv = st.in(sens);
+        do_input_grad(*m, st.in(pts), &sv, st.out(out), 0, S(s));
+        st.finish();
     });
 }
 
@@ -2359,7 +2472,9 @@ int mpart_coeff_grad(mpart_map m, mpart_cview pts, mpart_cview sens, mpart_view 
     return guarded([&] {
         check_map(m);
         m->note_capture(S(s));
-        do_coeff_grad(*m, pts, sens, out, S(s));
+        Staging st(S(s));
+        do_coeff_grad(*m, st.in(pts), st.in(sens), st.out(out), S(s));
+        st.finish();
     });
 }
 
@@ -2367,7 +2482,9 @@ int mpart_log_determinant_coeff_grad(mpart_map m, mpart_cview pts, mpart_view ou
     return guarded([&] {
         check_map(m);
         m->note_capture(S(s));
-        do_logdet_coeff_grad(*m, pts, out, S(s));
+        Staging st(S(s));
+        do_logdet_coeff_grad(*m, st.in(pts), st.out(out), S(s));
+        st.finish();
     });
 }
 
@@ -2375,7 +2492,9 @@ int mpart_log_determinant_input_grad(mpart_map m, mpart_cview pts, mpart_view ou
     return guarded([&] {
         check_map(m);
         m->note_capture(S(s));
-        do_input_grad(*m, pts, nullptr, out, 1, S(s));
+        Staging st(S(s));
+        do_input_grad(*m, st.in(pts), nullptr, st.out(out), 1, S(s));
+        st.finish();
     });
 }
 
@@ -2384,7 +2503,9 @@ int mpart_inverse(mpart_map m, mpart_cview x1, mpart_cview r, mpart_view out, do
     return guarded([&] {
         check_map(m);
         m->note_capture(S(s));
-        do_inverse(*m, x1, r, out, xtol, ytol, S(s));
+        Staging st(S(s));
+        do_inverse(*m, st.in(x1), st.in(r), st.out(out), xtol, ytol, S(s));
+        st.finish();
     });
 }
 
@@ -2394,7 +2515,10 @@ int mpart_kl_value_and_grad(mpart_map m, mpart_cview pts, double* loss, double* 
         m->note_capture(S(s));
         check_view("KL: pts", pts.rows, pts.cols, m->inDim, pts.cols);
         if (pts.cols == 0) throw ArgError("KL: no points");
-        do_fused(*m, pts, nullptr, 0, 0, nullptr, 0, 1, loss, grad, 1.0 / (double)pts.cols, S(s));
+        Staging st(S(s));
+        do_fused(*m, st.in(pts), nullptr, 0, 0, nullptr, 0, 1, st.out_vec(loss, 1, 1), st.out_vec(grad, m->numCoeffs, 1),
+                 1.0 / (double)pts.cols, S(s));
+        st.finish();
     });
 }
 
