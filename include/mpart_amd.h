@@ -110,6 +110,20 @@ const char* mpart_last_error(void);
  * their memory space in their type. */
 int mpart_is_host_memory(const void* p);
 
+/* The reference fails when g(df) is infinite at a quadrature node
+ * (MParT/MonotoneIntegrand.h:183-190: std::runtime_error on the host,
+ * Kokkos::abort in device code).  Here every kernel that evaluates the
+ * integrand (Evaluate, Gradient, CoeffGrad, Inverse, the KL passes and VJPs,
+ * the discrete-derivative LogDeterminant and its gradient, the batches) sets
+ * the map's status word instead, and no kernel stops.  mpart_check_integrand
+ * waits for `stream`, returns MPART_RUNTIME_ERROR with the reference's message
+ * when any call on the map met an infinite g since the last check, and clears
+ * the word.  The host-memory calls above check it themselves (they are
+ * synchronous): they fail the same way, after writing their outputs.
+ * mpart_train_map treats coefficients at which g is infinite as an infeasible
+ * trial point (objective +inf, the line search backtracks). */
+int mpart_check_integrand(mpart_map map, mpart_stream stream);
+
 /* Replaces mpart::Initialize (MParT/Initialization.h:78-116): idempotent. */
 int mpart_init(void);
 

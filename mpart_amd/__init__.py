@@ -106,6 +106,7 @@ _SIGS = {
     "mpart_kl_value_and_grad_noalloc": [_vp, _CView, _vp, _vp, _vp],
     "mpart_codegen_kernel_info": [_vp, ctypes.c_char_p, _vp, _vp, _vp],
     "mpart_is_host_memory": [_vp],
+    "mpart_check_integrand": [_vp, _vp],
     "mpart_kl_value_and_grad_plain": [_vp, _CView, _vp, _vp, _vp],
     "mpart_coeff_vjp": [_vp, _CView, _CView, _vp, _i64, _vp, _vp],
     "mpart_kl_input_grad": [_vp, _CView, _dbl, _CView, _vp],
@@ -484,6 +485,15 @@ class ConditionalMap:
     def InverseImpl(self, x1, r, out, xtol=1e-6, ytol=1e-6):
         _check(_lib.mpart_inverse(self._h, _view(x1), _view(r), _view(out), xtol, ytol, _stream()))
 
+    def CheckIntegrand(self):
+        """Raises the reference's RuntimeError (MonotoneIntegrand.h:183-190) when a
+        call on this map met an infinite g(df) at a quadrature node since the
+        last check, and clears the record; waits for the current stream.  The
+        methods below check by themselves when they return numpy arrays
+        (synchronous calls); with device tensors the check is left to the
+        caller, as the calls are asynchronous."""
+        _check(_lib.mpart_check_integrand(self._h, _stream()))
+
     # allocating methods -----------------------------------------------------
     def _run(self, fn, shape, *ins):
         torch = _torch()
@@ -493,6 +503,7 @@ class ConditionalMap:
         fn(*[t for t, _ in devs], out)
         if host:
             torch.cuda.current_stream().synchronize()
+            self.CheckIntegrand()
             return out.cpu().numpy()
         return out
 
@@ -508,6 +519,7 @@ class ConditionalMap:
         _check(_lib.mpart_evaluate_counted(self._h, _view(p), _view(out), ne.data_ptr(), _stream()))
         if host:
             torch.cuda.current_stream().synchronize()
+            self.CheckIntegrand()
             return out.cpu().numpy(), ne.cpu().numpy()
         return out, ne
 
@@ -539,6 +551,7 @@ class ConditionalMap:
         self.InverseImpl(xx, rr, out, xtol, ytol)
         if host_r or host_x:
             torch.cuda.current_stream().synchronize()
+            self.CheckIntegrand()
             return out.cpu().numpy()
         return out
 
@@ -557,6 +570,7 @@ class ConditionalMap:
         _check(fn(self._h, _view(p), loss.data_ptr(), grad.data_ptr(), _stream()))
         if host:
             torch.cuda.current_stream().synchronize()
+            self.CheckIntegrand()
             return float(loss.item()), grad.cpu().numpy()
         return loss, grad
 
@@ -591,6 +605,7 @@ class ConditionalMap:
         self.KLInputGradImpl(p, out, scale)
         if host:
             torch.cuda.current_stream().synchronize()
+            self.CheckIntegrand()
             return out.cpu().numpy()
         return out
 
@@ -610,6 +625,7 @@ class ConditionalMap:
         self.InputVJPImpl(p, s, ls, out)
         if host:
             torch.cuda.current_stream().synchronize()
+            self.CheckIntegrand()
             return out.cpu().numpy()
         return out
 
@@ -709,6 +725,11 @@ class MapBatch:
 
     def __len__(self):
         return len(self.maps)
+
+    def CheckIntegrand(self):
+        """ConditionalMap.CheckIntegrand for every map of the batch."""
+        for m in self.maps:
+            m.CheckIntegrand()
 
     @staticmethod
     def _views(ts):
@@ -833,7 +854,9 @@ class KLObjective:
 
     def _value(self, X, map):
         loss, _ = map.KLValueAndGrad(X)
-        return float(loss.item()) + 0.5 * map.outputDim * math.log(2.0 * math.pi)
+        v = float(loss.item())
+        map.CheckIntegrand()
+        return v + 0.5 * map.outputDim * math.log(2.0 * math.pi)
 
     def TrainError(self, map):
         return self._value(self.train, map)
@@ -847,7 +870,9 @@ class KLObjective:
         """(objective, coefficient gradient) on X (default: the training samples)."""
         X = self.train if X is None else _as_dev(X)[0]
         loss, grad = map.KLValueAndGrad(X)
-        return float(loss.item()) + 0.5 * map.outputDim * math.log(2.0 * math.pi), grad
+        v = float(loss.item())
+        map.CheckIntegrand()
+        return v + 0.5 * map.outputDim * math.log(2.0 * math.pi), grad
 
 
 def CreateGaussianKLObjective(train, test=None, dim=0):

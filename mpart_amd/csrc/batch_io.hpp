@@ -44,6 +44,7 @@ struct InvSeg {
     long os;
     double* o2;       // second copy of the root (the caller's output) or null
     long os2;
+    int* status;      // the map's MapDev::status
 };
 
 }  // namespace mpd

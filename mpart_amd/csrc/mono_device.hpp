@@ -118,6 +118,7 @@ struct MapDev {
     int lin;                   // LinearizedBasis bounds in use
     double lb, ub;             // its bounds (-inf / +inf when unused)
     QuadDev quad;
+    int* status;               // set to 1 when g is infinite at a quadrature node (NULL: not recorded)
 };
 
 // ---------------------------------------------------------------------------
@@ -259,6 +260,7 @@ struct DiagCoef {
     double nug; // nugget added to g inside the integrand
     int lin;        // LinearizedBasis in use (the same for every lane)
     double lb, ub;  // its bounds
+    int* status;    // MapDev::status
 };
 
 // The argument of the diagonal basis derivatives: a linearized basis has the
@@ -766,6 +768,7 @@ MPD_HD void make_diag(const MapDev& M, const CompDev& c, const double* C, double
     dc.lin = M.lin;
     dc.lb = M.lb;
     dc.ub = M.ub;
+    dc.status = M.status;
 #pragma unroll
     for (int q = 0; q <= PM; ++q) dc.cd[q] = (q <= c.p) ? C[q] * M.rho[q] : 0.0;
 }
@@ -808,18 +811,29 @@ struct NodeV<FL_DCOEF> {
     double F, gp, gpp, g1;
 };
 
+// The reference checks g at every integrand evaluation and fails when it is
+// infinite (MonotoneIntegrand.h:183-190: a runtime_error on the host, an
+// abort on a device); here the node records it in the map's status word,
+// which the host reads where a call is synchronous.
+template <int PM>
+MPD_HD void note_infinite_g(const DiagCoef<PM>& dc, double g) {
+    if (__builtin_isinf(g) && dc.status) *dc.status = 1;
+}
+
 template <int BK, int PK, int FL, int PM>
 MPD_HD NodeV<FL> eval_node(const DiagCoef<PM>& dc, double t) {
     double z = diag_eval<BK, false, PM>(dc, t * dc.x, nullptr);
     NodeV<FL> v;
     if constexpr (FL == FL_DCOEF) {
         const PosF2<PK> gv = posfunc2<PK>(z);
+        note_infinite_g<PM>(dc, gv.g);
         v.F = dc.x * (gv.g + dc.nug);
         v.gp = gv.gp;
         v.gpp = gv.gpp;
         v.g1 = gv.g + dc.nug;
     } else {
         PosF<PK> gv = posfunc<PK>(z);
+        note_infinite_g<PM>(dc, gv.g);
         v.F = dc.x * (gv.g + dc.nug);
         if constexpr (FL >= FL_COEF) v.h = dc.x * gv.gp;
         if constexpr (FL == FL_INPUT) v.g1 = gv.g + dc.nug;
@@ -915,6 +929,20 @@ __host__ __device__ __forceinline__ E qpop(const E* stk, E& top, int& sp, bool& 
     return top;
 }
 
+// A non-finite interval error means a non-finite node value: an infinite g
+// (where the reference fails, MonotoneIntegrand.h:183-190) or a NaN input.
+// The integral is then non-finite whatever follows, so the adaptive rules end
+// at once with it instead of subdividing every remaining interval down to
+// quadMaxSub (2^quadMaxSub intervals, as the reference's loop would for NaN
+// inputs), and the adjoint sums become NaN.
+template <int PM>
+MPD_HD void nan_adj(AdjAcc<PM>& A) {
+#pragma unroll
+    for (int q = 0; q <= PM; ++q) A.S[q] = __builtin_nan("");
+    A.U = __builtin_nan("");
+}
+MPD_HD inline bool nonfinite_err(double err) { return !(err <= __DBL_MAX__); }
+
 // AdaptiveSimpson::Integrate (Quadrature.h:553-676): depth-first, left first,
 // accepted intervals add intFine in left-to-right order.
 template <int BK, int PK, int FL, int PM>
@@ -946,6 +974,11 @@ __host__ __device__ double integrate_as(const QuadDev& Q, const DiagCoef<PM>& dc
         double fine = wl * (fa.F + 4.0 * flm.F + fm.F);
         fine += wr * (fm.F + 4.0 * frm.F + fb.F);
         const double err = fabs(fine - coarse);
+        if (nonfinite_err(err)) {
+            res += fine;
+            if constexpr (FL >= FL_COEF) nan_adj<PM>(A);
+            break;
+        }
         const double tol = fmax(Q.relTol * fabs(coarse), Q.absTol);
         if (((err < tol) || (level == Q.maxSub - 1)) && (level >= Q.minSub)) {
             res += fine;
@@ -1038,6 +1071,11 @@ __host__ __device__ double integrate_acc(const QuadDev& Q, const DiagCoef<PM>& d
             inner[i] = v;
         }
         const double err = fabs(fine - coarse);
+        if (nonfinite_err(err)) {  // as in integrate_as
+            res += fine;
+            if constexpr (FL >= FL_COEF) nan_adj<PM>(A);
+            break;
+        }
         const double tol = fmax(Q.relTol * fabs(coarse), Q.absTol);
         if (((err < tol) || (level == Q.maxSub - 1)) && (level >= Q.minSub)) {
             res += fine;
@@ -1136,6 +1174,7 @@ MPD_HD void comp_core(const MapDev& M, const CompDev& cd, const double* C, doubl
     double z2 = 0.0;
     const double z = diag_eval<BK, IN, PM>(dc, 1.0 * xk, IN ? &z2 : nullptr);
     const PosF<PK> gv = posfunc<PK>(z);
+    if (QK != QK_CC || M.quad.nf > 1) note_infinite_g<PM>(dc, gv.g);  // t = 1 is a node of the rule
     NodeV<FLV> n1;
     n1.F = dc.x * (gv.g + dc.nug);
     n1.h = dc.x * gv.gp;
@@ -1233,6 +1272,7 @@ MPD_HD void ivjp_core(const MapDev& M, const CompDev& cd, const double* C, doubl
     double s = 0.0, l = hasL ? l_in : 0.0;
     if (klMode || hasS) {
         const double f0 = f_at_zero<PM>(M, cd, C);
+        if (QK != QK_CC || M.quad.nf > 1) note_infinite_g<PM>(dc, gv.g);  // t = 1 is a node of the rule
         NodeV<FL_INPUT> n1;
         n1.F = dc.x * (gv.g + dc.nug);
         n1.h = dc.x * gv.gp;

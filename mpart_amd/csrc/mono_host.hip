@@ -348,6 +348,7 @@ struct mpart_map_s {
     // device plan
     MapDev dev{};
     DBuf dComps, dGroups, dTerms, dSlotDim, dDimSlot, dDimDeg;
+    DBuf dStatus;  // MapDev::status
     // Component-split fused schedule: [waveStart[0..4] | component order]
     DBuf dSplit;
     int splitOcc[2] = {0, 0};   // resident blocks per CU of the two builds
@@ -548,6 +549,10 @@ void build_plan(mpart_map_s& m) {
         }
     }
     M.nugget = o.nugget;
+    // status word of the integrand check (0: no infinite g met since the last check)
+    m.dStatus.ensure(sizeof(int));
+    hip_check(hipMemset(m.dStatus.p, 0, sizeof(int)), "hipMemset");
+    M.status = m.dStatus.as<int>();
 
     // Off-diagonal table slots: max off-diagonal degree of every coordinate.
     const unsigned D = m.inDim;
@@ -701,6 +706,23 @@ mpart_map_s* make_map(const mpart_options& o, std::vector<MultiSet> comps, bool 
 
 void check_map(mpart_map m) {
     if (!m) throw ArgError("null map handle");
+}
+
+// Waits for s, then fails with the reference's runtime error
+// (MonotoneIntegrand.h:183-190) when a kernel on the map met an infinite g at
+// a quadrature node since the last check, and clears the status word.
+const char* const kInfiniteIntegrand =
+    "MonotoneIntegrand: nan was encountered in value of g(df(...)). Use MonotoneIntegrand::setFailOnNaN for "
+    "enabling NaN propagation.";
+bool take_integrand_status(mpart_map_s& m, hipStream_t s) {
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    int v = 0;
+    hip_check(hipMemcpy(&v, m.dev.status, sizeof(int), hipMemcpyDeviceToHost), "hipMemcpy");
+    if (v) hip_check(hipMemset(m.dev.status, 0, sizeof(int)), "hipMemset");
+    return v != 0;
+}
+void check_integrand(mpart_map_s& m, hipStream_t s) {
+    if (take_integrand_status(m, s)) throw RtError(kInfiniteIntegrand);
 }
 void check_coeffs(mpart_map m) {
     if (!m->coeffs) throw RtError("The coefficients have not been set for this map");
@@ -1856,6 +1878,7 @@ void batch_inverse(mpart_batch_s& b, const std::vector<mpd::BatchIO>& io, double
                 mpd::InvSeg g{};
                 g.map = (int)i;
                 g.p = (int)ms.maxDeg[ms.dim - 1];
+                g.status = m->dev.status;
                 g.base = ps.Ntot;
                 g.N = d.N;
                 g.w = d.w;
@@ -2011,11 +2034,13 @@ struct KLObjectiveEval {
         hip_check(mpd::launch_copy(d, 1, K, 0, 1, m.coeffs, 0, m.coeffStride, s), "launch copy");
         do_fused(m, pts, nullptr, 0, 0, nullptr, 0, 1, d + K, d + K + 1, 1.0 / (double)pts.cols, s);
         hip_check(hipMemcpyAsync(host.data(), d + K, (K + 1) * sizeof(double), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
-        hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+        // an infinite integrand at these coefficients makes the trial point
+        // infeasible: the line search backtracks from it
+        const bool infG = take_integrand_status(m, s);
         ++evals;
         g.assign(host.begin() + 1, host.end());
         const double f = host[0] + offset;
-        return std::isnan(f) ? std::numeric_limits<double>::infinity() : f;
+        return (infG || std::isnan(f)) ? std::numeric_limits<double>::infinity() : f;
     }
 };
 
@@ -2265,6 +2290,13 @@ const char* mpart_last_error(void) { return g_err.c_str(); }
 
 int mpart_is_host_memory(const void* p) { return host_memory(p) ? 1 : 0; }
 
+int mpart_check_integrand(mpart_map m, mpart_stream s) {
+    return guarded([&] {
+        check_map(m);
+        check_integrand(*m, S(s));
+    });
+}
+
 int mpart_init(void) {
     return guarded([&] {
         static std::once_flag once;
@@ -2434,6 +2466,7 @@ int mpart_evaluate(mpart_map m, mpart_cview pts, mpart_view out, mpart_stream s)
         Staging st(S(s));
         do_evaluate(*m, st.in(pts), st.out(out), nullptr, S(s));
         st.finish();
+        if (st.staged()) check_integrand(*m, S(s));
     });
 }
 
@@ -2454,6 +2487,7 @@ int mpart_log_determinant(mpart_map m, mpart_cview pts, double* out, int64_t str
         double* o = st.out_vec(out, pts.cols, stride);
         do_logdet(*m, p, o, o != out ? 1 : stride, S(s));
         st.finish();
+        if (st.staged()) check_integrand(*m, S(s));
     });
 }
 
@@ -2465,6 +2499,7 @@ int mpart_gradient(mpart_map m, mpart_cview pts, mpart_cview sens, mpart_view ou
         const mpart_cview sv = st.in(sens);
         do_input_grad(*m, st.in(pts), &sv, st.out(out), 0, S(s));
         st.finish();
+        if (st.staged()) check_integrand(*m, S(s));
     });
 }
 
@@ -2475,6 +2510,7 @@ int mpart_coeff_grad(mpart_map m, mpart_cview pts, mpart_cview sens, mpart_view 
         Staging st(S(s));
         do_coeff_grad(*m, st.in(pts), st.in(sens), st.out(out), S(s));
         st.finish();
+        if (st.staged()) check_integrand(*m, S(s));
     });
 }
 
@@ -2485,6 +2521,7 @@ int mpart_log_determinant_coeff_grad(mpart_map m, mpart_cview pts, mpart_view ou
         Staging st(S(s));
         do_logdet_coeff_grad(*m, st.in(pts), st.out(out), S(s));
         st.finish();
+        if (st.staged()) check_integrand(*m, S(s));
     });
 }
 
@@ -2495,6 +2532,7 @@ int mpart_log_determinant_input_grad(mpart_map m, mpart_cview pts, mpart_view ou
         Staging st(S(s));
         do_input_grad(*m, st.in(pts), nullptr, st.out(out), 1, S(s));
         st.finish();
+        if (st.staged()) check_integrand(*m, S(s));
     });
 }
 
@@ -2506,6 +2544,7 @@ int mpart_inverse(mpart_map m, mpart_cview x1, mpart_cview r, mpart_view out, do
         Staging st(S(s));
         do_inverse(*m, st.in(x1), st.in(r), st.out(out), xtol, ytol, S(s));
         st.finish();
+        if (st.staged()) check_integrand(*m, S(s));
     });
 }
 
@@ -2519,6 +2558,7 @@ int mpart_kl_value_and_grad(mpart_map m, mpart_cview pts, double* loss, double* 
         do_fused(*m, st.in(pts), nullptr, 0, 0, nullptr, 0, 1, st.out_vec(loss, 1, 1), st.out_vec(grad, m->numCoeffs, 1),
                  1.0 / (double)pts.cols, S(s));
         st.finish();
+        if (st.staged()) check_integrand(*m, S(s));
     });
 }
 

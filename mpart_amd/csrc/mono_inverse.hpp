@@ -179,6 +179,7 @@ __global__ void __launch_bounds__(256) k_inverse_queue(MapDev M, const double* _
     dc.lin = LIN ? M.lin : 0;
     dc.lb = M.lb;
     dc.ub = M.ub;
+    dc.status = BATCH ? nullptr : M.status;
     double f0 = 0.0, yd = 0.0, x0 = 0.0;
     int phase = IP_START, bi = 0, it = 0;
     double step = 1.0, xlb = 0.0, xub = 0.0, ylb = 0.0, yub = 0.0, nhalf = 0.0, xc = 0.0;
@@ -241,6 +242,7 @@ __global__ void __launch_bounds__(256) k_inverse_queue(MapDev M, const double* _
                         op = S.o + local * S.os;
                         op2 = S.o2 ? S.o2 + local * S.os2 : nullptr;
                         dc.p = S.p;
+                        dc.status = S.status;
                     }
                 }
                 if (idx < N && !skip) {
@@ -316,7 +318,10 @@ __global__ void __launch_bounds__(256) k_inverse_queue(MapDev M, const double* _
                 fine_ += wr * (fm + 4.0 * frm + fb);
                 const double err = fabs(fine_ - coarse_);
                 const double tol = fmax(Q.relTol * fabs(coarse_), Q.absTol);
-                if (((err < tol) || (level == Q.maxSub - 1)) && (level >= Q.minSub)) {
+                if (nonfinite_err(err)) {
+                    res += fine_;  // non-finite already: ends the quadrature (integrate_as)
+                    qdone = true;
+                } else if (((err < tol) || (level == Q.maxSub - 1)) && (level >= Q.minSub)) {
                     res += fine_;
                     if (sp == 0) {
                         qdone = true;
@@ -380,7 +385,10 @@ __global__ void __launch_bounds__(256) k_inverse_queue(MapDev M, const double* _
                 }
                 const double err = fabs(fine_ - coarse_);
                 const double tol = fmax(Q.relTol * fabs(coarse_), Q.absTol);
-                if (((err < tol) || (level == Q.maxSub - 1)) && (level >= Q.minSub)) {
+                if (nonfinite_err(err)) {
+                    res += fine_;  // as above
+                    qdone = true;
+                } else if (((err < tol) || (level == Q.maxSub - 1)) && (level >= Q.minSub)) {
                     res += fine_;
                     if (sp == 0) {
                         qdone = true;
@@ -409,11 +417,14 @@ __global__ void __launch_bounds__(256) k_inverse_queue(MapDev M, const double* _
         }
         if (!qdone) continue;
 
-        // T at the current iterate; advance the bracket search or ITP.
+        // T at the current iterate; advance the bracket search or ITP.  A
+        // non-finite T ends the solve with NaN: with finite inputs it comes
+        // from an infinite g, where the reference fails.
         const double y = res + f0;
-        bool fail = false, done = false;
+        bool fail = nonfinite_err(fabs(y)), done = false;
         double root = 0.0;
-        if (phase == IP_START) {
+        if (fail) {
+        } else if (phase == IP_START) {
             // InverseSingleBracket and FindBracket evaluate T(x0) three times;
             // the values are identical.
             xlb = xub = x0;

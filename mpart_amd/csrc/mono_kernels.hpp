@@ -637,12 +637,15 @@ __device__ __forceinline__ double itp_point(double xlb, double xub, double ylb, 
 
 // Bracket (FindBracket, RootFinding.h:20-80) and ITP (Find_x_ITP, 82-178) for
 // T(x) = yd with T given by its diagonal coefficients dc (x_k replaced by the
-// iterate) and f0 = f(x_<k, 0).  Returns NaN when no bracket is found.
+// iterate) and f0 = f(x_<k, 0).  Returns NaN when no bracket is found, and
+// when T is not finite at an iterate (with finite inputs, an infinite g,
+// where the reference fails).
 template <int BK, int PK, int QK, int PM>
 __device__ double inverse_solve(const MapDev& M, const DiagCoef<PM>& dc, double f0, double x0, double yd, double xtol,
                                 double ytol) {
     double xlb = x0, xub = x0;
     double ylb = eval_T<BK, PK, QK, PM>(M, dc, f0, xlb);
+    if (nonfinite_err(fabs(ylb))) return NAN;
     double yub = ylb;
     int info = 0;
     {
@@ -657,6 +660,7 @@ __device__ double inverse_solve(const MapDev& M, const DiagCoef<PM>& dc, double 
             for (i = 0; i < maxIts; ++i) {
                 xlb = xub - stepSize;
                 ylb = eval_T<BK, PK, QK, PM>(M, dc, f0, xlb);
+                if (nonfinite_err(fabs(ylb))) return NAN;
                 if ((fabs((yub - ylb) / (xub - xlb)) < 1e-12) && ((xub - xlb) > 10)) { info = -1; break; }
                 if (ylb > yd) { yub = ylb; xub = xlb; stepSize *= 2.0; } else break;
             }
@@ -666,6 +670,7 @@ __device__ double inverse_solve(const MapDev& M, const DiagCoef<PM>& dc, double 
             for (i = 0; i < maxIts; ++i) {
                 xub = xlb + stepSize;
                 yub = eval_T<BK, PK, QK, PM>(M, dc, f0, xub);
+                if (nonfinite_err(fabs(yub))) return NAN;
                 if ((fabs((yub - ylb) / (xub - xlb)) < 1e-12) && ((xub - xlb) > 10)) { info = -1; break; }
                 if (yub < yd) { ylb = yub; xlb = xub; stepSize *= 2.0; } else break;
             }
@@ -677,6 +682,7 @@ __device__ double inverse_solve(const MapDev& M, const DiagCoef<PM>& dc, double 
     for (unsigned it = 0; it < ITP_MAX_ITS; ++it) {
         const double xc = itp_point(xlb, xub, ylb, yub, nhalf, (int)it, xtol);
         const double yc = eval_T<BK, PK, QK, PM>(M, dc, f0, xc);
+        if (nonfinite_err(fabs(yc))) return NAN;
         if (fabs(yc - yd) < ytol) {
             return xc;
         } else if (yc > yd) {

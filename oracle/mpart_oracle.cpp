@@ -24,8 +24,33 @@
 #include <vector>
 #include <algorithm>
 #include <omp.h>
+#include <atomic>
+#include <exception>
 
 namespace orc {
+
+// An exception may not leave an OpenMP region (the process would end): each
+// point's work catches it, the first one is kept, the remaining points are
+// skipped, and it is thrown again after the region, as the serial loop of
+// the reference would throw it.
+class OmpError {
+   public:
+    bool failed() const { return failed_.load(std::memory_order_relaxed); }
+    void keep() {
+#pragma omp critical(orc_omp_error)
+        {
+            if (!e_) e_ = std::current_exception();
+        }
+        failed_.store(true, std::memory_order_relaxed);
+    }
+    void rethrow() {
+        if (e_) std::rethrow_exception(e_);
+    }
+
+   private:
+    std::atomic<bool> failed_{false};
+    std::exception_ptr e_;
+};
 
 enum BasisType { PROB_HERMITE = 0, PHYS_HERMITE = 1, HERMITE_FN = 2 };
 enum PosFuncType { POS_EXP = 0, POS_SOFTPLUS = 1 };
@@ -1012,56 +1037,77 @@ struct Component : Map {
     // EvaluateImpl (255-299); note: the nugget is not passed (line 287).
     void evaluate(const CMat& pts, const double* w, double* out, long os, long* nevals) const {
         long N = pts.cols;
+        OmpError err;
 #pragma omp parallel
         {
             Scratch s = make_scratch(1);
 #pragma omp for schedule(dynamic, 64)
             for (long n = 0; n < N; ++n) {
-                PtView pt{&pts.p[n * pts.cs], pts.rs, dim()};
-                ex.fill_cache1(s.cache.data(), pt, D_NONE);
-                long ne = 0;
-                out[n * os] = evaluate_single(s, pt, pt(dim() - 1), w, 0.0, nevals ? &ne : nullptr);
-                if (nevals) nevals[n] = ne;
+                if (err.failed()) continue;
+                try {
+                    PtView pt{&pts.p[n * pts.cs], pts.rs, dim()};
+                    ex.fill_cache1(s.cache.data(), pt, D_NONE);
+                    long ne = 0;
+                    out[n * os] = evaluate_single(s, pt, pt(dim() - 1), w, 0.0, nevals ? &ne : nullptr);
+                    if (nevals) nevals[n] = ne;
+                } catch (...) {
+                    err.keep();
+                }
             }
         }
+        err.rethrow();
     }
 
     // ContinuousDerivative (488-531)
     void continuous_derivative(const CMat& pts, const double* w, double* derivs) const {
         long N = pts.cols;
+        OmpError err;
 #pragma omp parallel
         {
             Scratch s = make_scratch(1);
 #pragma omp for schedule(dynamic, 64)
             for (long n = 0; n < N; ++n) {
-                PtView pt{&pts.p[n * pts.cs], pts.rs, dim()};
-                ex.fill_cache1(s.cache.data(), pt, D_NONE);
-                ex.fill_cache2(s.cache.data(), pt(dim() - 1), D_DIAGONAL);
-                derivs[n] = ex.diag_deriv(s.cache.data(), w, 1);
-                derivs[n] = g.eval(derivs[n]);
+                if (err.failed()) continue;
+                try {
+                    PtView pt{&pts.p[n * pts.cs], pts.rs, dim()};
+                    ex.fill_cache1(s.cache.data(), pt, D_NONE);
+                    ex.fill_cache2(s.cache.data(), pt(dim() - 1), D_DIAGONAL);
+                    derivs[n] = ex.diag_deriv(s.cache.data(), w, 1);
+                    derivs[n] = g.eval(derivs[n]);
+                } catch (...) {
+                    err.keep();
+                }
             }
         }
+        err.rethrow();
     }
 
     // DiscreteDerivative (586-638)
     void discrete_derivative(const CMat& pts, const double* w, double* evals, double* derivs) const {
         long N = pts.cols;
+        OmpError err;
 #pragma omp parallel
         {
             Scratch s = make_scratch(2);
 #pragma omp for schedule(dynamic, 64)
             for (long n = 0; n < N; ++n) {
-                PtView pt{&pts.p[n * pts.cs], pts.rs, dim()};
-                ex.fill_cache1(s.cache.data(), pt, D_NONE);
-                Integrand I = integrand(s, pt, pt(dim() - 1), w, D_DIAGONAL, nugget, nullptr);
-                double both[2];
-                quad.integrate(s.ws.data(), I, 2, 0.0, 1.0, both);
-                evals[n] = both[0];
-                derivs[n] = both[1];
-                ex.fill_cache2(s.cache.data(), 0.0, D_NONE);
-                evals[n] += ex.evaluate(s.cache.data(), w);
+                if (err.failed()) continue;
+                try {
+                    PtView pt{&pts.p[n * pts.cs], pts.rs, dim()};
+                    ex.fill_cache1(s.cache.data(), pt, D_NONE);
+                    Integrand I = integrand(s, pt, pt(dim() - 1), w, D_DIAGONAL, nugget, nullptr);
+                    double both[2];
+                    quad.integrate(s.ws.data(), I, 2, 0.0, 1.0, both);
+                    evals[n] = both[0];
+                    derivs[n] = both[1];
+                    ex.fill_cache2(s.cache.data(), 0.0, D_NONE);
+                    evals[n] += ex.evaluate(s.cache.data(), w);
+                } catch (...) {
+                    err.keep();
+                }
             }
         }
+        err.rethrow();
     }
 
     // LogDeterminantImpl (105-125)
@@ -1081,103 +1127,138 @@ struct Component : Map {
     void coeff_jacobian(const CMat& pts, const double* w, double* evals, const Mat& jac) const {
         long N = pts.cols;
         unsigned nt = ex.nterms();
+        OmpError err;
 #pragma omp parallel
         {
             Scratch s = make_scratch(nt + 1);
 #pragma omp for schedule(dynamic, 64)
             for (long n = 0; n < N; ++n) {
-                PtView pt{&pts.p[n * pts.cs], pts.rs, dim()};
-                double* jv = &jac.p[n * jac.cs];
-                ex.fill_cache1(s.cache.data(), pt, D_NONE);
-                Integrand I = integrand(s, pt, pt(dim() - 1), w, D_PARAMETERS, nugget, nullptr);
-                quad.integrate(s.ws.data(), I, nt + 1, 0.0, 1.0, s.integral.data());
-                evals[n] = s.integral[0];
-                ex.fill_cache2(s.cache.data(), 0.0, D_NONE);
-                evals[n] += ex.coeff_deriv(s.cache.data(), w, jv, jac.rs);
-                for (unsigned t = 0; t < nt; ++t) jv[t * jac.rs] += s.integral[t + 1];
+                if (err.failed()) continue;
+                try {
+                    PtView pt{&pts.p[n * pts.cs], pts.rs, dim()};
+                    double* jv = &jac.p[n * jac.cs];
+                    ex.fill_cache1(s.cache.data(), pt, D_NONE);
+                    Integrand I = integrand(s, pt, pt(dim() - 1), w, D_PARAMETERS, nugget, nullptr);
+                    quad.integrate(s.ws.data(), I, nt + 1, 0.0, 1.0, s.integral.data());
+                    evals[n] = s.integral[0];
+                    ex.fill_cache2(s.cache.data(), 0.0, D_NONE);
+                    evals[n] += ex.coeff_deriv(s.cache.data(), w, jv, jac.rs);
+                    for (unsigned t = 0; t < nt; ++t) jv[t * jac.rs] += s.integral[t + 1];
+                } catch (...) {
+                    err.keep();
+                }
             }
         }
+        err.rethrow();
     }
 
     // InputJacobian (758-818)
     void input_jacobian(const CMat& pts, const double* w, double* evals, const Mat& jac) const {
         long N = pts.cols;
         unsigned D = dim();
+        OmpError err;
 #pragma omp parallel
         {
             Scratch s = make_scratch(D + 1);
 #pragma omp for schedule(dynamic, 64)
             for (long n = 0; n < N; ++n) {
-                PtView pt{&pts.p[n * pts.cs], pts.rs, D};
-                double* jv = &jac.p[n * jac.cs];
-                ex.fill_cache1(s.cache.data(), pt, D_INPUT);
-                Integrand I = integrand(s, pt, pt(D - 1), w, D_INPUT, nugget, nullptr);
-                quad.integrate(s.ws.data(), I, D + 1, 0.0, 1.0, s.integral.data());
-                evals[n] = s.integral[0];
-                ex.fill_cache2(s.cache.data(), 0.0, D_INPUT);
-                evals[n] += ex.input_deriv(s.cache.data(), w, jv, jac.rs);
-                for (unsigned d = 0; d + 1 < D; ++d) jv[d * jac.rs] += s.integral[d + 1];
-                jv[(D - 1) * jac.rs] = s.integral[D];
+                if (err.failed()) continue;
+                try {
+                    PtView pt{&pts.p[n * pts.cs], pts.rs, D};
+                    double* jv = &jac.p[n * jac.cs];
+                    ex.fill_cache1(s.cache.data(), pt, D_INPUT);
+                    Integrand I = integrand(s, pt, pt(D - 1), w, D_INPUT, nugget, nullptr);
+                    quad.integrate(s.ws.data(), I, D + 1, 0.0, 1.0, s.integral.data());
+                    evals[n] = s.integral[0];
+                    ex.fill_cache2(s.cache.data(), 0.0, D_INPUT);
+                    evals[n] += ex.input_deriv(s.cache.data(), w, jv, jac.rs);
+                    for (unsigned d = 0; d + 1 < D; ++d) jv[d * jac.rs] += s.integral[d + 1];
+                    jv[(D - 1) * jac.rs] = s.integral[D];
+                } catch (...) {
+                    err.keep();
+                }
             }
         }
+        err.rethrow();
     }
 
     // ContinuousMixedJacobian (825-874)
     void continuous_mixed_jacobian(const CMat& pts, const double* w, const Mat& jac) const {
         long N = pts.cols;
+        OmpError err;
 #pragma omp parallel
         {
             Scratch s = make_scratch(1);
 #pragma omp for schedule(dynamic, 64)
             for (long n = 0; n < N; ++n) {
-                PtView pt{&pts.p[n * pts.cs], pts.rs, dim()};
-                double* jv = &jac.p[n * jac.cs];
-                ex.fill_cache1(s.cache.data(), pt, D_NONE);
-                ex.fill_cache2(s.cache.data(), pt(dim() - 1), D_DIAGONAL);
-                double df = ex.mixed_coeff_deriv(s.cache.data(), w, 1, jv, jac.rs);
-                double dgdf = g.deriv(df);
-                for (unsigned i = 0; i < ex.nterms(); ++i) jv[i * jac.rs] *= dgdf;
+                if (err.failed()) continue;
+                try {
+                    PtView pt{&pts.p[n * pts.cs], pts.rs, dim()};
+                    double* jv = &jac.p[n * jac.cs];
+                    ex.fill_cache1(s.cache.data(), pt, D_NONE);
+                    ex.fill_cache2(s.cache.data(), pt(dim() - 1), D_DIAGONAL);
+                    double df = ex.mixed_coeff_deriv(s.cache.data(), w, 1, jv, jac.rs);
+                    double dgdf = g.deriv(df);
+                    for (unsigned i = 0; i < ex.nterms(); ++i) jv[i * jac.rs] *= dgdf;
+                } catch (...) {
+                    err.keep();
+                }
             }
         }
+        err.rethrow();
     }
 
     // ContinuousMixedInputJacobian (877-926)
     void continuous_mixed_input_jacobian(const CMat& pts, const double* w, const Mat& jac) const {
         long N = pts.cols;
         unsigned D = dim();
+        OmpError err;
 #pragma omp parallel
         {
             Scratch s = make_scratch(1);
 #pragma omp for schedule(dynamic, 64)
             for (long n = 0; n < N; ++n) {
-                PtView pt{&pts.p[n * pts.cs], pts.rs, D};
-                double* jv = &jac.p[n * jac.cs];
-                ex.fill_cache1(s.cache.data(), pt, D_MIXEDINPUT);
-                ex.fill_cache2(s.cache.data(), pt(D - 1), D_MIXEDINPUT);
-                double df = ex.mixed_input_deriv(s.cache.data(), w, jv, jac.rs);
-                double dgdf = g.deriv(df);
-                for (unsigned i = 0; i < D; ++i) jv[i * jac.rs] *= dgdf;
+                if (err.failed()) continue;
+                try {
+                    PtView pt{&pts.p[n * pts.cs], pts.rs, D};
+                    double* jv = &jac.p[n * jac.cs];
+                    ex.fill_cache1(s.cache.data(), pt, D_MIXEDINPUT);
+                    ex.fill_cache2(s.cache.data(), pt(D - 1), D_MIXEDINPUT);
+                    double df = ex.mixed_input_deriv(s.cache.data(), w, jv, jac.rs);
+                    double dgdf = g.deriv(df);
+                    for (unsigned i = 0; i < D; ++i) jv[i * jac.rs] *= dgdf;
+                } catch (...) {
+                    err.keep();
+                }
             }
         }
+        err.rethrow();
     }
 
     // DiscreteMixedJacobian (929-980)
     void discrete_mixed_jacobian(const CMat& pts, const double* w, const Mat& jac) const {
         long N = pts.cols;
         unsigned nt = ex.nterms();
+        OmpError err;
 #pragma omp parallel
         {
             Scratch s = make_scratch(nt + 1);
 #pragma omp for schedule(dynamic, 64)
             for (long n = 0; n < N; ++n) {
-                PtView pt{&pts.p[n * pts.cs], pts.rs, dim()};
-                double* jv = &jac.p[n * jac.cs];
-                ex.fill_cache1(s.cache.data(), pt, D_NONE);
-                Integrand I = integrand(s, pt, pt(dim() - 1), w, D_MIXEDCOEFF, nugget, nullptr);
-                quad.integrate(s.ws.data(), I, nt + 1, 0.0, 1.0, s.integral.data());
-                for (unsigned t = 0; t < nt; ++t) jv[t * jac.rs] += s.integral[t + 1];
+                if (err.failed()) continue;
+                try {
+                    PtView pt{&pts.p[n * pts.cs], pts.rs, dim()};
+                    double* jv = &jac.p[n * jac.cs];
+                    ex.fill_cache1(s.cache.data(), pt, D_NONE);
+                    Integrand I = integrand(s, pt, pt(dim() - 1), w, D_MIXEDCOEFF, nugget, nullptr);
+                    quad.integrate(s.ws.data(), I, nt + 1, 0.0, 1.0, s.integral.data());
+                    for (unsigned t = 0; t < nt; ++t) jv[t * jac.rs] += s.integral[t + 1];
+                } catch (...) {
+                    err.keep();
+                }
             }
         }
+        err.rethrow();
     }
 
     // GradientImpl (148-164)
@@ -1232,24 +1313,31 @@ struct Component : Map {
         if (xtol < 0 || ytol < 0) throw std::invalid_argument("Invalid tolerance");
         long numXs = xs.cols;
         if ((numXs != 1) && (numXs != numPts)) throw std::invalid_argument("Invalid argument sizes given to MonotoneComponent::Inverse");
+        OmpError err;
 #pragma omp parallel
         {
             Scratch s = make_scratch(1);
 #pragma omp for schedule(dynamic, 16)
             for (long n = 0; n < numPts; ++n) {
-                long xi = (numXs == 1) ? 0 : n;
-                PtView pt{&xs.p[xi * xs.cs], xs.rs, unsigned(xs.rows)};
-                bool hasNan = false;
-                for (unsigned i = 0; i < pt.n; ++i)
-                    if (std::isnan(pt(i))) { hasNan = true; break; }
-                if (hasNan) { out[n * os] = std::numeric_limits<double>::quiet_NaN(); continue; }
-                ex.fill_cache1(s.cache.data(), pt, D_NONE);
-                PtView ptd{pt.p, pt.stride, dim()};
-                auto f = [&](double x) { return evaluate_single(s, ptd, x, w, nugget, nullptr); };
-                int info;
-                out[n * os] = inverse_single_bracket(ys[n * ys_s], f, pt(pt.n - 1), xtol, ytol, info);
+                if (err.failed()) continue;
+                try {
+                    long xi = (numXs == 1) ? 0 : n;
+                    PtView pt{&xs.p[xi * xs.cs], xs.rs, unsigned(xs.rows)};
+                    bool hasNan = false;
+                    for (unsigned i = 0; i < pt.n; ++i)
+                        if (std::isnan(pt(i))) { hasNan = true; break; }
+                    if (hasNan) { out[n * os] = std::numeric_limits<double>::quiet_NaN(); continue; }
+                    ex.fill_cache1(s.cache.data(), pt, D_NONE);
+                    PtView ptd{pt.p, pt.stride, dim()};
+                    auto f = [&](double x) { return evaluate_single(s, ptd, x, w, nugget, nullptr); };
+                    int info;
+                    out[n * os] = inverse_single_bracket(ys[n * ys_s], f, pt(pt.n - 1), xtol, ytol, info);
+                } catch (...) {
+                    err.keep();
+                }
             }
         }
+        err.rethrow();
     }
 };
 

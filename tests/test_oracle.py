@@ -359,3 +359,35 @@ def test_kl_gradient_fd_and_constant():
     # Loss equals mean(0.5 |T|^2 - logdet) computed from Evaluate/LogDeterminant.
     T = tri.Evaluate(x)
     assert loss == pytest.approx(np.mean(0.5 * (T ** 2).sum(0) - tri.LogDeterminant(x)), rel=1e-13)
+
+
+# --------------------------------------------------------------------------
+# Infinite integrand (MParT/MonotoneIntegrand.h:183-190): g(df) = inf at a
+# quadrature node raises; LogDeterminant, which evaluates g outside the
+# integrand (ContinuousDerivative), does not.  One component in 1-D, order 2,
+# coefficient 1000 on the normalized He_2: df(y) = 2000 y / sqrt(2 sqrt(2 pi))
+# ~ 893 y, so exp(df) overflows for y > 0.795 (x = 1.0 reaches it at t > 0.795;
+# x = 0.5 never does).
+def _inf_component(posFunc="Exp", **kw):
+    c = orc.CreateComponent(orc.options(posFuncType=posFunc, **kw), 1, 2)
+    c.SetCoeffs(np.array([0.0, 0.0, 1000.0]))
+    return c
+
+
+def test_infinite_integrand_raises():
+    c = _inf_component()
+    y = c.Evaluate(np.array([[0.5]]))
+    assert np.isfinite(y).all() and y[0, 0] > 1e150  # ~ e^446.5 / 1786
+    with pytest.raises(RuntimeError, match="MonotoneIntegrand"):
+        c.Evaluate(np.array([[0.5, 1.0]]))
+    with pytest.raises(RuntimeError, match="MonotoneIntegrand"):
+        c.CoeffGrad(np.array([[1.0]]), np.ones((1, 1)))
+    with pytest.raises(RuntimeError, match="MonotoneIntegrand"):
+        c.Gradient(np.array([[1.0]]), np.ones((1, 1)))
+    # LogDeterminant does not integrate: g(df(1)) = inf gives log = inf, no error
+    assert c.LogDeterminant(np.array([[1.0]]))[0] == np.inf
+    # SoftPlus(893) is finite: no error
+    assert np.isfinite(_inf_component("SoftPlus").Evaluate(np.array([[1.0]]))).all()
+    # the discrete derivative integrates: its LogDeterminant raises
+    with pytest.raises(RuntimeError, match="MonotoneIntegrand"):
+        _inf_component(contDeriv=False).LogDeterminant(np.array([[1.0]]))
