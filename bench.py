@@ -313,6 +313,9 @@ def many_maps(steps, dmax=8, pmax=4, nmax=16384, coeff_grad=True, inverse=True):
         res["sequence"]["inverse_roundtrip_max_abs_err_finite"] = max(e.max().item() if e.numel() else 0.0 for e in ex)
         res["sequence"]["inverse_points_err_above_1e-6"] = int(sum((e > 1e-6).sum().item() for e in ex))
         res["sequence"]["inverse_points_err_above_1e-3"] = int(sum((e > 1e-3).sum().item() for e in ex))
+        # maps where some call met an infinite g (the reference would have
+        # failed there with its runtime error); their solves end with NaN
+        res["sequence"]["maps_with_infinite_g"] = _maps_with_infinite_g(maps)
         sched, trial = b.InverseSchedule()
         res["sequence"]["inverse_schedule"] = {"choice": sched, "trial_ms_schedules_1_2": trial,
                                                "meaning": "1: per-lane solve scheduling, one pass per component "
@@ -356,8 +359,22 @@ def profile_inverse_many_mode():
         b.InverseImpl(x1s, ys, xis)
         torch.cuda.synchronize()
         out[f"schedule_{sched}_ms"] = (time.perf_counter() - t0) * 1e3
+        out[f"schedule_{sched}_nan_roots"] = int(sum(int(torch.isnan(x).sum().item()) for x in xis))
+        out[f"schedule_{sched}_maps_with_infinite_g"] = _maps_with_infinite_g(maps)
         log(f"inverse schedule {sched}: {out[f'schedule_{sched}_ms']:.1f} ms")
     print(json.dumps({"profile": "inverse_many_maps_full_ranges", **out}), flush=True)
+
+
+def _maps_with_infinite_g(maps):
+    """How many maps met an infinite g at a quadrature node since their last
+    check (the reference's runtime-error condition); clears the records."""
+    n = 0
+    for m in maps:
+        try:
+            m.CheckIntegrand()
+        except RuntimeError:
+            n += 1
+    return n
 
 
 def single_component_setup():
