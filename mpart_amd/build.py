@@ -27,7 +27,7 @@ CFLAGS = [
 
 SOURCES = ["mono_host.hip", "codegen.hip", "k_eval.hip", "k_coef.hip", "k_input.hip", "k_inverse.hip", "k_inverse_batch.hip",
            "k_fused.hip", "k_batch.hip", "k_group.hip",
-           "k_group_fused.hip", "k_split.hip"]
+           "k_group_fused.hip", "k_split.hip", "k_split2.hip", "k_split_eval.hip"]
 ROCM_LIB = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib")
 
 

@@ -34,7 +34,7 @@ typedef __hip_internal::uint32_t uint32_t;
 namespace mpd {
 
 constexpr int MAXP = 12;        // largest last-coordinate degree held in registers
-constexpr int MAXND = 8;        // largest number of off-diagonal factors per term
+constexpr int MAXND = 8;        // largest factor count with an unrolled term-record loop (more: run-time loop)
 constexpr int MAXSUB = 32;      // largest adaptive depth (quadMaxSub)
 constexpr int MAXNF = 17;       // largest nested Clenshaw-Curtis fine rule
 constexpr int MAXORD = 16;      // largest off-diagonal degree in the basis table
@@ -690,6 +690,26 @@ MPD_HD void basis_vals_derivs(const MapDev& M, double x, double* b, double* db) 
 // ---------------------------------------------------------------------------
 // Group passes over the term records of one component.
 
+// Record length and off-diagonal product of a term record.  ND > 0 is the
+// map's factor-slot count known at compile time (the loop unrolls); ND = 0
+// reads it from the map, for terms with more factors than the unrolled cases.
+template <int ND>
+MPD_HD long rec_len(const MapDev& M) {
+    if constexpr (ND > 0) return ND + 1;
+    else return M.trec;
+}
+template <int ND>
+MPD_HD double rec_product(const MapDev& M, const uint32_t* rec, const double* tab, long stride) {
+    double P = tab[(long)rec[1] * stride];
+    if constexpr (ND > 0) {
+#pragma unroll
+        for (int e = 2; e <= ND; ++e) P *= tab[(long)rec[e] * stride];
+    } else {
+        for (int e = 2; e <= M.nd; ++e) P *= tab[(long)rec[e] * stride];
+    }
+    return P;
+}
+
 // C_q = sum_{j in group q} w_j P_j for q = 0..p.
 template <int ND, int PM>
 MPD_HD void coeff_groups_nd(const MapDev& M, const CompDev& c, const double* w, long ws,
@@ -700,11 +720,8 @@ MPD_HD void coeff_groups_nd(const MapDev& M, const CompDev& c, const double* w, 
         const GroupDev g = M.groups[c.groupOff + q];
         double acc = 0.0;
         for (int j = g.start; j < g.start + g.count; ++j) {
-            const uint32_t* rec = M.terms + (long)j * (ND + 1);
-            double P = tab[(long)rec[1] * stride];
-#pragma unroll
-            for (int e = 2; e <= ND; ++e) P *= tab[(long)rec[e] * stride];
-            acc += w[(long)rec[0] * ws] * P;
+            const uint32_t* rec = M.terms + (long)j * rec_len<ND>(M);
+            acc += w[(long)rec[0] * ws] * rec_product<ND>(M, rec, tab, stride);
         }
         C[q] = acc;
     }
@@ -721,7 +738,8 @@ MPD_HD void coeff_groups(const MapDev& M, const CompDev& c, const double* w, lon
         case 5: coeff_groups_nd<5, PM>(M, c, w, ws, tab, stride, C); break;
         case 6: coeff_groups_nd<6, PM>(M, c, w, ws, tab, stride, C); break;
         case 7: coeff_groups_nd<7, PM>(M, c, w, ws, tab, stride, C); break;
-        default: coeff_groups_nd<8, PM>(M, c, w, ws, tab, stride, C); break;
+        case 8: coeff_groups_nd<8, PM>(M, c, w, ws, tab, stride, C); break;
+        default: coeff_groups_nd<0, PM>(M, c, w, ws, tab, stride, C); break;
     }
 }
 
@@ -734,11 +752,8 @@ MPD_HD void visit_terms_nd(const MapDev& M, const CompDev& c, const double* tab,
         if (q > c.p) break;
         const GroupDev g = M.groups[c.groupOff + q];
         for (int j = g.start; j < g.start + g.count; ++j) {
-            const uint32_t* rec = M.terms + (long)j * (ND + 1);
-            double P = tab[(long)rec[1] * stride];
-#pragma unroll
-            for (int e = 2; e <= ND; ++e) P *= tab[(long)rec[e] * stride];
-            op(rec[0], P, q);
+            const uint32_t* rec = M.terms + (long)j * rec_len<ND>(M);
+            op(rec[0], rec_product<ND>(M, rec, tab, stride), q);
         }
     }
 }
@@ -754,7 +769,8 @@ MPD_HD void visit_terms(const MapDev& M, const CompDev& c, const double* tab, in
         case 5: visit_terms_nd<5, PM>(M, c, tab, stride, op); break;
         case 6: visit_terms_nd<6, PM>(M, c, tab, stride, op); break;
         case 7: visit_terms_nd<7, PM>(M, c, tab, stride, op); break;
-        default: visit_terms_nd<8, PM>(M, c, tab, stride, op); break;
+        case 8: visit_terms_nd<8, PM>(M, c, tab, stride, op); break;
+        default: visit_terms_nd<0, PM>(M, c, tab, stride, op); break;
     }
 }
 

@@ -577,7 +577,6 @@ void build_plan(mpart_map_s& m) {
                 if (s.dims[i] != s.dim - 1) off++;
             nd = std::max(nd, off);
         }
-    if (nd > mpd::MAXND) throw NotImpl("terms with more than 8 off-diagonal factors are not provided");
     const int trec = nd + 1;
     std::vector<CompDev> comps;
     std::vector<GroupDev> groups;

@@ -124,11 +124,8 @@ __device__ __forceinline__ void group_coeff_groups_nd(const MapDev& M, const Com
         const GroupDev g = M.groups[c.groupOff + q];
         double acc = 0.0;
         for (int j = g.start + gl; j < g.start + g.count; j += G) {
-            const uint32_t* rec = M.terms + (long)j * (ND + 1);
-            double P = tab[rec[1]];
-#pragma unroll
-            for (int e = 2; e <= ND; ++e) P *= tab[rec[e]];
-            acc += w[(long)rec[0] * ws] * P;
+            const uint32_t* rec = M.terms + (long)j * rec_len<ND>(M);
+            acc += w[(long)rec[0] * ws] * rec_product<ND>(M, rec, tab, 1);
         }
         C[q] = group_sum<G>(acc);
     }
@@ -145,7 +142,8 @@ __device__ __forceinline__ void group_coeff_groups(const MapDev& M, const CompDe
         case 5: group_coeff_groups_nd<5, PM, G>(M, c, w, ws, tab, gl, C); break;
         case 6: group_coeff_groups_nd<6, PM, G>(M, c, w, ws, tab, gl, C); break;
         case 7: group_coeff_groups_nd<7, PM, G>(M, c, w, ws, tab, gl, C); break;
-        default: group_coeff_groups_nd<8, PM, G>(M, c, w, ws, tab, gl, C); break;
+        case 8: group_coeff_groups_nd<8, PM, G>(M, c, w, ws, tab, gl, C); break;
+        default: group_coeff_groups_nd<0, PM, G>(M, c, w, ws, tab, gl, C); break;
     }
 }
 
@@ -425,6 +423,18 @@ __global__ void __launch_bounds__(256) k_logdet_coeff_grad_discrete(MapDev M, co
 //  mode 1: LogDeterminantInputGrad       (ContinuousMixedInputJacobian, 208-232, 877-926)
 // dT/dx_i = sum_q D^(i)_q (phi~_q(0) + rho_q S'_q) for i < k, dT/dx_k = U;
 // with D^(i)_q = sum_{j in group q} w_j dP_j/dx_i.
+// One factor of dP_j/dx_wrt: the derivative table for the factor in x_wrt
+// (has = true), the value table otherwise (slot 0 is 1.0).
+__device__ __forceinline__ void input_factor(const MapDev& M, uint32_t s, const double* tab, const double* dtab,
+                                             int stride, int wrt, bool& has, double& Pj) {
+    if (s != 0 && M.slotDim[s] == wrt) {
+        has = true;
+        Pj *= dtab[(long)s * stride];
+    } else {
+        Pj *= tab[(long)s * stride];
+    }
+}
+
 template <int ND, int PM>
 __device__ __forceinline__ void input_groups_nd(const MapDev& M, const CompDev& c, const double* w, long ws,
                                                 const double* tab, const double* dtab, int stride, int wrt,
@@ -435,18 +445,14 @@ __device__ __forceinline__ void input_groups_nd(const MapDev& M, const CompDev& 
         const GroupDev g = M.groups[c.groupOff + q];
         double acc = 0.0;
         for (int j = g.start; j < g.start + g.count; ++j) {
-            const uint32_t* rec = M.terms + (long)j * (ND + 1);
+            const uint32_t* rec = M.terms + (long)j * rec_len<ND>(M);
             bool has = false;
             double Pj = 1.0;
+            if constexpr (ND > 0) {
 #pragma unroll
-            for (int e = 1; e <= ND; ++e) {
-                const uint32_t s = rec[e];
-                if (s != 0 && M.slotDim[s] == wrt) {
-                    has = true;
-                    Pj *= dtab[(long)s * stride];
-                } else {
-                    Pj *= tab[(long)s * stride];
-                }
+                for (int e = 1; e <= ND; ++e) input_factor(M, rec[e], tab, dtab, stride, wrt, has, Pj);
+            } else {
+                for (int e = 1; e <= M.nd; ++e) input_factor(M, rec[e], tab, dtab, stride, wrt, has, Pj);
             }
             if (has) acc += w[(long)rec[0] * ws] * Pj;
         }
@@ -465,7 +471,8 @@ __device__ __forceinline__ void input_groups(const MapDev& M, const CompDev& c, 
         case 5: input_groups_nd<5, PM>(M, c, w, ws, tab, dtab, stride, wrt, Dq); break;
         case 6: input_groups_nd<6, PM>(M, c, w, ws, tab, dtab, stride, wrt, Dq); break;
         case 7: input_groups_nd<7, PM>(M, c, w, ws, tab, dtab, stride, wrt, Dq); break;
-        default: input_groups_nd<8, PM>(M, c, w, ws, tab, dtab, stride, wrt, Dq); break;
+        case 8: input_groups_nd<8, PM>(M, c, w, ws, tab, dtab, stride, wrt, Dq); break;
+        default: input_groups_nd<0, PM>(M, c, w, ws, tab, dtab, stride, wrt, Dq); break;
     }
 }
 
