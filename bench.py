@@ -606,6 +606,19 @@ def uneven_map_workload(steps):
         log(f"uneven 32-D map {name}: {ms:.2f} ms")
     out["kl_fused_points_per_s"] = N / (out["kl_fused_ms"] * 1e-3)
     out["fused_vs_plain_20000"] = out["kl_plain_20000_ms"] / out["kl_fused_20000_ms"]
+    # Roofline of the fused pass: the reference-algorithm flop count of these
+    # points' adaptive trees (mpart_amd/roofline.py algo_flops) over its wall time.
+    from mpart_amd.roofline import algo_flops
+
+    msets = [g.GetMultiIndexSet(c) for c in range(g.NumComponents())]
+    _, counts = g.EvaluateCounted(x)
+    counts = counts.cpu().numpy()
+    flops = algo_flops(msets, counts, exp_bijector=True)
+    out["mean_integrand_evals"] = float(counts.mean())
+    out["max_integrand_evals"] = int(counts.max())
+    out["kl_fused_algorithmic_flops"] = flops
+    out["kl_fused_achieved_tflops"] = flops / (out["kl_fused_ms"] * 1e-3) / 1e12
+    out["kl_fused_fp64_frac"] = out["kl_fused_achieved_tflops"] / 78.6
     lanes, trials = g.EvalSchedule()
     out["lanes_per_point"] = {"evaluate": lanes[0], "log_determinant": lanes[1], "fused": lanes[2],
                               "trial_ms_lanes_1_4_8_64_split": {"evaluate": trials[0],
@@ -776,6 +789,41 @@ def profile_grads_mode(args):
           flush=True)
 
 
+def profile_uneven_mode(args):
+    """For rocprofv3: the 32-D uneven map (orders max(1, 6 - k // 6),
+    HermiteFunctions / Exp, Student-t(5) points, seed 3, N = 200 000,
+    coefficients 0.01 N(0, 1), seed 5): Evaluate, LogDeterminant, then the
+    fused KL value + gradient, each once untimed (the schedule tuner times
+    its candidates there: lanes per point 1, 4, 8, 64 and the components split
+    over the wavefronts of a block, in two register budgets) and then `steps`
+    times on the schedule it picked."""
+    import torch
+
+    import mpart_amd as mt
+    from mpart_amd.data import Stream, student_t_points
+
+    orders = [max(1, 6 - k // 6) for k in range(32)]
+    o = mt.MapOptions(basisType=mt.BasisTypes.HermiteFunctions, posFuncType=mt.PosFuncTypes.Exp)
+    g = mt.TriangularMap([mt.CreateTotalOrderComponent(k + 1, orders[k], o) for k in range(32)])
+    g.SetCoeffs(0.01 * Stream(5).normal(g.numCoeffs))
+    n = 200_000
+    x = torch.from_numpy(student_t_points(32, n, nu=5, seed=3)).cuda()
+    y = torch.empty((32, n), dtype=torch.float64, device="cuda")
+    ld = torch.empty(n, dtype=torch.float64, device="cuda")
+    loss = torch.zeros(1, dtype=torch.float64, device="cuda")
+    grad = torch.zeros(g.numCoeffs, dtype=torch.float64, device="cuda")
+    for fn in (lambda: g.EvaluateImpl(x, y), lambda: g.LogDeterminantImpl(x, ld),
+               lambda: g.KLValueAndGradImpl(x, loss, grad)):
+        fn()
+        torch.cuda.synchronize()
+        for _ in range(args.steps):
+            fn()
+        torch.cuda.synchronize()
+    lanes, trials = g.EvalSchedule()
+    print(json.dumps({"profile": "uneven_map_split_kernels", "loss": float(loss.item()), "lanes": lanes,
+                      "fused_trial_ms": trials[2]}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -799,6 +847,7 @@ def main():
     ap.add_argument("--profile-grads", action="store_true")
     ap.add_argument("--profile-many", action="store_true")
     ap.add_argument("--profile-inverse-many", action="store_true")
+    ap.add_argument("--profile-uneven", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -824,6 +873,9 @@ def main():
         return
     if args.profile_inverse_many:
         profile_inverse_many_mode()
+        return
+    if args.profile_uneven:
+        profile_uneven_mode(args)
         return
     if args.profile_many:
         print(json.dumps({"profile": "many_maps_sequence", **many_maps(args.steps)}), flush=True)

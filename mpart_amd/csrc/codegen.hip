@@ -66,7 +66,14 @@ void emit_pin(std::ostringstream& o, const GenSpec& s, unsigned ndims, const cha
 }
 
 // The same with first derivatives db<d>_<k> (FillCache1 with Input flags).
+// With s.derivFromValues the derivatives are not held: term_deriv forms them
+// from the values (d phi_o / dx = M.dfac[o] phi_{o-1}), which keeps one
+// register pair per basis value instead of two.
 void emit_basis_derivs(std::ostringstream& o, const GenSpec& s, unsigned ndims) {
+    if (s.derivFromValues) {
+        emit_basis(o, s, ndims);
+        return;
+    }
     for (unsigned d = 0; d < ndims && d < s.dimDeg.size(); ++d) {
         const int deg = s.dimDeg[d];
         if (deg == 0) continue;
@@ -82,14 +89,20 @@ void emit_basis_derivs(std::ostringstream& o, const GenSpec& s, unsigned ndims) 
 
 void emit_pin_derivs(std::ostringstream& o, const GenSpec& s, unsigned ndims, const char* indent) {
     emit_pin(o, s, ndims, indent);
+    if (s.derivFromValues) return;
     for (unsigned d = 0; d < ndims && d < s.dimDeg.size(); ++d)
         for (int k = 1; k <= s.dimDeg[d]; ++k)
             o << indent << "asm volatile(\"\" : \"+v\"(db" << d << "_" << k << "));\n";
 }
 
 // d/dx_{dims[j]} of the off-diagonal product of term t (entry j of the term).
-std::string term_deriv(const GenComp& c, unsigned t, unsigned j) {
-    std::string expr = "db" + std::to_string(c.dims[j]) + "_" + std::to_string(c.orders[j]);
+std::string term_deriv(const GenSpec& s, const GenComp& c, unsigned t, unsigned j) {
+    const std::string d = std::to_string(c.dims[j]);
+    const unsigned k = c.orders[j];
+    std::string expr;
+    if (!s.derivFromValues) expr = "db" + d + "_" + std::to_string(k);
+    else if (k == 1) expr = "M.dfac[1]";
+    else expr = "(M.dfac[" + std::to_string(k) + "] * b" + d + "_" + std::to_string(k - 1) + ")";
     for (unsigned i = c.starts[t]; i < c.starts[t + 1]; ++i) {
         if (i == j || c.dims[i] == c.dim - 1) continue;
         expr = "(" + expr + " * b" + std::to_string(c.dims[i]) + "_" + std::to_string(c.orders[i]) + ")";
@@ -120,6 +133,7 @@ __device__ __forceinline__ void ivjp_body(const MapDev& M, const double* __restr
         long scs, const double* lsens, long lss, double lval, double scale, double* out, long ors, long ocs) {
     const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
+    const bool active = true;
     const double* pt = pts + n * pcs;
     const bool hasL = lsens != nullptr || lval != 0.0;
     const double lv = lsens ? lsens[n * lss] : lval;
@@ -128,7 +142,7 @@ __device__ __forceinline__ void ivjp_body(const MapDev& M, const double* __restr
     for (unsigned d = 0; d < s.D; ++d) o << "    double gx" << d << " = 0.0;\n";
     o << "#pragma unroll 1\n    for (int c = 0; c < " << ncomp << "; ++c) {\n";
     emit_pin_derivs(o, s, s.D - 1, "        ");
-    o << "        const CompDev cd = M.comps[c];\n        double C[GPM + 1];\n        switch (c) {\n";
+    o << "        const CompDev cd = ro(M.comps)[c];\n        double C[GPM + 1];\n        switch (c) {\n";
     for (unsigned c = 0; c < ncomp; ++c) {
         o << "        case " << c << ": {\n";
         emit_coeff_sums(o, s.comps[c]);
@@ -154,7 +168,7 @@ __device__ __forceinline__ void ivjp_body(const MapDev& M, const double* __restr
                 if (gc.dims[j] == gc.dim - 1) continue;
                 if (!any) o << "            { const double a = w[" << gc.coeffOff + t << "] * G[" << q << "];\n";
                 any = true;
-                o << "              gx" << gc.dims[j] << " += a * " << term_deriv(gc, t, j) << ";\n";
+                o << "              gx" << gc.dims[j] << " += a * " << term_deriv(s, gc, t, j) << ";\n";
             }
             if (any) o << "            }\n";
         }
@@ -225,7 +239,7 @@ std::string generate_source(const GenSpec& s, int part) {
         o << "#pragma unroll 1\n        for (int c = 0; c < " << ncomp << "; ++c) {\n";
         if (in) emit_pin_derivs(o, s, s.D - 1, "        ");
         else emit_pin(o, s, s.D - 1, "        ");
-        o << "        const CompDev cd = M.comps[c];\n        double C[GPM + 1];\n        switch (c) {\n";
+        o << "        const CompDev cd = ro(M.comps)[c];\n        double C[GPM + 1];\n        switch (c) {\n";
         for (unsigned c = 0; c < ncomp; ++c) {
             o << "        case " << c << ": {\n";
             emit_coeff_sums(o, s.comps[c]);
@@ -272,7 +286,7 @@ std::string generate_source(const GenSpec& s, int part) {
                         if (gc.dims[j] == gc.dim - 1) continue;
                         if (!any) o << "            { const double a = w[" << gc.coeffOff + t << "] * G[" << q << "];\n";
                         any = true;
-                        o << "              gx" << gc.dims[j] << " += a * " << term_deriv(gc, t, j) << ";\n";
+                        o << "              gx" << gc.dims[j] << " += a * " << term_deriv(s, gc, t, j) << ";\n";
                     }
                     if (any) o << "            }\n";
                 }
@@ -340,7 +354,7 @@ extern "C" __global__ void __launch_bounds__(256) mpart_gen_eval(MapDev M, const
     emit_basis(o, s, s.D - 1);
     o << "    double ldsum = 0.0;\n#pragma unroll 1\n    for (int c = cb; c < ce; ++c) {\n";
     emit_pin(o, s, s.D - 1, "        ");
-    o << "        const CompDev cd = M.comps[c];\n        double C[GPM + 1];\n        switch (c) {\n";
+    o << "        const CompDev cd = ro(M.comps)[c];\n        double C[GPM + 1];\n        switch (c) {\n";
     for (unsigned c = 0; c < ncomp; ++c) {
         o << "        case " << c << ": {\n";
         emit_coeff_sums(o, s.comps[c]);
@@ -387,7 +401,7 @@ extern "C" __global__ void __launch_bounds__(256) mpart_gen_ldcoef(MapDev M, con
     if (n >= N) return;
     const bool active = true;
     const double* pt = pts + n * pcs;
-    const CompDev cd = M.comps[c];
+    const CompDev cd = ro(M.comps)[c];
 )";
         // Only the coordinates of component c are needed; the basis of every
         // coordinate below the largest component is computed (cheap).

@@ -26,6 +26,9 @@ struct GenComp {
 struct GenSpec {
     int bk, pk, qk, pm;
     int lin = 0;  // LinearizedBasis bounds in use
+    // Input-gradient kernels form d phi_o / dx as M.dfac[o] * phi_{o-1} instead
+    // of holding a derivative per basis value (orthogonal polynomials, no bounds).
+    int derivFromValues = 0;
     unsigned D;
     unsigned numCoeffs;
     std::vector<int> dimDeg;  // largest off-diagonal order per coordinate

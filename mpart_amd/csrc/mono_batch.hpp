@@ -14,32 +14,48 @@
 
 namespace mpd {
 
+// Copies a read-only struct (size a multiple of 8 bytes) into LDS with the
+// whole block.  A tile's map and arguments are read from there: loads at a
+// uniform LDS address give uniform values, so the pointers in them feed
+// scalar loads of the term records.
+template <class T>
+__device__ __forceinline__ void block_copy(T& dst, const T* src) {
+    static_assert(sizeof(T) % 8 == 0, "copied as 8-byte words");
+    const uint64_t* s = reinterpret_cast<const uint64_t*>(src);
+    uint64_t* d = reinterpret_cast<uint64_t*>(&dst);
+    for (int i = threadIdx.x; i < (int)(sizeof(T) / 8); i += blockDim.x) d[i] = s[i];
+}
+
 template <int BK, int PK, int QK, int PM>
 __global__ void __launch_bounds__(256) k_batch(const MapDev* maps, const BatchIO* io, const int* tileMap,
                                                const long* tileStart, int ntiles, unsigned* counter, int method,
                                                double xtol, double ytol) {
     extern __shared__ double smem[];
     __shared__ int sTile;
+    __shared__ MapDev sMap;
+    __shared__ BatchIO sIO;
     const int tid = threadIdx.x;
     double* tab = smem + tid;
     const int stride = blockDim.x;
     while (true) {
         if (tid == 0) sTile = (int)atomicAdd(counter, 1u);
-        __syncthreads();
+        __syncthreads();  // the previous tile's reads of sMap / sIO are done
         const int t = sTile;
-        __syncthreads();
         if (t >= ntiles) break;
-        const int mi = tileMap[t];
-        const MapDev& M = maps[mi];
-        const BatchIO& B = io[mi];
-        const long n = tileStart[t] + tid;
+        const int mi = ro(tileMap)[t];
+        block_copy(sMap, maps + mi);
+        block_copy(sIO, io + mi);
+        __syncthreads();
+        const MapDev& M = sMap;
+        const BatchIO& B = sIO;
+        const long n = ro(tileStart)[t] + tid;
         if (n >= B.N) continue;
         if (method == BM_EVAL) {
             const double* pt = B.p + n * B.pcs;
             fill_values<BK>(M, M.D - 1, pt, B.prs, tab, stride);
             double ldsum = 0.0;
             for (int c = 0; c < M.ncomp; ++c) {
-                const CompDev cd = M.comps[c];
+                const CompDev cd = ro(M.comps)[c];
                 double C[PM + 1];
                 coeff_groups<PM>(M, cd, B.w + (long)cd.coeffOff * B.ws, B.ws, tab, stride, C);
                 const double xk = pt[(long)(cd.dim - 1) * B.prs];
@@ -61,7 +77,7 @@ __global__ void __launch_bounds__(256) k_batch(const MapDev* maps, const BatchIO
             const double* pt = B.p + n * B.pcs;
             fill_values<BK>(M, M.D - 1, pt, B.prs, tab, stride);
             for (int c = 0; c < M.ncomp; ++c) {
-                const CompDev cd = M.comps[c];
+                const CompDev cd = ro(M.comps)[c];
                 double C[PM + 1];
                 coeff_groups<PM>(M, cd, B.w + (long)cd.coeffOff * B.ws, B.ws, tab, stride, C);
                 DiagCoef<PM> dc;
@@ -101,7 +117,7 @@ __global__ void __launch_bounds__(256) k_batch(const MapDev* maps, const BatchIO
                 xf[i] = v;
             }
             for (int c = 0; c < M.ncomp; ++c) {
-                const CompDev cd = M.comps[c];
+                const CompDev cd = ro(M.comps)[c];
                 double x = NAN;
                 if (!bad) {
                     fill_values<BK>(M, cd.dim - 1, xf, 1, tab, stride);
@@ -133,20 +149,24 @@ __global__ void __launch_bounds__(256) k_batch_eval_mixed(const MapDev* maps, co
                                                           unsigned* counter, int grpSlots) {
     extern __shared__ double smem[];
     __shared__ int sTile;
+    __shared__ MapDev sMap;
+    __shared__ BatchIO sIO;
     const int tid = threadIdx.x;
     const int gl = tid & (G - 1);
     const int grp = tid / G;
     while (true) {
         if (tid == 0) sTile = (int)atomicAdd(counter, 1u);
-        __syncthreads();
+        __syncthreads();  // the previous tile's reads of sMap / sIO and its tables are done
         const int t = sTile;
-        __syncthreads();
         if (t >= ntiles) break;
-        const int mi = tileMap[t];
-        const MapDev& M = maps[mi];
-        const BatchIO& B = io[mi];
-        const bool group = tileLanes[t] > 1;  // uniform over the block
-        const long n = tileStart[t] + (group ? grp : tid);
+        const int mi = ro(tileMap)[t];
+        block_copy(sMap, maps + mi);
+        block_copy(sIO, io + mi);
+        __syncthreads();
+        const MapDev& M = sMap;
+        const BatchIO& B = sIO;
+        const bool group = ro(tileLanes)[t] > 1;  // uniform over the block
+        const long n = ro(tileStart)[t] + (group ? grp : tid);
         if (n >= B.N) continue;
         const double* pt = B.p + n * B.pcs;
         double* tab;
@@ -162,7 +182,7 @@ __global__ void __launch_bounds__(256) k_batch_eval_mixed(const MapDev* maps, co
         }
         double ldsum = 0.0;
         for (int c = 0; c < M.ncomp; ++c) {
-            const CompDev cd = M.comps[c];
+            const CompDev cd = ro(M.comps)[c];
             double C[PM + 1];
             if (group) group_coeff_groups<PM, G>(M, cd, B.w + (long)cd.coeffOff * B.ws, B.ws, tab, gl, C);
             else coeff_groups<PM>(M, cd, B.w + (long)cd.coeffOff * B.ws, B.ws, tab, stride, C);

@@ -31,7 +31,23 @@ typedef __hip_internal::uint32_t uint32_t;
 // per-point arithmetic on the CPU); the wavefront primitives stay device-only.
 #define MPD_HD __host__ __device__ __forceinline__
 
+// Read-only data of a launch (term records, groups, components, basis-slot
+// maps, coefficients) is read through the constant address space on the
+// device: a wavefront-uniform address then becomes a scalar load through the
+// scalar cache instead of one vector load per lane.  None of it changes while
+// a kernel runs.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define MPD_RO_AS __attribute__((address_space(4)))
+#else
+#define MPD_RO_AS
+#endif
+
 namespace mpd {
+
+template <class T>
+MPD_HD const MPD_RO_AS T* ro(const T* p) {
+    return (const MPD_RO_AS T*)p;
+}
 
 constexpr int MAXP = 12;        // largest last-coordinate degree held in registers
 constexpr int MAXND = 8;        // largest factor count with an unrolled term-record loop (more: run-time loop)
@@ -112,6 +128,7 @@ struct MapDev {
     double hfA[MAXORD + 1];    // Hermite-function recurrence: sqrt(0.5 (i-1))
     double hfB[MAXORD + 1];    //                              sqrt(0.5 i)
     double hfS[MAXORD + 1];    // (2^i i!)^(-1/2)
+    double dfac[MAXORD + 1];   // orthogonal polynomials: d phi_o / dx = dfac[o] phi_{o-1} (phi_0 = 1)
     double hfPi;               // pi^(-1/4)
     double hfSqrt2;            // sqrt(2)
     int discrete;              // contDeriv = false: log-determinants use the discrete derivative
@@ -537,9 +554,9 @@ template <int BK>
 __host__ __device__ void fill_values(const MapDev& M, int ndims, const double* pt, long prs, double* tab, int stride) {
     tab[0] = 1.0;
     for (int d = 0; d < ndims; ++d) {
-        const int deg = M.dimDeg[d];
+        const int deg = ro(M.dimDeg)[d];
         if (deg == 0) continue;
-        values_1d<BK>(M, deg, pt[d * prs], tab + (long)M.dimSlot[d] * stride - stride, stride);
+        values_1d<BK>(M, deg, pt[d * prs], tab + (long)ro(M.dimSlot)[d] * stride - stride, stride);
     }
 }
 
@@ -671,10 +688,10 @@ __host__ __device__ void fill_values_derivs(const MapDev& M, int ndims, const do
     tab[0] = 1.0;
     dtab[0] = 0.0;
     for (int d = 0; d < ndims; ++d) {
-        const int deg = M.dimDeg[d];
+        const int deg = ro(M.dimDeg)[d];
         if (deg == 0) continue;
-        values_derivs_1d<BK>(M, deg, pt[d * prs], tab + (long)M.dimSlot[d] * stride - stride,
-                             dtab + (long)M.dimSlot[d] * stride - stride, stride);
+        values_derivs_1d<BK>(M, deg, pt[d * prs], tab + (long)ro(M.dimSlot)[d] * stride - stride,
+                             dtab + (long)ro(M.dimSlot)[d] * stride - stride, stride);
     }
 }
 
@@ -698,8 +715,8 @@ MPD_HD long rec_len(const MapDev& M) {
     if constexpr (ND > 0) return ND + 1;
     else return M.trec;
 }
-template <int ND>
-MPD_HD double rec_product(const MapDev& M, const uint32_t* rec, const double* tab, long stride) {
+template <int ND, class RP>
+MPD_HD double rec_product(const MapDev& M, RP rec, const double* tab, long stride) {
     double P = tab[(long)rec[1] * stride];
     if constexpr (ND > 0) {
 #pragma unroll
@@ -717,11 +734,11 @@ MPD_HD void coeff_groups_nd(const MapDev& M, const CompDev& c, const double* w, 
 #pragma unroll
     for (int q = 0; q <= PM; ++q) {
         if (q > c.p) break;
-        const GroupDev g = M.groups[c.groupOff + q];
+        const GroupDev g = ro(M.groups)[c.groupOff + q];
         double acc = 0.0;
         for (int j = g.start; j < g.start + g.count; ++j) {
-            const uint32_t* rec = M.terms + (long)j * rec_len<ND>(M);
-            acc += w[(long)rec[0] * ws] * rec_product<ND>(M, rec, tab, stride);
+            const auto rec = ro(M.terms) + (long)j * rec_len<ND>(M);
+            acc += ro(w)[(long)rec[0] * ws] * rec_product<ND>(M, rec, tab, stride);
         }
         C[q] = acc;
     }
@@ -750,9 +767,9 @@ MPD_HD void visit_terms_nd(const MapDev& M, const CompDev& c, const double* tab,
 #pragma unroll
     for (int q = 0; q <= PM; ++q) {
         if (q > c.p) break;
-        const GroupDev g = M.groups[c.groupOff + q];
+        const GroupDev g = ro(M.groups)[c.groupOff + q];
         for (int j = g.start; j < g.start + g.count; ++j) {
-            const uint32_t* rec = M.terms + (long)j * rec_len<ND>(M);
+            const auto rec = ro(M.terms) + (long)j * rec_len<ND>(M);
             op(rec[0], rec_product<ND>(M, rec, tab, stride), q);
         }
     }

@@ -492,6 +492,17 @@ void build_plan(mpart_map_s& m) {
         M.norm[k] = (bk == mpd::BK_PHYS)
                         ? std::sqrt(std::sqrt(M_PI) * std::pow(2.0, static_cast<double>(k)) * factorial_u32(k))
                         : std::sqrt(std::sqrt(2.0 * M_PI) * factorial_u32(k));
+    // p_o' = A o p_{o-1} (OrthogonalPolynomial.h:87-118 in closed form): with
+    // phi_o = p_o / n_o, phi_o' = A o (n_{o-1} / n_o) phi_{o-1}, and phi_1' = A / n_1.
+    if (bk != mpd::BK_HF) {
+        const double A = (bk == mpd::BK_PHYS) ? 2.0 : 1.0;
+        M.dfac[0] = 0.0;
+        for (int k = 1; k <= mpd::MAXORD; ++k) {
+            const double raw = A * k;
+            if (M.normalize) M.dfac[k] = (k == 1) ? A / M.norm[1] : raw * M.norm[k - 1] / M.norm[k];
+            else M.dfac[k] = raw;
+        }
+    }
     M.discrete = o.contDeriv ? 0 : 1;
     M.lin = lin ? 1 : 0;
     M.lb = lin ? o.basisLB : -std::numeric_limits<double>::infinity();
@@ -781,6 +792,9 @@ mpg::GenSpec gen_spec(const mpart_map_s& m) {
     spec.pk = (m.variant / 10) % 10;
     spec.qk = m.variant % 10;
     spec.lin = m.dev.lin;
+    // Off-diagonal derivatives from the values (phi_o' = dfac[o] phi_{o-1}):
+    // orthogonal polynomials without LinearizedBasis bounds.
+    spec.derivFromValues = (!m.dev.lin && spec.bk != mpd::BK_HF) ? 1 : 0;
     spec.pm = std::max(1, m.dev.pmax);
     spec.D = m.inDim;
     spec.numCoeffs = m.numCoeffs;

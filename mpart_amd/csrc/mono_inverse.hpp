@@ -50,7 +50,7 @@ __device__ __forceinline__ void inv_setup_point(const MapDev& M, const double* w
         return;
     }
     const double x0 = pt[(xrows - 1) * xrs];
-    const CompDev cd = M.comps[c];
+    const CompDev cd = ro(M.comps)[c];
     fill_values<BK>(M, cd.dim - 1, pt, xrs, tab, (int)stride);
     double C[PM + 1];
     coeff_groups<PM>(M, cd, w + (long)cd.coeffOff * ws, ws, tab, (int)stride, C);
@@ -125,7 +125,7 @@ __global__ void __launch_bounds__(256) k_inverse_queue(MapDev M, const double* _
                                                        unsigned long long* counter, const InvSeg* __restrict__ segs,
                                                        const int* __restrict__ chunkSeg) {
     const QuadDev& Q = M.quad;
-    const int p = BATCH ? 0 : M.comps[c].p;
+    const int p = BATCH ? 0 : ro(M.comps)[c].p;
     // Clenshaw-Curtis node table of one interval, in the order integrate_acc
     // visits the nodes after the midpoint: position (cp or fp), coarse weight
     // index (-1: fine-only node) and fine weight index.

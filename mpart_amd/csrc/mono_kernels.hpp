@@ -33,7 +33,7 @@ __device__ __forceinline__ double* lane_table(double* smem_tab, double* gtab, lo
 // Largest coordinate whose table entries the components [cb, ce) read.
 __device__ __forceinline__ int table_dims(const MapDev& M, int cb, int ce) {
     int nd = 0;
-    for (int c = cb; c < ce; ++c) nd = max(nd, M.comps[c].dim - 1);
+    for (int c = cb; c < ce; ++c) nd = max(nd, ro(M.comps)[c].dim - 1);
     return nd;
 }
 
@@ -55,7 +55,7 @@ __global__ void __launch_bounds__(256) k_eval(MapDev M, const double* w, long ws
     fill_values<BK>(M, table_dims(M, cb, ce), pt, P.rs, tab, (int)stride);
     double ldsum = 0.0;
     for (int c = cb; c < ce; ++c) {
-        const CompDev cd = M.comps[c];
+        const CompDev cd = ro(M.comps)[c];
         double C[PM + 1];
         coeff_groups<PM>(M, cd, w + (long)cd.coeffOff * ws, ws, tab, (int)stride, C);
         const double xk = pt[(long)(cd.dim - 1) * P.rs];
@@ -109,8 +109,8 @@ __device__ __forceinline__ void group_fill_values(const MapDev& M, int ndims, co
                                                   int gl) {
     if (gl == 0) tab[0] = 1.0;
     for (int d = gl; d < ndims; d += G) {
-        const int deg = M.dimDeg[d];
-        if (deg > 0) values_1d<BK>(M, deg, pt[d * prs], tab + M.dimSlot[d] - 1, 1);
+        const int deg = ro(M.dimDeg)[d];
+        if (deg > 0) values_1d<BK>(M, deg, pt[d * prs], tab + ro(M.dimSlot)[d] - 1, 1);
     }
     wave_lds_sync();
 }
@@ -121,11 +121,11 @@ __device__ __forceinline__ void group_coeff_groups_nd(const MapDev& M, const Com
 #pragma unroll
     for (int q = 0; q <= PM; ++q) {
         if (q > c.p) break;
-        const GroupDev g = M.groups[c.groupOff + q];
+        const GroupDev g = ro(M.groups)[c.groupOff + q];
         double acc = 0.0;
         for (int j = g.start + gl; j < g.start + g.count; j += G) {
-            const uint32_t* rec = M.terms + (long)j * rec_len<ND>(M);
-            acc += w[(long)rec[0] * ws] * rec_product<ND>(M, rec, tab, 1);
+            const auto rec = ro(M.terms) + (long)j * rec_len<ND>(M);
+            acc += ro(w)[(long)rec[0] * ws] * rec_product<ND>(M, rec, tab, 1);
         }
         C[q] = group_sum<G>(acc);
     }
@@ -160,7 +160,7 @@ __global__ void __launch_bounds__(256) k_eval_grp(MapDev M, const double* w, lon
     group_fill_values<BK, G>(M, table_dims(M, cb, ce), pt, P.rs, tab, gl);
     double ldsum = 0.0;
     for (int c = cb; c < ce; ++c) {
-        const CompDev cd = M.comps[c];
+        const CompDev cd = ro(M.comps)[c];
         double C[PM + 1];
         group_coeff_groups<PM, G>(M, cd, w + (long)cd.coeffOff * ws, ws, tab, gl, C);
         const double xk = pt[(long)(cd.dim - 1) * P.rs];
@@ -222,7 +222,7 @@ __global__ void __launch_bounds__(256) k_fused_grp(MapDev M, const double* w, lo
         wave_lds_sync();
         group_fill_values<BK, G>(M, ndims, pt, P.rs, tab, gl);
         for (int c = 0; c < M.ncomp; ++c) {
-            const CompDev cd = M.comps[c];
+            const CompDev cd = ro(M.comps)[c];
             const double* wc = w + (long)cd.coeffOff * ws;
             double C[PM + 1];
             group_coeff_groups<PM, G>(M, cd, wc, ws, tab, gl, C);
@@ -236,7 +236,7 @@ __global__ void __launch_bounds__(256) k_fused_grp(MapDev M, const double* w, lo
 #pragma unroll
             for (int q = 0; q <= PM; ++q) {
                 if (q > cd.p) break;
-                const GroupDev g = M.groups[cd.groupOff + q];
+                const GroupDev g = ro(M.groups)[cd.groupOff + q];
                 const int end = g.start + g.count;
                 // four chunks of G terms per step, so their loads overlap
                 constexpr int U = 4;
@@ -249,7 +249,7 @@ __global__ void __launch_bounds__(256) k_fused_grp(MapDev M, const double* w, lo
                         v[u] = 0.0;
                         idx[u] = 0;
                         if (j < end) {
-                            const uint32_t* rec = M.terms + (long)j * M.trec;
+                            const auto rec = ro(M.terms) + (long)j * M.trec;
                             double Pj = tab[rec[1]];
                             for (int e = 2; e <= M.nd; ++e) Pj *= tab[rec[e]];
                             v[u] = Pj * Gq[q];
@@ -288,7 +288,7 @@ __global__ void __launch_bounds__(256) k_coeff_grad(MapDev M, const double* w, l
     long stride;
     double* tab = lane_table(smem, gtab, stride);
     const double* pt = P.p + n * P.cs;
-    const CompDev cd = M.comps[c];
+    const CompDev cd = ro(M.comps)[c];
     fill_values<BK>(M, cd.dim - 1, pt, P.rs, tab, (int)stride);
     double C[PM + 1];
     coeff_groups<PM>(M, cd, w + (long)cd.coeffOff * ws, ws, tab, (int)stride, C);
@@ -324,7 +324,7 @@ __global__ void __launch_bounds__(256) k_logdet_coeff_grad(MapDev M, const doubl
     long stride;
     double* tab = lane_table(smem, gtab, stride);
     const double* pt = P.p + n * P.cs;
-    const CompDev cd = M.comps[c];
+    const CompDev cd = ro(M.comps)[c];
     fill_values<BK>(M, cd.dim - 1, pt, P.rs, tab, (int)stride);
     double C[PM + 1];
     coeff_groups<PM>(M, cd, w + (long)cd.coeffOff * ws, ws, tab, (int)stride, C);
@@ -367,7 +367,7 @@ __global__ void __launch_bounds__(256) k_logdet_discrete(MapDev M, const double*
     fill_values<BK>(M, table_dims(M, cb, ce), pt, P.rs, tab, (int)stride);
     double ldsum = 0.0;
     for (int c = cb; c < ce; ++c) {
-        const CompDev cd = M.comps[c];
+        const CompDev cd = ro(M.comps)[c];
         double C[PM + 1];
         coeff_groups<PM>(M, cd, w + (long)cd.coeffOff * ws, ws, tab, (int)stride, C);
         DiagCoef<PM> dc;
@@ -393,7 +393,7 @@ __global__ void __launch_bounds__(256) k_logdet_coeff_grad_discrete(MapDev M, co
     long stride;
     double* tab = lane_table(smem, gtab, stride);
     const double* pt = P.p + n * P.cs;
-    const CompDev cd = M.comps[c];
+    const CompDev cd = ro(M.comps)[c];
     fill_values<BK>(M, cd.dim - 1, pt, P.rs, tab, (int)stride);
     double C[PM + 1];
     coeff_groups<PM>(M, cd, w + (long)cd.coeffOff * ws, ws, tab, (int)stride, C);
@@ -427,7 +427,7 @@ __global__ void __launch_bounds__(256) k_logdet_coeff_grad_discrete(MapDev M, co
 // (has = true), the value table otherwise (slot 0 is 1.0).
 __device__ __forceinline__ void input_factor(const MapDev& M, uint32_t s, const double* tab, const double* dtab,
                                              int stride, int wrt, bool& has, double& Pj) {
-    if (s != 0 && M.slotDim[s] == wrt) {
+    if (s != 0 && ro(M.slotDim)[s] == wrt) {
         has = true;
         Pj *= dtab[(long)s * stride];
     } else {
@@ -442,10 +442,10 @@ __device__ __forceinline__ void input_groups_nd(const MapDev& M, const CompDev& 
 #pragma unroll
     for (int q = 0; q <= PM; ++q) {
         if (q > c.p) break;
-        const GroupDev g = M.groups[c.groupOff + q];
+        const GroupDev g = ro(M.groups)[c.groupOff + q];
         double acc = 0.0;
         for (int j = g.start; j < g.start + g.count; ++j) {
-            const uint32_t* rec = M.terms + (long)j * rec_len<ND>(M);
+            const auto rec = ro(M.terms) + (long)j * rec_len<ND>(M);
             bool has = false;
             double Pj = 1.0;
             if constexpr (ND > 0) {
@@ -454,7 +454,7 @@ __device__ __forceinline__ void input_groups_nd(const MapDev& M, const CompDev& 
             } else {
                 for (int e = 1; e <= M.nd; ++e) input_factor(M, rec[e], tab, dtab, stride, wrt, has, Pj);
             }
-            if (has) acc += w[(long)rec[0] * ws] * Pj;
+            if (has) acc += ro(w)[(long)rec[0] * ws] * Pj;
         }
         Dq[q] = acc;
     }
@@ -493,7 +493,7 @@ __global__ void __launch_bounds__(256) k_input_grad(MapDev M, const double* w, l
     }
     double* dtab = tab + (long)M.nslots * stride;
     const double* pt = P.p + n * P.cs;
-    const CompDev cd = M.comps[c];
+    const CompDev cd = ro(M.comps)[c];
     const int k = cd.dim - 1;
     fill_values_derivs<BK>(M, k, pt, P.rs, tab, dtab, (int)stride);
     const double* wc = w + (long)cd.coeffOff * ws;
@@ -578,7 +578,7 @@ __global__ void __launch_bounds__(256) k_input_vjp(MapDev M, const double* w, lo
     const bool hasL = lsens != nullptr || lval != 0.0;
     const double lv = lsens ? lsens[n * lss] : lval;
     for (int c = 0; c < M.ncomp; ++c) {
-        const CompDev cd = M.comps[c];
+        const CompDev cd = ro(M.comps)[c];
         const int k = cd.dim - 1;
         const double* wc = w + (long)cd.coeffOff * ws;
         double C[PM + 1];
@@ -717,6 +717,50 @@ __device__ double inverse_solve(const MapDev& M, const DiagCoef<PM>& dc, double 
 // grid) and write one partial gradient row per block.
 //
 // LDS layout: [acc: (blockDim/64) * numCoeffs] [table: nslots * blockDim].
+// Compile-time factor-slot count for the term-record loops (ND as in
+// rec_product; 0: more than eight factors, a run-time loop).
+template <int N>
+struct NdTag {
+    static constexpr int value = N;
+};
+template <class F>
+__device__ __forceinline__ void dispatch_nd(int nd, F&& f) {
+    switch (nd) {
+        case 1: f(NdTag<1>{}); break;
+        case 2: f(NdTag<2>{}); break;
+        case 3: f(NdTag<3>{}); break;
+        case 4: f(NdTag<4>{}); break;
+        case 5: f(NdTag<5>{}); break;
+        case 6: f(NdTag<6>{}); break;
+        case 7: f(NdTag<7>{}); break;
+        case 8: f(NdTag<8>{}); break;
+        default: f(NdTag<0>{}); break;
+    }
+}
+
+// G[q] for a run-time q (a select chain over the register array).
+template <int PM>
+__device__ __forceinline__ double select_q(const double* G, int q) {
+    double g = G[0];
+#pragma unroll
+    for (int i = 1; i <= PM; ++i) g = (i == q) ? G[i] : g;
+    return g;
+}
+
+// Products G_q P_j of the 16 term records j0 .. j0+15, zero on an inactive
+// lane.  The records are the same for every lane of the wavefront, so with
+// ND known they come through scalar loads, several in flight at once.
+template <int ND>
+__device__ __forceinline__ void chunk16_products(const MapDev& M, int j0, const double* tab, long stride, double Gq,
+                                                 bool active, double* v) {
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        const auto rec = ro(M.terms) + (long)(j0 + u) * rec_len<ND>(M);
+        const double Pj = rec_product<ND>(M, rec, tab, stride);
+        v[u] = active ? Pj * Gq : 0.0;
+    }
+}
+
 template <int BK, int PK, int QK, int PM>
 __global__ void __launch_bounds__(256) k_fused(MapDev M, const double* w, long ws, PtsArg P, const double* sens,
                                                long srs, long scs, const double* lsens, long lss, int klMode,
@@ -755,7 +799,7 @@ __global__ void __launch_bounds__(256) k_fused(MapDev M, const double* w, long w
         const double* pt = P.p + n * P.cs;
         fill_values<BK>(M, ndims, pt, P.rs, tab, (int)stride);
         for (int c = 0; c < M.ncomp; ++c) {
-            const CompDev cd = M.comps[c];
+            const CompDev cd = ro(M.comps)[c];
             const double* wc = w + (long)cd.coeffOff * ws;
             double C[PM + 1];
             coeff_groups<PM>(M, cd, wc, ws, tab, (int)stride, C);
@@ -766,30 +810,25 @@ __global__ void __launch_bounds__(256) k_fused(MapDev M, const double* w, long w
             if (M.discrete) comp_core_discrete<BK, PK, QK, PM>(M, cd, C, xk, active, klMode, sv, lv, G, lossLane);
             else comp_core<BK, PK, QK, PM>(M, cd, C, xk, active, klMode, sv, lv, G, lossLane);
             // Gradient pass: chunks of 16 term records per group.
-#pragma unroll
-            for (int q = 0; q <= PM; ++q) {
-                if (q > cd.p) break;
-                const GroupDev g = M.groups[cd.groupOff + q];
-                const double Gq = G[q];
-                for (int j0 = g.start; j0 < g.start + g.countPad; j0 += 16) {
-                    double v[16];
-#pragma unroll
-                    for (int u = 0; u < 16; ++u) {
-                        const uint32_t* rec = M.terms + (long)(j0 + u) * M.trec;
-                        double Pj = tab[(long)rec[1] * stride];
-                        for (int e = 2; e <= M.nd; ++e) Pj *= tab[(long)rec[e] * stride];
-                        v[u] = active ? Pj * Gq : 0.0;
-                    }
-                    const double rsum = wave_reduce16(v, lane);
-                    if ((lane & 3) == 0) {
-                        const uint32_t idx = M.terms[(long)(j0 + (lane >> 2)) * M.trec];
-                        if (idx != DUMMY) {
-                            if (ldsAcc) myAcc[cd.coeffOff + idx] += rsum;
-                            else atomicAdd(gradGlobal + cd.coeffOff + idx, rsum);
+            dispatch_nd(M.nd, [&](auto tag) {
+                constexpr int ND = decltype(tag)::value;
+                for (int q = 0; q <= cd.p; ++q) {
+                    const GroupDev g = ro(M.groups)[cd.groupOff + q];
+                    const double Gq = select_q<PM>(G, q);
+                    for (int j0 = g.start; j0 < g.start + g.countPad; j0 += 16) {
+                        double v[16];
+                        chunk16_products<ND>(M, j0, tab, stride, Gq, active, v);
+                        const double rsum = wave_reduce16(v, lane);
+                        if ((lane & 3) == 0) {
+                            const uint32_t idx = ro(M.terms)[(long)(j0 + (lane >> 2)) * rec_len<ND>(M)];
+                            if (idx != DUMMY) {
+                                if (ldsAcc) myAcc[cd.coeffOff + idx] += rsum;
+                                else atomicAdd(gradGlobal + cd.coeffOff + idx, rsum);
+                            }
                         }
                     }
                 }
-            }
+            });
         }
     }
     // Loss: wave sum, then block sum.

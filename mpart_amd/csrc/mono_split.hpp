@@ -32,14 +32,14 @@ __global__ void __launch_bounds__(256, MINB) k_fused_split(MapDev M, const doubl
     extern __shared__ double tab[];
     __shared__ double lossWave[SPLIT_WAVES];
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: scalar loads below
     const int K = M.numCoeffs;
     double* row = rows + (long)blockIdx.x * (K + 1);
     for (int j = threadIdx.x; j <= K; j += blockDim.x) row[j] = 0.0;
     __threadfence();  // the zeros are in memory before any wavefront adds to the row
     __syncthreads();
     const int ndims = M.D - 1;
-    const int c0 = waveStart[wave], c1 = waveStart[wave + 1];
+    const int c0 = ro(waveStart)[wave], c1 = ro(waveStart)[wave + 1];
     double lossLane = 0.0;
     const long ntiles = (P.N + SPLIT_TILE - 1) / SPLIT_TILE;
     for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
@@ -50,13 +50,13 @@ __global__ void __launch_bounds__(256, MINB) k_fused_split(MapDev M, const doubl
         __syncthreads();  // the previous tile's table is no longer read
         if (wave == 0) tab[lane] = 1.0;
         for (int d = wave; d < ndims; d += SPLIT_WAVES) {
-            const int deg = M.dimDeg[d];
-            if (deg > 0) values_1d<BK>(M, deg, pt[d * P.rs], tab + (long)(M.dimSlot[d] - 1) * SPLIT_TILE + lane, SPLIT_TILE);
+            const int deg = ro(M.dimDeg)[d];
+            if (deg > 0) values_1d<BK>(M, deg, pt[d * P.rs], tab + (long)(ro(M.dimSlot)[d] - 1) * SPLIT_TILE + lane, SPLIT_TILE);
         }
         __syncthreads();
         const double* mytab = tab + lane;
         for (int i = c0; i < c1; ++i) {
-            const CompDev cd = M.comps[compOrder[i]];
+            const CompDev cd = ro(M.comps)[ro(compOrder)[i]];
             const double* wc = w + (long)cd.coeffOff * ws;
             double C[PM + 1];
             coeff_groups<PM>(M, cd, wc, ws, mytab, SPLIT_TILE, C);
@@ -68,27 +68,22 @@ __global__ void __launch_bounds__(256, MINB) k_fused_split(MapDev M, const doubl
             else comp_core<BK, PK, QK, PM>(M, cd, C, xk, active, klMode, sv, lv, G, lossLane);
             // Gradient pass: chunks of 16 term records per group, each chunk
             // reduced over the tile's points by the wavefront.
-#pragma unroll
-            for (int q = 0; q <= PM; ++q) {
-                if (q > cd.p) break;
-                const GroupDev g = M.groups[cd.groupOff + q];
-                const double Gq = G[q];
-                for (int j0 = g.start; j0 < g.start + g.countPad; j0 += 16) {
-                    double v[16];
-#pragma unroll
-                    for (int u = 0; u < 16; ++u) {
-                        const uint32_t* rec = M.terms + (long)(j0 + u) * M.trec;
-                        double Pj = mytab[(long)rec[1] * SPLIT_TILE];
-                        for (int e = 2; e <= M.nd; ++e) Pj *= mytab[(long)rec[e] * SPLIT_TILE];
-                        v[u] = active ? Pj * Gq : 0.0;
-                    }
-                    const double rsum = wave_reduce16(v, lane);
-                    if ((lane & 3) == 0) {
-                        const uint32_t idx = M.terms[(long)(j0 + (lane >> 2)) * M.trec];
-                        if (idx != DUMMY) (void)unsafeAtomicAdd(row + cd.coeffOff + idx, rsum);
+            dispatch_nd(M.nd, [&](auto tag) {
+                constexpr int ND = decltype(tag)::value;
+                for (int q = 0; q <= cd.p; ++q) {
+                    const GroupDev g = ro(M.groups)[cd.groupOff + q];
+                    const double Gq = select_q<PM>(G, q);
+                    for (int j0 = g.start; j0 < g.start + g.countPad; j0 += 16) {
+                        double v[16];
+                        chunk16_products<ND>(M, j0, mytab, SPLIT_TILE, Gq, active, v);
+                        const double rsum = wave_reduce16(v, lane);
+                        if ((lane & 3) == 0) {
+                            const uint32_t idx = ro(M.terms)[(long)(j0 + (lane >> 2)) * rec_len<ND>(M)];
+                            if (idx != DUMMY) (void)unsafeAtomicAdd(row + cd.coeffOff + idx, rsum);
+                        }
                     }
                 }
-            }
+            });
         }
     }
     const double l = wave_sum(lossLane);
@@ -120,9 +115,9 @@ __global__ void __launch_bounds__(256) k_eval_split(MapDev M, const double* w, l
                                                     const int* compOrder, const int* waveStart) {
     extern __shared__ double tab[];  // nslots >= SPLIT_WAVES (the host checks)
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: scalar loads below
     const int ndims = M.D - 1;
-    const int c0 = waveStart[wave], c1 = waveStart[wave + 1];
+    const int c0 = ro(waveStart)[wave], c1 = ro(waveStart)[wave + 1];
     const long ntiles = (P.N + SPLIT_TILE - 1) / SPLIT_TILE;
     for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const long n0 = tile * SPLIT_TILE + lane;
@@ -132,15 +127,15 @@ __global__ void __launch_bounds__(256) k_eval_split(MapDev M, const double* w, l
         __syncthreads();  // the previous tile's table and log-det terms are no longer read
         if (wave == 0) tab[lane] = 1.0;
         for (int d = wave; d < ndims; d += SPLIT_WAVES) {
-            const int deg = M.dimDeg[d];
-            if (deg > 0) values_1d<BK>(M, deg, pt[d * P.rs], tab + (long)(M.dimSlot[d] - 1) * SPLIT_TILE + lane, SPLIT_TILE);
+            const int deg = ro(M.dimDeg)[d];
+            if (deg > 0) values_1d<BK>(M, deg, pt[d * P.rs], tab + (long)(ro(M.dimSlot)[d] - 1) * SPLIT_TILE + lane, SPLIT_TILE);
         }
         __syncthreads();
         const double* mytab = tab + lane;
         double ldsum = 0.0;
         for (int i = c0; i < c1; ++i) {
-            const int c = compOrder[i];
-            const CompDev cd = M.comps[c];
+            const int c = ro(compOrder)[i];
+            const CompDev cd = ro(M.comps)[c];
             double C[PM + 1];
             coeff_groups<PM>(M, cd, w + (long)cd.coeffOff * ws, ws, mytab, SPLIT_TILE, C);
             const double xk = pt[(long)(cd.dim - 1) * P.rs];
