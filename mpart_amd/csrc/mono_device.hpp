@@ -53,7 +53,7 @@ constexpr int MAXP = 12;        // largest last-coordinate degree held in regist
 constexpr int MAXND = 8;        // largest factor count with an unrolled term-record loop (more: run-time loop)
 constexpr int MAXSUB = 32;      // largest adaptive depth (quadMaxSub)
 constexpr int MAXNF = 17;       // largest nested Clenshaw-Curtis fine rule
-constexpr int MAXORD = 16;      // largest off-diagonal degree in the basis table
+constexpr int MAXORD = 32;      // largest off-diagonal degree in the basis table
 constexpr uint32_t DUMMY = 0xFFFFFFFFu;
 
 enum BasisKind { BK_PH = 0, BK_HF = 1, BK_PHYS = 2 };
@@ -88,6 +88,10 @@ struct QuadDev {
     int maxSub, minSub;
     int nc, nf;  // coarse / fine rule sizes (2^L+1, 2^(L+1)+1)
     double cw[MAXNF], cp[MAXNF], fw[MAXNF], fp[MAXNF];
+    // Rules of more than MAXNF points live in device memory instead: cw, cp
+    // (nc each; nc = 0 for the fixed rule), then fw, fp (nf each).  NULL when
+    // the arrays above hold the rule.
+    const double* big;
 };
 
 // One monotone component of a map.
@@ -1046,10 +1050,102 @@ __host__ __device__ double integrate_as(const QuadDev& Q, const DiagCoef<PM>& dc
 // AdaptiveClenshawCurtis::Integrate (Quadrature.h:819-968).  NF is the fine
 // rule size when known at compile time (5 for quadPts = 5), or MAXNF with the
 // runtime size in Q.nf.
+// The same for a rule of more than MAXNF points (Q.big): run-time loops over
+// the rule in device memory, the same nodes, sums and order.  The interior
+// node values are not kept for the adjoint; an accepted interval evaluates
+// them again (the same positions, so the same values; not counted in nevals).
+template <int BK, int PK, int FL, int PM>
+__host__ __device__ double integrate_acc_big(const QuadDev& Q, const DiagCoef<PM>& dc, AdjAcc<PM>& A, int* nevals,
+                                             const NodeV<FL>* at1) {
+#pragma clang fp contract(off)  // as in integrate_as
+    AccEnt<FL> stk[MAXSUB];
+    AccEnt<FL> top;
+    int sp = 0;
+    bool inReg = false, inMem = false;
+    const int nf = Q.nf, nc = Q.nc;
+    const auto cw = ro(Q.big);
+    const auto cp = cw + nc;
+    const auto fw = cp + nc;
+    const auto fp = fw + nf;
+    const int cRight = nc - 1, cMid = cRight / 2;
+    const int fRight = nf - 1, fMid = fRight / 2;
+    NodeV<FL> fa = eval_node<BK, PK, FL, PM>(dc, 0.0);
+    NodeV<FL> fb = at1 ? *at1 : eval_node<BK, PK, FL, PM>(dc, 1.0);
+    int ne = 2;
+    double a = 0.0, b = 1.0, res = 0.0;
+    int level = 0;
+    while (true) {
+        const double mid = 0.5 * (a + b);
+        const double scale = 0.5 * (b - a);
+        NodeV<FL> fm = eval_node<BK, PK, FL, PM>(dc, mid);
+        ne += 1;
+        double coarse = scale * cw[0] * fa.F;
+        coarse += scale * cw[cMid] * fm.F;
+        coarse += scale * cw[cRight] * fb.F;
+        double fine = scale * fw[0] * fa.F;
+        fine += scale * fw[fMid] * fm.F;
+        fine += scale * fw[fRight] * fb.F;
+        for (int i = 1; i < cMid; ++i) {
+            const NodeV<FL> v = eval_node<BK, PK, FL, PM>(dc, mid + scale * cp[i]);
+            ne += 1;
+            coarse += scale * cw[i] * v.F;
+            fine += scale * fw[2 * i] * v.F;
+        }
+        for (int ii = cMid + 1; ii < cRight; ++ii) {
+            const NodeV<FL> v = eval_node<BK, PK, FL, PM>(dc, mid + scale * cp[ii]);
+            ne += 1;
+            coarse += scale * cw[ii] * v.F;
+            fine += scale * fw[2 * ii] * v.F;
+        }
+        for (int i = 1; i < nf; i += 2) {
+            const NodeV<FL> v = eval_node<BK, PK, FL, PM>(dc, mid + scale * fp[i]);
+            ne += 1;
+            fine += scale * fw[i] * v.F;
+        }
+        const double err = fabs(fine - coarse);
+        if (nonfinite_err(err)) {  // as in integrate_as
+            res += fine;
+            if constexpr (FL >= FL_COEF) nan_adj<PM>(A);
+            break;
+        }
+        const double tol = fmax(Q.relTol * fabs(coarse), Q.absTol);
+        if (((err < tol) || (level == Q.maxSub - 1)) && (level >= Q.minSub)) {
+            res += fine;
+            if constexpr (FL >= FL_COEF) {
+                adj_node<BK, FL, PM>(dc, a, scale * fw[0], fa, A);
+                adj_node<BK, FL, PM>(dc, mid, scale * fw[fMid], fm, A);
+                adj_node<BK, FL, PM>(dc, b, scale * fw[fRight], fb, A);
+                for (int i = 1; i < nf - 1; ++i) {
+                    if (i == fMid) continue;
+                    const double t = mid + scale * fp[i];
+                    adj_node<BK, FL, PM>(dc, t, scale * fw[i], eval_node<BK, PK, FL, PM>(dc, t), A);
+                }
+            }
+            if (sp == 0) break;
+            const AccEnt<FL> e = qpop(stk, top, sp, inReg, inMem);
+            a = b;
+            fa = fb;
+            level = e.level;
+            b = a + ldexp(1.0, -level);
+            fb = e.fb;
+        } else {
+            qpush(stk, top, sp, inReg, inMem, AccEnt<FL>{fb, level + 1});
+            b = mid;
+            fb = fm;
+            ++level;
+        }
+    }
+    if (nevals) *nevals += ne;
+    return res;
+}
+
 template <int BK, int PK, int FL, int NF, int PM>
 __host__ __device__ double integrate_acc(const QuadDev& Q, const DiagCoef<PM>& dc, AdjAcc<PM>& A, int* nevals,
                                          const NodeV<FL>* at1) {
 #pragma clang fp contract(off)  // as in integrate_as
+    if constexpr (NF == MAXNF) {
+        if (Q.big) return integrate_acc_big<BK, PK, FL, PM>(Q, dc, A, nevals, at1);
+    }
     AccEnt<FL> stk[MAXSUB];
     AccEnt<FL> top;
     int sp = 0;
@@ -1150,13 +1246,22 @@ MPD_HD double integrate_cc(const QuadDev& Q, const DiagCoef<PM>& dc, AdjAcc<PM>&
 #pragma clang fp contract(off)  // as in integrate_as
     const int n = Q.nf;
     double res = 0.0;
-    for (int i = 0; i < MAXNF; ++i) {
-        if (i >= n) break;
-        const double t = 0.5 * (1.0 + 0.0 + (1.0 - 0.0) * Q.fp[i]);
-        const NodeV<FL> v = (at1 && Q.fp[i] == 1.0) ? *at1 : eval_node<BK, PK, FL, PM>(dc, t);
-        const double w = 0.5 * (1.0 - 0.0) * Q.fw[i];
+    auto node = [&](double fpi, double fwi) {
+#pragma clang fp contract(off)
+        const double t = 0.5 * (1.0 + 0.0 + (1.0 - 0.0) * fpi);
+        const NodeV<FL> v = (at1 && fpi == 1.0) ? *at1 : eval_node<BK, PK, FL, PM>(dc, t);
+        const double w = 0.5 * (1.0 - 0.0) * fwi;
         res += w * v.F;
         if constexpr (FL >= FL_COEF) adj_node<BK, FL, PM>(dc, t, w, v, A);
+    };
+    if (Q.big) {  // more than MAXNF points: the rule in device memory
+        const auto fw = ro(Q.big) + 2 * Q.nc;
+        for (int i = 0; i < n; ++i) node(fw[n + i], fw[i]);
+    } else {
+        for (int i = 0; i < MAXNF; ++i) {
+            if (i >= n) break;
+            node(Q.fp[i], Q.fw[i]);
+        }
     }
     if (nevals) *nevals += n;
     return res;

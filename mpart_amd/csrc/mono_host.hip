@@ -349,6 +349,7 @@ struct mpart_map_s {
     MapDev dev{};
     DBuf dComps, dGroups, dTerms, dSlotDim, dDimSlot, dDimDeg;
     DBuf dStatus;  // MapDev::status
+    DBuf dRule;    // quadrature rule of more than MAXNF points (MapDev::quad.big)
     // Component-split fused schedule: [waveStart[0..4] | component order]
     DBuf dSplit;
     int splitOcc[2] = {0, 0};   // resident blocks per CU of the two builds
@@ -460,13 +461,20 @@ void build_plan(mpart_map_s& m) {
         for (unsigned i = 1; i < (n - 1); ++i) w[i] = 2.0 * w[i] / (n - 1.0);
         w[n - 1] = w[n - 1] / (n - 1.0);
     };
+    // Rules of more than MAXNF points go to device memory (QuadDev::big):
+    // cw, cp (nc each), then fw, fp (nf each).
+    std::vector<double> bigRule;
     if (o.quadType == MPART_QUAD_CLENSHAW_CURTIS) {
         // ClenshawCurtisQuadrature(quadPts) (src/MapFactoryImpl5.cpp:17)
         if (o.quadPts < 1) throw ArgError("ClenshawCurtis needs quadPts >= 1");
-        if (o.quadPts > (unsigned)mpd::MAXNF)
-            throw NotImpl("ClenshawCurtis with quadPts > " + std::to_string(mpd::MAXNF) + " is not provided");
-        rule(o.quadPts, M.quad.fw, M.quad.fp);
-        M.quad.nf = (int)o.quadPts;
+        const unsigned n = o.quadPts;
+        if (n > (unsigned)mpd::MAXNF) {
+            bigRule.resize(2 * (size_t)n);
+            rule(n, bigRule.data(), bigRule.data() + n);
+        } else {
+            rule(n, M.quad.fw, M.quad.fp);
+        }
+        M.quad.nf = (int)n;
         qk = mpd::QK_CC;
     }
     if (o.quadType == MPART_QUAD_ADAPTIVE_CLENSHAW_CURTIS) {
@@ -474,10 +482,17 @@ void build_plan(mpart_map_s& m) {
         // level = floor(log2(quadPts - 2)) (src/MapFactoryImpl4.cpp:18)
         const unsigned level = (unsigned)std::log2(o.quadPts - 2);
         if (level < 1) throw ArgError("AdaptiveClenshawCurtis needs a nested level >= 1 (quadPts >= 4)");
+        if (level > 28) throw ArgError("AdaptiveClenshawCurtis: quadPts too large for a nested rule");
         const unsigned nc = (unsigned)std::pow(2, level) + 1, nf = (unsigned)std::pow(2, level + 1) + 1;
-        if (nf > (unsigned)mpd::MAXNF) throw NotImpl("AdaptiveClenshawCurtis with quadPts > 17 is not provided");
-        rule(nc, M.quad.cw, M.quad.cp);
-        rule(nf, M.quad.fw, M.quad.fp);
+        if (nf > (unsigned)mpd::MAXNF) {
+            bigRule.resize(2 * (size_t)nc + 2 * (size_t)nf);
+            double* b = bigRule.data();
+            rule(nc, b, b + nc);
+            rule(nf, b + 2 * (size_t)nc, b + 2 * (size_t)nc + nf);
+        } else {
+            rule(nc, M.quad.cw, M.quad.cp);
+            rule(nf, M.quad.fw, M.quad.fp);
+        }
         M.quad.nc = (int)nc;
         M.quad.nf = (int)nf;
         qk = (nf == 5) ? mpd::QK_ACC5 : mpd::QK_ACC;
@@ -564,6 +579,12 @@ void build_plan(mpart_map_s& m) {
     m.dStatus.ensure(sizeof(int));
     hip_check(hipMemset(m.dStatus.p, 0, sizeof(int)), "hipMemset");
     M.status = m.dStatus.as<int>();
+    if (!bigRule.empty()) {
+        m.dRule.ensure(bigRule.size() * sizeof(double));
+        hip_check(hipMemcpy(m.dRule.p, bigRule.data(), bigRule.size() * sizeof(double), hipMemcpyHostToDevice),
+                  "hipMemcpy");
+        M.quad.big = m.dRule.as<double>();
+    }
 
     // Off-diagonal table slots: max off-diagonal degree of every coordinate.
     const unsigned D = m.inDim;

@@ -137,7 +137,9 @@ __global__ void __launch_bounds__(256) k_inverse_queue(MapDev M, const double* _
     const int cRight = nc - 1, cMid = cRight / 2;
     const int fRight = nf - 1, fMid = fRight / 2;
     const int nNodes = cRight - 1 + (nf - 1) / 2;  // per interval, midpoint included
-    if (QK == QK_ACC5 || QK == QK_ACC) {
+    // A rule of more than MAXNF points (Q.big) is read from device memory by
+    // the steps below; the tables here hold rules up to MAXNF points.
+    if ((QK == QK_ACC5 || QK == QK_ACC) && !Q.big) {
         if (threadIdx.x == 0) {
             int j = 1;
             for (int i = 1; i < cMid; ++i, ++j) {
@@ -287,11 +289,19 @@ __global__ void __launch_bounds__(256) k_inverse_queue(MapDev M, const double* _
             // The fixed rule: one step evaluates every node (integrate_cc).
 #pragma clang fp contract(off)  // sums and node positions as integrate_cc forms them
             double r = 0.0;
+            if (Q.big) {
+                const auto bw = ro(Q.big) + 2 * Q.nc;
+                for (int i = 0; i < Q.nf; ++i) {
+                    const double t = 0.5 * (1.0 + 0.0 + (1.0 - 0.0) * bw[Q.nf + i]);
+                    r += (0.5 * (1.0 - 0.0) * bw[i]) * eval_node<BK, PK, FL_VAL, PM>(dc, t).F;
+                }
+            } else {
 #pragma unroll
-            for (int i = 0; i < MAXNF; ++i) {
-                if (i >= Q.nf) break;
-                const double t = 0.5 * (1.0 + 0.0 + (1.0 - 0.0) * Q.fp[i]);
-                r += (0.5 * (1.0 - 0.0) * Q.fw[i]) * eval_node<BK, PK, FL_VAL, PM>(dc, t).F;
+                for (int i = 0; i < MAXNF; ++i) {
+                    if (i >= Q.nf) break;
+                    const double t = 0.5 * (1.0 + 0.0 + (1.0 - 0.0) * Q.fp[i]);
+                    r += (0.5 * (1.0 - 0.0) * Q.fw[i]) * eval_node<BK, PK, FL_VAL, PM>(dc, t).F;
+                }
             }
             res = r;
             qdone = true;
@@ -357,32 +367,70 @@ __global__ void __launch_bounds__(256) k_inverse_queue(MapDev M, const double* _
             const double mid = 0.5 * (qa + qb);
             const double scale = 0.5 * (qb - qa);
             const bool start = k < 0;
-            double Fv[NF];
+            double coarse_ = 0.0, fine_ = 0.0;
+            if (QK == QK_ACC && Q.big) {
+                // A rule in device memory: the nodes in the order of the
+                // tables above, each added to the sums as it is evaluated.
+                if (start) {
+                    fa = eval_node<BK, PK, FL_VAL, PM>(dc, 0.0).F;
+                    fb = eval_node<BK, PK, FL_VAL, PM>(dc, 1.0).F;
+                } else {
+                    const int nc_ = Q.nc;
+                    const auto bcw = ro(Q.big);
+                    const auto bcp = bcw + nc_;
+                    const auto bfw = bcp + nc_;
+                    const auto bfp = bfw + nf;
+                    fm = eval_node<BK, PK, FL_VAL, PM>(dc, mid).F;
+                    coarse_ = scale * bcw[0] * fa;
+                    coarse_ += scale * bcw[cMid] * fm;
+                    coarse_ += scale * bcw[cRight] * fb;
+                    fine_ = scale * bfw[0] * fa;
+                    fine_ += scale * bfw[fMid] * fm;
+                    fine_ += scale * bfw[fRight] * fb;
+                    for (int i = 1; i < cMid; ++i) {
+                        const double F = eval_node<BK, PK, FL_VAL, PM>(dc, mid + scale * bcp[i]).F;
+                        coarse_ += scale * bcw[i] * F;
+                        fine_ += scale * bfw[2 * i] * F;
+                    }
+                    for (int ii = cMid + 1; ii < cRight; ++ii) {
+                        const double F = eval_node<BK, PK, FL_VAL, PM>(dc, mid + scale * bcp[ii]).F;
+                        coarse_ += scale * bcw[ii] * F;
+                        fine_ += scale * bfw[2 * ii] * F;
+                    }
+                    for (int i = 1; i < nf; i += 2)
+                        fine_ += scale * bfw[i] * eval_node<BK, PK, FL_VAL, PM>(dc, mid + scale * bfp[i]).F;
+                }
+            } else {
+                double Fv[NF];
 #pragma unroll
-            for (int j = 0; j < NF; ++j) {
-                if (j >= nNodes) break;
-                const double t = start ? (j == 0 ? 0.0 : 1.0) : (j == 0 ? mid : mid + scale * nPos[j]);
-                Fv[j] = (start && j > 1) ? 0.0 : eval_node<BK, PK, FL_VAL, PM>(dc, t).F;
+                for (int j = 0; j < NF; ++j) {
+                    if (j >= nNodes) break;
+                    const double t = start ? (j == 0 ? 0.0 : 1.0) : (j == 0 ? mid : mid + scale * nPos[j]);
+                    Fv[j] = (start && j > 1) ? 0.0 : eval_node<BK, PK, FL_VAL, PM>(dc, t).F;
+                }
+                if (start) {
+                    fa = Fv[0];
+                    fb = Fv[1];
+                } else {
+                    fm = Fv[0];
+                    coarse_ = scale * sCw[0] * fa;
+                    coarse_ += scale * sCw[cMid] * fm;
+                    coarse_ += scale * sCw[cRight] * fb;
+                    fine_ = scale * sFw[0] * fa;
+                    fine_ += scale * sFw[fMid] * fm;
+                    fine_ += scale * sFw[fRight] * fb;
+#pragma unroll
+                    for (int j = 1; j < NF; ++j) {
+                        if (j >= nNodes) break;
+                        const int ci = nCw[j];
+                        if (ci >= 0) coarse_ += scale * sCw[ci] * Fv[j];
+                        fine_ += scale * sFw[nFw[j]] * Fv[j];
+                    }
+                }
             }
             if (start) {
-                fa = Fv[0];
-                fb = Fv[1];
                 k = 0;
             } else {
-                fm = Fv[0];
-                double coarse_ = scale * sCw[0] * fa;
-                coarse_ += scale * sCw[cMid] * fm;
-                coarse_ += scale * sCw[cRight] * fb;
-                double fine_ = scale * sFw[0] * fa;
-                fine_ += scale * sFw[fMid] * fm;
-                fine_ += scale * sFw[fRight] * fb;
-#pragma unroll
-                for (int j = 1; j < NF; ++j) {
-                    if (j >= nNodes) break;
-                    const int ci = nCw[j];
-                    if (ci >= 0) coarse_ += scale * sCw[ci] * Fv[j];
-                    fine_ += scale * sFw[nFw[j]] * Fv[j];
-                }
                 const double err = fabs(fine_ - coarse_);
                 const double tol = fmax(Q.relTol * fabs(coarse_), Q.absTol);
                 if (nonfinite_err(err)) {

@@ -78,3 +78,94 @@ def test_terms_with_many_factors(dim, codegen):
     fin = np.isfinite(xi_r)
     assert np.array_equal(fin, np.isfinite(xi_g))
     assert np.max(np.abs(xi_g[fin] - xi_r[fin])) <= 1e-9
+
+
+# Quadrature rules of more than 17 points (quadPts above 17 for the fixed
+# rule; the nested adaptive rule has 2^(L+1) + 1 fine points with
+# L = floor(log2(quadPts - 2)), MParT/MapFactoryImpl4.cpp:18): the rule is read
+# from device memory and the interior node values of an accepted interval are
+# evaluated again for the adjoint.
+BIG_RULES = [
+    dict(quadType="AdaptiveClenshawCurtis", quadPts=20, quadAbsTol=1e-9, quadRelTol=1e-9),  # nc 17, nf 33
+    dict(quadType="AdaptiveClenshawCurtis", quadPts=40, quadAbsTol=1e-9, quadRelTol=1e-9,
+         basisType="HermiteFunctions", posFuncType="Exp"),                                  # nc 33, nf 65
+    dict(quadType="ClenshawCurtis", quadPts=25),
+    dict(quadType="ClenshawCurtis", quadPts=48, basisType="PhysicistHermite", posFuncType="Exp"),
+]
+
+
+@pytest.mark.parametrize("ci", range(len(BIG_RULES)))
+@pytest.mark.parametrize("codegen", [False, True])
+def test_quadrature_rules_above_17_points(ci, codegen):
+    case = BIG_RULES[ci]
+    print(f"[gpu-case] big_rule case={case} codegen={codegen}", flush=True)
+    st = Stream(1600 + ci)
+    g = mt.CreateTotalOrderComponent(3, 3, gpu_opts(**case))
+    g.SetCodegen(codegen)
+    r = orc.CreateComponent(ref_opts(**case), 3, 3)
+    x = st.points(3, 600)
+    c = 0.2 * st.normal(g.numCoeffs)
+    g.SetCoeffs(c)
+    r.SetCoeffs(c)
+    yg, neg = g.EvaluateCounted(x)
+    yr, ner = r.Evaluate(x, return_nevals=True)
+    assert np.array_equal(neg, ner), f"tree mismatch at {np.count_nonzero(neg != ner)} points"
+    close(yg, yr, what="Evaluate")
+    close(g.LogDeterminant(x), r.LogDeterminant(x), what="LogDeterminant")
+    sens = st.normal(x.shape[1])[None, :]
+    close(g.CoeffGrad(x, sens), r.CoeffGrad(x, sens), what="CoeffGrad")
+    close(g.LogDeterminantCoeffGrad(x), r.LogDeterminantCoeffGrad(x), what="LogDeterminantCoeffGrad")
+    close(g.Gradient(x, sens), r.Gradient(x, sens), what="Gradient")
+    close(g.LogDeterminantInputGrad(x), r.LogDeterminantInputGrad(x), what="LogDeterminantInputGrad")
+    tgt = st.normal(x.shape[1])[None, :]
+    xi_g = g.Inverse(x, tgt)
+    xi_r = r.Inverse(x, tgt)
+    fin = np.isfinite(xi_r)
+    assert np.array_equal(fin, np.isfinite(xi_g))
+    assert np.max(np.abs(xi_g[fin] - xi_r[fin])) <= 1e-9
+    # a triangular map: the fused KL pass and its plain structure
+    gt = mt.CreateTriangular(3, 3, 2, gpu_opts(**case))
+    gt.SetCodegen(codegen)
+    rt = orc.CreateTriangular(3, 3, 2, ref_opts(**case))
+    ct = 0.1 * st.normal(gt.numCoeffs)
+    gt.SetCoeffs(ct)
+    rt.SetCoeffs(ct)
+    lr, gr = rt.KL(x)
+    lf, gf = gt.KLValueAndGrad(x)
+    lp, gp = gt.KLValueAndGrad(x, plain=True)
+    close(lf, lr, what="KL loss fused")
+    close(gf, gr, what="KL grad fused")
+    close(lp, lr, what="KL loss plain")
+    close(gp, gr, what="KL grad plain")
+
+
+def test_quadrature_rule_above_17_points_batched():
+    print("[gpu-case] big_rule batched", flush=True)
+    torch = pytest.importorskip("torch")
+    case = BIG_RULES[0]
+    st = Stream(1700)
+    maps, refs, xs = [], [], []
+    for k, (dim, order) in enumerate([(2, 2), (3, 3), (4, 2)]):
+        g = mt.CreateTriangular(dim, dim, order, gpu_opts(**case))
+        r = orc.CreateTriangular(dim, dim, order, ref_opts(**case))
+        c = 0.1 * st.normal(g.numCoeffs)
+        g.SetCoeffs(c)
+        r.SetCoeffs(c)
+        maps.append(g)
+        refs.append(r)
+        xs.append(torch.from_numpy(st.points(dim, 300 + 100 * k)).cuda())
+    b = mt.MapBatch(maps)
+    ys, lds = b.Evaluate(xs, logdet=True)
+    for y, ld, r, x in zip(ys, lds, refs, xs):
+        xn = x.cpu().numpy()
+        close(y.cpu().numpy(), r.Evaluate(xn), what="batched Evaluate")
+        close(ld.cpu().numpy(), r.LogDeterminant(xn), what="batched LogDeterminant")
+    for sched in (1, 2):
+        b.SetInverseSchedule(sched)
+        outs = b.Inverse([torch.zeros((0, x.shape[1]), dtype=torch.float64, device="cuda") for x in xs], ys)
+        for o, r, y in zip(outs, refs, ys):
+            xr = r.Inverse(np.zeros((0, y.shape[1])), y.cpu().numpy())
+            xg = o.cpu().numpy()
+            fin = np.isfinite(xr)
+            assert np.array_equal(fin, np.isfinite(xg)), sched
+            assert np.max(np.abs(xg[fin] - xr[fin])) <= 1e-9, sched
