@@ -560,7 +560,7 @@ def training_workload(tri, x, steps=50):
     return out
 
 
-def uneven_map_workload(steps):
+def uneven_map_workload(steps, plain_n=20_000):
     """32-D triangular map, component k of total order max(1, 6 - k // 6)
     (7 to 7 315 terms, 62 572 coefficients), HermiteFunctions / Exp / adaptive
     Simpson (tolerances 1e-6); N = 200 000 points with Student-t(5)
@@ -568,7 +568,7 @@ def uneven_map_workload(steps):
     specialization limit, so the generic kernels run with the autotuned
     point schedule (lanes per point).  Times Evaluate, LogDeterminant and the
     fused KL value + gradient; the plain KL path (per-component launches with
-    the two K x N Jacobians stored) on the first 20 000 points."""
+    the two K x N Jacobians stored) on the first plain_n points (20 000 by default)."""
     import torch
 
     import mpart_amd as mt
@@ -584,13 +584,13 @@ def uneven_map_workload(steps):
     ld = torch.empty(N, dtype=torch.float64, device="cuda")
     loss = torch.zeros(1, dtype=torch.float64, device="cuda")
     grad = torch.zeros(g.numCoeffs, dtype=torch.float64, device="cuda")
-    xs = x[:, :20_000].contiguous()
+    xs = x[:, :plain_n].contiguous()
     calls = {
         "evaluate": lambda: g.EvaluateImpl(x, y),
         "log_determinant": lambda: g.LogDeterminantImpl(x, ld),
         "kl_fused": lambda: g.KLValueAndGradImpl(x, loss, grad),
-        "kl_fused_20000": lambda: g.KLValueAndGradImpl(xs, loss, grad),
-        "kl_plain_20000": lambda: g.KLValueAndGradImpl(xs, loss, grad, plain=True),
+        f"kl_fused_{plain_n}": lambda: g.KLValueAndGradImpl(xs, loss, grad),
+        f"kl_plain_{plain_n}": lambda: g.KLValueAndGradImpl(xs, loss, grad, plain=True),
     }
     out = {"workload": "32-D triangular, orders max(1, 6 - k // 6), 62572 coefficients, HermiteFunctions/Exp/"
                        "AdaptiveSimpson, N=200000 Student-t(5)", "N": N, "num_coeffs": g.numCoeffs}
@@ -605,7 +605,7 @@ def uneven_map_workload(steps):
         out[name + "_ms"] = ms
         log(f"uneven 32-D map {name}: {ms:.2f} ms")
     out["kl_fused_points_per_s"] = N / (out["kl_fused_ms"] * 1e-3)
-    out["fused_vs_plain_20000"] = out["kl_plain_20000_ms"] / out["kl_fused_20000_ms"]
+    out[f"fused_vs_plain_{plain_n}"] = out[f"kl_plain_{plain_n}_ms"] / out[f"kl_fused_{plain_n}_ms"]
     # Roofline of the fused pass: the reference-algorithm flop count of these
     # points' adaptive trees (mpart_amd/roofline.py algo_flops) over its wall time.
     from mpart_amd.roofline import algo_flops
@@ -848,6 +848,9 @@ def main():
     ap.add_argument("--profile-many", action="store_true")
     ap.add_argument("--profile-inverse-many", action="store_true")
     ap.add_argument("--profile-uneven", action="store_true")
+    # points of the uneven map's plain (materialized Jacobian) comparison:
+    # 2 x 62 572 x N doubles of Jacobians, 1 MB per point
+    ap.add_argument("--uneven-plain-n", type=int, default=20_000)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -971,7 +974,7 @@ def main():
         single = single_component_workload(max(3, args.plain_steps))
     uneven = None
     if not args.no_uneven:
-        uneven = uneven_map_workload(2)
+        uneven = uneven_map_workload(2, args.uneven_plain_n)
 
     kinfo = kernel_info(tri, fused_schedule["choice"])
     th = traffic["headline"] if traffic else None

@@ -71,7 +71,12 @@ extern "C" {
  * mpart_default_options() are the reference defaults.  A finite basisLB or
  * basisUB selects the LinearizedBasis of the factory (MParT/MapFactory.h:226,
  * MParT/LinearizedBasis.h); quadType ClenshawCurtis is the fixed rule of
- * quadPts points (at most 17 here). */
+ * quadPts points.  quadMaxSub is at most 32: the reference keeps the path of
+ * its subdivision tree in a 32-bit mask (MParT/Quadrature.h:566,636-647),
+ * whose shifts are undefined past level 31.  The last coordinate's degree is
+ * at most 12 (the degree bound of the kernels' register arrays; the
+ * reference's 32-bit Factorial, MParT/Utilities/MathFunctions.h:16, wraps at
+ * 13! in the default normalization), other coordinates' degrees at most 32. */
 typedef struct mpart_options {
     int32_t basisType;
     int32_t posFuncType;

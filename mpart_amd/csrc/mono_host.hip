@@ -420,7 +420,8 @@ void build_plan(mpart_map_s& m) {
     if (o.posFuncType != MPART_POSFUNC_EXP && o.posFuncType != MPART_POSFUNC_SOFTPLUS)
         throw ArgError("unknown posFuncType");
     if (o.quadMaxSub < 1 || o.quadMaxSub > (unsigned)mpd::MAXSUB)
-        throw ArgError("quadMaxSub must be in [1, " + std::to_string(mpd::MAXSUB) + "]");
+        throw ArgError("quadMaxSub must be in [1, " + std::to_string(mpd::MAXSUB) +
+                       "] (deeper trees overflow the reference's 32-bit subdivision path, MParT/Quadrature.h:566)");
     if (o.quadMinSub >= o.quadMaxSub)
         throw ArgError("quadMinSub must be smaller than quadMaxSub (the subdivision depth is bounded by quadMaxSub)");
     if (o.nugget < 0) throw ArgError("nugget must be non-negative");
