@@ -437,6 +437,11 @@ int mpart_eval_schedule(mpart_map map, int32_t* lanes, double* trialMs);
  * milliseconds, -1 for a schedule left out because its kernel would spill
  * registers.  No reference counterpart. */
 int mpart_fused_schedule(mpart_map map, int32_t* choice, double* trialMs);
+/* Forces schedule `choice` (0-3 as above) for the map's specialized fused
+ * calls, or -1 to tune again on the next call (schedules differ in the order
+ * of the sums over points, so forcing one makes results repeatable across
+ * processes).  No reference counterpart. */
+int mpart_set_fused_schedule(mpart_map map, int32_t choice);
 
 /* Resources of one of the map's specialized (hipRTC-compiled) kernels, as the
  * HIP runtime reports them (hipFuncGetAttribute): registers per lane,

@@ -127,6 +127,7 @@ _SIGS = {
     "mpart_batch_eval_schedule": [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)],
     "mpart_codegen_prepare": [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)],
     "mpart_fused_schedule": [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)],
+    "mpart_set_fused_schedule": [_vp, ctypes.c_int32],
     "mpart_set_point_group": [_vp, ctypes.c_int32],
     "mpart_eval_schedule": [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)],
     "mpart_train_map": [_vp, _CView, ctypes.POINTER(_CTrainOptions), ctypes.POINTER(ctypes.c_double),
@@ -455,6 +456,10 @@ class ConditionalMap:
         _check(_lib.mpart_codegen_kernel_info(self._h, kernel.encode(), ctypes.byref(v), ctypes.byref(sc),
                                               ctypes.byref(lds)))
         return int(v.value), int(sc.value), int(lds.value)
+
+    def SetFusedSchedule(self, choice=-1):
+        """Forces the specialized fused kernel's schedule (0-3, mpart_set_fused_schedule) or -1 to tune again."""
+        _check(_lib.mpart_set_fused_schedule(self._h, int(choice)))
 
     def FusedSchedule(self):
         """(schedule index or -1 before tuning, trial times in ms) of the fused kernel."""

@@ -336,6 +336,9 @@ double* noalloc_scratch() {
 }  // namespace
 
 // ---------------------------------------------------------------------------
+// Schedules of the specialized fused pass the tuner times (gen_fused_launch).
+constexpr int GEN_FUSED_SCHEDULES = 4;
+
 struct mpart_map_s {
     mpart_options opts{};
     bool triangular = false;
@@ -401,7 +404,7 @@ struct mpart_map_s {
     // trials of the generic fused pass: lanes per point 1, 4, 8, 64, then the
     // component-split schedule
     double fusedGroupTrialMs[5] = {0, 0, 0, 0, 0};
-    double fusedTrialMs[4] = {0, 0, 0, 0};
+    double fusedTrialMs[GEN_FUSED_SCHEDULES] = {0, 0, 0, 0};
 };
 
 namespace {
@@ -1252,6 +1255,23 @@ void gen_fused_shape(mpart_map_s& m, mpg::GenModule* g, int& gblock, size_t& gsm
     m.genOccSmem = gsmem;
 }
 
+// One schedule of the specialized fused pass: mpart_gen_fused (cfg & 1: the
+// three-waves build) on a persistent grid of one or two blocks per resident
+// slot ((cfg >> 1) + 1).
+struct GenFusedLaunch {
+    hipFunction_t fn;
+    int block;
+    size_t smem;
+    long grid;
+};
+GenFusedLaunch gen_fused_launch(const mpart_map_s& m, const mpg::GenModule* g, int gblock, size_t gsmem, int cfg,
+                                long N) {
+    GenFusedLaunch L{(cfg & 1) ? g->fused3 : g->fused, gblock, gsmem, 1};
+    const long ntiles = std::max<long>(1, (N + gblock - 1) / gblock);
+    L.grid = std::min<long>(ntiles, (long)g_numCU * m.genOccK[cfg & 1] * ((cfg >> 1) + 1));
+    return L;
+}
+
 // Launch shape of the generic lane-per-point fused kernel: block size, LDS
 // bytes, whether the per-wave gradient accumulators (ldsAcc) and the basis
 // table (tabLds) fit in LDS, and the persistent grid.
@@ -1309,15 +1329,10 @@ void do_fused(mpart_map_s& m, const mpart_cview& p, const double* sens, long srs
         int gblock = 256;
         size_t gsmem = 0;
         gen_fused_shape(m, g, gblock, gsmem);
-        hipFunction_t fns[2] = {g->fused, g->fused3};
         const long ntiles = std::max<long>(1, (N + gblock - 1) / gblock);
-        // Schedule cfg: kernel (cfg & 1: compiler occupancy / three waves per
-        // SIMD) and persistent-grid multiple ((cfg >> 1) + 1).
-        auto grid_of = [&](int cfg) {
-            return std::min<long>(ntiles, (long)g_numCU * m.genOccK[cfg & 1] * ((cfg >> 1) + 1));
-        };
+        auto grid_of = [&](int cfg) { return gen_fused_launch(m, g, gblock, gsmem, cfg, N).grid; };
         long maxGrid = 1;
-        for (int cfg = 0; cfg < 4; ++cfg) maxGrid = std::max(maxGrid, grid_of(cfg));
+        for (int cfg = 0; cfg < GEN_FUSED_SCHEDULES; ++cfg) maxGrid = std::max(maxGrid, grid_of(cfg));
         // Partial rows of the widest schedule, then the reduction's chunk rows.
         m.partialGrad.ensure((size_t)(maxGrid + mpd::COLSUM_MAX_CHUNKS) * (K + 1) * sizeof(double));
         const double* w = gen_coeffs(m, s);
@@ -1332,7 +1347,8 @@ void do_fused(mpart_map_s& m, const mpart_cview& p, const double* sens, long srs
         double* part = m.partialGrad.as<double>();
         void* args[] = {&M, &w, &pts, &prs, &pcs, &NN, &sp, &srs_, &scs_, &lp, &lss_, &kl, &part};
         auto launch_cfg = [&](int cfg) {
-            gen_launch(fns[cfg & 1], grid_of(cfg), gblock, gsmem, s, args, "launch specialized fused");
+            const GenFusedLaunch L = gen_fused_launch(m, g, gblock, gsmem, cfg, N);
+            gen_launch(L.fn, L.grid, L.block, L.smem, s, args, "launch specialized fused");
         };
         mpg::GenModule* gi = gin ? gen_in_for(m) : nullptr;
         if (gi) {
@@ -1372,7 +1388,7 @@ void do_fused(mpart_map_s& m, const mpart_cview& p, const double* sens, long srs
                 hip_check(hipEventCreate(&e0), "hipEventCreate");
                 hip_check(hipEventCreate(&e1), "hipEventCreate");
                 int best = 0;
-                for (int cfg = 0; cfg < 4; ++cfg) {
+                for (int cfg = 0; cfg < GEN_FUSED_SCHEDULES; ++cfg) {
                     if ((cfg & 1) && m.genW3Spills) {
                         m.fusedTrialMs[cfg] = -1.0;
                         continue;
@@ -1572,8 +1588,8 @@ void do_fused_noalloc(mpart_map_s& m, const mpart_cview& p, double* loss, double
         size_t gsmem = 0;
         gen_fused_shape(m, g, gblock, gsmem);
         const int cfg = (m.fusedChoice < 0) ? 0 : m.fusedChoice;
-        const long ntiles = std::max<long>(1, (N + gblock - 1) / gblock);
-        long grid = std::min<long>(ntiles, (long)g_numCU * m.genOccK[cfg & 1] * ((cfg >> 1) + 1));
+        const GenFusedLaunch GL = gen_fused_launch(m, g, gblock, gsmem, cfg, N);
+        long grid = GL.grid;
         // Layout: [partial rows: grid x (K + 1)][reduction chunks: <= 64 x (K + 1)] ...
         // [contiguous copy of strided coefficients: K, at the end].
         const long extra = (m.coeffStride == 1) ? 0 : K;
@@ -1599,8 +1615,7 @@ void do_fused_noalloc(mpart_map_s& m, const mpart_cview& p, double* loss, double
         int kl = 1;
         double* part = scr;
         void* args[] = {&M, &w, &pts, &prs, &pcs, &NN, &sp, &srs, &scs, &lp, &lss, &kl, &part};
-        hipFunction_t fn = (cfg & 1) ? g->fused3 : g->fused;
-        if (N > 0) gen_launch(fn, grid, gblock, gsmem, s, args, "launch specialized fused");
+        if (N > 0) gen_launch(GL.fn, grid, GL.block, GL.smem, s, args, "launch specialized fused");
         hip_check(mpd::launch_colsum(part, N > 0 ? grid : 0, K + 1, K + 1, scale, grad, loss, part + grid * (K + 1), s),
                   "launch reduce");
         return;
@@ -2755,7 +2770,18 @@ int mpart_fused_schedule(mpart_map m, int32_t* choice, double* trialMs) {
         check_map(m);
         if (choice) *choice = m->fusedChoice;
         if (trialMs)
-            for (int i = 0; i < 4; ++i) trialMs[i] = m->fusedTrialMs[i];
+            for (int i = 0; i < GEN_FUSED_SCHEDULES; ++i) trialMs[i] = m->fusedTrialMs[i];
+    });
+}
+
+int mpart_set_fused_schedule(mpart_map m, int32_t choice) {
+    return guarded([&] {
+        check_map(m);
+        if (choice < -1 || choice >= GEN_FUSED_SCHEDULES)
+            throw ArgError("fused schedule must be -1 (autotune) or 0.." + std::to_string(GEN_FUSED_SCHEDULES - 1));
+        m->fusedChoice = choice;
+        if (choice < 0)
+            for (double& t : m->fusedTrialMs) t = 0.0;
     });
 }
 

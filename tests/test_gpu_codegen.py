@@ -184,3 +184,35 @@ def test_fused_kl_step_replays_in_a_graph():
         g.KLValueAndGradImpl(x, le, ge)  # eager, same buffers and schedule
         torch.cuda.synchronize()
         assert torch.equal(loss, le) and torch.equal(grad, ge), f"step {step}"
+
+
+
+def test_forced_fused_schedules():
+    # Each schedule sums the points in its own order: 1e-13 relative to each
+    # other and 1e-10 to the oracle; a forced schedule is kept.
+    torch = pytest.importorskip("torch")
+    _case("codegen_forced_schedules")
+    g = mt.CreateTriangular(6, 6, 3, mt.MapOptions())
+    r = orc.CreateTriangular(6, 6, 3, orc.options())
+    s = Stream(41)
+    xn = s.points(6, 6000)
+    x = torch.from_numpy(xn).cuda()
+    c = 0.1 * s.normal(g.numCoeffs)
+    g.SetCoeffs(c)
+    r.SetCoeffs(c)
+    out = {}
+    for ch in (0, 1, 2, 3):
+        g.SetFusedSchedule(ch)
+        l, gr = g.KLValueAndGrad(x)
+        out[ch] = (l.cpu().numpy().reshape(1), gr.cpu().numpy())
+        assert g.FusedSchedule()[0] == ch
+    for ch in (1, 2, 3):
+        close(out[ch][0], out[0][0], rtol=1e-13, what=f"loss, schedule {ch}")
+        close(out[ch][1], out[0][1], rtol=1e-13, what=f"gradient, schedule {ch}")
+    lr, gr_ = r.KL(xn)
+    close(out[2][0], np.array([lr]), what="loss vs oracle")
+    close(out[2][1], gr_, what="gradient vs oracle")
+    with pytest.raises(ValueError):
+        g.SetFusedSchedule(4)
+    g.SetFusedSchedule(-1)
+    assert g.FusedSchedule()[0] == -1
