@@ -31,11 +31,12 @@ hipError_t launch_fused_grp(int var, int G, const Launch& L, const MapDev& M, co
                             const double* lsens, long lss, int klMode, double* rows);
 int occupancy_fused_grp(int var, int G, int block, size_t smem);
 constexpr int FUSED_SPLIT_TILE = 64;   // points per tile of the component-split fused pass
-constexpr int FUSED_SPLIT_WAVES = 4;   // its wavefronts per block
-// Fused pass with the components split over the four wavefronts of a block
-// (mono_split.hpp): block 256, LDS 8 * nslots * 64 bytes, one partial row of
+constexpr int FUSED_SPLIT_WAVES = 8;   // its wavefronts per block
+// Fused pass with the components split over the eight wavefronts of a block
+// (mono_split.hpp): block 512, LDS 8 * nslots * 64 bytes, one partial row of
 // K + 1 doubles per block.
-// minb: 1, or 2 for the build capped at two wavefronts per SIMD.
+// minb: 1, or 2 for the build with two resident blocks per CU (four
+// wavefronts per SIMD, 128 registers).
 hipError_t launch_fused_split(int var, int minb, const Launch& L, const MapDev& M, const double* w, long ws,
                               const double* pts, long prs, long pcs, long N, const double* sens, long srs, long scs,
                               const double* lsens, long lss, int klMode, const int* compOrder, const int* waveStart,

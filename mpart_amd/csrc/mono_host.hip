@@ -661,7 +661,7 @@ void build_plan(mpart_map_s& m) {
         }
         comps.push_back(cd);
     }
-    // Component-split schedule: the components over the four wavefronts of a
+    // Component-split schedule: the components over the eight wavefronts of a
     // block, longest first to the least loaded wavefront (cost ~ terms, plus
     // the quadrature).
     {
@@ -1482,7 +1482,7 @@ void do_fused(mpart_map_s& m, const mpart_cview& p, const double* sens, long srs
     // when the tile's table does not fit or the map has one component.
     auto run_split = [&](int minb) -> bool {
         const size_t smem = (size_t)m.dev.nslots * mpd::FUSED_SPLIT_TILE * sizeof(double);
-        // (the kernel's static LDS is the four wavefronts' loss sums)
+        // (the kernel's static LDS is the eight wavefronts' loss sums)
         if (smem + mpd::FUSED_SPLIT_WAVES * sizeof(double) > LDS_LIMIT || m.comps.size() < 2) return false;
         if (minb != 2) minb = 1;
         int& occ = m.splitOcc[minb - 1];

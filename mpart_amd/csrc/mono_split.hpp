@@ -1,10 +1,10 @@
 // mono_split.hpp -- fused forward + adjoint for maps with many terms, with the
 // components of a map split over the wavefronts of a block.
 //
-// A block of four wavefronts takes a tile of 64 points at a time.  The tile's
+// A block of eight wavefronts takes a tile of 64 points at a time.  The tile's
 // basis table (every coordinate's values phi_1..phi_deg, slot-major: entry
 // (slot, point) at tab[slot * 64 + point]) sits once in LDS, filled by all
-// four wavefronts, so it costs 8 * nslots bytes per point instead of per lane.
+// eight wavefronts, so it costs 8 * nslots bytes per point instead of per lane.
 // Each wavefront then runs a lane per point over the components assigned to
 // it (a host-side partition of the components balanced by term count): the
 // C_q sums, the component's quadrature and the gradient pass, whose term loop
@@ -21,12 +21,12 @@
 namespace mpd {
 
 constexpr int SPLIT_TILE = 64;   // points per tile
-constexpr int SPLIT_WAVES = 4;   // wavefronts per block
+constexpr int SPLIT_WAVES = 8;   // wavefronts per block
 
 // MINB: resident blocks per CU the compiler must allow (1, or 2 to cap the
-// registers at two wavefronts per SIMD at the cost of spills).
+// registers at four wavefronts per SIMD at the cost of spills).
 template <int BK, int PK, int QK, int PM, int MINB>
-__global__ void __launch_bounds__(256, MINB) k_fused_split(MapDev M, const double* w, long ws, PtsArg P, const double* sens,
+__global__ void __launch_bounds__(SPLIT_WAVES * 64, MINB) k_fused_split(MapDev M, const double* w, long ws, PtsArg P, const double* sens,
                                                      long srs, long scs, const double* lsens, long lss, int klMode,
                                                      const int* compOrder, const int* waveStart, double* rows) {
     extern __shared__ double tab[];
@@ -101,7 +101,7 @@ __global__ void __launch_bounds__(256, MINB) k_fused_split(MapDev M, const doubl
 namespace mpd {
 
 // Evaluate and/or LogDeterminant of every component with the same split: a
-// block of four wavefronts per tile of 64 points, one shared basis table,
+// block of eight wavefronts per tile of 64 points, one shared basis table,
 // each wavefront a lane per point over its own components.  T goes straight
 // to the output row of the component; the log-determinant terms of the four
 // wavefronts are added in wavefront order at the end of the tile, so that sum
@@ -110,7 +110,7 @@ namespace mpd {
 // first rows once the tile's components are done, so the kernel needs no LDS
 // beyond the table.
 template <int BK, int PK, int QK, int PM>
-__global__ void __launch_bounds__(256) k_eval_split(MapDev M, const double* w, long ws, PtsArg P, OutArg out,
+__global__ void __launch_bounds__(SPLIT_WAVES * 64) k_eval_split(MapDev M, const double* w, long ws, PtsArg P, OutArg out,
                                                     double* ld, long lds, int ldAccumulate, int* nevals,
                                                     const int* compOrder, const int* waveStart) {
     extern __shared__ double tab[];  // nslots >= SPLIT_WAVES (the host checks)
