@@ -35,8 +35,8 @@ constexpr int FUSED_SPLIT_WAVES = 8;   // its wavefronts per block
 // Fused pass with the components split over the eight wavefronts of a block
 // (mono_split.hpp): block 512, LDS 8 * nslots * 64 bytes, one partial row of
 // K + 1 doubles per block.
-// minb: 1, or 2 for the build with two resident blocks per CU (four
-// wavefronts per SIMD, 128 registers).
+// minb: 1 for the build at two wavefronts per SIMD (256 registers), 2 for the
+// build at four (128 registers, with spills).
 hipError_t launch_fused_split(int var, int minb, const Launch& L, const MapDev& M, const double* w, long ws,
                               const double* pts, long prs, long pcs, long N, const double* sens, long srs, long scs,
                               const double* lsens, long lss, int klMode, const int* compOrder, const int* waveStart,

@@ -23,10 +23,13 @@ namespace mpd {
 constexpr int SPLIT_TILE = 64;   // points per tile
 constexpr int SPLIT_WAVES = 8;   // wavefronts per block
 
-// MINB: resident blocks per CU the compiler must allow (1, or 2 to cap the
-// registers at four wavefronts per SIMD at the cost of spills).
-template <int BK, int PK, int QK, int PM, int MINB>
-__global__ void __launch_bounds__(SPLIT_WAVES * 64, MINB) k_fused_split(MapDev M, const double* w, long ws, PtsArg P, const double* sens,
+// MINW: wavefronts per SIMD the compiler must allow (the second launch
+// bound): 2 (up to 256 registers) or 4 (up to 128, with spills).  The tile's
+// table allows two blocks of eight wavefronts per CU, so at 4 the LDS and
+// the registers both admit four wavefronts per SIMD; the latency of the
+// scalar record loads and the LDS reads wants them.
+template <int BK, int PK, int QK, int PM, int MINW>
+__global__ void __launch_bounds__(SPLIT_WAVES * 64, MINW) k_fused_split(MapDev M, const double* w, long ws, PtsArg P, const double* sens,
                                                      long srs, long scs, const double* lsens, long lss, int klMode,
                                                      const int* compOrder, const int* waveStart, double* rows) {
     extern __shared__ double tab[];
@@ -110,7 +113,8 @@ namespace mpd {
 // first rows once the tile's components are done, so the kernel needs no LDS
 // beyond the table.
 template <int BK, int PK, int QK, int PM>
-__global__ void __launch_bounds__(SPLIT_WAVES * 64) k_eval_split(MapDev M, const double* w, long ws, PtsArg P, OutArg out,
+// (four wavefronts per SIMD, as the fused split kernel's second build)
+__global__ void __launch_bounds__(SPLIT_WAVES * 64, 4) k_eval_split(MapDev M, const double* w, long ws, PtsArg P, OutArg out,
                                                     double* ld, long lds, int ldAccumulate, int* nevals,
                                                     const int* compOrder, const int* waveStart) {
     extern __shared__ double tab[];  // nslots >= SPLIT_WAVES (the host checks)
