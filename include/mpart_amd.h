@@ -166,6 +166,12 @@ int mpart_destroy(mpart_map map);
 /* inputDim / outputDim / numCoeffs of ConditionalMapBase. */
 int mpart_map_dims(mpart_map map, uint32_t* inputDim, uint32_t* outputDim, uint32_t* numCoeffs);
 
+/* The MapOptions the map was built with and whether it is a TriangularMap
+ * (1) or a single component (0); what a serialized map needs besides its
+ * multi-index sets and coefficients (the reference keeps them in the objects
+ * its cereal archives save, MParT/MonotoneComponent.h and TriangularMap.h). */
+int mpart_map_options(mpart_map map, mpart_options* opts, int32_t* triangular);
+
 /* Number of components and the multi-index set of component i (host arrays
  * sized by a first call with NULL pointers). */
 int mpart_map_num_components(mpart_map map, uint32_t* ncomps);

@@ -90,6 +90,7 @@ _SIGS = {
     "mpart_destroy": [_vp],
     "mpart_map_dims": [_vp, _U32P, _U32P, _U32P],
     "mpart_map_num_components": [_vp, _U32P],
+    "mpart_map_options": [_vp, _vp, ctypes.POINTER(ctypes.c_int32)],
     "mpart_component_multiset": [_vp, _u32, _U32P, _U32P, _U32P, _U32P, _U32P, _U32P],
     "mpart_set_coeffs": [_vp, _vp, _i64, _i64, _vp],
     "mpart_wrap_coeffs": [_vp, _vp, _i64, _i64],
@@ -386,6 +387,55 @@ class ConditionalMap:
         dim, s, d, o = _read_multiset(self._h, i)
         return FixedMultiIndexSet.from_csr(dim, s, d, o)
 
+    def Options(self):
+        """(MapOptions the map was built with, whether it is a TriangularMap)."""
+        c = _COptions()
+        tri = ctypes.c_int32()
+        _check(_lib.mpart_map_options(self._h, ctypes.byref(c), ctypes.byref(tri)))
+        o = MapOptions()
+        o.basisType = BasisTypes(c.basisType)
+        o.posFuncType = PosFuncTypes(c.posFuncType)
+        o.quadType = QuadTypes(c.quadType)
+        o.contDeriv = bool(c.contDeriv)
+        o.basisNorm = bool(c.basisNorm)
+        o.quadMaxSub, o.quadMinSub, o.quadPts = c.quadMaxSub, c.quadMinSub, c.quadPts
+        o.quadAbsTol, o.quadRelTol, o.nugget = c.quadAbsTol, c.quadRelTol, c.nugget
+        o.basisLB, o.basisUB = c.basisLB, c.basisUB
+        return o, bool(tri.value)
+
+    # byte string and pickle (ParameterizedFunctionBase ToBytes / FromBytes and
+    # py::pickle, bindings/python/src/ParameterizedFunctionBase.cpp:57-83) ------
+    def ToBytes(self):
+        """The map (options, multi-index sets, coefficients) as one byte string
+        (mpart_amd.serialization.map_state_to_bytes; not the reference's
+        polymorphic cereal bytes)."""
+        from . import serialization
+
+        opts, tri = self.Options()
+        msets = [self.GetMultiIndexSet(i) for i in range(self.NumComponents())]
+        coeffs = self.CoeffMap()
+        return serialization.map_state_to_bytes(opts, tri, msets, self.inputDim, self.outputDim, coeffs)
+
+    @staticmethod
+    def FromBytes(data):
+        """A new map from ToBytes' byte string."""
+        from . import serialization
+
+        opts = MapOptions()
+        tri, sets, inputDim, outputDim, coeffs = serialization.map_state_from_bytes(data, opts)
+        opts.basisType = BasisTypes(opts.basisType)
+        opts.posFuncType = PosFuncTypes(opts.posFuncType)
+        opts.quadType = QuadTypes(opts.quadType)
+        comps = [CreateComponent(FixedMultiIndexSet.from_csr(*st), opts) for st in sets]
+        m = TriangularMap(comps) if tri else comps[0]
+        if (m.inputDim, m.outputDim, m.numCoeffs) != (inputDim, outputDim, coeffs.size):
+            raise ValueError("byte string: the multi-index sets do not match the archived dimensions")
+        m.SetCoeffs(coeffs)
+        return m
+
+    def __reduce__(self):
+        return (ConditionalMap.FromBytes, (self.ToBytes(),))
+
     # coefficients ---------------------------------------------------------
     def SetCoeffs(self, coeffs):
         """Deep copy (src/ParameterizedFunctionBase.cpp:184-214)."""
@@ -677,12 +727,16 @@ def CreateTriangular(inputDim, outputDim, totalOrder, opts=None):
     return ConditionalMap(h.value)
 
 
-def TriangularMap(components):
-    """TriangularMap(components) with the components' coefficients copied in."""
+def TriangularMap(components, moveCoeffs=False):
+    """TriangularMap(components, moveCoeffs) (src/TriangularMap.cpp:10-61): the
+    components' coefficients are copied into the new map either way."""
     arr = (ctypes.c_void_p * len(components))(*[c.handle.value for c in components])
     h = ctypes.c_void_p()
     _check(_lib.mpart_create_triangular_from(arr, len(components), ctypes.byref(h)))
     return ConditionalMap(h.value)
+
+
+TriangularMap.FromBytes = ConditionalMap.FromBytes
 
 
 class MapBatch:

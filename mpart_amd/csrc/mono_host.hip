@@ -2453,6 +2453,14 @@ int mpart_map_dims(mpart_map m, uint32_t* i, uint32_t* o, uint32_t* c) {
     });
 }
 
+int mpart_map_options(mpart_map m, mpart_options* o, int32_t* triangular) {
+    return guarded([&] {
+        check_map(m);
+        if (o) *o = m->opts;
+        if (triangular) *triangular = m->triangular ? 1 : 0;
+    });
+}
+
 int mpart_map_num_components(mpart_map m, uint32_t* n) {
     return guarded([&] {
         check_map(m);

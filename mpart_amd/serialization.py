@@ -117,3 +117,50 @@ def map_from_bytes(data):
     if coeffs.size != numCoeffs:
         raise ValueError(f"archive holds {coeffs.size} coefficients, header says {numCoeffs}")
     return inputDim, outputDim, coeffs
+
+
+# ---------------------------------------------------------------------------
+# A whole map as one byte string (ToBytes / FromBytes / pickle).
+#
+# The reference's ToBytes is cereal's polymorphic archive of the map object
+# (bindings/python/src/ParameterizedFunctionBase.cpp:57-83): registered type
+# names, shared-pointer ids and each class's own fields.  Those type names
+# and class layouts are the reference's C++ types, which this library does
+# not have, so its byte string is not reproduced.  This one is assembled from
+# the archives above, which are the reference's formats:
+#   b"MPARTAMD", uint32 version (1), uint8 triangular,
+#   the MapOptions archive, uint32 component count,
+#   per component a uint64 length and its FixedMultiIndexSet archive,
+#   the map archive (inputDim, outputDim, numCoeffs, coefficient view).
+_MAGIC = b"MPARTAMD"
+_VERSION = 1
+
+
+def map_state_to_bytes(opts, triangular, msets, inputDim, outputDim, coeffs):
+    out = _MAGIC + struct.pack("<IB", _VERSION, 1 if triangular else 0)
+    out += options_to_bytes(opts)
+    out += struct.pack("<I", len(msets))
+    for s in msets:
+        b = multiset_to_bytes(s)
+        out += struct.pack("<Q", len(b)) + b
+    return out + map_to_bytes(inputDim, outputDim, coeffs)
+
+
+def map_state_from_bytes(data, opts):
+    """(triangular, [(dim, nzStarts, nzDims, nzOrders)], inputDim, outputDim,
+    coefficients); fills opts."""
+    data = bytes(data)
+    if data[:len(_MAGIC)] != _MAGIC:
+        raise ValueError("not a map byte string of this library (the reference's polymorphic cereal form is not read)")
+    r = _Reader(data)
+    r.raw(len(_MAGIC))
+    version, tri = r.take("<I"), r.take("<B")
+    if version != _VERSION:
+        raise ValueError(f"map byte string version {version} is not supported")
+    n_opts = struct.calcsize("<" + "".join(f[1:] for _, f in _OPTS))
+    options_from_bytes(r.raw(n_opts), opts)
+    sets = []
+    for _ in range(r.take("<I")):
+        sets.append(multiset_from_bytes(r.raw(r.take("<Q"))))
+    inputDim, outputDim, coeffs = map_from_bytes(data[r.i:])
+    return bool(tri), sets, inputDim, outputDim, coeffs

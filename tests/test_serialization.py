@@ -61,3 +61,34 @@ def test_map_coefficient_archive(tmp_path):
     f.write_bytes(b)
     i, o, c2 = mt.DeserializeMap(str(f))
     assert (i, o) == (5, 2) and np.array_equal(c2, c)
+
+
+def test_map_byte_string_layout_and_round_trip():
+    # ToBytes' layout: magic, version, triangular flag, the MapOptions archive,
+    # the component count, each FixedMultiIndexSet archive with its length,
+    # then the map archive (host side; the GPU test round-trips real maps).
+    o = mt.MapOptions(basisType=mt.BasisTypes.PhysicistHermite, quadType=mt.QuadTypes.AdaptiveClenshawCurtis,
+                      quadPts=9, nugget=0.5, basisLB=-2.0, basisUB=3.0)
+    sets = [mt.FixedMultiIndexSet(1, 2), mt.FixedMultiIndexSet(2, 3)]
+    c = np.linspace(0.0, 1.0, sets[0].Size() + sets[1].Size())
+    b = ser.map_state_to_bytes(o, True, sets, 2, 2, c)
+    assert b[:8] == b"MPARTAMD" and struct.unpack_from("<IB", b, 8) == (1, 1)
+    assert b[13:13 + 66] == ser.options_to_bytes(o)
+    assert struct.unpack_from("<I", b, 79)[0] == 2
+    n0 = struct.unpack_from("<Q", b, 83)[0]
+    assert b[91:91 + n0] == ser.multiset_to_bytes(sets[0])
+    assert b.endswith(ser.map_to_bytes(2, 2, c))
+    o2 = mt.MapOptions()
+    tri, got, i, j, c2 = ser.map_state_from_bytes(b, o2)
+    assert tri and (i, j) == (2, 2) and np.array_equal(c2, c)
+    for s, (dim, st, dm, od) in zip(sets, got):
+        assert dim == s.dim and np.array_equal(st, s.nzStarts) and np.array_equal(dm, s.nzDims)
+        assert np.array_equal(od, s.nzOrders)
+    for k in mt.MapOptions._FIELDS:
+        assert getattr(o2, k) == getattr(o, k), k
+    try:
+        ser.map_state_from_bytes(b"\x00" * 40, o2)
+    except ValueError:
+        pass
+    else:
+        raise AssertionError("a foreign byte string must be refused")
