@@ -361,6 +361,8 @@ def profile_inverse_many_mode():
         out[f"schedule_{sched}_ms"] = (time.perf_counter() - t0) * 1e3
         out[f"schedule_{sched}_nan_roots"] = int(sum(int(torch.isnan(x).sum().item()) for x in xis))
         out[f"schedule_{sched}_maps_with_infinite_g"] = _maps_with_infinite_g(maps)
+        if sched == 1:
+            out["schedule_1_handover"] = b.InverseHandoverStats()
         log(f"inverse schedule {sched}: {out[f'schedule_{sched}_ms']:.1f} ms")
     print(json.dumps({"profile": "inverse_many_maps_full_ranges", **out}), flush=True)
 

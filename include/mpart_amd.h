@@ -410,6 +410,28 @@ int mpart_batch_inverse(mpart_batch batch, const mpart_cview* x1, const mpart_cv
  * the schedule (0: not tuned yet) and the trial times in ms of 1 and 2.
  * No counterpart in the reference. */
 int mpart_batch_set_inverse_schedule(mpart_batch batch, int32_t schedule);
+
+/* Hand-over of the longest inverse solves (adaptive Simpson rules; the
+ * per-lane solve scheduling of mpart_inverse and of batch schedule 1): a
+ * solve that has taken more than solveSteps quadrature interval steps, or
+ * one of whose quadratures has taken more than quadSteps (0: no such
+ * limit), is solved again by a pass that gives each such solve a whole
+ * wavefront and spreads each of its quadratures over the 64 lanes, level by
+ * level of the subdivision tree.  The trees, sums and roots are the same as
+ * the one-lane solve's, bit for bit; only the time differs.  solveSteps = 0
+ * turns the hand-over off.  Defaults 4096 and 512 (the median solve takes a
+ * few hundred steps).  No counterpart in the reference, whose solves run a
+ * thread each (MParT/MonotoneComponent.h:331-443). */
+int mpart_set_inverse_handover(mpart_map map, int32_t solveSteps, int32_t quadSteps);
+int mpart_batch_set_inverse_handover(mpart_batch batch, int32_t solveSteps, int32_t quadSteps);
+/* Totals of the last Inverse call with the hand-over on (waits for stream):
+ * stats[0] solves handed over, [1] their T evaluations, [2] intervals
+ * evaluated level by level, [3] T evaluations integrated depth-first because
+ * the tree outgrew the wavefront's workspace, [4] ... because an interval
+ * error was not finite, [5] the largest tree (intervals), [6] the deepest
+ * level.  No counterpart in the reference. */
+int mpart_inverse_handover_stats(mpart_map map, uint64_t stats[7], void* stream);
+int mpart_batch_inverse_handover_stats(mpart_batch batch, uint64_t stats[7], void* stream);
 int mpart_batch_inverse_schedule(mpart_batch batch, int32_t* schedule, double* trialMs);
 
 int mpart_set_codegen(mpart_map map, int enable);

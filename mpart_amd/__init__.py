@@ -124,6 +124,10 @@ _SIGS = {
     "mpart_batch_inverse": [_vp, _vp, _vp, _vp, _dbl, _dbl, _vp],
     "mpart_batch_set_point_group": [_vp, ctypes.c_int32],
     "mpart_batch_set_inverse_schedule": [_vp, ctypes.c_int32],
+    "mpart_set_inverse_handover": [_vp, ctypes.c_int32, ctypes.c_int32],
+    "mpart_batch_set_inverse_handover": [_vp, ctypes.c_int32, ctypes.c_int32],
+    "mpart_inverse_handover_stats": [_vp, _vp, _vp],
+    "mpart_batch_inverse_handover_stats": [_vp, _vp, _vp],
     "mpart_batch_inverse_schedule": [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)],
     "mpart_batch_eval_schedule": [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)],
     "mpart_codegen_prepare": [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)],
@@ -325,6 +329,16 @@ def _read_multiset(h, i):
 
 
 # ---------------------------------------------------------------------------
+_HANDOVER_KEYS = ("solves", "t_evaluations", "intervals", "fallback_size", "fallback_nonfinite", "max_intervals",
+                  "max_level")
+
+
+def _handover_stats(fn, h):
+    out = (ctypes.c_uint64 * len(_HANDOVER_KEYS))()
+    _check(fn(h, out, _stream()))
+    return dict(zip(_HANDOVER_KEYS, (int(v) for v in out)))
+
+
 def _torch():
     import torch
     return torch
@@ -486,6 +500,15 @@ class ConditionalMap:
         secs = ctypes.c_double(0.0)
         _check(_lib.mpart_codegen_prepare(self._h, ctypes.byref(spec), ctypes.byref(secs)))
         return bool(spec.value), float(secs.value), _lib.mpart_codegen_error(self._h).decode()
+
+    def SetInverseHandover(self, solveSteps=4096, quadSteps=512):
+        """Inverse solves longer than solveSteps quadrature steps (or with a quadrature longer than
+        quadSteps) are finished by a wavefront each (``mpart_set_inverse_handover``); 0 turns it off."""
+        _check(_lib.mpart_set_inverse_handover(self._h, int(solveSteps), int(quadSteps)))
+
+    def InverseHandoverStats(self):
+        """Totals of the last Inverse call with the hand-over on (``mpart_inverse_handover_stats``)."""
+        return _handover_stats(_lib.mpart_inverse_handover_stats, self._h)
 
     def SetPointGroup(self, lanes=0):
         """Lanes per point of the generic Evaluate / LogDeterminant / fused pass (1, 4, 8, 64; 0 = autotune;
@@ -767,6 +790,14 @@ class MapBatch:
         """Batched Inverse schedule: 1 per-lane solve scheduling over every map's
         points, component by component; 2 a lane per point; 0 autotune."""
         _check(_lib.mpart_batch_set_inverse_schedule(self._h, int(schedule)))
+
+    def SetInverseHandover(self, solveSteps=4096, quadSteps=512):
+        """ConditionalMap.SetInverseHandover for the batch's schedule-1 Inverse."""
+        _check(_lib.mpart_batch_set_inverse_handover(self._h, int(solveSteps), int(quadSteps)))
+
+    def InverseHandoverStats(self):
+        """ConditionalMap.InverseHandoverStats for the batch."""
+        return _handover_stats(_lib.mpart_batch_inverse_handover_stats, self._h)
 
     def InverseSchedule(self):
         """(schedule: 1, 2, or 0 before tuning; [trial ms of schedule 1, of schedule 2])."""

@@ -47,4 +47,26 @@ struct InvSeg {
     int* status;      // the map's MapDev::status
 };
 
+// Hand-over of the longest inverse solves to k_inverse_heavy (mono_inverse.hpp):
+// the handed-over point indices (count of them, at most cap kept), the step
+// budgets (solveSteps 0: off), and one workspace of COOP_WS_DOUBLES_PER_NODE *
+// wsNodes doubles per wavefront of that pass.
+struct InvHeavy {
+    unsigned long long* count;
+    long* list;
+    long cap;
+    int solveSteps, quadSteps;
+    double* ws;  // per-wavefront workspace, COOP_WS_DOUBLES_PER_NODE * wsNodes doubles each
+    int wsNodes;
+    // Totals over a call (zeroed by its first pass; null: not kept): solves
+    // handed over, their T evaluations, intervals evaluated level by level,
+    // T evaluations integrated depth-first because the tree outgrew the
+    // workspace or met a non-finite interval error, the largest tree
+    // (intervals) and the deepest level reached.
+    unsigned long long* stats;
+};
+enum { HS_SOLVES = 0, HS_TEVALS, HS_NODES, HS_FALLBACK_SIZE, HS_FALLBACK_NONFINITE, HS_MAX_NODES, HS_MAX_LEVELS,
+       HS_COUNT };
+constexpr int COOP_WS_DOUBLES_PER_NODE = 7;
+
 }  // namespace mpd

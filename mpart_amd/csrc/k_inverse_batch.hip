@@ -3,7 +3,14 @@
 
 namespace mpd {
 
-__global__ void k_zero_u64_batch(unsigned long long* p) { *p = 0ull; }
+// Zeroes the point counter and the hand-over count of a pass, and on the
+// call's first pass (component 0) the hand-over statistics.
+__global__ void k_zero_u64_batch(unsigned long long* p, unsigned long long* q, unsigned long long* st) {
+    *p = 0ull;
+    if (q) *q = 0ull;
+    if (st)
+        for (int k = 0; k < HS_COUNT; ++k) st[k] = 0ull;
+}
 
 template <int B, int Q, int R, int PM>
 static int inverse_batch_occupancy(int block) {
@@ -27,18 +34,23 @@ int occupancy_inverse_batch(int var, int block) {
 hipError_t launch_inverse_batch(int var, const Launch& L, int ntiles, dim3 qgrid, dim3 qblock, const MapDev& M0,
                                 const MapDev* maps, const InvSeg* segs, const int* tileSeg, const long* tileStart,
                                 const int* chunkSeg, long Ntot, int c, double xtol, double ytol, double* rec,
-                                unsigned long long* counter) {
+                                unsigned long long* counter, const InvHeavy& H, int heavyBlocks) {
     if (ntiles <= 0 || Ntot <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_zero_u64_batch, dim3(1), dim3(1), 0, L.stream, counter);
+    hipLaunchKernelGGL(k_zero_u64_batch, dim3(1), dim3(1), 0, L.stream, counter, H.count, c == 0 ? H.stats : nullptr);
 #define CALL(B, Q, R, PM)                                                                                           \
     hipLaunchKernelGGL((k_inverse_setup_batch<B, Q, R, PM>), dim3((unsigned)ntiles), L.block, L.smem, L.stream,     \
                        maps, segs, tileSeg, tileStart, Ntot, c, rec);                                               \
-    if (M0.lin)                                                                                                   \
+    if (M0.lin) {                                                                                                 \
         hipLaunchKernelGGL((k_inverse_queue<B, Q, R, PM, true, true>), qgrid, qblock, 0, L.stream, M0, rec, Ntot,   \
-                           nullptr, 0, c, xtol, ytol, counter, segs, chunkSeg);                                   \
-    else                                                                                                          \
+                           nullptr, 0, c, xtol, ytol, counter, segs, chunkSeg, H);                                \
+        launch_inverse_heavy<B, Q, R, PM, true, true>(heavyBlocks, L.stream, M0, rec, Ntot, nullptr, 0,              \
+            c, xtol, ytol, H, segs, chunkSeg);                                                                       \
+    } else {                                                                                                      \
         hipLaunchKernelGGL((k_inverse_queue<B, Q, R, PM, false, true>), qgrid, qblock, 0, L.stream, M0, rec, Ntot,  \
-                           nullptr, 0, c, xtol, ytol, counter, segs, chunkSeg)
+                           nullptr, 0, c, xtol, ytol, counter, segs, chunkSeg, H);                                \
+        launch_inverse_heavy<B, Q, R, PM, false, true>(heavyBlocks, L.stream, M0, rec, Ntot, nullptr, 0,             \
+            c, xtol, ytol, H, segs, chunkSeg);                                                                       \
+    }
     MPD_DISPATCH(var, CALL)
 #undef CALL
     return hipGetLastError();

@@ -2,6 +2,7 @@
 // translation unit per kernel family so they compile in parallel).
 #pragma once
 #include <hip/hip_runtime.h>
+#include "batch_io.hpp"
 #include "mono_device.hpp"
 
 namespace mpd {
@@ -70,11 +71,12 @@ hipError_t launch_input_vjp(int var, const Launch& L, const MapDev& M, const dou
                             double* gtab);
 // Inverse of component c: a setup pass (shape L) writes per-point records to
 // rec, then the persistent solve pass (qgrid x qblock) takes points from the
-// counter at rec + records.
+// counter at rec + records; with H's budgets set (adaptive Simpson), the
+// solves it hands over then run in heavyBlocks one-wavefront blocks.
 hipError_t launch_inverse(int var, const Launch& L, dim3 qgrid, dim3 qblock, const MapDev& M, const double* w, long ws,
                           const double* xs, long xrows, long xcols, long xrs, long xcs, const double* ys, long yss,
                           long N, double* out, long os, int c, double xtol, double ytol, double* gtab, double* rec,
-                          unsigned long long* counter);
+                          unsigned long long* counter, const InvHeavy& H, int heavyBlocks);
 int occupancy_inverse(int var, int block);
 int inverse_record_rows(int var);
 // Batched Inverse of component c over many maps: a counter reset, the setup
@@ -84,7 +86,7 @@ struct InvSeg;
 hipError_t launch_inverse_batch(int var, const Launch& L, int ntiles, dim3 qgrid, dim3 qblock, const MapDev& M0,
                                 const MapDev* maps, const InvSeg* segs, const int* tileSeg, const long* tileStart,
                                 const int* chunkSeg, long Ntot, int c, double xtol, double ytol, double* rec,
-                                unsigned long long* counter);
+                                unsigned long long* counter, const InvHeavy& H, int heavyBlocks);
 int occupancy_inverse_batch(int var, int block);
 hipError_t launch_fused(int var, const Launch& L, const MapDev& M, const double* w, long ws, const double* pts,
                         long prs, long pcs, long N, const double* sens, long srs, long scs, const double* lsens,

@@ -338,6 +338,11 @@ double* noalloc_scratch() {
 // ---------------------------------------------------------------------------
 // Schedules of the specialized fused pass the tuner times (gen_fused_launch).
 constexpr int GEN_FUSED_SCHEDULES = 4;
+// Inverse hand-over defaults and sizes (heavy_args).
+constexpr int HEAVY_SOLVE_STEPS = 4096;
+constexpr int HEAVY_QUAD_STEPS = 512;
+constexpr int HEAVY_WS_NODES = 16384;
+constexpr long HEAVY_MAX_BLOCKS = 128;
 
 struct mpart_map_s {
     mpart_options opts{};
@@ -366,7 +371,9 @@ struct mpart_map_s {
 
     // work buffers
     DBuf partialGrad, partialLoss, gtab, wT, wLd, wG, wLG, wFull, wCoeffs;
-    DBuf wInv;  // inverse: per-point records + the point counter
+    DBuf wInv;  // inverse: per-point records + the point counter and the hand-over count
+    DBuf wHeavy;  // inverse hand-over: list of handed-over solves + per-wavefront workspaces
+    int heavySolveSteps = HEAVY_SOLVE_STEPS, heavyQuadSteps = HEAVY_QUAD_STEPS;
     // A call on this map has been captured in a graph: from then on the work
     // buffers are never freed while the map lives (a larger eager call
     // allocates new ones and retires the old ones, which the graph reads).
@@ -376,7 +383,8 @@ struct mpart_map_s {
         hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
         if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs == hipStreamCaptureStatusNone) return;
         captured = true;
-        for (DBuf* b : {&ownCoeffs, &partialGrad, &partialLoss, &gtab, &wT, &wLd, &wG, &wLG, &wFull, &wCoeffs, &wInv})
+        for (DBuf* b : {&ownCoeffs, &partialGrad, &partialLoss, &gtab, &wT, &wLd, &wG, &wLG, &wFull, &wCoeffs, &wInv,
+                        &wHeavy})
             b->keep = true;
     }
 
@@ -1168,6 +1176,46 @@ void do_input_vjp(mpart_map_s& m, const mpart_cview& p, int klMode, const mpart_
               "launch input_vjp");
 }
 
+// Hand-over of the longest inverse solves (adaptive Simpson): a queue lane
+// hands a solve to the one-wavefront-per-solve pass once the solve has taken
+// HEAVY_SOLVE_STEPS interval steps or one of its quadratures HEAVY_QUAD_STEPS
+// (the median solve takes a few hundred); the list holds up to N / 16 (at
+// least 1 024) solves, and each of the heavyBlocks wavefronts a workspace of
+// HEAVY_WS_NODES intervals per quadrature (larger trees are integrated
+// depth-first by all lanes).
+long heavy_cap(long N) { return std::max<long>(1024, N / 16); }
+int heavy_blocks(long N) { return (int)std::min<long>(HEAVY_MAX_BLOCKS, std::max<long>(1, N / 1024)); }
+// Layout of the hand-over buffer: [statistics (256 B) | list | workspaces].
+constexpr size_t HEAVY_STATS_BYTES = 256;
+mpd::InvHeavy heavy_args(DBuf& buf, unsigned long long* count, long N, int solveSteps, int quadSteps, int& blocks) {
+    static_assert(mpd::HS_COUNT * sizeof(unsigned long long) <= HEAVY_STATS_BYTES, "statistics block");
+    mpd::InvHeavy H{};
+    H.count = count;
+    blocks = 0;
+    if (solveSteps <= 0) return H;
+    const long cap = heavy_cap(N);
+    blocks = heavy_blocks(N);
+    const size_t listBytes = ((size_t)cap * sizeof(long) + 255) / 256 * 256;
+    const size_t wsBytes = (size_t)blocks * mpd::COOP_WS_DOUBLES_PER_NODE * HEAVY_WS_NODES * sizeof(double);
+    buf.ensure(HEAVY_STATS_BYTES + listBytes + wsBytes);
+    H.stats = buf.as<unsigned long long>();
+    H.list = reinterpret_cast<long*>(buf.as<char>() + HEAVY_STATS_BYTES);
+    H.cap = cap;
+    H.solveSteps = solveSteps;
+    H.quadSteps = quadSteps;
+    H.ws = reinterpret_cast<double*>(buf.as<char>() + HEAVY_STATS_BYTES + listBytes);
+    H.wsNodes = HEAVY_WS_NODES;
+    return H;
+}
+
+// Statistics of the last Inverse call that used the hand-over (zeros if none).
+void heavy_stats(const DBuf& buf, uint64_t* out, hipStream_t s) {
+    for (int k = 0; k < mpd::HS_COUNT; ++k) out[k] = 0;
+    if (!buf.p) return;
+    hip_check(hipMemcpyAsync(out, buf.p, mpd::HS_COUNT * sizeof(uint64_t), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+}
+
 // Inverse of component c: a lane-per-point setup pass writes each point's
 // record, then a persistent pass solves with per-lane scheduling (lanes take
 // new points as their solves end).
@@ -1185,9 +1233,12 @@ void inverse_component(mpart_map_s& m, const Shape& sh, const double* xs, long x
     const int block = 256;
     const long grid = std::min<long>((N + block - 1) / block,
                                      (long)g_numCU * mpd::occupancy_inverse(m.variant, block));
+    int heavyBlocks = 0;
+    const mpd::InvHeavy H =
+        heavy_args(m.wHeavy, counter + 1, N, m.heavySolveSteps, m.heavyQuadSteps, heavyBlocks);
     hip_check(mpd::launch_inverse(m.variant, sh.L, dim3((unsigned)std::max<long>(1, grid)), dim3(block), m.dev,
                                   m.coeffs, m.coeffStride, xs, xrows, xcols, xrs, xcs, ys, yss, N, out, os, c, xtol,
-                                  ytol, sh.gtab, rec, counter),
+                                  ytol, sh.gtab, rec, counter, H, heavyBlocks),
               "launch inverse");
 }
 
@@ -1712,7 +1763,7 @@ struct InvPlan {
     std::vector<long> fullOff;  // per map: offset of its point buffer (-1: none)
     long fullElems = 0;
     long maxNtot = 0;
-    DBuf dSegs, dTileSeg, dTileStart, dChunkSeg, dCounters, wFull, wRec;
+    DBuf dSegs, dTileSeg, dTileStart, dChunkSeg, dCounters, wFull, wRec, wHeavy;
 };
 
 struct mpart_batch_s {
@@ -1734,6 +1785,7 @@ struct mpart_batch_s {
     // 2 a lane per point; 0 autotune (timed on the first uncaptured call with
     // enough points; untuned calls take 2).  Frozen once a call is captured.
     int invSchedule = 0;
+    int heavySolveSteps = HEAVY_SOLVE_STEPS, heavyQuadSteps = HEAVY_QUAD_STEPS;  // hand-over (inverse_component)
     double invTrialMs[2] = {0, 0};
     bool invCaptured = false;  // an Inverse call has been captured (its schedule is frozen)
     std::vector<int> mapLanes;
@@ -1978,7 +2030,8 @@ void batch_inverse(mpart_batch_s& b, const std::vector<mpd::BatchIO>& io, double
         P.wFull.ensure(std::max<long>(P.fullElems, 1) * sizeof(double));
         const int rows = mpd::inverse_record_rows(b.variant);
         P.wRec.ensure((size_t)rows * std::max<long>(P.maxNtot, 1) * sizeof(double));
-        P.dCounters.ensure(std::max<size_t>(P.passes.size(), 1) * sizeof(unsigned long long));
+        // a point counter per pass, then the hand-over count
+        P.dCounters.ensure((std::max<size_t>(P.passes.size(), 1) + 1) * sizeof(unsigned long long));
         double* fullBuf = P.wFull.as<double>();
         for (size_t k = 0; k < segs.size(); ++k) {
             if (xOff[k] >= 0) segs[k].x = fullBuf + xOff[k];
@@ -2026,6 +2079,9 @@ void batch_inverse(mpart_batch_s& b, const std::vector<mpd::BatchIO>& io, double
     const int qblock = 256;
     const long qcap = (long)g_numCU * mpd::occupancy_inverse_batch(b.variant, qblock);
     const auto* segs = P.dSegs.as<mpd::InvSeg>();
+    int heavyBlocks = 0;
+    const mpd::InvHeavy H = heavy_args(P.wHeavy, P.dCounters.as<unsigned long long>() + std::max<size_t>(P.passes.size(), 1),
+                                       P.maxNtot, b.heavySolveSteps, b.heavyQuadSteps, heavyBlocks);
     for (size_t c = 0; c < P.passes.size(); ++c) {
         const InvPlan::Pass& ps = P.passes[c];
         const long qgrid = std::max<long>(1, std::min<long>((ps.Ntot + qblock - 1) / qblock, qcap));
@@ -2034,7 +2090,8 @@ void batch_inverse(mpart_batch_s& b, const std::vector<mpd::BatchIO>& io, double
                                             b.maps[0]->dev, b.dMaps.as<MapDev>(), segs + ps.seg0,
                                             P.dTileSeg.as<int>() + ps.tile0, P.dTileStart.as<long>() + ps.tile0,
                                             P.dChunkSeg.as<int>() + ps.chunk0, ps.Ntot, (int)c, xtol, ytol,
-                                            P.wRec.as<double>(), P.dCounters.as<unsigned long long>() + c),
+                                            P.wRec.as<double>(), P.dCounters.as<unsigned long long>() + c, H,
+                                            heavyBlocks),
                   "launch batched inverse");
     }
 }
@@ -2849,6 +2906,39 @@ int mpart_batch_set_inverse_schedule(mpart_batch b, int32_t schedule) {
         if (b->invCaptured && schedule != b->invSchedule)
             throw ArgError("the batch's Inverse has been captured in a graph: its schedule can no longer change");
         b->invSchedule = schedule;
+    });
+}
+
+int mpart_set_inverse_handover(mpart_map m, int32_t solveSteps, int32_t quadSteps) {
+    return guarded([&] {
+        check_map(m);
+        if (solveSteps < 0 || quadSteps < 0) throw ArgError("hand-over budgets must be non-negative");
+        m->heavySolveSteps = solveSteps;
+        m->heavyQuadSteps = quadSteps;
+    });
+}
+
+int mpart_batch_set_inverse_handover(mpart_batch b, int32_t solveSteps, int32_t quadSteps) {
+    return guarded([&] {
+        if (!b) throw ArgError("null argument");
+        if (solveSteps < 0 || quadSteps < 0) throw ArgError("hand-over budgets must be non-negative");
+        b->heavySolveSteps = solveSteps;
+        b->heavyQuadSteps = quadSteps;
+    });
+}
+
+int mpart_inverse_handover_stats(mpart_map m, uint64_t* stats, void* stream) {
+    return guarded([&] {
+        check_map(m);
+        if (!stats) throw ArgError("null argument");
+        heavy_stats(m->wHeavy, stats, (hipStream_t)stream);
+    });
+}
+
+int mpart_batch_inverse_handover_stats(mpart_batch b, uint64_t* stats, void* stream) {
+    return guarded([&] {
+        if (!b || !stats) throw ArgError("null argument");
+        heavy_stats(b->inv.wHeavy, stats, (hipStream_t)stream);
     });
 }
 

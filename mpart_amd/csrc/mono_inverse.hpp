@@ -111,6 +111,281 @@ struct InvStk<QK_AS> {
 // Solver phases.
 enum { IP_START = 0, IP_DOWN = 1, IP_UP = 2, IP_ITP = 3 };
 
+// ---------------------------------------------------------------------------
+// Hand-over of the longest solves.  A few solves cost thousands of times the
+// median (T evaluated where it spans hundreds of orders of magnitude, each
+// quadrature subdividing into thousands of intervals); one lane running such
+// a solve is the whole tail of a pass.  With adaptive Simpson, a queue lane
+// whose solve exceeds solveSteps interval steps, or whose current quadrature
+// exceeds quadSteps, hands the point over: it appends the point index to
+// list (count is the list length; appends past cap are not made, and that
+// lane keeps its solve) and takes the next point.  k_inverse_heavy then runs
+// every handed-over solve again from its record, one wavefront per solve,
+// each T evaluation spread over the 64 lanes (coop_integrate_as).  Budgets of
+// 0 turn the hand-over off.
+
+// Workspace of one wavefront: per processed interval (node) of one
+// quadrature, in level order, its left end index, end and midpoint values,
+// fine sum, split rank, leaf count, leaf offset; and the leaves in
+// left-to-right order.
+struct CoopWs {
+    double *fa, *fm, *fb, *fine, *leaf;
+    unsigned* idx;
+    int *srank, *cnt, *off;
+    int cap;
+};
+__device__ __forceinline__ CoopWs coop_ws(double* base, int cap) {
+    CoopWs W;
+    W.fa = base;
+    W.fm = base + cap;
+    W.fb = base + 2L * cap;
+    W.fine = base + 3L * cap;
+    W.leaf = base + 4L * cap;
+    int* ib = reinterpret_cast<int*>(base + 5L * cap);
+    W.idx = reinterpret_cast<unsigned*>(ib);
+    W.srank = ib + cap;
+    W.cnt = ib + 2L * cap;
+    W.off = ib + 3L * cap;
+    W.cap = cap;
+    return W;
+}
+
+__device__ __forceinline__ double lane_bcast(double v, int src) {
+    const unsigned long long u = __double_as_longlong(v);
+    const unsigned lo = __builtin_amdgcn_readlane((unsigned)u, src);
+    const unsigned hi = __builtin_amdgcn_readlane((unsigned)(u >> 32), src);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// integrate_as (FL_VAL) computed by the 64 lanes of a one-wavefront block.
+// The tree is built level by level: every pending interval of a level is
+// evaluated at once (its two quarter points, a lane each), tested, and the
+// children of the intervals that fail are written to the next level in
+// left-to-right order.  The intervals are the dyadic ones the depth-first
+// loop visits (positions idx 2^-level are exact), with the same node values,
+// sums and tests, so the tree is the same.  The accepted intervals are then
+// ranked left to right (leaf counts bottom-up, offsets top-down) and summed
+// in that order from 0, as the depth-first loop adds them: the result is
+// the same bit for bit.  A non-finite interval error (where integrate_as
+// stops early) or a tree beyond the workspace falls back to integrate_as in
+// every lane; the level pass records nothing in the status word, the
+// fallback records what integrate_as records.  Every lane returns the same
+// value.  lvl: LDS, MAXSUB + 2 level starts.
+// Adds a hand-over statistic (lane 0; st may be null).
+__device__ __forceinline__ void hs_add(unsigned long long* st, int k, unsigned long long v) {
+    if (st && threadIdx.x == 0) atomicAdd(st + k, v);
+}
+__device__ __forceinline__ void hs_max(unsigned long long* st, int k, unsigned long long v) {
+    if (st && threadIdx.x == 0) atomicMax(st + k, v);
+}
+
+template <int BK, int PK, int PM>
+__device__ double coop_integrate_as(const QuadDev& Q, const DiagCoef<PM>& dcIn, const CoopWs& W, int* lvl,
+                                    unsigned long long* st) {
+#pragma clang fp contract(off)  // as in integrate_as
+    const int lane = threadIdx.x;
+    DiagCoef<PM> dc = dcIn;
+    dc.status = nullptr;
+    {
+        const double t = lane == 0 ? 0.0 : (lane == 1 ? 1.0 : 0.5);
+        const double v = eval_node<BK, PK, FL_VAL, PM>(dc, t).F;
+        const double fa = lane_bcast(v, 0), fb = lane_bcast(v, 1), fm = lane_bcast(v, 2);
+        if (lane == 0) {
+            W.idx[0] = 0u;
+            W.fa[0] = fa;
+            W.fm[0] = fm;
+            W.fb[0] = fb;
+            lvl[0] = 0;
+        }
+    }
+    __syncthreads();
+    int start = 0, n = 1, L = 0;
+    bool fallback = false, nonfinite = false;
+    while (n > 0) {
+        const int next = start + n;
+        int nsplit = 0;
+        for (int base = 0; base < n; base += 64) {
+            const int j = base + lane;
+            const bool act = j < n;
+            const int node = start + j;
+            unsigned id = 0u;
+            double a = 0.0, b = 1.0, fa = 0.0, fm = 0.0, fb = 0.0;
+            if (act) {
+                id = W.idx[node];
+                fa = W.fa[node];
+                fm = W.fm[node];
+                fb = W.fb[node];
+                a = ldexp((double)id, -L);
+                b = ldexp((double)(id + 1u), -L);
+            }
+            const double m = 0.5 * (a + b);
+            const double lm = 0.5 * (a + m);
+            const double rm = 0.5 * (m + b);
+            double flm = 0.0, frm = 0.0;
+            if (act) {
+                flm = eval_node<BK, PK, FL_VAL, PM>(dc, lm).F;
+                frm = eval_node<BK, PK, FL_VAL, PM>(dc, rm).F;
+            }
+            const double coarse = div6(b - a) * (fa + 4.0 * fm + fb);
+            const double wl = div6(m - a), wr = div6(b - m);
+            double fine = wl * (fa + 4.0 * flm + fm);
+            fine += wr * (fm + 4.0 * frm + fb);
+            const double err = fabs(fine - coarse);
+            if (__ballot(act && nonfinite_err(err))) {
+                fallback = nonfinite = true;
+                break;
+            }
+            const double tol = fmax(Q.relTol * fabs(coarse), Q.absTol);
+            const bool accept = act && (((err < tol) || (L == Q.maxSub - 1)) && (L >= Q.minSub));
+            const bool split = act && !accept;
+            const unsigned long long mask = __ballot(split);
+            const int ns = __popcll(mask);
+            if (next + 2 * (nsplit + ns) > W.cap || L + 1 > MAXSUB) {
+                fallback = true;
+                break;
+            }
+            const int rank =
+                __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+            if (act) {
+                W.fine[node] = fine;
+                W.srank[node] = accept ? -1 : nsplit + rank;
+            }
+            if (split) {
+                const int ch = next + 2 * (nsplit + rank);
+                W.idx[ch] = 2u * id;
+                W.fa[ch] = fa;
+                W.fm[ch] = flm;
+                W.fb[ch] = fm;
+                W.idx[ch + 1] = 2u * id + 1u;
+                W.fa[ch + 1] = fm;
+                W.fm[ch + 1] = frm;
+                W.fb[ch + 1] = fb;
+            }
+            nsplit += ns;
+        }
+        if (fallback) break;
+        ++L;
+        if (lane == 0) lvl[L] = next;
+        __syncthreads();
+        start = next;
+        n = 2 * nsplit;
+    }
+    hs_add(st, HS_TEVALS, 1);
+    hs_max(st, HS_MAX_LEVELS, (unsigned long long)L);
+    if (fallback) {
+        hs_add(st, nonfinite ? HS_FALLBACK_NONFINITE : HS_FALLBACK_SIZE, 1);
+        __syncthreads();
+        AdjAcc<PM> A;
+        return integrate_as<BK, PK, FL_VAL, PM>(Q, dcIn, A, nullptr, nullptr);
+    }
+    hs_add(st, HS_NODES, (unsigned long long)start);
+    hs_max(st, HS_MAX_NODES, (unsigned long long)start);
+    // Levels 0 .. L-1 hold the tree (level k at lvl[k] .. lvl[k+1]-1; the
+    // children of level k start at lvl[k+1]).  Leaf counts, bottom-up.
+    for (int k = L - 1; k >= 0; --k) {
+        const int s = lvl[k], e = lvl[k + 1];
+        for (int j = s + lane; j < e; j += 64) {
+            const int r = W.srank[j];
+            W.cnt[j] = (r < 0) ? 1 : W.cnt[e + 2 * r] + W.cnt[e + 2 * r + 1];
+        }
+        __syncthreads();
+    }
+    // Offsets of the leaves in left-to-right order, top-down.
+    if (lane == 0) W.off[0] = 0;
+    __syncthreads();
+    for (int k = 0; k < L; ++k) {
+        const int s = lvl[k], e = lvl[k + 1];
+        for (int j = s + lane; j < e; j += 64) {
+            const int r = W.srank[j];
+            const int o = W.off[j];
+            if (r < 0) {
+                W.leaf[o] = W.fine[j];
+            } else {
+                const int c = e + 2 * r;
+                W.off[c] = o;
+                W.off[c + 1] = o + W.cnt[c];
+            }
+        }
+        __syncthreads();
+    }
+    const int nLeaves = W.cnt[0];
+    double res = 0.0;
+    for (int k = 0; k < nLeaves; ++k) res += W.leaf[k];
+    __syncthreads();  // the workspace is rewritten by the next call
+    return res;
+}
+
+// The handed-over solves (see InvHeavy): one wavefront (block of 64) per
+// solve, blocks looping over the list.  Each solve runs from the point's
+// record with inverse_solve_with and the cooperative T; all lanes hold the
+// same values, and lane 0 writes the root.
+template <int BK, int PK, int QK, int PM, bool LIN, bool BATCH>
+__global__ void __launch_bounds__(64) k_inverse_heavy(MapDev M, const double* __restrict__ rec, long N, double* out,
+                                                      long os, int c, double xtol, double ytol, InvHeavy H,
+                                                      const InvSeg* __restrict__ segs,
+                                                      const int* __restrict__ chunkSeg) {
+    __shared__ int lvl[MAXSUB + 2];
+    const unsigned long long total = *H.count;
+    const long cnt = (long)(total < (unsigned long long)H.cap ? total : (unsigned long long)H.cap);
+    const CoopWs W = coop_ws(H.ws + (long)blockIdx.x * COOP_WS_DOUBLES_PER_NODE * H.wsNodes, H.wsNodes);
+    for (long e = blockIdx.x; e < cnt; e += gridDim.x) {
+        const long n = H.list[e];
+        hs_add(H.stats, HS_SOLVES, 1);
+        DiagCoef<PM> dc;
+        dc.nug = M.nugget;
+        dc.x = 0.0;
+        dc.lin = LIN ? M.lin : 0;
+        dc.lb = M.lb;
+        dc.ub = M.ub;
+        double* op = nullptr;
+        double* op2 = nullptr;
+        if constexpr (BATCH) {
+            const InvSeg& S = segs[chunkSeg[n / INV_CHUNK]];
+            const long local = n - S.base;
+            op = S.o + local * S.os;
+            op2 = S.o2 ? S.o2 + local * S.os2 : nullptr;
+            dc.p = S.p;
+            dc.status = S.status;
+        } else {
+            dc.p = ro(M.comps)[c].p;
+            dc.status = M.status;
+        }
+#pragma unroll
+        for (int q = 0; q <= PM; ++q) dc.cd[q] = rec[q * N + n];
+        const double f0 = rec[(PM + 1) * N + n];
+        const double x0 = rec[(PM + 2) * N + n];
+        const double yd = rec[(PM + 3) * N + n];
+        const double root = inverse_solve_with(
+            [&](double x) {
+                DiagCoef<PM> d = dc;
+                d.x = x;
+                return coop_integrate_as<BK, PK, PM>(M.quad, d, W, lvl, H.stats) + f0;
+            },
+            x0, yd, xtol, ytol);
+        if (threadIdx.x == 0) {
+            if constexpr (BATCH) {
+                *op = root;
+                if (op2) *op2 = root;
+            } else {
+                out[n * os] = root;
+            }
+        }
+    }
+}
+
+
+// Launches k_inverse_heavy after a queue pass (adaptive Simpson with the
+// hand-over on; no kernel is built for the other rules).
+template <int BK, int PK, int QK, int PM, bool LIN, bool BATCH>
+inline void launch_inverse_heavy(int blocks, hipStream_t s, const MapDev& M, const double* rec, long N, double* out,
+                                 long os, int c, double xtol, double ytol, const InvHeavy& H, const InvSeg* segs,
+                                 const int* chunkSeg) {
+    if constexpr (QK == QK_AS) {
+        if (H.solveSteps > 0 && blocks > 0)
+            hipLaunchKernelGGL((k_inverse_heavy<BK, PK, QK, PM, LIN, BATCH>), dim3((unsigned)blocks), dim3(64), 0, s, M,
+                               rec, N, out, os, c, xtol, ytol, H, segs, chunkSeg);
+    }
+}
 
 // LIN: the map has LinearizedBasis bounds.  Without them the kernel is built
 // with the bound checks folded away (they cost ~15 % of the integrand work).
@@ -123,7 +398,7 @@ template <int BK, int PK, int QK, int PM, bool LIN, bool BATCH>
 __global__ void __launch_bounds__(256) k_inverse_queue(MapDev M, const double* __restrict__ rec, long N, double* out,
                                                        long os, int c, double xtol, double ytol,
                                                        unsigned long long* counter, const InvSeg* __restrict__ segs,
-                                                       const int* __restrict__ chunkSeg) {
+                                                       const int* __restrict__ chunkSeg, InvHeavy H) {
     const QuadDev& Q = M.quad;
     const int p = BATCH ? 0 : ro(M.comps)[c].p;
     // Clenshaw-Curtis node table of one interval, in the order integrate_acc
@@ -199,8 +474,14 @@ __global__ void __launch_bounds__(256) k_inverse_queue(MapDev M, const double* _
     // Wavefront-uniform range of point indices not yet handed to a lane.
     long qNext = 0, qEnd = 0;
     bool exhausted = false;
+    // Interval steps of the current solve and quadrature; whether this solve
+    // may still be handed over (InvHeavy).
+    const bool handOver = (QK == QK_AS) && H.solveSteps > 0;
+    int solveSt = 0, quadSt = 0;
+    bool mayHand = false;
 
     auto start_quad = [&](double x) {
+        quadSt = 0;
         dc.x = x;
         qa = 0.0;
         qb = 1.0;
@@ -208,6 +489,41 @@ __global__ void __launch_bounds__(256) k_inverse_queue(MapDev M, const double* _
         level = 0;
         sp = 0;
         k = (QK == QK_AS) ? -2 : -1;
+    };
+
+    // ITP steps from T(xc) = yc: the bracket update and the next iterate,
+    // until the solve is done or an iterate needs a quadrature; iterates at
+    // an end of the bracket take the value held for it (same_bits).
+    auto itp_steps = [&](double yc, bool& done, double& root) {
+        while (true) {
+            if (fabs(yc - yd) < ytol) {
+                done = true;
+                root = xc;
+                return;
+            }
+            if (yc > yd) {
+                xub = xc;
+                yub = yc;
+            } else {
+                xlb = xc;
+                ylb = yc;
+            }
+            if (((xub - xlb) < xtol) || ((yub - ylb) < ytol) || (unsigned)(it + 1) >= ITP_MAX_ITS) {
+                done = true;
+                root = 0.5 * (xub + xlb);
+                return;
+            }
+            ++it;
+            xc = itp_point(xlb, xub, ylb, yub, nhalf, it, xtol);
+            if (same_bits(xc, xlb)) {
+                yc = ylb;
+            } else if (same_bits(xc, xub)) {
+                yc = yub;
+            } else {
+                start_quad(xc);
+                return;
+            }
+        }
     };
 
     while (true) {
@@ -265,6 +581,8 @@ __global__ void __launch_bounds__(256) k_inverse_queue(MapDev M, const double* _
                         yd = rec[(PM + 3) * N + n];
                         busy = true;
                         phase = IP_START;
+                        solveSt = 0;
+                        mayHand = handOver;
                         start_quad(x0);
                     }
                 }
@@ -279,6 +597,19 @@ __global__ void __launch_bounds__(256) k_inverse_queue(MapDev M, const double* _
         }
         if (__ballot(busy) == 0) break;
         if (!busy) continue;
+        if (mayHand) {
+            ++solveSt;
+            ++quadSt;
+            if (solveSt > H.solveSteps || (H.quadSteps > 0 && quadSt > H.quadSteps)) {
+                const unsigned long long slot = atomicAdd(H.count, 1ull);
+                if (slot < (unsigned long long)H.cap) {
+                    H.list[slot] = n;  // k_inverse_heavy writes this root
+                    busy = false;
+                    continue;
+                }
+                mayHand = false;  // the list is full: the solve stays here
+            }
+        }
 
         // One step of this lane's quadrature: the integrand at every node of
         // its current interval (E evaluations, independent of each other),
@@ -523,31 +854,13 @@ __global__ void __launch_bounds__(256) k_inverse_queue(MapDev M, const double* _
                     it = 0;
                     nhalf = itp_nhalf(xlb, xub, xtol);
                     xc = itp_point(xlb, xub, ylb, yub, nhalf, it, xtol);
-                    start_quad(xc);
+                    if (same_bits(xc, xlb)) itp_steps(ylb, done, root);
+                    else if (same_bits(xc, xub)) itp_steps(yub, done, root);
+                    else start_quad(xc);
                 }
             }
         } else {
-            const double yc = y;
-            if (fabs(yc - yd) < ytol) {
-                done = true;
-                root = xc;
-            } else {
-                if (yc > yd) {
-                    xub = xc;
-                    yub = yc;
-                } else {
-                    xlb = xc;
-                    ylb = yc;
-                }
-                if (((xub - xlb) < xtol) || ((yub - ylb) < ytol) || (unsigned)(it + 1) >= ITP_MAX_ITS) {
-                    done = true;
-                    root = 0.5 * (xub + xlb);
-                } else {
-                    ++it;
-                    xc = itp_point(xlb, xub, ylb, yub, nhalf, it, xtol);
-                    start_quad(xc);
-                }
-            }
+            itp_steps(y, done, root);
         }
         if (fail || done) {
             const double v = fail ? NAN : root;

@@ -623,6 +623,17 @@ __device__ double eval_T(const MapDev& M, const DiagCoef<PM>& dc0, double f0, do
 // a product and a power of two (pow(w, 2) and pow(2, k) are exactly these in
 // a correctly rounded libm).
 constexpr unsigned ITP_MAX_ITS = 10000;
+
+// An ITP iterate equal (bit for bit) to an end of the bracket needs no
+// quadrature: T there is the value computed when that end was set, and the
+// same evaluation gives the same bits.  Where the bracket has shrunk to two
+// neighbouring doubles wider than xtol (roots of large magnitude: at 1e13 an
+// ulp is 2e-3), every iterate is such an end and the reference's loop
+// (RootFinding.h:158-174) runs to its 10 000 iterations without moving; the
+// solvers here take those iterations too, without integrating T again.
+__device__ __forceinline__ bool same_bits(double a, double b) {
+    return __double_as_longlong(a) == __double_as_longlong(b);
+}
 __device__ __forceinline__ double itp_nhalf(double xlb, double xub, double xtol) {
     return ceil(log2(0.5 * (xub - xlb) / xtol));
 }
@@ -643,30 +654,28 @@ __device__ __forceinline__ double itp_point(double xlb, double xub, double ylb, 
 }
 
 // Bracket (FindBracket, RootFinding.h:20-80) and ITP (Find_x_ITP, 82-178) for
-// T(x) = yd with T given by its diagonal coefficients dc (x_k replaced by the
-// iterate) and f0 = f(x_<k, 0).  Returns NaN when no bracket is found, and
-// when T is not finite at an iterate (with finite inputs, an infinite g,
-// where the reference fails).
-template <int BK, int PK, int QK, int PM>
-__device__ double inverse_solve(const MapDev& M, const DiagCoef<PM>& dc, double f0, double x0, double yd, double xtol,
-                                double ytol) {
+// T(x) = yd, T(x) = evT(x), starting at x0.  Returns NaN when no bracket is
+// found, and when T is not finite at an iterate (with finite inputs, an
+// infinite g, where the reference fails).  The reference evaluates T(x0)
+// three times (InverseSingleBracket, then FindBracket twice); the values are
+// identical, so it is evaluated once.
+template <class EvT>
+__device__ double inverse_solve_with(EvT&& evT, double x0, double yd, double xtol, double ytol) {
     double xlb = x0, xub = x0;
-    double ylb = eval_T<BK, PK, QK, PM>(M, dc, f0, xlb);
+    double ylb = evT(xlb);
     if (nonfinite_err(fabs(ylb))) return NAN;
     double yub = ylb;
     int info = 0;
     {
         const unsigned maxIts = 128;
         double stepSize = 1.0;
-        ylb = eval_T<BK, PK, QK, PM>(M, dc, f0, xlb);
-        yub = eval_T<BK, PK, QK, PM>(M, dc, f0, xub);
         if (ylb > yd) {
             double t = ylb; ylb = yub; yub = t;
             t = xlb; xlb = xub; xub = t;
             unsigned i;
             for (i = 0; i < maxIts; ++i) {
                 xlb = xub - stepSize;
-                ylb = eval_T<BK, PK, QK, PM>(M, dc, f0, xlb);
+                ylb = evT(xlb);
                 if (nonfinite_err(fabs(ylb))) return NAN;
                 if ((fabs((yub - ylb) / (xub - xlb)) < 1e-12) && ((xub - xlb) > 10)) { info = -1; break; }
                 if (ylb > yd) { yub = ylb; xub = xlb; stepSize *= 2.0; } else break;
@@ -676,7 +685,7 @@ __device__ double inverse_solve(const MapDev& M, const DiagCoef<PM>& dc, double 
             unsigned i;
             for (i = 0; i < maxIts; ++i) {
                 xub = xlb + stepSize;
-                yub = eval_T<BK, PK, QK, PM>(M, dc, f0, xub);
+                yub = evT(xub);
                 if (nonfinite_err(fabs(yub))) return NAN;
                 if ((fabs((yub - ylb) / (xub - xlb)) < 1e-12) && ((xub - xlb) > 10)) { info = -1; break; }
                 if (yub < yd) { ylb = yub; xlb = xub; stepSize *= 2.0; } else break;
@@ -688,7 +697,8 @@ __device__ double inverse_solve(const MapDev& M, const DiagCoef<PM>& dc, double 
     const double nhalf = itp_nhalf(xlb, xub, xtol);
     for (unsigned it = 0; it < ITP_MAX_ITS; ++it) {
         const double xc = itp_point(xlb, xub, ylb, yub, nhalf, (int)it, xtol);
-        const double yc = eval_T<BK, PK, QK, PM>(M, dc, f0, xc);
+        // T at an end of the bracket is the value held for that end (itp_known)
+        const double yc = same_bits(xc, xlb) ? ylb : (same_bits(xc, xub) ? yub : evT(xc));
         if (nonfinite_err(fabs(yc))) return NAN;
         if (fabs(yc - yd) < ytol) {
             return xc;
@@ -702,6 +712,12 @@ __device__ double inverse_solve(const MapDev& M, const DiagCoef<PM>& dc, double 
         if (((xub - xlb) < xtol) || ((yub - ylb) < ytol)) break;
     }
     return 0.5 * (xub + xlb);
+}
+
+template <int BK, int PK, int QK, int PM>
+__device__ double inverse_solve(const MapDev& M, const DiagCoef<PM>& dc, double f0, double x0, double yd, double xtol,
+                                double ytol) {
+    return inverse_solve_with([&](double x) { return eval_T<BK, PK, QK, PM>(M, dc, f0, x); }, x0, yd, xtol, ytol);
 }
 
 // ---------------------------------------------------------------------------
