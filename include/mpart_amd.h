@@ -411,15 +411,17 @@ int mpart_batch_inverse(mpart_batch batch, const mpart_cview* x1, const mpart_cv
  * No counterpart in the reference. */
 int mpart_batch_set_inverse_schedule(mpart_batch batch, int32_t schedule);
 
-/* Hand-over of the longest inverse solves (adaptive Simpson rules; the
- * per-lane solve scheduling of mpart_inverse and of batch schedule 1): a
- * solve that has taken more than solveSteps quadrature interval steps, or
+/* Hand-over of the longest inverse solves (adaptive quadrature rules; the
+ * per-lane solve scheduling of mpart_inverse and of batch schedule 1): once
+ * a wavefront of the solve pass finds no point left to take, a solve of it
+ * that has taken more than solveSteps quadrature interval steps, or
  * one of whose quadratures has taken more than quadSteps (0: no such
  * limit), is solved again by a pass that gives each such solve a whole
  * wavefront and spreads each of its quadratures over the 64 lanes, level by
- * level of the subdivision tree.  The trees, sums and roots are the same as
+ * level of the subdivision tree (adaptive Simpson and adaptive
+ * Clenshaw-Curtis; the fixed rule has no such tail).  The trees, sums and roots are the same as
  * the one-lane solve's, bit for bit; only the time differs.  solveSteps = 0
- * turns the hand-over off.  Defaults 4096 and 512 (the median solve takes a
+ * turns the hand-over off.  Defaults 2048 and 512 (the median solve takes a
  * few hundred steps).  No counterpart in the reference, whose solves run a
  * thread each (MParT/MonotoneComponent.h:331-443). */
 int mpart_set_inverse_handover(mpart_map map, int32_t solveSteps, int32_t quadSteps);

@@ -501,9 +501,10 @@ class ConditionalMap:
         _check(_lib.mpart_codegen_prepare(self._h, ctypes.byref(spec), ctypes.byref(secs)))
         return bool(spec.value), float(secs.value), _lib.mpart_codegen_error(self._h).decode()
 
-    def SetInverseHandover(self, solveSteps=4096, quadSteps=512):
-        """Inverse solves longer than solveSteps quadrature steps (or with a quadrature longer than
-        quadSteps) are finished by a wavefront each (``mpart_set_inverse_handover``); 0 turns it off."""
+    def SetInverseHandover(self, solveSteps=2048, quadSteps=512):
+        """At the end of an Inverse pass, solves longer than solveSteps quadrature steps (or with a
+        quadrature longer than quadSteps) are finished by a wavefront each
+        (``mpart_set_inverse_handover``); 0 turns it off."""
         _check(_lib.mpart_set_inverse_handover(self._h, int(solveSteps), int(quadSteps)))
 
     def InverseHandoverStats(self):
@@ -791,7 +792,7 @@ class MapBatch:
         points, component by component; 2 a lane per point; 0 autotune."""
         _check(_lib.mpart_batch_set_inverse_schedule(self._h, int(schedule)))
 
-    def SetInverseHandover(self, solveSteps=4096, quadSteps=512):
+    def SetInverseHandover(self, solveSteps=2048, quadSteps=512):
         """ConditionalMap.SetInverseHandover for the batch's schedule-1 Inverse."""
         _check(_lib.mpart_batch_set_inverse_handover(self._h, int(solveSteps), int(quadSteps)))
 

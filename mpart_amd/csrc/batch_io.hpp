@@ -48,16 +48,13 @@ struct InvSeg {
 };
 
 // Hand-over of the longest inverse solves to k_inverse_heavy (mono_inverse.hpp):
-// the handed-over point indices (count of them, at most cap kept), the step
-// budgets (solveSteps 0: off), and one workspace of COOP_WS_DOUBLES_PER_NODE *
-// wsNodes doubles per wavefront of that pass.
+// the handed-over point indices (count of them, at most cap kept) and the
+// step budgets (solveSteps 0: off).
 struct InvHeavy {
     unsigned long long* count;
     long* list;
     long cap;
     int solveSteps, quadSteps;
-    double* ws;  // per-wavefront workspace, COOP_WS_DOUBLES_PER_NODE * wsNodes doubles each
-    int wsNodes;
     // Totals over a call (zeroed by its first pass; null: not kept): solves
     // handed over, their T evaluations, intervals evaluated level by level,
     // T evaluations integrated depth-first because the tree outgrew the
@@ -67,6 +64,9 @@ struct InvHeavy {
 };
 enum { HS_SOLVES = 0, HS_TEVALS, HS_NODES, HS_FALLBACK_SIZE, HS_FALLBACK_NONFINITE, HS_MAX_NODES, HS_MAX_LEVELS,
        HS_COUNT };
-constexpr int COOP_WS_DOUBLES_PER_NODE = 7;
+// Intervals one cooperative quadrature keeps in LDS (56 bytes each; a larger
+// tree is integrated depth-first instead).
+constexpr int COOP_NODES = 1024;
+constexpr int COOP_LDS_BYTES = COOP_NODES * 56;
 
 }  // namespace mpd

@@ -339,10 +339,9 @@ double* noalloc_scratch() {
 // Schedules of the specialized fused pass the tuner times (gen_fused_launch).
 constexpr int GEN_FUSED_SCHEDULES = 4;
 // Inverse hand-over defaults and sizes (heavy_args).
-constexpr int HEAVY_SOLVE_STEPS = 4096;
+constexpr int HEAVY_SOLVE_STEPS = 2048;
 constexpr int HEAVY_QUAD_STEPS = 512;
-constexpr int HEAVY_WS_NODES = 16384;
-constexpr long HEAVY_MAX_BLOCKS = 128;
+constexpr long HEAVY_MAX_BLOCKS = 1024;
 
 struct mpart_map_s {
     mpart_options opts{};
@@ -1176,16 +1175,17 @@ void do_input_vjp(mpart_map_s& m, const mpart_cview& p, int klMode, const mpart_
               "launch input_vjp");
 }
 
-// Hand-over of the longest inverse solves (adaptive Simpson): a queue lane
-// hands a solve to the one-wavefront-per-solve pass once the solve has taken
-// HEAVY_SOLVE_STEPS interval steps or one of its quadratures HEAVY_QUAD_STEPS
-// (the median solve takes a few hundred); the list holds up to N / 16 (at
-// least 1 024) solves, and each of the heavyBlocks wavefronts a workspace of
-// HEAVY_WS_NODES intervals per quadrature (larger trees are integrated
-// depth-first by all lanes).
+// Hand-over of the longest inverse solves (adaptive rules): once no point is
+// left to take, a queue lane hands a solve to the one-wavefront-per-solve
+// pass if the solve has taken HEAVY_SOLVE_STEPS interval steps or one of its
+// quadratures HEAVY_QUAD_STEPS (the median solve takes a few hundred); the
+// list holds up to N / 16 (at least 1 024) solves, taken by heavyBlocks
+// wavefronts (one per 1 024 points, at most HEAVY_MAX_BLOCKS), each with an
+// LDS workspace of COOP_NODES intervals per quadrature (larger trees are
+// integrated depth-first by all lanes).
 long heavy_cap(long N) { return std::max<long>(1024, N / 16); }
 int heavy_blocks(long N) { return (int)std::min<long>(HEAVY_MAX_BLOCKS, std::max<long>(1, N / 1024)); }
-// Layout of the hand-over buffer: [statistics (256 B) | list | workspaces].
+// Layout of the hand-over buffer: [statistics (256 B) | list].
 constexpr size_t HEAVY_STATS_BYTES = 256;
 mpd::InvHeavy heavy_args(DBuf& buf, unsigned long long* count, long N, int solveSteps, int quadSteps, int& blocks) {
     static_assert(mpd::HS_COUNT * sizeof(unsigned long long) <= HEAVY_STATS_BYTES, "statistics block");
@@ -1195,16 +1195,12 @@ mpd::InvHeavy heavy_args(DBuf& buf, unsigned long long* count, long N, int solve
     if (solveSteps <= 0) return H;
     const long cap = heavy_cap(N);
     blocks = heavy_blocks(N);
-    const size_t listBytes = ((size_t)cap * sizeof(long) + 255) / 256 * 256;
-    const size_t wsBytes = (size_t)blocks * mpd::COOP_WS_DOUBLES_PER_NODE * HEAVY_WS_NODES * sizeof(double);
-    buf.ensure(HEAVY_STATS_BYTES + listBytes + wsBytes);
+    buf.ensure(HEAVY_STATS_BYTES + (size_t)cap * sizeof(long));
     H.stats = buf.as<unsigned long long>();
     H.list = reinterpret_cast<long*>(buf.as<char>() + HEAVY_STATS_BYTES);
     H.cap = cap;
     H.solveSteps = solveSteps;
     H.quadSteps = quadSteps;
-    H.ws = reinterpret_cast<double*>(buf.as<char>() + HEAVY_STATS_BYTES + listBytes);
-    H.wsNodes = HEAVY_WS_NODES;
     return H;
 }
 

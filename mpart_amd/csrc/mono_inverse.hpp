@@ -115,19 +115,22 @@ enum { IP_START = 0, IP_DOWN = 1, IP_UP = 2, IP_ITP = 3 };
 // Hand-over of the longest solves.  A few solves cost thousands of times the
 // median (T evaluated where it spans hundreds of orders of magnitude, each
 // quadrature subdividing into thousands of intervals); one lane running such
-// a solve is the whole tail of a pass.  With adaptive Simpson, a queue lane
-// whose solve exceeds solveSteps interval steps, or whose current quadrature
-// exceeds quadSteps, hands the point over: it appends the point index to
-// list (count is the list length; appends past cap are not made, and that
-// lane keeps its solve) and takes the next point.  k_inverse_heavy then runs
+// a solve is the whole tail of a pass.  With an adaptive rule, once the
+// wavefront finds no point left to take, a queue lane whose solve exceeds
+// solveSteps interval steps, or whose current quadrature exceeds quadSteps,
+// hands the point over: it appends the point index to list (count is the
+// list length; appends past cap are not made, and that lane keeps its
+// solve) and goes idle.  (Long solves met earlier stay in their lanes: the
+// other lanes keep taking points meanwhile, and a wavefront spends more
+// time on a solve of a small tree than its lanes' share of it.)  k_inverse_heavy then runs
 // every handed-over solve again from its record, one wavefront per solve,
-// each T evaluation spread over the 64 lanes (coop_integrate_as).  Budgets of
+// each T evaluation spread over the 64 lanes (coop_integrate).  Budgets of
 // 0 turn the hand-over off.
 
-// Workspace of one wavefront: per processed interval (node) of one
-// quadrature, in level order, its left end index, end and midpoint values,
-// fine sum, split rank, leaf count, leaf offset; and the leaves in
-// left-to-right order.
+// Workspace of one wavefront (LDS, COOP_LDS_BYTES): per processed interval
+// (node) of one quadrature, in level order, its left end index, end and
+// midpoint values, fine sum, split rank, leaf count, leaf offset; and the
+// leaves in left-to-right order.
 struct CoopWs {
     double *fa, *fm, *fb, *fine, *leaf;
     unsigned* idx;
@@ -135,6 +138,7 @@ struct CoopWs {
     int cap;
 };
 __device__ __forceinline__ CoopWs coop_ws(double* base, int cap) {
+    static_assert(COOP_LDS_BYTES == COOP_NODES * (5 * sizeof(double) + 4 * sizeof(int)), "workspace layout");
     CoopWs W;
     W.fa = base;
     W.fm = base + cap;
@@ -157,20 +161,6 @@ __device__ __forceinline__ double lane_bcast(double v, int src) {
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-// integrate_as (FL_VAL) computed by the 64 lanes of a one-wavefront block.
-// The tree is built level by level: every pending interval of a level is
-// evaluated at once (its two quarter points, a lane each), tested, and the
-// children of the intervals that fail are written to the next level in
-// left-to-right order.  The intervals are the dyadic ones the depth-first
-// loop visits (positions idx 2^-level are exact), with the same node values,
-// sums and tests, so the tree is the same.  The accepted intervals are then
-// ranked left to right (leaf counts bottom-up, offsets top-down) and summed
-// in that order from 0, as the depth-first loop adds them: the result is
-// the same bit for bit.  A non-finite interval error (where integrate_as
-// stops early) or a tree beyond the workspace falls back to integrate_as in
-// every lane; the level pass records nothing in the status word, the
-// fallback records what integrate_as records.  Every lane returns the same
-// value.  lvl: LDS, MAXSUB + 2 level starts.
 // Adds a hand-over statistic (lane 0; st may be null).
 __device__ __forceinline__ void hs_add(unsigned long long* st, int k, unsigned long long v) {
     if (st && threadIdx.x == 0) atomicAdd(st + k, v);
@@ -179,14 +169,67 @@ __device__ __forceinline__ void hs_max(unsigned long long* st, int k, unsigned l
     if (st && threadIdx.x == 0) atomicMax(st + k, v);
 }
 
+// One interval [a, b] of the nested Clenshaw-Curtis rule, as integrate_acc
+// and integrate_acc_big form it: the midpoint value (returned in fm) and the
+// coarse and fine sums, the nodes and terms in their order.
 template <int BK, int PK, int PM>
-__device__ double coop_integrate_as(const QuadDev& Q, const DiagCoef<PM>& dcIn, const CoopWs& W, int* lvl,
-                                    unsigned long long* st) {
+__device__ __forceinline__ void acc_interval(const QuadDev& Q, const DiagCoef<PM>& dc, double a, double b, double fa,
+                                             double fb, double& fm, double& coarse, double& fine) {
+#pragma clang fp contract(off)  // as in integrate_acc
+    const int nf = Q.nf, nc = Q.nc;
+    const int cRight = nc - 1, cMid = cRight / 2;
+    const int fRight = nf - 1, fMid = fRight / 2;
+    const double* cw = Q.big ? Q.big : Q.cw;
+    const double* cp = Q.big ? Q.big + nc : Q.cp;
+    const double* fw = Q.big ? Q.big + 2 * nc : Q.fw;
+    const double* fp = Q.big ? Q.big + 2 * nc + nf : Q.fp;
+    const double mid = 0.5 * (a + b);
+    const double scale = 0.5 * (b - a);
+    fm = eval_node<BK, PK, FL_VAL, PM>(dc, mid).F;
+    coarse = scale * cw[0] * fa;
+    coarse += scale * cw[cMid] * fm;
+    coarse += scale * cw[cRight] * fb;
+    fine = scale * fw[0] * fa;
+    fine += scale * fw[fMid] * fm;
+    fine += scale * fw[fRight] * fb;
+    for (int i = 1; i < cMid; ++i) {
+        const double v = eval_node<BK, PK, FL_VAL, PM>(dc, mid + scale * cp[i]).F;
+        coarse += scale * cw[i] * v;
+        fine += scale * fw[2 * i] * v;
+    }
+    for (int ii = cMid + 1; ii < cRight; ++ii) {
+        const double v = eval_node<BK, PK, FL_VAL, PM>(dc, mid + scale * cp[ii]).F;
+        coarse += scale * cw[ii] * v;
+        fine += scale * fw[2 * ii] * v;
+    }
+    for (int i = 1; i < nf; i += 2) fine += scale * fw[i] * eval_node<BK, PK, FL_VAL, PM>(dc, mid + scale * fp[i]).F;
+}
+
+// The adaptive rules (integrate_as, integrate_acc; FL_VAL) computed by the 64
+// lanes of a one-wavefront block.  The tree is built level by level: every
+// pending interval of a level is evaluated at once (a lane each: Simpson's
+// two quarter points, or the nested rule's midpoint and interior nodes),
+// tested, and the children of the intervals that fail are written to the
+// next level in left-to-right order.  The intervals are the dyadic ones the
+// depth-first loop visits (positions idx 2^-level are exact), with the same
+// node values, sums and tests, so the tree is the same.  The accepted
+// intervals are then ranked left to right (leaf counts bottom-up, offsets
+// top-down) and summed in that order from 0, as the depth-first loop adds
+// them: the result is the same bit for bit.  A non-finite interval error
+// (where the depth-first loop stops early) or a tree beyond the workspace
+// falls back to the depth-first loop in every lane; the level pass records
+// nothing in the status word, the fallback records what the loop records.
+// Every lane returns the same value.  lvl: LDS, MAXSUB + 2 level starts.
+template <int BK, int PK, int QK, int PM>
+__device__ double coop_integrate(const QuadDev& Q, const DiagCoef<PM>& dcIn, const CoopWs& W, int* lvl,
+                                 unsigned long long* st) {
 #pragma clang fp contract(off)  // as in integrate_as
+    constexpr bool AS = (QK == QK_AS);
     const int lane = threadIdx.x;
     DiagCoef<PM> dc = dcIn;
     dc.status = nullptr;
     {
+        // the depth-first loops evaluate F(0), F(1) and (Simpson) F(1/2) first
         const double t = lane == 0 ? 0.0 : (lane == 1 ? 1.0 : 0.5);
         const double v = eval_node<BK, PK, FL_VAL, PM>(dc, t).F;
         const double fa = lane_bcast(v, 0), fb = lane_bcast(v, 1), fm = lane_bcast(v, 2);
@@ -213,23 +256,27 @@ __device__ double coop_integrate_as(const QuadDev& Q, const DiagCoef<PM>& dcIn, 
             if (act) {
                 id = W.idx[node];
                 fa = W.fa[node];
-                fm = W.fm[node];
+                if constexpr (AS) fm = W.fm[node];
                 fb = W.fb[node];
                 a = ldexp((double)id, -L);
                 b = ldexp((double)(id + 1u), -L);
             }
-            const double m = 0.5 * (a + b);
-            const double lm = 0.5 * (a + m);
-            const double rm = 0.5 * (m + b);
-            double flm = 0.0, frm = 0.0;
-            if (act) {
-                flm = eval_node<BK, PK, FL_VAL, PM>(dc, lm).F;
-                frm = eval_node<BK, PK, FL_VAL, PM>(dc, rm).F;
+            double coarse = 0.0, fine = 0.0, f1 = 0.0, f2 = 0.0;  // Simpson: the quarter points; else f1 = midpoint
+            if constexpr (AS) {
+                const double m = 0.5 * (a + b);
+                const double lm = 0.5 * (a + m);
+                const double rm = 0.5 * (m + b);
+                if (act) {
+                    f1 = eval_node<BK, PK, FL_VAL, PM>(dc, lm).F;
+                    f2 = eval_node<BK, PK, FL_VAL, PM>(dc, rm).F;
+                }
+                coarse = div6(b - a) * (fa + 4.0 * fm + fb);
+                const double wl = div6(m - a), wr = div6(b - m);
+                fine = wl * (fa + 4.0 * f1 + fm);
+                fine += wr * (fm + 4.0 * f2 + fb);
+            } else {
+                if (act) acc_interval<BK, PK, PM>(Q, dc, a, b, fa, fb, f1, coarse, fine);
             }
-            const double coarse = div6(b - a) * (fa + 4.0 * fm + fb);
-            const double wl = div6(m - a), wr = div6(b - m);
-            double fine = wl * (fa + 4.0 * flm + fm);
-            fine += wr * (fm + 4.0 * frm + fb);
             const double err = fabs(fine - coarse);
             if (__ballot(act && nonfinite_err(err))) {
                 fallback = nonfinite = true;
@@ -251,15 +298,19 @@ __device__ double coop_integrate_as(const QuadDev& Q, const DiagCoef<PM>& dcIn, 
                 W.srank[node] = accept ? -1 : nsplit + rank;
             }
             if (split) {
+                // Simpson: [a, m] holds (fa, f(lm), fm), [m, b] (fm, f(rm), fb);
+                // the nested rule: [a, mid] holds (fa, f(mid)), [mid, b] (f(mid), fb)
                 const int ch = next + 2 * (nsplit + rank);
                 W.idx[ch] = 2u * id;
                 W.fa[ch] = fa;
-                W.fm[ch] = flm;
-                W.fb[ch] = fm;
+                W.fb[ch] = AS ? fm : f1;
                 W.idx[ch + 1] = 2u * id + 1u;
-                W.fa[ch + 1] = fm;
-                W.fm[ch + 1] = frm;
+                W.fa[ch + 1] = AS ? fm : f1;
                 W.fb[ch + 1] = fb;
+                if constexpr (AS) {
+                    W.fm[ch] = f1;
+                    W.fm[ch + 1] = f2;
+                }
             }
             nsplit += ns;
         }
@@ -276,7 +327,7 @@ __device__ double coop_integrate_as(const QuadDev& Q, const DiagCoef<PM>& dcIn, 
         hs_add(st, nonfinite ? HS_FALLBACK_NONFINITE : HS_FALLBACK_SIZE, 1);
         __syncthreads();
         AdjAcc<PM> A;
-        return integrate_as<BK, PK, FL_VAL, PM>(Q, dcIn, A, nullptr, nullptr);
+        return integrate<BK, PK, QK, FL_VAL, PM>(Q, dcIn, A, nullptr);
     }
     hs_add(st, HS_NODES, (unsigned long long)start);
     hs_max(st, HS_MAX_NODES, (unsigned long long)start);
@@ -310,6 +361,7 @@ __device__ double coop_integrate_as(const QuadDev& Q, const DiagCoef<PM>& dcIn, 
     }
     const int nLeaves = W.cnt[0];
     double res = 0.0;
+#pragma unroll 8
     for (int k = 0; k < nLeaves; ++k) res += W.leaf[k];
     __syncthreads();  // the workspace is rewritten by the next call
     return res;
@@ -325,9 +377,10 @@ __global__ void __launch_bounds__(64) k_inverse_heavy(MapDev M, const double* __
                                                       const InvSeg* __restrict__ segs,
                                                       const int* __restrict__ chunkSeg) {
     __shared__ int lvl[MAXSUB + 2];
+    extern __shared__ double coopLds[];  // COOP_LDS_BYTES
     const unsigned long long total = *H.count;
     const long cnt = (long)(total < (unsigned long long)H.cap ? total : (unsigned long long)H.cap);
-    const CoopWs W = coop_ws(H.ws + (long)blockIdx.x * COOP_WS_DOUBLES_PER_NODE * H.wsNodes, H.wsNodes);
+    const CoopWs W = coop_ws(coopLds, COOP_NODES);
     for (long e = blockIdx.x; e < cnt; e += gridDim.x) {
         const long n = H.list[e];
         hs_add(H.stats, HS_SOLVES, 1);
@@ -359,7 +412,7 @@ __global__ void __launch_bounds__(64) k_inverse_heavy(MapDev M, const double* __
             [&](double x) {
                 DiagCoef<PM> d = dc;
                 d.x = x;
-                return coop_integrate_as<BK, PK, PM>(M.quad, d, W, lvl, H.stats) + f0;
+                return coop_integrate<BK, PK, QK, PM>(M.quad, d, W, lvl, H.stats) + f0;
             },
             x0, yd, xtol, ytol);
         if (threadIdx.x == 0) {
@@ -374,16 +427,17 @@ __global__ void __launch_bounds__(64) k_inverse_heavy(MapDev M, const double* __
 }
 
 
-// Launches k_inverse_heavy after a queue pass (adaptive Simpson with the
-// hand-over on; no kernel is built for the other rules).
+// Launches k_inverse_heavy after a queue pass (the adaptive rules with the
+// hand-over on; no kernel is built for the fixed rule, whose solves take one
+// step per T evaluation).
 template <int BK, int PK, int QK, int PM, bool LIN, bool BATCH>
 inline void launch_inverse_heavy(int blocks, hipStream_t s, const MapDev& M, const double* rec, long N, double* out,
                                  long os, int c, double xtol, double ytol, const InvHeavy& H, const InvSeg* segs,
                                  const int* chunkSeg) {
-    if constexpr (QK == QK_AS) {
+    if constexpr (QK != QK_CC) {
         if (H.solveSteps > 0 && blocks > 0)
-            hipLaunchKernelGGL((k_inverse_heavy<BK, PK, QK, PM, LIN, BATCH>), dim3((unsigned)blocks), dim3(64), 0, s, M,
-                               rec, N, out, os, c, xtol, ytol, H, segs, chunkSeg);
+            hipLaunchKernelGGL((k_inverse_heavy<BK, PK, QK, PM, LIN, BATCH>), dim3((unsigned)blocks), dim3(64),
+                               COOP_LDS_BYTES, s, M, rec, N, out, os, c, xtol, ytol, H, segs, chunkSeg);
     }
 }
 
@@ -476,7 +530,7 @@ __global__ void __launch_bounds__(256) k_inverse_queue(MapDev M, const double* _
     bool exhausted = false;
     // Interval steps of the current solve and quadrature; whether this solve
     // may still be handed over (InvHeavy).
-    const bool handOver = (QK == QK_AS) && H.solveSteps > 0;
+    const bool handOver = (QK != QK_CC) && H.solveSteps > 0;
     int solveSt = 0, quadSt = 0;
     bool mayHand = false;
 
@@ -600,7 +654,9 @@ __global__ void __launch_bounds__(256) k_inverse_queue(MapDev M, const double* _
         if (mayHand) {
             ++solveSt;
             ++quadSt;
-            if (solveSt > H.solveSteps || (H.quadSteps > 0 && quadSt > H.quadSteps)) {
+            // only once no point is left to take: before that the lane's long
+            // solve costs the wavefront nothing it could spend elsewhere
+            if (exhausted && (solveSt > H.solveSteps || (H.quadSteps > 0 && quadSt > H.quadSteps))) {
                 const unsigned long long slot = atomicAdd(H.count, 1ull);
                 if (slot < (unsigned long long)H.cap) {
                     H.list[slot] = n;  // k_inverse_heavy writes this root
