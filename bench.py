@@ -410,7 +410,7 @@ def profile_inverse_mode(args):
         torch.cuda.synchronize()
         times.append((time.perf_counter() - t0) * 1e3)
     print(json.dumps({"profile": "inverse_c2_random_targets", "ms": times,
-                      "nan": int(torch.isnan(xi).sum().item())}), flush=True)
+                      "nan": int(torch.isnan(xi).sum().item()), "handover": g.InverseHandoverStats()}), flush=True)
 
 
 def single_component_workload(steps):

@@ -414,26 +414,26 @@ int mpart_batch_set_inverse_schedule(mpart_batch batch, int32_t schedule);
 /* Hand-over of the longest inverse solves (adaptive quadrature rules; the
  * per-lane solve scheduling of mpart_inverse and of batch schedule 1): once
  * a wavefront of the solve pass finds no point left to take, a solve of it
- * that has taken more than solveSteps quadrature interval steps, or
- * one of whose quadratures has taken more than quadSteps (0: no such
- * limit), is solved again by a pass that gives each such solve a whole
- * wavefront and spreads each of its quadratures over the 64 lanes, level by
- * level of the subdivision tree (adaptive Simpson and adaptive
- * Clenshaw-Curtis; the fixed rule has no such tail).  The trees, sums and roots are the same as
- * the one-lane solve's, bit for bit; only the time differs.  solveSteps = 0
- * turns the hand-over off.  Defaults 2048 and 512 (the median solve takes a
- * few hundred steps).  No counterpart in the reference, whose solves run a
+ * that has taken more than solveSteps quadrature interval steps is solved
+ * again by a pass that gives each such solve a whole wavefront and spreads
+ * each of its quadratures over the 64 lanes, level by level of the
+ * subdivision tree (adaptive Simpson and adaptive Clenshaw-Curtis; the fixed
+ * rule has no such tail).  The trees, sums and roots are the same as the
+ * one-lane solve's, bit for bit; only the time differs.  solveSteps = 0
+ * turns the hand-over off.  Default 1024 (the median solve takes a few
+ * hundred steps).  No counterpart in the reference, whose solves run a
  * thread each (MParT/MonotoneComponent.h:331-443). */
-int mpart_set_inverse_handover(mpart_map map, int32_t solveSteps, int32_t quadSteps);
-int mpart_batch_set_inverse_handover(mpart_batch batch, int32_t solveSteps, int32_t quadSteps);
+int mpart_set_inverse_handover(mpart_map map, int32_t solveSteps);
+int mpart_batch_set_inverse_handover(mpart_batch batch, int32_t solveSteps);
 /* Totals of the last Inverse call with the hand-over on (waits for stream):
  * stats[0] solves handed over, [1] their T evaluations, [2] intervals
  * evaluated level by level, [3] T evaluations integrated depth-first because
  * the tree outgrew the wavefront's workspace, [4] ... because an interval
  * error was not finite, [5] the largest tree (intervals), [6] the deepest
- * level.  No counterpart in the reference. */
-int mpart_inverse_handover_stats(mpart_map map, uint64_t stats[7], void* stream);
-int mpart_batch_inverse_handover_stats(mpart_batch batch, uint64_t stats[7], void* stream);
+ * level, [7] the most T evaluations of one solve, [8] the longest solve in
+ * ticks of the 100 MHz device clock.  No counterpart in the reference. */
+int mpart_inverse_handover_stats(mpart_map map, uint64_t stats[9], void* stream);
+int mpart_batch_inverse_handover_stats(mpart_batch batch, uint64_t stats[9], void* stream);
 int mpart_batch_inverse_schedule(mpart_batch batch, int32_t* schedule, double* trialMs);
 
 int mpart_set_codegen(mpart_map map, int enable);

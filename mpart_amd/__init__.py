@@ -124,8 +124,8 @@ _SIGS = {
     "mpart_batch_inverse": [_vp, _vp, _vp, _vp, _dbl, _dbl, _vp],
     "mpart_batch_set_point_group": [_vp, ctypes.c_int32],
     "mpart_batch_set_inverse_schedule": [_vp, ctypes.c_int32],
-    "mpart_set_inverse_handover": [_vp, ctypes.c_int32, ctypes.c_int32],
-    "mpart_batch_set_inverse_handover": [_vp, ctypes.c_int32, ctypes.c_int32],
+    "mpart_set_inverse_handover": [_vp, ctypes.c_int32],
+    "mpart_batch_set_inverse_handover": [_vp, ctypes.c_int32],
     "mpart_inverse_handover_stats": [_vp, _vp, _vp],
     "mpart_batch_inverse_handover_stats": [_vp, _vp, _vp],
     "mpart_batch_inverse_schedule": [_vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)],
@@ -330,7 +330,7 @@ def _read_multiset(h, i):
 
 # ---------------------------------------------------------------------------
 _HANDOVER_KEYS = ("solves", "t_evaluations", "intervals", "fallback_size", "fallback_nonfinite", "max_intervals",
-                  "max_level")
+                  "max_level", "max_solve_t_evaluations", "max_solve_ticks")
 
 
 def _handover_stats(fn, h):
@@ -501,11 +501,10 @@ class ConditionalMap:
         _check(_lib.mpart_codegen_prepare(self._h, ctypes.byref(spec), ctypes.byref(secs)))
         return bool(spec.value), float(secs.value), _lib.mpart_codegen_error(self._h).decode()
 
-    def SetInverseHandover(self, solveSteps=2048, quadSteps=512):
-        """At the end of an Inverse pass, solves longer than solveSteps quadrature steps (or with a
-        quadrature longer than quadSteps) are finished by a wavefront each
-        (``mpart_set_inverse_handover``); 0 turns it off."""
-        _check(_lib.mpart_set_inverse_handover(self._h, int(solveSteps), int(quadSteps)))
+    def SetInverseHandover(self, solveSteps=1024):
+        """At the end of an Inverse pass, solves longer than solveSteps quadrature interval steps are
+        finished by a wavefront each (``mpart_set_inverse_handover``); 0 turns it off."""
+        _check(_lib.mpart_set_inverse_handover(self._h, int(solveSteps)))
 
     def InverseHandoverStats(self):
         """Totals of the last Inverse call with the hand-over on (``mpart_inverse_handover_stats``)."""
@@ -792,9 +791,9 @@ class MapBatch:
         points, component by component; 2 a lane per point; 0 autotune."""
         _check(_lib.mpart_batch_set_inverse_schedule(self._h, int(schedule)))
 
-    def SetInverseHandover(self, solveSteps=2048, quadSteps=512):
+    def SetInverseHandover(self, solveSteps=1024):
         """ConditionalMap.SetInverseHandover for the batch's schedule-1 Inverse."""
-        _check(_lib.mpart_batch_set_inverse_handover(self._h, int(solveSteps), int(quadSteps)))
+        _check(_lib.mpart_batch_set_inverse_handover(self._h, int(solveSteps)))
 
     def InverseHandoverStats(self):
         """ConditionalMap.InverseHandoverStats for the batch."""

@@ -49,24 +49,27 @@ struct InvSeg {
 
 // Hand-over of the longest inverse solves to k_inverse_heavy (mono_inverse.hpp):
 // the handed-over point indices (count of them, at most cap kept) and the
-// step budgets (solveSteps 0: off).
+// step budget (solveSteps 0: off).
 struct InvHeavy {
     unsigned long long* count;
     long* list;
     long cap;
-    int solveSteps, quadSteps;
+    int solveSteps;
     // Totals over a call (zeroed by its first pass; null: not kept): solves
     // handed over, their T evaluations, intervals evaluated level by level,
     // T evaluations integrated depth-first because the tree outgrew the
     // workspace or met a non-finite interval error, the largest tree
-    // (intervals) and the deepest level reached.
+    // (intervals), the deepest level reached, the most T evaluations of one
+    // solve, and the longest solve in ticks of the 100 MHz device clock.
     unsigned long long* stats;
 };
 enum { HS_SOLVES = 0, HS_TEVALS, HS_NODES, HS_FALLBACK_SIZE, HS_FALLBACK_NONFINITE, HS_MAX_NODES, HS_MAX_LEVELS,
-       HS_COUNT };
-// Intervals one cooperative quadrature keeps in LDS (56 bytes each; a larger
-// tree is integrated depth-first instead).
-constexpr int COOP_NODES = 1024;
-constexpr int COOP_LDS_BYTES = COOP_NODES * 56;
+       HS_MAX_SOLVE_TEVALS, HS_MAX_SOLVE_TICKS, HS_COUNT };
+// Intervals one cooperative quadrature keeps in LDS (60 bytes each, plus 64
+// node values of a round; a larger tree is integrated depth-first instead).
+// 384 intervals (23.5 KB: six one-wavefront blocks per CU) hold the trees
+// of the long solves met so far (at most 289 intervals).
+constexpr int COOP_NODES = 384;
+constexpr int COOP_LDS_BYTES = COOP_NODES * 60 + 64 * 8;
 
 }  // namespace mpd

@@ -3,13 +3,15 @@
 
 namespace mpd {
 
-// Zeroes the point counter and the hand-over count of a pass, and on the
-// call's first pass (component 0) the hand-over statistics.
-__global__ void k_zero_u64(unsigned long long* p, unsigned long long* q, unsigned long long* st) {
+// Zeroes the point counter and the hand-over count of a pass, writes the
+// hand-over arguments where the solve pass reads them (Hd), and on the
+// call's first pass (component 0) zeroes the hand-over statistics.
+__global__ void k_zero_u64(unsigned long long* p, InvHeavy H, InvHeavy* Hd, int first) {
     *p = 0ull;
-    if (q) *q = 0ull;
-    if (st)
-        for (int k = 0; k < HS_COUNT; ++k) st[k] = 0ull;
+    if (H.count) *H.count = 0ull;
+    if (Hd) *Hd = H;
+    if (first && H.stats)
+        for (int k = 0; k < HS_COUNT; ++k) H.stats[k] = 0ull;
 }
 
 template <int B, int Q, int R, int PM>
@@ -44,19 +46,19 @@ int inverse_record_rows(int var) {
 hipError_t launch_inverse(int var, const Launch& L, dim3 qgrid, dim3 qblock, const MapDev& M, const double* w, long ws,
                           const double* xs, long xrows, long xcols, long xrs, long xcs, const double* ys, long yss,
                           long N, double* out, long os, int c, double xtol, double ytol, double* gtab, double* rec,
-                          unsigned long long* counter, const InvHeavy& H, int heavyBlocks) {
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, L.stream, counter, H.count, c == 0 ? H.stats : nullptr);
+                          unsigned long long* counter, const InvHeavy& H, InvHeavy* Hd, int heavyBlocks) {
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, L.stream, counter, H, Hd, c == 0 ? 1 : 0);
 #define CALL(B, Q, R, PM)                                                                                          \
     hipLaunchKernelGGL((k_inverse_setup<B, Q, R, PM>), L.grid, L.block, L.smem, L.stream, M, w, ws, xs, xrows, xcols, \
                        xrs, xcs, ys, yss, N, c, gtab, rec);                                                        \
     if (M.lin) {                                                                                                 \
         hipLaunchKernelGGL((k_inverse_queue<B, Q, R, PM, true, false>), qgrid, qblock, 0, L.stream, M, rec, N, out, \
-                           os, c, xtol, ytol, counter, nullptr, nullptr, H);                                     \
+                           os, c, xtol, ytol, counter, nullptr, nullptr, Hd, H.solveSteps);                                     \
         launch_inverse_heavy<B, Q, R, PM, true, false>(heavyBlocks, L.stream, M, rec, N, out, os,                    \
             c, xtol, ytol, H, nullptr, nullptr);                                                                     \
     } else {                                                                                                     \
         hipLaunchKernelGGL((k_inverse_queue<B, Q, R, PM, false, false>), qgrid, qblock, 0, L.stream, M, rec, N, out, \
-                           os, c, xtol, ytol, counter, nullptr, nullptr, H);                                     \
+                           os, c, xtol, ytol, counter, nullptr, nullptr, Hd, H.solveSteps);                                     \
         launch_inverse_heavy<B, Q, R, PM, false, false>(heavyBlocks, L.stream, M, rec, N, out, os,                   \
             c, xtol, ytol, H, nullptr, nullptr);                                                                     \
     }

@@ -3,13 +3,15 @@
 
 namespace mpd {
 
-// Zeroes the point counter and the hand-over count of a pass, and on the
-// call's first pass (component 0) the hand-over statistics.
-__global__ void k_zero_u64_batch(unsigned long long* p, unsigned long long* q, unsigned long long* st) {
+// Zeroes the point counter and the hand-over count of a pass, writes the
+// hand-over arguments where the solve pass reads them (Hd), and on the
+// call's first pass (component 0) zeroes the hand-over statistics.
+__global__ void k_zero_u64_batch(unsigned long long* p, InvHeavy H, InvHeavy* Hd, int first) {
     *p = 0ull;
-    if (q) *q = 0ull;
-    if (st)
-        for (int k = 0; k < HS_COUNT; ++k) st[k] = 0ull;
+    if (H.count) *H.count = 0ull;
+    if (Hd) *Hd = H;
+    if (first && H.stats)
+        for (int k = 0; k < HS_COUNT; ++k) H.stats[k] = 0ull;
 }
 
 template <int B, int Q, int R, int PM>
@@ -34,20 +36,20 @@ int occupancy_inverse_batch(int var, int block) {
 hipError_t launch_inverse_batch(int var, const Launch& L, int ntiles, dim3 qgrid, dim3 qblock, const MapDev& M0,
                                 const MapDev* maps, const InvSeg* segs, const int* tileSeg, const long* tileStart,
                                 const int* chunkSeg, long Ntot, int c, double xtol, double ytol, double* rec,
-                                unsigned long long* counter, const InvHeavy& H, int heavyBlocks) {
+                                unsigned long long* counter, const InvHeavy& H, InvHeavy* Hd, int heavyBlocks) {
     if (ntiles <= 0 || Ntot <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_zero_u64_batch, dim3(1), dim3(1), 0, L.stream, counter, H.count, c == 0 ? H.stats : nullptr);
+    hipLaunchKernelGGL(k_zero_u64_batch, dim3(1), dim3(1), 0, L.stream, counter, H, Hd, c == 0 ? 1 : 0);
 #define CALL(B, Q, R, PM)                                                                                           \
     hipLaunchKernelGGL((k_inverse_setup_batch<B, Q, R, PM>), dim3((unsigned)ntiles), L.block, L.smem, L.stream,     \
                        maps, segs, tileSeg, tileStart, Ntot, c, rec);                                               \
     if (M0.lin) {                                                                                                 \
         hipLaunchKernelGGL((k_inverse_queue<B, Q, R, PM, true, true>), qgrid, qblock, 0, L.stream, M0, rec, Ntot,   \
-                           nullptr, 0, c, xtol, ytol, counter, segs, chunkSeg, H);                                \
+                           nullptr, 0, c, xtol, ytol, counter, segs, chunkSeg, Hd, H.solveSteps);                                \
         launch_inverse_heavy<B, Q, R, PM, true, true>(heavyBlocks, L.stream, M0, rec, Ntot, nullptr, 0,              \
             c, xtol, ytol, H, segs, chunkSeg);                                                                       \
     } else {                                                                                                      \
         hipLaunchKernelGGL((k_inverse_queue<B, Q, R, PM, false, true>), qgrid, qblock, 0, L.stream, M0, rec, Ntot,  \
-                           nullptr, 0, c, xtol, ytol, counter, segs, chunkSeg, H);                                \
+                           nullptr, 0, c, xtol, ytol, counter, segs, chunkSeg, Hd, H.solveSteps);                                \
         launch_inverse_heavy<B, Q, R, PM, false, true>(heavyBlocks, L.stream, M0, rec, Ntot, nullptr, 0,             \
             c, xtol, ytol, H, segs, chunkSeg);                                                                       \
     }

@@ -71,12 +71,13 @@ hipError_t launch_input_vjp(int var, const Launch& L, const MapDev& M, const dou
                             double* gtab);
 // Inverse of component c: a setup pass (shape L) writes per-point records to
 // rec, then the persistent solve pass (qgrid x qblock) takes points from the
-// counter at rec + records; with H's budgets set (adaptive Simpson), the
-// solves it hands over then run in heavyBlocks one-wavefront blocks.
+// counter at rec + records; with H's budget set (adaptive rules), the solves
+// it hands over (reading H from Hd, device memory) then run in heavyBlocks
+// one-wavefront blocks.
 hipError_t launch_inverse(int var, const Launch& L, dim3 qgrid, dim3 qblock, const MapDev& M, const double* w, long ws,
                           const double* xs, long xrows, long xcols, long xrs, long xcs, const double* ys, long yss,
                           long N, double* out, long os, int c, double xtol, double ytol, double* gtab, double* rec,
-                          unsigned long long* counter, const InvHeavy& H, int heavyBlocks);
+                          unsigned long long* counter, const InvHeavy& H, InvHeavy* Hd, int heavyBlocks);
 int occupancy_inverse(int var, int block);
 int inverse_record_rows(int var);
 // Batched Inverse of component c over many maps: a counter reset, the setup
@@ -86,7 +87,7 @@ struct InvSeg;
 hipError_t launch_inverse_batch(int var, const Launch& L, int ntiles, dim3 qgrid, dim3 qblock, const MapDev& M0,
                                 const MapDev* maps, const InvSeg* segs, const int* tileSeg, const long* tileStart,
                                 const int* chunkSeg, long Ntot, int c, double xtol, double ytol, double* rec,
-                                unsigned long long* counter, const InvHeavy& H, int heavyBlocks);
+                                unsigned long long* counter, const InvHeavy& H, InvHeavy* Hd, int heavyBlocks);
 int occupancy_inverse_batch(int var, int block);
 hipError_t launch_fused(int var, const Launch& L, const MapDev& M, const double* w, long ws, const double* pts,
                         long prs, long pcs, long N, const double* sens, long srs, long scs, const double* lsens,

@@ -339,9 +339,8 @@ double* noalloc_scratch() {
 // Schedules of the specialized fused pass the tuner times (gen_fused_launch).
 constexpr int GEN_FUSED_SCHEDULES = 4;
 // Inverse hand-over defaults and sizes (heavy_args).
-constexpr int HEAVY_SOLVE_STEPS = 2048;
-constexpr int HEAVY_QUAD_STEPS = 512;
-constexpr long HEAVY_MAX_BLOCKS = 1024;
+constexpr int HEAVY_SOLVE_STEPS = 1024;
+constexpr long HEAVY_MAX_BLOCKS = 2048;
 
 struct mpart_map_s {
     mpart_options opts{};
@@ -372,7 +371,7 @@ struct mpart_map_s {
     DBuf partialGrad, partialLoss, gtab, wT, wLd, wG, wLG, wFull, wCoeffs;
     DBuf wInv;  // inverse: per-point records + the point counter and the hand-over count
     DBuf wHeavy;  // inverse hand-over: list of handed-over solves + per-wavefront workspaces
-    int heavySolveSteps = HEAVY_SOLVE_STEPS, heavyQuadSteps = HEAVY_QUAD_STEPS;
+    int heavySolveSteps = HEAVY_SOLVE_STEPS;
     // A call on this map has been captured in a graph: from then on the work
     // buffers are never freed while the map lives (a larger eager call
     // allocates new ones and retires the old ones, which the graph reads).
@@ -1177,30 +1176,34 @@ void do_input_vjp(mpart_map_s& m, const mpart_cview& p, int klMode, const mpart_
 
 // Hand-over of the longest inverse solves (adaptive rules): once no point is
 // left to take, a queue lane hands a solve to the one-wavefront-per-solve
-// pass if the solve has taken HEAVY_SOLVE_STEPS interval steps or one of its
-// quadratures HEAVY_QUAD_STEPS (the median solve takes a few hundred); the
+// pass if the solve has taken HEAVY_SOLVE_STEPS interval steps (the median
+// solve takes a few hundred); the
 // list holds up to N / 16 (at least 1 024) solves, taken by heavyBlocks
-// wavefronts (one per 1 024 points, at most HEAVY_MAX_BLOCKS), each with an
+// wavefronts (one per 256 points, at most HEAVY_MAX_BLOCKS), each with an
 // LDS workspace of COOP_NODES intervals per quadrature (larger trees are
 // integrated depth-first by all lanes).
 long heavy_cap(long N) { return std::max<long>(1024, N / 16); }
-int heavy_blocks(long N) { return (int)std::min<long>(HEAVY_MAX_BLOCKS, std::max<long>(1, N / 1024)); }
-// Layout of the hand-over buffer: [statistics (256 B) | list].
+int heavy_blocks(long N) { return (int)std::min<long>(HEAVY_MAX_BLOCKS, std::max<long>(1, N / 256)); }
+// Layout of the hand-over buffer: [statistics (256 B) | the arguments as the
+// solve pass reads them (256 B) | list].
 constexpr size_t HEAVY_STATS_BYTES = 256;
-mpd::InvHeavy heavy_args(DBuf& buf, unsigned long long* count, long N, int solveSteps, int quadSteps, int& blocks) {
+mpd::InvHeavy heavy_args(DBuf& buf, unsigned long long* count, long N, int solveSteps, int& blocks,
+                         mpd::InvHeavy*& Hd) {
     static_assert(mpd::HS_COUNT * sizeof(unsigned long long) <= HEAVY_STATS_BYTES, "statistics block");
+    static_assert(sizeof(mpd::InvHeavy) <= 256, "argument block");
     mpd::InvHeavy H{};
     H.count = count;
     blocks = 0;
+    Hd = nullptr;
     if (solveSteps <= 0) return H;
     const long cap = heavy_cap(N);
     blocks = heavy_blocks(N);
-    buf.ensure(HEAVY_STATS_BYTES + (size_t)cap * sizeof(long));
+    buf.ensure(HEAVY_STATS_BYTES + 256 + (size_t)cap * sizeof(long));
     H.stats = buf.as<unsigned long long>();
-    H.list = reinterpret_cast<long*>(buf.as<char>() + HEAVY_STATS_BYTES);
+    Hd = reinterpret_cast<mpd::InvHeavy*>(buf.as<char>() + HEAVY_STATS_BYTES);
+    H.list = reinterpret_cast<long*>(buf.as<char>() + HEAVY_STATS_BYTES + 256);
     H.cap = cap;
     H.solveSteps = solveSteps;
-    H.quadSteps = quadSteps;
     return H;
 }
 
@@ -1230,11 +1233,11 @@ void inverse_component(mpart_map_s& m, const Shape& sh, const double* xs, long x
     const long grid = std::min<long>((N + block - 1) / block,
                                      (long)g_numCU * mpd::occupancy_inverse(m.variant, block));
     int heavyBlocks = 0;
-    const mpd::InvHeavy H =
-        heavy_args(m.wHeavy, counter + 1, N, m.heavySolveSteps, m.heavyQuadSteps, heavyBlocks);
+    mpd::InvHeavy* Hd = nullptr;
+    const mpd::InvHeavy H = heavy_args(m.wHeavy, counter + 1, N, m.heavySolveSteps, heavyBlocks, Hd);
     hip_check(mpd::launch_inverse(m.variant, sh.L, dim3((unsigned)std::max<long>(1, grid)), dim3(block), m.dev,
                                   m.coeffs, m.coeffStride, xs, xrows, xcols, xrs, xcs, ys, yss, N, out, os, c, xtol,
-                                  ytol, sh.gtab, rec, counter, H, heavyBlocks),
+                                  ytol, sh.gtab, rec, counter, H, Hd, heavyBlocks),
               "launch inverse");
 }
 
@@ -1781,7 +1784,7 @@ struct mpart_batch_s {
     // 2 a lane per point; 0 autotune (timed on the first uncaptured call with
     // enough points; untuned calls take 2).  Frozen once a call is captured.
     int invSchedule = 0;
-    int heavySolveSteps = HEAVY_SOLVE_STEPS, heavyQuadSteps = HEAVY_QUAD_STEPS;  // hand-over (inverse_component)
+    int heavySolveSteps = HEAVY_SOLVE_STEPS;  // hand-over (inverse_component)
     double invTrialMs[2] = {0, 0};
     bool invCaptured = false;  // an Inverse call has been captured (its schedule is frozen)
     std::vector<int> mapLanes;
@@ -2076,8 +2079,9 @@ void batch_inverse(mpart_batch_s& b, const std::vector<mpd::BatchIO>& io, double
     const long qcap = (long)g_numCU * mpd::occupancy_inverse_batch(b.variant, qblock);
     const auto* segs = P.dSegs.as<mpd::InvSeg>();
     int heavyBlocks = 0;
+    mpd::InvHeavy* Hd = nullptr;
     const mpd::InvHeavy H = heavy_args(P.wHeavy, P.dCounters.as<unsigned long long>() + std::max<size_t>(P.passes.size(), 1),
-                                       P.maxNtot, b.heavySolveSteps, b.heavyQuadSteps, heavyBlocks);
+                                       P.maxNtot, b.heavySolveSteps, heavyBlocks, Hd);
     for (size_t c = 0; c < P.passes.size(); ++c) {
         const InvPlan::Pass& ps = P.passes[c];
         const long qgrid = std::max<long>(1, std::min<long>((ps.Ntot + qblock - 1) / qblock, qcap));
@@ -2087,7 +2091,7 @@ void batch_inverse(mpart_batch_s& b, const std::vector<mpd::BatchIO>& io, double
                                             P.dTileSeg.as<int>() + ps.tile0, P.dTileStart.as<long>() + ps.tile0,
                                             P.dChunkSeg.as<int>() + ps.chunk0, ps.Ntot, (int)c, xtol, ytol,
                                             P.wRec.as<double>(), P.dCounters.as<unsigned long long>() + c, H,
-                                            heavyBlocks),
+                                            Hd, heavyBlocks),
                   "launch batched inverse");
     }
 }
@@ -2905,21 +2909,19 @@ int mpart_batch_set_inverse_schedule(mpart_batch b, int32_t schedule) {
     });
 }
 
-int mpart_set_inverse_handover(mpart_map m, int32_t solveSteps, int32_t quadSteps) {
+int mpart_set_inverse_handover(mpart_map m, int32_t solveSteps) {
     return guarded([&] {
         check_map(m);
-        if (solveSteps < 0 || quadSteps < 0) throw ArgError("hand-over budgets must be non-negative");
+        if (solveSteps < 0) throw ArgError("the hand-over budget must be non-negative");
         m->heavySolveSteps = solveSteps;
-        m->heavyQuadSteps = quadSteps;
     });
 }
 
-int mpart_batch_set_inverse_handover(mpart_batch b, int32_t solveSteps, int32_t quadSteps) {
+int mpart_batch_set_inverse_handover(mpart_batch b, int32_t solveSteps) {
     return guarded([&] {
         if (!b) throw ArgError("null argument");
-        if (solveSteps < 0 || quadSteps < 0) throw ArgError("hand-over budgets must be non-negative");
+        if (solveSteps < 0) throw ArgError("the hand-over budget must be non-negative");
         b->heavySolveSteps = solveSteps;
-        b->heavyQuadSteps = quadSteps;
     });
 }
 

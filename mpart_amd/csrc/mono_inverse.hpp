@@ -117,8 +117,7 @@ enum { IP_START = 0, IP_DOWN = 1, IP_UP = 2, IP_ITP = 3 };
 // quadrature subdividing into thousands of intervals); one lane running such
 // a solve is the whole tail of a pass.  With an adaptive rule, once the
 // wavefront finds no point left to take, a queue lane whose solve exceeds
-// solveSteps interval steps, or whose current quadrature exceeds quadSteps,
-// hands the point over: it appends the point index to list (count is the
+// solveSteps interval steps hands the point over: it appends the point index to list (count is the
 // list length; appends past cap are not made, and that lane keeps its
 // solve) and goes idle.  (Long solves met earlier stay in their lanes: the
 // other lanes keep taking points meanwhile, and a wavefront spends more
@@ -129,27 +128,31 @@ enum { IP_START = 0, IP_DOWN = 1, IP_UP = 2, IP_ITP = 3 };
 
 // Workspace of one wavefront (LDS, COOP_LDS_BYTES): per processed interval
 // (node) of one quadrature, in level order, its left end index, end and
-// midpoint values, fine sum, split rank, leaf count, leaf offset; and the
-// leaves in left-to-right order.
+// midpoint values, fine sum, split rank, leaf count, leaf offset and slot in
+// the round's speculative node values; the leaves in left-to-right order;
+// and the node values of one round (64).
 struct CoopWs {
-    double *fa, *fm, *fb, *fine, *leaf;
+    double *fa, *fm, *fb, *fine, *leaf, *spec;
     unsigned* idx;
-    int *srank, *cnt, *off;
+    int *srank, *cnt, *off, *sp;
     int cap;
 };
 __device__ __forceinline__ CoopWs coop_ws(double* base, int cap) {
-    static_assert(COOP_LDS_BYTES == COOP_NODES * (5 * sizeof(double) + 4 * sizeof(int)), "workspace layout");
+    static_assert(COOP_LDS_BYTES == COOP_NODES * (5 * sizeof(double) + 5 * sizeof(int)) + 64 * sizeof(double),
+                  "workspace layout");
     CoopWs W;
     W.fa = base;
     W.fm = base + cap;
     W.fb = base + 2L * cap;
     W.fine = base + 3L * cap;
     W.leaf = base + 4L * cap;
-    int* ib = reinterpret_cast<int*>(base + 5L * cap);
+    W.spec = base + 5L * cap;
+    int* ib = reinterpret_cast<int*>(base + 5L * cap + 64);
     W.idx = reinterpret_cast<unsigned*>(ib);
     W.srank = ib + cap;
     W.cnt = ib + 2L * cap;
     W.off = ib + 3L * cap;
+    W.sp = ib + 4L * cap;
     W.cap = cap;
     return W;
 }
@@ -161,48 +164,94 @@ __device__ __forceinline__ double lane_bcast(double v, int src) {
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-// Adds a hand-over statistic (lane 0; st may be null).
-__device__ __forceinline__ void hs_add(unsigned long long* st, int k, unsigned long long v) {
-    if (st && threadIdx.x == 0) atomicAdd(st + k, v);
-}
-__device__ __forceinline__ void hs_max(unsigned long long* st, int k, unsigned long long v) {
-    if (st && threadIdx.x == 0) atomicMax(st + k, v);
-}
+// Hand-over statistics of one wavefront, kept in registers and added to the
+// call's totals (InvHeavy::stats) once, when the wavefront is done: atomics
+// on those few words at every T evaluation serialize in the L2.
+struct HeavyStats {
+    unsigned long long v[HS_COUNT];
+    __device__ void add(int k, unsigned long long x) { v[k] += x; }
+    __device__ void max(int k, unsigned long long x) { v[k] = x > v[k] ? x : v[k]; }
+    __device__ void flush(unsigned long long* st) const {
+        if (!st || threadIdx.x != 0) return;
+#pragma unroll
+        for (int k = 0; k < HS_COUNT; ++k) {
+            if (k == HS_MAX_NODES || k == HS_MAX_LEVELS || k == HS_MAX_SOLVE_TEVALS || k == HS_MAX_SOLVE_TICKS) {
+                if (v[k]) atomicMax(st + k, v[k]);
+            } else if (v[k]) {
+                atomicAdd(st + k, v[k]);
+            }
+        }
+    }
+};
 
-// One interval [a, b] of the nested Clenshaw-Curtis rule, as integrate_acc
-// and integrate_acc_big form it: the midpoint value (returned in fm) and the
-// coarse and fine sums, the nodes and terms in their order.
-template <int BK, int PK, int PM>
-__device__ __forceinline__ void acc_interval(const QuadDev& Q, const DiagCoef<PM>& dc, double a, double b, double fa,
-                                             double fb, double& fm, double& coarse, double& fine) {
+// The nested Clenshaw-Curtis rule on one interval, as integrate_acc and
+// integrate_acc_big form it.  Its E = acc_nodes(Q) new nodes are numbered in
+// their order of evaluation there: the midpoint, the coarse rule's interior
+// nodes left of it and right of it, then the fine-only nodes.
+struct AccRule {
+    const double *cw, *cp, *fw, *fp;
+    int nf, cRight, cMid, fRight, fMid;
+};
+__device__ __forceinline__ AccRule acc_rule(const QuadDev& Q) {
+    AccRule R;
+    const int nc = Q.nc;
+    R.nf = Q.nf;
+    R.cw = Q.big ? Q.big : Q.cw;
+    R.cp = Q.big ? Q.big + nc : Q.cp;
+    R.fw = Q.big ? Q.big + 2 * nc : Q.fw;
+    R.fp = Q.big ? Q.big + 2 * nc + R.nf : Q.fp;
+    R.cRight = nc - 1;
+    R.cMid = R.cRight / 2;
+    R.fRight = R.nf - 1;
+    R.fMid = R.fRight / 2;
+    return R;
+}
+// coarse interior nodes left of the midpoint and in all (the midpoint aside)
+__device__ __forceinline__ int acc_left(const AccRule& R) { return R.cMid > 1 ? R.cMid - 1 : 0; }
+__device__ __forceinline__ int acc_interior(const AccRule& R) {
+    return acc_left(R) + (R.cRight - R.cMid - 1 > 0 ? R.cRight - R.cMid - 1 : 0);
+}
+__device__ __forceinline__ int acc_nodes(const AccRule& R) {
+    return 1 + acc_interior(R) + (R.nf - 1) / 2;  // midpoint, coarse interior, fine-only
+}
+// Position of node e of [a, b].
+__device__ __forceinline__ double acc_node_pos(const AccRule& R, int e, double a, double b) {
 #pragma clang fp contract(off)  // as in integrate_acc
-    const int nf = Q.nf, nc = Q.nc;
-    const int cRight = nc - 1, cMid = cRight / 2;
-    const int fRight = nf - 1, fMid = fRight / 2;
-    const double* cw = Q.big ? Q.big : Q.cw;
-    const double* cp = Q.big ? Q.big + nc : Q.cp;
-    const double* fw = Q.big ? Q.big + 2 * nc : Q.fw;
-    const double* fp = Q.big ? Q.big + 2 * nc + nf : Q.fp;
     const double mid = 0.5 * (a + b);
     const double scale = 0.5 * (b - a);
-    fm = eval_node<BK, PK, FL_VAL, PM>(dc, mid).F;
-    coarse = scale * cw[0] * fa;
-    coarse += scale * cw[cMid] * fm;
-    coarse += scale * cw[cRight] * fb;
-    fine = scale * fw[0] * fa;
-    fine += scale * fw[fMid] * fm;
-    fine += scale * fw[fRight] * fb;
-    for (int i = 1; i < cMid; ++i) {
-        const double v = eval_node<BK, PK, FL_VAL, PM>(dc, mid + scale * cp[i]).F;
-        coarse += scale * cw[i] * v;
-        fine += scale * fw[2 * i] * v;
+    if (e == 0) return mid;
+    const int nci = acc_interior(R);
+    if (e <= nci) {
+        const int k = e - 1;  // k-th coarse interior node: i = 1 .. cMid-1, then cMid+1 .. cRight-1
+        const int c1 = acc_left(R);
+        const int ci = (k < c1) ? k + 1 : R.cMid + 1 + (k - c1);
+        return mid + scale * R.cp[ci];
     }
-    for (int ii = cMid + 1; ii < cRight; ++ii) {
-        const double v = eval_node<BK, PK, FL_VAL, PM>(dc, mid + scale * cp[ii]).F;
-        coarse += scale * cw[ii] * v;
-        fine += scale * fw[2 * ii] * v;
+    const int k = e - 1 - nci;  // k-th fine-only node: i = 1, 3, 5, ...
+    return mid + scale * R.fp[2 * k + 1];
+}
+// Coarse and fine sums of [a, b] from its end values and its node values v[e].
+__device__ __forceinline__ void acc_sums(const AccRule& R, double a, double b, double fa, double fb, const double* v,
+                                         double& coarse, double& fine) {
+#pragma clang fp contract(off)  // as in integrate_acc
+    const double scale = 0.5 * (b - a);
+    const double fm = v[0];
+    coarse = scale * R.cw[0] * fa;
+    coarse += scale * R.cw[R.cMid] * fm;
+    coarse += scale * R.cw[R.cRight] * fb;
+    fine = scale * R.fw[0] * fa;
+    fine += scale * R.fw[R.fMid] * fm;
+    fine += scale * R.fw[R.fRight] * fb;
+    int e = 1;
+    for (int i = 1; i < R.cMid; ++i, ++e) {
+        coarse += scale * R.cw[i] * v[e];
+        fine += scale * R.fw[2 * i] * v[e];
     }
-    for (int i = 1; i < nf; i += 2) fine += scale * fw[i] * eval_node<BK, PK, FL_VAL, PM>(dc, mid + scale * fp[i]).F;
+    for (int ii = R.cMid + 1; ii < R.cRight; ++ii, ++e) {
+        coarse += scale * R.cw[ii] * v[e];
+        fine += scale * R.fw[2 * ii] * v[e];
+    }
+    for (int i = 1; i < R.nf; i += 2, ++e) fine += scale * R.fw[i] * v[e];
 }
 
 // The adaptive rules (integrate_as, integrate_acc; FL_VAL) computed by the 64
@@ -222,13 +271,21 @@ __device__ __forceinline__ void acc_interval(const QuadDev& Q, const DiagCoef<PM
 // Every lane returns the same value.  lvl: LDS, MAXSUB + 2 level starts.
 template <int BK, int PK, int QK, int PM>
 __device__ double coop_integrate(const QuadDev& Q, const DiagCoef<PM>& dcIn, const CoopWs& W, int* lvl,
-                                 unsigned long long* st) {
+                                 HeavyStats& st) {
 #pragma clang fp contract(off)  // as in integrate_as
     constexpr bool AS = (QK == QK_AS);
     const int lane = threadIdx.x;
     DiagCoef<PM> dc = dcIn;
     dc.status = nullptr;
-    {
+    AccRule R{};
+    int E = 2;  // new nodes per interval
+    if constexpr (!AS) {
+        R = acc_rule(Q);
+        E = acc_nodes(R);
+    }
+    bool fallback = E > 64, nonfinite = false;
+    int start = 0, n = 1, L = 0;
+    if (!fallback) {
         // the depth-first loops evaluate F(0), F(1) and (Simpson) F(1/2) first
         const double t = lane == 0 ? 0.0 : (lane == 1 ? 1.0 : 0.5);
         const double v = eval_node<BK, PK, FL_VAL, PM>(dc, t).F;
@@ -240,97 +297,173 @@ __device__ double coop_integrate(const QuadDev& Q, const DiagCoef<PM>& dcIn, con
             W.fb[0] = fb;
             lvl[0] = 0;
         }
+        __syncthreads();
     }
-    __syncthreads();
-    int start = 0, n = 1, L = 0;
-    bool fallback = false, nonfinite = false;
-    while (n > 0) {
-        const int next = start + n;
-        int nsplit = 0;
-        for (int base = 0; base < n; base += 64) {
-            const int j = base + lane;
-            const bool act = j < n;
-            const int node = start + j;
-            unsigned id = 0u;
-            double a = 0.0, b = 1.0, fa = 0.0, fm = 0.0, fb = 0.0;
-            if (act) {
-                id = W.idx[node];
-                fa = W.fa[node];
-                if constexpr (AS) fm = W.fm[node];
-                fb = W.fb[node];
-                a = ldexp((double)id, -L);
-                b = ldexp((double)(id + 1u), -L);
+    // Rounds.  A round takes the pending intervals of level L (n of them, from
+    // lvl[L] = start) and, when n E <= 32, the speculative subtrees of depth
+    // d below them (the largest d <= 6 with n E (2^d - 1) <= 64 that stays
+    // above quadMaxSub): every new node of every interval of those subtrees
+    // gets a lane and is evaluated at once (into W.spec).  Levels L .. L+d-1
+    // are then tested one after the other from those values, a lane per
+    // interval, and only the children of intervals that split are kept, so
+    // the tree is the depth-first loop's.  With n E > 32 a round covers the
+    // one level L, 64 / E of its intervals at a time.  W.sp holds the slot of
+    // an interval's node values in W.spec (its pending ancestor's index in
+    // the chunk times 2^d - 1, plus its place in the subtree in heap order).
+    while (n > 0 && !fallback) {
+        int d = 1;
+        if (n * E <= 32) {
+            while (d < 6 && n * E * ((2 << d) - 1) <= 64 && L + d < Q.maxSub) ++d;
+        }
+        const int S = (1 << d) - 1;
+        const int chunk = (d == 1) ? 64 / E : n;
+        int nsplit = 0;  // splits so far in level L (the chunks of a one-level round)
+        for (int c0 = 0; c0 < n && !fallback; c0 += chunk) {
+            const int cn = min(chunk, n - c0);
+            {  // evaluate: task t -> pending interval i of the chunk, subtree interval h (heap order), node e
+                const int t = lane;
+                if (t < cn * S * E) {
+                    const int ii = t / (S * E);
+                    const int r = t - ii * (S * E);
+                    const int h = r / E, e = r - h * E;
+                    const int l = 31 - __builtin_clz((unsigned)(h + 1));
+                    const unsigned id = (W.idx[start + c0 + ii] << l) + (unsigned)(h + 1 - (1 << l));
+                    const double a = ldexp((double)id, -(L + l));
+                    const double b = ldexp((double)(id + 1u), -(L + l));
+                    double x;
+                    if constexpr (AS) {
+                        const double m = 0.5 * (a + b);
+                        x = (e == 0) ? 0.5 * (a + m) : 0.5 * (m + b);
+                    } else {
+                        x = acc_node_pos(R, e, a, b);
+                    }
+                    W.spec[t] = eval_node<BK, PK, FL_VAL, PM>(dc, x).F;
+                }
+                if (lane < cn) W.sp[start + c0 + lane] = lane * S;
             }
-            double coarse = 0.0, fine = 0.0, f1 = 0.0, f2 = 0.0;  // Simpson: the quarter points; else f1 = midpoint
-            if constexpr (AS) {
-                const double m = 0.5 * (a + b);
-                const double lm = 0.5 * (a + m);
-                const double rm = 0.5 * (m + b);
+            __syncthreads();
+            // test the round's levels: intervals ls .. ls + ln - 1 of the
+            // level that starts at lvlS and holds lvlN intervals
+            int lvlS = start, lvlN = n, ls = start + c0, ln = cn;
+            for (int l = 0; l < d; ++l) {
+                const int Lc = L;  // the level under test (L moves down with l)
+                const bool last = (l == d - 1);
+                const int childS = lvlS + lvlN;
+                const int k = lane;
+                const bool act = k < ln;
+                const int node = ls + k;
+                unsigned id = 0u;
+                int sp = 0;
+                double a = 0.0, b = 1.0, fa = 0.0, fm = 0.0, fb = 0.0, coarse = 0.0, fine = 0.0, v0 = 0.0, v1 = 0.0;
                 if (act) {
-                    f1 = eval_node<BK, PK, FL_VAL, PM>(dc, lm).F;
-                    f2 = eval_node<BK, PK, FL_VAL, PM>(dc, rm).F;
+                    id = W.idx[node];
+                    sp = W.sp[node];
+                    fa = W.fa[node];
+                    fb = W.fb[node];
+                    a = ldexp((double)id, -Lc);
+                    b = ldexp((double)(id + 1u), -Lc);
+                    v0 = W.spec[sp * E];
+                    if constexpr (AS) {
+                        fm = W.fm[node];
+                        v1 = W.spec[sp * E + 1];
+                        const double m = 0.5 * (a + b);
+                        coarse = div6(b - a) * (fa + 4.0 * fm + fb);
+                        const double wl = div6(m - a), wr = div6(b - m);
+                        fine = wl * (fa + 4.0 * v0 + fm);
+                        fine += wr * (fm + 4.0 * v1 + fb);
+                    } else {
+                        acc_sums(R, a, b, fa, fb, W.spec + sp * E, coarse, fine);
+                    }
                 }
-                coarse = div6(b - a) * (fa + 4.0 * fm + fb);
-                const double wl = div6(m - a), wr = div6(b - m);
-                fine = wl * (fa + 4.0 * f1 + fm);
-                fine += wr * (fm + 4.0 * f2 + fb);
-            } else {
-                if (act) acc_interval<BK, PK, PM>(Q, dc, a, b, fa, fb, f1, coarse, fine);
-            }
-            const double err = fabs(fine - coarse);
-            if (__ballot(act && nonfinite_err(err))) {
-                fallback = nonfinite = true;
-                break;
-            }
-            const double tol = fmax(Q.relTol * fabs(coarse), Q.absTol);
-            const bool accept = act && (((err < tol) || (L == Q.maxSub - 1)) && (L >= Q.minSub));
-            const bool split = act && !accept;
-            const unsigned long long mask = __ballot(split);
-            const int ns = __popcll(mask);
-            if (next + 2 * (nsplit + ns) > W.cap || L + 1 > MAXSUB) {
-                fallback = true;
-                break;
-            }
-            const int rank =
-                __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
-            if (act) {
-                W.fine[node] = fine;
-                W.srank[node] = accept ? -1 : nsplit + rank;
-            }
-            if (split) {
-                // Simpson: [a, m] holds (fa, f(lm), fm), [m, b] (fm, f(rm), fb);
-                // the nested rule: [a, mid] holds (fa, f(mid)), [mid, b] (f(mid), fb)
-                const int ch = next + 2 * (nsplit + rank);
-                W.idx[ch] = 2u * id;
-                W.fa[ch] = fa;
-                W.fb[ch] = AS ? fm : f1;
-                W.idx[ch + 1] = 2u * id + 1u;
-                W.fa[ch + 1] = AS ? fm : f1;
-                W.fb[ch + 1] = fb;
-                if constexpr (AS) {
-                    W.fm[ch] = f1;
-                    W.fm[ch + 1] = f2;
+                const double err = fabs(fine - coarse);
+                if (__ballot(act && nonfinite_err(err))) {
+                    fallback = nonfinite = true;
+                    break;
                 }
+                const double tol = fmax(Q.relTol * fabs(coarse), Q.absTol);
+                const bool accept = act && (((err < tol) || (Lc == Q.maxSub - 1)) && (Lc >= Q.minSub));
+                const bool split = act && !accept;
+                const unsigned long long mask = __ballot(split);
+                const int ns = __popcll(mask);
+                const int base = nsplit;  // splits before this chunk in level Lc
+                if (childS + 2 * (base + ns) > W.cap || Lc + 1 > MAXSUB) {
+                    fallback = true;
+                    break;
+                }
+                const int rank =
+                    __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+                if (act) {
+                    W.fine[node] = fine;
+                    W.srank[node] = accept ? -1 : base + rank;
+                }
+                if (split) {
+                    // Simpson: [a, m] holds (fa, f(lm), fm), [m, b] (fm, f(rm), fb);
+                    // the nested rule: [a, mid] holds (fa, f(mid)), [mid, b] (f(mid), fb)
+                    const int ch = childS + 2 * (base + rank);
+                    const int h = sp % S;
+                    W.idx[ch] = 2u * id;
+                    W.fa[ch] = fa;
+                    W.fb[ch] = AS ? fm : v0;
+                    W.idx[ch + 1] = 2u * id + 1u;
+                    W.fa[ch + 1] = AS ? fm : v0;
+                    W.fb[ch + 1] = fb;
+                    if constexpr (AS) {
+                        W.fm[ch] = v0;
+                        W.fm[ch + 1] = v1;
+                    }
+                    if (!last) {
+                        W.sp[ch] = sp + h + 1;
+                        W.sp[ch + 1] = sp + h + 2;
+                    }
+                }
+                nsplit += ns;
+                __syncthreads();
+                if (last) break;
+                // level L + l is complete (a subtree round has one chunk): move down
+                ++L;
+                if (lane == 0) lvl[L] = childS;
+                lvlS = childS;
+                lvlN = 2 * ns;
+                ls = childS;
+                ln = lvlN;
+                start = childS;
+                n = lvlN;
+                nsplit = 0;
+                if (ln == 0) break;
             }
-            nsplit += ns;
+            if (d > 1) break;  // the subtree round is done
         }
         if (fallback) break;
-        ++L;
-        if (lane == 0) lvl[L] = next;
-        __syncthreads();
-        start = next;
-        n = 2 * nsplit;
+        if (d == 1) {
+            // the last tested level (L, in chunks) is complete
+            ++L;
+            const int next = start + n;
+            if (lane == 0) lvl[L] = next;
+            __syncthreads();
+            start = next;
+            n = 2 * nsplit;
+        } else if (n > 0) {
+            // the round's last tested level is L; its children were written
+            // after it, nsplit of its intervals split
+            const int next = start + n;
+            ++L;
+            if (lane == 0) lvl[L] = next;
+            __syncthreads();
+            start = next;
+            n = 2 * nsplit;
+        }
     }
-    hs_add(st, HS_TEVALS, 1);
-    hs_max(st, HS_MAX_LEVELS, (unsigned long long)L);
+    st.add(HS_TEVALS, 1);
+    st.max(HS_MAX_LEVELS, (unsigned long long)L);
     if (fallback) {
-        hs_add(st, nonfinite ? HS_FALLBACK_NONFINITE : HS_FALLBACK_SIZE, 1);
+        st.add(nonfinite ? HS_FALLBACK_NONFINITE : HS_FALLBACK_SIZE, 1);
         __syncthreads();
         AdjAcc<PM> A;
         return integrate<BK, PK, QK, FL_VAL, PM>(Q, dcIn, A, nullptr);
     }
-    hs_add(st, HS_NODES, (unsigned long long)start);
-    hs_max(st, HS_MAX_NODES, (unsigned long long)start);
+    st.add(HS_NODES, (unsigned long long)start);
+    st.max(HS_MAX_NODES, (unsigned long long)start);
+    __syncthreads();  // lvl[L]
     // Levels 0 .. L-1 hold the tree (level k at lvl[k] .. lvl[k+1]-1; the
     // children of level k start at lvl[k+1]).  Leaf counts, bottom-up.
     for (int k = L - 1; k >= 0; --k) {
@@ -381,9 +514,10 @@ __global__ void __launch_bounds__(64) k_inverse_heavy(MapDev M, const double* __
     const unsigned long long total = *H.count;
     const long cnt = (long)(total < (unsigned long long)H.cap ? total : (unsigned long long)H.cap);
     const CoopWs W = coop_ws(coopLds, COOP_NODES);
+    HeavyStats hs{};
     for (long e = blockIdx.x; e < cnt; e += gridDim.x) {
         const long n = H.list[e];
-        hs_add(H.stats, HS_SOLVES, 1);
+        hs.add(HS_SOLVES, 1);
         DiagCoef<PM> dc;
         dc.nug = M.nugget;
         dc.x = 0.0;
@@ -408,13 +542,18 @@ __global__ void __launch_bounds__(64) k_inverse_heavy(MapDev M, const double* __
         const double f0 = rec[(PM + 1) * N + n];
         const double x0 = rec[(PM + 2) * N + n];
         const double yd = rec[(PM + 3) * N + n];
+        const unsigned long long t0 = wall_clock64();
+        unsigned long long nT = 0;
         const double root = inverse_solve_with(
             [&](double x) {
                 DiagCoef<PM> d = dc;
                 d.x = x;
-                return coop_integrate<BK, PK, QK, PM>(M.quad, d, W, lvl, H.stats) + f0;
+                ++nT;
+                return coop_integrate<BK, PK, QK, PM>(M.quad, d, W, lvl, hs) + f0;
             },
             x0, yd, xtol, ytol);
+        hs.max(HS_MAX_SOLVE_TEVALS, nT);
+        hs.max(HS_MAX_SOLVE_TICKS, wall_clock64() - t0);
         if (threadIdx.x == 0) {
             if constexpr (BATCH) {
                 *op = root;
@@ -424,6 +563,7 @@ __global__ void __launch_bounds__(64) k_inverse_heavy(MapDev M, const double* __
             }
         }
     }
+    hs.flush(H.stats);
 }
 
 
@@ -452,7 +592,8 @@ template <int BK, int PK, int QK, int PM, bool LIN, bool BATCH>
 __global__ void __launch_bounds__(256) k_inverse_queue(MapDev M, const double* __restrict__ rec, long N, double* out,
                                                        long os, int c, double xtol, double ytol,
                                                        unsigned long long* counter, const InvSeg* __restrict__ segs,
-                                                       const int* __restrict__ chunkSeg, InvHeavy H) {
+                                                       const int* __restrict__ chunkSeg,
+                                                       const InvHeavy* __restrict__ Hd, int solveSteps) {
     const QuadDev& Q = M.quad;
     const int p = BATCH ? 0 : ro(M.comps)[c].p;
     // Clenshaw-Curtis node table of one interval, in the order integrate_acc
@@ -528,14 +669,12 @@ __global__ void __launch_bounds__(256) k_inverse_queue(MapDev M, const double* _
     // Wavefront-uniform range of point indices not yet handed to a lane.
     long qNext = 0, qEnd = 0;
     bool exhausted = false;
-    // Interval steps of the current solve and quadrature; whether this solve
-    // may still be handed over (InvHeavy).
-    const bool handOver = (QK != QK_CC) && H.solveSteps > 0;
-    int solveSt = 0, quadSt = 0;
-    bool mayHand = false;
+    // Interval steps the current solve may still take before it is handed
+    // over once no point is left (InvHeavy); __INT_MAX__: never.
+    const bool handOver = (QK != QK_CC) && solveSteps > 0;
+    int budget = __INT_MAX__;
 
     auto start_quad = [&](double x) {
-        quadSt = 0;
         dc.x = x;
         qa = 0.0;
         qb = 1.0;
@@ -545,42 +684,22 @@ __global__ void __launch_bounds__(256) k_inverse_queue(MapDev M, const double* _
         k = (QK == QK_AS) ? -2 : -1;
     };
 
-    // ITP steps from T(xc) = yc: the bracket update and the next iterate,
-    // until the solve is done or an iterate needs a quadrature; iterates at
-    // an end of the bracket take the value held for it (same_bits).
-    auto itp_steps = [&](double yc, bool& done, double& root) {
-        while (true) {
-            if (fabs(yc - yd) < ytol) {
-                done = true;
-                root = xc;
-                return;
-            }
-            if (yc > yd) {
-                xub = xc;
-                yub = yc;
-            } else {
-                xlb = xc;
-                ylb = yc;
-            }
-            if (((xub - xlb) < xtol) || ((yub - ylb) < ytol) || (unsigned)(it + 1) >= ITP_MAX_ITS) {
-                done = true;
-                root = 0.5 * (xub + xlb);
-                return;
-            }
-            ++it;
-            xc = itp_point(xlb, xub, ylb, yub, nhalf, it, xtol);
-            if (same_bits(xc, xlb)) {
-                yc = ylb;
-            } else if (same_bits(xc, xub)) {
-                yc = yub;
-            } else {
-                start_quad(xc);
-                return;
+    while (true) {
+        // Hand-over (InvHeavy), only once no point is left to take: before
+        // that a lane's long solve costs the wavefront nothing it could spend
+        // elsewhere.  `exhausted` is the same in every lane.
+        if (handOver && exhausted) {
+            if (busy && budget < 0) {
+                const InvHeavy H = *Hd;  // read here only, not held through the loop
+                const unsigned long long slot = atomicAdd(H.count, 1ull);
+                if (slot < (unsigned long long)H.cap) {
+                    H.list[slot] = n;  // k_inverse_heavy writes this root
+                    busy = false;
+                } else {
+                    budget = __INT_MAX__;  // the list is full: the solve stays here
+                }
             }
         }
-    };
-
-    while (true) {
         // Refill lanes without a point.
         while (true) {
             const bool need = !busy && !finished;
@@ -635,8 +754,7 @@ __global__ void __launch_bounds__(256) k_inverse_queue(MapDev M, const double* _
                         yd = rec[(PM + 3) * N + n];
                         busy = true;
                         phase = IP_START;
-                        solveSt = 0;
-                        mayHand = handOver;
+                        budget = handOver ? solveSteps : __INT_MAX__;
                         start_quad(x0);
                     }
                 }
@@ -651,21 +769,7 @@ __global__ void __launch_bounds__(256) k_inverse_queue(MapDev M, const double* _
         }
         if (__ballot(busy) == 0) break;
         if (!busy) continue;
-        if (mayHand) {
-            ++solveSt;
-            ++quadSt;
-            // only once no point is left to take: before that the lane's long
-            // solve costs the wavefront nothing it could spend elsewhere
-            if (exhausted && (solveSt > H.solveSteps || (H.quadSteps > 0 && quadSt > H.quadSteps))) {
-                const unsigned long long slot = atomicAdd(H.count, 1ull);
-                if (slot < (unsigned long long)H.cap) {
-                    H.list[slot] = n;  // k_inverse_heavy writes this root
-                    busy = false;
-                    continue;
-                }
-                mayHand = false;  // the list is full: the solve stays here
-            }
-        }
+        if (handOver) --budget;
 
         // One step of this lane's quadrature: the integrand at every node of
         // its current interval (E evaluations, independent of each other),
@@ -910,13 +1014,31 @@ __global__ void __launch_bounds__(256) k_inverse_queue(MapDev M, const double* _
                     it = 0;
                     nhalf = itp_nhalf(xlb, xub, xtol);
                     xc = itp_point(xlb, xub, ylb, yub, nhalf, it, xtol);
-                    if (same_bits(xc, xlb)) itp_steps(ylb, done, root);
-                    else if (same_bits(xc, xub)) itp_steps(yub, done, root);
-                    else start_quad(xc);
+                    start_quad(xc);
                 }
             }
         } else {
-            itp_steps(y, done, root);
+            const double yc = y;
+            if (fabs(yc - yd) < ytol) {
+                done = true;
+                root = xc;
+            } else {
+                if (yc > yd) {
+                    xub = xc;
+                    yub = yc;
+                } else {
+                    xlb = xc;
+                    ylb = yc;
+                }
+                if (((xub - xlb) < xtol) || ((yub - ylb) < ytol) || (unsigned)(it + 1) >= ITP_MAX_ITS) {
+                    done = true;
+                    root = 0.5 * (xub + xlb);
+                } else {
+                    ++it;
+                    xc = itp_point(xlb, xub, ylb, yub, nhalf, it, xtol);
+                    start_quad(xc);
+                }
+            }
         }
         if (fail || done) {
             const double v = fail ? NAN : root;

@@ -629,8 +629,9 @@ constexpr unsigned ITP_MAX_ITS = 10000;
 // same evaluation gives the same bits.  Where the bracket has shrunk to two
 // neighbouring doubles wider than xtol (roots of large magnitude: at 1e13 an
 // ulp is 2e-3), every iterate is such an end and the reference's loop
-// (RootFinding.h:158-174) runs to its 10 000 iterations without moving; the
-// solvers here take those iterations too, without integrating T again.
+// (RootFinding.h:158-174) runs to its 10 000 iterations without moving;
+// inverse_solve_with takes those iterations too, without integrating T again
+// (the per-lane queue hands such a solve over, mono_inverse.hpp).
 __device__ __forceinline__ bool same_bits(double a, double b) {
     return __double_as_longlong(a) == __double_as_longlong(b);
 }

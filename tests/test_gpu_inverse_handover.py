@@ -1,7 +1,7 @@
 """Hand-over of long inverse solves to the one-wavefront-per-solve pass.
 
 A queue lane hands its solve over once its wavefront finds no point left to
-take; with the budgets at one step, that is every solve still running then
+take; with the budget at one step, that is every solve still running then
 (up to the list's capacity; with a few hundred points, nearly all of them),
 and each is solved again by a wavefront whose 64 lanes build each adaptive
 quadrature tree level by level.  The trees, node values and sums are
@@ -15,7 +15,7 @@ memory), NaN inputs, targets outside the range (NaN roots
 after a long bracket search), triangular maps (one pass per component), the
 batched queue (schedule 1), an overflowing integrand (the pass falls back to
 the depth-first loop, which records the status word), and the default
-budgets on a large input.
+budget on a large input.
 """
 import numpy as np
 import pytest
@@ -71,21 +71,21 @@ def _close_roots(got, ref):
 
 
 @pytest.mark.parametrize("ci", range(len(ADAPTIVE)))
-@pytest.mark.parametrize("budgets", [(1, 0), (1 << 20, 1)])
-def test_handed_over_solves_equal_one_lane_solves(ci, budgets):
+@pytest.mark.parametrize("budget", [1, 64])
+def test_handed_over_solves_equal_one_lane_solves(ci, budget):
     case = ADAPTIVE[ci]
-    _case("handover", case=case, budgets=budgets)
+    _case("handover", case=case, budget=budget)
     g, c, x, tgt = _component(case, 1800 + ci, 900)
     xd = torch.from_numpy(x).cuda()
     td = torch.from_numpy(tgt).cuda()
-    g.SetInverseHandover(0, 0)
+    g.SetInverseHandover(0)
     off = g.Inverse(xd, td)
-    g.SetInverseHandover(*budgets)
+    g.SetInverseHandover(budget)
     on = g.Inverse(xd, td)
     _same(on, off)
     st = g.InverseHandoverStats()
     print(f"[gpu-case] handover stats {st}", flush=True)
-    assert st["solves"] > 100
+    assert st["solves"] > (100 if budget == 1 else 0)
     r = orc.CreateComponent(ref_opts(**case), 3, 3)
     r.SetCoeffs(c)
     ref = r.Inverse(x, tgt)
@@ -102,9 +102,9 @@ def test_handover_list_full_and_triangular():
     g, c, x, tgt = _component({}, 1850, 6000)
     xd = torch.from_numpy(x).cuda()
     td = torch.from_numpy(tgt).cuda()
-    g.SetInverseHandover(0, 0)
+    g.SetInverseHandover(0)
     off = g.Inverse(xd, td)
-    g.SetInverseHandover(1, 0)
+    g.SetInverseHandover(1)
     _same(g.Inverse(xd, td), off)
     # a triangular map: one queue pass and one hand-over pass per component
     _case("handover_triangular")
@@ -114,9 +114,9 @@ def test_handover_list_full_and_triangular():
     n = 3000
     y = torch.from_numpy(2.0 * s.normal(3 * n).reshape(3, n)).cuda()
     empty = torch.zeros((0, n), dtype=torch.float64, device="cuda")
-    t.SetInverseHandover(0, 0)
+    t.SetInverseHandover(0)
     toff = t.Inverse(empty, y)
-    t.SetInverseHandover(1, 0)
+    t.SetInverseHandover(1)
     _same(t.Inverse(empty, y), toff)
 
 
@@ -132,9 +132,9 @@ def test_handover_batched_queue():
     x1 = [torch.zeros((0, y.shape[1]), dtype=torch.float64, device="cuda") for y in ys]
     b = mt.MapBatch(maps)
     b.SetInverseSchedule(1)
-    b.SetInverseHandover(0, 0)
+    b.SetInverseHandover(0)
     off = [o.clone() for o in b.Inverse(x1, ys)]
-    b.SetInverseHandover(1, 0)
+    b.SetInverseHandover(1)
     on = b.Inverse(x1, ys)
     for a, c in zip(on, off):
         _same(a, c)
@@ -155,13 +155,13 @@ def test_handover_overflowing_integrand_records_status():
     g = mt.CreateTotalOrderComponent(1, 2, o)
     g.SetCoeffs(np.array([0.0, 0.0, 500.0]))
     x1 = np.zeros((1, 1))
-    for budgets in [(0, 0), (1, 0)]:
-        g.SetInverseHandover(*budgets)
+    for budget in [0, 1]:
+        g.SetInverseHandover(budget)
         small = g.Inverse(x1, np.array([[1e30]]))
         with pytest.raises(RuntimeError, match="MonotoneIntegrand"):
             g.Inverse(x1, np.array([[1e200]]))
         g.CheckIntegrand()
-        if budgets[0] == 0:
+        if budget == 0:
             ref_small = small
         else:
             assert np.array_equal(small, ref_small)
@@ -179,7 +179,7 @@ def test_handover_clenshaw_curtis_large():
     n = 100_000
     xd = torch.from_numpy(s.points(6, n)).cuda()
     td = torch.from_numpy(s.normal(n)[None, :]).cuda()
-    g.SetInverseHandover(0, 0)
+    g.SetInverseHandover(0)
     off = g.Inverse(xd, td)
     g.SetInverseHandover()
     _same(g.Inverse(xd, td), off)
@@ -189,15 +189,15 @@ def test_handover_clenshaw_curtis_large():
 
 
 def test_default_handover_large_input():
-    # The default budgets on 200 000 wide targets (dims 4, order 3): the
+    # The default budget on 200 000 wide targets (dims 4, order 3): the
     # queue hands over its longest solves; roots equal the one-lane solver's.
     _case("handover_default_large")
     g, c, x, tgt = _component({}, 1870, 200_000, dim=4, order=3, scale=0.15)
     xd = torch.from_numpy(x).cuda()
     td = torch.from_numpy(tgt).cuda()
-    g.SetInverseHandover(0, 0)
+    g.SetInverseHandover(0)
     off = g.Inverse(xd, td)
     g.SetInverseHandover()
     _same(g.Inverse(xd, td), off)
     with pytest.raises(ValueError):
-        g.SetInverseHandover(-1, 0)
+        g.SetInverseHandover(-1)
