@@ -67,9 +67,9 @@ enum { HS_SOLVES = 0, HS_TEVALS, HS_NODES, HS_FALLBACK_SIZE, HS_FALLBACK_NONFINI
        HS_MAX_SOLVE_TEVALS, HS_MAX_SOLVE_TICKS, HS_COUNT };
 // Intervals one cooperative quadrature keeps in LDS (60 bytes each, plus 64
 // node values of a round; a larger tree is integrated depth-first instead).
-// 384 intervals (23.5 KB: six one-wavefront blocks per CU) hold the trees
+// 320 intervals (19.3 KB: eight one-wavefront blocks per CU) hold the trees
 // of the long solves met so far (at most 289 intervals).
-constexpr int COOP_NODES = 384;
+constexpr int COOP_NODES = 320;
 constexpr int COOP_LDS_BYTES = COOP_NODES * 60 + 64 * 8;
 
 }  // namespace mpd
