@@ -36,12 +36,20 @@ int occupancy_inverse_batch(int var, int block) {
 hipError_t launch_inverse_batch(int var, const Launch& L, int ntiles, dim3 qgrid, dim3 qblock, const MapDev& M0,
                                 const MapDev* maps, const InvSeg* segs, const int* tileSeg, const long* tileStart,
                                 const int* chunkSeg, long Ntot, int c, double xtol, double ytol, double* rec,
-                                unsigned long long* counter, const InvHeavy& H, InvHeavy* Hd, int heavyBlocks) {
+                                unsigned long long* counter, const InvHeavy& H, InvHeavy* Hd, int heavyBlocks,
+                                int ntilesGrp, int tileN, size_t grpSmem) {
     if (ntiles <= 0 || Ntot <= 0) return hipSuccess;
+    const int ntiles1 = ntiles - ntilesGrp;
+    const int parts = (tileN + 63) / 64;  // 256-lane blocks of 64 points (4 lanes each) per tile
     hipLaunchKernelGGL(k_zero_u64_batch, dim3(1), dim3(1), 0, L.stream, counter, H, Hd, c == 0 ? 1 : 0);
 #define CALL(B, Q, R, PM)                                                                                           \
-    hipLaunchKernelGGL((k_inverse_setup_batch<B, Q, R, PM>), dim3((unsigned)ntiles), L.block, L.smem, L.stream,     \
-                       maps, segs, tileSeg, tileStart, Ntot, c, rec);                                               \
+    if (ntiles1 > 0)                                                                                              \
+        hipLaunchKernelGGL((k_inverse_setup_batch<B, Q, R, PM>), dim3((unsigned)ntiles1), L.block, L.smem, L.stream, \
+                           maps, segs, tileSeg, tileStart, Ntot, c, rec);                                          \
+    if (ntilesGrp > 0)                                                                                            \
+        hipLaunchKernelGGL((k_inverse_setup_batch_grp<B, Q, R, PM, 4>), dim3((unsigned)(ntilesGrp * parts)),       \
+                           dim3(256), grpSmem, L.stream, maps, segs, tileSeg + ntiles1, tileStart + ntiles1,       \
+                           tileN, Ntot, c, rec);                                                                   \
     if (M0.lin) {                                                                                                 \
         hipLaunchKernelGGL((k_inverse_queue<B, Q, R, PM, true, true>), qgrid, qblock, 0, L.stream, M0, rec, Ntot,   \
                            nullptr, 0, c, xtol, ytol, counter, segs, chunkSeg, Hd, H.solveSteps);                                \

@@ -81,13 +81,16 @@ hipError_t launch_inverse(int var, const Launch& L, dim3 qgrid, dim3 qblock, con
 int occupancy_inverse(int var, int block);
 int inverse_record_rows(int var);
 // Batched Inverse of component c over many maps: a counter reset, the setup
-// pass (ntiles blocks of L.block points, L.smem bytes of basis tables), then
-// the persistent solve pass over the Ntot indices of the segments.
+// pass (tiles of tileN = L.block points, a block each with L.smem bytes of
+// basis tables; the last ntilesGrp tiles, of maps with many terms, with 4
+// lanes per point in 256-lane blocks of grpSmem bytes), then the persistent
+// solve pass over the Ntot indices of the segments.
 struct InvSeg;
 hipError_t launch_inverse_batch(int var, const Launch& L, int ntiles, dim3 qgrid, dim3 qblock, const MapDev& M0,
                                 const MapDev* maps, const InvSeg* segs, const int* tileSeg, const long* tileStart,
                                 const int* chunkSeg, long Ntot, int c, double xtol, double ytol, double* rec,
-                                unsigned long long* counter, const InvHeavy& H, InvHeavy* Hd, int heavyBlocks);
+                                unsigned long long* counter, const InvHeavy& H, InvHeavy* Hd, int heavyBlocks,
+                                int ntilesGrp, int tileN, size_t grpSmem);
 int occupancy_inverse_batch(int var, int block);
 hipError_t launch_fused(int var, const Launch& L, const MapDev& M, const double* w, long ws, const double* pts,
                         long prs, long pcs, long N, const double* sens, long srs, long scs, const double* lsens,

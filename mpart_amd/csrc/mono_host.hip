@@ -1755,6 +1755,7 @@ struct BatchSlot {
 struct InvPlan {
     struct Pass {
         long seg0, nseg, tile0, ntiles, chunk0, Ntot;
+        long ntilesGrp;  // the last ntilesGrp tiles: maps with 4 lanes per point in the setup
     };
     bool valid = false;
     std::vector<mpd::BatchIO> io;
@@ -1788,6 +1789,8 @@ struct mpart_batch_s {
     double invTrialMs[2] = {0, 0};
     bool invCaptured = false;  // an Inverse call has been captured (its schedule is frozen)
     std::vector<int> mapLanes;
+    size_t grpSetupSmem = 0;  // LDS of the 4-lanes-per-point Inverse setup (0: not used)
+    size_t laneSetupSmem = 0; // with it, the LDS of the other maps' lane-per-point setup
     int grpSlots = 1;
     size_t smemGrp = 0;
     int occGrp = 0;
@@ -1839,6 +1842,10 @@ mpart_batch_s* make_batch(const mpart_map* maps, uint32_t n) {
     const size_t mixedSmem =
         std::max((size_t)256 * laneSlots, (size_t)(256 / 4) * grpSlots) * sizeof(double);
     b->smemGrp = (anyGroup && mixedSmem <= LDS_LIMIT) ? mixedSmem : 0;
+    // the batched Inverse's setup of those maps: 64 points (4 lanes each) per 256-lane block
+    const size_t grpSetup = (size_t)64 * grpSlots * sizeof(double);
+    b->grpSetupSmem = (anyGroup && grpSetup <= LDS_LIMIT) ? grpSetup : 0;
+    b->laneSetupSmem = (size_t)laneSlots * block * sizeof(double);
     upload(b->dMaps, devs);
     for (auto& sl : b->slot) sl.dCounter.ensure(256);
     return b.release();
@@ -1970,7 +1977,9 @@ void batch_inverse(mpart_batch_s& b, const std::vector<mpd::BatchIO>& io, double
         std::vector<long> tileStart;
         const long tileN = b.block;
         for (size_t c = 0; c < ncompMax; ++c) {
-            InvPlan::Pass ps{(long)segs.size(), 0, (long)tileSeg.size(), 0, (long)chunkSeg.size(), 0};
+            InvPlan::Pass ps{(long)segs.size(), 0, (long)tileSeg.size(), 0, (long)chunkSeg.size(), 0, 0};
+            std::vector<int> grpSeg;  // tiles of maps with 4 lanes per point, appended after the others
+            std::vector<long> grpStart;
             for (size_t i = 0; i < nm; ++i) {
                 mpart_map m = b.maps[i];
                 const mpd::BatchIO& d = io[i];
@@ -2011,15 +2020,19 @@ void batch_inverse(mpart_batch_s& b, const std::vector<mpd::BatchIO>& io, double
                     g.os2 = 0;
                 }
                 const int si = (int)segs.size() - (int)ps.seg0;
+                const bool grp = b.grpSetupSmem > 0 && b.mapLanes[i] == 4;
                 for (long t0 = 0; t0 < d.N; t0 += tileN) {
-                    tileSeg.push_back(si);
-                    tileStart.push_back(t0);
+                    (grp ? grpSeg : tileSeg).push_back(si);
+                    (grp ? grpStart : tileStart).push_back(t0);
                 }
                 const long padded = (d.N + mpd::INV_CHUNK - 1) / mpd::INV_CHUNK * mpd::INV_CHUNK;
                 for (long k = 0; k < padded / mpd::INV_CHUNK; ++k) chunkSeg.push_back(si);
                 ps.Ntot += padded;
                 segs.push_back(g);
             }
+            ps.ntilesGrp = (long)grpSeg.size();
+            tileSeg.insert(tileSeg.end(), grpSeg.begin(), grpSeg.end());
+            tileStart.insert(tileStart.end(), grpStart.begin(), grpStart.end());
             ps.nseg = (long)segs.size() - ps.seg0;
             ps.ntiles = (long)tileSeg.size() - ps.tile0;
             P.maxNtot = std::max(P.maxNtot, ps.Ntot);
@@ -2085,13 +2098,14 @@ void batch_inverse(mpart_batch_s& b, const std::vector<mpd::BatchIO>& io, double
     for (size_t c = 0; c < P.passes.size(); ++c) {
         const InvPlan::Pass& ps = P.passes[c];
         const long qgrid = std::max<long>(1, std::min<long>((ps.Ntot + qblock - 1) / qblock, qcap));
-        mpd::Launch L{dim3(1), dim3(b.block), b.smem, s};
+        // the lane-per-point setup holds only the tables of the maps without 4 lanes per point
+        mpd::Launch L{dim3(1), dim3(b.block), b.grpSetupSmem > 0 ? b.laneSetupSmem : b.smem, s};
         hip_check(mpd::launch_inverse_batch(b.variant, L, (int)ps.ntiles, dim3((unsigned)qgrid), dim3(qblock),
                                             b.maps[0]->dev, b.dMaps.as<MapDev>(), segs + ps.seg0,
                                             P.dTileSeg.as<int>() + ps.tile0, P.dTileStart.as<long>() + ps.tile0,
                                             P.dChunkSeg.as<int>() + ps.chunk0, ps.Ntot, (int)c, xtol, ytol,
                                             P.wRec.as<double>(), P.dCounters.as<unsigned long long>() + c, H,
-                                            Hd, heavyBlocks),
+                                            Hd, heavyBlocks, (int)ps.ntilesGrp, b.block, b.grpSetupSmem),
                   "launch batched inverse");
     }
 }

@@ -93,6 +93,55 @@ __global__ void __launch_bounds__(256) k_inverse_setup_batch(const MapDev* __res
                                     stride, rec + S.base + n, R);
 }
 
+// Batched setup with G lanes per point, for maps with many terms (the
+// tiles after the first ntiles1 of a pass, each of tileN points of one map):
+// block b covers blockDim / G points of tile b / parts, the G lanes of a
+// point share one basis table in LDS (M.nslots doubles, at most the batch's
+// grpSlots) and split the term sums (group_coeff_groups, a butterfly sum of
+// G partial sums: the same C_q up to the order of summation); lane 0 of the
+// group writes the record.  Lane-per-point tables of such maps fill the LDS
+// with a few wavefronts per CU; a table per G lanes gives G times as many.
+template <int BK, int PK, int QK, int PM, int G>
+__global__ void __launch_bounds__(256) k_inverse_setup_batch_grp(const MapDev* __restrict__ maps,
+                                                                 const InvSeg* __restrict__ segs,
+                                                                 const int* __restrict__ tileSeg,
+                                                                 const long* __restrict__ tileStart, int tileN, long R,
+                                                                 int c, double* rec) {
+    extern __shared__ double smem[];
+    const int per = blockDim.x / G;
+    const int parts = (tileN + per - 1) / per;
+    const int tile = blockIdx.x / parts, part = blockIdx.x - tile * parts;
+    const InvSeg& S = segs[tileSeg[tile]];
+    const int gl = threadIdx.x & (G - 1), grp = threadIdx.x / G;
+    const int k = part * per + grp;
+    const long n = tileStart[tile] + k;
+    if (k >= tileN || n >= S.N) return;  // the whole group leaves together
+    const MapDev& M = maps[S.map];
+    double* tab = smem + (long)grp * M.nslots;
+    const long xi = (S.xcols == 1) ? 0 : n;
+    const double* pt = S.x + xi * S.xcs;
+    double* r = rec + S.base + n;
+    bool bad = false;
+    for (long i = 0; i < S.xrows; ++i) bad |= isnan(pt[i * S.xrs]);
+    if (gl == 0) r[(PM + 3) * R] = S.y[n * S.yss];
+    if (bad) {
+        if (gl == 0) r[(PM + 2) * R] = NAN;
+        return;
+    }
+    const double x0 = pt[(S.xrows - 1) * S.xrs];
+    const CompDev cd = ro(M.comps)[c];
+    group_fill_values<BK, G>(M, cd.dim - 1, pt, S.xrs, tab, gl);
+    double C[PM + 1];
+    group_coeff_groups<PM, G>(M, cd, S.w + (long)cd.coeffOff * S.ws, S.ws, tab, gl, C);
+    if (gl == 0) {
+        DiagCoef<PM> dc;
+        make_diag<PM>(M, cd, C, x0, M.nugget, dc);
+        for (int q = 0; q <= PM; ++q) r[q * R] = dc.cd[q];
+        r[(PM + 1) * R] = f_at_zero<PM>(M, cd, C);
+        r[(PM + 2) * R] = x0;
+    }
+}
+
 // Pending right half of the depth-first quadrature (the entries of
 // integrate_as / integrate_acc: the interval's left end and its value are
 // those of the leaf accepted before it is popped, its width 2^-level; Simpson
