@@ -46,11 +46,16 @@ int inverse_record_rows(int var) {
 hipError_t launch_inverse(int var, const Launch& L, dim3 qgrid, dim3 qblock, const MapDev& M, const double* w, long ws,
                           const double* xs, long xrows, long xcols, long xrs, long xcs, const double* ys, long yss,
                           long N, double* out, long os, int c, double xtol, double ytol, double* gtab, double* rec,
-                          unsigned long long* counter, const InvHeavy& H, InvHeavy* Hd, int heavyBlocks) {
+                          unsigned long long* counter, const InvHeavy& H, InvHeavy* Hd, int heavyBlocks,
+                          size_t grpSmem) {
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, L.stream, counter, H, Hd, c == 0 ? 1 : 0);
 #define CALL(B, Q, R, PM)                                                                                          \
-    hipLaunchKernelGGL((k_inverse_setup<B, Q, R, PM>), L.grid, L.block, L.smem, L.stream, M, w, ws, xs, xrows, xcols, \
-                       xrs, xcs, ys, yss, N, c, gtab, rec);                                                        \
+    if (grpSmem > 0)                                                                                             \
+        hipLaunchKernelGGL((k_inverse_setup_grp<B, Q, R, PM, 4>), dim3((unsigned)((N + 63) / 64)), dim3(256),     \
+                           grpSmem, L.stream, M, w, ws, xs, xrows, xcols, xrs, xcs, ys, yss, N, c, rec);           \
+    else                                                                                                         \
+        hipLaunchKernelGGL((k_inverse_setup<B, Q, R, PM>), L.grid, L.block, L.smem, L.stream, M, w, ws, xs, xrows,  \
+                           xcols, xrs, xcs, ys, yss, N, c, gtab, rec);                                             \
     if (M.lin) {                                                                                                 \
         hipLaunchKernelGGL((k_inverse_queue<B, Q, R, PM, true, false>), qgrid, qblock, 0, L.stream, M, rec, N, out, \
                            os, c, xtol, ytol, counter, nullptr, nullptr, Hd, H.solveSteps);                                     \

@@ -69,15 +69,18 @@ hipError_t launch_input_vjp(int var, const Launch& L, const MapDev& M, const dou
                             long prs, long pcs, long N, int klMode, const double* sens, long srs, long scs,
                             const double* lsens, long lss, double lval, double scale, double* out, long ors, long ocs,
                             double* gtab);
-// Inverse of component c: a setup pass (shape L) writes per-point records to
-// rec, then the persistent solve pass (qgrid x qblock) takes points from the
+// Inverse of component c: a setup pass (shape L; or, with grpSmem > 0, 4
+// lanes per point in 256-lane blocks of grpSmem bytes, for maps with many
+// terms) writes per-point records to rec, then the persistent solve pass
+// (qgrid x qblock) takes points from the
 // counter at rec + records; with H's budget set (adaptive rules), the solves
 // it hands over (reading H from Hd, device memory) then run in heavyBlocks
 // one-wavefront blocks.
 hipError_t launch_inverse(int var, const Launch& L, dim3 qgrid, dim3 qblock, const MapDev& M, const double* w, long ws,
                           const double* xs, long xrows, long xcols, long xrs, long xcs, const double* ys, long yss,
                           long N, double* out, long os, int c, double xtol, double ytol, double* gtab, double* rec,
-                          unsigned long long* counter, const InvHeavy& H, InvHeavy* Hd, int heavyBlocks);
+                          unsigned long long* counter, const InvHeavy& H, InvHeavy* Hd, int heavyBlocks,
+                          size_t grpSmem);
 int occupancy_inverse(int var, int block);
 int inverse_record_rows(int var);
 // Batched Inverse of component c over many maps: a counter reset, the setup

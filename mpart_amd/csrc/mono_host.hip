@@ -1184,6 +1184,13 @@ void do_input_vjp(mpart_map_s& m, const mpart_cview& p, int klMode, const mpart_
 // integrated depth-first by all lanes).
 long heavy_cap(long N) { return std::max<long>(1024, N / 16); }
 int heavy_blocks(long N) { return (int)std::min<long>(HEAVY_MAX_BLOCKS, std::max<long>(1, N / 256)); }
+// Maps whose Evaluate in a batch, and whose Inverse setup, take 4 lanes per
+// point: above the specialization limit, or with a per-lane basis table that
+// does not fit 256 lanes in LDS.
+bool many_terms(const mpart_map_s& m) {
+    return m.numCoeffs >= GEN_MAX_COEFFS || (size_t)m.dev.nslots * 256 * sizeof(double) > LDS_LIMIT;
+}
+
 // Layout of the hand-over buffer: [statistics (256 B) | the arguments as the
 // solve pass reads them (256 B) | list].
 constexpr size_t HEAVY_STATS_BYTES = 256;
@@ -1235,9 +1242,12 @@ void inverse_component(mpart_map_s& m, const Shape& sh, const double* xs, long x
     int heavyBlocks = 0;
     mpd::InvHeavy* Hd = nullptr;
     const mpd::InvHeavy H = heavy_args(m.wHeavy, counter + 1, N, m.heavySolveSteps, heavyBlocks, Hd);
+    // maps with many terms: the 4-lanes-per-point setup of the batched Inverse (the same records)
+    const size_t grpSmem = many_terms(m) ? (size_t)64 * m.dev.nslots * sizeof(double) : 0;
     hip_check(mpd::launch_inverse(m.variant, sh.L, dim3((unsigned)std::max<long>(1, grid)), dim3(block), m.dev,
                                   m.coeffs, m.coeffStride, xs, xrows, xcols, xrs, xcs, ys, yss, N, out, os, c, xtol,
-                                  ytol, sh.gtab, rec, counter, H, Hd, heavyBlocks),
+                                  ytol, sh.gtab, rec, counter, H, Hd, heavyBlocks,
+                                  grpSmem <= LDS_LIMIT ? grpSmem : 0),
               "launch inverse");
 }
 
@@ -1831,8 +1841,7 @@ mpart_batch_s* make_batch(const mpart_map* maps, uint32_t n) {
     int laneSlots = 1, grpSlots = 1;
     bool anyGroup = false;
     for (uint32_t i = 0; i < n; ++i) {
-        const bool big = maps[i]->numCoeffs >= GEN_MAX_COEFFS ||
-                         (size_t)maps[i]->dev.nslots * 256 * sizeof(double) > LDS_LIMIT;
+        const bool big = many_terms(*maps[i]);
         b->mapLanes.push_back(big ? 4 : 1);
         anyGroup = anyGroup || big;
         if (big) grpSlots = std::max(grpSlots, maps[i]->dev.nslots);

@@ -142,6 +142,44 @@ __global__ void __launch_bounds__(256) k_inverse_setup_batch_grp(const MapDev* _
     }
 }
 
+// The same for one map (the single-map Inverse of a map with many terms):
+// block b covers points b * blockDim / G onwards.  The arithmetic is that
+// of k_inverse_setup_batch_grp, so the records, and the roots, are those of
+// the batched Inverse.
+template <int BK, int PK, int QK, int PM, int G>
+__global__ void __launch_bounds__(256) k_inverse_setup_grp(MapDev M, const double* w, long ws, const double* xs,
+                                                           long xrows, long xcols, long xrs, long xcs,
+                                                           const double* ys, long yss, long N, int c, double* rec) {
+    extern __shared__ double smem[];
+    const int per = blockDim.x / G;
+    const int gl = threadIdx.x & (G - 1), grp = threadIdx.x / G;
+    const long n = (long)blockIdx.x * per + grp;
+    if (n >= N) return;  // the whole group leaves together
+    double* tab = smem + (long)grp * M.nslots;
+    const long xi = (xcols == 1) ? 0 : n;
+    const double* pt = xs + xi * xcs;
+    double* r = rec + n;
+    bool bad = false;
+    for (long i = 0; i < xrows; ++i) bad |= isnan(pt[i * xrs]);
+    if (gl == 0) r[(PM + 3) * N] = ys[n * yss];
+    if (bad) {
+        if (gl == 0) r[(PM + 2) * N] = NAN;
+        return;
+    }
+    const double x0 = pt[(xrows - 1) * xrs];
+    const CompDev cd = ro(M.comps)[c];
+    group_fill_values<BK, G>(M, cd.dim - 1, pt, xrs, tab, gl);
+    double C[PM + 1];
+    group_coeff_groups<PM, G>(M, cd, w + (long)cd.coeffOff * ws, ws, tab, gl, C);
+    if (gl == 0) {
+        DiagCoef<PM> dc;
+        make_diag<PM>(M, cd, C, x0, M.nugget, dc);
+        for (int q = 0; q <= PM; ++q) r[q * N] = dc.cd[q];
+        r[(PM + 1) * N] = f_at_zero<PM>(M, cd, C);
+        r[(PM + 2) * N] = x0;
+    }
+}
+
 // Pending right half of the depth-first quadrature (the entries of
 // integrate_as / integrate_acc: the interval's left end and its value are
 // those of the leaf accepted before it is popped, its width 2^-level; Simpson

@@ -150,3 +150,40 @@ def test_large_triangular_inverse_equals_sequential():
     seq = _sequential(g).Inverse([empty], [y])[0]
     assert torch.equal(torch.isnan(single), torch.isnan(seq))
     assert torch.equal(torch.nan_to_num(single), torch.nan_to_num(seq))
+
+
+def test_many_term_map_setup_single_equals_batched_and_oracle():
+    # A map whose basis table does not fit 256 lanes (9-D, order 4: 37 table
+    # slots) takes 4 lanes per point in the Inverse setup, in the single-map
+    # path as in the batch: the records, and so the roots, are the same bit
+    # for bit.  Against the oracle: 1e-9, except where a comparison of T with
+    # the target is decided by its last bits (the 4-lane sums round
+    # differently) and the two solvers stop at different points of the
+    # tolerance window, which later components of the triangular map inherit
+    # (the bound of test_gpu_many_maps_inverse.py: at most 10 % of the
+    # coordinates, each within 1e-5).
+    _case("inverse_many_terms_setup")
+    s = Stream(995)
+    g = mt.CreateTriangular(9, 9, 4, mt.MapOptions())
+    c = 0.1 * s.normal(g.numCoeffs)
+    g.SetCoeffs(c)
+    n = 3000
+    y = 2.0 * s.normal(9 * n).reshape(9, n)
+    yd = torch.from_numpy(y).cuda()
+    empty = torch.zeros((0, n), dtype=torch.float64, device="cuda")
+    single = g.Inverse(empty, yd)
+    b = mt.MapBatch([g])
+    b.SetInverseSchedule(1)
+    batched = b.Inverse([empty], [yd])[0]
+    assert torch.equal(torch.isnan(single), torch.isnan(batched))
+    assert torch.equal(torch.nan_to_num(single), torch.nan_to_num(batched))
+    r = orc.CreateTriangular(9, 9, 4, ref_opts())
+    r.SetCoeffs(c)
+    ref = r.Inverse(np.zeros((0, n)), y)
+    got = single.cpu().numpy()
+    fin = np.isfinite(ref)
+    assert np.array_equal(fin, np.isfinite(got))
+    d = np.abs(got[fin] - ref[fin])
+    off = int(np.count_nonzero(d > 1e-9))
+    print(f"[gpu-case] roots off the oracle's by more than 1e-9: {off} of {d.size}, max {np.max(d):.3g}", flush=True)
+    assert off <= max(1, d.size // 10) and np.max(d) <= 1e-5
