@@ -342,11 +342,12 @@ __device__ __forceinline__ void acc_sums(const AccRule& R, double a, double b, d
 }
 
 // The adaptive rules (integrate_as, integrate_acc; FL_VAL) computed by the 64
-// lanes of a one-wavefront block.  The tree is built level by level: every
-// pending interval of a level is evaluated at once (a lane each: Simpson's
-// two quarter points, or the nested rule's midpoint and interior nodes),
-// tested, and the children of the intervals that fail are written to the
-// next level in left-to-right order.  The intervals are the dyadic ones the
+// lanes of a one-wavefront block.  The tree is built level by level, in
+// rounds (below): the new nodes of the pending intervals (Simpson's two
+// quarter points, or the nested rule's midpoint and interior nodes) are
+// evaluated a lane per node, the intervals are tested a lane per interval,
+// and the children of the intervals that fail are written to the next level
+// in left-to-right order.  The intervals are the dyadic ones the
 // depth-first loop visits (positions idx 2^-level are exact), with the same
 // node values, sums and tests, so the tree is the same.  The accepted
 // intervals are then ranked left to right (leaf counts bottom-up, offsets
