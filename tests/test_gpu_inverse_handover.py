@@ -14,8 +14,9 @@ PhysicistHermite, LinearizedBasis bounds, a 33-point rule in device
 memory), NaN inputs, targets outside the range (NaN roots
 after a long bracket search), triangular maps (one pass per component), the
 batched queue (schedule 1), an overflowing integrand (the pass falls back to
-the depth-first loop, which records the status word), and the default
-budget on a large input.
+the depth-first loop, which records the status word), trees larger than the
+wavefront's workspace (the same fallback), and the default budget on a
+large input.
 """
 import numpy as np
 import pytest
@@ -201,3 +202,22 @@ def test_default_handover_large_input():
     _same(g.Inverse(xd, td), off)
     with pytest.raises(ValueError):
         g.SetInverseHandover(-1)
+
+
+def test_handover_trees_beyond_the_workspace():
+    # Tolerances of 1e-13 make the quadrature trees larger than the
+    # wavefront's LDS workspace (320 intervals): those T evaluations fall back
+    # to the depth-first loop in every lane, with the same roots bit for bit.
+    case = dict(quadAbsTol=1e-13, quadRelTol=1e-13, quadMaxSub=24)
+    _case("handover_workspace_fallback", case=case)
+    g, c, x, tgt = _component(case, 1890, 300, scale=0.3)
+    xd = torch.from_numpy(x).cuda()
+    td = torch.from_numpy(tgt).cuda()
+    g.SetInverseHandover(0)
+    off = g.Inverse(xd, td)
+    g.SetInverseHandover(1)
+    on = g.Inverse(xd, td)
+    _same(on, off)
+    st = g.InverseHandoverStats()
+    print(f"[gpu-case] handover stats {st}", flush=True)
+    assert st["solves"] > 0 and st["fallback_size"] > 0
