@@ -316,6 +316,7 @@ def many_maps(steps, dmax=8, pmax=4, nmax=16384, coeff_grad=True, inverse=True):
         # maps where some call met an infinite g (the reference would have
         # failed there with its runtime error); their solves end with NaN
         res["sequence"]["maps_with_infinite_g"] = _maps_with_infinite_g(maps)
+        res["sequence"]["inverse_handover"] = b.InverseHandoverStats()
         sched, trial = b.InverseSchedule()
         res["sequence"]["inverse_schedule"] = {"choice": sched, "trial_ms_schedules_1_2": trial,
                                                "meaning": "1: per-lane solve scheduling, one pass per component "
@@ -448,6 +449,8 @@ def single_component_workload(steps):
         out[name + "_ms"] = ms
         out[name + "_points_per_s"] = N / (ms * 1e-3)
         log(f"single component {name}: {ms:.3f} ms")
+    # the Inverse's tail: solves handed over to a wavefront each (last call)
+    out["inverse_handover"] = g.InverseHandoverStats()
     del cg
     return out
 
