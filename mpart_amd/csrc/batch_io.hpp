@@ -52,6 +52,7 @@ struct InvSeg {
 // step budget (solveSteps 0: off).
 struct InvHeavy {
     unsigned long long* count;
+    unsigned long long* take;  // next list entry for k_inverse_heavy's blocks to take
     long* list;
     long cap;
     int solveSteps;
@@ -65,11 +66,11 @@ struct InvHeavy {
 };
 enum { HS_SOLVES = 0, HS_TEVALS, HS_NODES, HS_FALLBACK_SIZE, HS_FALLBACK_NONFINITE, HS_MAX_NODES, HS_MAX_LEVELS,
        HS_MAX_SOLVE_TEVALS, HS_MAX_SOLVE_TICKS, HS_COUNT };
-// Intervals one cooperative quadrature keeps in LDS (60 bytes each, plus 64
+// Intervals one cooperative quadrature keeps in LDS (48 bytes each, plus 64
 // node values of a round; a larger tree is integrated depth-first instead).
-// 320 intervals (19.3 KB: eight one-wavefront blocks per CU) hold the trees
+// 320 intervals (15.5 KB: ten one-wavefront blocks per CU) hold the trees
 // of the long solves met so far (at most 289 intervals).
 constexpr int COOP_NODES = 320;
-constexpr int COOP_LDS_BYTES = COOP_NODES * 60 + 64 * 8;
+constexpr int COOP_LDS_BYTES = COOP_NODES * 48 + 64 * 8;
 
 }  // namespace mpd

@@ -9,6 +9,7 @@ namespace mpd {
 __global__ void k_zero_u64_batch(unsigned long long* p, InvHeavy H, InvHeavy* Hd, int first) {
     *p = 0ull;
     if (H.count) *H.count = 0ull;
+    if (H.take) *H.take = 0ull;
     if (Hd) *Hd = H;
     if (first && H.stats)
         for (int k = 0; k < HS_COUNT; ++k) H.stats[k] = 0ull;

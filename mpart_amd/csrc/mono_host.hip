@@ -340,7 +340,7 @@ double* noalloc_scratch() {
 constexpr int GEN_FUSED_SCHEDULES = 4;
 // Inverse hand-over defaults and sizes (heavy_args).
 constexpr int HEAVY_SOLVE_STEPS = 1024;
-constexpr long HEAVY_MAX_BLOCKS = 2048;
+constexpr long HEAVY_MAX_BLOCKS = 2560;  // ten 15.5 KB one-wavefront blocks per CU of 256
 
 struct mpart_map_s {
     mpart_options opts{};
@@ -1178,8 +1178,8 @@ void do_input_vjp(mpart_map_s& m, const mpart_cview& p, int klMode, const mpart_
 // left to take, a queue lane hands a solve to the one-wavefront-per-solve
 // pass if the solve has taken HEAVY_SOLVE_STEPS interval steps (the median
 // solve takes a few hundred); the
-// list holds up to N / 16 (at least 1 024) solves, taken by heavyBlocks
-// wavefronts (one per 256 points, at most HEAVY_MAX_BLOCKS), each with an
+// list holds up to N / 16 (at least 1 024) solves, taken one at a time by
+// heavyBlocks wavefronts (one per 256 points, at most HEAVY_MAX_BLOCKS), each with an
 // LDS workspace of COOP_NODES intervals per quadrature (larger trees are
 // integrated depth-first by all lanes).
 long heavy_cap(long N) { return std::max<long>(1024, N / 16); }
@@ -1192,12 +1192,12 @@ bool many_terms(const mpart_map_s& m) {
 }
 
 // Layout of the hand-over buffer: [statistics (256 B) | the arguments as the
-// solve pass reads them (256 B) | list].
+// solve pass reads them (128 B), the take counter (128 B) | list].
 constexpr size_t HEAVY_STATS_BYTES = 256;
 mpd::InvHeavy heavy_args(DBuf& buf, unsigned long long* count, long N, int solveSteps, int& blocks,
                          mpd::InvHeavy*& Hd) {
     static_assert(mpd::HS_COUNT * sizeof(unsigned long long) <= HEAVY_STATS_BYTES, "statistics block");
-    static_assert(sizeof(mpd::InvHeavy) <= 256, "argument block");
+    static_assert(sizeof(mpd::InvHeavy) <= 128, "argument block");
     mpd::InvHeavy H{};
     H.count = count;
     blocks = 0;
@@ -1208,6 +1208,7 @@ mpd::InvHeavy heavy_args(DBuf& buf, unsigned long long* count, long N, int solve
     buf.ensure(HEAVY_STATS_BYTES + 256 + (size_t)cap * sizeof(long));
     H.stats = buf.as<unsigned long long>();
     Hd = reinterpret_cast<mpd::InvHeavy*>(buf.as<char>() + HEAVY_STATS_BYTES);
+    H.take = reinterpret_cast<unsigned long long*>(buf.as<char>() + HEAVY_STATS_BYTES + 128);
     H.list = reinterpret_cast<long*>(buf.as<char>() + HEAVY_STATS_BYTES + 256);
     H.cap = cap;
     H.solveSteps = solveSteps;

@@ -215,9 +215,11 @@ enum { IP_START = 0, IP_DOWN = 1, IP_UP = 2, IP_ITP = 3 };
 
 // Workspace of one wavefront (LDS, COOP_LDS_BYTES): per processed interval
 // (node) of one quadrature, in level order, its left end index, end and
-// midpoint values, fine sum, split rank, leaf count, leaf offset and slot in
-// the round's speculative node values; the leaves in left-to-right order;
-// and the node values of one round (64).
+// midpoint values, fine sum, split rank, leaf count and slot in the round's
+// speculative node values; and the node values of one round (64).  The
+// leaves in left-to-right order and their offsets are written only once the
+// tree is complete, when the left-end values and the slots are no longer
+// read: they share those arrays.
 struct CoopWs {
     double *fa, *fm, *fb, *fine, *leaf, *spec;
     unsigned* idx;
@@ -225,21 +227,21 @@ struct CoopWs {
     int cap;
 };
 __device__ __forceinline__ CoopWs coop_ws(double* base, int cap) {
-    static_assert(COOP_LDS_BYTES == COOP_NODES * (5 * sizeof(double) + 5 * sizeof(int)) + 64 * sizeof(double),
+    static_assert(COOP_LDS_BYTES == COOP_NODES * (4 * sizeof(double) + 4 * sizeof(int)) + 64 * sizeof(double),
                   "workspace layout");
     CoopWs W;
     W.fa = base;
     W.fm = base + cap;
     W.fb = base + 2L * cap;
     W.fine = base + 3L * cap;
-    W.leaf = base + 4L * cap;
-    W.spec = base + 5L * cap;
-    int* ib = reinterpret_cast<int*>(base + 5L * cap + 64);
+    W.leaf = W.fa;  // after the rounds
+    W.spec = base + 4L * cap;
+    int* ib = reinterpret_cast<int*>(base + 4L * cap + 64);
     W.idx = reinterpret_cast<unsigned*>(ib);
     W.srank = ib + cap;
     W.cnt = ib + 2L * cap;
-    W.off = ib + 3L * cap;
-    W.sp = ib + 4L * cap;
+    W.sp = ib + 3L * cap;
+    W.off = W.sp;  // after the rounds
     W.cap = cap;
     return W;
 }
@@ -589,7 +591,7 @@ __device__ double coop_integrate(const QuadDev& Q, const DiagCoef<PM>& dcIn, con
 }
 
 // The handed-over solves (see InvHeavy): one wavefront (block of 64) per
-// solve, blocks looping over the list.  Each solve runs from the point's
+// solve, blocks taking list entries from a counter until none is left.  Each solve runs from the point's
 // record with inverse_solve_with and the cooperative T; all lanes hold the
 // same values, and lane 0 writes the root.
 template <int BK, int PK, int QK, int PM, bool LIN, bool BATCH>
@@ -601,9 +603,16 @@ __global__ void __launch_bounds__(64) k_inverse_heavy(MapDev M, const double* __
     extern __shared__ double coopLds[];  // COOP_LDS_BYTES
     const unsigned long long total = *H.count;
     const long cnt = (long)(total < (unsigned long long)H.cap ? total : (unsigned long long)H.cap);
+    __shared__ long eTake;
     const CoopWs W = coop_ws(coopLds, COOP_NODES);
     HeavyStats hs{};
-    for (long e = blockIdx.x; e < cnt; e += gridDim.x) {
+    while (true) {
+        // solves are taken one at a time, so that long ones spread over the blocks
+        if (threadIdx.x == 0) eTake = (long)atomicAdd(H.take, 1ull);
+        __syncthreads();
+        const long e = eTake;
+        __syncthreads();
+        if (e >= cnt) break;
         const long n = H.list[e];
         hs.add(HS_SOLVES, 1);
         DiagCoef<PM> dc;
